@@ -1,0 +1,154 @@
+"""Seeded synthetic AMOEBA systems for the nonbonded path.
+
+* ``water_box``: rigid AMOEBA-water-2003 molecules (O-H 0.9572 A, H-O-H
+  108.5 deg) with oxygens placed by random sequential insertion (no O-O pair
+  closer than ``min_oo``) and uniformly random orientations.
+* ``chain_in_water``: a self-avoiding random-walk chain with random moments
+  solvated by waters.
+
+Every generator takes a seed for numpy's PCG64 and is deterministic.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .forcefield import (FRAME_BISECTOR, FRAME_NONE, FRAME_ZTHENX, WATER_H_MPOLE, WATER_HOH, WATER_O_MPOLE,
+                         WATER_OH, WATER_POLARITY, WATER_THOLE, WATER_VDW, Topology, masks_from_bonds,
+                         vdw_pair_table)
+
+
+def _random_rotations(rng, n):
+    """Uniform random rotation matrices from uniform unit quaternions."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    R[:, 0, 1] = 2 * (x * y - z * w)
+    R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w)
+    R[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w)
+    R[:, 2, 1] = 2 * (y * z + x * w)
+    R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def sequential_insertion(rng, n, box, min_dist, max_tries=2000):
+    """Random sequential insertion in a cubic/orthorhombic box with a hash grid."""
+    box = np.asarray(box, float)[:3]
+    ncell = np.maximum((box / min_dist).astype(int), 1)
+    csize = box / ncell
+    grid = {}
+    pts = np.empty((n, 3))
+    md2 = min_dist * min_dist
+    count = 0
+    batch = 4096
+    while count < n:
+        cand = rng.random((batch, 3)) * box
+        for c in cand:
+            ci = (c // csize).astype(int) % ncell
+            ok = True
+            for dx in (-1, 0, 1):
+                if not ok:
+                    break
+                for dy in (-1, 0, 1):
+                    if not ok:
+                        break
+                    for dz in (-1, 0, 1):
+                        key = ((ci[0] + dx) % ncell[0], (ci[1] + dy) % ncell[1], (ci[2] + dz) % ncell[2])
+                        lst = grid.get(key)
+                        if not lst:
+                            continue
+                        d = pts[lst] - c
+                        d -= box * np.round(d / box)
+                        if np.any(np.einsum("ij,ij->i", d, d) < md2):
+                            ok = False
+                            break
+            if ok:
+                pts[count] = c
+                grid.setdefault((ci[0], ci[1], ci[2]), []).append(count)
+                count += 1
+                if count == n:
+                    break
+        max_tries -= 1
+        if max_tries <= 0:
+            raise RuntimeError("sequential insertion did not converge")
+    return pts
+
+
+def water_geometry():
+    """Body-frame water: O at the origin, bisector along +z, molecule in xz."""
+    half = np.deg2rad(WATER_HOH) / 2.0
+    h1 = WATER_OH * np.array([np.sin(half), 0.0, np.cos(half)])
+    h2 = WATER_OH * np.array([-np.sin(half), 0.0, np.cos(half)])
+    return np.zeros(3), h1, h2
+
+
+def water_topology(n_waters, moment_scale=(1.0, 1.0), atom_offset=0, molecule_offset=0):
+    """Topology arrays for ``n_waters`` AMOEBA waters in O,H,H order."""
+    n = 3 * n_waters
+    frame = np.zeros(n, np.int32)
+    axis = -np.ones((n, 3), np.int32)
+    mp = np.zeros((n, 10))
+    pol = np.zeros(n)
+    vcls = np.zeros(n, np.int32)
+    red_i = np.arange(n, dtype=np.int32) + atom_offset
+    red_v = np.zeros(n)
+    o_m = WATER_O_MPOLE.copy()
+    h_m = WATER_H_MPOLE.copy()
+    for m in (o_m, h_m):
+        m[1:4] *= moment_scale[0]
+        m[4:10] *= moment_scale[1]
+    bonds = []
+    for w in range(n_waters):
+        o, h1, h2 = 3 * w, 3 * w + 1, 3 * w + 2
+        g = atom_offset
+        frame[o] = FRAME_BISECTOR
+        axis[o] = (h1 + g, h2 + g, -1)
+        frame[h1] = FRAME_ZTHENX
+        axis[h1] = (o + g, h2 + g, -1)
+        frame[h2] = FRAME_ZTHENX
+        axis[h2] = (o + g, h1 + g, -1)
+        mp[o] = o_m
+        mp[h1] = h_m
+        mp[h2] = h_m
+        pol[o] = WATER_POLARITY["O"]
+        pol[h1] = pol[h2] = WATER_POLARITY["H"]
+        vcls[o] = 0
+        vcls[h1] = vcls[h2] = 1
+        red_i[h1] = o + g
+        red_i[h2] = o + g
+        red_v[h1] = red_v[h2] = WATER_VDW["H"][2]
+        bonds += [(o, h1), (o, h2)]
+    off, atoms, rels = masks_from_bonds(n, bonds)
+    atoms = atoms + atom_offset
+    tab = vdw_pair_table([WATER_VDW["O"][0], WATER_VDW["H"][0]], [WATER_VDW["O"][1], WATER_VDW["H"][1]])
+    mol = np.repeat(np.arange(n_waters, dtype=np.int32), 3) + molecule_offset
+    return Topology(n=n, frame=frame, axis=axis, mpole=mp, polarity=pol, thole=np.full(n, WATER_THOLE),
+                    pdamp=pol ** (1.0 / 6.0), group=mol.copy(), molecule=mol, vdw_class=vcls, vdw_table=tab,
+                    red_index=red_i, red_value=red_v, mask_off=off, mask_atom=atoms, mask_rel=rels)
+
+
+def water_box(n_waters, box_length, seed=0, min_oo=2.6, moment_scale=(1.0, 1.0)):
+    """Seeded AMOEBA water box.  Returns (xyz (3n,), box[6], Topology)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    box = np.array([box_length, box_length, box_length, 90.0, 90.0, 90.0])
+    opos = sequential_insertion(rng, n_waters, box, min_oo)
+    R = _random_rotations(rng, n_waters)
+    o, h1, h2 = water_geometry()
+    xyz = np.empty((n_waters, 3, 3))
+    xyz[:, 0] = opos
+    xyz[:, 1] = opos + R @ h1
+    xyz[:, 2] = opos + R @ h2
+    top = water_topology(n_waters, moment_scale)
+    return xyz.reshape(-1), box, top
+
+
+WATER_CONFIGS = {
+    # name: (n_waters, box length, grid, ewald cutoff, vdw cutoff, polar eps)
+    "water-648": (216, 18.643, (24, 24, 24), 7.0, 9.0, 1.0e-6),
+    "water-20736": (6912, 59.1, (64, 64, 64), 7.0, 9.0, 1.0e-5),
+    "water-288000": (96000, 144.3, (144, 144, 144), 7.0, 9.0, 1.0e-5),
+}
