@@ -1,0 +1,409 @@
+// nb_api.hip -- the C ABI of include/amoeba_nb.h and the evaluation driver.
+//
+// nb_evaluate follows ForceFieldEnergy.energy -> VanDerWaals.energy ->
+// ParticleMeshEwald.energy (ForceFieldEnergy.java:1638-1800;
+// ParticleMeshEwald.java:837-954, 1871-2183):
+//   list check/rebuild, sorted coordinates, multipole rotation,
+//   vdW, permanent PME (spread, convolution, gather), permanent real-space
+//   field, direct dipoles, PCG, reciprocal and real-space energies,
+//   torques -> gradients in caller order.
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "nb_state.h"
+
+namespace {
+
+enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT };
+
+void cell_from_box(const double *box, CellD &c) {
+  memset(&c, 0, sizeof(c));
+  if (!box || box[0] <= 0.0) {
+    c.aperiodic = 1;
+    return;
+  }
+  double a = box[0], b = box[1], cc = box[2], al = box[3], be = box[4], ga = box[5];
+  double cos_a, sin_b, cos_b, sin_g, cos_g, beta_term, gamma_term;
+  if (al == 90.0 && be == 90.0 && ga == 90.0) {
+    cos_a = 0.0; cos_b = 0.0; sin_b = 1.0; sin_g = 1.0; cos_g = 0.0; beta_term = 0.0; gamma_term = 1.0;
+  } else {
+    double d2r = M_PI / 180.0;
+    cos_a = cos(al * d2r);
+    sin_b = sin(be * d2r); cos_b = cos(be * d2r);
+    sin_g = sin(ga * d2r); cos_g = cos(ga * d2r);
+    beta_term = (cos_a - cos_b * cos_g) / sin_g;
+    gamma_term = sqrt(sin_b * sin_b - beta_term * beta_term);
+  }
+  c.vol = sin_g * gamma_term * a * b * cc;
+  double Ai[3][3] = {{a, 0.0, 0.0}, {b * cos_g, b * sin_g, 0.0}, {cc * cos_b, cc * beta_term, cc * gamma_term}};
+  double det = Ai[0][0] * (Ai[1][1] * Ai[2][2] - Ai[1][2] * Ai[2][1]) - Ai[0][1] * (Ai[1][0] * Ai[2][2] - Ai[1][2] * Ai[2][0]) +
+               Ai[0][2] * (Ai[1][0] * Ai[2][1] - Ai[1][1] * Ai[2][0]);
+  double id = 1.0 / det;
+  double A[3][3];
+  A[0][0] = (Ai[1][1] * Ai[2][2] - Ai[1][2] * Ai[2][1]) * id;
+  A[0][1] = (Ai[0][2] * Ai[2][1] - Ai[0][1] * Ai[2][2]) * id;
+  A[0][2] = (Ai[0][1] * Ai[1][2] - Ai[0][2] * Ai[1][1]) * id;
+  A[1][0] = (Ai[1][2] * Ai[2][0] - Ai[1][0] * Ai[2][2]) * id;
+  A[1][1] = (Ai[0][0] * Ai[2][2] - Ai[0][2] * Ai[2][0]) * id;
+  A[1][2] = (Ai[0][2] * Ai[1][0] - Ai[0][0] * Ai[1][2]) * id;
+  A[2][0] = (Ai[1][0] * Ai[2][1] - Ai[1][1] * Ai[2][0]) * id;
+  A[2][1] = (Ai[0][1] * Ai[2][0] - Ai[0][0] * Ai[2][1]) * id;
+  A[2][2] = (Ai[0][0] * Ai[1][1] - Ai[0][1] * Ai[1][0]) * id;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) c.A[i][j] = A[i][j];
+  c.A00 = A[0][0]; c.A10 = A[1][0]; c.A11 = A[1][1]; c.A20 = A[2][0]; c.A21 = A[2][1]; c.A22 = A[2][2];
+  c.Ai00 = Ai[0][0]; c.Ai10 = Ai[1][0]; c.Ai11 = Ai[1][1]; c.Ai20 = Ai[2][0]; c.Ai21 = Ai[2][1]; c.Ai22 = Ai[2][2];
+}
+
+template <typename T>
+void upload(T **dst, const T *src, size_t count) {
+  HIPCHECK(hipMalloc((void **)dst, sizeof(T) * (count > 0 ? count : 1)));
+  if (count) HIPCHECK(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+}
+
+template <typename T>
+void alloc(T **dst, size_t count) {
+  HIPCHECK(hipMalloc((void **)dst, sizeof(T) * (count > 0 ? count : 1)));
+}
+
+void free_all(NbHandle *h) {
+  void *ptrs[] = {h->d_frame, h->d_axis, h->d_group0, h->d_vclass0, h->d_redidx, h->d_mpole_local, h->d_polar0,
+                  h->d_thole0, h->d_ipd0, h->d_redval, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_vdw_table,
+                  h->d_xyz, h->d_xyz_list, h->d_perm, h->d_iperm, h->d_cellid, h->d_cellcount, h->d_cellstart,
+                  h->d_nbr, h->d_ew, h->d_pc, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_red4, h->d_mp,
+                  h->d_gmp, h->d_polar, h->d_group, h->d_vclass, h->d_grid, h->d_recip, h->d_theta, h->d_igrid,
+                  h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
+                  h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
+                  h->d_scal, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (h->h_pinned) hipHostFree(h->h_pinned);
+  if (h->stream) hipStreamDestroy(h->stream);
+}
+
+int fail(NbHandle *h, const std::exception &e, int code) {
+  if (h) h->err = e.what();
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nb_abi_version(void) { return NB_ABI_VERSION; }
+
+int nb_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
+  if (!params || !top || !handle || top->n_atoms <= 0) return NB_ERR_INVALID;
+  int ndev = nb_device_count();
+  if (ndev <= 0) return NB_ERR_NO_DEVICE;
+  NbHandle *h = new NbHandle();
+  try {
+    h->prm = *params;
+    h->n = top->n_atoms;
+    h->device = params->device;
+    HIPCHECK(hipSetDevice(h->device));
+    HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    int n = h->n;
+    if (n >= (int)NB_INDEX_MASK) throw std::runtime_error("too many atoms");
+    for (int d = 0; d < 3; d++) {
+      int m = params->grid[d];
+      if (params->do_pme && m < 4) throw std::runtime_error("PME grid must be at least 4 per dimension");
+      while (m > 1 && m % 2 == 0) m /= 2;
+      while (m > 1 && m % 3 == 0) m /= 3;
+      while (m > 1 && m % 5 == 0) m /= 5;
+      if (params->do_pme && m != 1) throw std::runtime_error("PME grid dimensions must factor over 2, 3 and 5");
+    }
+    if (params->pme_order != 5) throw std::runtime_error("only B-spline order 5 is implemented");
+    // every index a kernel dereferences is validated here
+    for (int i = 0; i < n; i++) {
+      int f = top->frame[i];
+      if (f < NB_FRAME_NONE || f > NB_FRAME_THREEFOLD) throw std::runtime_error("invalid frame type");
+      int need = f == NB_FRAME_NONE ? 0 : (f == NB_FRAME_ZONLY ? 1 : (f == NB_FRAME_ZTHENBISECTOR || f == NB_FRAME_THREEFOLD ? 3 : 2));
+      for (int a = 0; a < need; a++) {
+        int k = top->axis[3 * i + a];
+        if (k < 0 || k >= n || k == i) throw std::runtime_error("invalid frame axis atom");
+      }
+      int r = top->red_index[i];
+      if (r < 0 || r >= n) throw std::runtime_error("invalid reduction index");
+      if (top->vdw_class[i] < 0 || top->vdw_class[i] >= top->n_vdw_class) throw std::runtime_error("invalid vdW class");
+      if (top->mask_offset[i + 1] < top->mask_offset[i]) throw std::runtime_error("invalid mask offsets");
+    }
+    if (top->mask_offset[0] != 0) throw std::runtime_error("invalid mask offsets");
+    for (int t = 0; t < top->mask_offset[n]; t++)
+      if (top->mask_atom[t] < 0 || top->mask_atom[t] >= n) throw std::runtime_error("invalid mask atom");
+    std::vector<double> ipd(n);
+    for (int i = 0; i < n; i++) ipd[i] = top->pdamp[i] > 0.0 ? 1.0 / top->pdamp[i] : INFINITY;
+    upload(&h->d_frame, top->frame, n);
+    upload(&h->d_axis, top->axis, 3 * (size_t)n);
+    upload(&h->d_group0, top->group, n);
+    upload(&h->d_vclass0, top->vdw_class, n);
+    upload(&h->d_redidx, top->red_index, n);
+    upload(&h->d_mpole_local, top->multipole, 10 * (size_t)n);
+    upload(&h->d_polar0, top->polarizability, n);
+    upload(&h->d_thole0, top->thole, n);
+    upload(&h->d_ipd0, ipd.data(), n);
+    upload(&h->d_redval, top->red_value, n);
+    int nnz = top->mask_offset[n];
+    upload(&h->d_mask_off, top->mask_offset, n + 1);
+    upload(&h->d_mask_atom, top->mask_atom, nnz);
+    upload(&h->d_mask_rel, top->mask_rel, nnz);
+    h->n_class = top->n_vdw_class;
+    upload(&h->d_vdw_table, top->vdw_table, 2 * (size_t)h->n_class * h->n_class);
+    size_t n3 = 3 * (size_t)n;
+    alloc(&h->d_xyz, n3);
+    alloc(&h->d_xyz_list, n3);
+    alloc(&h->d_perm, n);
+    alloc(&h->d_iperm, n);
+    alloc(&h->d_cellid, n);
+    alloc(&h->d_nnb, n);
+    alloc(&h->d_new, n);
+    alloc(&h->d_npc, n);
+    alloc(&h->d_flags, 8);
+    HIPCHECK(hipMemset(h->d_flags, 0, 8 * sizeof(int32_t)));
+    alloc(&h->d_pos4, n);
+    alloc(&h->d_red4, n);
+    alloc(&h->d_mp, n);
+    HIPCHECK(hipMemset(h->d_mp, 0, sizeof(AtomMP) * n));
+    alloc(&h->d_gmp, 10 * (size_t)n);
+    alloc(&h->d_polar, n);
+    alloc(&h->d_group, n);
+    alloc(&h->d_vclass, n);
+    alloc(&h->d_theta, 60 * (size_t)n);
+    alloc(&h->d_igrid, n3);
+    alloc(&h->d_fmp, 10 * (size_t)n);
+    alloc(&h->d_phip, 30 * (size_t)n);
+    alloc(&h->d_phiu, 30 * (size_t)n);
+    alloc(&h->d_phiup, 30 * (size_t)n);
+    HIPCHECK(hipMemset(h->d_phip, 0, sizeof(double) * 30 * n));
+    HIPCHECK(hipMemset(h->d_phiu, 0, sizeof(double) * 30 * n));
+    HIPCHECK(hipMemset(h->d_phiup, 0, sizeof(double) * 30 * n));
+    double **vecs[] = {&h->d_field, &h->d_fieldp, &h->d_u, &h->d_up, &h->d_udir, &h->d_updir, &h->d_r, &h->d_rp,
+                       &h->d_z, &h->d_zp, &h->d_p, &h->d_pp, &h->d_vec, &h->d_vecp, &h->d_grad_s, &h->d_torque_s,
+                       &h->d_gred_s, &h->d_grad};
+    for (double **v : vecs) {
+      alloc(v, n3);
+      HIPCHECK(hipMemset(*v, 0, sizeof(double) * n3));
+    }
+    alloc(&h->d_scal, 16);
+    alloc(&h->d_energy, 16);
+    alloc(&h->d_count, 4);
+    HIPCHECK(hipHostMalloc((void **)&h->h_pinned, sizeof(double) * 64, hipHostMallocDefault));
+    memset(&h->last, 0, sizeof(h->last));
+    *handle = h;
+    return NB_OK;
+  } catch (std::exception &e) {
+    int code = NB_ERR_HIP;
+    if (std::string(e.what()).find("hip") == std::string::npos) code = NB_ERR_INVALID;
+    free_all(h);
+    delete h;
+    return code;
+  }
+}
+
+int nb_set_cell(void *handle, const double box[6]) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    double zero[6] = {0, 0, 0, 90, 90, 90};
+    const double *b = box ? box : zero;
+    for (int d = 0; d < 6; d++) h->box[d] = b[d];
+    cell_from_box(b, h->cell);
+    h->cell_set = true;
+    h->list_valid = false;
+    if (h->prm.do_pme && !h->cell.aperiodic && h->prm.ewald_alpha > 0.0) nb_pme_setup(h);
+    return NB_OK;
+  } catch (std::exception &e) {
+    return fail(h, e, NB_ERR_HIP);
+  }
+}
+
+int nb_set_positions(void *handle, const double *xyz) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h || !xyz) return NB_ERR_INVALID;
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    HIPCHECK(hipMemcpyAsync(h->d_xyz, xyz, sizeof(double) * 3 * h->n, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    h->positions_set = true;
+    return NB_OK;
+  } catch (std::exception &e) {
+    return fail(h, e, NB_ERR_HIP);
+  }
+}
+
+int nb_set_positions_device(void *handle, const void *d_xyz) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h || !d_xyz) return NB_ERR_INVALID;
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    HIPCHECK(hipMemcpyAsync(h->d_xyz, d_xyz, sizeof(double) * 3 * h->n, hipMemcpyDeviceToDevice, h->stream));
+    h->positions_set = true;
+    return NB_OK;
+  } catch (std::exception &e) {
+    return fail(h, e, NB_ERR_HIP);
+  }
+}
+
+int nb_set_lambda(void *handle, double lambda) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  if (lambda < 0.0 || lambda > 1.0) return NB_ERR_INVALID;
+  return NB_OK;
+}
+
+int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  if (!h->cell_set || !h->positions_set) {
+    h->err = "nb_set_cell and nb_set_positions must be called before nb_evaluate";
+    return NB_ERR_INVALID;
+  }
+  nb_energies e;
+  memset(&e, 0, sizeof(e));
+  int status = NB_OK;
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    int n = h->n;
+    bool rebuild = nb_need_rebuild(h);
+    if (rebuild) nb_build_lists(h);
+    else nb_gather_sorted(h);
+    e.list_rebuilt = rebuild ? 1 : 0;
+    nb_zero_outputs(h);
+    if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
+    if (h->prm.do_vdw) nb_vdw(h);
+    bool polar = h->prm.polarization != NB_POL_NONE;
+    bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+    if (h->prm.do_pme) {
+      if (use_recip) nb_pme_permanent(h);
+      nb_perm_field(h);
+      if (polar) {
+        nb_direct(h);
+        if (h->prm.polarization == NB_POL_MUTUAL) {
+          status = nb_pcg(h, &e);
+        } else {
+          nb_induced_field(h, h->d_u, h->d_up, h->d_field, h->d_fieldp);
+        }
+      }
+      if (use_recip) nb_recip_energy(h, polar);
+      nb_real_energy(h, polar);
+    }
+    nb_finalize_gradient(h);
+    double en[8];
+    int64_t cnt[4];
+    int32_t fl[8];
+    HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    memcpy(en, h->h_pinned, sizeof(en));
+    memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
+    memcpy(fl, h->h_pinned + 12, sizeof(fl));
+    e.perm_real = en[E_PERM_REAL];
+    e.pol_real = en[E_POL_REAL];
+    e.vdw = en[E_VDW];
+    if (use_recip && h->prm.do_pme) {
+      e.perm_self = en[E_PERM_SELF];
+      e.perm_recip = en[E_PERM_RECIP];
+      e.pol_self = en[E_POL_SELF];
+      e.pol_recip = en[E_POL_RECIP];
+      double q = en[E_QTOT];
+      if (q != 0.0) e.perm_charge_corr = -0.5 * h->prm.electric * M_PI * q * q / (h->cell.vol * h->prm.ewald_alpha * h->prm.ewald_alpha);
+    }
+    e.n_vdw = cnt[0] / 2;
+    e.n_perm = cnt[1] / 2 + ((use_recip && h->prm.do_pme) ? n : 0);
+    e.perm = e.perm_real + e.perm_self + e.perm_recip + e.perm_charge_corr;
+    e.pol = e.pol_real + e.pol_self + e.pol_recip;
+    e.total = e.perm + e.pol + e.vdw;
+    if (status == NB_OK && fl[4]) status = NB_ERR_BAD_TORQUE;
+    if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
+    h->last = e;
+    if (out) *out = e;
+    if (status != NB_OK) {
+      const char *msg[] = {"", "NaN energy", "SCF convergence failure (residual increased)",
+                           "maximum SCF iterations reached", "undefined torque"};
+      h->err = status < 5 ? msg[status] : "error";
+    }
+    return status;
+  } catch (std::exception &ex) {
+    return fail(h, ex, NB_ERR_HIP);
+  }
+}
+
+int nb_get_gradient(void *handle, double *g) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h || !g) return NB_ERR_INVALID;
+  try {
+    HIPCHECK(hipMemcpyAsync(g, h->d_grad, sizeof(double) * 3 * h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    return NB_OK;
+  } catch (std::exception &e) {
+    return fail(h, e, NB_ERR_HIP);
+  }
+}
+
+const void *nb_gradient_device(void *handle) {
+  NbHandle *h = (NbHandle *)handle;
+  return h ? h->d_grad : nullptr;
+}
+
+int nb_get_induced(void *handle, double *u, double *ucr) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  try {
+    int n = h->n;
+    std::vector<double> us(3 * n), ups(3 * n);
+    std::vector<int32_t> perm(n);
+    HIPCHECK(hipMemcpyAsync(us.data(), h->d_u, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(ups.data(), h->d_up, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipMemcpyAsync(perm.data(), h->d_perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    for (int s = 0; s < n; s++) {
+      int i = perm[s];
+      for (int d = 0; d < 3; d++) {
+        if (u) u[3 * i + d] = us[3 * s + d];
+        if (ucr) ucr[3 * i + d] = ups[3 * s + d];
+      }
+    }
+    return NB_OK;
+  } catch (std::exception &e) {
+    return fail(h, e, NB_ERR_HIP);
+  }
+}
+
+int nb_get_virial(void *handle, double v[9]) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h || !v) return NB_ERR_INVALID;
+  h->err = "virial is not implemented";
+  return NB_ERR_INVALID;
+}
+
+int nb_get_dudx_dl(void *handle, double *g) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h || !g) return NB_ERR_INVALID;
+  h->err = "dU/dX/dlambda is not implemented";
+  return NB_ERR_INVALID;
+}
+
+const char *nb_last_error(void *handle) {
+  NbHandle *h = (NbHandle *)handle;
+  return h ? h->err.c_str() : "invalid handle";
+}
+
+void nb_destroy(void *handle) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return;
+  hipSetDevice(h->device);
+  free_all(h);
+  delete h;
+}
+
+}  // extern "C"

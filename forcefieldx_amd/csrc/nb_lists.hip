@@ -1,0 +1,264 @@
+// nb_lists.hip -- spatial sort and Verlet neighbor list on the GPU
+// (NeighborList.java:351-372, 514-578, 777-1120, 1296-1545 restated for a
+// one-thread-per-atom full list in spatially sorted order).
+//
+// Atoms are binned by fractional coordinate into cells at least one list
+// cutoff wide, counting-sorted by cell (ties broken by caller index, so the
+// order is deterministic), and each sorted atom scans the 27 (or fewer for
+// thin boxes) neighbouring cells.  The list is full (both directions) so the
+// pair kernels accumulate only their own atom and need no atomics.  The list
+// is rebuilt only when some atom moved more than half the buffer since the
+// last build (NeighborList.java:314, 777-802).
+#include <stdexcept>
+
+#include "nb_state.h"
+
+namespace {
+
+__global__ void k_max_disp(int n, const double *__restrict__ xyz, const double *__restrict__ xyz0, double lim2,
+                           int32_t *flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double dx = xyz[3 * i] - xyz0[3 * i], dy = xyz[3 * i + 1] - xyz0[3 * i + 1], dz = xyz[3 * i + 2] - xyz0[3 * i + 2];
+  if (dx * dx + dy * dy + dz * dz > lim2) atomicOr(&flags[3], 1);
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) {
+  int m = i % n;
+  return m < 0 ? m + n : m;
+}
+
+__device__ __forceinline__ int cell_of(const CellD &c, int nc0, int nc1, int nc2, double x, double y, double z) {
+  int ci[3];
+  double f[3];
+  if (c.aperiodic) return 0;
+  f[0] = x * c.A00 + y * c.A10 + z * c.A20;
+  f[1] = y * c.A11 + z * c.A21;
+  f[2] = z * c.A22;
+  int nc[3] = {nc0, nc1, nc2};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double w = f[d] - floor(f[d]);
+    int q = (int)(w * nc[d]);
+    ci[d] = q >= nc[d] ? nc[d] - 1 : (q < 0 ? 0 : q);
+  }
+  return ci[0] + nc0 * (ci[1] + nc1 * ci[2]);
+}
+
+__global__ void k_cell_assign(int n, const double *__restrict__ xyz, CellD cell, int nc0, int nc1, int nc2,
+                              int32_t *__restrict__ cellid, int32_t *__restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell_of(cell, nc0, nc1, nc2, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+  cellid[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// exclusive scan of count[0..m) into start[0..m], single workgroup
+__global__ void k_scan(int m, const int32_t *__restrict__ count, int32_t *__restrict__ start) {
+  __shared__ int32_t part[1024];
+  int t = threadIdx.x, nt = blockDim.x;
+  int chunk = (m + nt - 1) / nt;
+  int b = t * chunk, e = min(m, b + chunk);
+  int s = 0;
+  for (int i = b; i < e; i++) s += count[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int i = 0; i < nt; i++) {
+      int v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+    start[m] = acc;
+  }
+  __syncthreads();
+  int acc = part[t];
+  for (int i = b; i < e; i++) {
+    start[i] = acc;
+    acc += count[i];
+  }
+}
+
+__global__ void k_cell_scatter(int n, const int32_t *__restrict__ cellid, const int32_t *__restrict__ start,
+                               int32_t *__restrict__ cursor, int32_t *__restrict__ perm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cellid[i];
+  int slot = atomicAdd(&cursor[c], 1);
+  perm[start[c] + slot] = i;
+}
+
+// order each cell's segment by caller index (deterministic sort)
+__global__ void k_cell_sortseg(int m, const int32_t *__restrict__ start, int32_t *__restrict__ perm) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  int b = start[c], e = start[c + 1];
+  for (int i = b + 1; i < e; i++) {
+    int v = perm[i];
+    int j = i - 1;
+    while (j >= b && perm[j] > v) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+__global__ void k_iperm(int n, const int32_t *__restrict__ perm, int32_t *__restrict__ iperm) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) iperm[perm[s]] = s;
+}
+
+// static per-atom data in sorted order
+__global__ void k_gather_static(int n, const int32_t *__restrict__ perm, const double *__restrict__ polar0,
+                                const double *__restrict__ thole0, const double *__restrict__ ipd0,
+                                const int32_t *__restrict__ group0, const int32_t *__restrict__ vclass0,
+                                double *__restrict__ polar, int32_t *__restrict__ group, int32_t *__restrict__ vclass,
+                                AtomMP *__restrict__ mp) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int i = perm[s];
+  polar[s] = polar0[i];
+  group[s] = group0[i];
+  vclass[s] = vclass0[i];
+  mp[s].ipd = ipd0[i];
+  mp[s].thole = thole0[i];
+}
+
+// bonded relation of caller atoms i,k (0 if none): linear scan of i's CSR row
+__device__ __forceinline__ int relation(const int32_t *__restrict__ off, const int32_t *__restrict__ atom,
+                                        const int32_t *__restrict__ rel, int i, int k) {
+  int b = off[i], e = off[i + 1];
+  for (int t = b; t < e; t++)
+    if (atom[t] == k) return rel[t];
+  return 0;
+}
+
+__global__ void __launch_bounds__(256) k_build_list(int n, const double4 *__restrict__ pos, const int32_t *__restrict__ perm,
+                                                    const int32_t *__restrict__ start, CellD cell, int nc0, int nc1,
+                                                    int nc2, double cut2, const int32_t *__restrict__ group,
+                                                    const int32_t *__restrict__ moff, const int32_t *__restrict__ matom,
+                                                    const int32_t *__restrict__ mrel, uint32_t *__restrict__ nbr,
+                                                    int maxnb, int32_t *__restrict__ nnb, int32_t *__restrict__ flags) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double4 pi = pos[s];
+  int ci = cell_of(cell, nc0, nc1, nc2, pi.x, pi.y, pi.z);
+  int cx = ci % nc0, cy = (ci / nc0) % nc1, cz = ci / (nc0 * nc1);
+  int lo0 = nc0 >= 3 ? -1 : 0, hi0 = nc0 >= 2 ? 1 : 0;
+  int lo1 = nc1 >= 3 ? -1 : 0, hi1 = nc1 >= 2 ? 1 : 0;
+  int lo2 = nc2 >= 3 ? -1 : 0, hi2 = nc2 >= 2 ? 1 : 0;
+  int i = perm[s];
+  int gi = group[s];
+  int cnt = 0;
+  for (int dz = lo2; dz <= hi2; dz++)
+    for (int dy = lo1; dy <= hi1; dy++)
+      for (int dx = lo0; dx <= hi0; dx++) {
+        int q = wrapi(cx + dx, nc0) + nc0 * (wrapi(cy + dy, nc1) + nc1 * wrapi(cz + dz, nc2));
+        int b = start[q], e = start[q + 1];
+        for (int t = b; t < e; t++) {
+          if (t == s) continue;
+          double4 pk = pos[t];
+          double ddx = pk.x - pi.x, ddy = pk.y - pi.y, ddz = pk.z - pi.z;
+          double r2 = image(cell, ddx, ddy, ddz);
+          if (r2 > cut2) continue;
+          int code = relation(moff, matom, mrel, i, perm[t]) + (group[t] == gi ? 8 : 0);
+          if (cnt < maxnb) nbr[(size_t)cnt * n + s] = (uint32_t)t | ((uint32_t)code << NB_CODE_SHIFT);
+          cnt++;
+        }
+      }
+  nnb[s] = cnt;
+  if (cnt > maxnb) atomicMax(&flags[0], cnt);
+}
+
+}  // namespace
+
+static int estimate_neighbors(const NbHandle *h, double r) {
+  double vol = h->cell.aperiodic ? 1.0 : h->cell.vol;
+  double rho = h->cell.aperiodic ? 0.12 : h->n / vol;
+  double v = 4.0 / 3.0 * M_PI * r * r * r;
+  int est = (int)(rho * v * 1.4) + 64;
+  if (est > h->n) est = h->n;
+  return est < 16 ? 16 : est;
+}
+
+bool nb_need_rebuild(NbHandle *h) {
+  if (!h->list_valid) return true;
+  int n = h->n;
+  int32_t zero = 0;
+  HIPCHECK(hipMemcpyAsync(h->d_flags + 3, &zero, sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+  double lim = 0.5 * h->prm.list_buffer;
+  hipLaunchKernelGGL(k_max_disp, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, h->d_xyz, h->d_xyz_list, lim * lim,
+                     h->d_flags);
+  int32_t flag = 0;
+  HIPCHECK(hipMemcpyAsync(&flag, h->d_flags + 3, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  return flag != 0;
+}
+
+void nb_build_lists(NbHandle *h) {
+  int n = h->n;
+  double cut = (h->prm.vdw_cutoff > h->prm.ewald_cutoff ? h->prm.vdw_cutoff : h->prm.ewald_cutoff) + h->prm.list_buffer;
+  // cell grid
+  int nc[3] = {1, 1, 1};
+  if (!h->cell.aperiodic) {
+    for (int d = 0; d < 3; d++) {
+      double col = sqrt(h->cell.A[0][d] * h->cell.A[0][d] + h->cell.A[1][d] * h->cell.A[1][d] +
+                        h->cell.A[2][d] * h->cell.A[2][d]);
+      int q = (int)floor(1.0 / (col * cut));
+      nc[d] = q < 1 ? 1 : (q > 128 ? 128 : q);
+    }
+  }
+  int m = nc[0] * nc[1] * nc[2];
+  if (m + 1 > h->ncell_cap) {
+    if (h->d_cellcount) { hipFree(h->d_cellcount); hipFree(h->d_cellstart); }
+    h->ncell_cap = m + 1;
+    HIPCHECK(hipMalloc(&h->d_cellcount, sizeof(int32_t) * (m + 1)));
+    HIPCHECK(hipMalloc(&h->d_cellstart, sizeof(int32_t) * (m + 1)));
+  }
+  h->ncell[0] = nc[0]; h->ncell[1] = nc[1]; h->ncell[2] = nc[2];
+  h->ncell_tot = m;
+  dim3 B(256), G((n + 255) / 256);
+  HIPCHECK(hipMemsetAsync(h->d_cellcount, 0, sizeof(int32_t) * (m + 1), h->stream));
+  hipLaunchKernelGGL(k_cell_assign, G, B, 0, h->stream, n, h->d_xyz, h->cell, nc[0], nc[1], nc[2], h->d_cellid,
+                     h->d_cellcount);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, m, h->d_cellcount, h->d_cellstart);
+  HIPCHECK(hipMemsetAsync(h->d_cellcount, 0, sizeof(int32_t) * (m + 1), h->stream));
+  hipLaunchKernelGGL(k_cell_scatter, G, B, 0, h->stream, n, h->d_cellid, h->d_cellstart, h->d_cellcount, h->d_perm);
+  hipLaunchKernelGGL(k_cell_sortseg, dim3((m + 255) / 256), B, 0, h->stream, m, h->d_cellstart, h->d_perm);
+  hipLaunchKernelGGL(k_iperm, G, B, 0, h->stream, n, h->d_perm, h->d_iperm);
+  hipLaunchKernelGGL(k_gather_static, G, B, 0, h->stream, n, h->d_perm, h->d_polar0, h->d_thole0, h->d_ipd0,
+                     h->d_group0, h->d_vclass0, h->d_polar, h->d_group, h->d_vclass, h->d_mp);
+  nb_gather_sorted(h);
+  // neighbor rows, grown on overflow
+  if (h->maxnb == 0) h->maxnb = estimate_neighbors(h, cut);
+  for (int attempt = 0; attempt < 4; attempt++) {
+    if (!h->d_nbr) HIPCHECK(hipMalloc(&h->d_nbr, sizeof(uint32_t) * (size_t)h->maxnb * n));
+    int32_t zeros[4] = {0, 0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(h->d_flags, zeros, sizeof(zeros), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_build_list, G, B, 0, h->stream, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell, nc[0],
+                       nc[1], nc[2], cut * cut, h->d_group, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_nbr,
+                       h->maxnb, h->d_nnb, h->d_flags);
+    int32_t f0 = 0;
+    HIPCHECK(hipMemcpyAsync(&f0, h->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    if (f0 <= h->maxnb) break;
+    if (attempt == 3) throw std::runtime_error("neighbor list capacity could not be established");
+    hipFree(h->d_nbr);
+    h->d_nbr = nullptr;
+    h->maxnb = f0 + f0 / 4 + 32;
+  }
+  // Ewald / preconditioner rows sized from the list row
+  if (h->maxew == 0) {
+    h->maxew = estimate_neighbors(h, h->prm.ewald_cutoff);
+    if (h->maxew > h->maxnb) h->maxew = h->maxnb;
+    h->maxpc = estimate_neighbors(h, h->prm.precond_cutoff);
+    if (h->maxpc > h->maxew) h->maxpc = h->maxew;
+    HIPCHECK(hipMalloc(&h->d_ew, sizeof(uint32_t) * (size_t)h->maxew * n));
+    HIPCHECK(hipMalloc(&h->d_pc, sizeof(uint32_t) * (size_t)h->maxpc * n));
+  }
+  HIPCHECK(hipMemcpyAsync(h->d_xyz_list, h->d_xyz, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, h->stream));
+  h->list_valid = true;
+}

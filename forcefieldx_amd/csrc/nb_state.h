@@ -1,0 +1,129 @@
+// nb_state.h -- device state of one AMOEBA nonbonded handle.
+//
+// Layout in HBM (all per-atom arrays in *sorted* order unless noted):
+//   perm[s] / iperm[i]       sorted position <-> caller atom index
+//   pos4[s]                  double4 x, y, z, 0 (Cartesian, caller coordinates)
+//   mp[s]                    AtomMP: q, d, Q/3 (global frame), 1/pdamp, thole
+//   polar[s], group[s]       polarizability, polarization group
+//   nbr[t * n + s]           full Verlet list (column-major: coalesced over s),
+//                            entry = k_sorted | mask code << 27
+//   ew / pc [t * n + s]      Ewald (r <= off) and preconditioner (r < 4.5) rows
+//   grid                     interleaved complex PME grid (x fastest)
+// Gradients are returned in caller order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/amoeba_nb.h"
+#include "nb_device.h"
+
+#define NB_CODE_SHIFT 27
+#define NB_INDEX_MASK ((1u << NB_CODE_SHIFT) - 1u)
+
+struct MaskScales { double m[32], p[32], d[32], v[32]; };
+
+struct NbHandle {
+  nb_params prm;
+  int n = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // topology (caller order)
+  int32_t *d_frame = nullptr, *d_axis = nullptr, *d_group0 = nullptr, *d_vclass0 = nullptr, *d_redidx = nullptr;
+  double *d_mpole_local = nullptr, *d_polar0 = nullptr, *d_thole0 = nullptr, *d_ipd0 = nullptr, *d_redval = nullptr;
+  int32_t *d_mask_off = nullptr, *d_mask_atom = nullptr, *d_mask_rel = nullptr;
+  int n_class = 0;
+  double *d_vdw_table = nullptr;
+  MaskScales scales;
+
+  // cell
+  CellD cell;
+  double box[6];
+  bool cell_set = false;
+
+  // coordinates
+  double *d_xyz = nullptr;        // caller order, interleaved
+  double *d_xyz_list = nullptr;   // coordinates at the last list build
+  bool positions_set = false;
+
+  // sorting / lists
+  int32_t *d_perm = nullptr, *d_iperm = nullptr, *d_cellid = nullptr, *d_cellcount = nullptr, *d_cellstart = nullptr;
+  int ncell[3] = {1, 1, 1};
+  int ncell_tot = 0, ncell_cap = 0;
+  uint32_t *d_nbr = nullptr;  int maxnb = 0;
+  uint32_t *d_ew = nullptr;   int maxew = 0;
+  uint32_t *d_pc = nullptr;   int maxpc = 0;
+  int32_t *d_nnb = nullptr, *d_new = nullptr, *d_npc = nullptr;
+  int32_t *d_flags = nullptr;     // [0] overflow nbr, [1] overflow ew, [2] overflow pc, [3] rebuild needed
+  bool list_valid = false;
+
+  // sorted per-atom data
+  double4 *d_pos4 = nullptr;
+  double4 *d_red4 = nullptr;      // reduced vdW sites
+  AtomMP *d_mp = nullptr;
+  double *d_gmp = nullptr;        // [10n] global multipoles (full Q), sorted
+  double *d_polar = nullptr;
+  int32_t *d_group = nullptr, *d_vclass = nullptr;
+
+  // PME
+  int K[3] = {0, 0, 0};
+  long ntot = 0;
+  double *d_grid = nullptr;       // interleaved complex [2 ntot]
+  double *d_recip = nullptr;      // influence function [ntot]
+  double *d_theta = nullptr;      // [n][3][5][4]
+  int32_t *d_igrid = nullptr;     // [n][3]
+  double *d_fmp = nullptr;        // [10n] fractional multipoles
+  double *d_phip = nullptr;       // [30n] permanent phi: 20 fractional + 10 Cartesian
+  double *d_phiu = nullptr;       // [30n] induced u phi
+  double *d_phiup = nullptr;      // [30n] induced uCR phi
+  double *d_tw = nullptr;         // twiddle tables
+  double afrac[3][3];             // a[f][c] = K_f A[c][f]
+  bool recip_valid = false;
+
+  // fields / dipoles / PCG vectors (sorted, [3n] each, xyz interleaved)
+  double *d_field = nullptr, *d_fieldp = nullptr;
+  double *d_u = nullptr, *d_up = nullptr, *d_udir = nullptr, *d_updir = nullptr;
+  double *d_r = nullptr, *d_rp = nullptr, *d_z = nullptr, *d_zp = nullptr, *d_p = nullptr, *d_pp = nullptr;
+  double *d_vec = nullptr, *d_vecp = nullptr;
+  double *d_scal = nullptr;       // PCG scalars
+
+  // outputs
+  double *d_grad_s = nullptr;     // [3n] sorted direct gradient
+  double *d_torque_s = nullptr;   // [3n]
+  double *d_gred_s = nullptr;     // [3n] vdW gradient on reduced sites
+  double *d_grad = nullptr;       // [3n] caller order
+  double *d_energy = nullptr;     // per-block energy partials
+  int64_t *d_count = nullptr;
+  int energy_blocks = 0;
+  double *h_pinned = nullptr;
+
+  nb_energies last;
+};
+
+// launch helpers implemented in the kernel files
+void nb_build_lists(NbHandle *h);
+bool nb_need_rebuild(NbHandle *h);
+void nb_gather_sorted(NbHandle *h);
+void nb_rotate(NbHandle *h);
+void nb_pme_setup(NbHandle *h);
+void nb_pme_permanent(NbHandle *h);
+void nb_pme_induced(NbHandle *h, const double *u, const double *up);
+void nb_perm_field(NbHandle *h);
+void nb_direct(NbHandle *h);
+void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f, double *fp);
+void nb_precond_field(NbHandle *h, const double *r, const double *rp, double *f, double *fp);
+int nb_pcg(NbHandle *h, nb_energies *out);
+void nb_recip_energy(NbHandle *h, bool polar);
+void nb_real_energy(NbHandle *h, bool polar);
+void nb_vdw(NbHandle *h);
+void nb_finalize_gradient(NbHandle *h);
+void nb_zero_outputs(NbHandle *h);
+
+#define HIPCHECK(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)

@@ -1,0 +1,173 @@
+"""Host-side mirror of the reference's energy API for the nonbonded path.
+
+``ForceFieldEnergy`` implements ``energy(x)`` / ``energyAndGradient(x, g)``
+(Potential interface, OptimizationInterface.java:69-129;
+ForceFieldEnergy.java:1806-1893) over the HIP library, and exposes the
+per-term objects ``ParticleMeshEwald`` and ``VanDerWaals`` with the
+reference's getters (ParticleMeshEwald.java:1014-1127, VanDerWaals.java:
+573-583).  Errors from the library are raised as ``EnergyException`` the way
+the reference throws them (RealSpaceEnergyRegion.java:384-391,
+PCGSolver.java:553-569, ReduceRegion.java:208-229); on an SCF failure the
+caller-visible policy of ParticleMeshEwald.selfConsistentField
+(:2300-2325) is applied: retry with direct polarization when
+``direct_fallback`` is set.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import ffi
+from .forcefield import NonbondedParams
+
+
+class EnergyException(RuntimeError):
+    """ffx.potential.utils.EnergyException"""
+
+
+class VanDerWaals:
+    def __init__(self, ffe):
+        self._ffe = ffe
+
+    def getEnergy(self):
+        return self._ffe._last["vdw"]
+
+    def getInteractions(self):
+        return int(self._ffe._last["n_vdw"])
+
+    def energy(self, gradient=False, print_=False):
+        self._ffe._evaluate(gradient)
+        return self.getEnergy()
+
+
+class ParticleMeshEwald:
+    def __init__(self, ffe):
+        self._ffe = ffe
+
+    def energy(self, gradient=False, print_=False):
+        self._ffe._evaluate(gradient)
+        return self.getTotalElectrostaticEnergy()
+
+    def _e(self, k):
+        return self._ffe._last[k]
+
+    def getPermanentEnergy(self):
+        return self._e("perm")
+
+    def getPermRealEnergy(self):
+        return self._e("perm_real")
+
+    def getPermSelfEnergy(self):
+        return self._e("perm_self")
+
+    def getPermRecipEnergy(self):
+        return self._e("perm_recip")
+
+    def getPolarizationEnergy(self):
+        return self._e("pol")
+
+    def getIndRealEnergy(self):
+        return self._e("pol_real")
+
+    def getIndSelfEnergy(self):
+        return self._e("pol_self")
+
+    def getIndRecipEnergy(self):
+        return self._e("pol_recip")
+
+    def getTotalElectrostaticEnergy(self):
+        return self._e("perm") + self._e("pol")
+
+    def getInteractions(self):
+        return int(self._e("n_perm"))
+
+    def getScfIterations(self):
+        return int(self._e("pcg_iters"))
+
+    def getInducedDipoles(self):
+        """(u, uCR) in caller order, (n, 3) each."""
+        return self._ffe._handle.induced()
+
+
+class ForceFieldEnergy:
+    """Nonbonded part of ForceFieldEnergy on the MI355X library."""
+
+    def __init__(self, topology, box, params: NonbondedParams | None = None, precision=None, device=0,
+                 direct_fallback=True):
+        self.params = params or NonbondedParams()
+        self.topology = topology
+        self.box = np.asarray(box if box is not None else [0, 0, 0, 90, 90, 90], float)
+        grid = self.params.resolved_grid(self.box) if self.box[0] > 0 else (16, 16, 16)
+        self.grid = grid
+        self._nbp = ffi.params_from(self.params, grid, precision, device)
+        self._handle = ffi.Handle(self._nbp, topology, device)
+        self._handle.set_cell(self.box)
+        self._x = None
+        self._last = None
+        self.direct_fallback = direct_fallback
+        self.vanDerWaals = VanDerWaals(self)
+        self.particleMeshEwald = ParticleMeshEwald(self)
+
+    # -- Potential interface ------------------------------------------------
+    def getNumberOfVariables(self):
+        return 3 * self.topology.n
+
+    def getCoordinates(self, x=None):
+        if x is None:
+            return None if self._x is None else self._x.copy()
+        x[:] = self._x
+        return x
+
+    def setCoordinates(self, x):
+        self._x = np.array(x, dtype=np.float64).reshape(-1)
+        self._handle.set_positions(self._x)
+
+    def energy(self, x=None, verbose=False):
+        if x is not None:
+            self.setCoordinates(x)
+        self._evaluate(False)
+        return self._last["total"]
+
+    def energyAndGradient(self, x, g=None, verbose=False):
+        self.setCoordinates(x)
+        self._evaluate(True)
+        grad = (self._fallback_grad if self._fallback_grad is not None else self._handle.gradient()).reshape(-1)
+        if g is not None:
+            g[:] = grad
+        return self._last["total"], grad
+
+    def getVanDerWaalsEnergy(self):
+        return self._last["vdw"]
+
+    def getPermanentMultipoleEnergy(self):
+        return self._last["perm"]
+
+    def getPolarizationEnergy(self):
+        return self._last["pol"]
+
+    def getTotalEnergy(self):
+        return self._last["total"]
+
+    def destroy(self):
+        self._handle.close()
+
+    # -- internals -----------------------------------------------------------
+    def _evaluate(self, gradient):
+        self._fallback_grad = None
+        e, rc = self._handle.evaluate(ffi.NB_GRAD if gradient else 0, check=False)
+        if rc in (ffi.NB_ERR_SCF_DIVERGED, ffi.NB_ERR_SCF_MAXITER) and self.direct_fallback:
+            # direct polarization for this evaluation only; the mutual handle is kept
+            p = ffi.params_from(self.params, self.grid, self._nbp.precision, self._nbp.device)
+            p.polarization = 1
+            tmp = ffi.Handle(p, self.topology, self._nbp.device)
+            try:
+                tmp.set_cell(self.box)
+                tmp.set_positions(self._x)
+                e, rc = tmp.evaluate(ffi.NB_GRAD if gradient else 0, check=False)
+                if rc == ffi.NB_OK:
+                    self._fallback_grad = tmp.gradient()
+            finally:
+                tmp.close()
+        if rc != ffi.NB_OK:
+            raise EnergyException(self._handle.L.nb_last_error(self._handle.h).decode())
+        self._last = e
+        return e

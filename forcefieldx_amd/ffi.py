@@ -1,0 +1,214 @@
+"""ctypes binding of libamoeba_nb.so (include/amoeba_nb.h).
+
+This is the Python counterpart of the FFM downcall stubs a Java host would
+declare.  Structures mirror the C header field by field.
+The library is the HIP build in forcefieldx_amd/_build; there is no CPU
+fallback: loading fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libamoeba_nb.so")
+
+NB_OK, NB_ERR_NAN_ENERGY, NB_ERR_SCF_DIVERGED, NB_ERR_SCF_MAXITER, NB_ERR_BAD_TORQUE, NB_ERR_HIP, \
+    NB_ERR_INVALID, NB_ERR_NO_DEVICE = range(8)
+NB_GRAD, NB_VIRIAL, NB_DUDL, NB_FINITE_PCG = 1, 2, 4, 8
+NB_PREC_FP64, NB_PREC_MIXED = 0, 1
+
+EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
+            "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
+            "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class NbParams(C.Structure):
+    _fields_ = [("ewald_cutoff", C.c_double), ("ewald_alpha", C.c_double), ("vdw_cutoff", C.c_double),
+                ("vdw_taper", C.c_double), ("list_buffer", C.c_double), ("pme_order", C.c_int32),
+                ("grid", C.c_int32 * 3), ("polar_eps", C.c_double), ("electric", C.c_double),
+                ("precond_cutoff", C.c_double), ("precond_scale", C.c_double), ("m_scale", C.c_double * 4),
+                ("p_scale", C.c_double * 4), ("intra14", C.c_double), ("d11", C.c_double),
+                ("v_scale", C.c_double * 3), ("halgren_delta", C.c_double), ("halgren_gamma", C.c_double),
+                ("polarization", C.c_int32), ("do_vdw", C.c_int32), ("do_pme", C.c_int32),
+                ("precision", C.c_int32), ("max_scf_iter", C.c_int32), ("device", C.c_int32)]
+
+
+class NbTopology(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("frame", _ip), ("axis", _ip), ("multipole", _dp),
+                ("polarizability", _dp), ("thole", _dp), ("pdamp", _dp), ("group", _ip), ("vdw_class", _ip),
+                ("n_vdw_class", C.c_int32), ("vdw_table", _dp), ("red_index", _ip), ("red_value", _dp),
+                ("mask_offset", _ip), ("mask_atom", _ip), ("mask_rel", _ip)]
+
+
+class NbEnergies(C.Structure):
+    _fields_ = [("perm_real", C.c_double), ("perm_self", C.c_double), ("perm_recip", C.c_double),
+                ("perm_charge_corr", C.c_double), ("pol_real", C.c_double), ("pol_self", C.c_double),
+                ("pol_recip", C.c_double), ("vdw", C.c_double), ("perm", C.c_double), ("pol", C.c_double),
+                ("total", C.c_double), ("dudl", C.c_double), ("d2udl2", C.c_double), ("pcg_iters", C.c_int32),
+                ("pcg_eps", C.c_double), ("n_vdw", C.c_int64), ("n_perm", C.c_int64),
+                ("list_rebuilt", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with forcefieldx_amd.build.build() "
+                           "(the nonbonded path has no CPU fallback)")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.nb_create.argtypes = [C.POINTER(NbParams), C.POINTER(NbTopology), C.POINTER(vp)]
+    L.nb_set_cell.argtypes = [vp, _dp]
+    L.nb_set_positions.argtypes = [vp, _dp]
+    L.nb_set_positions_device.argtypes = [vp, vp]
+    L.nb_set_lambda.argtypes = [vp, C.c_double]
+    L.nb_evaluate.argtypes = [vp, C.c_uint32, C.POINTER(NbEnergies)]
+    L.nb_get_gradient.argtypes = [vp, _dp]
+    L.nb_gradient_device.argtypes = [vp]
+    L.nb_gradient_device.restype = vp
+    L.nb_get_induced.argtypes = [vp, _dp, _dp]
+    L.nb_get_virial.argtypes = [vp, _dp]
+    L.nb_get_dudx_dl.argtypes = [vp, _dp]
+    L.nb_last_error.argtypes = [vp]
+    L.nb_last_error.restype = C.c_char_p
+    L.nb_destroy.argtypes = [vp]
+    L.nb_destroy.restype = None
+    for f in ("nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
+              "nb_evaluate", "nb_get_gradient", "nb_get_induced", "nb_get_virial", "nb_get_dudx_dl",
+              "nb_abi_version", "nb_device_count"):
+        getattr(L, f).restype = C.c_int
+    L.nb_test_convolution.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
+    L.nb_test_convolution.restype = C.c_int
+    _lib = L
+    return L
+
+
+class NbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"amoeba_nb error {code}: {msg}")
+        self.code = code
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Handle:
+    """Owns one nb handle and keeps the topology arrays alive for its lifetime."""
+
+    def __init__(self, params: NbParams, topology, device=0):
+        L = load()
+        t = topology
+        self._arrays = dict(
+            frame=np.ascontiguousarray(t.frame, np.int32), axis=np.ascontiguousarray(t.axis, np.int32).reshape(-1),
+            mpole=np.ascontiguousarray(t.mpole, np.float64).reshape(-1),
+            polarity=np.ascontiguousarray(t.polarity, np.float64), thole=np.ascontiguousarray(t.thole, np.float64),
+            pdamp=np.ascontiguousarray(t.pdamp, np.float64), group=np.ascontiguousarray(t.group, np.int32),
+            vclass=np.ascontiguousarray(t.vdw_class, np.int32),
+            vtab=np.ascontiguousarray(t.vdw_table, np.float64).reshape(-1),
+            redi=np.ascontiguousarray(t.red_index, np.int32), redv=np.ascontiguousarray(t.red_value, np.float64),
+            moff=np.ascontiguousarray(t.mask_off, np.int32), matom=np.ascontiguousarray(t.mask_atom, np.int32),
+            mrel=np.ascontiguousarray(t.mask_rel, np.int32))
+        a = self._arrays
+        top = NbTopology(t.n, _p(a["frame"], C.c_int32), _p(a["axis"], C.c_int32), _p(a["mpole"], C.c_double),
+                         _p(a["polarity"], C.c_double), _p(a["thole"], C.c_double), _p(a["pdamp"], C.c_double),
+                         _p(a["group"], C.c_int32), _p(a["vclass"], C.c_int32), int(t.vdw_table.shape[0]),
+                         _p(a["vtab"], C.c_double), _p(a["redi"], C.c_int32), _p(a["redv"], C.c_double),
+                         _p(a["moff"], C.c_int32), _p(a["matom"], C.c_int32), _p(a["mrel"], C.c_int32))
+        params.device = device
+        self.params = params
+        self.n = t.n
+        h = C.c_void_p()
+        rc = L.nb_create(C.byref(params), C.byref(top), C.byref(h))
+        if rc != NB_OK:
+            raise NbError(rc, "nb_create failed")
+        self.h = h
+        self.L = L
+
+    def _check(self, rc):
+        if rc != NB_OK:
+            raise NbError(rc, self.L.nb_last_error(self.h).decode())
+
+    def set_cell(self, box):
+        b = np.ascontiguousarray(box if box is not None else [0, 0, 0, 90, 90, 90], np.float64)
+        self._check(self.L.nb_set_cell(self.h, _p(b, C.c_double)))
+
+    def set_positions(self, xyz):
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        self._check(self.L.nb_set_positions(self.h, _p(x, C.c_double)))
+
+    def set_positions_device(self, ptr):
+        self._check(self.L.nb_set_positions_device(self.h, C.c_void_p(ptr)))
+
+    def evaluate(self, flags=NB_GRAD, check=True):
+        e = NbEnergies()
+        rc = self.L.nb_evaluate(self.h, flags, C.byref(e))
+        if check:
+            self._check(rc)
+        return e.as_dict(), rc
+
+    def gradient(self):
+        g = np.zeros(3 * self.n)
+        self._check(self.L.nb_get_gradient(self.h, _p(g, C.c_double)))
+        return g.reshape(self.n, 3)
+
+    def gradient_device_ptr(self):
+        return self.L.nb_gradient_device(self.h)
+
+    def induced(self):
+        u = np.zeros(3 * self.n)
+        up = np.zeros(3 * self.n)
+        self._check(self.L.nb_get_induced(self.h, _p(u, C.c_double), _p(up, C.c_double)))
+        return u.reshape(self.n, 3), up.reshape(self.n, 3)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def params_from(p, grid, precision=None, device=0):
+    """NbParams from forcefieldx_amd.forcefield.NonbondedParams."""
+    prec = p.precision if precision is None else precision
+    return NbParams(p.ewald_cutoff, p.ewald_alpha, p.vdw_cutoff, p.vdw_taper, p.list_buffer, p.pme_order,
+                    (C.c_int32 * 3)(*grid), p.polar_eps, p.electric, p.precond_cutoff, p.precond_scale,
+                    (C.c_double * 4)(*p.m_scale), (C.c_double * 4)(*p.p_scale), p.intra14, p.d11,
+                    (C.c_double * 3)(*p.v_scale), p.halgren_delta, p.halgren_gamma, p.polarization, int(p.do_vdw),
+                    int(p.do_pme), prec, p.max_scf_iter, device)
+
+
+def device_count():
+    try:
+        return load().nb_device_count()
+    except Exception:
+        return 0
+
+
+def test_convolution(data, recip, nx, ny, nz):
+    """GPU convolution of the library's FFT kernels (test entry)."""
+    d = np.ascontiguousarray(data, np.float64).copy()
+    r = np.ascontiguousarray(recip, np.float64)
+    rc = load().nb_test_convolution(_p(d, C.c_double), _p(r, C.c_double), nx, ny, nz)
+    if rc != NB_OK:
+        raise NbError(rc, "nb_test_convolution failed")
+    return d

@@ -35,8 +35,45 @@ def _random_rotations(rng, n):
     return R
 
 
+def _sysgen():
+    import ctypes as C
+    import os
+    from .build import BUILD
+    path = os.path.join(BUILD, "libsysgen.so")
+    if not os.path.exists(path):
+        return None
+    L = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    L.rsa_place.argtypes = [dp, C.c_int, dp, C.c_int, C.c_int, dp, C.c_double]
+    L.rsa_place.restype = C.c_int
+    return L
+
+
 def sequential_insertion(rng, n, box, min_dist, max_tries=2000):
-    """Random sequential insertion in a cubic/orthorhombic box with a hash grid."""
+    """Random sequential insertion in an orthorhombic box.
+
+    Candidates are drawn in batches of 4096 from ``rng``; each is accepted if
+    no placed point lies closer than ``min_dist`` (minimum image)."""
+    L = _sysgen()
+    if L is not None:
+        import ctypes as C
+        b = np.ascontiguousarray(np.asarray(box, float)[:3])
+        pts = np.zeros((n, 3))
+        count = 0
+        dp = C.POINTER(C.c_double)
+        while count < n:
+            cand = np.ascontiguousarray(rng.random((4096, 3)) * b)
+            count = L.rsa_place(cand.ctypes.data_as(dp), 4096, pts.ctypes.data_as(dp), count, n,
+                                b.ctypes.data_as(dp), float(min_dist))
+            max_tries -= 1
+            if max_tries <= 0:
+                raise RuntimeError("sequential insertion did not converge")
+        return pts
+    return _sequential_insertion_py(rng, n, box, min_dist, max_tries)
+
+
+def _sequential_insertion_py(rng, n, box, min_dist, max_tries=2000):
+    """Pure-Python form of sequential_insertion (same acceptance sequence)."""
     box = np.asarray(box, float)[:3]
     ncell = np.maximum((box / min_dist).astype(int), 1)
     csize = box / ncell

@@ -1,0 +1,169 @@
+/*
+ * amoeba_nb.h -- C ABI of the MI355X AMOEBA nonbonded energy-and-gradient
+ * library (libamoeba_nb.so, HIP kernels for gfx950).
+ *
+ * This is the drop-in boundary below Force Field X's ForceFieldEnergy /
+ * ParticleMeshEwald / VanDerWaals objects.  Plain pointers and sizes only;
+ * all host arrays are caller-owned and copied.  One device context per
+ * handle; calls on one handle are not thread-safe, separate handles may be
+ * used concurrently.
+ *
+ * Units follow the reference: Angstrom, e, kcal/mol, Debye for polar-eps.
+ * Coordinates and gradients are interleaved x,y,z per atom, exactly the
+ * double[] x / double[] g of Potential.energyAndGradient
+ * (modules/numerics/src/main/java/ffx/numerics/OptimizationInterface.java:91).
+ */
+#ifndef AMOEBA_NB_H
+#define AMOEBA_NB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_ABI_VERSION 1
+
+/* Return codes.  Map onto the reference's failure behaviour:
+ * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
+ * SCF divergence / iteration cap (PCGSolver.java:553-569), undefined torque
+ * (ReduceRegion.java:208-229). */
+enum {
+  NB_OK = 0,
+  NB_ERR_NAN_ENERGY = 1,
+  NB_ERR_SCF_DIVERGED = 2,
+  NB_ERR_SCF_MAXITER = 3,
+  NB_ERR_BAD_TORQUE = 4,
+  NB_ERR_HIP = 5,
+  NB_ERR_INVALID = 6,
+  NB_ERR_NO_DEVICE = 7
+};
+
+/* Local frame definitions (MultipoleType.MultipoleFrameDefinition order). */
+enum { NB_FRAME_NONE = 0, NB_FRAME_ZONLY = 1, NB_FRAME_ZTHENX = 2, NB_FRAME_BISECTOR = 3,
+       NB_FRAME_ZTHENBISECTOR = 4, NB_FRAME_THREEFOLD = 5 };
+
+/* Polarization (Polarization.java) */
+enum { NB_POL_NONE = 0, NB_POL_DIRECT = 1, NB_POL_MUTUAL = 2 };
+
+/* Arithmetic form. */
+enum { NB_PREC_FP64 = 0, NB_PREC_MIXED = 1 };
+
+/* nb_evaluate flags */
+enum { NB_GRAD = 1u, NB_VIRIAL = 2u, NB_DUDL = 4u, NB_FINITE_PCG = 8u };
+
+typedef struct {
+  double ewald_cutoff;      /* real-space Ewald cutoff, A (EwaldParameters.java:55) */
+  double ewald_alpha;       /* Ewald coefficient, 1/A (EwaldParameters.java:71) */
+  double vdw_cutoff;        /* A (VanDerWaalsForm.java:85) */
+  double vdw_taper;         /* taper start as a fraction of the cutoff (0.9) */
+  double list_buffer;       /* neighbor-list buffer, A (ForceFieldEnergy.java:244) */
+  int32_t pme_order;        /* B-spline order (5) */
+  int32_t grid[3];          /* PME grid */
+  double polar_eps;         /* PCG convergence, RMS Debye (ParticleMeshEwald.java:171) */
+  double electric;          /* 332.0637133 (Constants.java:192) */
+  double precond_cutoff;    /* 4.5 A (PCGSolver.java:300) */
+  double precond_scale;     /* 2.0 (PCGSolver.java:316) */
+  double m_scale[4];        /* mpole-12..15-scale */
+  double p_scale[4];        /* polar-12..15-scale */
+  double intra14;           /* polar-14-intra */
+  double d11;               /* direct-11-scale */
+  double v_scale[3];        /* vdw-12..14-scale */
+  double halgren_delta;     /* 0.07 */
+  double halgren_gamma;     /* 0.12 */
+  int32_t polarization;     /* NB_POL_* */
+  int32_t do_vdw;
+  int32_t do_pme;
+  int32_t precision;        /* NB_PREC_* */
+  int32_t max_scf_iter;     /* 1000 */
+  int32_t device;           /* HIP device ordinal */
+} nb_params;
+
+typedef struct {
+  int32_t n_atoms;
+  const int32_t *frame;        /* [n] NB_FRAME_* */
+  const int32_t *axis;         /* [3n] z, x, y frame atoms, -1 when unused */
+  const double *multipole;     /* [10n] local frame q, d(3), Q(xx,yy,zz,xy,xz,yz) in e, e*A, e*A^2 */
+  const double *polarizability;/* [n] A^3 */
+  const double *thole;         /* [n] */
+  const double *pdamp;         /* [n] polarizability^(1/6); <= 0: undamped */
+  const int32_t *group;        /* [n] polarization group id (ip11 equivalence class) */
+  const int32_t *vdw_class;    /* [n] row of vdw_table */
+  int32_t n_vdw_class;
+  const double *vdw_table;     /* [n_vdw_class^2 * 2] combined (1/Rmin, eps) per class pair */
+  const int32_t *red_index;    /* [n] heavy atom a reduced hydrogen rides on (self otherwise) */
+  const double *red_value;     /* [n] reduction factor (0 = none) */
+  const int32_t *mask_offset;  /* [n+1] CSR offsets of bonded relations */
+  const int32_t *mask_atom;    /* [nnz] partner */
+  const int32_t *mask_rel;     /* [nnz] 2 = 1-2, 3 = 1-3, 4 = 1-4, 5 = 1-5 */
+} nb_topology;
+
+typedef struct {
+  double perm_real, perm_self, perm_recip, perm_charge_corr;
+  double pol_real, pol_self, pol_recip;
+  double vdw;
+  double perm, pol, total;
+  double dudl, d2udl2;
+  int32_t pcg_iters;
+  double pcg_eps;
+  int64_t n_vdw, n_perm;
+  int32_t list_rebuilt;
+} nb_energies;
+
+/* Build a handle: uploads topology, allocates device buffers.
+ * Replaces the construction of ParticleMeshEwald/VanDerWaals
+ * (ParticleMeshEwald.java:400-700, VanDerWaals.java:308-404). */
+int nb_create(const nb_params *params, const nb_topology *top, void **handle);
+
+/* Unit cell a, b, c (A), alpha, beta, gamma (deg); a <= 0: aperiodic.
+ * Replaces ParticleMeshEwald.setCrystal (ParticleMeshEwald.java:1244-1283). */
+int nb_set_cell(void *handle, const double box[6]);
+
+/* Host coordinates [3n]; replaces the coordinate load of
+ * InitializationRegion (InitializationRegion.java:316-321). */
+int nb_set_positions(void *handle, const double *xyz);
+
+/* Device-resident coordinates [3n] (copied device-to-device). */
+int nb_set_positions_device(void *handle, const void *d_xyz);
+
+/* Alchemical state; replaces LambdaInterface.setLambda (LambdaInterface.java:73). */
+int nb_set_lambda(void *handle, double lambda);
+
+/* One energy (+ gradient) evaluation: ForceFieldEnergy.energy(gradient, print)
+ * restricted to VanDerWaals.energy + ParticleMeshEwald.energy
+ * (ForceFieldEnergy.java:1748-1755).  out may be NULL. */
+int nb_evaluate(void *handle, uint32_t flags, nb_energies *out);
+
+/* Gradient of the last evaluation [3n] (ForceFieldEnergy.fillGradient,
+ * ForceFieldEnergy.java:2848-2884). */
+int nb_get_gradient(void *handle, double *g);
+
+/* Device pointer of the gradient [3n] doubles of the last evaluation. */
+const void *nb_gradient_device(void *handle);
+
+/* Induced dipoles u (direct-masked) and uCR (energy-masked) [3n] each;
+ * ParticleMeshEwald.inducedDipole / inducedDipoleCR (ParticleMeshEwald.java:276). */
+int nb_get_induced(void *handle, double *u, double *ucr);
+
+/* Virial (3x3, row-major, kcal/mol); not provided by the reference. */
+int nb_get_virial(void *handle, double v[9]);
+
+/* dU/dX/dlambda [3n] (ParticleMeshEwald.getdEdXdL, :1199-1242). */
+int nb_get_dudx_dl(void *handle, double *g);
+
+/* Human-readable message for the last error on this handle. */
+const char *nb_last_error(void *handle);
+
+/* Release all device memory of the handle. */
+void nb_destroy(void *handle);
+
+/* ABI version (NB_ABI_VERSION). */
+int nb_abi_version(void);
+
+/* Number of HIP devices visible (0 without a GPU). */
+int nb_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+"""HIP library against the FP64 oracle (FP64 form: 1e-8 relative)."""
+import numpy as np
+import pytest
+
+from forcefieldx_amd import ffi
+from forcefieldx_amd.energy import ForceFieldEnergy
+from forcefieldx_amd.forcefield import NonbondedParams
+from forcefieldx_amd.systems import water_box
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-8  # FP64 form parity bar
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def run_both(oracle, xyz, box, top, params):
+    sys_ = oracle.System(xyz, box, top)
+    eo, go, uo, upo = oracle.evaluate(sys_, params)
+    ffe = ForceFieldEnergy(top, box, params, precision=ffi.NB_PREC_FP64)
+    e, g = ffe.energyAndGradient(xyz)
+    u, up = ffe.particleMeshEwald.getInducedDipoles()
+    last = dict(ffe._last)
+    ffe.destroy()
+    return eo, go, uo, upo, last, g.reshape(-1, 3), u, up
+
+
+@pytest.mark.parametrize("pol", [0, 1, 2])
+def test_water648_fp64_parity(oracle, water648, pol):
+    xyz, box, top = water648
+    p = NonbondedParams(vdw_cutoff=9.0, grid=(24, 24, 24), polar_eps=1e-6, polarization=pol)
+    print(f"water-648 FP64 polarization={pol}", flush=True)
+    eo, go, uo, upo, e, g, u, up = run_both(oracle, xyz, box, top, p)
+    for k in ("perm_real", "perm_self", "perm_recip", "pol_real", "pol_self", "pol_recip", "vdw", "total"):
+        ref = eo[k]
+        assert abs(e[k] - ref) <= REL * max(abs(ref), 1.0), (k, e[k], ref)
+    assert e["n_vdw"] == eo["n_vdw"]
+    assert e["n_perm"] == eo["n_perm"]
+    if pol == 2:
+        assert e["pcg_iters"] == eo["pcg_iters"]
+    assert rel_err(g, go) <= REL
+    if pol:
+        assert rel_err(u, uo) <= REL
+        assert rel_err(up, upo) <= REL
+
+
+def test_gpu_convolution_identity():
+    for nx, ny, nz in ((32, 32, 32), (24, 24, 24), (20, 18, 10), (144, 16, 24)):
+        rng = np.random.default_rng(7)
+        data = np.zeros(2 * nx * ny * nz)
+        data[0::2] = rng.random(nx * ny * nz)
+        data[1::2] = rng.random(nx * ny * nz)
+        out = ffi.test_convolution(data, np.ones(nx * ny * nz), nx, ny, nz)
+        assert np.allclose(out / (nx * ny * nz), data, atol=1e-13, rtol=0), (nx, ny, nz)
+
+
+def test_gpu_convolution_matches_oracle(oracle):
+    nx, ny, nz = 24, 20, 18
+    rng = np.random.default_rng(11)
+    data = np.zeros(2 * nx * ny * nz)
+    data[0::2] = rng.random(nx * ny * nz)
+    recip = oracle.influence([18.0, 16.0, 15.0, 90, 90, 90], (nx, ny, nz), 5, 0.545)
+    ref = oracle.convolution(data, recip, nx, ny, nz)
+    out = ffi.test_convolution(data, recip, nx, ny, nz)
+    assert rel_err(out, ref) < 1e-12
+
+
+def test_repeat_evaluation_without_rebuild(oracle, water648):
+    xyz, box, top = water648
+    p = NonbondedParams(vdw_cutoff=9.0, grid=(24, 24, 24), polar_eps=1e-6)
+    ffe = ForceFieldEnergy(top, box, p, precision=ffi.NB_PREC_FP64)
+    e1, g1 = ffe.energyAndGradient(xyz)
+    assert ffe._last["list_rebuilt"] == 1
+    x2 = xyz + 0.01 * np.random.default_rng(2).standard_normal(xyz.shape)
+    e2, g2 = ffe.energyAndGradient(x2)
+    assert ffe._last["list_rebuilt"] == 0
+    ffe.destroy()
+    eo = oracle.evaluate(oracle.System(x2, box, top), p)
+    assert abs(e2 - eo[0]["total"]) <= REL * abs(eo[0]["total"])
+    assert rel_err(g2.reshape(-1, 3), eo[1]) <= REL
