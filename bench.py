@@ -1,0 +1,202 @@
+"""Benchmark: AMOEBA nonbonded energy+gradient evaluations per second on one MI355X.
+
+Workload: water-20736 -- 6912 rigid AMOEBA-2003 waters (20736 atoms), seeded
+random insertion (O-O >= 2.6 A), 59.1 A cubic box, PME grid 64^3 order 5,
+Ewald cutoff 7 A (alpha 0.545), vdW cutoff 9 A, PCG to 1e-5 D.
+
+A step is one full nonbonded energy+gradient evaluation (vdW, permanent PME,
+PCG induced dipoles, reciprocal and real-space energies, torques ->
+gradients) with coordinates already resident in HBM.  The mixed form is the
+headline value; the FP64 form is timed beside it and the force relative RMS
+error of mixed against FP64 is reported.  The CPU baseline is the FP64 C
+oracle (same algorithm) on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "AMOEBA E+grad evals/s: mixed vs FP64 GPU at force rel RMS err <=1e-5"
+N_WATERS, BOX, GRID, SEED = 6912, 59.1, (64, 64, 64), 2
+
+# algorithmic work per pair / point (SURVEY-level counts of the reference formulas)
+FLOPS_PER_PAIR = {"k_real_energy": 1400.0, "k_perm_field": 220.0, "k_dip_field": 80.0, "k_vdw": 60.0}
+PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
+PEAK_HBM_GBS = 8000.0
+
+
+def kernel_family(name):
+    for k in ("k_real_energy", "k_perm_field", "k_dip_field", "k_vdw", "k_fft_axis", "k_spread_perm",
+              "k_spread_ind", "k_gather"):
+        if k in name:
+            return k
+    return name
+
+
+def roofline_for(name, avg_ms, launches_per_step, pairs, n, K, prec):
+    """Algorithmic work of one launch of kernel `name`, divided by its average duration."""
+    fam = kernel_family(name)
+    secs = avg_ms * 1e-3
+    real = "f32" if (prec == "mixed" and fam in FLOPS_PER_PAIR) else "f64"
+    if fam in FLOPS_PER_PAIR:
+        npairs = pairs["vdw"] if fam == "k_vdw" else pairs["ewald"]
+        if fam == "k_dip_field" and "true" in name:
+            npairs = pairs["precond"]
+        flops = FLOPS_PER_PAIR[fam] * npairs
+        return {"bound": "valu", "achieved": flops / secs / 1e12, "peak": PEAK_TFLOPS[real], "unit": "TFLOP/s",
+                "frac": flops / secs / 1e12 / PEAK_TFLOPS[real], "traffic": None,
+                "kernel": name, "avg_launch_ms": avg_ms, "work_per_launch": f"{flops:.4g} flop",
+                "work_model": f"{FLOPS_PER_PAIR[fam]:.0f} flop/pair x {npairs} pairs"}
+    nbytes = None
+    model = None
+    if fam == "k_fft_axis":
+        nbytes = 2 * 16.0 * K ** 3
+        model = "read+write of the complex FP64 grid: 2 x 16 B x K^3"
+    elif fam in ("k_spread_perm", "k_spread_ind"):
+        nbytes = n * (60 * 8 + 12 + 80) + 2 * 16.0 * K ** 3
+        model = "theta/igrid/multipole stream + grid read-modify-write"
+    elif fam == "k_gather":
+        nbytes = n * (60 * 8 + 12 + 30 * 8 * 2) + 16.0 * K ** 3
+        model = "theta/igrid stream + grid read + phi write"
+    if nbytes is None:
+        return {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": name, "avg_launch_ms": avg_ms}
+    gbs = nbytes / secs / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+            "traffic": None, "kernel": name, "avg_launch_ms": avg_ms, "work_per_launch": f"{nbytes:.4g} B",
+            "work_model": model}
+
+
+def time_gpu(top, box, xyz, params, prec, steps, warmup, profile):
+    from forcefieldx_amd import ffi
+    from forcefieldx_amd.energy import ForceFieldEnergy
+    ffe = ForceFieldEnergy(top, box, params, precision=prec)
+    ffe.setCoordinates(xyz)
+    h = ffe._handle
+    for _ in range(warmup):
+        ffe._evaluate(True)
+    if profile:
+        h.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ffe._evaluate(True)
+    t1 = time.perf_counter()
+    prof = h.profile() if profile else {}
+    if profile:
+        h.set_profiling(False)
+    last = dict(ffe._last)
+    g = h.gradient()
+    u, up = h.induced()
+    ffe.destroy()
+    return (t1 - t0) / steps, last, g, u, prof
+
+
+def cpu_baseline(top, box, xyz, params, budget_s=12.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.build()
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    s = pyoracle.System(xyz, box, top)
+    t0 = time.perf_counter()
+    count = 0
+    while True:
+        e, g, u, up = pyoracle.evaluate(s, params, nthreads=nth)
+        count += 1
+        if time.perf_counter() - t0 > budget_s or count >= 50:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": count / dt, "unit": "evals/s", "cores": nth, "kind": "port",
+            "sample": f"{count} full FP64 E+G evaluations of the same water-20736 workload "
+                      f"(oracle/amoeba_oracle.c, OpenMP, {dt:.1f} s)"}, e, g
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see the sharding note in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from forcefieldx_amd import build as nbbuild
+    from forcefieldx_amd import ffi
+    from forcefieldx_amd.forcefield import NonbondedParams
+    from forcefieldx_amd.systems import water_box
+    if not os.path.exists(ffi.LIB_PATH):
+        nbbuild.build()
+    xyz, box, top = water_box(N_WATERS, BOX, seed=SEED)
+    params = NonbondedParams(vdw_cutoff=9.0, grid=GRID, polar_eps=1e-5)
+
+    t64, e64, g64, u64, prof64 = time_gpu(top, box, xyz, params, ffi.NB_PREC_FP64, args.steps, args.warmup, False)
+    tmx, emx, gmx, umx, profmx = time_gpu(top, box, xyz, params, ffi.NB_PREC_MIXED, args.steps, args.warmup,
+                                          not args.no_profile)
+    frc_rms = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
+    frc_max = float(np.max(np.abs(gmx - g64)) / np.max(np.abs(g64)))
+    ind_rms = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+
+    roof = None
+    if profmx:
+        name, (tot, cnt) = max(profmx.items(), key=lambda kv: kv[1][0])
+        pairs = {"ewald": int(emx["n_perm"]) - top.n, "vdw": int(emx["n_vdw"]), "precond": None}
+        roof = roofline_for(name, tot / cnt, cnt / args.steps, pairs, top.n, GRID[0], "mixed")
+        roof["share_of_step"] = tot / (tmx * 1e3 * args.steps)
+
+    cpu = None
+    e_cpu = None
+    if not args.no_cpu:
+        cpu, e_cpu, g_cpu = cpu_baseline(top, box, xyz, params)
+
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / tmx,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmx * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG",
+        "data": "synthetic",
+        "config": {"workload": "water-20736: 6912 AMOEBA-2003 waters, 59.1 A cubic box, PME 64^3 order 5, "
+                               "Ewald 7 A, vdW 9 A, PCG 1e-5 D", "atoms": top.n, "grid": list(GRID),
+                   "seed": SEED, "parallelism": "single GPU"},
+        "fp64_evals_per_s": 1.0 / t64,
+        "fp64_ms_per_step": t64 * 1e3,
+        "mixed_over_fp64": t64 / tmx,
+        "force_rel_rms_err": frc_rms,
+        "force_rel_max_err": frc_max,
+        "induced_rel_rms_err": ind_rms,
+        "energy_mixed_minus_fp64": emx["total"] - e64["total"],
+        "energy_fp64": e64["total"],
+        "pcg_iters": {"mixed": emx["pcg_iters"], "fp64": e64["pcg_iters"]},
+        "pairs": {"ewald": int(e64["n_perm"]) - top.n, "vdw": int(e64["n_vdw"])},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = out["value"] / cpu["value"]
+        out["energy_fp64_minus_oracle"] = e64["total"] - e_cpu["total"]
+    if profmx:
+        out["kernel_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in
+                                     sorted(profmx.items(), key=lambda kv: -kv[1][0])}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

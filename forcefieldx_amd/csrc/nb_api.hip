@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <algorithm>
 #include <vector>
 
 #include "nb_state.h"
@@ -72,7 +73,7 @@ void free_all(NbHandle *h) {
   void *ptrs[] = {h->d_frame, h->d_axis, h->d_group0, h->d_vclass0, h->d_redidx, h->d_mpole_local, h->d_polar0,
                   h->d_thole0, h->d_ipd0, h->d_redval, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_vdw_table,
                   h->d_xyz, h->d_xyz_list, h->d_perm, h->d_iperm, h->d_cellid, h->d_cellcount, h->d_cellstart,
-                  h->d_nbr, h->d_ew, h->d_pc, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_red4, h->d_mp,
+                  h->d_nbr, h->d_ew, h->d_pc, h->d_sp, h->d_nsp, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_red4, h->d_mp,
                   h->d_gmp, h->d_polar, h->d_group, h->d_vclass, h->d_grid, h->d_recip, h->d_theta, h->d_igrid,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
@@ -80,6 +81,10 @@ void free_all(NbHandle *h) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
+  for (auto &r : h->prof_pool) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
   if (h->stream) hipStreamDestroy(h->stream);
 }
 
@@ -90,7 +95,66 @@ int fail(NbHandle *h, const std::exception &e, int code) {
 
 }  // namespace
 
+int nb_prof_begin(NbHandle *h, const char *name) {
+  if (h->prof_n == (int)h->prof_pool.size()) {
+    NbHandle::ProfRec r;
+    HIPCHECK(hipEventCreate(&r.a));
+    HIPCHECK(hipEventCreate(&r.b));
+    h->prof_pool.push_back(r);
+  }
+  int slot = h->prof_n++;
+  h->prof_pool[slot].name = name;
+  HIPCHECK(hipEventRecord(h->prof_pool[slot].a, h->stream));
+  return slot;
+}
+
+void nb_prof_end(NbHandle *h, int slot) { HIPCHECK(hipEventRecord(h->prof_pool[slot].b, h->stream)); }
+
+// accumulate elapsed times by kernel name; requires a completed stream
+void nb_prof_harvest(NbHandle *h) {
+  for (int i = 0; i < h->prof_n; i++) {
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, h->prof_pool[i].a, h->prof_pool[i].b));
+    const std::string &nm = h->prof_pool[i].name;
+    size_t k = 0;
+    while (k < h->prof_names.size() && h->prof_names[k] != nm) k++;
+    if (k == h->prof_names.size()) {
+      h->prof_names.push_back(nm);
+      h->prof_ms.push_back(0.0);
+      h->prof_count.push_back(0);
+    }
+    h->prof_ms[k] += ms;
+    h->prof_count[k] += 1;
+  }
+  h->prof_n = 0;
+}
+
 extern "C" {
+
+int nb_set_profiling(void *handle, int on) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  h->profiling = on != 0;
+  h->prof_names.clear();
+  h->prof_ms.clear();
+  h->prof_count.clear();
+  h->prof_n = 0;
+  return NB_OK;
+}
+
+int nb_profile_count(void *handle) {
+  NbHandle *h = (NbHandle *)handle;
+  return h ? (int)h->prof_names.size() : 0;
+}
+
+int nb_profile_entry(void *handle, int idx, char *name, int cap, double *total_ms, int64_t *launches) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h || idx < 0 || idx >= (int)h->prof_names.size() || !name || cap <= 0) return NB_ERR_INVALID;
+  snprintf(name, cap, "%s", h->prof_names[idx].c_str());
+  if (total_ms) *total_ms = h->prof_ms[idx];
+  if (launches) *launches = h->prof_count[idx];
+  return NB_OK;
+}
 
 int nb_abi_version(void) { return NB_ABI_VERSION; }
 
@@ -166,6 +230,13 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_nnb, n);
     alloc(&h->d_new, n);
     alloc(&h->d_npc, n);
+    alloc(&h->d_nsp, n);
+    {
+      int mx = 0;
+      for (int i = 0; i < n; i++) mx = std::max(mx, top->mask_offset[i + 1] - top->mask_offset[i]);
+      h->maxsp = mx + 16;
+      alloc(&h->d_sp, (size_t)h->maxsp * n);
+    }
     alloc(&h->d_flags, 8);
     HIPCHECK(hipMemset(h->d_flags, 0, 8 * sizeof(int32_t)));
     alloc(&h->d_pos4, n);
@@ -290,7 +361,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
         if (h->prm.polarization == NB_POL_MUTUAL) {
           status = nb_pcg(h, &e);
         } else {
-          nb_induced_field(h, h->d_u, h->d_up, h->d_field, h->d_fieldp);
+          nb_induced_field(h, h->d_u, h->d_up, h->d_field, h->d_fieldp, true);
         }
       }
       if (use_recip) nb_recip_energy(h, polar);
@@ -304,6 +375,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
     HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
+    if (h->profiling) nb_prof_harvest(h);
     memcpy(en, h->h_pinned, sizeof(en));
     memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
     memcpy(fl, h->h_pinned + 12, sizeof(fl));

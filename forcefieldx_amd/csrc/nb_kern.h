@@ -1,0 +1,102 @@
+// nb_kern.h -- helpers shared by the kernel translation units.
+#pragma once
+#include "nb_state.h"
+
+// mask-code -> scale tables (code = bonded relation 0/2..5 + 8 * same group)
+struct ScaleTab { double m[16], p[16], d[16], v[16]; int masked[16]; };
+
+// energy slots of NbHandle::d_energy
+enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT, E_NSLOT };
+// PCG scalar slots of NbHandle::d_scal
+enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_NSLOT };
+
+static inline ScaleTab scale_tab(const NbHandle *h) {
+  ScaleTab t;
+  for (int c = 0; c < 16; c++) {
+    int rel = c & 7, sg = (c >> 3) & 1;
+    t.m[c] = (rel >= 2 && rel <= 5) ? h->prm.m_scale[rel - 2] : 1.0;
+    double p = 1.0;
+    if (rel >= 2 && rel <= 4) {
+      p = h->prm.p_scale[rel - 2];
+      if (rel == 4 && sg) p *= h->prm.intra14;
+    }
+    t.p[c] = p;
+    t.d[c] = sg ? h->prm.d11 : 1.0;
+    t.v[c] = (rel >= 2 && rel <= 4) ? h->prm.v_scale[rel - 2] : 1.0;
+    t.masked[c] = (t.m[c] != 1.0 || t.p[c] != 1.0 || t.d[c] != 1.0) ? 1 : 0;
+  }
+  return t;
+}
+
+static inline dim3 grid_for(long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+static __device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block sum then one atomic per block; every thread of the block must call
+static __device__ __forceinline__ void block_add(double v, double *dst) {
+  __shared__ double part[16];
+  v = wave_sum_d(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) part[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++) s += part[i];
+    atomicAdd(dst, s);
+  }
+}
+
+static __device__ __forceinline__ void block_add_ll(long long v, int64_t *dst) {
+  __shared__ long long partl[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) partl[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long s = 0;
+    int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++) s += partl[i];
+    atomicAdd((unsigned long long *)dst, (unsigned long long)s);
+  }
+}
+
+// sum over the LPA consecutive lanes that share one atom
+template <int LPA> static __device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = LPA / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPA);
+  return v;
+}
+template <int LPA> static __device__ __forceinline__ int group_sum_i(int v) {
+#pragma unroll
+  for (int o = LPA / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LPA);
+  return v;
+}
+template <int LPA> static __device__ __forceinline__ void group_sum3(V3<double> &a) {
+  a.x = group_sum<LPA>(a.x);
+  a.y = group_sum<LPA>(a.y);
+  a.z = group_sum<LPA>(a.z);
+}
+
+static __device__ __forceinline__ V3<double> ld3d(const double *x, int i) {
+  return V3<double>{x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+}
+template <typename T> static __device__ __forceinline__ V3<T> ld3t(const double *x, int i) {
+  return V3<T>{(T)x[3 * i], (T)x[3 * i + 1], (T)x[3 * i + 2]};
+}
+static __device__ __forceinline__ void st3(double *x, int i, V3<double> v) {
+  x[3 * i] = v.x; x[3 * i + 1] = v.y; x[3 * i + 2] = v.z;
+}
+template <typename T> static __device__ __forceinline__ void acc3(V3<double> &a, const V3<T> &b) {
+  a.x += (double)b.x; a.y += (double)b.y; a.z += (double)b.z;
+}
+static __device__ __forceinline__ void add3(double *x, int i, V3<double> v) {
+  x[3 * i] += v.x; x[3 * i + 1] += v.y; x[3 * i + 2] += v.z;
+}

@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(256) k_build_list(int n, const double4 *__rest
           double r2 = image(cell, ddx, ddy, ddz);
           if (r2 > cut2) continue;
           int code = relation(moff, matom, mrel, i, perm[t]) + (group[t] == gi ? 8 : 0);
-          if (cnt < maxnb) nbr[(size_t)cnt * n + s] = (uint32_t)t | ((uint32_t)code << NB_CODE_SHIFT);
+          if (cnt < maxnb) nbr[(size_t)s * maxnb + cnt] = (uint32_t)t | ((uint32_t)code << NB_CODE_SHIFT);
           cnt++;
         }
       }
@@ -190,7 +190,7 @@ bool nb_need_rebuild(NbHandle *h) {
   int32_t zero = 0;
   HIPCHECK(hipMemcpyAsync(h->d_flags + 3, &zero, sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
   double lim = 0.5 * h->prm.list_buffer;
-  hipLaunchKernelGGL(k_max_disp, dim3((n + 255) / 256), dim3(256), 0, h->stream, n, h->d_xyz, h->d_xyz_list, lim * lim,
+  NB_LAUNCH(k_max_disp, dim3((n + 255) / 256), dim3(256), 0, n, h->d_xyz, h->d_xyz_list, lim * lim,
                      h->d_flags);
   int32_t flag = 0;
   HIPCHECK(hipMemcpyAsync(&flag, h->d_flags + 3, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
@@ -222,14 +222,14 @@ void nb_build_lists(NbHandle *h) {
   h->ncell_tot = m;
   dim3 B(256), G((n + 255) / 256);
   HIPCHECK(hipMemsetAsync(h->d_cellcount, 0, sizeof(int32_t) * (m + 1), h->stream));
-  hipLaunchKernelGGL(k_cell_assign, G, B, 0, h->stream, n, h->d_xyz, h->cell, nc[0], nc[1], nc[2], h->d_cellid,
+  NB_LAUNCH(k_cell_assign, G, B, 0, n, h->d_xyz, h->cell, nc[0], nc[1], nc[2], h->d_cellid,
                      h->d_cellcount);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->stream, m, h->d_cellcount, h->d_cellstart);
+  NB_LAUNCH(k_scan, dim3(1), dim3(1024), 0, m, h->d_cellcount, h->d_cellstart);
   HIPCHECK(hipMemsetAsync(h->d_cellcount, 0, sizeof(int32_t) * (m + 1), h->stream));
-  hipLaunchKernelGGL(k_cell_scatter, G, B, 0, h->stream, n, h->d_cellid, h->d_cellstart, h->d_cellcount, h->d_perm);
-  hipLaunchKernelGGL(k_cell_sortseg, dim3((m + 255) / 256), B, 0, h->stream, m, h->d_cellstart, h->d_perm);
-  hipLaunchKernelGGL(k_iperm, G, B, 0, h->stream, n, h->d_perm, h->d_iperm);
-  hipLaunchKernelGGL(k_gather_static, G, B, 0, h->stream, n, h->d_perm, h->d_polar0, h->d_thole0, h->d_ipd0,
+  NB_LAUNCH(k_cell_scatter, G, B, 0, n, h->d_cellid, h->d_cellstart, h->d_cellcount, h->d_perm);
+  NB_LAUNCH(k_cell_sortseg, dim3((m + 255) / 256), B, 0, m, h->d_cellstart, h->d_perm);
+  NB_LAUNCH(k_iperm, G, B, 0, n, h->d_perm, h->d_iperm);
+  NB_LAUNCH(k_gather_static, G, B, 0, n, h->d_perm, h->d_polar0, h->d_thole0, h->d_ipd0,
                      h->d_group0, h->d_vclass0, h->d_polar, h->d_group, h->d_vclass, h->d_mp);
   nb_gather_sorted(h);
   // neighbor rows, grown on overflow
@@ -238,7 +238,7 @@ void nb_build_lists(NbHandle *h) {
     if (!h->d_nbr) HIPCHECK(hipMalloc(&h->d_nbr, sizeof(uint32_t) * (size_t)h->maxnb * n));
     int32_t zeros[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(h->d_flags, zeros, sizeof(zeros), hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_build_list, G, B, 0, h->stream, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell, nc[0],
+    NB_LAUNCH(k_build_list, G, B, 0, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell, nc[0],
                        nc[1], nc[2], cut * cut, h->d_group, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_nbr,
                        h->maxnb, h->d_nnb, h->d_flags);
     int32_t f0 = 0;

@@ -1,0 +1,459 @@
+// nb_pairs.hip -- real-space pair kernels (gfx950).
+//
+// LPA consecutive lanes of a wavefront share one (spatially sorted) atom and
+// walk its neighbor row in stride LPA, so a wavefront reads LPA consecutive
+// list entries per atom and the chip sees n * LPA threads.  Per-lane partial
+// sums are FP64 (also in the mixed form: the pair arithmetic is FP32, the
+// accumulation is not) and are combined with cross-lane shuffles; each atom's
+// field / force / torque is written once, by lane 0, without atomics.
+//
+//  * k_filter: PermanentFieldRegion's side effect of building the Ewald
+//    (r <= off) and preconditioner (r < 4.5 A) neighbor lists
+//    (PermanentFieldRegion.java:679-714), as an ordered ballot compaction.
+//  * k_perm_field: permanent real-space field and fieldCR
+//    (PermanentFieldRegion.java:704-808).
+//  * k_dip_field: induced-dipole real-space field (InducedDipoleFieldRegion.
+//    java:365-450) and the preconditioner field (PCGSolver.java:1270-1390).
+//  * k_real_energy: permanent and polarization energy, force and torque
+//    (RealSpaceEnergyRegion.java:803-2177).
+//  * k_vdw: buffered 14-7 (VanDerWaals.java:1638-1900).
+#include <cmath>
+#include <stdexcept>
+
+#include "nb_kern.h"
+
+#define NB_LPA 16
+
+namespace {
+
+template <int LPA>
+__device__ __forceinline__ unsigned long long group_mask(unsigned long long ballot) {
+  int base = (threadIdx.x & 63) & ~(LPA - 1);
+  unsigned long long m = LPA == 64 ? ballot : (ballot >> base) & ((1ull << LPA) - 1ull);
+  return m;
+}
+
+template <int LPA>
+__global__ void __launch_bounds__(256) k_filter(int n, const double4 *__restrict__ pos, const uint32_t *__restrict__ nbr,
+                                                const int32_t *__restrict__ nnb, int maxnb, CellD cell, double off2,
+                                                double pcut, ScaleTab sc, uint32_t *__restrict__ ew, int maxew,
+                                                int32_t *__restrict__ new_, uint32_t *__restrict__ pc, int maxpc,
+                                                int32_t *__restrict__ npc, uint32_t *__restrict__ sp, int maxsp,
+                                                int32_t *__restrict__ nsp, int32_t *__restrict__ flags) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  bool valid = s < n;
+  int cnt = valid ? nnb[s] : 0;
+  double4 pi = valid ? pos[s] : make_double4(0, 0, 0, 0);
+  int ne = 0, np = 0, ns = 0;
+  const uint32_t *row = nbr + (size_t)(valid ? s : 0) * maxnb;
+  uint32_t *erow = ew + (size_t)(valid ? s : 0) * maxew;
+  uint32_t *prow = pc + (size_t)(valid ? s : 0) * maxpc;
+  uint32_t *srow = sp + (size_t)(valid ? s : 0) * maxsp;
+  for (int base = 0; base < cnt; base += LPA) {
+    int t = base + lane;
+    bool ke = false, kp = false, ks = false;
+    uint32_t e = 0;
+    if (t < cnt) {
+      e = row[t];
+      double4 pk = pos[e & NB_INDEX_MASK];
+      double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+      double r2 = image(cell, dx, dy, dz);
+      ke = r2 <= off2;
+      kp = ke && sqrt(r2) < pcut;
+      ks = ke && sc.masked[e >> NB_CODE_SHIFT];
+    }
+    unsigned long long me = group_mask<LPA>(__ballot(ke));
+    unsigned long long mp = group_mask<LPA>(__ballot(kp));
+    unsigned long long ms = group_mask<LPA>(__ballot(ks));
+    unsigned long long below = (1ull << lane) - 1ull;
+    if (ke) {
+      int q = ne + __popcll(me & below);
+      if (q < maxew) erow[q] = e;
+    }
+    if (kp) {
+      int q = np + __popcll(mp & below);
+      if (q < maxpc) prow[q] = e;
+    }
+    if (ks) {
+      int q = ns + __popcll(ms & below);
+      if (q < maxsp) srow[q] = e;
+    }
+    ne += __popcll(me);
+    np += __popcll(mp);
+    ns += __popcll(ms);
+  }
+  if (valid && lane == 0) {
+    new_[s] = ne;
+    npc[s] = np;
+    nsp[s] = ns;
+    if (ne > maxew) atomicMax(&flags[1], ne);
+    if (np > maxpc) atomicMax(&flags[2], np);
+    if (ns > maxsp) atomicMax(&flags[5], ns);
+  }
+}
+
+// one permanent-field pair in arithmetic U, accumulated in FP64
+template <typename U>
+__device__ __forceinline__ void perm_field_pair(const double4 &pi, const AtomMP &ai, const double4 &pk,
+                                                const AtomMP &ak, int code, const CellD &cell, double aewald,
+                                                const ScaleTab &sc, V3<double> &fd, V3<double> &fp) {
+  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+  double r2 = image(cell, dx, dy, dz);
+  MP<U> K = load_mp<U>(ak);
+  U r = (U)sqrt(r2), rinv = (U)1 / r;
+  V3<U> R{(U)-dx, (U)-dy, (U)-dz};  // from k to i
+  U bn[4];
+  ewald_bn<U, 3>(r, rinv, (U)aewald, bn);
+  Thole<U> th = thole<U, false>(r, R, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
+  U rr3 = rinv * rinv * rinv, rr5 = (U)3 * rr3 * rinv * rinv, rr7 = (U)5 * rr5 * rinv * rinv;
+  U ds = (U)sc.d[code], ps = (U)sc.p[code];
+  acc3(fd, mp_field(R, K, bn[1] - ((U)1 - th.s3 * ds) * rr3, bn[2] - ((U)1 - th.s5 * ds) * rr5,
+                    bn[3] - ((U)1 - th.s7 * ds) * rr7));
+  acc3(fp, mp_field(R, K, bn[1] - ((U)1 - th.s3 * ps) * rr3, bn[2] - ((U)1 - th.s5 * ps) * rr5,
+                    bn[3] - ((U)1 - th.s7 * ps) * rr7));
+}
+
+// T = float (mixed form): masked pairs, whose screened-minus-bare radial
+// differences cancel catastrophically at bonded distances, are skipped in the
+// main row and evaluated in FP64 from the short masked row.
+template <typename T, int LPA>
+__global__ void __launch_bounds__(256) k_perm_field(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
+                                                    const uint32_t *__restrict__ ew, const int32_t *__restrict__ new_,
+                                                    int maxew, const uint32_t *__restrict__ sp,
+                                                    const int32_t *__restrict__ nsp, int maxsp, CellD cell,
+                                                    double aewald, ScaleTab sc, double *__restrict__ field,
+                                                    double *__restrict__ fieldp) {
+  constexpr bool MIXED = sizeof(T) == 4;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  bool valid = s < n;
+  V3<double> fd{0, 0, 0}, fp{0, 0, 0};
+  if (valid) {
+    double4 pi = pos[s];
+    AtomMP ai = mp[s];
+    int m = new_[s];
+    const uint32_t *row = ew + (size_t)s * maxew;
+    for (int t = lane; t < m; t += LPA) {
+      uint32_t e = row[t];
+      int k = e & NB_INDEX_MASK, code = e >> NB_CODE_SHIFT;
+      if (MIXED && sc.masked[code]) continue;
+      perm_field_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, fd, fp);
+    }
+    if (MIXED) {
+      int ms = nsp[s];
+      const uint32_t *srow = sp + (size_t)s * maxsp;
+      for (int t = lane; t < ms; t += LPA) {
+        uint32_t e = srow[t];
+        int k = e & NB_INDEX_MASK;
+        perm_field_pair<double>(pi, ai, pos[k], mp[k], e >> NB_CODE_SHIFT, cell, aewald, sc, fd, fp);
+      }
+    }
+  }
+  group_sum3<LPA>(fd);
+  group_sum3<LPA>(fp);
+  if (valid && lane == 0) {
+    st3(field, s, fd);
+    st3(fieldp, s, fp);
+  }
+}
+
+// PRE: dipoles are polar * r (preconditioner field, aewald = 0)
+template <typename T, bool PRE, int LPA>
+__global__ void __launch_bounds__(256) k_dip_field(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
+                                                   const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
+                                                   int maxl, CellD cell, double aewald, const double *__restrict__ u,
+                                                   const double *__restrict__ up, const double *__restrict__ polar,
+                                                   double *__restrict__ f, double *__restrict__ fp) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  bool valid = s < n;
+  V3<double> a{0, 0, 0}, b{0, 0, 0};
+  if (valid) {
+    double4 pi = pos[s];
+    T ipdi = (T)mp[s].ipd, ti = (T)mp[s].thole;
+    int m = cnt[s];
+    const uint32_t *row = lst + (size_t)s * maxl;
+    for (int t = lane; t < m; t += LPA) {
+      int k = row[t] & NB_INDEX_MASK;
+      double4 pk = pos[k];
+      double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+      double r2 = image(cell, dx, dy, dz);
+      T r = (T)sqrt(r2), rinv = (T)1 / r;
+      V3<T> R{(T)dx, (T)dy, (T)dz};
+      T bn[3];
+      ewald_bn<T, 2>(r, rinv, (T)aewald, bn);
+      Thole<T> th = thole<T, false>(r, R, ipdi, (T)mp[k].ipd, ti, (T)mp[k].thole);
+      T rr3 = rinv * rinv * rinv, rr5 = (T)3 * rr3 * rinv * rinv;
+      T B1 = bn[1] - ((T)1 - th.s3) * rr3, B2 = bn[2] - ((T)1 - th.s5) * rr5;
+      V3<T> uk = ld3t<T>(u, k), pk3 = ld3t<T>(up, k);
+      if (PRE) {
+        T pol = (T)polar[k];
+        uk = pol * uk;
+        pk3 = pol * pk3;
+      }
+      acc3(a, dip_field(R, uk, B1, B2));
+      acc3(b, dip_field(R, pk3, B1, B2));
+    }
+  }
+  group_sum3<LPA>(a);
+  group_sum3<LPA>(b);
+  if (valid && lane == 0) {
+    st3(f, s, a);
+    st3(fp, s, b);
+  }
+}
+
+// one permanent + polarization energy pair in arithmetic U (FP64 accumulation)
+template <typename U>
+__device__ __forceinline__ void real_energy_pair(const double4 &pi, const AtomMP &ai, const double *u,
+                                                 const double *up, int s, const double4 &pk, const AtomMP &ak, int k,
+                                                 int code, const CellD &cell, double aewald, const ScaleTab &sc,
+                                                 int polar, int direct, double &ep, double &ei, long long &cnt,
+                                                 V3<double> &F, V3<double> &Tq) {
+  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+  double r2 = image(cell, dx, dy, dz);
+  MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
+  V3<U> r{(U)dx, (U)dy, (U)dz};
+  U rl = (U)sqrt(r2), rinv = (U)1 / rl;
+  U bn[7], rr[7];
+  ewald_bn<U, 6>(rl, rinv, (U)aewald, bn);
+  coulomb_rr<U, 6>(rinv, rr);
+  PermGeo<U> g;
+  perm_geometry(r, I, K, g);
+  V3<U> f{0, 0, 0}, tq{0, 0, 0};
+  U epair = perm_terms<U, true>(r, I, K, g, bn, (U)1, f, tq);
+  U s1 = (U)1 - (U)sc.m[code];
+  if (s1 != (U)0) epair -= s1 * perm_terms<U, true>(r, I, K, g, rr, -s1, f, tq);
+  if (epair != (U)0) cnt++;
+  ep += (double)epair;
+  if (polar) {
+    Thole<U> th = thole<U, true>(rl, r, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
+    ei += (double)pol_terms<U, true>(r, I, K, ld3t<U>(u, s), ld3t<U>(up, s), ld3t<U>(u, k), ld3t<U>(up, k), bn, rr, th,
+                                     (U)sc.p[code], (U)sc.d[code], direct != 0, f, tq);
+  }
+  acc3(F, f);
+  acc3(Tq, tq);
+}
+
+template <typename T, int LPA>
+__global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
+                                                     const uint32_t *__restrict__ ew, const int32_t *__restrict__ new_,
+                                                     int maxew, const uint32_t *__restrict__ sp,
+                                                     const int32_t *__restrict__ nsp, int maxsp, CellD cell,
+                                                     double aewald, ScaleTab sc, int polar, int direct,
+                                                     const double *__restrict__ u, const double *__restrict__ up,
+                                                     double el, double *__restrict__ grad, double *__restrict__ torque,
+                                                     double *__restrict__ energy, int64_t *__restrict__ count) {
+  constexpr bool MIXED = sizeof(T) == 4;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  bool valid = s < n;
+  double ep = 0.0, ei = 0.0;
+  long long cnt = 0;
+  V3<double> F{0, 0, 0}, Tq{0, 0, 0};
+  if (valid) {
+    double4 pi = pos[s];
+    AtomMP ai = mp[s];
+    int m = new_[s];
+    const uint32_t *row = ew + (size_t)s * maxew;
+    for (int t = lane; t < m; t += LPA) {
+      uint32_t e = row[t];
+      int k = e & NB_INDEX_MASK, code = e >> NB_CODE_SHIFT;
+      if (MIXED && sc.masked[code]) continue;
+      real_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, polar, direct, ep, ei, cnt, F,
+                          Tq);
+    }
+    if (MIXED) {
+      int ms = nsp[s];
+      const uint32_t *srow = sp + (size_t)s * maxsp;
+      for (int t = lane; t < ms; t += LPA) {
+        uint32_t e = srow[t];
+        int k = e & NB_INDEX_MASK;
+        real_energy_pair<double>(pi, ai, u, up, s, pos[k], mp[k], k, e >> NB_CODE_SHIFT, cell, aewald, sc, polar,
+                                 direct, ep, ei, cnt, F, Tq);
+      }
+    }
+  }
+  group_sum3<LPA>(F);
+  group_sum3<LPA>(Tq);
+  if (valid && lane == 0) {
+    add3(grad, s, el * F);
+    add3(torque, s, el * Tq);
+  }
+  block_add(0.5 * el * ep, energy + E_PERM_REAL);
+  block_add(0.5 * el * ei, energy + E_POL_REAL);
+  block_add_ll(cnt, count + 1);
+}
+
+template <typename T, int LPA>
+__global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ red, const int32_t *__restrict__ vclass,
+                                             const uint32_t *__restrict__ nbr, const int32_t *__restrict__ nnb, int maxnb,
+                                             CellD cell, const double *__restrict__ vtab, int nclass, ScaleTab sc,
+                                             double off, double cut, double dl, double gm, double *__restrict__ gred,
+                                             double *__restrict__ energy, int64_t *__restrict__ count) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  bool valid = s < n;
+  double esum = 0.0;
+  long long cnt = 0;
+  V3<double> G{0, 0, 0};
+  if (valid) {
+    double off2 = off * off, cut2 = cut * cut;
+    // MultiplicativeSwitch (f(cut) = 1, f(off) = 0, zero first and second
+    // derivatives at both ends) in the normalised variable x = (r - cut) /
+    // (off - cut): f = 1 - 10 x^3 + 15 x^4 - 6 x^5.  Identical polynomial to
+    // the monomial form of MultiplicativeSwitch.java:74-96, but without its
+    // cancellation, which FP32 cannot absorb.
+    T wcut = (T)cut, iw = (T)(1.0 / (off - cut));
+    T d1 = (T)dl, g1 = (T)1 + (T)gm, gg = (T)gm;
+    T t1n = (T)pow(1.0 + dl, 7.0);
+    double4 pi = red[s];
+    int ci = vclass[s];
+    int m = nnb[s];
+    const uint32_t *row = nbr + (size_t)s * maxnb;
+    for (int t = lane; t < m; t += LPA) {
+      uint32_t en = row[t];
+      int k = en & NB_INDEX_MASK, code = en >> NB_CODE_SHIFT;
+      double vs = sc.v[code];
+      if (!(vs > 0.0)) continue;
+      double4 pk = red[k];
+      double dx = pi.x - pk.x, dy = pi.y - pk.y, dz = pi.z - pk.z;
+      double r2 = image(cell, dx, dy, dz);
+      if (r2 > off2) continue;
+      const double *pe = vtab + 2 * (ci * nclass + vclass[k]);
+      T irv = (T)pe[0];
+      if (!(irv > (T)0)) continue;
+      cnt++;
+      T rr = (T)sqrt(r2);
+      T ev = (T)(vs * pe[1]);
+      T rho = rr * irv;
+      T rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
+      T rd = rho + d1, rd2 = rd * rd, rd6 = rd2 * rd2 * rd2, rd7 = rd6 * rd;
+      T t1d = (T)1 / rd7, t2d = (T)1 / (rho7 + gg);
+      T t1 = t1n * t1d, t2a = g1 * t2d, t2 = t2a - (T)2;
+      T eik = ev * t1 * t2;
+      T tp = 1, dtp = 0;
+      if (r2 > cut2) {
+        T x = (rr - wcut) * iw, x2 = x * x;
+        tp = (T)1 + x2 * x * ((T)-10 + x * ((T)15 - (T)6 * x));
+        dtp = x2 * ((T)-30 + x * ((T)60 - (T)30 * x)) * iw;
+      }
+      esum += (double)(eik * tp);
+      T dt1 = t1 * (T)7 * rd6 * irv * t1d;
+      T dt2 = t2a * (T)7 * rho6 * irv * t2d;
+      T dedr = -ev * (dt1 * t2 + t1 * dt2);
+      T ds = (eik * dtp + dedr * tp) / rr;
+      acc3(G, V3<T>{ds * (T)dx, ds * (T)dy, ds * (T)dz});
+    }
+  }
+  group_sum3<LPA>(G);
+  if (valid && lane == 0) st3(gred, s, G);
+  block_add(0.5 * esum, energy + E_VDW);
+  block_add_ll(cnt, count);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void nb_perm_field(NbHandle *h) {
+  int n = h->n;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
+  ScaleTab sc = scale_tab(h);
+  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  double off2 = h->prm.ewald_cutoff * h->prm.ewald_cutoff;
+  for (int attempt = 0;; attempt++) {
+    int32_t zeros[5] = {0, 0, 0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(h->d_flags + 1, zeros, sizeof(zeros), hipMemcpyHostToDevice, h->stream));
+    NB_LAUNCH(k_filter<NB_LPA>, G, B, 0, n, h->d_pos4, h->d_nbr, h->d_nnb, h->maxnb, h->cell, off2,
+              h->prm.precond_cutoff, sc, h->d_ew, h->maxew, h->d_new, h->d_pc, h->maxpc, h->d_npc, h->d_sp, h->maxsp,
+              h->d_nsp, h->d_flags);
+    int32_t f[5];
+    HIPCHECK(hipMemcpyAsync(f, h->d_flags + 1, sizeof(f), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    if (f[0] <= h->maxew && f[1] <= h->maxpc && f[4] <= h->maxsp) break;
+    if (attempt == 3) throw std::runtime_error("Ewald list capacity could not be established");
+    if (f[0] > h->maxew) {
+      hipFree(h->d_ew);
+      h->maxew = f[0] + f[0] / 4 + 16;
+      HIPCHECK(hipMalloc(&h->d_ew, sizeof(uint32_t) * (size_t)h->maxew * n));
+    }
+    if (f[1] > h->maxpc) {
+      hipFree(h->d_pc);
+      h->maxpc = f[1] + f[1] / 4 + 16;
+      HIPCHECK(hipMalloc(&h->d_pc, sizeof(uint32_t) * (size_t)h->maxpc * n));
+    }
+    if (f[4] > h->maxsp) {
+      hipFree(h->d_sp);
+      h->maxsp = f[4] + f[4] / 4 + 16;
+      HIPCHECK(hipMalloc(&h->d_sp, sizeof(uint32_t) * (size_t)h->maxsp * n));
+    }
+  }
+  if (h->prm.precision == NB_PREC_MIXED)
+    NB_LAUNCH((k_perm_field<float, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
+              h->maxsp, h->cell, aw, sc, h->d_field, h->d_fieldp);
+  else
+    NB_LAUNCH((k_perm_field<double, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
+              h->maxsp, h->cell, aw, sc, h->d_field, h->d_fieldp);
+}
+
+void nb_dip_field_real(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool precond) {
+  int n = h->n;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = (use_recip && !precond) ? h->prm.ewald_alpha : 0.0;
+  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  const uint32_t *lst = precond ? h->d_pc : h->d_ew;
+  const int32_t *cnt = precond ? h->d_npc : h->d_new;
+  int maxl = precond ? h->maxpc : h->maxew;
+  bool mixed = h->prm.precision == NB_PREC_MIXED;
+  if (precond) {
+    if (mixed)
+      NB_LAUNCH((k_dip_field<float, true, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u, up,
+                h->d_polar, f, fp);
+    else
+      NB_LAUNCH((k_dip_field<double, true, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u, up,
+                h->d_polar, f, fp);
+  } else {
+    if (mixed)
+      NB_LAUNCH((k_dip_field<float, false, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u,
+                up, h->d_polar, f, fp);
+    else
+      NB_LAUNCH((k_dip_field<double, false, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u,
+                up, h->d_polar, f, fp);
+  }
+}
+
+void nb_real_energy(NbHandle *h, bool polar) {
+  int n = h->n;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
+  ScaleTab sc = scale_tab(h);
+  int direct = h->prm.polarization == NB_POL_DIRECT;
+  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  if (h->prm.precision == NB_PREC_MIXED)
+    NB_LAUNCH((k_real_energy<float, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
+              h->maxsp, h->cell, aw, sc, (int)polar, direct, h->d_u, h->d_up, h->prm.electric, h->d_grad_s, h->d_torque_s, h->d_energy,
+              h->d_count);
+  else
+    NB_LAUNCH((k_real_energy<double, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
+              h->maxsp, h->cell, aw, sc, (int)polar, direct, h->d_u, h->d_up, h->prm.electric, h->d_grad_s, h->d_torque_s, h->d_energy,
+              h->d_count);
+}
+
+void nb_vdw(NbHandle *h) {
+  int n = h->n;
+  ScaleTab sc = scale_tab(h);
+  double off = h->prm.vdw_cutoff, cut = h->prm.vdw_taper * off;
+  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  if (h->prm.precision == NB_PREC_MIXED)
+    NB_LAUNCH((k_vdw<float, NB_LPA>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,
+              h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,
+              h->d_energy, h->d_count);
+  else
+    NB_LAUNCH((k_vdw<double, NB_LPA>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,
+              h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,
+              h->d_energy, h->d_count);
+}

@@ -19,7 +19,7 @@
 #include <stdexcept>
 #include <vector>
 
-#include "nb_state.h"
+#include "nb_kern.h"
 
 namespace {
 
@@ -143,109 +143,47 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
 
 __device__ __forceinline__ int wrapg(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 
-// one thread per atom, 125-point scatter; permanent density in the real part
-__global__ void k_spread_perm(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
-                              const double *__restrict__ fmp, int K0, int K1, int K2, double *__restrict__ grid) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+// One thread per (atom, grid point [, real/imaginary part]): consecutive
+// lanes hit consecutive x points (and, for the induced grid, the two
+// interleaved parts of a point), so every atomic instruction of a wavefront
+// touches a handful of cache lines instead of one line per lane.
+__global__ void __launch_bounds__(256) k_spread_perm(int n, const double *__restrict__ theta,
+                                                     const int32_t *__restrict__ igrid, const double *__restrict__ fmp,
+                                                     int K0, int K1, int K2, double *__restrict__ grid) {
+  long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int s = (int)(gid / 125), r = (int)(gid - (long)s * 125);
   if (s >= n) return;
-  const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *tz = tx + 40;
-  const double *m = fmp + 10 * s;
-  int i0 = igrid[3 * s] + 1, j0 = igrid[3 * s + 1] + 1, k0 = igrid[3 * s + 2] + 1;
-  for (int kz = 0; kz < 5; kz++) {
-    int k = wrapg(k0 + kz, K2);
-    double v0 = tz[4 * kz], v1 = tz[4 * kz + 1], v2 = tz[4 * kz + 2];
-    for (int ky = 0; ky < 5; ky++) {
-      int j = wrapg(j0 + ky, K1);
-      double u0 = ty[4 * ky], u1 = ty[4 * ky + 1], u2 = ty[4 * ky + 2];
-      double t0 = (m[0] * v0 + m[3] * v1 + m[6] * v2) * u0 + (m[2] * v0 + m[9] * v1) * u1 + m[5] * v0 * u2;
-      double t1 = (m[1] * v0 + m[8] * v1) * u0 + m[7] * v0 * u1;
-      double t2 = m[4] * v0 * u0;
-      size_t row = (size_t)K0 * (j + (size_t)K1 * k);
-      for (int kx = 0; kx < 5; kx++) {
-        int i = wrapg(i0 + kx, K0);
-        const double *w = tx + 4 * kx;
-        atomicAdd(&grid[2 * (row + i)], w[0] * t0 + w[1] * t1 + w[2] * t2);
-      }
-    }
-  }
+  int kz = r / 25, ky = (r / 5) % 5, kx = r % 5;
+  const double *tx = theta + (size_t)s * 60, *u = tx + 20 + 4 * ky, *v = tx + 40 + 4 * kz, *w = tx + 4 * kx;
+  const double *m = fmp + 10 * (size_t)s;
+  int k = wrapg(igrid[3 * s + 2] + 1 + kz, K2), j = wrapg(igrid[3 * s + 1] + 1 + ky, K1);
+  int i = wrapg(igrid[3 * s] + 1 + kx, K0);
+  double t0 = (m[0] * v[0] + m[3] * v[1] + m[6] * v[2]) * u[0] + (m[2] * v[0] + m[9] * v[1]) * u[1] + m[5] * v[0] * u[2];
+  double t1 = (m[1] * v[0] + m[8] * v[1]) * u[0] + m[7] * v[0] * u[1];
+  double t2 = m[4] * v[0] * u[0];
+  atomicAdd(&grid[2 * ((size_t)K0 * (j + (size_t)K1 * k) + i)], w[0] * t0 + w[1] * t1 + w[2] * t2);
 }
 
 // induced dipoles: u in the real part, uCR in the imaginary part
-__global__ void k_spread_ind(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
-                             const double *__restrict__ u, const double *__restrict__ up, Frac fr, int K0, int K1,
-                             int K2, double *__restrict__ grid) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_spread_ind(int n, const double *__restrict__ theta,
+                                                    const int32_t *__restrict__ igrid, const double *__restrict__ u3,
+                                                    const double *__restrict__ up3, Frac fr, int K0, int K1, int K2,
+                                                    double *__restrict__ grid) {
+  long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int s = (int)(gid / 250), r = (int)(gid - (long)s * 250);
   if (s >= n) return;
-  const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *tz = tx + 40;
-  double d[3], q[3];
-  for (int j = 0; j < 3; j++) {
-    d[j] = fr.a[j][0] * u[3 * s] + fr.a[j][1] * u[3 * s + 1] + fr.a[j][2] * u[3 * s + 2];
-    q[j] = fr.a[j][0] * up[3 * s] + fr.a[j][1] * up[3 * s + 1] + fr.a[j][2] * up[3 * s + 2];
-  }
-  int i0 = igrid[3 * s] + 1, j0 = igrid[3 * s + 1] + 1, k0 = igrid[3 * s + 2] + 1;
-  for (int kz = 0; kz < 5; kz++) {
-    int k = wrapg(k0 + kz, K2);
-    double v0 = tz[4 * kz], v1 = tz[4 * kz + 1];
-    for (int ky = 0; ky < 5; ky++) {
-      int j = wrapg(j0 + ky, K1);
-      double u0 = ty[4 * ky], u1 = ty[4 * ky + 1];
-      double t0 = d[2] * v1 * u0 + d[1] * v0 * u1, t1 = d[0] * v0 * u0;
-      double s0 = q[2] * v1 * u0 + q[1] * v0 * u1, s1 = q[0] * v0 * u0;
-      size_t row = (size_t)K0 * (j + (size_t)K1 * k);
-      for (int kx = 0; kx < 5; kx++) {
-        int i = wrapg(i0 + kx, K0);
-        const double *w = tx + 4 * kx;
-        atomicAdd(&grid[2 * (row + i)], w[0] * t0 + w[1] * t1);
-        atomicAdd(&grid[2 * (row + i) + 1], w[0] * s0 + w[1] * s1);
-      }
-    }
-  }
-}
-
-// 20 fractional derivatives of one grid part (im = 0 real, 1 imaginary)
-template <int NP>
-__device__ __forceinline__ void gather20(const double *__restrict__ grid, const double *tx, const double *ty,
-                                         const double *tz, int i0, int j0, int k0, int K0, int K1, int K2,
-                                         double phi[NP][20]) {
-  for (int p = 0; p < NP; p++)
-    for (int c = 0; c < 20; c++) phi[p][c] = 0.0;
-  for (int kz = 0; kz < 5; kz++) {
-    int k = wrapg(k0 + kz, K2);
-    double tu[NP][10];
-    for (int p = 0; p < NP; p++)
-      for (int c = 0; c < 10; c++) tu[p][c] = 0.0;
-    for (int ky = 0; ky < 5; ky++) {
-      int j = wrapg(j0 + ky, K1);
-      size_t row = (size_t)K0 * (j + (size_t)K1 * k);
-      double t[NP][4];
-      for (int p = 0; p < NP; p++) t[p][0] = t[p][1] = t[p][2] = t[p][3] = 0.0;
-      for (int kx = 0; kx < 5; kx++) {
-        int i = wrapg(i0 + kx, K0);
-        const double *w = tx + 4 * kx;
-        for (int p = 0; p < NP; p++) {
-          double q = grid[2 * (row + i) + p];
-          t[p][0] += q * w[0]; t[p][1] += q * w[1]; t[p][2] += q * w[2]; t[p][3] += q * w[3];
-        }
-      }
-      const double *u = ty + 4 * ky;
-      for (int p = 0; p < NP; p++) {
-        tu[p][0] += t[p][0] * u[0]; tu[p][1] += t[p][1] * u[0]; tu[p][2] += t[p][0] * u[1];
-        tu[p][3] += t[p][2] * u[0]; tu[p][4] += t[p][1] * u[1]; tu[p][5] += t[p][0] * u[2];
-        tu[p][6] += t[p][3] * u[0]; tu[p][7] += t[p][2] * u[1]; tu[p][8] += t[p][1] * u[2]; tu[p][9] += t[p][0] * u[3];
-      }
-    }
-    const double *v = tz + 4 * kz;
-    for (int p = 0; p < NP; p++) {
-      double *ph = phi[p];
-      const double *a = tu[p];
-      ph[0] += a[0] * v[0]; ph[1] += a[1] * v[0]; ph[2] += a[2] * v[0]; ph[3] += a[0] * v[1];
-      ph[4] += a[3] * v[0]; ph[5] += a[5] * v[0]; ph[6] += a[0] * v[2];
-      ph[7] += a[4] * v[0]; ph[8] += a[1] * v[1]; ph[9] += a[2] * v[1];
-      ph[10] += a[6] * v[0]; ph[11] += a[9] * v[0]; ph[12] += a[0] * v[3];
-      ph[13] += a[7] * v[0]; ph[14] += a[3] * v[1]; ph[15] += a[8] * v[0];
-      ph[16] += a[5] * v[1]; ph[17] += a[1] * v[2]; ph[18] += a[2] * v[2]; ph[19] += a[4] * v[1];
-    }
-  }
+  int part = r & 1;
+  r >>= 1;
+  int kz = r / 25, ky = (r / 5) % 5, kx = r % 5;
+  const double *tx = theta + (size_t)s * 60, *u = tx + 20 + 4 * ky, *v = tx + 40 + 4 * kz, *w = tx + 4 * kx;
+  const double *src = part ? up3 : u3;
+  double d[3];
+  for (int c = 0; c < 3; c++)
+    d[c] = fr.a[c][0] * src[3 * s] + fr.a[c][1] * src[3 * s + 1] + fr.a[c][2] * src[3 * s + 2];
+  int k = wrapg(igrid[3 * s + 2] + 1 + kz, K2), j = wrapg(igrid[3 * s + 1] + 1 + ky, K1);
+  int i = wrapg(igrid[3 * s] + 1 + kx, K0);
+  double t0 = d[2] * v[1] * u[0] + d[1] * v[0] * u[1], t1 = d[0] * v[0] * u[0];
+  atomicAdd(&grid[2 * ((size_t)K0 * (j + (size_t)K1 * k) + i) + part], w[0] * t0 + w[1] * t1);
 }
 
 __device__ __forceinline__ void phi_cart(const Frac &fr, const double *f, double *c) {
@@ -259,22 +197,97 @@ __device__ __forceinline__ void phi_cart(const Frac &fr, const double *f, double
   c[4] = C[0][0]; c[5] = C[1][1]; c[6] = C[2][2]; c[7] = C[0][1]; c[8] = C[0][2]; c[9] = C[1][2];
 }
 
-// out[30 s]: 20 fractional + 10 Cartesian
-template <int NP>
-__global__ void k_gather(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
-                         const double *__restrict__ grid, Frac fr, int K0, int K1, int K2, double *__restrict__ out0,
-                         double *__restrict__ out1) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const double *tx = theta + (size_t)s * 60;
-  double phi[NP][20];
-  gather20<NP>(grid, tx, tx + 20, tx + 40, igrid[3 * s] + 1, igrid[3 * s + 1] + 1, igrid[3 * s + 2] + 1, K0, K1, K2,
-               phi);
-  double *outs[2] = {out0, out1};
-  for (int p = 0; p < NP; p++) {
-    double *o = outs[p] + 30 * (size_t)s;
-    for (int c = 0; c < 20; c++) o[c] = phi[p][c];
-    phi_cart(fr, phi[p], o + 20);
+// Gather: 8 lanes per atom, lane kz < 5 sums one z plane (25 rows of 5
+// points) into fractional derivatives, then the lanes are combined with
+// shuffles.  FULL: all 20 derivatives t000..t111 plus the Cartesian
+// transform (out[30 s] = 20 fractional + 10 Cartesian).  Otherwise only the
+// first derivatives are formed and the Cartesian field components are
+// written to out[30 s + 21..23] (what the PCG iterations consume).
+template <int NP, bool FULL>
+__global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
+                                                const double *__restrict__ grid, Frac fr, int K0, int K1, int K2,
+                                                double *__restrict__ out0, double *__restrict__ out1) {
+  constexpr int NC = FULL ? 20 : 4;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt >> 3, lane = gt & 7;
+  bool valid = s < n;
+  double phi[NP][NC];
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+#pragma unroll
+    for (int c = 0; c < NC; c++) phi[p][c] = 0.0;
+  if (valid && lane < 5) {
+    const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *v = tx + 40 + 4 * lane;
+    int k = wrapg(igrid[3 * s + 2] + 1 + lane, K2);
+    int j0 = igrid[3 * s + 1] + 1, i0 = igrid[3 * s] + 1;
+    int ix[5];
+#pragma unroll
+    for (int kx = 0; kx < 5; kx++) ix[kx] = wrapg(i0 + kx, K0);
+    double tu[NP][10];
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int c = 0; c < 10; c++) tu[p][c] = 0.0;
+    for (int ky = 0; ky < 5; ky++) {
+      int j = wrapg(j0 + ky, K1);
+      size_t row = (size_t)K0 * (j + (size_t)K1 * k);
+      double t[NP][4];
+#pragma unroll
+      for (int p = 0; p < NP; p++) t[p][0] = t[p][1] = t[p][2] = t[p][3] = 0.0;
+#pragma unroll
+      for (int kx = 0; kx < 5; kx++) {
+        const double *w = tx + 4 * kx;
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+          double q = grid[2 * (row + ix[kx]) + p];
+          t[p][0] += q * w[0]; t[p][1] += q * w[1];
+          if (FULL) { t[p][2] += q * w[2]; t[p][3] += q * w[3]; }
+        }
+      }
+      const double *uu = ty + 4 * ky;
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        tu[p][0] += t[p][0] * uu[0]; tu[p][1] += t[p][1] * uu[0]; tu[p][2] += t[p][0] * uu[1];
+        if (FULL) {
+          tu[p][3] += t[p][2] * uu[0]; tu[p][4] += t[p][1] * uu[1]; tu[p][5] += t[p][0] * uu[2];
+          tu[p][6] += t[p][3] * uu[0]; tu[p][7] += t[p][2] * uu[1]; tu[p][8] += t[p][1] * uu[2];
+          tu[p][9] += t[p][0] * uu[3];
+        }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      double *ph = phi[p];
+      const double *a = tu[p];
+      if (FULL) {
+        ph[0] = a[0] * v[0]; ph[1] = a[1] * v[0]; ph[2] = a[2] * v[0]; ph[3] = a[0] * v[1];
+        ph[4] = a[3] * v[0]; ph[5] = a[5] * v[0]; ph[6] = a[0] * v[2];
+        ph[7] = a[4] * v[0]; ph[8] = a[1] * v[1]; ph[9] = a[2] * v[1];
+        ph[10] = a[6] * v[0]; ph[11] = a[9] * v[0]; ph[12] = a[0] * v[3];
+        ph[13] = a[7] * v[0]; ph[14] = a[3] * v[1]; ph[15] = a[8] * v[0];
+        ph[16] = a[5] * v[1]; ph[17] = a[1] * v[2]; ph[18] = a[2] * v[2]; ph[19] = a[4] * v[1];
+      } else {
+        ph[1] = a[1] * v[0]; ph[2] = a[2] * v[0]; ph[3] = a[0] * v[1];
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; p++)
+#pragma unroll
+    for (int c = 0; c < NC; c++) phi[p][c] = group_sum<8>(phi[p][c]);
+  if (valid && lane == 0) {
+    double *outs[2] = {out0, out1};
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+      double *o = outs[p] + 30 * (size_t)s;
+      if (FULL) {
+        for (int c = 0; c < 20; c++) o[c] = phi[p][c];
+        phi_cart(fr, phi[p], o + 20);
+      } else {
+        for (int c = 0; c < 3; c++)
+          o[21 + c] = fr.a[0][c] * phi[p][1] + fr.a[1][c] * phi[p][2] + fr.a[2][c] * phi[p][3];
+      }
+    }
   }
 }
 
@@ -423,8 +436,8 @@ Radices factor(int N) {
 }
 
 int lines_per_block(int N) {
-  int L = 1536 / N;
-  if (L > 16) L = 16;
+  int L = 512 / N;
+  if (L > 32) L = 32;
   if (L < 1) L = 1;
   return L;
 }
@@ -549,45 +562,48 @@ static void convolution(NbHandle *h) {
   size_t shx = sizeof(double2) * (2 * (size_t)Lx * K0 + K0);
   size_t shy = sizeof(double2) * (2 * (size_t)Ly * K1 + K1);
   size_t shz = sizeof(double2) * (2 * (size_t)Lz * K2 + K2);
-  dim3 B(256);
+  dim3 B(128);
   dim3 Gx((K1 * K2 + Lx - 1) / Lx), Gy((K0 * K2 + Ly - 1) / Ly), Gz((K0 * K1 + Lz - 1) / Lz);
-  hipLaunchKernelGGL((k_fft_axis<0, false>), Gx, B, shx, h->stream, g, K0, K1, K2, Lx, rx, twx, 1.0, nullptr);
-  hipLaunchKernelGGL((k_fft_axis<1, false>), Gy, B, shy, h->stream, g, K0, K1, K2, Ly, ry, twy, 1.0, nullptr);
-  hipLaunchKernelGGL((k_fft_axis<2, true>), Gz, B, shz, h->stream, g, K0, K1, K2, Lz, rz, twz, 1.0, h->d_recip);
-  hipLaunchKernelGGL((k_fft_axis<1, false>), Gy, B, shy, h->stream, g, K0, K1, K2, Ly, ry, twy, -1.0, nullptr);
-  hipLaunchKernelGGL((k_fft_axis<0, false>), Gx, B, shx, h->stream, g, K0, K1, K2, Lx, rx, twx, -1.0, nullptr);
+  NB_LAUNCH((k_fft_axis<0, false>), Gx, B, shx, g, K0, K1, K2, Lx, rx, twx, 1.0, nullptr);
+  NB_LAUNCH((k_fft_axis<1, false>), Gy, B, shy, g, K0, K1, K2, Ly, ry, twy, 1.0, nullptr);
+  NB_LAUNCH((k_fft_axis<2, true>), Gz, B, shz, g, K0, K1, K2, Lz, rz, twz, 1.0, h->d_recip);
+  NB_LAUNCH((k_fft_axis<1, false>), Gy, B, shy, g, K0, K1, K2, Ly, ry, twy, -1.0, nullptr);
+  NB_LAUNCH((k_fft_axis<0, false>), Gx, B, shx, g, K0, K1, K2, Lx, rx, twx, -1.0, nullptr);
 }
 
 void nb_pme_permanent(NbHandle *h) {
   int n = h->n;
   dim3 B(128), G((n + 127) / 128);
   Frac fr = frac_of(h);
-  hipLaunchKernelGGL(k_bspline_frac, G, B, 0, h->stream, n, h->d_pos4, h->d_gmp, h->cell, h->K[0], h->K[1], h->K[2],
-                     fr, h->d_theta, h->d_igrid, h->d_fmp);
+  NB_LAUNCH(k_bspline_frac, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, h->K[0], h->K[1], h->K[2], fr, h->d_theta,
+            h->d_igrid, h->d_fmp);
   HIPCHECK(hipMemsetAsync(h->d_grid, 0, sizeof(double) * 2 * h->ntot, h->stream));
-  hipLaunchKernelGGL(k_spread_perm, G, B, 0, h->stream, n, h->d_theta, h->d_igrid, h->d_fmp, h->K[0], h->K[1], h->K[2],
-                     h->d_grid);
+  NB_LAUNCH(k_spread_perm, grid_for((long)n * 125, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_fmp, h->K[0],
+            h->K[1], h->K[2], h->d_grid);
   convolution(h);
-  hipLaunchKernelGGL((k_gather<1>), G, B, 0, h->stream, n, h->d_theta, h->d_igrid, h->d_grid, fr, h->K[0], h->K[1],
-                     h->K[2], h->d_phip, (double *)nullptr);
+  NB_LAUNCH((k_gather<1, true>), grid_for((long)n * 8, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_grid, fr,
+            h->K[0], h->K[1], h->K[2], h->d_phip, (double *)nullptr);
 }
 
-void nb_pme_induced(NbHandle *h, const double *u, const double *up) {
+void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full) {
   int n = h->n;
-  dim3 B(128), G((n + 127) / 128);
   Frac fr = frac_of(h);
   HIPCHECK(hipMemsetAsync(h->d_grid, 0, sizeof(double) * 2 * h->ntot, h->stream));
-  hipLaunchKernelGGL(k_spread_ind, G, B, 0, h->stream, n, h->d_theta, h->d_igrid, u, up, fr, h->K[0], h->K[1], h->K[2],
-                     h->d_grid);
+  NB_LAUNCH(k_spread_ind, grid_for((long)n * 250, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, u, up, fr, h->K[0],
+            h->K[1], h->K[2], h->d_grid);
   convolution(h);
-  hipLaunchKernelGGL((k_gather<2>), G, B, 0, h->stream, n, h->d_theta, h->d_igrid, h->d_grid, fr, h->K[0], h->K[1],
-                     h->K[2], h->d_phiu, h->d_phiup);
+  if (full)
+    NB_LAUNCH((k_gather<2, true>), grid_for((long)n * 8, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_grid, fr,
+              h->K[0], h->K[1], h->K[2], h->d_phiu, h->d_phiup);
+  else
+    NB_LAUNCH((k_gather<2, false>), grid_for((long)n * 8, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_grid, fr,
+              h->K[0], h->K[1], h->K[2], h->d_phiu, h->d_phiup);
 }
 
 void nb_rotate(NbHandle *h) {
   int n = h->n;
   dim3 B(128), G((n + 127) / 128);
-  hipLaunchKernelGGL(k_rotate, G, B, 0, h->stream, n, h->d_xyz, h->d_frame, h->d_axis, h->d_mpole_local, h->d_iperm,
+  NB_LAUNCH(k_rotate, G, B, 0, n, h->d_xyz, h->d_frame, h->d_axis, h->d_mpole_local, h->d_iperm,
                      h->d_redidx, h->d_redval, h->d_mp, h->d_gmp, h->d_red4);
 }
 

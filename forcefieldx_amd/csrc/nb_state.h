@@ -5,15 +5,16 @@
 //   pos4[s]                  double4 x, y, z, 0 (Cartesian, caller coordinates)
 //   mp[s]                    AtomMP: q, d, Q/3 (global frame), 1/pdamp, thole
 //   polar[s], group[s]       polarizability, polarization group
-//   nbr[t * n + s]           full Verlet list (column-major: coalesced over s),
-//                            entry = k_sorted | mask code << 27
-//   ew / pc [t * n + s]      Ewald (r <= off) and preconditioner (r < 4.5) rows
+//   nbr[s * maxnb + t]       full Verlet list, one row per atom (lanes of an
+//                            atom read consecutive entries), k_sorted | code << 27
+//   ew / pc [s * max + t]    Ewald (r <= off) and preconditioner (r < 4.5) rows
 //   grid                     interleaved complex PME grid (x fastest)
 // Gradients are returned in caller order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/amoeba_nb.h"
@@ -56,7 +57,8 @@ struct NbHandle {
   uint32_t *d_nbr = nullptr;  int maxnb = 0;
   uint32_t *d_ew = nullptr;   int maxew = 0;
   uint32_t *d_pc = nullptr;   int maxpc = 0;
-  int32_t *d_nnb = nullptr, *d_new = nullptr, *d_npc = nullptr;
+  uint32_t *d_sp = nullptr;   int maxsp = 0;   // masked (bonded / same-group) pairs within the Ewald cutoff
+  int32_t *d_nnb = nullptr, *d_new = nullptr, *d_npc = nullptr, *d_nsp = nullptr;
   int32_t *d_flags = nullptr;     // [0] overflow nbr, [1] overflow ew, [2] overflow pc, [3] rebuild needed
   bool list_valid = false;
 
@@ -101,6 +103,15 @@ struct NbHandle {
   double *h_pinned = nullptr;
 
   nb_energies last;
+
+  // profiling
+  bool profiling = false;
+  struct ProfRec { std::string name; hipEvent_t a, b; };
+  std::vector<ProfRec> prof_pool;
+  int prof_n = 0;
+  std::vector<std::string> prof_names;
+  std::vector<double> prof_ms;
+  std::vector<long long> prof_count;
 };
 
 // launch helpers implemented in the kernel files
@@ -110,10 +121,10 @@ void nb_gather_sorted(NbHandle *h);
 void nb_rotate(NbHandle *h);
 void nb_pme_setup(NbHandle *h);
 void nb_pme_permanent(NbHandle *h);
-void nb_pme_induced(NbHandle *h, const double *u, const double *up);
+void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full);
 void nb_perm_field(NbHandle *h);
 void nb_direct(NbHandle *h);
-void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f, double *fp);
+void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool full);
 void nb_precond_field(NbHandle *h, const double *r, const double *rp, double *f, double *fp);
 int nb_pcg(NbHandle *h, nb_energies *out);
 void nb_recip_energy(NbHandle *h, bool polar);
@@ -126,4 +137,18 @@ void nb_zero_outputs(NbHandle *h);
   do {                                                                               \
     hipError_t e_ = (x);                                                             \
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Optional per-launch HIP-event timing (nb_set_profiling); the events are
+// recorded on the handle's stream around each launch and harvested after the
+// evaluation's final synchronisation.
+int nb_prof_begin(NbHandle *h, const char *name);
+void nb_prof_end(NbHandle *h, int slot);
+void nb_prof_harvest(NbHandle *h);
+
+#define NB_LAUNCH(KERNEL, G, B, SH, ...)                              \
+  do {                                                                \
+    int slot_ = h->profiling ? nb_prof_begin(h, #KERNEL) : -1;        \
+    hipLaunchKernelGGL(KERNEL, G, B, SH, h->stream, __VA_ARGS__);     \
+    if (slot_ >= 0) nb_prof_end(h, slot_);                            \
   } while (0)

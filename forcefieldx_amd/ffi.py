@@ -22,7 +22,8 @@ NB_PREC_FP64, NB_PREC_MIXED = 0, 1
 
 EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
-            "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count"]
+            "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
+            "nb_set_profiling", "nb_profile_count", "nb_profile_entry"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -91,6 +92,12 @@ def load(path=LIB_PATH):
               "nb_evaluate", "nb_get_gradient", "nb_get_induced", "nb_get_virial", "nb_get_dudx_dl",
               "nb_abi_version", "nb_device_count"):
         getattr(L, f).restype = C.c_int
+    L.nb_set_profiling.argtypes = [vp, C.c_int]
+    L.nb_set_profiling.restype = C.c_int
+    L.nb_profile_count.argtypes = [vp]
+    L.nb_profile_count.restype = C.c_int
+    L.nb_profile_entry.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_int64)]
+    L.nb_profile_entry.restype = C.c_int
     L.nb_test_convolution.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
     L.nb_test_convolution.restype = C.c_int
     _lib = L
@@ -174,6 +181,20 @@ class Handle:
         up = np.zeros(3 * self.n)
         self._check(self.L.nb_get_induced(self.h, _p(u, C.c_double), _p(up, C.c_double)))
         return u.reshape(self.n, 3), up.reshape(self.n, 3)
+
+    def set_profiling(self, on=True):
+        self._check(self.L.nb_set_profiling(self.h, int(on)))
+
+    def profile(self):
+        """{kernel name: (total ms, launches)} accumulated since set_profiling."""
+        out = {}
+        buf = C.create_string_buffer(256)
+        ms = C.c_double()
+        cnt = C.c_int64()
+        for i in range(self.L.nb_profile_count(self.h)):
+            self._check(self.L.nb_profile_entry(self.h, i, buf, 256, C.byref(ms), C.byref(cnt)))
+            out[buf.value.decode()] = (ms.value, cnt.value)
+        return out
 
     def close(self):
         if getattr(self, "h", None):

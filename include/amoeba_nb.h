@@ -163,6 +163,15 @@ int nb_abi_version(void);
 /* Number of HIP devices visible (0 without a GPU). */
 int nb_device_count(void);
 
+/* Kernel timing with HIP events on the handle's stream (no counterpart in the
+ * reference, whose timers are PMETimings.java:47-120 and
+ * ReciprocalSpace.printTimings :645-718).  When on, every launch of an
+ * evaluation is bracketed by events; totals accumulate per kernel name until
+ * profiling is switched again. */
+int nb_set_profiling(void *handle, int on);
+int nb_profile_count(void *handle);
+int nb_profile_entry(void *handle, int idx, char *name, int cap, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

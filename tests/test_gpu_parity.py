@@ -81,3 +81,19 @@ def test_repeat_evaluation_without_rebuild(oracle, water648):
     eo = oracle.evaluate(oracle.System(x2, box, top), p)
     assert abs(e2 - eo[0]["total"]) <= REL * abs(eo[0]["total"])
     assert rel_err(g2.reshape(-1, 3), eo[1]) <= REL
+
+
+def test_water648_mixed_against_fp64(oracle, water648):
+    """Mixed form (FP32 pair arithmetic, FP64 accumulation/PME/PCG) against the
+    FP64 oracle: force relative RMS error <= 1e-5."""
+    xyz, box, top = water648
+    p = NonbondedParams(vdw_cutoff=9.0, grid=(24, 24, 24), polar_eps=1e-6)
+    print("water-648 mixed", flush=True)
+    eo, go, uo, upo = oracle.evaluate(oracle.System(xyz, box, top), p)
+    ffe = ForceFieldEnergy(top, box, p, precision=ffi.NB_PREC_MIXED)
+    e, g = ffe.energyAndGradient(xyz)
+    u, up = ffe.particleMeshEwald.getInducedDipoles()
+    ffe.destroy()
+    assert rel_err(g.reshape(-1, 3), go) <= 1e-5
+    assert rel_err(u, uo) <= 1e-5
+    assert abs(e - eo["total"]) <= 1e-6 * abs(eo["perm_self"])
