@@ -78,21 +78,32 @@ def roofline_for(name, avg_ms, launches_per_step, pairs, n, K, prec):
 
 
 def time_gpu(top, box, xyz, params, prec, steps, warmup, profile):
-    from forcefieldx_amd import ffi
+    """Warm up, optionally find the dominant kernel with a 2-step fully timed
+    calibration, then time `steps` evaluations.  In the timed region HIP events
+    bracket only the launches of the dominant kernel (on the library's
+    stream), so the kernel's average duration is measured live at negligible
+    overhead."""
     from forcefieldx_amd.energy import ForceFieldEnergy
     ffe = ForceFieldEnergy(top, box, params, precision=prec)
     ffe.setCoordinates(xyz)
     h = ffe._handle
     for _ in range(warmup):
         ffe._evaluate(True)
+    calib = {}
     if profile:
         h.set_profiling(True)
+        for _ in range(2):
+            ffe._evaluate(True)
+        calib = h.profile()
+        dominant = max(calib.items(), key=lambda kv: kv[1][0])[0]
+        h.set_profiling(True, kernel_filter=dominant)
     t0 = time.perf_counter()
     for _ in range(steps):
         ffe._evaluate(True)
     t1 = time.perf_counter()
-    prof = h.profile() if profile else {}
+    prof = {}
     if profile:
+        prof = {"calibration": calib, "timed": h.profile()}
         h.set_profiling(False)
     last = dict(ffe._last)
     g = h.gradient()
@@ -150,10 +161,11 @@ def main():
 
     roof = None
     if profmx:
-        name, (tot, cnt) = max(profmx.items(), key=lambda kv: kv[1][0])
+        name, (tot, cnt) = max(profmx["timed"].items(), key=lambda kv: kv[1][0])
         pairs = {"ewald": int(emx["n_perm"]) - top.n, "vdw": int(emx["n_vdw"]), "precond": None}
         roof = roofline_for(name, tot / cnt, cnt / args.steps, pairs, top.n, GRID[0], "mixed")
         roof["share_of_step"] = tot / (tmx * 1e3 * args.steps)
+        roof["launches_per_step"] = cnt / args.steps
 
     cpu = None
     e_cpu = None
@@ -193,8 +205,8 @@ def main():
         out["gpu_over_cpu"] = out["value"] / cpu["value"]
         out["energy_fp64_minus_oracle"] = e64["total"] - e_cpu["total"]
     if profmx:
-        out["kernel_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in
-                                     sorted(profmx.items(), key=lambda kv: -kv[1][0])}
+        out["kernel_ms_per_step_calibration"] = {k: round(v[0] / 2, 4) for k, v in
+                                                 sorted(profmx["calibration"].items(), key=lambda kv: -kv[1][0])}
     print(json.dumps(out))
 
 

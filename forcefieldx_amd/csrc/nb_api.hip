@@ -142,6 +142,13 @@ int nb_set_profiling(void *handle, int on) {
   return NB_OK;
 }
 
+int nb_set_profile_filter(void *handle, const char *substring) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  h->prof_filter = substring ? substring : "";
+  return NB_OK;
+}
+
 int nb_profile_count(void *handle) {
   NbHandle *h = (NbHandle *)handle;
   return h ? (int)h->prof_names.size() : 0;

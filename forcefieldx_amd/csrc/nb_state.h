@@ -106,6 +106,7 @@ struct NbHandle {
 
   // profiling
   bool profiling = false;
+  std::string prof_filter;  // when set, only launches whose kernel name contains it are timed
   struct ProfRec { std::string name; hipEvent_t a, b; };
   std::vector<ProfRec> prof_pool;
   int prof_n = 0;
@@ -143,12 +144,16 @@ void nb_zero_outputs(NbHandle *h);
 // recorded on the handle's stream around each launch and harvested after the
 // evaluation's final synchronisation.
 int nb_prof_begin(NbHandle *h, const char *name);
+static inline bool nb_prof_wanted(const NbHandle *h, const char *name) {
+  return h->profiling && (h->prof_filter.empty() || h->prof_filter.find(name) != std::string::npos ||
+                          std::string(name).find(h->prof_filter) != std::string::npos);
+}
 void nb_prof_end(NbHandle *h, int slot);
 void nb_prof_harvest(NbHandle *h);
 
 #define NB_LAUNCH(KERNEL, G, B, SH, ...)                              \
   do {                                                                \
-    int slot_ = h->profiling ? nb_prof_begin(h, #KERNEL) : -1;        \
+    int slot_ = nb_prof_wanted(h, #KERNEL) ? nb_prof_begin(h, #KERNEL) : -1; \
     hipLaunchKernelGGL(KERNEL, G, B, SH, h->stream, __VA_ARGS__);     \
     if (slot_ >= 0) nb_prof_end(h, slot_);                            \
   } while (0)

@@ -23,7 +23,7 @@ NB_PREC_FP64, NB_PREC_MIXED = 0, 1
 EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
-            "nb_set_profiling", "nb_profile_count", "nb_profile_entry"]
+            "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -94,6 +94,8 @@ def load(path=LIB_PATH):
         getattr(L, f).restype = C.c_int
     L.nb_set_profiling.argtypes = [vp, C.c_int]
     L.nb_set_profiling.restype = C.c_int
+    L.nb_set_profile_filter.argtypes = [vp, C.c_char_p]
+    L.nb_set_profile_filter.restype = C.c_int
     L.nb_profile_count.argtypes = [vp]
     L.nb_profile_count.restype = C.c_int
     L.nb_profile_entry.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_int64)]
@@ -182,8 +184,9 @@ class Handle:
         self._check(self.L.nb_get_induced(self.h, _p(u, C.c_double), _p(up, C.c_double)))
         return u.reshape(self.n, 3), up.reshape(self.n, 3)
 
-    def set_profiling(self, on=True):
+    def set_profiling(self, on=True, kernel_filter=""):
         self._check(self.L.nb_set_profiling(self.h, int(on)))
+        self._check(self.L.nb_set_profile_filter(self.h, kernel_filter.encode()))
 
     def profile(self):
         """{kernel name: (total ms, launches)} accumulated since set_profiling."""

@@ -169,6 +169,8 @@ int nb_device_count(void);
  * evaluation is bracketed by events; totals accumulate per kernel name until
  * profiling is switched again. */
 int nb_set_profiling(void *handle, int on);
+/* Restrict the timing to kernels whose name contains `substring` ("" = all). */
+int nb_set_profile_filter(void *handle, const char *substring);
 int nb_profile_count(void *handle);
 int nb_profile_entry(void *handle, int idx, char *name, int cap, double *total_ms, int64_t *launches);
 

@@ -732,6 +732,7 @@ static void nlist_build(nlist_t *L, const cell_t *c, int n, const double *xyz, d
   }
   int *cellof = NULL, *cstart = NULL, *corder = NULL;
   int nc = ncell[0] * ncell[1] * ncell[2];
+  if (nc < 1) nc = 1;
   if (use_cells) {
     cellof = malloc(n * sizeof(int));
     cstart = calloc(nc + 1, sizeof(int));
