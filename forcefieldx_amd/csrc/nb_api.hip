@@ -75,9 +75,10 @@ void free_all(NbHandle *h) {
                   h->d_xyz, h->d_xyz_list, h->d_perm, h->d_iperm, h->d_cellid, h->d_cellcount, h->d_cellstart,
                   h->d_nbr, h->d_ew, h->d_pc, h->d_sp, h->d_nsp, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_red4, h->d_mp,
                   h->d_gmp, h->d_polar, h->d_group, h->d_vclass, h->d_grid, h->d_recip, h->d_theta, h->d_igrid,
+                  h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                  h->d_scal, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+                  h->d_scal, h->d_part, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -182,11 +183,16 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     h->device = params->device;
     HIPCHECK(hipSetDevice(h->device));
     HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    {
+      hipDeviceProp_t prop;
+      HIPCHECK(hipGetDeviceProperties(&prop, h->device));
+      h->lds_max = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    }
     int n = h->n;
     if (n >= (int)NB_INDEX_MASK) throw std::runtime_error("too many atoms");
     for (int d = 0; d < 3; d++) {
       int m = params->grid[d];
-      if (params->do_pme && m < 4) throw std::runtime_error("PME grid must be at least 4 per dimension");
+      if (params->do_pme && m < 8) throw std::runtime_error("PME grid must be at least 8 per dimension");
       while (m > 1 && m % 2 == 0) m /= 2;
       while (m > 1 && m % 3 == 0) m /= 3;
       while (m > 1 && m % 5 == 0) m /= 5;
@@ -255,6 +261,9 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_group, n);
     alloc(&h->d_vclass, n);
     alloc(&h->d_theta, 60 * (size_t)n);
+    alloc(&h->d_bid, n);
+    alloc(&h->d_brank, n);
+    alloc(&h->d_batom, n);
     alloc(&h->d_igrid, n3);
     alloc(&h->d_fmp, 10 * (size_t)n);
     alloc(&h->d_phip, 30 * (size_t)n);
@@ -271,6 +280,8 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
       HIPCHECK(hipMemset(*v, 0, sizeof(double) * n3));
     }
     alloc(&h->d_scal, 16);
+    h->part_stride = (3 * n + 255) / 256;
+    alloc(&h->d_part, 16 * (size_t)h->part_stride);
     alloc(&h->d_energy, 16);
     alloc(&h->d_count, 4);
     HIPCHECK(hipHostMalloc((void **)&h->h_pinned, sizeof(double) * 64, hipHostMallocDefault));
@@ -367,8 +378,9 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
         nb_direct(h);
         if (h->prm.polarization == NB_POL_MUTUAL) {
           status = nb_pcg(h, &e);
-        } else {
-          nb_induced_field(h, h->d_u, h->d_up, h->d_field, h->d_fieldp, true);
+        } else if (use_recip) {
+          // direct polarization: only the reciprocal potentials of u_direct are needed
+          nb_pme_induced(h, h->d_u, h->d_up, true);
         }
       }
       if (use_recip) nb_recip_energy(h, polar);

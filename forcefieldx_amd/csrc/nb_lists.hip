@@ -1,6 +1,6 @@
 // nb_lists.hip -- spatial sort and Verlet neighbor list on the GPU
 // (NeighborList.java:351-372, 514-578, 777-1120, 1296-1545 restated for a
-// one-thread-per-atom full list in spatially sorted order).
+// full list in spatially sorted order, built 16 lanes per atom).
 //
 // Atoms are binned by fractional coordinate into cells at least one list
 // cutoff wide, counting-sorted by cell (ties broken by caller index, so the
@@ -11,7 +11,7 @@
 // last build (NeighborList.java:314, 777-802).
 #include <stdexcept>
 
-#include "nb_state.h"
+#include "nb_kern.h"
 
 namespace {
 
@@ -90,19 +90,23 @@ __global__ void k_cell_scatter(int n, const int32_t *__restrict__ cellid, const 
   perm[start[c] + slot] = i;
 }
 
-// order each cell's segment by caller index (deterministic sort)
-__global__ void k_cell_sortseg(int m, const int32_t *__restrict__ start, int32_t *__restrict__ perm) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// order each cell's segment by caller index (deterministic): one workgroup per
+// cell ranks every member by counting the members with a smaller index
+__global__ void __launch_bounds__(256) k_cell_rank(int m, const int32_t *__restrict__ start,
+                                                   const int32_t *__restrict__ unsorted, int32_t *__restrict__ perm) {
+  __shared__ int32_t mem[2048];
+  int c = blockIdx.x;
   if (c >= m) return;
-  int b = start[c], e = start[c + 1];
-  for (int i = b + 1; i < e; i++) {
-    int v = perm[i];
-    int j = i - 1;
-    while (j >= b && perm[j] > v) {
-      perm[j + 1] = perm[j];
-      j--;
-    }
-    perm[j + 1] = v;
+  int b = start[c], e = start[c + 1], cnt = e - b;
+  bool in_lds = cnt <= 2048;
+  if (in_lds)
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) mem[t] = unsorted[b + t];
+  __syncthreads();
+  const int32_t *src = in_lds ? mem : unsorted + b;
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    int v = src[t], rank = 0;
+    for (int j = 0; j < cnt; j++) rank += src[j] < v;
+    perm[b + rank] = v;
   }
 }
 
@@ -136,14 +140,19 @@ __device__ __forceinline__ int relation(const int32_t *__restrict__ off, const i
   return 0;
 }
 
+// LPA lanes per sorted atom scan the neighbouring cells LPA candidates at a
+// time; hits are compacted in candidate order with a ballot, so each row lists
+// its neighbours in (cell, sorted index) order exactly as a serial scan would.
+template <int LPA>
 __global__ void __launch_bounds__(256) k_build_list(int n, const double4 *__restrict__ pos, const int32_t *__restrict__ perm,
                                                     const int32_t *__restrict__ start, CellD cell, int nc0, int nc1,
                                                     int nc2, double cut2, const int32_t *__restrict__ group,
                                                     const int32_t *__restrict__ moff, const int32_t *__restrict__ matom,
                                                     const int32_t *__restrict__ mrel, uint32_t *__restrict__ nbr,
                                                     int maxnb, int32_t *__restrict__ nnb, int32_t *__restrict__ flags) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  if (s >= n) return;  // whole groups leave together
   double4 pi = pos[s];
   int ci = cell_of(cell, nc0, nc1, nc2, pi.x, pi.y, pi.z);
   int cx = ci % nc0, cy = (ci / nc0) % nc1, cz = ci / (nc0 * nc1);
@@ -153,24 +162,35 @@ __global__ void __launch_bounds__(256) k_build_list(int n, const double4 *__rest
   int i = perm[s];
   int gi = group[s];
   int cnt = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int gbase = (threadIdx.x & 63) & ~(LPA - 1);
+  uint32_t *row = nbr + (size_t)s * maxnb;
   for (int dz = lo2; dz <= hi2; dz++)
     for (int dy = lo1; dy <= hi1; dy++)
       for (int dx = lo0; dx <= hi0; dx++) {
         int q = wrapi(cx + dx, nc0) + nc0 * (wrapi(cy + dy, nc1) + nc1 * wrapi(cz + dz, nc2));
         int b = start[q], e = start[q + 1];
-        for (int t = b; t < e; t++) {
-          if (t == s) continue;
-          double4 pk = pos[t];
-          double ddx = pk.x - pi.x, ddy = pk.y - pi.y, ddz = pk.z - pi.z;
-          double r2 = image(cell, ddx, ddy, ddz);
-          if (r2 > cut2) continue;
-          int code = relation(moff, matom, mrel, i, perm[t]) + (group[t] == gi ? 8 : 0);
-          if (cnt < maxnb) nbr[(size_t)s * maxnb + cnt] = (uint32_t)t | ((uint32_t)code << NB_CODE_SHIFT);
-          cnt++;
+        for (int base = b; base < e; base += LPA) {
+          int t = base + lane;
+          bool hit = false;
+          if (t < e && t != s) {
+            double4 pk = pos[t];
+            double ddx = pk.x - pi.x, ddy = pk.y - pi.y, ddz = pk.z - pi.z;
+            hit = image(cell, ddx, ddy, ddz) <= cut2;
+          }
+          unsigned long long m = (__ballot(hit) >> gbase) & ((LPA == 64) ? ~0ull : ((1ull << LPA) - 1ull));
+          if (hit) {
+            int slot = cnt + __popcll(m & below);
+            int code = relation(moff, matom, mrel, i, perm[t]) + (group[t] == gi ? 8 : 0);
+            if (slot < maxnb) row[slot] = (uint32_t)t | ((uint32_t)code << NB_CODE_SHIFT);
+          }
+          cnt += __popcll(m);
         }
       }
-  nnb[s] = cnt;
-  if (cnt > maxnb) atomicMax(&flags[0], cnt);
+  if (lane == 0) {
+    nnb[s] = cnt;
+    if (cnt > maxnb) atomicMax(&flags[0], cnt);
+  }
 }
 
 }  // namespace
@@ -226,8 +246,9 @@ void nb_build_lists(NbHandle *h) {
                      h->d_cellcount);
   NB_LAUNCH(k_scan, dim3(1), dim3(1024), 0, m, h->d_cellcount, h->d_cellstart);
   HIPCHECK(hipMemsetAsync(h->d_cellcount, 0, sizeof(int32_t) * (m + 1), h->stream));
-  NB_LAUNCH(k_cell_scatter, G, B, 0, n, h->d_cellid, h->d_cellstart, h->d_cellcount, h->d_perm);
-  NB_LAUNCH(k_cell_sortseg, dim3((m + 255) / 256), B, 0, m, h->d_cellstart, h->d_perm);
+  // unsorted cell segments go to iperm (rebuilt just after), then are ranked into perm
+  NB_LAUNCH(k_cell_scatter, G, B, 0, n, h->d_cellid, h->d_cellstart, h->d_cellcount, h->d_iperm);
+  NB_LAUNCH(k_cell_rank, dim3(m), B, 0, m, h->d_cellstart, h->d_iperm, h->d_perm);
   NB_LAUNCH(k_iperm, G, B, 0, n, h->d_perm, h->d_iperm);
   NB_LAUNCH(k_gather_static, G, B, 0, n, h->d_perm, h->d_polar0, h->d_thole0, h->d_ipd0,
                      h->d_group0, h->d_vclass0, h->d_polar, h->d_group, h->d_vclass, h->d_mp);
@@ -238,7 +259,7 @@ void nb_build_lists(NbHandle *h) {
     if (!h->d_nbr) HIPCHECK(hipMalloc(&h->d_nbr, sizeof(uint32_t) * (size_t)h->maxnb * n));
     int32_t zeros[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(h->d_flags, zeros, sizeof(zeros), hipMemcpyHostToDevice, h->stream));
-    NB_LAUNCH(k_build_list, G, B, 0, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell, nc[0],
+    NB_LAUNCH(k_build_list<16>, grid_for((long)n * 16, 256), B, 0, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell, nc[0],
                        nc[1], nc[2], cut * cut, h->d_group, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_nbr,
                        h->maxnb, h->d_nnb, h->d_flags);
     int32_t f0 = 0;

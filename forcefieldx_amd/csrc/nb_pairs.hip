@@ -101,7 +101,7 @@ __device__ __forceinline__ void perm_field_pair(const double4 &pi, const AtomMP 
   double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
   double r2 = image(cell, dx, dy, dz);
   MP<U> K = load_mp<U>(ak);
-  U r = (U)sqrt(r2), rinv = (U)1 / r;
+  U r = sqrt((U)r2), rinv = (U)1 / r;
   V3<U> R{(U)-dx, (U)-dy, (U)-dz};  // from k to i
   U bn[4];
   ewald_bn<U, 3>(r, rinv, (U)aewald, bn);
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const double4 *__restr
       double4 pk = pos[k];
       double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
       double r2 = image(cell, dx, dy, dz);
-      T r = (T)sqrt(r2), rinv = (T)1 / r;
+      T r = sqrt((T)r2), rinv = (T)1 / r;
       V3<T> R{(T)dx, (T)dy, (T)dz};
       T bn[3];
       ewald_bn<T, 2>(r, rinv, (T)aewald, bn);
@@ -215,7 +215,7 @@ __device__ __forceinline__ void real_energy_pair(const double4 &pi, const AtomMP
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
   V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = (U)sqrt(r2), rinv = (U)1 / rl;
+  U rl = sqrt((U)r2), rinv = (U)1 / rl;
   U bn[7], rr[7];
   ewald_bn<U, 6>(rl, rinv, (U)aewald, bn);
   coulomb_rr<U, 6>(rinv, rr);
@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
       T irv = (T)pe[0];
       if (!(irv > (T)0)) continue;
       cnt++;
-      T rr = (T)sqrt(r2);
+      T rr = sqrt((T)r2);
       T ev = (T)(vs * pe[1]);
       T rho = rr * irv;
       T rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;

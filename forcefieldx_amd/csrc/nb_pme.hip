@@ -4,17 +4,28 @@
 //    MultipoleType.java:385-540; rotateMultipole :1031-1058), written into the
 //    sorted AtomMP records together with the reduced vdW sites.
 //  * k_bspline_frac: order-5 B-splines with 3 derivatives and fractional
-//    multipoles (ReciprocalSpace.java:1344-1397, 2778-2834).
-//  * k_spread_*: 125-point scatter with FP64 atomics into the interleaved
-//    complex grid (real part: u / permanent, imaginary part: uCR), which is
-//    the SpatialPermanentLoop / SpatialInducedLoop density
-//    (ReciprocalSpace.java:1445-1643) without octant colouring.
-//  * k_fft_*: in-LDS Stockham FFT along x, y and a fused
-//    forward-z / influence multiply / inverse-z kernel (Complex3DParallel.
-//    convolution, Complex3DParallel.java:445-453, 667-798); the inverse is
-//    un-normalised as in the reference.
-//  * k_gather_*: 20 fractional potential derivatives and the Cartesian
+//    multipoles (ReciprocalSpace.java:1344-1397, 2778-2834), plus the binning
+//    of atoms into grid bricks.
+//  * k_spread_*: the SpatialPermanentLoop / SpatialInducedLoop density
+//    (ReciprocalSpace.java:1445-1643).  Instead of the reference's 8-colour
+//    octant decomposition, one workgroup owns one brick of the grid: it
+//    accumulates the 125-point stencils of its atoms in LDS (LDS atomics) and
+//    writes the brick, including its 4-point spill into the next bricks, to a
+//    brick buffer; k_brick_combine then sums, for every grid point, its own
+//    brick and the spill of up to 7 neighbours into the interleaved complex
+//    grid (real part: permanent / u, imaginary part: uCR), inside the load of
+//    the forward x transform.  No global atomics.
+//  * k_ct_* / k_fft_lines: in-place in-LDS Stockham FFT (radix 2/3/4/5) along
+//    x (fused with the brick combine on the forward pass) and y, and a fused
+//    forward-z / influence multiply / inverse-z kernel
+//    (Complex3DParallel.convolution, Complex3DParallel.java:445-453, 667-798);
+//    the inverse is un-normalised as in the reference.
+//  * k_gather: 20 fractional potential derivatives and the Cartesian
 //    transform (ReciprocalSpace.java:2264-2455, 2467-2776, 1163-1205).
+//
+// Every grid kernel is templated on the grid real T: double in the FP64 form,
+// float in the mixed form (spread, FFT, convolution and gather arithmetic in
+// FP32; the potentials handed back are FP64).
 #include <cmath>
 #include <stdexcept>
 #include <vector>
@@ -24,6 +35,8 @@
 namespace {
 
 struct Frac { double a[3][3]; };  // a[f][c] = K_f A[c][f]
+
+template <typename T> struct alignas(2 * sizeof(T)) CX { T x, y; };
 
 // ---------------------------------------------------------------------------
 // local frames
@@ -105,28 +118,41 @@ __global__ void k_rotate(int n, const double *__restrict__ xyz, const int32_t *_
 }
 
 // ---------------------------------------------------------------------------
-// B-splines and fractional multipoles
+// bricks: axis d is cut into nb bricks, brick b = [b K / nb, (b+1) K / nb)
 // ---------------------------------------------------------------------------
+struct Bricks { int K[3], nb[3], ext; };
+
+__host__ __device__ __forceinline__ int brick_of(int p, int K, int nb) { return ((p + 1) * nb - 1) / K; }
+__host__ __device__ __forceinline__ int brick_start(int b, int K, int nb) { return (b * K) / nb; }
+
+__device__ __forceinline__ int wrapg(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
+
+// B-splines, grid start, fractional multipoles and brick binning
 __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const double *__restrict__ gmp, CellD cell,
-                               int K0, int K1, int K2, Frac fr, double *__restrict__ theta,
-                               int32_t *__restrict__ igrid, double *__restrict__ fmp) {
+                               Bricks bk, Frac fr, double *__restrict__ theta, int32_t *__restrict__ igrid,
+                               double *__restrict__ fmp, int32_t *__restrict__ bcount, int32_t *__restrict__ bid,
+                               int32_t *__restrict__ brank) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   double4 p = pos[s];
-  int K[3] = {K0, K1, K2};
+  int b3[3];
   for (int d = 0; d < 3; d++) {
     double w = p.x * cell.A[0][d] + p.y * cell.A[1][d] + p.z * cell.A[2][d];
     double u = w - floor(w + 0.5) + 0.5;
-    double frx = K[d] * u;
+    double frx = bk.K[d] * u;
     int ifr = (int)frx;
     double bx = frx - ifr;
     igrid[3 * s + d] = ifr - 5;
+    b3[d] = brick_of(wrapg(ifr - 4, bk.K[d]), bk.K[d], bk.nb[d]);
     double th[5][4];
     bspline4<5>(bx, th);
     double *o = theta + ((size_t)s * 3 + d) * 20;
     for (int k = 0; k < 5; k++)
       for (int j = 0; j < 4; j++) o[4 * k + j] = th[k][j];
   }
+  int b = b3[0] + bk.nb[0] * (b3[1] + bk.nb[1] * b3[2]);
+  bid[s] = b;
+  brank[s] = atomicAdd(&bcount[b], 1);
   const double *g = gmp + 10 * s;
   double *f = fmp + 10 * s;
   f[0] = g[0];
@@ -141,59 +167,177 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
   f[7] = 2.0 * F[0][1] * t; f[8] = 2.0 * F[0][2] * t; f[9] = 2.0 * F[1][2] * t;
 }
 
-__device__ __forceinline__ int wrapg(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
-
-// One thread per (atom, grid point [, real/imaginary part]): consecutive
-// lanes hit consecutive x points (and, for the induced grid, the two
-// interleaved parts of a point), so every atomic instruction of a wavefront
-// touches a handful of cache lines instead of one line per lane.
-__global__ void __launch_bounds__(256) k_spread_perm(int n, const double *__restrict__ theta,
-                                                     const int32_t *__restrict__ igrid, const double *__restrict__ fmp,
-                                                     int K0, int K1, int K2, double *__restrict__ grid) {
-  long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int s = (int)(gid / 125), r = (int)(gid - (long)s * 125);
-  if (s >= n) return;
-  int kz = r / 25, ky = (r / 5) % 5, kx = r % 5;
-  const double *tx = theta + (size_t)s * 60, *u = tx + 20 + 4 * ky, *v = tx + 40 + 4 * kz, *w = tx + 4 * kx;
-  const double *m = fmp + 10 * (size_t)s;
-  int k = wrapg(igrid[3 * s + 2] + 1 + kz, K2), j = wrapg(igrid[3 * s + 1] + 1 + ky, K1);
-  int i = wrapg(igrid[3 * s] + 1 + kx, K0);
-  double t0 = (m[0] * v[0] + m[3] * v[1] + m[6] * v[2]) * u[0] + (m[2] * v[0] + m[9] * v[1]) * u[1] + m[5] * v[0] * u[2];
-  double t1 = (m[1] * v[0] + m[8] * v[1]) * u[0] + m[7] * v[0] * u[1];
-  double t2 = m[4] * v[0] * u[0];
-  atomicAdd(&grid[2 * ((size_t)K0 * (j + (size_t)K1 * k) + i)], w[0] * t0 + w[1] * t1 + w[2] * t2);
+__global__ void k_zero_i(int m, int32_t *__restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) x[i] = 0;
 }
 
-// induced dipoles: u in the real part, uCR in the imaginary part
-__global__ void __launch_bounds__(256) k_spread_ind(int n, const double *__restrict__ theta,
-                                                    const int32_t *__restrict__ igrid, const double *__restrict__ u3,
-                                                    const double *__restrict__ up3, Frac fr, int K0, int K1, int K2,
-                                                    double *__restrict__ grid) {
-  long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int s = (int)(gid / 250), r = (int)(gid - (long)s * 250);
-  if (s >= n) return;
-  int part = r & 1;
-  r >>= 1;
-  int kz = r / 25, ky = (r / 5) % 5, kx = r % 5;
-  const double *tx = theta + (size_t)s * 60, *u = tx + 20 + 4 * ky, *v = tx + 40 + 4 * kz, *w = tx + 4 * kx;
-  const double *src = part ? up3 : u3;
-  double d[3];
-  for (int c = 0; c < 3; c++)
-    d[c] = fr.a[c][0] * src[3 * s] + fr.a[c][1] * src[3 * s + 1] + fr.a[c][2] * src[3 * s + 2];
-  int k = wrapg(igrid[3 * s + 2] + 1 + kz, K2), j = wrapg(igrid[3 * s + 1] + 1 + ky, K1);
-  int i = wrapg(igrid[3 * s] + 1 + kx, K0);
-  double t0 = d[2] * v[1] * u[0] + d[1] * v[0] * u[1], t1 = d[0] * v[0] * u[0];
-  atomicAdd(&grid[2 * ((size_t)K0 * (j + (size_t)K1 * k) + i) + part], w[0] * t0 + w[1] * t1);
+// exclusive scan of the brick counts (one workgroup of 1024)
+__global__ void __launch_bounds__(1024) k_brick_scan(int nb, const int32_t *__restrict__ bcount, int32_t *__restrict__ bstart) {
+  __shared__ int32_t part[1024];
+  int t = threadIdx.x;
+  int per = (nb + 1023) / 1024;
+  int lo = t * per, hi = min(nb, lo + per);
+  int sum = 0;
+  for (int b = lo; b < hi; b++) sum += bcount[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - sum;
+  for (int b = lo; b < hi; b++) {
+    bstart[b] = run;
+    run += bcount[b];
+  }
+  if (t == 1023) bstart[nb] = part[1023];
 }
 
-__device__ __forceinline__ void phi_cart(const Frac &fr, const double *f, double *c) {
+__global__ void k_brick_fill(int n, const int32_t *__restrict__ bid, const int32_t *__restrict__ brank,
+                             const int32_t *__restrict__ bstart, int32_t *__restrict__ batom) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int q = bstart[bid[s]] + brank[s];
+  if (q < n) batom[q] = s;
+}
+
+// local offset of an atom's first spline point inside its brick, per axis
+__device__ __forceinline__ void brick_local(const Bricks &bk, int b, const int32_t *ig, int o[3]) {
+  int bx = b % bk.nb[0], r = b / bk.nb[0];
+  int by = r % bk.nb[1], bz = r / bk.nb[1];
+  int bb[3] = {bx, by, bz};
+#pragma unroll
+  for (int d = 0; d < 3; d++) o[d] = wrapg(ig[d] + 1, bk.K[d]) - brick_start(bb[d], bk.K[d], bk.nb[d]);
+}
+
+// One workgroup per brick.  Thread (atom a, ky, kz) forms the y-z weight and
+// adds 5 consecutive x points into the LDS brick.  The stencil values are
+// computed in T; the LDS accumulation is FP64 in both forms (FP64 LDS atomics
+// are also the faster ones on gfx950) and the brick is stored as T.
+template <typename T, bool IND>
+__global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const int32_t *__restrict__ bstart,
+                                                      const int32_t *__restrict__ batom,
+                                                      const double *__restrict__ theta,
+                                                      const int32_t *__restrict__ igrid,
+                                                      const double *__restrict__ fmp, const double *__restrict__ u3,
+                                                      const double *__restrict__ up3, Frac fr, T *__restrict__ bbuf) {
+  constexpr int NP = IND ? 2 : 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double *lds = (double *)smem_raw;
+  const int e = bk.ext, e3 = e * e * e;
+  const int b = blockIdx.x;
+  for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) lds[t] = 0.0;
+  __syncthreads();
+  const int a0 = bstart[b], na = min(bstart[b + 1], n) - a0;
+  for (int q = threadIdx.x; q < na * 25; q += blockDim.x) {
+    int a = q / 25, yz = q - a * 25;
+    int ky = yz % 5, kz = yz / 5;
+    int s = batom[a0 + a];
+    int o[3];
+    brick_local(bk, b, igrid + 3 * s, o);
+    const double *tx = theta + (size_t)s * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
+    int base = ((o[2] + kz) * e + (o[1] + ky)) * e + o[0];
+    // every global load is issued before the first LDS atomic: the compiler
+    // does not move loads across atomics, so interleaving them serialises
+    // five load latencies per thread
+    double wx[5][3];
+#pragma unroll
+    for (int kx = 0; kx < 5; kx++) {
+      wx[kx][0] = tx[4 * kx];
+      wx[kx][1] = tx[4 * kx + 1];
+      wx[kx][2] = IND ? 0.0 : tx[4 * kx + 2];
+    }
+    T val[NP][5];
+    if (!IND) {
+      const double *m = fmp + 10 * (size_t)s;
+      T u0 = (T)uu[0], u1 = (T)uu[1], u2 = (T)uu[2], v0 = (T)vv[0], v1 = (T)vv[1], v2 = (T)vv[2];
+      T t0 = ((T)m[0] * v0 + (T)m[3] * v1 + (T)m[6] * v2) * u0 + ((T)m[2] * v0 + (T)m[9] * v1) * u1 + (T)m[5] * v0 * u2;
+      T t1 = ((T)m[1] * v0 + (T)m[8] * v1) * u0 + (T)m[7] * v0 * u1;
+      T t2 = (T)m[4] * v0 * u0;
+#pragma unroll
+      for (int kx = 0; kx < 5; kx++) val[0][kx] = (T)wx[kx][0] * t0 + (T)wx[kx][1] * t1 + (T)wx[kx][2] * t2;
+    } else {
+      T u0 = (T)uu[0], u1 = (T)uu[1], v0 = (T)vv[0], v1 = (T)vv[1];
+#pragma unroll
+      for (int p = 0; p < NP; p++) {
+        const double *src = (p ? up3 : u3) + 3 * (size_t)s;
+        T d0 = (T)(fr.a[0][0] * src[0] + fr.a[0][1] * src[1] + fr.a[0][2] * src[2]);
+        T d1 = (T)(fr.a[1][0] * src[0] + fr.a[1][1] * src[1] + fr.a[1][2] * src[2]);
+        T d2 = (T)(fr.a[2][0] * src[0] + fr.a[2][1] * src[1] + fr.a[2][2] * src[2]);
+        T t0 = d2 * v1 * u0 + d1 * v0 * u1, t1 = d0 * v0 * u0;
+#pragma unroll
+        for (int kx = 0; kx < 5; kx++) val[p][kx] = (T)wx[kx][0] * t0 + (T)wx[kx][1] * t1;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int kx = 0; kx < 5; kx++) atomicAdd(&lds[p * e3 + base + kx], (double)val[p][kx]);
+  }
+  __syncthreads();
+  T *out = bbuf + (size_t)b * NP * e3;
+  for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) out[t] = (T)lds[t];
+}
+
+// grid point (x, y, z) = own brick at local offset o plus, along each axis
+// where o < 4, the spill of the previous brick at local offset len_prev + o
+template <typename T, int NP>
+__device__ __forceinline__ CX<T> brick_point(const Bricks &bk, const T *__restrict__ bbuf, int cx, int cy, int cz) {
+  int c[3] = {cx, cy, cz};
+  int bo[3], lo[3], bp[3], lp[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    int K = bk.K[d], nb = bk.nb[d];
+    bo[d] = brick_of(c[d], K, nb);
+    int st = brick_start(bo[d], K, nb);
+    lo[d] = c[d] - st;
+    bp[d] = bo[d] == 0 ? nb - 1 : bo[d] - 1;
+    int lenp = (bo[d] == 0 ? K : st) - brick_start(bp[d], K, nb);
+    lp[d] = lenp + lo[d];
+  }
+  const int e = bk.ext, e3 = e * e * e;
+  T acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = (T)0;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
+    if ((dx && lo[0] >= 4) || (dy && lo[1] >= 4) || (dz && lo[2] >= 4)) continue;
+    int bx = dx ? bp[0] : bo[0], by = dy ? bp[1] : bo[1], bz = dz ? bp[2] : bo[2];
+    int x = dx ? lp[0] : lo[0], y = dy ? lp[1] : lo[1], z = dz ? lp[2] : lo[2];
+    size_t bi = (size_t)bx + (size_t)bk.nb[0] * (by + (size_t)bk.nb[1] * bz);
+    const T *src = bbuf + bi * NP * e3 + (z * e + y) * e + x;
+#pragma unroll
+    for (int p = 0; p < NP; p++) acc[p] += src[(size_t)p * e3];
+  }
+  CX<T> v;
+  v.x = acc[0];
+  v.y = NP == 2 ? acc[NP - 1] : (T)0;
+  return v;
+}
+
+template <typename T, int NP>
+__global__ void __launch_bounds__(256) k_brick_combine(Bricks bk, const T *__restrict__ bbuf, CX<T> *__restrict__ grid) {
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ntot = (long)bk.K[0] * bk.K[1] * bk.K[2];
+  if (g >= ntot) return;
+  int cx = (int)(g % bk.K[0]);
+  long r = g / bk.K[0];
+  grid[g] = brick_point<T, NP>(bk, bbuf, cx, (int)(r % bk.K[1]), (int)(r / bk.K[1]));
+}
+
+template <typename T>
+__device__ __forceinline__ void phi_cart(const Frac &fr, const T *f, double *c) {
   c[0] = f[0];
   for (int j = 0; j < 3; j++) c[1 + j] = fr.a[0][j] * f[1] + fr.a[1][j] * f[2] + fr.a[2][j] * f[3];
-  double F[3][3] = {{f[4], f[7], f[8]}, {f[7], f[5], f[9]}, {f[8], f[9], f[6]}}, T[3][3], C[3][3];
+  double F[3][3] = {{f[4], f[7], f[8]}, {f[7], f[5], f[9]}, {f[8], f[9], f[6]}}, T3[3][3], C[3][3];
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) T[a][b] = fr.a[0][a] * F[0][b] + fr.a[1][a] * F[1][b] + fr.a[2][a] * F[2][b];
+    for (int b = 0; b < 3; b++) T3[a][b] = fr.a[0][a] * F[0][b] + fr.a[1][a] * F[1][b] + fr.a[2][a] * F[2][b];
   for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) C[a][b] = T[a][0] * fr.a[0][b] + T[a][1] * fr.a[1][b] + T[a][2] * fr.a[2][b];
+    for (int b = 0; b < 3; b++) C[a][b] = T3[a][0] * fr.a[0][b] + T3[a][1] * fr.a[1][b] + T3[a][2] * fr.a[2][b];
   c[4] = C[0][0]; c[5] = C[1][1]; c[6] = C[2][2]; c[7] = C[0][1]; c[8] = C[0][2]; c[9] = C[1][2];
 }
 
@@ -203,48 +347,63 @@ __device__ __forceinline__ void phi_cart(const Frac &fr, const double *f, double
 // transform (out[30 s] = 20 fractional + 10 Cartesian).  Otherwise only the
 // first derivatives are formed and the Cartesian field components are
 // written to out[30 s + 21..23] (what the PCG iterations consume).
-template <int NP, bool FULL>
+template <typename T>
+__device__ __forceinline__ T group8(T v) {
+#pragma unroll
+  for (int o = 4; o > 0; o >>= 1) v += __shfl_xor(v, o, 8);
+  return v;
+}
+
+template <typename T, int NP, bool FULL>
 __global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
-                                                const double *__restrict__ grid, Frac fr, int K0, int K1, int K2,
+                                                const CX<T> *__restrict__ grid, Frac fr, int K0, int K1, int K2,
                                                 double *__restrict__ out0, double *__restrict__ out1) {
   constexpr int NC = FULL ? 20 : 4;
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
   bool valid = s < n;
-  double phi[NP][NC];
+  T phi[NP][NC];
 #pragma unroll
   for (int p = 0; p < NP; p++)
 #pragma unroll
-    for (int c = 0; c < NC; c++) phi[p][c] = 0.0;
+    for (int c = 0; c < NC; c++) phi[p][c] = (T)0;
   if (valid && lane < 5) {
-    const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *v = tx + 40 + 4 * lane;
+    const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *vd = tx + 40 + 4 * lane;
+    T v[4] = {(T)vd[0], (T)vd[1], (T)vd[2], (T)vd[3]};
+    T wx[5][4];
+#pragma unroll
+    for (int kx = 0; kx < 5; kx++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) wx[kx][q] = (T)tx[4 * kx + q];
     int k = wrapg(igrid[3 * s + 2] + 1 + lane, K2);
     int j0 = igrid[3 * s + 1] + 1, i0 = igrid[3 * s] + 1;
     int ix[5];
 #pragma unroll
     for (int kx = 0; kx < 5; kx++) ix[kx] = wrapg(i0 + kx, K0);
-    double tu[NP][10];
+    T tu[NP][10];
 #pragma unroll
     for (int p = 0; p < NP; p++)
 #pragma unroll
-      for (int c = 0; c < 10; c++) tu[p][c] = 0.0;
+      for (int c = 0; c < 10; c++) tu[p][c] = (T)0;
     for (int ky = 0; ky < 5; ky++) {
       int j = wrapg(j0 + ky, K1);
       size_t row = (size_t)K0 * (j + (size_t)K1 * k);
-      double t[NP][4];
+      T t[NP][4];
 #pragma unroll
-      for (int p = 0; p < NP; p++) t[p][0] = t[p][1] = t[p][2] = t[p][3] = 0.0;
+      for (int p = 0; p < NP; p++) t[p][0] = t[p][1] = t[p][2] = t[p][3] = (T)0;
 #pragma unroll
       for (int kx = 0; kx < 5; kx++) {
-        const double *w = tx + 4 * kx;
+        CX<T> q2 = grid[row + ix[kx]];
+        T qv[2] = {q2.x, q2.y};
 #pragma unroll
         for (int p = 0; p < NP; p++) {
-          double q = grid[2 * (row + ix[kx]) + p];
-          t[p][0] += q * w[0]; t[p][1] += q * w[1];
-          if (FULL) { t[p][2] += q * w[2]; t[p][3] += q * w[3]; }
+          T q = qv[p];
+          t[p][0] += q * wx[kx][0]; t[p][1] += q * wx[kx][1];
+          if (FULL) { t[p][2] += q * wx[kx][2]; t[p][3] += q * wx[kx][3]; }
         }
       }
-      const double *uu = ty + 4 * ky;
+      const double *ud = ty + 4 * ky;
+      T uu[4] = {(T)ud[0], (T)ud[1], (T)ud[2], (T)ud[3]};
 #pragma unroll
       for (int p = 0; p < NP; p++) {
         tu[p][0] += t[p][0] * uu[0]; tu[p][1] += t[p][1] * uu[0]; tu[p][2] += t[p][0] * uu[1];
@@ -257,8 +416,8 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict_
     }
 #pragma unroll
     for (int p = 0; p < NP; p++) {
-      double *ph = phi[p];
-      const double *a = tu[p];
+      T *ph = phi[p];
+      const T *a = tu[p];
       if (FULL) {
         ph[0] = a[0] * v[0]; ph[1] = a[1] * v[0]; ph[2] = a[2] * v[0]; ph[3] = a[0] * v[1];
         ph[4] = a[3] * v[0]; ph[5] = a[5] * v[0]; ph[6] = a[0] * v[2];
@@ -274,152 +433,192 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict_
 #pragma unroll
   for (int p = 0; p < NP; p++)
 #pragma unroll
-    for (int c = 0; c < NC; c++) phi[p][c] = group_sum<8>(phi[p][c]);
+    for (int c = 0; c < NC; c++) phi[p][c] = group8(phi[p][c]);
   if (valid && lane == 0) {
     double *outs[2] = {out0, out1};
 #pragma unroll
     for (int p = 0; p < NP; p++) {
       double *o = outs[p] + 30 * (size_t)s;
       if (FULL) {
-        for (int c = 0; c < 20; c++) o[c] = phi[p][c];
-        phi_cart(fr, phi[p], o + 20);
+        for (int c = 0; c < 20; c++) o[c] = (double)phi[p][c];
+        phi_cart<T>(fr, phi[p], o + 20);
       } else {
         for (int c = 0; c < 3; c++)
-          o[21 + c] = fr.a[0][c] * phi[p][1] + fr.a[1][c] * phi[p][2] + fr.a[2][c] * phi[p][3];
+          o[21 + c] = fr.a[0][c] * (double)phi[p][1] + fr.a[1][c] * (double)phi[p][2] + fr.a[2][c] * (double)phi[p][3];
       }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// FFT: in-LDS Stockham autosort, radix 2/3/4/5.  One workgroup transforms
-// L lines of length N held in LDS (ping-pong).  Lines along y and z are
-// grouped by consecutive x so global loads and stores are coalesced.
+// FFT: in-place Stockham autosort in LDS, radix 2/3/4/5.  A pass reads every
+// butterfly input of the workgroup into registers, synchronises, and writes
+// the outputs back to the same buffer, so one LDS buffer suffices.  Lines
+// live at buf[l * sl + e * se]; LINE_FAST puts consecutive threads on
+// consecutive lines (used when lines are the contiguous direction).
 // ---------------------------------------------------------------------------
 struct Radices { int r[12]; int nr; };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename T>
+__device__ __forceinline__ CX<T> cmul(CX<T> a, CX<T> b) {
+  return CX<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
+template <typename T> __device__ __forceinline__ CX<T> cadd(CX<T> a, CX<T> b) { return CX<T>{a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ CX<T> csub(CX<T> a, CX<T> b) { return CX<T>{a.x - b.x, a.y - b.y}; }
+// i * s * a for s = +-1
+template <typename T> __device__ __forceinline__ CX<T> cis(CX<T> a, T s) { return CX<T>{-s * a.y, s * a.x}; }
 
-// y[m] = sum_k x[k] exp(sign 2 pi i m k / R) for R in {2,3,4,5}
-template <int R>
-__device__ __forceinline__ void small_dft(double2 *v, double sign) {
-  double2 o[R];
-#pragma unroll
-  for (int m = 0; m < R; m++) {
-    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-      int q = (m * k) % R;
-      double c, s;
-      if (q == 0) { c = 1.0; s = 0.0; }
-      else { sincospi(2.0 * q / R, &s, &c); s *= sign; }
-      acc.x += v[k].x * c - v[k].y * s;
-      acc.y += v[k].x * s + v[k].y * c;
-    }
-    o[m] = acc;
+// y[m] = sum_k x[k] exp(s 2 pi i m k / R), s = +-1
+template <int R, typename T>
+__device__ __forceinline__ void small_dft(CX<T> *v, T s) {
+  if (R == 2) {
+    CX<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if (R == 3) {
+    const T c = (T)-0.5, sn = (T)0.86602540378443864676;  // cos, sin of 2 pi / 3
+    CX<T> t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    CX<T> m{v[0].x + c * t.x, v[0].y + c * t.y};
+    CX<T> r = cis(CX<T>{sn * d.x, sn * d.y}, s);
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  } else if (R == 4) {
+    CX<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]), c = cadd(v[1], v[3]), d = cis(csub(v[1], v[3]), s);
+    v[0] = cadd(a, c);
+    v[1] = cadd(b, d);
+    v[2] = csub(a, c);
+    v[3] = csub(b, d);
+  } else {
+    // cos and sin of 2 pi / 5 and 4 pi / 5
+    const T c1 = (T)0.30901699437494742410, s1 = (T)0.95105651629515357212;
+    const T c2 = (T)-0.80901699437494742410, s2 = (T)0.58778525229247312917;
+    CX<T> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]), t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    CX<T> m1{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+    CX<T> m2{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    CX<T> r1 = cis(CX<T>{s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y}, s);
+    CX<T> r2 = cis(CX<T>{s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y}, s);
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(m1, r1);
+    v[4] = csub(m1, r1);
+    v[2] = cadd(m2, r2);
+    v[3] = csub(m2, r2);
   }
-#pragma unroll
-  for (int m = 0; m < R; m++) v[m] = o[m];
 }
 
-// one Stockham pass over all L lines in LDS
-template <int R>
-__device__ __forceinline__ void stockham_pass(const double2 *__restrict__ x, double2 *__restrict__ y, int N, int L,
-                                              int Ns, const double2 *__restrict__ tw, double sign) {
-  int nb = N / R;
-  int stride = N / (Ns * R);
-  for (int b = threadIdx.x; b < L * nb; b += blockDim.x) {
-    int line = b / nb, j = b - line * nb;
-    const double2 *xl = x + line * N;
-    double2 *yl = y + line * N;
-    int jm = j % Ns;
-    double2 v[R];
+// one in-place pass; EPT bounds the elements per thread (N * nlines <= EPT * blockDim)
+template <int R, int EPT, bool LINE_FAST, typename T>
+__device__ __forceinline__ void lds_pass(CX<T> *buf, int N, int nlines, int sl, int se, int Ns, const CX<T> *tw, T s) {
+  constexpr int MAXB = (EPT + R - 1) / R;
+  const int nb = N / R, total = nlines * nb, stride = N / (Ns * R);
+  CX<T> v[MAXB][R];
+  int where[MAXB];
 #pragma unroll
-    for (int k = 0; k < R; k++) {
-      double2 a = xl[j + k * nb];
-      if (k > 0 && jm > 0) {
-        double2 w = tw[(jm * k * stride) % N];
-        w.y *= sign;
-        a = cmul(a, w);
+  for (int q = 0; q < MAXB; q++) {
+    int b = q * blockDim.x + threadIdx.x;
+    where[q] = -1;
+    if (b < total) {
+      int line, j;
+      if (LINE_FAST) { line = b % nlines; j = b / nlines; }
+      else { j = b % nb; line = b / nb; }
+      int jm = j % Ns;
+      const CX<T> *x = buf + line * sl;
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        CX<T> a = x[(j + k * nb) * se];
+        if (k > 0 && jm > 0) {
+          CX<T> w = tw[jm * k * stride];  // jm k stride < N
+          w.y *= s;
+          a = cmul(a, w);
+        }
+        v[q][k] = a;
       }
-      v[k] = a;
+      small_dft<R, T>(v[q], s);
+      where[q] = line * sl + ((j / Ns) * Ns * R + jm) * se;
     }
-    small_dft<R>(v, sign);
-    int base = (j / Ns) * Ns * R + jm;
-#pragma unroll
-    for (int k = 0; k < R; k++) yl[base + k * Ns] = v[k];
-  }
-}
-
-// in-LDS transform of L lines; returns the buffer holding the result
-__device__ double2 *lds_fft(double2 *a, double2 *b, int N, int L, const Radices &rad, const double2 *tw, double sign) {
-  int Ns = 1;
-  for (int p = 0; p < rad.nr; p++) {
-    __syncthreads();
-    int R = rad.r[p];
-    if (R == 4) stockham_pass<4>(a, b, N, L, Ns, tw, sign);
-    else if (R == 2) stockham_pass<2>(a, b, N, L, Ns, tw, sign);
-    else if (R == 3) stockham_pass<3>(a, b, N, L, Ns, tw, sign);
-    else stockham_pass<5>(a, b, N, L, Ns, tw, sign);
-    Ns *= R;
-    double2 *t = a; a = b; b = t;
   }
   __syncthreads();
-  return a;
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    if (where[q] >= 0) {
+#pragma unroll
+      for (int k = 0; k < R; k++) buf[where[q] + k * Ns * se] = v[q][k];
+    }
+  }
+  __syncthreads();
 }
 
-// axis transform.  Line `l` of the workgroup: AXIS 0 -> (y,z) rows, contiguous
-// elements; AXIS 1 -> (x,z) with x fastest across lines; AXIS 2 -> (x,y).
-// FUSE_CONV (AXIS 2 only): forward, multiply by influence, inverse.
-template <int AXIS, bool FUSE_CONV>
-__global__ void __launch_bounds__(256) k_fft_axis(double2 *__restrict__ grid, int K0, int K1, int K2, int L, Radices rad,
-                                                  const double2 *__restrict__ twg, double sign,
-                                                  const double *__restrict__ recip) {
-  extern __shared__ double2 lds[];
-  int N = AXIS == 0 ? K0 : (AXIS == 1 ? K1 : K2);
-  // tw[j] = exp(+2 pi i j / N); passes conjugate it for sign < 0
-  double2 *a = lds, *b = lds + (size_t)L * N, *tw = lds + 2 * (size_t)L * N;
+template <int EPT, bool LINE_FAST, typename T>
+__device__ void lds_fft(CX<T> *buf, int N, int nlines, int sl, int se, const Radices &rad, const CX<T> *tw, T s) {
+  int Ns = 1;
+  for (int p = 0; p < rad.nr; p++) {
+    int R = rad.r[p];
+    if (R == 4) lds_pass<4, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
+    else if (R == 2) lds_pass<2, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
+    else if (R == 3) lds_pass<3, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
+    else lds_pass<5, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
+    Ns *= R;
+  }
+}
+
+// lines along one axis, L lines per workgroup.  AXIS 0: contiguous rows;
+// AXIS 1: L consecutive x at fixed z; AXIS 2: L consecutive (x, y) line
+// indices, with the influence multiply and inverse transform fused (CONV).
+template <typename T, int EPT, int AXIS, bool CONV>
+__global__ void __launch_bounds__(256) k_fft_lines(CX<T> *__restrict__ grid, int K0, int K1, int K2, int L, Radices rad,
+                                                   const CX<T> *__restrict__ twg, T s, const T *__restrict__ recip) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int N = AXIS == 0 ? K0 : (AXIS == 1 ? K1 : K2);
+  CX<T> *buf = (CX<T> *)smem_raw, *tw = buf + (size_t)L * N;
   for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
-  // line bookkeeping
-  int nlines = AXIS == 0 ? K1 * K2 : (AXIS == 1 ? K0 * K2 : K0 * K1);
-  int first = blockIdx.x * L;
-  int cnt = min(L, nlines - first);
-  // element (line l, position e) -> grid index
+  // line groups: AXIS 0 -> L rows; AXIS 1 -> x block at fixed z; AXIS 2 -> L (x,y) indices
+  int cnt, first = 0, xb = 0, z = 0;
+  if (AXIS == 0) {
+    first = blockIdx.x * L;
+    cnt = min(L, K1 * K2 - first);
+  } else if (AXIS == 1) {
+    int nxb = (K0 + L - 1) / L;
+    xb = (blockIdx.x % nxb) * L;
+    z = blockIdx.x / nxb;
+    cnt = min(L, K0 - xb);
+  } else {
+    first = blockIdx.x * L;
+    cnt = min(L, K0 * K1 - first);
+  }
   auto gidx = [&](int l, int e) -> size_t {
-    int line = first + l;
-    if (AXIS == 0) return (size_t)line * K0 + e;
-    if (AXIS == 1) {
-      int x = line % K0, z = line / K0;
-      return (size_t)x + (size_t)K0 * (e + (size_t)K1 * z);
-    }
-    return (size_t)line + (size_t)K0 * K1 * e;
+    if (AXIS == 0) return (size_t)(first + l) * K0 + e;
+    if (AXIS == 1) return (size_t)(xb + l) + (size_t)K0 * (e + (size_t)K1 * z);
+    return (size_t)(first + l) + (size_t)K0 * K1 * e;
   };
+  // LDS layout: AXIS 0 lines contiguous (buf[l N + e]); otherwise lines fastest (buf[e L + l])
   for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
     int l, e;
     if (AXIS == 0) { l = t / N; e = t - l * N; }
     else { e = t / L; l = t - e * L; }
-    a[l * N + e] = (l < cnt) ? grid[gidx(l, e)] : make_double2(0.0, 0.0);
+    CX<T> v{(T)0, (T)0};
+    if (l < cnt) v = grid[gidx(l, e)];
+    buf[t] = v;
   }
-  double2 *res = lds_fft(a, b, N, L, rad, tw, sign);
-  if (FUSE_CONV) {
+  __syncthreads();
+  if (AXIS == 0) lds_fft<EPT, false, T>(buf, N, L, N, 1, rad, tw, s);
+  else lds_fft<EPT, true, T>(buf, N, L, 1, L, rad, tw, s);
+  if (CONV) {
     for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
       int e = t / L, l = t - e * L;
       if (l < cnt) {
-        double g = recip[(size_t)(first + l) + (size_t)K0 * K1 * e];
-        res[l * N + e].x *= g;
-        res[l * N + e].y *= g;
+        T g = recip[(size_t)(first + l) + (size_t)K0 * K1 * e];
+        buf[t].x *= g;
+        buf[t].y *= g;
       }
     }
-    double2 *other = (res == a) ? b : a;
-    res = lds_fft(res, other, N, L, rad, tw, -sign);
+    __syncthreads();
+    lds_fft<EPT, true, T>(buf, N, L, 1, L, rad, tw, -s);
   }
   for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
     int l, e;
     if (AXIS == 0) { l = t / N; e = t - l * N; }
     else { e = t / L; l = t - e * L; }
-    if (l < cnt) grid[gidx(l, e)] = res[l * N + e];
+    if (l < cnt) grid[gidx(l, e)] = buf[t];
   }
 }
 
@@ -435,12 +634,207 @@ Radices factor(int N) {
   return r;
 }
 
+constexpr int LINE_EPT = 8;    // elements per thread of the line kernels (256 threads)
+
 int lines_per_block(int N) {
-  int L = 512 / N;
-  if (L > 32) L = 32;
-  if (L < 1) L = 1;
+  int L = 256 * LINE_EPT / N;
+  if (L > 16) L = 16;
+  if (L < 1) throw std::runtime_error("PME grid dimension too large for the line FFT");
   return L;
 }
+
+
+// ---------------------------------------------------------------------------
+// Line FFTs with the length N, the lines per workgroup and the block size
+// known at compile time (every index division is by a constant and the radix
+// schedule is unrolled).  Used for the grid sizes listed in NB_CT_SIZES; other
+// sizes take the runtime-N kernels above.
+// ---------------------------------------------------------------------------
+constexpr int CT_BS = 256;
+
+template <int M> struct RadixOf {
+  static constexpr int v = (M % 4 == 0) ? 4 : (M % 2 == 0) ? 2 : (M % 3 == 0) ? 3 : 5;
+};
+template <int N> struct CtLines {
+  static constexpr int raw = 1024 / N;
+  static constexpr int x = raw < 1 ? 1 : (raw > 32 ? 32 : raw);
+  static constexpr int yz = raw < 1 ? 1 : (raw > 16 ? 16 : raw);
+};
+
+template <int R, int N, int Ns, int NL, int SL, int SE, bool LF, typename T>
+__device__ __forceinline__ void ct_pass(CX<T> *buf, const CX<T> *tw, T s) {
+  constexpr int nb = N / R, total = NL * nb, stride = N / (Ns * R);
+  constexpr int MAXB = (total + CT_BS - 1) / CT_BS;
+  CX<T> v[MAXB][R];
+  int where[MAXB];
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int b = q * CT_BS + threadIdx.x;
+    where[q] = -1;
+    if (b < total) {
+      const int line = LF ? b % NL : b / nb;
+      const int j = LF ? b / NL : b % nb;
+      const int jm = j % Ns;
+      const CX<T> *x = buf + line * SL;
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        CX<T> a = x[(j + k * nb) * SE];
+        if (Ns > 1 && k > 0) {  // tw[0] = 1 exactly, so jm = 0 needs no test
+          CX<T> w = tw[jm * k * stride];
+          w.y *= s;
+          a = cmul(a, w);
+        }
+        v[q][k] = a;
+      }
+      small_dft<R, T>(v[q], s);
+      where[q] = line * SL + ((j / Ns) * Ns * R + jm) * SE;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    if (where[q] >= 0) {
+#pragma unroll
+      for (int k = 0; k < R; k++) buf[where[q] + k * Ns * SE] = v[q][k];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int Ns, int NL, int SL, int SE, bool LF, typename T>
+__device__ __forceinline__ void ct_fft(CX<T> *buf, const CX<T> *tw, T s) {
+  if constexpr (Ns < N) {
+    constexpr int R = RadixOf<N / Ns>::v;
+    ct_pass<R, N, Ns, NL, SL, SE, LF, T>(buf, tw, s);
+    ct_fft<N, Ns * R, NL, SL, SE, LF, T>(buf, tw, s);
+  }
+}
+
+// x lines (contiguous), rows padded by one element in LDS.  MODE 0 reads the
+// grid; MODE 1 / 2 assembles each point from the brick buffer (1 or 2 parts),
+// which fuses the brick combine into the forward transform.
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(CT_BS) k_ct_x(CX<T> *__restrict__ grid, int K1, int K2,
+                                                const CX<T> *__restrict__ twg, T s, Bricks bk,
+                                                const T *__restrict__ bbuf) {
+  constexpr int NL = CtLines<N>::x, SL = N + 1;
+  __shared__ CX<T> buf[NL * SL];
+  __shared__ CX<T> tw[N];
+  constexpr int TOT = NL * N, PER = (TOT + CT_BS - 1) / CT_BS;
+  const int first = blockIdx.x * NL, nlines = K1 * K2;
+  for (int t = threadIdx.x; t < N; t += CT_BS) tw[t] = twg[t];
+  // all loads of the thread are issued before the first LDS store
+  T re[PER], im[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x, l = t / N, e = t - l * N, line = first + l;
+    re[q] = (T)0;
+    im[q] = (T)0;
+    if (t < TOT && line < nlines) {
+      CX<T> v;
+      if constexpr (MODE == 0) v = grid[(size_t)line * N + e];
+      else v = brick_point<T, MODE>(bk, bbuf, e, line % K1, line / K1);
+      re[q] = v.x;
+      im[q] = v.y;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x, l = t / N, e = t - l * N;
+    if (t < TOT) buf[l * SL + e] = CX<T>{re[q], im[q]};
+  }
+  __syncthreads();
+  ct_fft<N, 1, NL, SL, 1, false, T>(buf, tw, s);
+  for (int t = threadIdx.x; t < NL * N; t += CT_BS) {
+    int l = t / N, e = t - l * N, line = first + l;
+    if (line < nlines) grid[(size_t)line * N + e] = buf[l * SL + e];
+  }
+}
+
+// y lines: NL consecutive x at one z
+template <typename T, int N>
+__global__ void __launch_bounds__(CT_BS) k_ct_y(CX<T> *__restrict__ grid, int K0, const CX<T> *__restrict__ twg, T s) {
+  constexpr int NL = CtLines<N>::yz;
+  __shared__ CX<T> buf[NL * N];
+  __shared__ CX<T> tw[N];
+  const int nxb = (K0 + NL - 1) / NL, xb = (blockIdx.x % nxb) * NL, z = blockIdx.x / nxb;
+  const int cnt = min(NL, K0 - xb);
+  constexpr int TOT = NL * N, PER = (TOT + CT_BS - 1) / CT_BS;
+  for (int t = threadIdx.x; t < N; t += CT_BS) tw[t] = twg[t];
+  T re[PER], im[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x, e = t / NL, l = t - e * NL;
+    re[q] = (T)0;
+    im[q] = (T)0;
+    if (t < TOT && l < cnt) {
+      CX<T> v = grid[(size_t)(xb + l) + (size_t)K0 * (e + (size_t)N * z)];
+      re[q] = v.x;
+      im[q] = v.y;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x;
+    if (t < TOT) buf[t] = CX<T>{re[q], im[q]};
+  }
+  __syncthreads();
+  ct_fft<N, 1, NL, 1, NL, true, T>(buf, tw, s);
+  for (int t = threadIdx.x; t < NL * N; t += CT_BS) {
+    int e = t / NL, l = t - e * NL;
+    if (l < cnt) grid[(size_t)(xb + l) + (size_t)K0 * (e + (size_t)N * z)] = buf[t];
+  }
+}
+
+// z lines: forward, influence multiply, un-normalised inverse
+template <typename T, int N>
+__global__ void __launch_bounds__(CT_BS) k_ct_z(CX<T> *__restrict__ grid, int K01, const CX<T> *__restrict__ twg,
+                                                const T *__restrict__ recip) {
+  constexpr int NL = CtLines<N>::yz;
+  __shared__ CX<T> buf[NL * N];
+  __shared__ CX<T> tw[N];
+  const int first = blockIdx.x * NL, cnt = min(NL, K01 - first);
+  constexpr int TOT = NL * N, PER = (TOT + CT_BS - 1) / CT_BS;
+  for (int t = threadIdx.x; t < N; t += CT_BS) tw[t] = twg[t];
+  T re[PER], im[PER], gr[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x, e = t / NL, l = t - e * NL;
+    re[q] = (T)0;
+    im[q] = (T)0;
+    gr[q] = (T)0;
+    if (t < TOT && l < cnt) {
+      size_t gi = (size_t)(first + l) + (size_t)K01 * e;
+      CX<T> v = grid[gi];
+      re[q] = v.x;
+      im[q] = v.y;
+      gr[q] = recip[gi];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x;
+    if (t < TOT) buf[t] = CX<T>{re[q], im[q]};
+  }
+  __syncthreads();
+  ct_fft<N, 1, NL, 1, NL, true, T>(buf, tw, (T)1);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    int t = q * CT_BS + threadIdx.x;
+    if (t < TOT) {
+      buf[t].x *= gr[q];
+      buf[t].y *= gr[q];
+    }
+  }
+  __syncthreads();
+  ct_fft<N, 1, NL, 1, NL, true, T>(buf, tw, (T)-1);
+  for (int t = threadIdx.x; t < NL * N; t += CT_BS) {
+    int e = t / NL, l = t - e * NL;
+    if (l < cnt) grid[(size_t)(first + l) + (size_t)K01 * e] = buf[t];
+  }
+}
+
+#define NB_CT_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(48) X(60) X(64) X(72) X(80) X(96) X(120) X(128) X(144)
 
 }  // namespace
 
@@ -454,16 +848,21 @@ static Frac frac_of(const NbHandle *h) {
   return f;
 }
 
+static Bricks bricks_of(const NbHandle *h) {
+  Bricks b;
+  for (int d = 0; d < 3; d++) { b.K[d] = h->K[d]; b.nb[d] = h->nbk[d]; }
+  b.ext = h->bext;
+  return b;
+}
+
 // influence function of the reference (generalizedInfluenceFunction,
 // ReciprocalSpace.java:1045-1121; discreteFTMod :435-492), computed on the host
 static void bspline_modulus(std::vector<double> &bsmod, int n, int order) {
-  double th[5][4];
   // weights at x = 0 from the same recursion as the kernels
   double w[8][8] = {{0}};
   w[1][1] = 1.0;
   for (int p = 2; p <= order; p++)
     for (int k = 0; k < p; k++) w[p][k + 1] = ((0.0 + p - 1 - k) * w[p - 1][k] + (k + 1 - 0.0) * w[p - 1][k + 1]) / (p - 1);
-  (void)th;
   std::vector<double> arr(n > order + 1 ? n : order + 1, 0.0);
   for (int k = 0; k < order; k++) arr[1 + k] = w[order][k + 1];
   bsmod.assign(n, 0.0);
@@ -500,18 +899,57 @@ static void bspline_modulus(std::vector<double> &bsmod, int n, int order) {
   }
 }
 
+static void twiddles(const int K[3], std::vector<double> &tw) {
+  tw.assign(2 * (K[0] + K[1] + K[2]), 0.0);
+  int off = 0;
+  for (int d = 0; d < 3; d++) {
+    for (int j = 0; j < K[d]; j++) {
+      double a = 2.0 * M_PI * j / K[d];  // exp(+2 pi i j / K)
+      tw[2 * (off + j)] = cos(a);
+      tw[2 * (off + j) + 1] = sin(a);
+    }
+    off += K[d];
+  }
+}
+
+static void free_grid(NbHandle *h) {
+  void *ptrs[] = {h->d_grid, h->d_recip, h->d_recipf, h->d_tw, h->d_twf, h->d_bcount, h->d_bstart, h->d_bbuf};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  h->d_grid = nullptr; h->d_recip = nullptr; h->d_recipf = nullptr; h->d_tw = nullptr; h->d_twf = nullptr;
+  h->d_bcount = nullptr; h->d_bstart = nullptr; h->d_bbuf = nullptr;
+}
+
 void nb_pme_setup(NbHandle *h) {
   int K0 = h->prm.grid[0], K1 = h->prm.grid[1], K2 = h->prm.grid[2];
+  int K[3] = {K0, K1, K2};
+  for (int d = 0; d < 3; d++)
+    if (K[d] < 8) throw std::runtime_error("PME grid dimensions must be at least 8");
+  for (int d = 0; d < 3; d++) factor(K[d]);
   long ntot = (long)K0 * K1 * K2;
   if (ntot != h->ntot || !h->d_grid) {
-    if (h->d_grid) { hipFree(h->d_grid); hipFree(h->d_recip); hipFree(h->d_tw); }
+    free_grid(h);
+    int nbk[3], ext = 0;
+    for (int d = 0; d < 3; d++) {
+      nbk[d] = std::max(2, (K[d] + 7) / 8);
+      int mx = 0;
+      for (int b = 0; b < nbk[d]; b++) mx = std::max(mx, brick_start(b + 1, K[d], nbk[d]) - brick_start(b, K[d], nbk[d]));
+      ext = std::max(ext, mx + 4);
+      h->nbk[d] = nbk[d];
+    }
+    h->bext = ext;
+    h->nbricks = nbk[0] * nbk[1] * nbk[2];
     HIPCHECK(hipMalloc(&h->d_grid, sizeof(double) * 2 * ntot));
     HIPCHECK(hipMalloc(&h->d_recip, sizeof(double) * ntot));
+    HIPCHECK(hipMalloc(&h->d_recipf, sizeof(float) * ntot));
     HIPCHECK(hipMalloc(&h->d_tw, sizeof(double) * 2 * (K0 + K1 + K2)));
+    HIPCHECK(hipMalloc(&h->d_twf, sizeof(float) * 2 * (K0 + K1 + K2)));
+    HIPCHECK(hipMalloc(&h->d_bcount, sizeof(int32_t) * h->nbricks));
+    HIPCHECK(hipMalloc(&h->d_bstart, sizeof(int32_t) * (h->nbricks + 1)));
+    HIPCHECK(hipMalloc(&h->d_bbuf, sizeof(double) * 2 * (size_t)h->nbricks * ext * ext * ext));
   }
   h->K[0] = K0; h->K[1] = K1; h->K[2] = K2;
   h->ntot = ntot;
-  int K[3] = {K0, K1, K2};
   for (int f = 0; f < 3; f++)
     for (int c = 0; c < 3; c++) h->afrac[f][c] = K[f] * h->cell.A[c][f];
   std::vector<double> mx, my, mz;
@@ -537,67 +975,164 @@ void nb_pme_setup(NbHandle *h) {
     }
   }
   rec[0] = 0.0;
+  std::vector<float> recf(rec.begin(), rec.end());
   HIPCHECK(hipMemcpy(h->d_recip, rec.data(), sizeof(double) * ntot, hipMemcpyHostToDevice));
-  std::vector<double> tw(2 * (K0 + K1 + K2));
-  int off = 0;
-  for (int d = 0; d < 3; d++) {
-    for (int j = 0; j < K[d]; j++) {
-      double a = 2.0 * M_PI * j / K[d];  // exp(+2 pi i j / K)
-      tw[2 * (off + j)] = cos(a);
-      tw[2 * (off + j) + 1] = sin(a);
-    }
-    off += K[d];
-  }
+  HIPCHECK(hipMemcpy(h->d_recipf, recf.data(), sizeof(float) * ntot, hipMemcpyHostToDevice));
+  std::vector<double> tw;
+  twiddles(K, tw);
+  std::vector<float> twf(tw.begin(), tw.end());
   HIPCHECK(hipMemcpy(h->d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(h->d_twf, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
   h->recip_valid = true;
 }
 
-// forward x, forward y, fused z (forward * influence * inverse), inverse y, inverse x
-static void convolution(NbHandle *h) {
+// Compile-time line kernels when the axis length is one of NB_CT_SIZES,
+// runtime-N line kernels otherwise.  Each returns false when it has no
+// specialisation for the length.
+template <typename T>
+static bool ct_x(NbHandle *h, int mode, CX<T> *g, const CX<T> *tw, T s, const Bricks &bk, const T *bb) {
+  int K1 = h->K[1], K2 = h->K[2];
+  switch (h->K[0]) {
+#define NB_X(N)                                                                                      \
+  case N: {                                                                                          \
+    dim3 G((K1 * K2 + CtLines<N>::x - 1) / CtLines<N>::x);                                           \
+    if (mode == 0) NB_LAUNCH((k_ct_x<T, N, 0>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, bk, bb);        \
+    else if (mode == 1) NB_LAUNCH((k_ct_x<T, N, 1>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, bk, bb);   \
+    else NB_LAUNCH((k_ct_x<T, N, 2>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, bk, bb);                  \
+    return true;                                                                                     \
+  }
+    NB_CT_SIZES(NB_X)
+#undef NB_X
+    default: return false;
+  }
+}
+
+template <typename T>
+static bool ct_y(NbHandle *h, CX<T> *g, const CX<T> *tw, T s) {
+  int K0 = h->K[0], K2 = h->K[2];
+  switch (h->K[1]) {
+#define NB_Y(N)                                                                                      \
+  case N: {                                                                                          \
+    dim3 G(((K0 + CtLines<N>::yz - 1) / CtLines<N>::yz) * K2);                                       \
+    NB_LAUNCH((k_ct_y<T, N>), G, dim3(CT_BS), 0, g, K0, tw, s);                                      \
+    return true;                                                                                     \
+  }
+    NB_CT_SIZES(NB_Y)
+#undef NB_Y
+    default: return false;
+  }
+}
+
+template <typename T>
+static bool ct_z(NbHandle *h, CX<T> *g, const CX<T> *tw, const T *recip) {
+  int K01 = h->K[0] * h->K[1];
+  switch (h->K[2]) {
+#define NB_Z(N)                                                                                      \
+  case N: {                                                                                          \
+    dim3 G((K01 + CtLines<N>::yz - 1) / CtLines<N>::yz);                                             \
+    NB_LAUNCH((k_ct_z<T, N>), G, dim3(CT_BS), 0, g, K01, tw, recip);                                 \
+    return true;                                                                                     \
+  }
+    NB_CT_SIZES(NB_Z)
+#undef NB_Z
+    default: return false;
+  }
+}
+
+// forward x (assembling the grid from the bricks when np > 0), forward y,
+// fused z (forward * influence * inverse), inverse y, inverse x
+template <typename T>
+static void transform_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0, const T *recip) {
   int K0 = h->K[0], K1 = h->K[1], K2 = h->K[2];
-  double2 *g = (double2 *)h->d_grid;
-  const double2 *twx = (const double2 *)h->d_tw, *twy = twx + K0, *twz = twy + K1;
-  Radices rx = factor(K0), ry = factor(K1), rz = factor(K2);
+  const CX<T> *twx = tw0, *twy = twx + K0, *twz = twy + K1;
+  Bricks bk = bricks_of(h);
+  const T *bb = (const T *)h->d_bbuf;
   int Lx = lines_per_block(K0), Ly = lines_per_block(K1), Lz = lines_per_block(K2);
-  size_t shx = sizeof(double2) * (2 * (size_t)Lx * K0 + K0);
-  size_t shy = sizeof(double2) * (2 * (size_t)Ly * K1 + K1);
-  size_t shz = sizeof(double2) * (2 * (size_t)Lz * K2 + K2);
-  dim3 B(128);
-  dim3 Gx((K1 * K2 + Lx - 1) / Lx), Gy((K0 * K2 + Ly - 1) / Ly), Gz((K0 * K1 + Lz - 1) / Lz);
-  NB_LAUNCH((k_fft_axis<0, false>), Gx, B, shx, g, K0, K1, K2, Lx, rx, twx, 1.0, nullptr);
-  NB_LAUNCH((k_fft_axis<1, false>), Gy, B, shy, g, K0, K1, K2, Ly, ry, twy, 1.0, nullptr);
-  NB_LAUNCH((k_fft_axis<2, true>), Gz, B, shz, g, K0, K1, K2, Lz, rz, twz, 1.0, h->d_recip);
-  NB_LAUNCH((k_fft_axis<1, false>), Gy, B, shy, g, K0, K1, K2, Ly, ry, twy, -1.0, nullptr);
-  NB_LAUNCH((k_fft_axis<0, false>), Gx, B, shx, g, K0, K1, K2, Lx, rx, twx, -1.0, nullptr);
+  size_t shx = sizeof(CX<T>) * ((size_t)Lx * K0 + K0), shy = sizeof(CX<T>) * ((size_t)Ly * K1 + K1);
+  size_t shz = sizeof(CX<T>) * ((size_t)Lz * K2 + K2);
+  dim3 B(256);
+  dim3 Gx((K1 * K2 + Lx - 1) / Lx), Gy(((K0 + Ly - 1) / Ly) * K2), Gz((K0 * K1 + Lz - 1) / Lz);
+  const T *nul = nullptr;
+  for (int pass = 0; pass < 2; pass++) {
+    T s = pass == 0 ? (T)1 : (T)-1;
+    if (pass == 0) {
+      if (!ct_x<T>(h, np, g, twx, s, bk, bb)) {
+        if (np == 1) NB_LAUNCH((k_brick_combine<T, 1>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g);
+        if (np == 2) NB_LAUNCH((k_brick_combine<T, 2>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g);
+        NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul);
+      }
+    }
+    if (!ct_y<T>(h, g, twy, s))
+      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, s, nul);
+    if (pass == 0) {
+      if (!ct_z<T>(h, g, twz, recip))
+        NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, true>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, recip);
+    } else {
+      if (!ct_x<T>(h, 0, g, twx, s, bk, bb))
+        NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul);
+    }
+  }
+}
+
+static bool mixed_grid(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXED; }
+
+// np = 0: the grid already holds the data; np = 1, 2: it is assembled from the bricks
+static void convolution(NbHandle *h, int np) {
+  if (mixed_grid(h))
+    transform_t<float>(h, np, (CX<float> *)h->d_grid, (const CX<float> *)h->d_twf, h->d_recipf);
+  else
+    transform_t<double>(h, np, (CX<double> *)h->d_grid, (const CX<double> *)h->d_tw, h->d_recip);
+}
+
+template <typename T, bool IND>
+static void spread_t(NbHandle *h, const double *u, const double *up) {
+  Bricks bk = bricks_of(h);
+  constexpr int NP = IND ? 2 : 1;
+  size_t e3 = (size_t)bk.ext * bk.ext * bk.ext;
+  Frac fr = frac_of(h);
+  NB_LAUNCH((k_spread_brick<T, IND>), dim3(h->nbricks), dim3(256), sizeof(double) * NP * e3, h->n, bk, h->d_bstart, h->d_batom,
+            h->d_theta, h->d_igrid, h->d_fmp, u, up, fr, (T *)h->d_bbuf);
+}
+
+template <typename T, int NP, bool FULL>
+static void gather_t(NbHandle *h, double *o0, double *o1) {
+  Frac fr = frac_of(h);
+  NB_LAUNCH((k_gather<T, NP, FULL>), grid_for((long)h->n * 8, 256), dim3(256), 0, h->n, h->d_theta, h->d_igrid,
+            (const CX<T> *)h->d_grid, fr, h->K[0], h->K[1], h->K[2], o0, o1);
 }
 
 void nb_pme_permanent(NbHandle *h) {
   int n = h->n;
   dim3 B(128), G((n + 127) / 128);
   Frac fr = frac_of(h);
-  NB_LAUNCH(k_bspline_frac, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, h->K[0], h->K[1], h->K[2], fr, h->d_theta,
-            h->d_igrid, h->d_fmp);
-  HIPCHECK(hipMemsetAsync(h->d_grid, 0, sizeof(double) * 2 * h->ntot, h->stream));
-  NB_LAUNCH(k_spread_perm, grid_for((long)n * 125, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_fmp, h->K[0],
-            h->K[1], h->K[2], h->d_grid);
-  convolution(h);
-  NB_LAUNCH((k_gather<1, true>), grid_for((long)n * 8, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_grid, fr,
-            h->K[0], h->K[1], h->K[2], h->d_phip, (double *)nullptr);
+  NB_LAUNCH(k_zero_i, grid_for(h->nbricks, 256), dim3(256), 0, h->nbricks, h->d_bcount);
+  NB_LAUNCH(k_bspline_frac, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, bricks_of(h), fr, h->d_theta, h->d_igrid,
+            h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
+  NB_LAUNCH(k_brick_scan, dim3(1), dim3(1024), 0, h->nbricks, h->d_bcount, h->d_bstart);
+  NB_LAUNCH(k_brick_fill, grid_for(n, 256), dim3(256), 0, n, h->d_bid, h->d_brank, h->d_bstart, h->d_batom);
+  if (mixed_grid(h)) {
+    spread_t<float, false>(h, nullptr, nullptr);
+    convolution(h, 1);
+    gather_t<float, 1, true>(h, h->d_phip, nullptr);
+  } else {
+    spread_t<double, false>(h, nullptr, nullptr);
+    convolution(h, 1);
+    gather_t<double, 1, true>(h, h->d_phip, nullptr);
+  }
 }
 
 void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full) {
-  int n = h->n;
-  Frac fr = frac_of(h);
-  HIPCHECK(hipMemsetAsync(h->d_grid, 0, sizeof(double) * 2 * h->ntot, h->stream));
-  NB_LAUNCH(k_spread_ind, grid_for((long)n * 250, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, u, up, fr, h->K[0],
-            h->K[1], h->K[2], h->d_grid);
-  convolution(h);
-  if (full)
-    NB_LAUNCH((k_gather<2, true>), grid_for((long)n * 8, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_grid, fr,
-              h->K[0], h->K[1], h->K[2], h->d_phiu, h->d_phiup);
-  else
-    NB_LAUNCH((k_gather<2, false>), grid_for((long)n * 8, 256), dim3(256), 0, n, h->d_theta, h->d_igrid, h->d_grid, fr,
-              h->K[0], h->K[1], h->K[2], h->d_phiu, h->d_phiup);
+  if (mixed_grid(h)) {
+    spread_t<float, true>(h, u, up);
+    convolution(h, 2);
+    if (full) gather_t<float, 2, true>(h, h->d_phiu, h->d_phiup);
+    else gather_t<float, 2, false>(h, h->d_phiu, h->d_phiup);
+  } else {
+    spread_t<double, true>(h, u, up);
+    convolution(h, 2);
+    if (full) gather_t<double, 2, true>(h, h->d_phiu, h->d_phiup);
+    else gather_t<double, 2, false>(h, h->d_phiu, h->d_phiup);
+  }
 }
 
 void nb_rotate(NbHandle *h) {
@@ -607,36 +1142,52 @@ void nb_rotate(NbHandle *h) {
                      h->d_redidx, h->d_redval, h->d_mp, h->d_gmp, h->d_red4);
 }
 
-// standalone convolution entry used by the FFT parity test
-extern "C" int nb_test_convolution(double *data, const double *recip, int nx, int ny, int nz) {
+// standalone convolution entry used by the FFT parity test (precision:
+// NB_PREC_FP64 or NB_PREC_MIXED selects the FP64 or FP32 transform)
+extern "C" int nb_test_convolution(double *data, const double *recip, int nx, int ny, int nz, int precision) {
+  NbHandle h;
   try {
-    NbHandle h;
     HIPCHECK(hipStreamCreate(&h.stream));
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, dev));
+    h.lds_max = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    h.prm.precision = precision;
     long ntot = (long)nx * ny * nz;
     h.K[0] = nx; h.K[1] = ny; h.K[2] = nz; h.ntot = ntot;
     HIPCHECK(hipMalloc(&h.d_grid, sizeof(double) * 2 * ntot));
     HIPCHECK(hipMalloc(&h.d_recip, sizeof(double) * ntot));
+    HIPCHECK(hipMalloc(&h.d_recipf, sizeof(float) * ntot));
     HIPCHECK(hipMalloc(&h.d_tw, sizeof(double) * 2 * (nx + ny + nz)));
-    std::vector<double> tw(2 * (nx + ny + nz));
-    int K[3] = {nx, ny, nz}, off = 0;
-    for (int d = 0; d < 3; d++) {
-      for (int j = 0; j < K[d]; j++) {
-        tw[2 * (off + j)] = cos(2.0 * M_PI * j / K[d]);
-        tw[2 * (off + j) + 1] = sin(2.0 * M_PI * j / K[d]);
-      }
-      off += K[d];
-    }
+    HIPCHECK(hipMalloc(&h.d_twf, sizeof(float) * 2 * (nx + ny + nz)));
+    int K[3] = {nx, ny, nz};
+    std::vector<double> tw;
+    twiddles(K, tw);
+    std::vector<float> twf(tw.begin(), tw.end());
     HIPCHECK(hipMemcpy(h.d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(h.d_grid, data, sizeof(double) * 2 * ntot, hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(h.d_recip, recip, sizeof(double) * ntot, hipMemcpyHostToDevice));
-    convolution(&h);
-    HIPCHECK(hipStreamSynchronize(h.stream));
-    HIPCHECK(hipMemcpy(data, h.d_grid, sizeof(double) * 2 * ntot, hipMemcpyDeviceToHost));
-    hipFree(h.d_grid); hipFree(h.d_recip); hipFree(h.d_tw);
+    HIPCHECK(hipMemcpy(h.d_twf, twf.data(), sizeof(float) * twf.size(), hipMemcpyHostToDevice));
+    if (precision == NB_PREC_MIXED) {
+      std::vector<float> gf(data, data + 2 * ntot), rf(recip, recip + ntot);
+      HIPCHECK(hipMemcpy(h.d_grid, gf.data(), sizeof(float) * 2 * ntot, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(h.d_recipf, rf.data(), sizeof(float) * ntot, hipMemcpyHostToDevice));
+      convolution(&h, 0);
+      HIPCHECK(hipStreamSynchronize(h.stream));
+      HIPCHECK(hipMemcpy(gf.data(), h.d_grid, sizeof(float) * 2 * ntot, hipMemcpyDeviceToHost));
+      for (long i = 0; i < 2 * ntot; i++) data[i] = gf[i];
+    } else {
+      HIPCHECK(hipMemcpy(h.d_grid, data, sizeof(double) * 2 * ntot, hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(h.d_recip, recip, sizeof(double) * ntot, hipMemcpyHostToDevice));
+      convolution(&h, 0);
+      HIPCHECK(hipStreamSynchronize(h.stream));
+      HIPCHECK(hipMemcpy(data, h.d_grid, sizeof(double) * 2 * ntot, hipMemcpyDeviceToHost));
+    }
+    free_grid(&h);
     hipStreamDestroy(h.stream);
-    h.d_grid = nullptr;
     return 0;
   } catch (std::exception &e) {
+    free_grid(&h);
+    if (h.stream) hipStreamDestroy(h.stream);
     return NB_ERR_HIP;
   }
 }

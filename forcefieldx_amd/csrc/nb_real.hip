@@ -69,11 +69,52 @@ __global__ void k_ind_reduce(int n, double a3, const double *__restrict__ u, con
 }
 
 // ---------------------------------------------------------------------------
-// PCG vector kernels (PCGSolver.java:609-1178)
+// PCG vector kernels (PCGSolver.java:609-1178).  Dot products are reduced
+// without atomics: every block writes its partial sum to part[slot][block],
+// and each consumer block re-sums the partials in a fixed order (so the
+// result is deterministic and no buffer has to be cleared).
 // ---------------------------------------------------------------------------
 __global__ void k_fill(int m, double *x, double v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) x[i] = v;
+}
+
+// block sums of NV values -> part[slot_k * stride + blockIdx.x]
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *part, const int (&slot)[NV], int stride) {
+  __shared__ double red[NV][4];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum_d(v[k]);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    int k = threadIdx.x;
+    part[(size_t)slot[k] * stride + blockIdx.x] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+  }
+}
+
+// totals of NV partial rows (each of nb entries); every thread receives them
+template <int NV>
+__device__ __forceinline__ void block_totals(const double *part, const int (&slot)[NV], int stride, int nb,
+                                             double (&tot)[NV]) {
+  __shared__ double red[NV][4];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    v[k] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v[k] += part[(size_t)slot[k] * stride + i];
+    v[k] = wave_sum_d(v[k]);
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k][w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) tot[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
 }
 
 __global__ void k_pcg_init_r(int n, const double *__restrict__ polar, const double *__restrict__ ud,
@@ -91,85 +132,116 @@ __global__ void k_pcg_init_r(int n, const double *__restrict__ polar, const doub
   }
 }
 
-// z = polar (E_pre + scale r); optionally p = z; accumulates r.z into slots
-__global__ void k_pcg_z(int n, const double *__restrict__ polar, double psc, const double *__restrict__ f,
-                        const double *__restrict__ fp, const double *__restrict__ r, const double *__restrict__ rp,
-                        double *__restrict__ z, double *__restrict__ zp, double *__restrict__ p, double *__restrict__ pp,
-                        int setp, double *__restrict__ scal, int slot) {
+// z = polar (E_pre + scale r); optionally p = z; partial r.z into slots (slot, slot + 1)
+__global__ void __launch_bounds__(256) k_pcg_z(int n, const double *__restrict__ polar, double psc,
+                                               const double *__restrict__ f, const double *__restrict__ fp,
+                                               const double *__restrict__ r, const double *__restrict__ rp,
+                                               double *__restrict__ z, double *__restrict__ zp, double *__restrict__ p,
+                                               double *__restrict__ pp, int setp, double *__restrict__ part, int stride,
+                                               int slot) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  double rz = 0.0, rzp = 0.0;
+  double v[2] = {0.0, 0.0};
   if (q < 3 * n) {
     double a = polar[q / 3];
     double zz = a * (f[q] + psc * r[q]), zzp = a * (fp[q] + psc * rp[q]);
     z[q] = zz;
     zp[q] = zzp;
     if (setp) { p[q] = zz; pp[q] = zzp; }
-    rz = r[q] * zz;
-    rzp = rp[q] * zzp;
+    v[0] = r[q] * zz;
+    v[1] = rp[q] * zzp;
   }
-  block_add(rz, scal + slot);
-  block_add(rzp, scal + slot + 1);
+  const int sl[2] = {slot, slot + 1};
+  block_partials<2>(v, part, sl, stride);
 }
 
-// vec = A p = p / polar - E(p); accumulates p.Ap and r.z
-__global__ void k_pcg_ap(int n, const double *__restrict__ polar, const double *__restrict__ p,
-                         const double *__restrict__ pp, const double *__restrict__ f, const double *__restrict__ fp,
-                         const double *__restrict__ r, const double *__restrict__ rp, const double *__restrict__ z,
-                         const double *__restrict__ zp, double *__restrict__ vec, double *__restrict__ vecp,
-                         double *__restrict__ scal) {
+// total induced field of p (real space f + self a3 p - reciprocal phi),
+// vec = A p = p / polar - E(p); partial p.Ap and r.z
+__global__ void __launch_bounds__(256) k_pcg_ap(int n, double a3, const double *__restrict__ polar,
+                                                const double *__restrict__ p, const double *__restrict__ pp,
+                                                const double *__restrict__ f, const double *__restrict__ fp,
+                                                const double *__restrict__ phiu, const double *__restrict__ phiup,
+                                                const double *__restrict__ r, const double *__restrict__ rp,
+                                                const double *__restrict__ z, const double *__restrict__ zp,
+                                                double *__restrict__ vec, double *__restrict__ vecp,
+                                                double *__restrict__ part, int stride) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  double a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (q < 3 * n) {
-    double a = polar[q / 3];
-    double v = 0.0, vp = 0.0;
-    if (a > 0.0) {
-      v = p[q] / a - f[q];
-      vp = pp[q] / a - fp[q];
+    int s = q / 3, d = q - 3 * s;
+    double a = polar[s];
+    double ft = f[q], fpt = fp[q];
+    if (phiu) {
+      ft += a3 * p[q] - phiu[30 * (size_t)s + 21 + d];
+      fpt += a3 * pp[q] - phiup[30 * (size_t)s + 21 + d];
     }
-    vec[q] = v;
-    vecp[q] = vp;
-    a1 = p[q] * v;
-    a2 = pp[q] * vp;
-    a3 = r[q] * z[q];
-    a4 = rp[q] * zp[q];
+    double w = 0.0, wp = 0.0;
+    if (a > 0.0) {
+      w = p[q] / a - ft;
+      wp = pp[q] / a - fpt;
+    }
+    vec[q] = w;
+    vecp[q] = wp;
+    v[0] = p[q] * w;
+    v[1] = pp[q] * wp;
+    v[2] = r[q] * z[q];
+    v[3] = rp[q] * zp[q];
   }
-  block_add(a1, scal + S_PAP);
-  block_add(a2, scal + S_PAPP);
-  block_add(a3, scal + S_RZ);
-  block_add(a4, scal + S_RZP);
+  const int sl[4] = {S_PAP, S_PAPP, S_RZ, S_RZP};
+  block_partials<4>(v, part, sl, stride);
 }
 
-__global__ void k_pcg_update(int n, const double *__restrict__ scal_in, const double *__restrict__ vec,
-                             const double *__restrict__ vecp, const double *__restrict__ p, const double *__restrict__ pp,
-                             double *__restrict__ r, double *__restrict__ rp, double *__restrict__ u,
-                             double *__restrict__ up, double *__restrict__ scal) {
+__global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restrict__ vec,
+                                                    const double *__restrict__ vecp, const double *__restrict__ p,
+                                                    const double *__restrict__ pp, double *__restrict__ r,
+                                                    double *__restrict__ rp, double *__restrict__ u,
+                                                    double *__restrict__ up, double *__restrict__ part, int stride,
+                                                    int nb) {
+  const int in[4] = {S_PAP, S_PAPP, S_RZ, S_RZP};
+  double t[4];
+  block_totals<4>(part, in, stride, nb, t);
+  double al = t[2], alp = t[3];
+  if (t[0] != 0.0) al /= t[0];
+  if (t[1] != 0.0) alp /= t[1];
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  double pap = scal_in[S_PAP], papp = scal_in[S_PAPP];
-  double al = scal_in[S_RZ], alp = scal_in[S_RZP];
-  if (pap != 0.0) al /= pap;
-  if (papp != 0.0) alp /= papp;
-  double e1 = 0.0, e2 = 0.0;
+  double v[2] = {0.0, 0.0};
   if (q < 3 * n) {
     double rr = r[q] - al * vec[q], rrp = rp[q] - alp * vecp[q];
     r[q] = rr;
     rp[q] = rrp;
     u[q] += al * p[q];
     up[q] += alp * pp[q];
-    e1 = rr * rr;
-    e2 = rrp * rrp;
+    v[0] = rr * rr;
+    v[1] = rrp * rrp;
   }
-  block_add(e1, scal + S_EPS);
-  block_add(e2, scal + S_EPSP);
+  __syncthreads();  // the LDS of block_totals is reused by block_partials
+  const int out[2] = {S_EPS, S_EPSP};
+  block_partials<2>(v, part, out, stride);
 }
 
-__global__ void k_pcg_p(int n, const double *__restrict__ scal, const double *__restrict__ z,
-                        const double *__restrict__ zp, double *__restrict__ p, double *__restrict__ pp) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 3 * n) return;
-  double prev = scal[S_RZ], prevp = scal[S_RZP];
+// residual norms for the convergence test: scal[S_EPS], scal[S_EPSP]
+__global__ void __launch_bounds__(256) k_pcg_eps(const double *__restrict__ part, int stride, int nb,
+                                                 double *__restrict__ scal) {
+  const int in[2] = {S_EPS, S_EPSP};
+  double t[2];
+  block_totals<2>(part, in, stride, nb, t);
+  if (threadIdx.x == 0) {
+    scal[S_EPS] = t[0];
+    scal[S_EPSP] = t[1];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pcg_p(int n, const double *__restrict__ z, const double *__restrict__ zp,
+                                               double *__restrict__ p, double *__restrict__ pp,
+                                               const double *__restrict__ part, int stride, int nb) {
+  const int in[4] = {S_RZ, S_RZP, S_RZN, S_RZPN};
+  double t[4];
+  block_totals<4>(part, in, stride, nb, t);
+  double prev = t[0], prevp = t[1];
   if (prev == 0.0) prev = 1.0;
   if (prevp == 0.0) prevp = 1.0;
-  double b = scal[S_RZN] / prev, bp = scal[S_RZPN] / prevp;
+  double b = t[2] / prev, bp = t[3] / prevp;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 3 * n) return;
   p[q] = z[q] + b * p[q];
   pp[q] = zp[q] + bp * pp[q];
 }
@@ -419,31 +491,37 @@ void nb_precond_field(NbHandle *h, const double *r, const double *rp, double *f,
 int nb_pcg(NbHandle *h, nb_energies *out) {
   int n = h->n;
   dim3 B(256), G = grid_for(3 * n, 256);
+  const int nb = (int)G.x, stride = h->part_stride;
+  if (nb > stride) throw std::runtime_error("PCG partial-sum buffer too small");
   double psc = h->prm.precond_scale;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
+  double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
   double *f = h->d_field, *fp = h->d_fieldp;  // permanent fields are no longer needed
+  double *part = h->d_part;
   nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
   NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp,
                      h->d_r, h->d_rp);
   nb_precond_field(h, h->d_r, h->d_rp, f, fp);
-  NB_LAUNCH(k_fill, dim3(1), dim3(64), 0, S_NSLOT, h->d_scal, 0.0);
   NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
-                     h->d_pp, 1, h->d_scal, (int)S_RZN);
+                     h->d_pp, 1, part, stride, (int)S_RZN);
   int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
   double eps = 100.0, prev;
   int iter = 0, status = NB_OK;
-  double sc[S_NSLOT];
   for (;;) {
-    NB_LAUNCH(k_fill, dim3(1), dim3(64), 0, S_NSLOT, h->d_scal, 0.0);
-    nb_induced_field(h, h->d_p, h->d_pp, f, fp, false);
-    NB_LAUNCH(k_pcg_ap, G, B, 0, n, h->d_polar, h->d_p, h->d_pp, f, fp, h->d_r, h->d_rp, h->d_z,
-                       h->d_zp, h->d_vec, h->d_vecp, h->d_scal);
-    NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_scal, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r,
-                       h->d_rp, h->d_u, h->d_up, h->d_scal);
+    // E(p) in real space; the reciprocal and self parts are folded into k_pcg_ap
+    nb_dip_field_real(h, h->d_p, h->d_pp, f, fp, false);
+    if (use_recip) nb_pme_induced(h, h->d_p, h->d_pp, false);
+    NB_LAUNCH(k_pcg_ap, G, B, 0, n, a3, h->d_polar, h->d_p, h->d_pp, f, fp,
+              use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
+              h->d_vec, h->d_vecp, part, stride);
+    NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
+              h->d_up, part, stride, nb);
+    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal);
     HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
-    for (int q = 0; q < S_NSLOT; q++) sc[q] = h->h_pinned[q];
     prev = eps;
-    double e = sc[S_EPS] > sc[S_EPSP] ? sc[S_EPS] : sc[S_EPSP];
+    double e = h->h_pinned[S_EPS] > h->h_pinned[S_EPSP] ? h->h_pinned[S_EPS] : h->h_pinned[S_EPSP];
     iter++;
     eps = 4.80321 * sqrt(e / n);
     if (!(eps <= prev)) { status = NB_ERR_SCF_DIVERGED; break; }
@@ -451,13 +529,14 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
     if (eps < h->prm.polar_eps) break;
     nb_precond_field(h, h->d_r, h->d_rp, f, fp);
     NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
-                       h->d_pp, 0, h->d_scal, (int)S_RZN);
-    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_scal, h->d_z, h->d_zp, h->d_p, h->d_pp);
+                       h->d_pp, 0, part, stride, (int)S_RZN);
+    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_z, h->d_zp, h->d_p, h->d_pp, part, stride, nb);
   }
   out->pcg_iters = iter;
   out->pcg_eps = eps;
-  // final induced field: fills the induced reciprocal potentials for the energy
-  nb_induced_field(h, h->d_u, h->d_up, f, fp, true);
+  // reciprocal potentials of the final dipoles (20 derivatives each) for the
+  // energy and gradient; their real-space field is not needed
+  if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
   return status;
 }
 

@@ -73,8 +73,22 @@ struct NbHandle {
   // PME
   int K[3] = {0, 0, 0};
   long ntot = 0;
-  double *d_grid = nullptr;       // interleaved complex [2 ntot]
+  double *d_grid = nullptr;       // interleaved complex [2 ntot] (FP64 form) or float2 [ntot] (mixed form)
   double *d_recip = nullptr;      // influence function [ntot]
+  float *d_recipf = nullptr;      // influence function, FP32 copy (mixed form)
+  float *d_twf = nullptr;         // twiddle tables, FP32 copy
+  // grid bricks: bricks along axis d cover [b K_d / nbk_d, (b+1) K_d / nbk_d);
+  // each atom is binned by its first B-spline point, and a brick's buffer holds
+  // its points plus the 4-point spill into the next brick (edge bext)
+  int nbk[3] = {0, 0, 0};
+  int bext = 0, nbricks = 0;
+  int32_t *d_bcount = nullptr;    // [nbricks] atoms per brick
+  int32_t *d_bstart = nullptr;    // [nbricks + 1] exclusive scan
+  int32_t *d_bid = nullptr;       // [n] brick of each sorted atom
+  int32_t *d_brank = nullptr;     // [n] rank of the atom within its brick
+  int32_t *d_batom = nullptr;     // [n] sorted atoms grouped by brick
+  double *d_bbuf = nullptr;       // [nbricks][2][bext^3] brick partial sums (FP64 or FP32)
+  size_t lds_max = 65536;         // dynamic LDS available to one workgroup
   double *d_theta = nullptr;      // [n][3][5][4]
   int32_t *d_igrid = nullptr;     // [n][3]
   double *d_fmp = nullptr;        // [10n] fractional multipoles
@@ -91,6 +105,8 @@ struct NbHandle {
   double *d_r = nullptr, *d_rp = nullptr, *d_z = nullptr, *d_zp = nullptr, *d_p = nullptr, *d_pp = nullptr;
   double *d_vec = nullptr, *d_vecp = nullptr;
   double *d_scal = nullptr;       // PCG scalars
+  double *d_part = nullptr;       // [S_NSLOT][part_stride] per-block partial dot products
+  int part_stride = 0;
 
   // outputs
   double *d_grad_s = nullptr;     // [3n] sorted direct gradient

@@ -100,7 +100,7 @@ def load(path=LIB_PATH):
     L.nb_profile_count.restype = C.c_int
     L.nb_profile_entry.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_int64)]
     L.nb_profile_entry.restype = C.c_int
-    L.nb_test_convolution.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int]
+    L.nb_test_convolution.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.nb_test_convolution.restype = C.c_int
     _lib = L
     return L
@@ -228,11 +228,12 @@ def device_count():
         return 0
 
 
-def test_convolution(data, recip, nx, ny, nz):
-    """GPU convolution of the library's FFT kernels (test entry)."""
+def test_convolution(data, recip, nx, ny, nz, precision=NB_PREC_FP64):
+    """GPU convolution of the library's FFT kernels (test entry); precision
+    NB_PREC_MIXED runs the FP32 transform."""
     d = np.ascontiguousarray(data, np.float64).copy()
     r = np.ascontiguousarray(recip, np.float64)
-    rc = load().nb_test_convolution(_p(d, C.c_double), _p(r, C.c_double), nx, ny, nz)
+    rc = load().nb_test_convolution(_p(d, C.c_double), _p(r, C.c_double), nx, ny, nz, precision)
     if rc != NB_OK:
         raise NbError(rc, "nb_test_convolution failed")
     return d

@@ -47,25 +47,37 @@ def test_water648_fp64_parity(oracle, water648, pol):
         assert rel_err(up, upo) <= REL
 
 
-def test_gpu_convolution_identity():
-    for nx, ny, nz in ((32, 32, 32), (24, 24, 24), (20, 18, 10), (144, 16, 24)):
+# grids that take the per-plane 2D kernel and grids whose planes exceed it
+# (line kernels along x and y): (96, 96, 12) in both precisions, (64, 64, 16)
+# in FP64 only
+CONV_DIMS = ((32, 32, 32), (24, 24, 24), (20, 18, 10), (144, 16, 24), (96, 96, 12), (64, 64, 16), (30, 45, 8))
+
+
+@pytest.mark.parametrize("prec,tol", [(ffi.NB_PREC_FP64, 1e-13), (ffi.NB_PREC_MIXED, 2e-6)])
+def test_gpu_convolution_identity(prec, tol):
+    """Forward FFT, multiply by 1, un-normalised inverse FFT = N * input."""
+    for nx, ny, nz in CONV_DIMS:
+        print(f"convolution identity {nx}x{ny}x{nz} precision={prec}", flush=True)
         rng = np.random.default_rng(7)
         data = np.zeros(2 * nx * ny * nz)
         data[0::2] = rng.random(nx * ny * nz)
         data[1::2] = rng.random(nx * ny * nz)
-        out = ffi.test_convolution(data, np.ones(nx * ny * nz), nx, ny, nz)
-        assert np.allclose(out / (nx * ny * nz), data, atol=1e-13, rtol=0), (nx, ny, nz)
+        out = ffi.test_convolution(data, np.ones(nx * ny * nz), nx, ny, nz, prec)
+        assert np.allclose(out / (nx * ny * nz), data, atol=tol, rtol=0), (nx, ny, nz)
 
 
-def test_gpu_convolution_matches_oracle(oracle):
-    nx, ny, nz = 24, 20, 18
+@pytest.mark.parametrize("prec,tol", [(ffi.NB_PREC_FP64, 1e-12), (ffi.NB_PREC_MIXED, 1e-5)])
+@pytest.mark.parametrize("dims", [(24, 20, 18), (96, 96, 12)])
+def test_gpu_convolution_matches_oracle(oracle, prec, tol, dims):
+    nx, ny, nz = dims
     rng = np.random.default_rng(11)
     data = np.zeros(2 * nx * ny * nz)
     data[0::2] = rng.random(nx * ny * nz)
+    data[1::2] = rng.random(nx * ny * nz)
     recip = oracle.influence([18.0, 16.0, 15.0, 90, 90, 90], (nx, ny, nz), 5, 0.545)
     ref = oracle.convolution(data, recip, nx, ny, nz)
-    out = ffi.test_convolution(data, recip, nx, ny, nz)
-    assert rel_err(out, ref) < 1e-12
+    out = ffi.test_convolution(data, recip, nx, ny, nz, prec)
+    assert rel_err(out, ref) < tol
 
 
 def test_repeat_evaluation_without_rebuild(oracle, water648):
@@ -97,3 +109,25 @@ def test_water648_mixed_against_fp64(oracle, water648):
     assert rel_err(g.reshape(-1, 3), go) <= 1e-5
     assert rel_err(u, uo) <= 1e-5
     assert abs(e - eo["total"]) <= 1e-6 * abs(eo["perm_self"])
+
+
+def test_water20736_mixed_against_fp64():
+    """Full-size workload (6912 waters, 64^3 grid): mixed form against the FP64
+    form of the library, force relative RMS error <= 1e-5 (FP64 form itself is
+    pinned to the oracle at 648 atoms above and through energies here)."""
+    xyz, box, top = water_box(6912, 59.1, seed=2)
+    p = NonbondedParams(vdw_cutoff=9.0, grid=(64, 64, 64), polar_eps=1e-5)
+    out = {}
+    for prec in (ffi.NB_PREC_FP64, ffi.NB_PREC_MIXED):
+        print(f"water-20736 precision={prec}", flush=True)
+        ffe = ForceFieldEnergy(top, box, p, precision=prec)
+        e, g = ffe.energyAndGradient(xyz)
+        u, up = ffe.particleMeshEwald.getInducedDipoles()
+        out[prec] = (e, g.reshape(-1, 3), u, dict(ffe._last))
+        ffe.destroy()
+    e64, g64, u64, l64 = out[ffi.NB_PREC_FP64]
+    emx, gmx, umx, lmx = out[ffi.NB_PREC_MIXED]
+    assert l64["n_vdw"] == lmx["n_vdw"] and l64["n_perm"] == lmx["n_perm"]
+    assert rel_err(gmx, g64) <= 1e-5
+    assert rel_err(umx, u64) <= 1e-5
+    assert abs(emx - e64) <= 1e-6 * abs(l64["perm_self"])
