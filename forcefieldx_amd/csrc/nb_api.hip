@@ -78,7 +78,7 @@ void free_all(NbHandle *h) {
                   h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                  h->d_scal, h->d_part, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+                  h->d_scal, h->d_part, h->d_dip, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -86,6 +86,10 @@ void free_all(NbHandle *h) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
   }
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->ev_eps) hipEventDestroy(h->ev_eps);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
 }
 
@@ -105,11 +109,11 @@ int nb_prof_begin(NbHandle *h, const char *name) {
   }
   int slot = h->prof_n++;
   h->prof_pool[slot].name = name;
-  HIPCHECK(hipEventRecord(h->prof_pool[slot].a, h->stream));
+  HIPCHECK(hipEventRecord(h->prof_pool[slot].a, h->cur));
   return slot;
 }
 
-void nb_prof_end(NbHandle *h, int slot) { HIPCHECK(hipEventRecord(h->prof_pool[slot].b, h->stream)); }
+void nb_prof_end(NbHandle *h, int slot) { HIPCHECK(hipEventRecord(h->prof_pool[slot].b, h->cur)); }
 
 // accumulate elapsed times by kernel name; requires a completed stream
 void nb_prof_harvest(NbHandle *h) {
@@ -183,6 +187,11 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     h->device = params->device;
     HIPCHECK(hipSetDevice(h->device));
     HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_eps, hipEventDisableTiming));
+    h->cur = h->stream;
     {
       hipDeviceProp_t prop;
       HIPCHECK(hipGetDeviceProperties(&prop, h->device));
@@ -274,12 +283,13 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     HIPCHECK(hipMemset(h->d_phiup, 0, sizeof(double) * 30 * n));
     double **vecs[] = {&h->d_field, &h->d_fieldp, &h->d_u, &h->d_up, &h->d_udir, &h->d_updir, &h->d_r, &h->d_rp,
                        &h->d_z, &h->d_zp, &h->d_p, &h->d_pp, &h->d_vec, &h->d_vecp, &h->d_grad_s, &h->d_torque_s,
-                       &h->d_gred_s, &h->d_grad};
+                       &h->d_gred_s, &h->d_grad, &h->d_grad_r, &h->d_torque_r};
     for (double **v : vecs) {
       alloc(v, n3);
       HIPCHECK(hipMemset(*v, 0, sizeof(double) * n3));
     }
     alloc(&h->d_scal, 16);
+    alloc(&h->d_dip, 12 * (size_t)n);
     h->part_stride = (3 * n + 255) / 256;
     alloc(&h->d_part, 16 * (size_t)h->part_stride);
     alloc(&h->d_energy, 16);
@@ -362,42 +372,71 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
   try {
     HIPCHECK(hipSetDevice(h->device));
     int n = h->n;
-    bool rebuild = nb_need_rebuild(h);
-    if (rebuild) nb_build_lists(h);
-    else nb_gather_sorted(h);
-    e.list_rebuilt = rebuild ? 1 : 0;
-    nb_zero_outputs(h);
-    if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
-    if (h->prm.do_vdw) nb_vdw(h);
-    bool polar = h->prm.polarization != NB_POL_NONE;
-    bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
-    if (h->prm.do_pme) {
-      if (use_recip) nb_pme_permanent(h);
-      nb_perm_field(h);
-      if (polar) {
-        nb_direct(h);
-        if (h->prm.polarization == NB_POL_MUTUAL) {
-          status = nb_pcg(h, &e);
-        } else if (use_recip) {
-          // direct polarization: only the reciprocal potentials of u_direct are needed
-          nb_pme_induced(h, h->d_u, h->d_up, true);
-        }
-      }
-      if (use_recip) nb_recip_energy(h, polar);
-      nb_real_energy(h, polar);
-    }
-    nb_finalize_gradient(h);
     double en[8];
     int64_t cnt[4];
     int32_t fl[8];
-    HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(hipStreamSynchronize(h->stream));
-    if (h->profiling) nb_prof_harvest(h);
-    memcpy(en, h->h_pinned, sizeof(en));
-    memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
-    memcpy(fl, h->h_pinned + 12, sizeof(fl));
+    bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+    int rebuilt = 0;
+    // A filtered (Ewald / preconditioner / masked) row that overflowed its
+    // capacity is only detected when the flags come back with the energies;
+    // the rows are then grown and the evaluation repeated.
+    for (int attempt = 0;; attempt++) {
+      status = NB_OK;
+      bool rebuild = nb_need_rebuild(h);
+      if (rebuild) nb_build_lists(h);
+      else nb_gather_sorted(h);
+      rebuilt |= rebuild ? 1 : 0;
+      nb_zero_outputs(h);
+      if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
+      // vdW runs on the side stream, overlapping everything up to the gradient assembly
+      if (h->prm.do_vdw) {
+        nb_fork(h);
+        NbSide side(h);
+        nb_vdw(h);
+      }
+      bool polar = h->prm.polarization != NB_POL_NONE;
+      if (h->prm.do_pme) {
+        nb_filter_lists(h);
+        // real-space permanent field (side) || permanent PME (main)
+        nb_fork(h);
+        {
+          NbSide side(h);
+          nb_perm_field(h);
+        }
+        if (use_recip) nb_pme_permanent(h);
+        nb_join(h);
+        if (polar) {
+          nb_direct(h);
+          if (h->prm.polarization == NB_POL_MUTUAL) {
+            status = nb_pcg(h, &e);
+          } else if (use_recip) {
+            // direct polarization: only the reciprocal potentials of u_direct are needed
+            nb_pme_induced(h, h->d_u, h->d_up, true);
+          }
+        }
+        // real-space energies (side) || reciprocal energy (main)
+        nb_fork(h);
+        {
+          NbSide side(h);
+          nb_real_energy(h, polar);
+        }
+        if (use_recip) nb_recip_energy(h, polar);
+      }
+      nb_join(h);
+      nb_finalize_gradient(h);
+      HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipStreamSynchronize(h->stream));
+      if (h->profiling) nb_prof_harvest(h);
+      memcpy(en, h->h_pinned, sizeof(en));
+      memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
+      memcpy(fl, h->h_pinned + 12, sizeof(fl));
+      if (!h->prm.do_pme || !nb_grow_filtered_rows(h, fl)) break;
+      if (attempt == 3) throw std::runtime_error("Ewald list capacity could not be established");
+      memset(&e, 0, sizeof(e));
+    }
+    e.list_rebuilt = rebuilt;
     e.perm_real = en[E_PERM_REAL];
     e.pol_real = en[E_POL_REAL];
     e.vdw = en[E_VDW];

@@ -100,3 +100,40 @@ template <typename T> static __device__ __forceinline__ void acc3(V3<double> &a,
 static __device__ __forceinline__ void add3(double *x, int i, V3<double> v) {
   x[3 * i] += v.x; x[3 * i + 1] += v.y; x[3 * i + 2] += v.z;
 }
+
+// Packed per-atom record read by the dipole-field kernels: FP64 position,
+// Thole parameters and the two dipole vectors in the arithmetic type (FP32 in
+// the mixed form), so a neighbour costs one 64 B (FP32) or 96 B (FP64) record.
+// The static part is written with the sorted coordinates; the dipoles are
+// written by whichever kernel produces the vectors whose field comes next
+// (u_direct, polar * r, or the conjugate direction p).
+template <typename T> struct DipRec;
+template <> struct alignas(16) DipRec<float> { double x, y, z; float ipd, thole, u[3], v[3]; };
+template <> struct alignas(16) DipRec<double> { double x, y, z, ipd, thole, u[3], v[3], pad; };
+
+__device__ __forceinline__ void dip_put(void *rec, int mixed, int s, int d, double a, double b) {
+  if (mixed) {
+    DipRec<float> *r = (DipRec<float> *)rec + s;
+    r->u[d] = (float)a;
+    r->v[d] = (float)b;
+  } else {
+    DipRec<double> *r = (DipRec<double> *)rec + s;
+    r->u[d] = a;
+    r->v[d] = b;
+  }
+}
+
+__device__ __forceinline__ void dip_put_static(void *rec, int mixed, int s, double x, double y, double z, double ipd,
+                                               double thole) {
+  if (mixed) {
+    DipRec<float> *r = (DipRec<float> *)rec + s;
+    r->x = x; r->y = y; r->z = z;
+    r->ipd = (float)ipd;
+    r->thole = (float)thole;
+  } else {
+    DipRec<double> *r = (DipRec<double> *)rec + s;
+    r->x = x; r->y = y; r->z = z;
+    r->ipd = ipd;
+    r->thole = thole;
+  }
+}

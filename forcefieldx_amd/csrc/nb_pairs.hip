@@ -84,9 +84,11 @@ __global__ void __launch_bounds__(256) k_filter(int n, const double4 *__restrict
     ns += __popcll(ms);
   }
   if (valid && lane == 0) {
-    new_[s] = ne;
-    npc[s] = np;
-    nsp[s] = ns;
+    // counts are clamped to the row capacity (rows are never read past it);
+    // the flags carry the true maxima so the caller can grow the rows and redo
+    new_[s] = min(ne, maxew);
+    npc[s] = min(np, maxpc);
+    nsp[s] = min(ns, maxsp);
     if (ne > maxew) atomicMax(&flags[1], ne);
     if (np > maxpc) atomicMax(&flags[2], np);
     if (ns > maxsp) atomicMax(&flags[5], ns);
@@ -114,17 +116,15 @@ __device__ __forceinline__ void perm_field_pair(const double4 &pi, const AtomMP 
                     bn[3] - ((U)1 - th.s7 * ps) * rr7));
 }
 
-// T = float (mixed form): masked pairs, whose screened-minus-bare radial
-// differences cancel catastrophically at bonded distances, are skipped in the
-// main row and evaluated in FP64 from the short masked row.
-template <typename T, int LPA>
+// LIST 0: the whole Ewald row (FP64 form); LIST 1: the row without the masked
+// pairs, whose screened-minus-bare radial differences cancel catastrophically
+// at bonded distances (mixed form, T = float); LIST 2: only the masked row in
+// FP64, added to the fields LIST 1 wrote (mixed form)
+template <typename T, int LPA, int LIST>
 __global__ void __launch_bounds__(256) k_perm_field(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
-                                                    const uint32_t *__restrict__ ew, const int32_t *__restrict__ new_,
-                                                    int maxew, const uint32_t *__restrict__ sp,
-                                                    const int32_t *__restrict__ nsp, int maxsp, CellD cell,
-                                                    double aewald, ScaleTab sc, double *__restrict__ field,
-                                                    double *__restrict__ fieldp) {
-  constexpr bool MIXED = sizeof(T) == 4;
+                                                    const uint32_t *__restrict__ lst, const int32_t *__restrict__ lcnt,
+                                                    int maxl, CellD cell, double aewald, ScaleTab sc,
+                                                    double *__restrict__ field, double *__restrict__ fieldp) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
@@ -132,68 +132,56 @@ __global__ void __launch_bounds__(256) k_perm_field(int n, const double4 *__rest
   if (valid) {
     double4 pi = pos[s];
     AtomMP ai = mp[s];
-    int m = new_[s];
-    const uint32_t *row = ew + (size_t)s * maxew;
+    int m = lcnt[s];
+    const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
       uint32_t e = row[t];
       int k = e & NB_INDEX_MASK, code = e >> NB_CODE_SHIFT;
-      if (MIXED && sc.masked[code]) continue;
+      if (LIST == 1 && sc.masked[code]) continue;
       perm_field_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, fd, fp);
-    }
-    if (MIXED) {
-      int ms = nsp[s];
-      const uint32_t *srow = sp + (size_t)s * maxsp;
-      for (int t = lane; t < ms; t += LPA) {
-        uint32_t e = srow[t];
-        int k = e & NB_INDEX_MASK;
-        perm_field_pair<double>(pi, ai, pos[k], mp[k], e >> NB_CODE_SHIFT, cell, aewald, sc, fd, fp);
-      }
     }
   }
   group_sum3<LPA>(fd);
   group_sum3<LPA>(fp);
   if (valid && lane == 0) {
-    st3(field, s, fd);
-    st3(fieldp, s, fp);
+    if (LIST == 2) {
+      add3(field, s, fd);
+      add3(fieldp, s, fp);
+    } else {
+      st3(field, s, fd);
+      st3(fieldp, s, fp);
+    }
   }
 }
 
-// PRE: dipoles are polar * r (preconditioner field, aewald = 0)
-template <typename T, bool PRE, int LPA>
-__global__ void __launch_bounds__(256) k_dip_field(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
+// real-space field of the packed dipoles over one list (Ewald or preconditioner)
+template <typename T, int LPA>
+__global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__restrict__ rec,
                                                    const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
-                                                   int maxl, CellD cell, double aewald, const double *__restrict__ u,
-                                                   const double *__restrict__ up, const double *__restrict__ polar,
-                                                   double *__restrict__ f, double *__restrict__ fp) {
+                                                   int maxl, CellD cell, double aewald, double *__restrict__ f,
+                                                   double *__restrict__ fp) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   V3<double> a{0, 0, 0}, b{0, 0, 0};
   if (valid) {
-    double4 pi = pos[s];
-    T ipdi = (T)mp[s].ipd, ti = (T)mp[s].thole;
+    const DipRec<T> ri = rec[s];
     int m = cnt[s];
     const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
       int k = row[t] & NB_INDEX_MASK;
-      double4 pk = pos[k];
-      double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+      const DipRec<T> rk = rec[k];
+      double dx = rk.x - ri.x, dy = rk.y - ri.y, dz = rk.z - ri.z;
       double r2 = image(cell, dx, dy, dz);
       T r = sqrt((T)r2), rinv = (T)1 / r;
       V3<T> R{(T)dx, (T)dy, (T)dz};
       T bn[3];
       ewald_bn<T, 2>(r, rinv, (T)aewald, bn);
-      Thole<T> th = thole<T, false>(r, R, ipdi, (T)mp[k].ipd, ti, (T)mp[k].thole);
+      Thole<T> th = thole<T, false>(r, R, ri.ipd, rk.ipd, ri.thole, rk.thole);
       T rr3 = rinv * rinv * rinv, rr5 = (T)3 * rr3 * rinv * rinv;
       T B1 = bn[1] - ((T)1 - th.s3) * rr3, B2 = bn[2] - ((T)1 - th.s5) * rr5;
-      V3<T> uk = ld3t<T>(u, k), pk3 = ld3t<T>(up, k);
-      if (PRE) {
-        T pol = (T)polar[k];
-        uk = pol * uk;
-        pk3 = pol * pk3;
-      }
-      acc3(a, dip_field(R, uk, B1, B2));
-      acc3(b, dip_field(R, pk3, B1, B2));
+      acc3(a, dip_field(R, V3<T>{rk.u[0], rk.u[1], rk.u[2]}, B1, B2));
+      acc3(b, dip_field(R, V3<T>{rk.v[0], rk.v[1], rk.v[2]}, B1, B2));
     }
   }
   group_sum3<LPA>(a);
@@ -204,75 +192,86 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const double4 *__restr
   }
 }
 
-// one permanent + polarization energy pair in arithmetic U (FP64 accumulation)
+// Real-space energy, force and torque, split into a permanent kernel and a
+// polarization kernel (the combined pair function needs more than 128 VGPRs
+// and ran one wavefront per SIMD).  Both add into the same sorted gradient and
+// torque arrays, one after the other on the handle's stream.
 template <typename U>
-__device__ __forceinline__ void real_energy_pair(const double4 &pi, const AtomMP &ai, const double *u,
-                                                 const double *up, int s, const double4 &pk, const AtomMP &ak, int k,
-                                                 int code, const CellD &cell, double aewald, const ScaleTab &sc,
-                                                 int polar, int direct, double &ep, double &ei, long long &cnt,
-                                                 V3<double> &F, V3<double> &Tq) {
+__device__ __forceinline__ void perm_energy_pair(const double4 &pi, const AtomMP &ai, const double4 &pk,
+                                                 const AtomMP &ak, int code, const CellD &cell, double aewald,
+                                                 const ScaleTab &sc, double &ep, long long &cnt, V3<double> &F,
+                                                 V3<double> &Tq) {
   double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
   V3<U> r{(U)dx, (U)dy, (U)dz};
   U rl = sqrt((U)r2), rinv = (U)1 / rl;
-  U bn[7], rr[7];
-  ewald_bn<U, 6>(rl, rinv, (U)aewald, bn);
-  coulomb_rr<U, 6>(rinv, rr);
+  U bn[6], rr[6];
+  ewald_bn<U, 5>(rl, rinv, (U)aewald, bn);
   PermGeo<U> g;
   perm_geometry(r, I, K, g);
   V3<U> f{0, 0, 0}, tq{0, 0, 0};
   U epair = perm_terms<U, true>(r, I, K, g, bn, (U)1, f, tq);
   U s1 = (U)1 - (U)sc.m[code];
-  if (s1 != (U)0) epair -= s1 * perm_terms<U, true>(r, I, K, g, rr, -s1, f, tq);
+  if (s1 != (U)0) {
+    coulomb_rr<U, 5>(rinv, rr);
+    epair -= s1 * perm_terms<U, true>(r, I, K, g, rr, -s1, f, tq);
+  }
   if (epair != (U)0) cnt++;
   ep += (double)epair;
-  if (polar) {
-    Thole<U> th = thole<U, true>(rl, r, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
-    ei += (double)pol_terms<U, true>(r, I, K, ld3t<U>(u, s), ld3t<U>(up, s), ld3t<U>(u, k), ld3t<U>(up, k), bn, rr, th,
-                                     (U)sc.p[code], (U)sc.d[code], direct != 0, f, tq);
-  }
   acc3(F, f);
   acc3(Tq, tq);
 }
 
-template <typename T, int LPA>
+template <typename U>
+__device__ __forceinline__ void pol_energy_pair(const double4 &pi, const AtomMP &ai, const double *u, const double *up,
+                                                int s, const double4 &pk, const AtomMP &ak, int k, int code,
+                                                const CellD &cell, double aewald, const ScaleTab &sc, int direct,
+                                                double &ei, V3<double> &F, V3<double> &Tq) {
+  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+  double r2 = image(cell, dx, dy, dz);
+  MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
+  V3<U> r{(U)dx, (U)dy, (U)dz};
+  U rl = sqrt((U)r2), rinv = (U)1 / rl;
+  U bn[5], rr[5];
+  ewald_bn<U, 4>(rl, rinv, (U)aewald, bn);
+  coulomb_rr<U, 4>(rinv, rr);
+  V3<U> f{0, 0, 0}, tq{0, 0, 0};
+  Thole<U> th = thole<U, true>(rl, r, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
+  ei += (double)pol_terms<U, true>(r, I, K, ld3t<U>(u, s), ld3t<U>(up, s), ld3t<U>(u, k), ld3t<U>(up, k), bn, rr, th,
+                                   (U)sc.p[code], (U)sc.d[code], direct != 0, f, tq);
+  acc3(F, f);
+  acc3(Tq, tq);
+}
+
+// POL = false: permanent energy (slot E_PERM_REAL, pair count); POL = true:
+// polarization energy (slot E_POL_REAL).  LIST 0: the whole Ewald row (FP64
+// form); LIST 1: the Ewald row without the masked pairs (mixed form, T =
+// float); LIST 2: only the masked row, in FP64 (mixed form).
+template <typename T, int LPA, bool POL, int LIST>
 __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
-                                                     const uint32_t *__restrict__ ew, const int32_t *__restrict__ new_,
-                                                     int maxew, const uint32_t *__restrict__ sp,
-                                                     const int32_t *__restrict__ nsp, int maxsp, CellD cell,
-                                                     double aewald, ScaleTab sc, int polar, int direct,
+                                                     const uint32_t *__restrict__ lst, const int32_t *__restrict__ lcnt,
+                                                     int maxl, CellD cell, double aewald, ScaleTab sc, int direct,
                                                      const double *__restrict__ u, const double *__restrict__ up,
                                                      double el, double *__restrict__ grad, double *__restrict__ torque,
                                                      double *__restrict__ energy, int64_t *__restrict__ count) {
-  constexpr bool MIXED = sizeof(T) == 4;
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
-  double ep = 0.0, ei = 0.0;
+  double e = 0.0;
   long long cnt = 0;
   V3<double> F{0, 0, 0}, Tq{0, 0, 0};
   if (valid) {
     double4 pi = pos[s];
     AtomMP ai = mp[s];
-    int m = new_[s];
-    const uint32_t *row = ew + (size_t)s * maxew;
+    int m = lcnt[s];
+    const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
-      uint32_t e = row[t];
-      int k = e & NB_INDEX_MASK, code = e >> NB_CODE_SHIFT;
-      if (MIXED && sc.masked[code]) continue;
-      real_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, polar, direct, ep, ei, cnt, F,
-                          Tq);
-    }
-    if (MIXED) {
-      int ms = nsp[s];
-      const uint32_t *srow = sp + (size_t)s * maxsp;
-      for (int t = lane; t < ms; t += LPA) {
-        uint32_t e = srow[t];
-        int k = e & NB_INDEX_MASK;
-        real_energy_pair<double>(pi, ai, u, up, s, pos[k], mp[k], k, e >> NB_CODE_SHIFT, cell, aewald, sc, polar,
-                                 direct, ep, ei, cnt, F, Tq);
-      }
+      uint32_t en = row[t];
+      int k = en & NB_INDEX_MASK, code = en >> NB_CODE_SHIFT;
+      if (LIST == 1 && sc.masked[code]) continue;
+      if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, e, F, Tq);
+      else perm_energy_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, e, cnt, F, Tq);
     }
   }
   group_sum3<LPA>(F);
@@ -281,9 +280,8 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
     add3(grad, s, el * F);
     add3(torque, s, el * Tq);
   }
-  block_add(0.5 * el * ep, energy + E_PERM_REAL);
-  block_add(0.5 * el * ei, energy + E_POL_REAL);
-  block_add_ll(cnt, count + 1);
+  block_add(0.5 * el * e, energy + (POL ? E_POL_REAL : E_PERM_REAL));
+  if (!POL) block_add_ll(cnt, count + 1);
 }
 
 template <typename T, int LPA>
@@ -358,49 +356,77 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+__global__ void k_clear_flags(int32_t *__restrict__ flags) {
+  if (threadIdx.x >= 1 && threadIdx.x <= 5) flags[threadIdx.x] = 0;
+}
+
+// Ewald / preconditioner / masked rows of this evaluation from the Verlet list
+// (PermanentFieldRegion.java:679-714).  No host synchronisation: overflow is
+// reported in flags[1], [2], [5] and handled by nb_grow_filtered_rows after the
+// evaluation.
+void nb_filter_lists(NbHandle *h) {
+  int n = h->n;
+  ScaleTab sc = scale_tab(h);
+  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  double off2 = h->prm.ewald_cutoff * h->prm.ewald_cutoff;
+  NB_LAUNCH(k_clear_flags, dim3(1), dim3(64), 0, h->d_flags);
+  NB_LAUNCH(k_filter<NB_LPA>, G, B, 0, n, h->d_pos4, h->d_nbr, h->d_nnb, h->maxnb, h->cell, off2,
+            h->prm.precond_cutoff, sc, h->d_ew, h->maxew, h->d_new, h->d_pc, h->maxpc, h->d_npc, h->d_sp, h->maxsp,
+            h->d_nsp, h->d_flags);
+}
+
+// fl: the 8 flags read back after an evaluation.  Returns true (after growing
+// the rows) when a filtered row overflowed, in which case the evaluation must
+// be repeated.
+bool nb_grow_filtered_rows(NbHandle *h, const int32_t *fl) {
+  int n = h->n;
+  bool grow = false;
+  if (fl[1] > h->maxew) {
+    hipFree(h->d_ew);
+    h->d_ew = nullptr;
+    h->maxew = fl[1] + fl[1] / 4 + 16;
+    HIPCHECK(hipMalloc(&h->d_ew, sizeof(uint32_t) * (size_t)h->maxew * n));
+    grow = true;
+  }
+  if (fl[2] > h->maxpc) {
+    hipFree(h->d_pc);
+    h->d_pc = nullptr;
+    h->maxpc = fl[2] + fl[2] / 4 + 16;
+    HIPCHECK(hipMalloc(&h->d_pc, sizeof(uint32_t) * (size_t)h->maxpc * n));
+    grow = true;
+  }
+  if (fl[5] > h->maxsp) {
+    hipFree(h->d_sp);
+    h->d_sp = nullptr;
+    h->maxsp = fl[5] + fl[5] / 4 + 16;
+    HIPCHECK(hipMalloc(&h->d_sp, sizeof(uint32_t) * (size_t)h->maxsp * n));
+    grow = true;
+  }
+  return grow;
+}
+
+// real-space permanent field and fieldCR over the filtered Ewald rows
 void nb_perm_field(NbHandle *h) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   ScaleTab sc = scale_tab(h);
   dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
-  double off2 = h->prm.ewald_cutoff * h->prm.ewald_cutoff;
-  for (int attempt = 0;; attempt++) {
-    int32_t zeros[5] = {0, 0, 0, 0, 0};
-    HIPCHECK(hipMemcpyAsync(h->d_flags + 1, zeros, sizeof(zeros), hipMemcpyHostToDevice, h->stream));
-    NB_LAUNCH(k_filter<NB_LPA>, G, B, 0, n, h->d_pos4, h->d_nbr, h->d_nnb, h->maxnb, h->cell, off2,
-              h->prm.precond_cutoff, sc, h->d_ew, h->maxew, h->d_new, h->d_pc, h->maxpc, h->d_npc, h->d_sp, h->maxsp,
-              h->d_nsp, h->d_flags);
-    int32_t f[5];
-    HIPCHECK(hipMemcpyAsync(f, h->d_flags + 1, sizeof(f), hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(hipStreamSynchronize(h->stream));
-    if (f[0] <= h->maxew && f[1] <= h->maxpc && f[4] <= h->maxsp) break;
-    if (attempt == 3) throw std::runtime_error("Ewald list capacity could not be established");
-    if (f[0] > h->maxew) {
-      hipFree(h->d_ew);
-      h->maxew = f[0] + f[0] / 4 + 16;
-      HIPCHECK(hipMalloc(&h->d_ew, sizeof(uint32_t) * (size_t)h->maxew * n));
-    }
-    if (f[1] > h->maxpc) {
-      hipFree(h->d_pc);
-      h->maxpc = f[1] + f[1] / 4 + 16;
-      HIPCHECK(hipMalloc(&h->d_pc, sizeof(uint32_t) * (size_t)h->maxpc * n));
-    }
-    if (f[4] > h->maxsp) {
-      hipFree(h->d_sp);
-      h->maxsp = f[4] + f[4] / 4 + 16;
-      HIPCHECK(hipMalloc(&h->d_sp, sizeof(uint32_t) * (size_t)h->maxsp * n));
-    }
+  if (h->prm.precision == NB_PREC_MIXED) {
+    NB_LAUNCH((k_perm_field<float, NB_LPA, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->cell, aw,
+              sc, h->d_field, h->d_fieldp);
+    NB_LAUNCH((k_perm_field<double, 4, 2>), grid_for((long)n * 4, 256), B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
+              h->maxsp, h->cell, aw, sc, h->d_field, h->d_fieldp);
+  } else {
+    NB_LAUNCH((k_perm_field<double, NB_LPA, 0>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->cell, aw,
+              sc, h->d_field, h->d_fieldp);
   }
-  if (h->prm.precision == NB_PREC_MIXED)
-    NB_LAUNCH((k_perm_field<float, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
-              h->maxsp, h->cell, aw, sc, h->d_field, h->d_fieldp);
-  else
-    NB_LAUNCH((k_perm_field<double, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
-              h->maxsp, h->cell, aw, sc, h->d_field, h->d_fieldp);
 }
 
-void nb_dip_field_real(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool precond) {
+// field of the dipoles currently held in the packed records (d_dip): the
+// Ewald-screened field over the Ewald rows, or (precond) the bare Coulomb +
+// Thole field over the preconditioner rows
+void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = (use_recip && !precond) ? h->prm.ewald_alpha : 0.0;
@@ -408,23 +434,15 @@ void nb_dip_field_real(NbHandle *h, const double *u, const double *up, double *f
   const uint32_t *lst = precond ? h->d_pc : h->d_ew;
   const int32_t *cnt = precond ? h->d_npc : h->d_new;
   int maxl = precond ? h->maxpc : h->maxew;
-  bool mixed = h->prm.precision == NB_PREC_MIXED;
-  if (precond) {
-    if (mixed)
-      NB_LAUNCH((k_dip_field<float, true, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u, up,
-                h->d_polar, f, fp);
-    else
-      NB_LAUNCH((k_dip_field<double, true, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u, up,
-                h->d_polar, f, fp);
-  } else {
-    if (mixed)
-      NB_LAUNCH((k_dip_field<float, false, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u,
-                up, h->d_polar, f, fp);
-    else
-      NB_LAUNCH((k_dip_field<double, false, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, lst, cnt, maxl, h->cell, aw, u,
-                up, h->d_polar, f, fp);
-  }
+  if (h->prm.precision == NB_PREC_MIXED)
+    NB_LAUNCH((k_dip_field<float, NB_LPA>), G, B, 0, n, (const DipRec<float> *)h->d_dip, lst, cnt, maxl, h->cell, aw,
+              f, fp);
+  else
+    NB_LAUNCH((k_dip_field<double, NB_LPA>), G, B, 0, n, (const DipRec<double> *)h->d_dip, lst, cnt, maxl, h->cell,
+              aw, f, fp);
 }
+
+#define NB_LPA_SP 4
 
 void nb_real_energy(NbHandle *h, bool polar) {
   int n = h->n;
@@ -432,15 +450,28 @@ void nb_real_energy(NbHandle *h, bool polar) {
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   ScaleTab sc = scale_tab(h);
   int direct = h->prm.polarization == NB_POL_DIRECT;
-  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
-  if (h->prm.precision == NB_PREC_MIXED)
-    NB_LAUNCH((k_real_energy<float, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
-              h->maxsp, h->cell, aw, sc, (int)polar, direct, h->d_u, h->d_up, h->prm.electric, h->d_grad_s, h->d_torque_s, h->d_energy,
+  dim3 B(256), G = grid_for((long)n * NB_LPA, 256), Gs = grid_for((long)n * NB_LPA_SP, 256);
+  double el = h->prm.electric;
+  if (h->prm.precision == NB_PREC_MIXED) {
+    NB_LAUNCH((k_real_energy<float, NB_LPA, false, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+              h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
+    NB_LAUNCH((k_real_energy<double, NB_LPA_SP, false, 2>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
+              h->maxsp, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
               h->d_count);
-  else
-    NB_LAUNCH((k_real_energy<double, NB_LPA>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->d_sp, h->d_nsp,
-              h->maxsp, h->cell, aw, sc, (int)polar, direct, h->d_u, h->d_up, h->prm.electric, h->d_grad_s, h->d_torque_s, h->d_energy,
-              h->d_count);
+    if (polar) {
+      NB_LAUNCH((k_real_energy<float, NB_LPA, true, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+                h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
+      NB_LAUNCH((k_real_energy<double, NB_LPA_SP, true, 2>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
+                h->maxsp, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
+                h->d_count);
+    }
+  } else {
+    NB_LAUNCH((k_real_energy<double, NB_LPA, false, 0>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+              h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
+    if (polar)
+      NB_LAUNCH((k_real_energy<double, NB_LPA, true, 0>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+                h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
+  }
 }
 
 void nb_vdw(NbHandle *h) {

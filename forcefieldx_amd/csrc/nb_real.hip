@@ -28,18 +28,20 @@ namespace {
 
 
 __global__ void k_gather_pos(int n, const int32_t *__restrict__ perm, const double *__restrict__ xyz,
-                             double4 *__restrict__ pos) {
+                             const AtomMP *__restrict__ mp, double4 *__restrict__ pos, void *rec, int mixed) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
-  pos[s] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
+  double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  pos[s] = make_double4(x, y, z, 0.0);
+  dip_put_static(rec, mixed, s, x, y, z, mp[s].ipd, mp[s].thole);
 }
 
 // self + reciprocal permanent field and the direct dipoles
 __global__ void k_direct(int n, const double *__restrict__ gmp, const double *__restrict__ phip, double a3, int recip,
                          const double *__restrict__ polar, double *__restrict__ field, double *__restrict__ fieldp,
                          double *__restrict__ u, double *__restrict__ up, double *__restrict__ ud,
-                         double *__restrict__ upd) {
+                         double *__restrict__ upd, void *rec, int mixed) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   double a = polar[s];
@@ -54,6 +56,7 @@ __global__ void k_direct(int n, const double *__restrict__ gmp, const double *__
     }
     u[3 * s + d] = ud[3 * s + d] = a * f;
     up[3 * s + d] = upd[3 * s + d] = a * fp;
+    dip_put(rec, mixed, s, d, a * f, a * fp);
   }
 }
 
@@ -117,19 +120,22 @@ __device__ __forceinline__ void block_totals(const double *part, const int (&slo
   for (int k = 0; k < NV; k++) tot[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
 }
 
+// r0 = (u_direct - u0) / polar + E(u0); the packed records receive polar * r0
 __global__ void k_pcg_init_r(int n, const double *__restrict__ polar, const double *__restrict__ ud,
                              const double *__restrict__ upd, const double *__restrict__ u, const double *__restrict__ up,
                              const double *__restrict__ f, const double *__restrict__ fp, double *__restrict__ r,
-                             double *__restrict__ rp) {
+                             double *__restrict__ rp, void *rec, int mixed) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= 3 * n) return;
   double a = polar[q / 3];
+  double rr = 0.0, rrp = 0.0;
   if (a > 0.0) {
-    r[q] = (ud[q] - u[q]) / a + f[q];
-    rp[q] = (upd[q] - up[q]) / a + fp[q];
-  } else {
-    r[q] = rp[q] = 0.0;
+    rr = (ud[q] - u[q]) / a + f[q];
+    rrp = (upd[q] - up[q]) / a + fp[q];
   }
+  r[q] = rr;
+  rp[q] = rrp;
+  dip_put(rec, mixed, q / 3, q % 3, a * rr, a * rrp);
 }
 
 // z = polar (E_pre + scale r); optionally p = z; partial r.z into slots (slot, slot + 1)
@@ -138,7 +144,7 @@ __global__ void __launch_bounds__(256) k_pcg_z(int n, const double *__restrict__
                                                const double *__restrict__ r, const double *__restrict__ rp,
                                                double *__restrict__ z, double *__restrict__ zp, double *__restrict__ p,
                                                double *__restrict__ pp, int setp, double *__restrict__ part, int stride,
-                                               int slot) {
+                                               int slot, void *rec, int mixed) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < 3 * n) {
@@ -146,7 +152,11 @@ __global__ void __launch_bounds__(256) k_pcg_z(int n, const double *__restrict__
     double zz = a * (f[q] + psc * r[q]), zzp = a * (fp[q] + psc * rp[q]);
     z[q] = zz;
     zp[q] = zzp;
-    if (setp) { p[q] = zz; pp[q] = zzp; }
+    if (setp) {
+      p[q] = zz;
+      pp[q] = zzp;
+      dip_put(rec, mixed, q / 3, q % 3, zz, zzp);
+    }
     v[0] = r[q] * zz;
     v[1] = rp[q] * zzp;
   }
@@ -190,12 +200,13 @@ __global__ void __launch_bounds__(256) k_pcg_ap(int n, double a3, const double *
   block_partials<4>(v, part, sl, stride);
 }
 
-__global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restrict__ vec,
+// the packed records receive polar * r for the preconditioner field
+__global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restrict__ polar, const double *__restrict__ vec,
                                                     const double *__restrict__ vecp, const double *__restrict__ p,
                                                     const double *__restrict__ pp, double *__restrict__ r,
                                                     double *__restrict__ rp, double *__restrict__ u,
                                                     double *__restrict__ up, double *__restrict__ part, int stride,
-                                                    int nb) {
+                                                    int nb, void *rec, int mixed) {
   const int in[4] = {S_PAP, S_PAPP, S_RZ, S_RZP};
   double t[4];
   block_totals<4>(part, in, stride, nb, t);
@@ -210,6 +221,8 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
     rp[q] = rrp;
     u[q] += al * p[q];
     up[q] += alp * pp[q];
+    double a = polar[q / 3];
+    dip_put(rec, mixed, q / 3, q % 3, a * rr, a * rrp);
     v[0] = rr * rr;
     v[1] = rrp * rrp;
   }
@@ -230,9 +243,11 @@ __global__ void __launch_bounds__(256) k_pcg_eps(const double *__restrict__ part
   }
 }
 
+// p = z + beta p; the packed records receive p
 __global__ void __launch_bounds__(256) k_pcg_p(int n, const double *__restrict__ z, const double *__restrict__ zp,
                                                double *__restrict__ p, double *__restrict__ pp,
-                                               const double *__restrict__ part, int stride, int nb) {
+                                               const double *__restrict__ part, int stride, int nb, void *rec,
+                                               int mixed) {
   const int in[4] = {S_RZ, S_RZP, S_RZN, S_RZPN};
   double t[4];
   block_totals<4>(part, in, stride, nb, t);
@@ -242,8 +257,10 @@ __global__ void __launch_bounds__(256) k_pcg_p(int n, const double *__restrict__
   double b = t[2] / prev, bp = t[3] / prevp;
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= 3 * n) return;
-  p[q] = z[q] + b * p[q];
-  pp[q] = zp[q] + bp * pp[q];
+  double a = z[q] + b * p[q], ap = zp[q] + bp * pp[q];
+  p[q] = a;
+  pp[q] = ap;
+  dip_put(rec, mixed, q / 3, q % 3, a, ap);
 }
 
 // ---------------------------------------------------------------------------
@@ -322,7 +339,7 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
       tq[1] += el * (-m[3] * sp[1] + m[1] * sp[3] - tt * (m[8] * sp[4] + m[9] * sp[7] + m[6] * sp[8] - m[4] * sp[8] - m[7] * sp[9] - m[8] * sp[6]));
       tq[2] += el * (-m[1] * sp[2] + m[2] * sp[1] - tt * (m[4] * sp[7] + m[7] * sp[5] + m[8] * sp[9] - m[7] * sp[4] - m[5] * sp[7] - m[9] * sp[8]));
     }
-    for (int a = 0; a < 3; a++) { grad[3 * s + a] += df[a]; torque[3 * s + a] += tq[a]; }
+    for (int a = 0; a < 3; a++) { grad[3 * s + a] = df[a]; torque[3 * s + a] = tq[a]; }
   }
   block_add(eself, energy + E_PERM_SELF);
   block_add(0.5 * el * erec, energy + E_PERM_RECIP);
@@ -346,12 +363,14 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
                            const int32_t *__restrict__ frame, const int32_t *__restrict__ axis,
                            const int32_t *__restrict__ redidx, const double *__restrict__ redval,
                            const double *__restrict__ grad_s, const double *__restrict__ torque_s,
+                           const double *__restrict__ grad_r, const double *__restrict__ torque_r, int recip,
                            const double *__restrict__ gred_s, int do_vdw, int do_pme, double *__restrict__ grad,
                            int32_t *__restrict__ flags) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
   V3<double> g = do_pme ? ld3d(grad_s, s) : V3<double>{0, 0, 0};
+  if (do_pme && recip) g += ld3d(grad_r, s);
   if (do_vdw) {
     V3<double> gv = ld3d(gred_s, s);
     int h = redidx[i];
@@ -368,6 +387,7 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
   int fr = frame[i];
   if (fr == NB_FRAME_NONE) return;
   V3<double> trq = ld3d(torque_s, s);
+  if (recip) trq += ld3d(torque_r, s);
   if (!isfinite(trq.x) || !isfinite(trq.y) || !isfinite(trq.z)) atomicOr(&flags[4], 1);
   int iz = axis[3 * i], ix = axis[3 * i + 1], iy = axis[3 * i + 2];
   V3<double> o = ld3d(xyz, i);
@@ -443,18 +463,26 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
 // host launchers
 // ---------------------------------------------------------------------------
 
+static int mixed_of(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXED ? 1 : 0; }
+
 void nb_gather_sorted(NbHandle *h) {
-  NB_LAUNCH(k_gather_pos, grid_for(h->n, 256), dim3(256), 0, h->n, h->d_perm, h->d_xyz, h->d_pos4);
+  NB_LAUNCH(k_gather_pos, grid_for(h->n, 256), dim3(256), 0, h->n, h->d_perm, h->d_xyz, h->d_mp, h->d_pos4,
+            (void *)h->d_dip, mixed_of(h));
+}
+
+// one launch clears the accumulated outputs of an evaluation
+__global__ void k_zero_outputs(int m, double *__restrict__ a, double *__restrict__ b, double *__restrict__ c,
+                               double *__restrict__ d, double *__restrict__ energy, int64_t *__restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; d[i] = 0.0; }
+  if (i < 16) energy[i] = 0.0;
+  if (i < 4) count[i] = 0;
 }
 
 void nb_zero_outputs(NbHandle *h) {
-  int n = h->n;
-  NB_LAUNCH(k_fill, grid_for(3 * n, 256), dim3(256), 0, 3 * n, h->d_grad_s, 0.0);
-  NB_LAUNCH(k_fill, grid_for(3 * n, 256), dim3(256), 0, 3 * n, h->d_torque_s, 0.0);
-  NB_LAUNCH(k_fill, grid_for(3 * n, 256), dim3(256), 0, 3 * n, h->d_gred_s, 0.0);
-  NB_LAUNCH(k_fill, grid_for(3 * n, 256), dim3(256), 0, 3 * n, h->d_grad, 0.0);
-  NB_LAUNCH(k_fill, dim3(1), dim3(64), 0, 16, h->d_energy, 0.0);
-  NB_LAUNCH(k_fill, dim3(1), dim3(64), 0, 4, (double *)h->d_count, 0.0);
+  int n3 = 3 * h->n;
+  NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_grad_s, h->d_torque_s,
+            h->d_gred_s, h->d_grad, h->d_energy, h->d_count);
 }
 
 void nb_direct(NbHandle *h) {
@@ -463,30 +491,36 @@ void nb_direct(NbHandle *h) {
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
   NB_LAUNCH(k_direct, grid_for(n, 256), dim3(256), 0, n, h->d_gmp, h->d_phip, a3, (int)use_recip,
-                     h->d_polar, h->d_field, h->d_fieldp, h->d_u, h->d_up, h->d_udir, h->d_updir);
+                     h->d_polar, h->d_field, h->d_fieldp, h->d_u, h->d_up, h->d_udir, h->d_updir, (void *)h->d_dip,
+                     mixed_of(h));
 }
 
-void nb_dip_field_real(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool precond);
+void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond);
 
 // total induced field: real space + reciprocal + self (ParticleMeshEwald.
-// computeInduceDipoleField, :763-804).  full = false gathers only the field
-// (used inside the PCG loop); full = true also fills the 20 potential
-// derivatives needed by the energy and gradient.
+// computeInduceDipoleField, :763-804).  The real-space part reads the packed
+// records, which must hold (u, up); full = false gathers only the field,
+// full = true also fills the 20 potential derivatives.
 void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool full) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
-  nb_dip_field_real(h, u, up, f, fp, false);
+  // real space on the side stream, reciprocal space on the main stream
+  nb_fork(h);
+  {
+    NbSide side(h);
+    nb_dip_field_real(h, f, fp, false);
+  }
+  if (use_recip) nb_pme_induced(h, u, up, full);
+  nb_join(h);
   if (use_recip) {
-    nb_pme_induced(h, u, up, full);
     double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
     NB_LAUNCH(k_ind_reduce, grid_for(n, 256), dim3(256), 0, n, a3, u, up, h->d_phiu, h->d_phiup, f, fp);
   }
 }
 
-void nb_precond_field(NbHandle *h, const double *r, const double *rp, double *f, double *fp) {
-  nb_dip_field_real(h, r, rp, f, fp, true);
-}
+// preconditioner field of polar * r (the packed records must hold it)
+void nb_precond_field(NbHandle *h, double *f, double *fp) { nb_dip_field_real(h, f, fp, true); }
 
 int nb_pcg(NbHandle *h, nb_energies *out) {
   int n = h->n;
@@ -500,26 +534,43 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
   double *f = h->d_field, *fp = h->d_fieldp;  // permanent fields are no longer needed
   double *part = h->d_part;
   nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
+  void *rec = (void *)h->d_dip;
+  int mx = mixed_of(h);
   NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp,
-                     h->d_r, h->d_rp);
-  nb_precond_field(h, h->d_r, h->d_rp, f, fp);
+                     h->d_r, h->d_rp, rec, mx);
+  nb_precond_field(h, f, fp);
   NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
-                     h->d_pp, 1, part, stride, (int)S_RZN);
+                     h->d_pp, 1, part, stride, (int)S_RZN, rec, mx);
   int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
   double eps = 100.0, prev;
   int iter = 0, status = NB_OK;
   for (;;) {
-    // E(p) in real space; the reciprocal and self parts are folded into k_pcg_ap
-    nb_dip_field_real(h, h->d_p, h->d_pp, f, fp, false);
+    // E(p): real space on the side stream || reciprocal space on the main
+    // stream; the reciprocal and self parts are folded into k_pcg_ap
+    nb_fork(h);
+    {
+      NbSide side(h);
+      nb_dip_field_real(h, f, fp, false);
+    }
     if (use_recip) nb_pme_induced(h, h->d_p, h->d_pp, false);
+    nb_join(h);
     NB_LAUNCH(k_pcg_ap, G, B, 0, n, a3, h->d_polar, h->d_p, h->d_pp, f, fp,
               use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
               h->d_vec, h->d_vecp, part, stride);
-    NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
-              h->d_up, part, stride, nb);
+    NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
+              h->d_up, part, stride, nb, rec, mx);
     NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal);
     HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
-    HIPCHECK(hipStreamSynchronize(h->stream));
+    HIPCHECK(hipEventRecord(h->ev_eps, h->stream));
+    // The next direction is queued before the convergence test is read back,
+    // so the GPU stays busy during the host round trip.  If the test ends the
+    // loop, these kernels have only rewritten the field, z, p and the packed
+    // records, none of which is read after the solve.
+    nb_precond_field(h, f, fp);
+    NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
+                       h->d_pp, 0, part, stride, (int)S_RZN, rec, mx);
+    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_z, h->d_zp, h->d_p, h->d_pp, part, stride, nb, rec, mx);
+    HIPCHECK(hipEventSynchronize(h->ev_eps));
     prev = eps;
     double e = h->h_pinned[S_EPS] > h->h_pinned[S_EPSP] ? h->h_pinned[S_EPS] : h->h_pinned[S_EPSP];
     iter++;
@@ -527,10 +578,6 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
     if (!(eps <= prev)) { status = NB_ERR_SCF_DIVERGED; break; }
     if (iter >= maxit) { status = NB_ERR_SCF_MAXITER; break; }
     if (eps < h->prm.polar_eps) break;
-    nb_precond_field(h, h->d_r, h->d_rp, f, fp);
-    NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
-                       h->d_pp, 0, part, stride, (int)S_RZN);
-    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_z, h->d_zp, h->d_p, h->d_pp, part, stride, nb);
   }
   out->pcg_iters = iter;
   out->pcg_eps = eps;
@@ -547,12 +594,13 @@ void nb_recip_energy(NbHandle *h, bool polar) {
     for (int b = 0; b < 3; b++) fr.a[a][b] = h->afrac[a][b];
   NB_LAUNCH(k_recip_energy, grid_for(n, 128), dim3(128), 0, n, h->d_gmp, h->d_fmp, h->d_phip,
                      h->d_phiu, h->d_phiup, h->d_u, h->d_up, fr, h->prm.electric, h->prm.ewald_alpha, (int)polar,
-                     (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_grad_s, h->d_torque_s, h->d_energy);
+                     (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_grad_r, h->d_torque_r, h->d_energy);
 }
 
 void nb_finalize_gradient(NbHandle *h) {
   int n = h->n;
+  int recip = (!h->cell.aperiodic && h->prm.ewald_alpha > 0.0) ? 1 : 0;
   NB_LAUNCH(k_finalize, grid_for(n, 128), dim3(128), 0, n, h->d_perm, h->d_xyz, h->d_frame,
-                     h->d_axis, h->d_redidx, h->d_redval, h->d_grad_s, h->d_torque_s, h->d_gred_s, (int)h->prm.do_vdw,
-                     (int)h->prm.do_pme, h->d_grad, h->d_flags);
+                     h->d_axis, h->d_redidx, h->d_redval, h->d_grad_s, h->d_torque_s, h->d_grad_r, h->d_torque_r,
+                     recip, h->d_gred_s, (int)h->prm.do_vdw, (int)h->prm.do_pme, h->d_grad, h->d_flags);
 }

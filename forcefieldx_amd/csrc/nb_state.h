@@ -29,7 +29,10 @@ struct NbHandle {
   nb_params prm;
   int n = 0;
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // main stream (PME chain, PCG vectors, bookkeeping)
+  hipStream_t stream2 = nullptr;  // side stream (real-space pair kernels that overlap the PME chain)
+  hipStream_t cur = nullptr;      // stream that NB_LAUNCH uses
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_eps = nullptr;
   std::string err;
 
   // topology (caller order)
@@ -105,12 +108,15 @@ struct NbHandle {
   double *d_r = nullptr, *d_rp = nullptr, *d_z = nullptr, *d_zp = nullptr, *d_p = nullptr, *d_pp = nullptr;
   double *d_vec = nullptr, *d_vecp = nullptr;
   double *d_scal = nullptr;       // PCG scalars
+  double *d_dip = nullptr;        // [n] packed dipole records of the field kernels (96 B each)
   double *d_part = nullptr;       // [S_NSLOT][part_stride] per-block partial dot products
   int part_stride = 0;
 
   // outputs
   double *d_grad_s = nullptr;     // [3n] sorted direct gradient
   double *d_torque_s = nullptr;   // [3n]
+  double *d_grad_r = nullptr;     // [3n] reciprocal-space gradient (written concurrently with d_grad_s)
+  double *d_torque_r = nullptr;   // [3n] reciprocal-space torque
   double *d_gred_s = nullptr;     // [3n] vdW gradient on reduced sites
   double *d_grad = nullptr;       // [3n] caller order
   double *d_energy = nullptr;     // per-block energy partials
@@ -139,10 +145,12 @@ void nb_rotate(NbHandle *h);
 void nb_pme_setup(NbHandle *h);
 void nb_pme_permanent(NbHandle *h);
 void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full);
+void nb_filter_lists(NbHandle *h);
+bool nb_grow_filtered_rows(NbHandle *h, const int32_t *fl);
 void nb_perm_field(NbHandle *h);
 void nb_direct(NbHandle *h);
 void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool full);
-void nb_precond_field(NbHandle *h, const double *r, const double *rp, double *f, double *fp);
+void nb_precond_field(NbHandle *h, double *f, double *fp);
 int nb_pcg(NbHandle *h, nb_energies *out);
 void nb_recip_energy(NbHandle *h, bool polar);
 void nb_real_energy(NbHandle *h, bool polar);
@@ -170,6 +178,24 @@ void nb_prof_harvest(NbHandle *h);
 #define NB_LAUNCH(KERNEL, G, B, SH, ...)                              \
   do {                                                                \
     int slot_ = nb_prof_wanted(h, #KERNEL) ? nb_prof_begin(h, #KERNEL) : -1; \
-    hipLaunchKernelGGL(KERNEL, G, B, SH, h->stream, __VA_ARGS__);     \
+    hipLaunchKernelGGL(KERNEL, G, B, SH, h->cur, __VA_ARGS__);        \
     if (slot_ >= 0) nb_prof_end(h, slot_);                            \
   } while (0)
+
+// Fork / join between the main and the side stream.  nb_fork makes the side
+// stream wait for everything issued so far on the main stream; nb_join makes
+// the main stream wait for everything issued so far on the side stream.
+// Inside an NbSide scope, NB_LAUNCH targets the side stream.
+static inline void nb_fork(NbHandle *h) {
+  HIPCHECK(hipEventRecord(h->ev_fork, h->stream));
+  HIPCHECK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+}
+static inline void nb_join(NbHandle *h) {
+  HIPCHECK(hipEventRecord(h->ev_join, h->stream2));
+  HIPCHECK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+}
+struct NbSide {
+  NbHandle *h;
+  explicit NbSide(NbHandle *hh) : h(hh) { h->cur = h->stream2; }
+  ~NbSide() { h->cur = h->stream; }
+};

@@ -131,3 +131,19 @@ def test_water20736_mixed_against_fp64():
     assert rel_err(gmx, g64) <= 1e-5
     assert rel_err(umx, u64) <= 1e-5
     assert abs(emx - e64) <= 1e-6 * abs(l64["perm_self"])
+
+
+def test_dense_cluster_regrows_filtered_rows(oracle, water648):
+    """216 waters packed in one corner of a 30 A box: the row capacities
+    estimated from the mean density are too small, the evaluation detects the
+    overflow, grows the rows and repeats; the result must still match the
+    oracle at the FP64 bar."""
+    xyz, _, top = water648
+    box = [30.0, 30.0, 30.0, 90.0, 90.0, 90.0]
+    p = NonbondedParams(vdw_cutoff=9.0, grid=(32, 32, 32), polar_eps=1e-6)
+    print("dense cluster in a 30 A box", flush=True)
+    eo, go, uo, upo, e, g, u, up = run_both(oracle, xyz, box, top, p)
+    assert e["n_perm"] == eo["n_perm"] and e["n_vdw"] == eo["n_vdw"]
+    assert abs(e["total"] - eo["total"]) <= REL * abs(eo["total"])
+    assert rel_err(g, go) <= REL
+    assert rel_err(u, uo) <= REL
