@@ -29,52 +29,103 @@ sys.path.insert(0, ROOT)
 METRIC = "AMOEBA E+grad evals/s: mixed vs FP64 GPU at force rel RMS err <=1e-5"
 N_WATERS, BOX, GRID, SEED = 6912, 59.1, (64, 64, 64), 2
 
-# algorithmic work per pair / point (SURVEY-level counts of the reference formulas)
+# Algorithmic work of one launch of each kernel family (bytes that must cross
+# HBM at least once, and FLOPs per pair counted from the reference formulas).
+# s = bytes per real of the grid / pair arithmetic (4 mixed, 8 FP64).
 FLOPS_PER_PAIR = {"k_real_energy": 1400.0, "k_perm_field": 220.0, "k_dip_field": 80.0, "k_vdw": 60.0}
 PEAK_TFLOPS = {"f32": 157.3, "f64": 78.6}
 PEAK_HBM_GBS = 8000.0
 
 
 def kernel_family(name):
-    for k in ("k_real_energy", "k_perm_field", "k_dip_field", "k_vdw", "k_fft_axis", "k_spread_perm",
-              "k_spread_ind", "k_gather"):
+    for k in ("k_real_energy", "k_perm_field", "k_dip_field", "k_vdw", "k_ct_x", "k_ct_y", "k_ct_z",
+              "k_spread_brick", "k_gather", "k_filter"):
         if k in name:
             return k
     return name
 
 
-def roofline_for(name, avg_ms, launches_per_step, pairs, n, K, prec):
-    """Algorithmic work of one launch of kernel `name`, divided by its average duration."""
+def work_model(name, w):
+    """(bytes, description, flops or None) of one launch.  w: n, K, s (real
+    size), ewald_rows (entries of the Ewald rows of a launch = 2 x Ewald
+    pairs), vdw_rows, nbricks, ext."""
     fam = kernel_family(name)
+    n, K, s = w["n"], w["K"], w["s"]
+    K3 = K ** 3
+    cx = 2 * s  # complex grid point
+    if fam == "k_dip_field":
+        rec = 64 if s == 4 else 96
+        rows = w["ewald_rows"]
+        b = n * (rec + 4 + 48) + 4 * rows
+        return b, f"{rec} B record + 4 B count + 48 B field per atom, 4 B per row entry ({rows} entries)", \
+            FLOPS_PER_PAIR[fam] * rows
+    if fam == "k_real_energy":
+        rows = w["ewald_rows"]
+        b = n * (32 + 96 + 2 * 48 + 48) + 4 * rows
+        return b, "position, multipole, gradient/torque read-modify-write, dipoles per atom + row entries", \
+            FLOPS_PER_PAIR[fam] * rows / 2
+    if fam == "k_perm_field":
+        rows = w["ewald_rows"]
+        return n * (32 + 96 + 48) + 4 * rows, "position, multipole, two fields per atom + row entries", \
+            FLOPS_PER_PAIR[fam] * rows
+    if fam == "k_vdw":
+        rows = w["vdw_rows"]
+        return n * (32 + 4 + 24) + 4 * rows, "reduced site, class, gradient per atom + Verlet row entries", \
+            FLOPS_PER_PAIR[fam] * rows
+    if fam in ("k_ct_y", "k_ct_z") or (fam == "k_ct_x" and "0>" in name):
+        b = 2 * K3 * cx + (K3 * s if fam == "k_ct_z" else 0)
+        return b, "complex grid read + write" + (" + influence function" if fam == "k_ct_z" else ""), None
+    if fam == "k_ct_x":
+        nbuf = w["nbricks"] * w["ext"] ** 3 * (2 if "2>" in name else 1) * s
+        return nbuf + K3 * cx, "brick buffer read + complex grid write", None
+    if fam == "k_spread_brick":
+        parts = 2 if "true" in name else 1
+        per_atom = 480 + 12 + 4 + (48 if parts == 2 else 80)
+        return n * per_atom + w["nbricks"] * w["ext"] ** 3 * parts * s, \
+            "B-spline weights, grid start, dipoles / multipoles per atom + brick buffer write", None
+    if fam == "k_gather":
+        return n * (480 + 12) + K3 * cx + n * 2 * 8 * (30 if "true" in name else 3), \
+            "B-spline weights per atom + grid read + potentials written", None
+    return None, None, None
+
+
+def roofline_for(name, avg_ms, w, prec):
+    """HBM roofline of the kernel: algorithmic bytes of one launch / its average
+    duration; pair kernels also carry their VALU rate against the FP32 / FP64
+    vector peak."""
     secs = avg_ms * 1e-3
-    real = "f32" if (prec == "mixed" and fam in FLOPS_PER_PAIR) else "f64"
-    if fam in FLOPS_PER_PAIR:
-        npairs = pairs["vdw"] if fam == "k_vdw" else pairs["ewald"]
-        if fam == "k_dip_field" and "true" in name:
-            npairs = pairs["precond"]
-        flops = FLOPS_PER_PAIR[fam] * npairs
-        return {"bound": "valu", "achieved": flops / secs / 1e12, "peak": PEAK_TFLOPS[real], "unit": "TFLOP/s",
-                "frac": flops / secs / 1e12 / PEAK_TFLOPS[real], "traffic": None,
-                "kernel": name, "avg_launch_ms": avg_ms, "work_per_launch": f"{flops:.4g} flop",
-                "work_model": f"{FLOPS_PER_PAIR[fam]:.0f} flop/pair x {npairs} pairs"}
-    nbytes = None
-    model = None
-    if fam == "k_fft_axis":
-        nbytes = 2 * 16.0 * K ** 3
-        model = "read+write of the complex FP64 grid: 2 x 16 B x K^3"
-    elif fam in ("k_spread_perm", "k_spread_ind"):
-        nbytes = n * (60 * 8 + 12 + 80) + 2 * 16.0 * K ** 3
-        model = "theta/igrid/multipole stream + grid read-modify-write"
-    elif fam == "k_gather":
-        nbytes = n * (60 * 8 + 12 + 30 * 8 * 2) + 16.0 * K ** 3
-        model = "theta/igrid stream + grid read + phi write"
-    if nbytes is None:
-        return {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
-                "traffic": None, "kernel": name, "avg_launch_ms": avg_ms}
-    gbs = nbytes / secs / 1e9
-    return {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-            "traffic": None, "kernel": name, "avg_launch_ms": avg_ms, "work_per_launch": f"{nbytes:.4g} B",
-            "work_model": model}
+    b, model, flops = work_model(name, w)
+    out = {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None, "traffic": None,
+           "kernel": name, "avg_launch_ms": avg_ms}
+    if b is not None:
+        gbs = b / secs / 1e9
+        out.update({"achieved": gbs, "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes_per_launch": b,
+                    "bytes_model": model})
+    if flops is not None:
+        real = "f32" if prec == "mixed" else "f64"
+        tf = flops / secs / 1e12
+        out.update({"valu_tflops": tf, "valu_peak_tflops": PEAK_TFLOPS[real], "valu_frac": tf / PEAK_TFLOPS[real],
+                    "flops_per_launch": flops})
+    return out
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when the summary has no line for it."""
+    path = os.path.join(ROOT, "profiles", "water-20736_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = name.strip("()").split("(")[0]
+    for line in open(path):
+        if line.startswith("| `") and "|" in line[3:]:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells[0].strip("`") == key:
+                try:
+                    return float(cells[-1])
+                except ValueError:
+                    return None
+    return None
 
 
 def time_gpu(top, box, xyz, params, prec, steps, warmup, profile):
@@ -138,6 +189,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB device-to-device copy of known size (reference point for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters); not timed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -149,6 +203,13 @@ def main():
     from forcefieldx_amd.systems import water_box
     if not os.path.exists(ffi.LIB_PATH):
         nbbuild.build()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     xyz, box, top = water_box(N_WATERS, BOX, seed=SEED)
     params = NonbondedParams(vdw_cutoff=9.0, grid=GRID, polar_eps=1e-5)
 
@@ -162,10 +223,13 @@ def main():
     roof = None
     if profmx:
         name, (tot, cnt) = max(profmx["timed"].items(), key=lambda kv: kv[1][0])
-        pairs = {"ewald": int(emx["n_perm"]) - top.n, "vdw": int(emx["n_vdw"]), "precond": None}
-        roof = roofline_for(name, tot / cnt, cnt / args.steps, pairs, top.n, GRID[0], "mixed")
+        nbk = max(2, (GRID[0] + 7) // 8)
+        w = {"n": top.n, "K": GRID[0], "s": 4, "ewald_rows": 2 * (int(emx["n_perm"]) - top.n),
+             "vdw_rows": 2 * int(emx["n_vdw"]), "nbricks": nbk ** 3, "ext": -(-GRID[0] // nbk) + 4}
+        roof = roofline_for(name, tot / cnt, w, "mixed")
         roof["share_of_step"] = tot / (tmx * 1e3 * args.steps)
         roof["launches_per_step"] = cnt / args.steps
+        roof["traffic"] = measured_traffic(name)
 
     cpu = None
     e_cpu = None

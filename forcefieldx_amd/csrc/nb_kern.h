@@ -30,6 +30,21 @@ static inline ScaleTab scale_tab(const NbHandle *h) {
 
 static inline dim3 grid_for(long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
+// Workgroups are dispatched to the 8 XCDs of the device round-robin, and each
+// XCD has its own L2.  xcd_block() gives XCD x the contiguous logical blocks
+// [x per, (x + 1) per), so spatially sorted atoms that share neighbours are
+// processed behind one L2.  Launch with xcd_grid (a multiple of 8 blocks);
+// logical blocks past the end fall out through the kernels' bounds checks.
+constexpr int NB_XCD = 8;
+static inline dim3 xcd_grid(long n, int b) {
+  long nb = (n + b - 1) / b, per = (nb + NB_XCD - 1) / NB_XCD;
+  return dim3((unsigned)(per * NB_XCD));
+}
+static __device__ __forceinline__ int xcd_block() {
+  int per = gridDim.x / NB_XCD;
+  return (blockIdx.x % NB_XCD) * per + blockIdx.x / NB_XCD;
+}
+
 static __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(256) k_filter(int n, const double4 *__restrict
                                                 int32_t *__restrict__ new_, uint32_t *__restrict__ pc, int maxpc,
                                                 int32_t *__restrict__ npc, uint32_t *__restrict__ sp, int maxsp,
                                                 int32_t *__restrict__ nsp, int32_t *__restrict__ flags) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   int cnt = valid ? nnb[s] : 0;
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) k_perm_field(int n, const double4 *__rest
                                                     const uint32_t *__restrict__ lst, const int32_t *__restrict__ lcnt,
                                                     int maxl, CellD cell, double aewald, ScaleTab sc,
                                                     double *__restrict__ field, double *__restrict__ fieldp) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   V3<double> fd{0, 0, 0}, fp{0, 0, 0};
@@ -154,13 +154,14 @@ __global__ void __launch_bounds__(256) k_perm_field(int n, const double4 *__rest
   }
 }
 
-// real-space field of the packed dipoles over one list (Ewald or preconditioner)
-template <typename T, int LPA>
+// real-space field of the packed dipoles over one list: Ewald-screened over
+// the Ewald rows, or (PRE) bare Coulomb + Thole over the preconditioner rows
+template <typename T, int LPA, bool PRE>
 __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__restrict__ rec,
                                                    const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
                                                    int maxl, CellD cell, double aewald, double *__restrict__ f,
                                                    double *__restrict__ fp) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   V3<double> a{0, 0, 0}, b{0, 0, 0};
@@ -175,11 +176,18 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
       double r2 = image(cell, dx, dy, dz);
       T r = sqrt((T)r2), rinv = (T)1 / r;
       V3<T> R{(T)dx, (T)dy, (T)dz};
-      T bn[3];
-      ewald_bn<T, 2>(r, rinv, (T)aewald, bn);
       Thole<T> th = thole<T, false>(r, R, ri.ipd, rk.ipd, ri.thole, rk.thole);
       T rr3 = rinv * rinv * rinv, rr5 = (T)3 * rr3 * rinv * rinv;
-      T B1 = bn[1] - ((T)1 - th.s3) * rr3, B2 = bn[2] - ((T)1 - th.s5) * rr5;
+      T B1, B2;
+      if (PRE) {
+        B1 = th.s3 * rr3;
+        B2 = th.s5 * rr5;
+      } else {
+        T bn[3];
+        ewald_bn<T, 2>(r, rinv, (T)aewald, bn);
+        B1 = bn[1] - ((T)1 - th.s3) * rr3;
+        B2 = bn[2] - ((T)1 - th.s5) * rr5;
+      }
       acc3(a, dip_field(R, V3<T>{rk.u[0], rk.u[1], rk.u[2]}, B1, B2));
       acc3(b, dip_field(R, V3<T>{rk.v[0], rk.v[1], rk.v[2]}, B1, B2));
     }
@@ -255,7 +263,7 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
                                                      const double *__restrict__ u, const double *__restrict__ up,
                                                      double el, double *__restrict__ grad, double *__restrict__ torque,
                                                      double *__restrict__ energy, int64_t *__restrict__ count) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   double e = 0.0;
@@ -290,7 +298,7 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
                                              CellD cell, const double *__restrict__ vtab, int nclass, ScaleTab sc,
                                              double off, double cut, double dl, double gm, double *__restrict__ gred,
                                              double *__restrict__ energy, int64_t *__restrict__ count) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   double esum = 0.0;
@@ -367,7 +375,7 @@ __global__ void k_clear_flags(int32_t *__restrict__ flags) {
 void nb_filter_lists(NbHandle *h) {
   int n = h->n;
   ScaleTab sc = scale_tab(h);
-  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   double off2 = h->prm.ewald_cutoff * h->prm.ewald_cutoff;
   NB_LAUNCH(k_clear_flags, dim3(1), dim3(64), 0, h->d_flags);
   NB_LAUNCH(k_filter<NB_LPA>, G, B, 0, n, h->d_pos4, h->d_nbr, h->d_nnb, h->maxnb, h->cell, off2,
@@ -411,11 +419,11 @@ void nb_perm_field(NbHandle *h) {
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   ScaleTab sc = scale_tab(h);
-  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   if (h->prm.precision == NB_PREC_MIXED) {
     NB_LAUNCH((k_perm_field<float, NB_LPA, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->cell, aw,
               sc, h->d_field, h->d_fieldp);
-    NB_LAUNCH((k_perm_field<double, 4, 2>), grid_for((long)n * 4, 256), B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
+    NB_LAUNCH((k_perm_field<double, 4, 2>), xcd_grid((long)n * 4, 256), B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
               h->maxsp, h->cell, aw, sc, h->d_field, h->d_fieldp);
   } else {
     NB_LAUNCH((k_perm_field<double, NB_LPA, 0>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->cell, aw,
@@ -430,16 +438,17 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = (use_recip && !precond) ? h->prm.ewald_alpha : 0.0;
-  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   const uint32_t *lst = precond ? h->d_pc : h->d_ew;
   const int32_t *cnt = precond ? h->d_npc : h->d_new;
   int maxl = precond ? h->maxpc : h->maxew;
-  if (h->prm.precision == NB_PREC_MIXED)
-    NB_LAUNCH((k_dip_field<float, NB_LPA>), G, B, 0, n, (const DipRec<float> *)h->d_dip, lst, cnt, maxl, h->cell, aw,
-              f, fp);
-  else
-    NB_LAUNCH((k_dip_field<double, NB_LPA>), G, B, 0, n, (const DipRec<double> *)h->d_dip, lst, cnt, maxl, h->cell,
-              aw, f, fp);
+  bool mixed = h->prm.precision == NB_PREC_MIXED;
+  const DipRec<float> *rf = (const DipRec<float> *)h->d_dip;
+  const DipRec<double> *rd = (const DipRec<double> *)h->d_dip;
+  if (mixed && precond) NB_LAUNCH((k_dip_field<float, NB_LPA, true>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp);
+  else if (mixed) NB_LAUNCH((k_dip_field<float, NB_LPA, false>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp);
+  else if (precond) NB_LAUNCH((k_dip_field<double, NB_LPA, true>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp);
+  else NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp);
 }
 
 #define NB_LPA_SP 4
@@ -450,7 +459,7 @@ void nb_real_energy(NbHandle *h, bool polar) {
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   ScaleTab sc = scale_tab(h);
   int direct = h->prm.polarization == NB_POL_DIRECT;
-  dim3 B(256), G = grid_for((long)n * NB_LPA, 256), Gs = grid_for((long)n * NB_LPA_SP, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256), Gs = xcd_grid((long)n * NB_LPA_SP, 256);
   double el = h->prm.electric;
   if (h->prm.precision == NB_PREC_MIXED) {
     NB_LAUNCH((k_real_energy<float, NB_LPA, false, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
@@ -478,7 +487,7 @@ void nb_vdw(NbHandle *h) {
   int n = h->n;
   ScaleTab sc = scale_tab(h);
   double off = h->prm.vdw_cutoff, cut = h->prm.vdw_taper * off;
-  dim3 B(256), G = grid_for((long)n * NB_LPA, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   if (h->prm.precision == NB_PREC_MIXED)
     NB_LAUNCH((k_vdw<float, NB_LPA>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,
               h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,

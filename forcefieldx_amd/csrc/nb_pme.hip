@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict_
                                                 const CX<T> *__restrict__ grid, Frac fr, int K0, int K1, int K2,
                                                 double *__restrict__ out0, double *__restrict__ out1) {
   constexpr int NC = FULL ? 20 : 4;
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
   bool valid = s < n;
   T phi[NP][NC];
@@ -1097,7 +1097,7 @@ static void spread_t(NbHandle *h, const double *u, const double *up) {
 template <typename T, int NP, bool FULL>
 static void gather_t(NbHandle *h, double *o0, double *o1) {
   Frac fr = frac_of(h);
-  NB_LAUNCH((k_gather<T, NP, FULL>), grid_for((long)h->n * 8, 256), dim3(256), 0, h->n, h->d_theta, h->d_igrid,
+  NB_LAUNCH((k_gather<T, NP, FULL>), xcd_grid((long)h->n * 8, 256), dim3(256), 0, h->n, h->d_theta, h->d_igrid,
             (const CX<T> *)h->d_grid, fr, h->K[0], h->K[1], h->K[2], o0, o1);
 }
 
