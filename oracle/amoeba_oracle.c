@@ -1464,6 +1464,70 @@ static int pcg_solve(ctx_t *c, ora_energies *out) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* reciprocal strain derivative, grid part: for every wave vector m the      */
+/* energy weight w(m) = el/2 recip(m) [|Fp|^2 + Re Fp*(Fu + Fv) (+ Re Fu* Fv  */
+/* for mutual polarization)] times (2 (1 + pi^2 m^2/a^2) m_a m_b / m^2 -     */
+/* delta_ab), the derivative of exp(-pi^2 m^2/a^2) / (V m^2) under strain.   */
+/* Fp, Fu, Fv: transforms of the permanent, u and uCR grids; u and uCR share */
+/* one complex grid and are separated with F(-m) = conj F(m) of a real grid. */
+/* ------------------------------------------------------------------------ */
+static void recip_grid_virial(ctx_t *c, int polar, int mutual, double *vir) {
+  pme_t *p = &c->pme;
+  int n = c->n, nx = p->K[0], ny = p->K[1], nz = p->K[2];
+  long nt = p->ntot;
+  double el = c->prm->electric, aw = c->aewald;
+  double *fpm = malloc(2 * nt * sizeof(double)), *fuv = NULL;
+  pme_spread(p, n, c->fmp, NULL, NULL);
+  memcpy(fpm, p->grid, 2 * nt * sizeof(double));
+  ora_fft3d(fpm, nx, ny, nz, 0);
+  if (polar) {
+    double *fu = malloc(3 * n * sizeof(double)), *fv = malloc(3 * n * sizeof(double));
+    for (int i = 0; i < n; i++) {
+      frac_dipole(p, c->ind + 3 * i, fu + 3 * i);
+      frac_dipole(p, c->indp + 3 * i, fv + 3 * i);
+    }
+    pme_spread(p, n, NULL, fu, fv);
+    fuv = malloc(2 * nt * sizeof(double));
+    memcpy(fuv, p->grid, 2 * nt * sizeof(double));
+    ora_fft3d(fuv, nx, ny, nz, 0);
+    free(fu); free(fv);
+  }
+  double pt = (M_PI / aw) * (M_PI / aw);
+  double w9[9] = {0};
+  for (int kz = 0; kz < nz; kz++) {
+    int l = kz >= (nz + 1) / 2 ? kz - nz : kz;
+    for (int ky = 0; ky < ny; ky++) {
+      int k = ky >= (ny + 1) / 2 ? ky - ny : ky;
+      for (int kx = 0; kx < nx; kx++) {
+        int h = kx >= (nx + 1) / 2 ? kx - nx : kx;
+        long idx = kx + (long)nx * (ky + (long)ny * kz);
+        double r = p->recip[idx];
+        if (r == 0.0) continue;
+        double m[3];
+        for (int d = 0; d < 3; d++) m[d] = c->cell.A[d][0] * h + c->cell.A[d][1] * k + c->cell.A[d][2] * l;
+        double m2 = dot3(m, m);
+        double pr = fpm[2 * idx], pi = fpm[2 * idx + 1];
+        double ws = pr * pr + pi * pi;
+        if (polar) {
+          long nidx = ((nx - kx) % nx) + (long)nx * (((ny - ky) % ny) + (long)ny * ((nz - kz) % nz));
+          double ar = fuv[2 * idx], ai = fuv[2 * idx + 1], br = fuv[2 * nidx], bi = -fuv[2 * nidx + 1];
+          double ur = 0.5 * (ar + br), ui = 0.5 * (ai + bi);   /* Fu = (F(m) + conj F(-m)) / 2 */
+          double vr = 0.5 * (ai - bi), vi = -0.5 * (ar - br);  /* Fv = (F(m) - conj F(-m)) / 2i */
+          ws += pr * (ur + vr) + pi * (ui + vi);
+          if (mutual) ws += ur * vr + ui * vi;
+        }
+        double w = 0.5 * el * r * ws, f = 2.0 * (1.0 + pt * m2) / m2;
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) w9[3 * a + b] += w * (f * m[a] * m[b] - (a == b ? 1.0 : 0.0));
+      }
+    }
+  }
+  for (int q = 0; q < 9; q++) vir[q] += w9[q];
+  free(fpm);
+  free(fuv);
+}
+
+/* ------------------------------------------------------------------------ */
 /* reciprocal energy and gradient: ReciprocalEnergyRegion.java:381-746      */
 /* ------------------------------------------------------------------------ */
 static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
@@ -1509,7 +1573,44 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
   out->perm_recip = 0.5 * el * erecip;
   out->perm_charge_corr = 0.0;
   if (qtot != 0.0) out->perm_charge_corr = -0.5 * el * M_PI * qtot * qtot / (c->cell.vol * aw * aw);
+  /* strain derivative: the charge correction scales as 1/V; with the global
+   * multipoles held fixed, the wave vectors' strain dependence gives
+   * -d_b dphi/dx_a - (2/3) Q_bc d2phi/dx_a dx_c per atom (Cartesian
+   * multipoles, potentials of the permanent grid) */
+  for (int a = 0; a < 3; a++) out->vir[4 * a] -= out->perm_charge_corr;
+  for (int i = 0; i < n; i++) {
+    const double *m = c->gmp + 10 * i, *ph = c->phi_p_c + 10 * i;
+    double Q[3][3], H[3][3];
+    sym_full(m + 4, Q);
+    sym_full(ph + 4, H);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double t = m[1 + b] * ph[1 + a];
+        for (int cc = 0; cc < 3; cc++) t += 2.0 / 3.0 * Q[b][cc] * H[a][cc];
+        out->vir[3 * a + b] -= el * t;
+      }
+  }
   if (!polar) return;
+  /* polarization: permanent multipoles in the potentials of u + uCR, u + uCR
+   * in the permanent potential, and (mutual) u in the potential of uCR and
+   * uCR in that of u; each with weight 1/2 */
+  for (int i = 0; i < n; i++) {
+    const double *m = c->gmp + 10 * i, *pp = c->phi_p_c + 10 * i;
+    const double *u = c->ind + 3 * i, *v = c->indp + 3 * i;
+    const double *cu = c->phi_u_c + 10 * i, *cv = c->phi_pp_c + 10 * i;
+    double Q[3][3], Hw[3][3], hw[6];
+    sym_full(m + 4, Q);
+    for (int t = 0; t < 6; t++) hw[t] = cu[4 + t] + cv[4 + t];
+    sym_full(hw, Hw);
+    int mutual = c->prm->polarization == ORA_POL_MUTUAL;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double t = (u[b] + v[b]) * pp[1 + a] + m[1 + b] * (cu[1 + a] + cv[1 + a]);
+        for (int cc = 0; cc < 3; cc++) t += 2.0 / 3.0 * Q[b][cc] * Hw[a][cc];
+        if (mutual) t += v[b] * cu[1 + a] + u[b] * cv[1 + a];
+        out->vir[3 * a + b] -= 0.5 * el * t;
+      }
+  }
   /* induced dipoles */
   double eselfi = 0, erecipi = 0;
   for (int i = 0; i < n; i++) {
@@ -1575,7 +1676,8 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
   double *acc = calloc((size_t)nth * 6 * n, sizeof(double));
   double ep = 0.0, ei = 0.0;
   int cnt = 0;
-  #pragma omp parallel num_threads(nth) reduction(+ : ep, ei, cnt)
+  double w[9] = {0};
+  #pragma omp parallel num_threads(nth) reduction(+ : ep, ei, cnt, w[:9])
   {
     int tid = 0;
 #ifdef _OPENMP
@@ -1612,6 +1714,9 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
         }
         if (e != 0.0) cnt++;
         ep += e;
+        /* pair strain derivative: dE/dr = -(i-side gradient), r = x_k - x_i */
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) w[3 * a + b] -= el * f1[a] * r[b];
         for (int a = 0; a < 3; a++) {
           ag[3 * i + a] += el * f1[a];
           ag[3 * k + a] -= el * f1[a];
@@ -1628,6 +1733,8 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
           double f[3], a2[3], a3[3];
           ei += pol_terms(r, &I, &K, c->ind + 3 * i, c->indp + 3 * i, c->ind + 3 * k, c->indp + 3 * k, &cx, 1, f,
                           a2, a3);
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) w[3 * a + b] -= el * f[a] * r[b];
           for (int a = 0; a < 3; a++) {
             ag[3 * i + a] += el * f[a];
             ag[3 * k + a] -= el * f[a];
@@ -1649,6 +1756,7 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
   out->perm_real = el * ep;
   out->pol_real = el * ei;
   out->n_perm = cnt;
+  for (int q = 0; q < 9; q++) out->vir[q] += w[q];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1671,7 +1779,7 @@ void ora_reduced_xyz(const ora_system *s, double *red) {
   }
 }
 
-static double vdw_energy(ctx_t *c, double *grad, int *count) {
+static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir) {
   const ora_system *s = c->s;
   const ora_params *prm = c->prm;
   int n = c->n, nth = c->nth;
@@ -1688,8 +1796,9 @@ static double vdw_energy(ctx_t *c, double *grad, int *count) {
   double t1n = pow(1.0 + dl, 7.0), g1 = 1.0 + gm;
   double e = 0.0;
   int cnt = 0;
+  double w[9] = {0};
   double *acc = calloc((size_t)nth * 3 * n, sizeof(double));
-  #pragma omp parallel num_threads(nth) reduction(+ : e, cnt)
+  #pragma omp parallel num_threads(nth) reduction(+ : e, cnt, w[:9])
   {
     int tid = 0;
 #ifdef _OPENMP
@@ -1740,6 +1849,9 @@ static double vdw_energy(ctx_t *c, double *grad, int *count) {
         double ir = 1.0 / r;
         int rk = s->red_index[k];
         double rv_k = s->red_value[k];
+        /* pair strain derivative (dE/dr)_a r_b with r = red_k - red_i = -d */
+        for (int a2 = 0; a2 < 3; a2++)
+          for (int b2 = 0; b2 < 3; b2++) w[3 * a2 + b2] += ds * ir * d[a2] * d[b2];
         for (int q = 0; q < 3; q++) {
           double de = ds * d[q] * ir;
           if (ri >= 0 && ri != i && rv_i > 0.0) {
@@ -1766,6 +1878,8 @@ static double vdw_energy(ctx_t *c, double *grad, int *count) {
   free(acc);
   free(red);
   *count = cnt;
+  if (vir)
+    for (int q = 0; q < 9; q++) vir[q] += w[q];
   return e;
 }
 
@@ -1828,7 +1942,7 @@ int ora_evaluate(const ora_system *s, const ora_params *prm, ora_energies *out, 
   double lcut = (prm->vdw_cutoff > prm->ewald_cutoff ? prm->vdw_cutoff : prm->ewald_cutoff) + prm->list_buffer;
   nlist_build(&c->nl, &c->cell, n, s->xyz, lcut);
 
-  if (prm->do_vdw) out->vdw = vdw_energy(c, c->grad, &out->n_vdw);
+  if (prm->do_vdw) out->vdw = vdw_energy(c, c->grad, &out->n_vdw, out->vir);
 
   if (prm->do_pme) {
     c->gmp = malloc(10 * s3 / 3);
@@ -1879,7 +1993,10 @@ int ora_evaluate(const ora_system *s, const ora_params *prm, ora_energies *out, 
         free(f); free(fp);
       }
     }
-    if (c->use_recip) recip_energy(c, out, polar);
+    if (c->use_recip) {
+      recip_energy(c, out, polar);
+      recip_grid_virial(c, polar, prm->polarization == ORA_POL_MUTUAL, out->vir);
+    }
     real_energy(c, out, polar);
     out->n_perm += c->use_recip ? n : 0;
     /* torques to gradients: ReduceRegion.TorqueLoop */
@@ -1897,6 +2014,12 @@ int ora_evaluate(const ora_system *s, const ora_params *prm, ora_energies *out, 
         for (int j = 0; j < 4; j++)
           if (idx[j] >= 0)
             for (int d = 0; d < 3; d++) c->grad[3 * idx[j] + d] += g4[j][d];
+        /* frame rotation under strain: the frame-atom gradients against the
+         * frame vectors (they sum to zero with the centre's) */
+        for (int j = 0; j < 3; j++)
+          if (idx[j] >= 0)
+            for (int a = 0; a < 3; a++)
+              for (int b = 0; b < 3; b++) out->vir[3 * a + b] += g4[j][a] * (s->xyz[3 * idx[j] + b] - o[b]);
       }
     }
     if (ind) memcpy(ind, c->ind, s3);

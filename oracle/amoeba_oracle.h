@@ -85,6 +85,7 @@ typedef struct {
   double pcg_eps;
   int n_vdw, n_perm;
   int status;               /* 0 ok, 1 nan energy, 2 scf diverged, 3 scf maxiter, 4 bad torque */
+  double vir[9];            /* strain derivative W_ab = dE/d(eps_ab), row-major, kcal/mol */
 } ora_energies;
 
 /* Full evaluation.  grad [3n] (kcal/mol/A), ind, indcr [3n] (e*A) may be NULL. */

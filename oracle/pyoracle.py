@@ -54,10 +54,12 @@ class OraEnergies(C.Structure):
                 ("perm_charge_corr", C.c_double), ("pol_real", C.c_double), ("pol_self", C.c_double),
                 ("pol_recip", C.c_double), ("vdw", C.c_double), ("perm", C.c_double), ("pol", C.c_double),
                 ("total", C.c_double), ("pcg_iters", C.c_int), ("pcg_eps", C.c_double), ("n_vdw", C.c_int),
-                ("n_perm", C.c_int), ("status", C.c_int)]
+                ("n_perm", C.c_int), ("status", C.c_int), ("vir", C.c_double * 9)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["vir"] = np.array(list(self.vir)).reshape(3, 3)
+        return d
 
 
 _lib = None

@@ -16,6 +16,8 @@ Each check is chosen so that a plausible mistake in the oracle fails it:
     reciprocal and polarization terms included).
   * Buffered 14-7: energy -eps and the closed-form slope at r = R*.
   * Translation and rotation invariance; the forces sum to zero.
+  * The strain derivative (virial) against central differences of the
+    energy under box strain, cubic and triclinic.
 """
 import numpy as np
 import pytest
@@ -231,3 +233,54 @@ def test_translation_rotation_invariance(oracle):
         assert abs(e1[k] - e0[k]) <= 1e-10 * max(1.0, abs(e0[k]))
         assert abs(e2[k] - e0[k]) <= 1e-9 * max(1.0, abs(e0[k]))
     assert np.linalg.norm(g2 - g0 @ q.T) <= 1e-8 * np.linalg.norm(g0)
+
+
+def box_matrix(box):
+    """Rows = cell vectors in the canonical orientation (a along x, b in xy)."""
+    a, b, c, al, be, ga = box
+    ca, cb, cg, sg = (np.cos(np.radians(al)), np.cos(np.radians(be)), np.cos(np.radians(ga)),
+                      np.sin(np.radians(ga)))
+    bt = (ca - cb * cg) / sg
+    return np.array([[a, 0, 0], [b * cg, b * sg, 0], [c * cb, c * bt, c * np.sqrt(1 - cb * cb - bt * bt)]])
+
+
+def box_params(M):
+    a, b, c = np.linalg.norm(M, axis=1)
+    ang = lambda u, v: np.degrees(np.arccos(u @ v / np.linalg.norm(u) / np.linalg.norm(v)))
+    return [a, b, c, ang(M[1], M[2]), ang(M[0], M[2]), ang(M[0], M[1])]
+
+
+def strained(xyz, box, eps):
+    """Homogeneous deformation (1 + eps) of cell and atoms (fractional
+    coordinates kept), returned in the canonical orientation."""
+    Ai = box_matrix(box)
+    frac = np.asarray(xyz).reshape(-1, 3) @ np.linalg.inv(Ai)
+    box2 = box_params(Ai @ (np.eye(3) + eps).T)
+    return frac @ box_matrix(box2), box2
+
+
+@pytest.mark.parametrize("pol", [0, 1, 2])
+@pytest.mark.parametrize("shear", [0.0, 0.12])
+def test_virial_central_differences(oracle, pol, shear):
+    """W_ab = dE/d eps_ab against central differences of the energy under
+    symmetric strains, in a cubic and a sheared (triclinic) cell."""
+    xyz, box, top = water_box(32, 9.9, seed=4)
+    S = np.zeros((3, 3))
+    S[0, 1] = S[1, 0] = shear / 2
+    S[0, 2] = S[2, 0] = -shear / 3
+    xyz, box = strained(xyz, list(box), S)
+    p = NonbondedParams(vdw_cutoff=4.8, ewald_cutoff=4.8, grid=(20, 20, 20), polar_eps=1e-12, polarization=pol)
+    W = oracle.evaluate(oracle.System(xyz, box, top), p)[0]["vir"]
+    h = 1e-5
+    fd = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(a, 3):
+            E = np.zeros((3, 3))
+            E[a, b] += 0.5
+            E[b, a] += 0.5
+            ep = oracle.evaluate(oracle.System(*strained(xyz, box, h * E), top), p)[0]["total"]
+            em = oracle.evaluate(oracle.System(*strained(xyz, box, -h * E), top), p)[0]["total"]
+            fd[a, b] = fd[b, a] = (ep - em) / (2 * h)
+    scale = np.abs(fd).max()
+    assert np.abs(W - W.T).max() <= 1e-9 * scale
+    assert np.abs(W - fd).max() <= 1e-6 * scale, (W, fd)
