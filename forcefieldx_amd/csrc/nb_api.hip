@@ -13,11 +13,10 @@
 #include <algorithm>
 #include <vector>
 
-#include "nb_state.h"
+#include "nb_kern.h"
 
 namespace {
 
-enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT };
 
 void cell_from_box(const double *box, CellD &c) {
   memset(&c, 0, sizeof(c));
@@ -78,7 +77,7 @@ void free_all(NbHandle *h) {
                   h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                  h->d_scal, h->d_part, h->d_dip, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+                  h->d_scal, h->d_part, h->d_dip, h->d_vpart, h->d_vir, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -289,6 +288,7 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
       HIPCHECK(hipMemset(*v, 0, sizeof(double) * n3));
     }
     alloc(&h->d_scal, 16);
+    alloc(&h->d_vir, 16);
     alloc(&h->d_dip, 12 * (size_t)n);
     h->part_stride = (3 * n + 255) / 256;
     alloc(&h->d_part, 16 * (size_t)h->part_stride);
@@ -377,11 +377,26 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
     int32_t fl[8];
     bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
     int rebuilt = 0;
+    h->want_vir = (flags & NB_VIRIAL) != 0;
+    h->vir_valid = false;
+    if (h->want_vir) {
+      if (h->cell.aperiodic) throw std::runtime_error("the virial needs a periodic cell");
+      // 9 partial sums per block of every contributing launch
+      long pb = xcd_grid((long)n * 16, 256).x, sb = xcd_grid((long)n * 4, 256).x, ab = (n + 127) / 128;
+      long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + 64;
+      if (need > h->vpart_cap) {
+        if (h->d_vpart) hipFree(h->d_vpart);
+        h->d_vpart = nullptr;
+        HIPCHECK(hipMalloc(&h->d_vpart, sizeof(double) * 9 * need));
+        h->vpart_cap = need;
+      }
+    }
     // A filtered (Ewald / preconditioner / masked) row that overflowed its
     // capacity is only detected when the flags come back with the energies;
     // the rows are then grown and the evaluation repeated.
     for (int attempt = 0;; attempt++) {
       status = NB_OK;
+      h->vpart_used = 0;
       bool rebuild = nb_need_rebuild(h);
       if (rebuild) nb_build_lists(h);
       else nb_gather_sorted(h);
@@ -414,6 +429,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
             nb_pme_induced(h, h->d_u, h->d_up, true);
           }
         }
+        if (h->want_vir && use_recip) nb_grid_virial(h, polar);
         // real-space energies (side) || reciprocal energy (main)
         nb_fork(h);
         {
@@ -424,6 +440,10 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
       }
       nb_join(h);
       nb_finalize_gradient(h);
+      if (h->want_vir) {
+        nb_vir_reduce(h);
+        HIPCHECK(hipMemcpyAsync(h->h_pinned + 16, h->d_vir, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
+      }
       HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
@@ -437,6 +457,9 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
       memset(&e, 0, sizeof(e));
     }
     e.list_rebuilt = rebuilt;
+    if (h->want_vir) {
+      for (int q = 0; q < 9; q++) h->vir[q] = h->h_pinned[16 + q];
+    }
     e.perm_real = en[E_PERM_REAL];
     e.pol_real = en[E_POL_REAL];
     e.vdw = en[E_VDW];
@@ -447,6 +470,11 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
       e.pol_recip = en[E_POL_RECIP];
       double q = en[E_QTOT];
       if (q != 0.0) e.perm_charge_corr = -0.5 * h->prm.electric * M_PI * q * q / (h->cell.vol * h->prm.ewald_alpha * h->prm.ewald_alpha);
+    }
+    if (h->want_vir) {
+      // the net-charge correction scales as 1 / V
+      for (int a = 0; a < 3; a++) h->vir[4 * a] -= e.perm_charge_corr;
+      h->vir_valid = true;
     }
     e.n_vdw = cnt[0] / 2;
     e.n_perm = cnt[1] / 2 + ((use_recip && h->prm.do_pme) ? n : 0);
@@ -512,8 +540,12 @@ int nb_get_induced(void *handle, double *u, double *ucr) {
 int nb_get_virial(void *handle, double v[9]) {
   NbHandle *h = (NbHandle *)handle;
   if (!h || !v) return NB_ERR_INVALID;
-  h->err = "virial is not implemented";
-  return NB_ERR_INVALID;
+  if (!h->vir_valid) {
+    h->err = "the last evaluation did not compute the virial (pass NB_VIRIAL)";
+    return NB_ERR_INVALID;
+  }
+  for (int q = 0; q < 9; q++) v[q] = h->vir[q];
+  return NB_OK;
 }
 
 int nb_get_dudx_dl(void *handle, double *g) {

@@ -83,6 +83,24 @@ static __device__ __forceinline__ void block_add_ll(long long v, int64_t *dst) {
   }
 }
 
+// block sums of 9 values (a 3x3 strain derivative) -> part[9 * blockIdx.x + q];
+// every thread of the block must call
+static __device__ __forceinline__ void block_store9(double (&w)[9], double *part) {
+  __shared__ double red9[9][16];
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    double v = wave_sum_d(w[q]);
+    if (lane == 0) red9[q][wv] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    double t = 0.0;
+    for (int i = 0; i < nw; i++) t += red9[threadIdx.x][i];
+    part[9 * (size_t)blockIdx.x + threadIdx.x] = t;
+  }
+}
+
 // sum over the LPA consecutive lanes that share one atom
 template <int LPA> static __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll

@@ -200,6 +200,18 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
   }
 }
 
+// Strain derivative of one full-row visit: with r = x_k - x_i and f the i-side
+// gradient, dE/dr = -f and the pair term (dE/dr)_a r_b is shared by the two
+// visits of the pair.
+template <typename U>
+__device__ __forceinline__ void pair_strain(double *w9, const V3<U> &f, const V3<U> &r) {
+  double fa[3] = {(double)f.x, (double)f.y, (double)f.z}, rb[3] = {(double)r.x, (double)r.y, (double)r.z};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) w9[3 * a + b] -= 0.5 * fa[a] * rb[b];
+}
+
 // Real-space energy, force and torque, split into a permanent kernel and a
 // polarization kernel (the combined pair function needs more than 128 VGPRs
 // and ran one wavefront per SIMD).  Both add into the same sorted gradient and
@@ -208,7 +220,7 @@ template <typename U>
 __device__ __forceinline__ void perm_energy_pair(const double4 &pi, const AtomMP &ai, const double4 &pk,
                                                  const AtomMP &ak, int code, const CellD &cell, double aewald,
                                                  const ScaleTab &sc, double &ep, long long &cnt, V3<double> &F,
-                                                 V3<double> &Tq) {
+                                                 V3<double> &Tq, double *w9) {
   double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
@@ -229,13 +241,14 @@ __device__ __forceinline__ void perm_energy_pair(const double4 &pi, const AtomMP
   ep += (double)epair;
   acc3(F, f);
   acc3(Tq, tq);
+  if (w9) pair_strain(w9, f, r);
 }
 
 template <typename U>
 __device__ __forceinline__ void pol_energy_pair(const double4 &pi, const AtomMP &ai, const double *u, const double *up,
                                                 int s, const double4 &pk, const AtomMP &ak, int k, int code,
                                                 const CellD &cell, double aewald, const ScaleTab &sc, int direct,
-                                                double &ei, V3<double> &F, V3<double> &Tq) {
+                                                double &ei, V3<double> &F, V3<double> &Tq, double *w9) {
   double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
@@ -250,25 +263,30 @@ __device__ __forceinline__ void pol_energy_pair(const double4 &pi, const AtomMP 
                                    (U)sc.p[code], (U)sc.d[code], direct != 0, f, tq);
   acc3(F, f);
   acc3(Tq, tq);
+  if (w9) pair_strain(w9, f, r);
 }
 
 // POL = false: permanent energy (slot E_PERM_REAL, pair count); POL = true:
 // polarization energy (slot E_POL_REAL).  LIST 0: the whole Ewald row (FP64
 // form); LIST 1: the Ewald row without the masked pairs (mixed form, T =
 // float); LIST 2: only the masked row, in FP64 (mixed form).
-template <typename T, int LPA, bool POL, int LIST>
+// VIR: also the pair strain derivative, 9 partial sums per block into vpart
+template <typename T, int LPA, bool POL, int LIST, bool VIR>
 __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
                                                      const uint32_t *__restrict__ lst, const int32_t *__restrict__ lcnt,
                                                      int maxl, CellD cell, double aewald, ScaleTab sc, int direct,
                                                      const double *__restrict__ u, const double *__restrict__ up,
                                                      double el, double *__restrict__ grad, double *__restrict__ torque,
-                                                     double *__restrict__ energy, int64_t *__restrict__ count) {
+                                                     double *__restrict__ energy, int64_t *__restrict__ count,
+                                                     double *__restrict__ vpart) {
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   double e = 0.0;
   long long cnt = 0;
   V3<double> F{0, 0, 0}, Tq{0, 0, 0};
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double *wp = VIR ? w9 : nullptr;
   if (valid) {
     double4 pi = pos[s];
     AtomMP ai = mp[s];
@@ -278,9 +296,14 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
       uint32_t en = row[t];
       int k = en & NB_INDEX_MASK, code = en >> NB_CODE_SHIFT;
       if (LIST == 1 && sc.masked[code]) continue;
-      if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, e, F, Tq);
-      else perm_energy_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, e, cnt, F, Tq);
+      if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, e, F, Tq, wp);
+      else perm_energy_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, e, cnt, F, Tq, wp);
     }
+  }
+  if (VIR) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) w9[q] *= el;
+    block_store9(w9, vpart);
   }
   group_sum3<LPA>(F);
   group_sum3<LPA>(Tq);
@@ -292,18 +315,20 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
   if (!POL) block_add_ll(cnt, count + 1);
 }
 
-template <typename T, int LPA>
+template <typename T, int LPA, bool VIR>
 __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ red, const int32_t *__restrict__ vclass,
                                              const uint32_t *__restrict__ nbr, const int32_t *__restrict__ nnb, int maxnb,
                                              CellD cell, const double *__restrict__ vtab, int nclass, ScaleTab sc,
                                              double off, double cut, double dl, double gm, double *__restrict__ gred,
-                                             double *__restrict__ energy, int64_t *__restrict__ count) {
+                                             double *__restrict__ energy, int64_t *__restrict__ count,
+                                             double *__restrict__ vpart) {
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   double esum = 0.0;
   long long cnt = 0;
   V3<double> G{0, 0, 0};
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (valid) {
     double off2 = off * off, cut2 = cut * cut;
     // MultiplicativeSwitch (f(cut) = 1, f(off) = 0, zero first and second
@@ -351,12 +376,21 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
       T dedr = -ev * (dt1 * t2 + t1 * dt2);
       T ds = (eik * dtp + dedr * tp) / rr;
       acc3(G, V3<T>{ds * (T)dx, ds * (T)dy, ds * (T)dz});
+      if (VIR) {
+        // d = red_i - red_k and the i-side gradient ds d: half of ds d_a d_b per visit
+        double dd[3] = {dx, dy, dz}, h = 0.5 * (double)ds;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int b = 0; b < 3; b++) w9[3 * a + b] += h * dd[a] * dd[b];
+      }
     }
   }
   group_sum3<LPA>(G);
   if (valid && lane == 0) st3(gred, s, G);
   block_add(0.5 * esum, energy + E_VDW);
   block_add_ll(cnt, count);
+  if (VIR) block_store9(w9, vpart);
 }
 
 }  // namespace
@@ -453,7 +487,8 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
 
 #define NB_LPA_SP 4
 
-void nb_real_energy(NbHandle *h, bool polar) {
+template <bool VIR>
+static void real_energy_t(NbHandle *h, bool polar) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
@@ -461,26 +496,34 @@ void nb_real_energy(NbHandle *h, bool polar) {
   int direct = h->prm.polarization == NB_POL_DIRECT;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256), Gs = xcd_grid((long)n * NB_LPA_SP, 256);
   double el = h->prm.electric;
+  auto vr = [&](dim3 g) { return VIR ? nb_vir_region(h, g.x) : (double *)nullptr; };
   if (h->prm.precision == NB_PREC_MIXED) {
-    NB_LAUNCH((k_real_energy<float, NB_LPA, false, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
-              h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
-    NB_LAUNCH((k_real_energy<double, NB_LPA_SP, false, 2>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
+    NB_LAUNCH((k_real_energy<float, NB_LPA, false, 1, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+              h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count, vr(G));
+    NB_LAUNCH((k_real_energy<double, NB_LPA_SP, false, 2, VIR>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
               h->maxsp, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
-              h->d_count);
+              h->d_count, vr(Gs));
     if (polar) {
-      NB_LAUNCH((k_real_energy<float, NB_LPA, true, 1>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
-                h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
-      NB_LAUNCH((k_real_energy<double, NB_LPA_SP, true, 2>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
+      NB_LAUNCH((k_real_energy<float, NB_LPA, true, 1, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+                h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count, vr(G));
+      NB_LAUNCH((k_real_energy<double, NB_LPA_SP, true, 2, VIR>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
                 h->maxsp, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
-                h->d_count);
+                h->d_count, vr(Gs));
     }
   } else {
-    NB_LAUNCH((k_real_energy<double, NB_LPA, false, 0>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
-              h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
+    NB_LAUNCH((k_real_energy<double, NB_LPA, false, 0, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new,
+              h->maxew, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
+              h->d_count, vr(G));
     if (polar)
-      NB_LAUNCH((k_real_energy<double, NB_LPA, true, 0>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
-                h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count);
+      NB_LAUNCH((k_real_energy<double, NB_LPA, true, 0, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new,
+                h->maxew, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
+                h->d_count, vr(G));
   }
+}
+
+void nb_real_energy(NbHandle *h, bool polar) {
+  if (h->want_vir) real_energy_t<true>(h, polar);
+  else real_energy_t<false>(h, polar);
 }
 
 void nb_vdw(NbHandle *h) {
@@ -488,12 +531,15 @@ void nb_vdw(NbHandle *h) {
   ScaleTab sc = scale_tab(h);
   double off = h->prm.vdw_cutoff, cut = h->prm.vdw_taper * off;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
-  if (h->prm.precision == NB_PREC_MIXED)
-    NB_LAUNCH((k_vdw<float, NB_LPA>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,
-              h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,
-              h->d_energy, h->d_count);
-  else
-    NB_LAUNCH((k_vdw<double, NB_LPA>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,
-              h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,
-              h->d_energy, h->d_count);
+  bool mixed = h->prm.precision == NB_PREC_MIXED, vir = h->want_vir;
+  double *vp = vir ? nb_vir_region(h, G.x) : nullptr;
+#define NB_VDW(T, V)                                                                                              \
+  NB_LAUNCH((k_vdw<T, NB_LPA, V>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,       \
+            h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,     \
+            h->d_energy, h->d_count, vp)
+  if (mixed && vir) NB_VDW(float, true);
+  else if (mixed) NB_VDW(float, false);
+  else if (vir) NB_VDW(double, true);
+  else NB_VDW(double, false);
+#undef NB_VDW
 }

@@ -35,6 +35,7 @@
 namespace {
 
 struct Frac { double a[3][3]; };  // a[f][c] = K_f A[c][f]
+struct Frac2v { double a[3][3]; };  // cell.A (fractional = Cartesian . A)
 
 template <typename T> struct alignas(2 * sizeof(T)) CX { T x, y; };
 
@@ -786,8 +787,9 @@ __global__ void __launch_bounds__(CT_BS) k_ct_y(CX<T> *__restrict__ grid, int K0
   }
 }
 
-// z lines: forward, influence multiply, un-normalised inverse
-template <typename T, int N>
+// z lines: forward, influence multiply, un-normalised inverse (CONV), or the
+// forward transform alone
+template <typename T, int N, bool CONV>
 __global__ void __launch_bounds__(CT_BS) k_ct_z(CX<T> *__restrict__ grid, int K01, const CX<T> *__restrict__ twg,
                                                 const T *__restrict__ recip) {
   constexpr int NL = CtLines<N>::yz;
@@ -808,7 +810,7 @@ __global__ void __launch_bounds__(CT_BS) k_ct_z(CX<T> *__restrict__ grid, int K0
       CX<T> v = grid[gi];
       re[q] = v.x;
       im[q] = v.y;
-      gr[q] = recip[gi];
+      if (CONV) gr[q] = recip[gi];
     }
   }
 #pragma unroll
@@ -818,16 +820,18 @@ __global__ void __launch_bounds__(CT_BS) k_ct_z(CX<T> *__restrict__ grid, int K0
   }
   __syncthreads();
   ct_fft<N, 1, NL, 1, NL, true, T>(buf, tw, (T)1);
+  if (CONV) {
 #pragma unroll
-  for (int q = 0; q < PER; q++) {
-    int t = q * CT_BS + threadIdx.x;
-    if (t < TOT) {
-      buf[t].x *= gr[q];
-      buf[t].y *= gr[q];
+    for (int q = 0; q < PER; q++) {
+      int t = q * CT_BS + threadIdx.x;
+      if (t < TOT) {
+        buf[t].x *= gr[q];
+        buf[t].y *= gr[q];
+      }
     }
+    __syncthreads();
+    ct_fft<N, 1, NL, 1, NL, true, T>(buf, tw, (T)-1);
   }
-  __syncthreads();
-  ct_fft<N, 1, NL, 1, NL, true, T>(buf, tw, (T)-1);
   for (int t = threadIdx.x; t < NL * N; t += CT_BS) {
     int e = t / NL, l = t - e * NL;
     if (l < cnt) grid[(size_t)(first + l) + (size_t)K01 * e] = buf[t];
@@ -913,9 +917,11 @@ static void twiddles(const int K[3], std::vector<double> &tw) {
 }
 
 static void free_grid(NbHandle *h) {
-  void *ptrs[] = {h->d_grid, h->d_recip, h->d_recipf, h->d_tw, h->d_twf, h->d_bcount, h->d_bstart, h->d_bbuf};
+  void *ptrs[] = {h->d_grid, h->d_recip, h->d_recipf, h->d_tw, h->d_twf, h->d_bcount, h->d_bstart, h->d_bbuf,
+                  h->d_fgrid};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  h->d_fgrid = nullptr;
   h->d_grid = nullptr; h->d_recip = nullptr; h->d_recipf = nullptr; h->d_tw = nullptr; h->d_twf = nullptr;
   h->d_bcount = nullptr; h->d_bstart = nullptr; h->d_bbuf = nullptr;
 }
@@ -1030,7 +1036,8 @@ static bool ct_z(NbHandle *h, CX<T> *g, const CX<T> *tw, const T *recip) {
 #define NB_Z(N)                                                                                      \
   case N: {                                                                                          \
     dim3 G((K01 + CtLines<N>::yz - 1) / CtLines<N>::yz);                                             \
-    NB_LAUNCH((k_ct_z<T, N>), G, dim3(CT_BS), 0, g, K01, tw, recip);                                 \
+    if (recip) NB_LAUNCH((k_ct_z<T, N, true>), G, dim3(CT_BS), 0, g, K01, tw, recip);                \
+    else NB_LAUNCH((k_ct_z<T, N, false>), G, dim3(CT_BS), 0, g, K01, tw, recip);                     \
     return true;                                                                                     \
   }
     NB_CT_SIZES(NB_Z)
@@ -1072,6 +1079,94 @@ static void transform_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0, const T
         NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul);
     }
   }
+}
+
+// forward transform only (x assembled from the bricks when np > 0), used for
+// the strain derivative
+template <typename T>
+static void forward_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0) {
+  int K0 = h->K[0], K1 = h->K[1], K2 = h->K[2];
+  const CX<T> *twx = tw0, *twy = twx + K0, *twz = twy + K1;
+  Bricks bk = bricks_of(h);
+  const T *bb = (const T *)h->d_bbuf;
+  int Lx = lines_per_block(K0), Ly = lines_per_block(K1), Lz = lines_per_block(K2);
+  size_t shx = sizeof(CX<T>) * ((size_t)Lx * K0 + K0), shy = sizeof(CX<T>) * ((size_t)Ly * K1 + K1);
+  size_t shz = sizeof(CX<T>) * ((size_t)Lz * K2 + K2);
+  dim3 B(256), Gx((K1 * K2 + Lx - 1) / Lx), Gy(((K0 + Ly - 1) / Ly) * K2), Gz((K0 * K1 + Lz - 1) / Lz);
+  const T *nul = nullptr;
+  if (!ct_x<T>(h, np, g, twx, (T)1, bk, bb)) {
+    if (np == 1) NB_LAUNCH((k_brick_combine<T, 1>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g);
+    if (np == 2) NB_LAUNCH((k_brick_combine<T, 2>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g);
+    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, (T)1, nul);
+  }
+  if (!ct_y<T>(h, g, twy, (T)1))
+    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, (T)1, nul);
+  if (!ct_z<T>(h, g, twz, nul))
+    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, false>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, nul);
+}
+
+// Reciprocal strain derivative, grid part: at every wave vector m the energy
+// weight w(m) = el/2 recip(m) [|Fp|^2 + Re Fp*(Fu + Fv) (+ Re Fu* Fv, mutual)]
+// times 2 (1 + pi^2 m^2 / a^2) m_a m_b / m^2 - delta_ab.  fp: transform of the
+// permanent grid; fi: transform of the induced grid (u real, uCR imaginary;
+// the two are separated with F(-m) = conj F(m) of a real grid) or null.
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_virial(int K0, int K1, int K2, const CX<T> *__restrict__ fp,
+                                                     const CX<T> *__restrict__ fi, int mutual,
+                                                     const double *__restrict__ recip, Frac2v A, double pt, double el,
+                                                     double *__restrict__ vpart) {
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ntot = (long)K0 * K1 * K2;
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (g < ntot && recip[g] != 0.0) {
+    int kx = (int)(g % K0);
+    long r1 = g / K0;
+    int ky = (int)(r1 % K1), kz = (int)(r1 / K1);
+    int hh = kx >= (K0 + 1) / 2 ? kx - K0 : kx, k = ky >= (K1 + 1) / 2 ? ky - K1 : ky;
+    int l = kz >= (K2 + 1) / 2 ? kz - K2 : kz;
+    double m[3];
+    for (int d = 0; d < 3; d++) m[d] = A.a[d][0] * hh + A.a[d][1] * k + A.a[d][2] * l;
+    double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    double pr = (double)fp[g].x, pim = (double)fp[g].y;
+    double ws = pr * pr + pim * pim;
+    if (fi) {
+      long ng = (long)((K0 - kx) % K0) + (long)K0 * (((K1 - ky) % K1) + (long)K1 * ((K2 - kz) % K2));
+      double ar = (double)fi[g].x, ai = (double)fi[g].y, br = (double)fi[ng].x, bi = -(double)fi[ng].y;
+      double ur = 0.5 * (ar + br), ui = 0.5 * (ai + bi);
+      double vr = 0.5 * (ai - bi), vi = -0.5 * (ar - br);
+      ws += pr * (ur + vr) + pim * (ui + vi);
+      if (mutual) ws += ur * vr + ui * vi;
+    }
+    double w = 0.5 * el * recip[g] * ws, f = 2.0 * (1.0 + pt * m2) / m2;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) w9[3 * a + b] = w * (f * m[a] * m[b] - (a == b ? 1.0 : 0.0));
+  }
+  block_store9(w9, vpart);
+}
+
+template <typename T>
+static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
+  CX<T> *g = (CX<T> *)h->d_grid, *fpg = (CX<T> *)h->d_fgrid;
+  Frac fr = frac_of(h);
+  Bricks bk = bricks_of(h);
+  size_t e3 = (size_t)bk.ext * bk.ext * bk.ext;
+  NB_LAUNCH((k_spread_brick<T, false>), dim3(h->nbricks), dim3(256), sizeof(double) * e3, h->n, bk, h->d_bstart,
+            h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, (const double *)nullptr, (const double *)nullptr, fr,
+            (T *)h->d_bbuf);
+  forward_t<T>(h, 1, fpg, tw);
+  if (polar) {
+    NB_LAUNCH((k_spread_brick<T, true>), dim3(h->nbricks), dim3(256), sizeof(double) * 2 * e3, h->n, bk, h->d_bstart,
+              h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, h->d_u, h->d_up, fr, (T *)h->d_bbuf);
+    forward_t<T>(h, 2, g, tw);
+  }
+  Frac2v A;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) A.a[a][b] = h->cell.A[a][b];
+  double aw = h->prm.ewald_alpha, pt = (M_PI / aw) * (M_PI / aw);
+  dim3 G = grid_for(h->ntot, 256);
+  NB_LAUNCH((k_grid_virial<T>), G, dim3(256), 0, h->K[0], h->K[1], h->K[2], (const CX<T> *)fpg,
+            polar ? (const CX<T> *)g : (const CX<T> *)nullptr, (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_recip,
+            A, pt, h->prm.electric, nb_vir_region(h, G.x));
 }
 
 static bool mixed_grid(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXED; }
@@ -1190,4 +1285,12 @@ extern "C" int nb_test_convolution(double *data, const double *recip, int nx, in
     if (h.stream) hipStreamDestroy(h.stream);
     return NB_ERR_HIP;
   }
+}
+
+// reciprocal grid part of the strain derivative (main stream, after the
+// induced potentials have been gathered)
+void nb_grid_virial(NbHandle *h, bool polar) {
+  if (!h->d_fgrid) HIPCHECK(hipMalloc(&h->d_fgrid, sizeof(double) * 2 * h->ntot));
+  if (mixed_grid(h)) grid_virial_t<float>(h, polar, (const CX<float> *)h->d_twf);
+  else grid_virial_t<double>(h, polar, (const CX<double> *)h->d_tw);
 }

@@ -272,9 +272,11 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
                                const double *__restrict__ phip, const double *__restrict__ phiu,
                                const double *__restrict__ phiup, const double *__restrict__ u,
                                const double *__restrict__ up, Frac2 fr, double el, double aw, int polar, int mutual,
-                               double *__restrict__ grad, double *__restrict__ torque, double *__restrict__ energy) {
+                               double *__restrict__ grad, double *__restrict__ torque, double *__restrict__ energy,
+                               double *__restrict__ vpart) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double eself = 0, erec = 0, qt = 0, eselfi = 0, ereci = 0;
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (s < n) {
     const double *m = gmp + 10 * s, *fm = fmp + 10 * s;
     const double *fp = phip + 30 * (size_t)s, *ph = fp + 20;
@@ -340,7 +342,37 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
       tq[2] += el * (-m[1] * sp[2] + m[2] * sp[1] - tt * (m[4] * sp[7] + m[7] * sp[5] + m[8] * sp[9] - m[7] * sp[4] - m[5] * sp[7] - m[9] * sp[8]));
     }
     for (int a = 0; a < 3; a++) { grad[3 * s + a] = df[a]; torque[3 * s + a] = tq[a]; }
+    if (vpart) {
+      // Strain derivative with the global multipoles held fixed: the wave
+      // vectors' strain dependence gives -d_b dphi_a - (2/3) Q_bc dphi_ac per
+      // atom; polarization adds the same for u + uCR in the permanent
+      // potential, the multipoles in the potentials of u + uCR and (mutual)
+      // u / uCR in each other's potential, each with weight 1/2.
+      double Q[3][3] = {{m[4], m[7], m[8]}, {m[7], m[5], m[9]}, {m[8], m[9], m[6]}};
+      double H[3][3] = {{ph[4], ph[7], ph[8]}, {ph[7], ph[5], ph[9]}, {ph[8], ph[9], ph[6]}};
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          double t = m[1 + b] * ph[1 + a];
+          for (int c = 0; c < 3; c++) t += 2.0 / 3.0 * Q[b][c] * H[a][c];
+          w9[3 * a + b] -= el * t;
+        }
+      if (polar) {
+        const double *ui = u + 3 * s, *vi = up + 3 * s;
+        const double *cu = phiu + 30 * (size_t)s + 20, *cv = phiup + 30 * (size_t)s + 20;
+        double hw[6];
+        for (int t = 0; t < 6; t++) hw[t] = cu[4 + t] + cv[4 + t];
+        double Hw[3][3] = {{hw[0], hw[3], hw[4]}, {hw[3], hw[1], hw[5]}, {hw[4], hw[5], hw[2]}};
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) {
+            double t = (ui[b] + vi[b]) * ph[1 + a] + m[1 + b] * (cu[1 + a] + cv[1 + a]);
+            for (int c = 0; c < 3; c++) t += 2.0 / 3.0 * Q[b][c] * Hw[a][c];
+            if (mutual) t += vi[b] * cu[1 + a] + ui[b] * cv[1 + a];
+            w9[3 * a + b] -= 0.5 * el * t;
+          }
+      }
+    }
   }
+  if (vpart) block_store9(w9, vpart);
   block_add(eself, energy + E_PERM_SELF);
   block_add(0.5 * el * erec, energy + E_PERM_RECIP);
   block_add(qt, energy + E_QTOT);
@@ -365,96 +397,126 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
                            const double *__restrict__ grad_s, const double *__restrict__ torque_s,
                            const double *__restrict__ grad_r, const double *__restrict__ torque_r, int recip,
                            const double *__restrict__ gred_s, int do_vdw, int do_pme, double *__restrict__ grad,
-                           int32_t *__restrict__ flags) {
+                           int32_t *__restrict__ flags, double *__restrict__ vpart) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int i = perm[s];
-  V3<double> g = do_pme ? ld3d(grad_s, s) : V3<double>{0, 0, 0};
-  if (do_pme && recip) g += ld3d(grad_r, s);
-  if (do_vdw) {
-    V3<double> gv = ld3d(gred_s, s);
-    int h = redidx[i];
-    double f = redval[i];
-    if (h >= 0 && h != i && f > 0.0) {
-      g += f * gv;
-      aadd(grad, h, (1.0 - f) * gv);
-    } else {
-      g += gv;
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < n) {
+    int i = perm[s];
+    V3<double> g = do_pme ? ld3d(grad_s, s) : V3<double>{0, 0, 0};
+    if (do_pme && recip) g += ld3d(grad_r, s);
+    if (do_vdw) {
+      V3<double> gv = ld3d(gred_s, s);
+      int h = redidx[i];
+      double f = redval[i];
+      if (h >= 0 && h != i && f > 0.0) {
+        g += f * gv;
+        aadd(grad, h, (1.0 - f) * gv);
+      } else {
+        g += gv;
+      }
+    }
+    aadd(grad, i, g);
+    int fr = frame[i];
+    if (do_pme && fr != NB_FRAME_NONE) {
+      V3<double> trq = ld3d(torque_s, s);
+      if (recip) trq += ld3d(torque_r, s);
+      if (!isfinite(trq.x) || !isfinite(trq.y) || !isfinite(trq.z)) atomicOr(&flags[4], 1);
+      int iz = axis[3 * i], ix = axis[3 * i + 1], iy = axis[3 * i + 2];
+      V3<double> o = ld3d(xyz, i);
+      V3<double> u = ld3d(xyz, iz) - o;
+      V3<double> vz = u, vx{0, 0, 0}, vy{0, 0, 0};  // frame vectors (strain term)
+      double ru = sqrt(dot(u, u));
+      V3<double> v, w;
+      if (fr != NB_FRAME_ZONLY) {
+        v = ld3d(xyz, ix) - o;
+        vx = v;
+      } else {
+        v = V3<double>{1.0, 0.0, 0.0};
+        if (fabs(u.x / ru) > 0.866) v = V3<double>{0.0, 1.0, 0.0};
+      }
+      double rv = sqrt(dot(v, v));
+      if (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD) {
+        w = ld3d(xyz, iy) - o;
+        vy = w;
+      } else {
+        w = cross(u, v);
+      }
+      double rw = sqrt(dot(w, w));
+      u = (1.0 / ru) * u; v = (1.0 / rv) * v; w = (1.0 / rw) * w;
+      V3<double> uv = unit_d(cross(v, u)), uw = unit_d(cross(w, u)), vw = unit_d(cross(w, v));
+      double cuv = dot(u, v), suv = sqrt(1.0 - cuv * cuv);
+      double dpu = -dot(trq, u), dpv = -dot(trq, v), dpw = -dot(trq, w);
+      V3<double> gz{0, 0, 0}, gx{0, 0, 0}, gy{0, 0, 0};
+      if (fr == NB_FRAME_ZONLY) {
+        gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
+      } else if (fr == NB_FRAME_ZTHENX) {
+        gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
+        gx = (-dpu / (rv * suv)) * uv;
+      } else if (fr == NB_FRAME_BISECTOR) {
+        gz = (dpv / (ru * suv)) * uv + (0.5 * dpw / ru) * uw;
+        gx = (-dpu / (rv * suv)) * uv + (0.5 * dpw / rv) * vw;
+      } else if (fr == NB_FRAME_ZTHENBISECTOR) {
+        V3<double> r = v + w;
+        V3<double> sv = unit_d(cross(u, r));
+        r = unit_d(r);
+        V3<double> ur = unit_d(cross(r, u)), us = unit_d(cross(sv, u));
+        double cur = dot(u, r), sur = sqrt(1.0 - cur * cur);
+        double cvs = dot(v, sv), svs = sqrt(1.0 - cvs * cvs);
+        double cws = dot(w, sv), sws = sqrt(1.0 - cws * cws);
+        V3<double> t1 = unit_d(v - cvs * sv), t2 = unit_d(w - cws * sv);
+        double ut1c = dot(u, t1), ut1s = sqrt(1.0 - ut1c * ut1c);
+        double ut2c = dot(u, t2), ut2s = sqrt(1.0 - ut2c * ut2c);
+        double dpr = -dot(trq, r), dps = -dot(trq, sv);
+        gz = (dpr / (ru * sur)) * ur + (dps / ru) * us;
+        gx = (dpu / (rv * (ut1s + ut2s))) * (svs * sv - cvs * t1);
+        gy = (dpu / (rw * (ut1s + ut2s))) * (sws * sv - cws * t2);
+      } else if (fr == NB_FRAME_THREEFOLD) {
+        V3<double> sum = u + v + w;
+        double rs = sqrt(dot(sum, sum));
+        sum = (1.0 / rs) * sum;
+        V3<double> ax[3] = {u, v, w};
+        double rr3[3] = {ru, rv, rw};
+        V3<double> out[3];
+        for (int a = 0; a < 3; a++) {
+          V3<double> e = ax[a];
+          V3<double> r = unit_d(ax[(a + 1) % 3] + ax[(a + 2) % 3]);
+          double c = dot(r, e), sn = sqrt(1.0 - c * c);
+          double dpr = -dot(trq, r);
+          V3<double> del = unit_d(cross(r, e));
+          double dpd = -dot(trq, del);
+          V3<double> eps = cross(del, e);
+          out[a] = (dpr / (rr3[a] * sn)) * del + (dpd * dot(e, sum) / (rr3[a] * rs)) * eps;
+        }
+        gz = out[0]; gx = out[1]; gy = out[2];
+      }
+      aadd(grad, iz, gz);
+      if (ix >= 0 && fr != NB_FRAME_ZONLY) aadd(grad, ix, gx);
+      if (iy >= 0 && (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD)) aadd(grad, iy, gy);
+      aadd(grad, i, -(gz + gx + gy));
+      if (vpart) {
+        // frame rotation under strain: frame-atom gradients against the frame vectors
+        double G3[3][3] = {{gz.x, gz.y, gz.z}, {gx.x, gx.y, gx.z}, {gy.x, gy.y, gy.z}};
+        double V3a[3][3] = {{vz.x, vz.y, vz.z}, {vx.x, vx.y, vx.z}, {vy.x, vy.y, vy.z}};
+        for (int j = 0; j < 3; j++)
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) w9[3 * a + b] += G3[j][a] * V3a[j][b];
+      }
     }
   }
-  aadd(grad, i, g);
-  if (!do_pme) return;
-  int fr = frame[i];
-  if (fr == NB_FRAME_NONE) return;
-  V3<double> trq = ld3d(torque_s, s);
-  if (recip) trq += ld3d(torque_r, s);
-  if (!isfinite(trq.x) || !isfinite(trq.y) || !isfinite(trq.z)) atomicOr(&flags[4], 1);
-  int iz = axis[3 * i], ix = axis[3 * i + 1], iy = axis[3 * i + 2];
-  V3<double> o = ld3d(xyz, i);
-  V3<double> u = ld3d(xyz, iz) - o;
-  double ru = sqrt(dot(u, u));
-  V3<double> v, w;
-  if (fr != NB_FRAME_ZONLY) {
-    v = ld3d(xyz, ix) - o;
-  } else {
-    v = V3<double>{1.0, 0.0, 0.0};
-    if (fabs(u.x / ru) > 0.866) v = V3<double>{0.0, 1.0, 0.0};
+  if (vpart) block_store9(w9, vpart);
+}
+
+__global__ void k_vir_reduce(long nblk, const double *__restrict__ part, double *__restrict__ vir) {
+  __shared__ double red[9][4];
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = 0; q < 9; q++) {
+    double t = 0.0;
+    for (long b = threadIdx.x; b < nblk; b += blockDim.x) t += part[9 * b + q];
+    t = wave_sum_d(t);
+    if (lane == 0) red[q][wv] = t;
   }
-  double rv = sqrt(dot(v, v));
-  if (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD) w = ld3d(xyz, iy) - o;
-  else w = cross(u, v);
-  double rw = sqrt(dot(w, w));
-  u = (1.0 / ru) * u; v = (1.0 / rv) * v; w = (1.0 / rw) * w;
-  V3<double> uv = unit_d(cross(v, u)), uw = unit_d(cross(w, u)), vw = unit_d(cross(w, v));
-  double cuv = dot(u, v), suv = sqrt(1.0 - cuv * cuv);
-  double dpu = -dot(trq, u), dpv = -dot(trq, v), dpw = -dot(trq, w);
-  V3<double> gz{0, 0, 0}, gx{0, 0, 0}, gy{0, 0, 0};
-  if (fr == NB_FRAME_ZONLY) {
-    gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
-  } else if (fr == NB_FRAME_ZTHENX) {
-    gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
-    gx = (-dpu / (rv * suv)) * uv;
-  } else if (fr == NB_FRAME_BISECTOR) {
-    gz = (dpv / (ru * suv)) * uv + (0.5 * dpw / ru) * uw;
-    gx = (-dpu / (rv * suv)) * uv + (0.5 * dpw / rv) * vw;
-  } else if (fr == NB_FRAME_ZTHENBISECTOR) {
-    V3<double> r = v + w;
-    V3<double> sv = unit_d(cross(u, r));
-    r = unit_d(r);
-    V3<double> ur = unit_d(cross(r, u)), us = unit_d(cross(sv, u));
-    double cur = dot(u, r), sur = sqrt(1.0 - cur * cur);
-    double cvs = dot(v, sv), svs = sqrt(1.0 - cvs * cvs);
-    double cws = dot(w, sv), sws = sqrt(1.0 - cws * cws);
-    V3<double> t1 = unit_d(v - cvs * sv), t2 = unit_d(w - cws * sv);
-    double ut1c = dot(u, t1), ut1s = sqrt(1.0 - ut1c * ut1c);
-    double ut2c = dot(u, t2), ut2s = sqrt(1.0 - ut2c * ut2c);
-    double dpr = -dot(trq, r), dps = -dot(trq, sv);
-    gz = (dpr / (ru * sur)) * ur + (dps / ru) * us;
-    gx = (dpu / (rv * (ut1s + ut2s))) * (svs * sv - cvs * t1);
-    gy = (dpu / (rw * (ut1s + ut2s))) * (sws * sv - cws * t2);
-  } else if (fr == NB_FRAME_THREEFOLD) {
-    V3<double> sum = u + v + w;
-    double rs = sqrt(dot(sum, sum));
-    sum = (1.0 / rs) * sum;
-    V3<double> ax[3] = {u, v, w};
-    double rr3[3] = {ru, rv, rw};
-    V3<double> out[3];
-    for (int a = 0; a < 3; a++) {
-      V3<double> e = ax[a];
-      V3<double> r = unit_d(ax[(a + 1) % 3] + ax[(a + 2) % 3]);
-      double c = dot(r, e), sn = sqrt(1.0 - c * c);
-      double dpr = -dot(trq, r);
-      V3<double> del = unit_d(cross(r, e));
-      double dpd = -dot(trq, del);
-      V3<double> eps = cross(del, e);
-      out[a] = (dpr / (rr3[a] * sn)) * del + (dpd * dot(e, sum) / (rr3[a] * rs)) * eps;
-    }
-    gz = out[0]; gx = out[1]; gy = out[2];
-  }
-  aadd(grad, iz, gz);
-  if (ix >= 0 && fr != NB_FRAME_ZONLY) aadd(grad, ix, gx);
-  if (iy >= 0 && (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD)) aadd(grad, iy, gy);
-  aadd(grad, i, -(gz + gx + gy));
+  __syncthreads();
+  if (threadIdx.x < 9) vir[threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
 }
 
 }  // namespace
@@ -594,7 +656,8 @@ void nb_recip_energy(NbHandle *h, bool polar) {
     for (int b = 0; b < 3; b++) fr.a[a][b] = h->afrac[a][b];
   NB_LAUNCH(k_recip_energy, grid_for(n, 128), dim3(128), 0, n, h->d_gmp, h->d_fmp, h->d_phip,
                      h->d_phiu, h->d_phiup, h->d_u, h->d_up, fr, h->prm.electric, h->prm.ewald_alpha, (int)polar,
-                     (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_grad_r, h->d_torque_r, h->d_energy);
+                     (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_grad_r, h->d_torque_r, h->d_energy,
+                     h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr);
 }
 
 void nb_finalize_gradient(NbHandle *h) {
@@ -602,5 +665,19 @@ void nb_finalize_gradient(NbHandle *h) {
   int recip = (!h->cell.aperiodic && h->prm.ewald_alpha > 0.0) ? 1 : 0;
   NB_LAUNCH(k_finalize, grid_for(n, 128), dim3(128), 0, n, h->d_perm, h->d_xyz, h->d_frame,
                      h->d_axis, h->d_redidx, h->d_redval, h->d_grad_s, h->d_torque_s, h->d_grad_r, h->d_torque_r,
-                     recip, h->d_gred_s, (int)h->prm.do_vdw, (int)h->prm.do_pme, h->d_grad, h->d_flags);
+                     recip, h->d_gred_s, (int)h->prm.do_vdw, (int)h->prm.do_pme, h->d_grad, h->d_flags,
+                     h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr);
+}
+
+// next free region of nblocks x 9 partial sums in d_vpart (host bookkeeping;
+// the regions are laid out in launch order and reset every evaluation)
+double *nb_vir_region(NbHandle *h, long nblocks) {
+  if (h->vpart_used + nblocks > h->vpart_cap) throw std::runtime_error("virial partial-sum buffer too small");
+  double *p = h->d_vpart + 9 * h->vpart_used;
+  h->vpart_used += nblocks;
+  return p;
+}
+
+void nb_vir_reduce(NbHandle *h) {
+  NB_LAUNCH(k_vir_reduce, dim3(1), dim3(256), 0, h->vpart_used, h->d_vpart, h->d_vir);
 }

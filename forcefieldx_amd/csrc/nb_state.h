@@ -110,6 +110,14 @@ struct NbHandle {
   double *d_scal = nullptr;       // PCG scalars
   double *d_dip = nullptr;        // [n] packed dipole records of the field kernels (96 B each)
   double *d_part = nullptr;       // [S_NSLOT][part_stride] per-block partial dot products
+  // strain derivative (NB_VIRIAL): each contributing kernel writes 9 partial
+  // sums per block into its own region of d_vpart; k_vir_reduce sums them
+  double *d_vpart = nullptr;
+  long vpart_cap = 0, vpart_used = 0;
+  double *d_vir = nullptr;        // [9]
+  double *d_fgrid = nullptr;      // forward transform of the permanent grid (virial)
+  bool want_vir = false, vir_valid = false;
+  double vir[9];
   int part_stride = 0;
 
   // outputs
@@ -157,6 +165,9 @@ void nb_real_energy(NbHandle *h, bool polar);
 void nb_vdw(NbHandle *h);
 void nb_finalize_gradient(NbHandle *h);
 void nb_zero_outputs(NbHandle *h);
+double *nb_vir_region(NbHandle *h, long nblocks);
+void nb_grid_virial(NbHandle *h, bool polar);
+void nb_vir_reduce(NbHandle *h);
 
 #define HIPCHECK(x)                                                                  \
   do {                                                                               \

@@ -151,9 +151,20 @@ class ForceFieldEnergy:
         self._handle.close()
 
     # -- internals -----------------------------------------------------------
-    def _evaluate(self, gradient):
+    def energyGradientAndVirial(self, x):
+        """Energy, gradient and the strain derivative W_ab = dE/d eps_ab
+        (kcal/mol).  The reference has no analytic virial (it differentiates
+        the lattice numerically, XtalEnergy.java:495-511)."""
+        self.setCoordinates(x)
+        self._evaluate(True, virial=True)
+        if self._fallback_grad is not None:
+            raise EnergyException("no virial after the direct-polarization fallback")
+        return self._last["total"], self._handle.gradient().reshape(-1), self._handle.virial()
+
+    def _evaluate(self, gradient, virial=False):
         self._fallback_grad = None
-        e, rc = self._handle.evaluate(ffi.NB_GRAD if gradient else 0, check=False)
+        flags = (ffi.NB_GRAD if gradient else 0) | (ffi.NB_VIRIAL if virial else 0)
+        e, rc = self._handle.evaluate(flags, check=False)
         if rc in (ffi.NB_ERR_SCF_DIVERGED, ffi.NB_ERR_SCF_MAXITER) and self.direct_fallback:
             # direct polarization for this evaluation only; the mutual handle is kept
             p = ffi.params_from(self.params, self.grid, self._nbp.precision, self._nbp.device)

@@ -175,6 +175,13 @@ class Handle:
         self._check(self.L.nb_get_gradient(self.h, _p(g, C.c_double)))
         return g.reshape(self.n, 3)
 
+    def virial(self):
+        """Strain derivative W_ab = dE/d eps_ab (kcal/mol, 3x3) of the last
+        evaluation made with NB_VIRIAL."""
+        v = np.zeros(9)
+        self._check(self.L.nb_get_virial(self.h, _p(v, C.c_double)))
+        return v.reshape(3, 3)
+
     def gradient_device_ptr(self):
         return self.L.nb_gradient_device(self.h)
 

@@ -147,3 +147,44 @@ def test_dense_cluster_regrows_filtered_rows(oracle, water648):
     assert abs(e["total"] - eo["total"]) <= REL * abs(eo["total"])
     assert rel_err(g, go) <= REL
     assert rel_err(u, uo) <= REL
+
+
+def _strained(xyz, box, eps):
+    from test_oracle_physics import strained
+    return strained(xyz, box, eps)
+
+
+@pytest.mark.parametrize("pol", [0, 1, 2])
+@pytest.mark.parametrize("shear", [0.0, 0.12])
+def test_virial_fp64_matches_oracle(oracle, water648, pol, shear):
+    """FP64 virial (dE/d strain) of the library against the oracle's analytic
+    virial, which is pinned by box-strain central differences; cubic and
+    sheared (triclinic) cells."""
+    xyz, box, top = water648
+    S = np.zeros((3, 3))
+    S[0, 1] = S[1, 0] = shear / 2
+    S[0, 2] = S[2, 0] = -shear / 3
+    xyz, box = _strained(xyz, list(box), S)
+    p = NonbondedParams(vdw_cutoff=8.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-6, polarization=pol)
+    print(f"virial FP64 polarization={pol} shear={shear}", flush=True)
+    eo = oracle.evaluate(oracle.System(xyz, box, top), p)[0]
+    ffe = ForceFieldEnergy(top, box, p, precision=ffi.NB_PREC_FP64)
+    e, g, W = ffe.energyGradientAndVirial(xyz.reshape(-1))
+    ffe.destroy()
+    scale = np.abs(eo["vir"]).max()
+    assert np.abs(W - eo["vir"]).max() <= REL * scale, (W, eo["vir"])
+    # antisymmetric part: rotation variance of dipoles converged to 1e-6 D only
+    assert np.abs(W - W.T).max() <= 1e-7 * scale
+
+
+def test_virial_mixed_against_fp64(water648):
+    xyz, box, top = water648
+    p = NonbondedParams(vdw_cutoff=9.0, grid=(24, 24, 24), polar_eps=1e-6)
+    out = {}
+    for prec in (ffi.NB_PREC_FP64, ffi.NB_PREC_MIXED):
+        print(f"virial precision={prec}", flush=True)
+        ffe = ForceFieldEnergy(top, box, p, precision=prec)
+        out[prec] = ffe.energyGradientAndVirial(xyz)[2]
+        ffe.destroy()
+    w64, wmx = out[ffi.NB_PREC_FP64], out[ffi.NB_PREC_MIXED]
+    assert np.abs(wmx - w64).max() <= 1e-5 * np.abs(w64).max()
