@@ -64,6 +64,9 @@ class NonbondedParams:
     lambda_value: float = 1.0
     vdw_lambda_alpha: float = 0.25     # VanDerWaals.java:599-624
     vdw_lambda_exponent: float = 3.0
+    vdw_lambda_end: float = 1.0        # VanDerWaals.java:605-609
+    intermolecular_softcore: bool = False
+    intramolecular_softcore: bool = False
     perm_lambda_alpha: float = 2.0     # AlchemicalParameters.java:171-249
     perm_lambda_exponent: float = 3.0
     perm_lambda_start: float = 0.4

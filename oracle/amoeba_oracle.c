@@ -1779,7 +1779,40 @@ void ora_reduced_xyz(const ora_system *s, double *red) {
   }
 }
 
-static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir) {
+/* Softcore factors of VanDerWaals.setLambda (:599-653): lambda is rescaled by
+ * the end point, sc1 = alpha (1 - L)^2 shifts both buffered denominators and
+ * sc2 = L^beta scales eps; derivatives are with respect to the rescaled L. */
+typedef struct { double sc1, dsc1, d2sc1, sc2, dsc2, d2sc2; } softcore_t;
+
+static softcore_t softcore_factors(const ora_params *prm) {
+  softcore_t f;
+  double L = prm->lambda, end = prm->vdw_lambda_end > 0.0 ? prm->vdw_lambda_end : 1.0;
+  L = L > end ? 1.0 : L / end;
+  double a = prm->vdw_lambda_alpha, b = prm->vdw_lambda_exponent;
+  f.sc1 = a * (1.0 - L) * (1.0 - L);
+  f.dsc1 = -2.0 * a * (1.0 - L);
+  f.d2sc1 = 2.0 * a;
+  f.sc2 = pow(L, b);
+  f.dsc2 = b * pow(L, b - 1.0);
+  f.d2sc2 = b >= 2.0 ? b * (b - 1.0) * pow(L, b - 2.0) : 0.0;
+  return f;
+}
+
+/* softcore pair: exactly one atom soft, or both soft under the inter- /
+ * intramolecular flags (VanDerWaals.java:1689-1737) */
+static int soft_pair(const ora_system *s, const ora_params *prm, int i, int k) {
+  if (!s->soft) return 0;
+  int si = s->soft[i] != 0, sk = s->soft[k] != 0;
+  if (si != sk) return 1;
+  if (si && sk) {
+    int same = s->molecule[i] == s->molecule[k];
+    return (prm->intermolecular_softcore && !same) || (prm->intramolecular_softcore && same);
+  }
+  return 0;
+}
+
+static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir, double *dudl, double *d2udl2,
+                         double *lgrad) {
   const ora_system *s = c->s;
   const ora_params *prm = c->prm;
   int n = c->n, nth = c->nth;
@@ -1794,17 +1827,18 @@ static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir) {
   double c4 = 15.0 * (a + b) / den, c5 = -6.0 / den;
   double dl = prm->halgren_delta, gm = prm->halgren_gamma;
   double t1n = pow(1.0 + dl, 7.0), g1 = 1.0 + gm;
-  double e = 0.0;
+  softcore_t sf = softcore_factors(prm);
+  double e = 0.0, el1 = 0.0, el2 = 0.0;
   int cnt = 0;
   double w[9] = {0};
-  double *acc = calloc((size_t)nth * 3 * n, sizeof(double));
-  #pragma omp parallel num_threads(nth) reduction(+ : e, cnt, w[:9])
+  double *acc = calloc((size_t)nth * 6 * n, sizeof(double));
+  #pragma omp parallel num_threads(nth) reduction(+ : e, cnt, w[:9], el1, el2)
   {
     int tid = 0;
 #ifdef _OPENMP
     tid = omp_get_thread_num();
 #endif
-    double *ag = acc + (size_t)tid * 3 * n;
+    double *ag = acc + (size_t)tid * 6 * n, *al = ag + 3 * n;
     double *mask = malloc(n * sizeof(double));
     for (int k = 0; k < n; k++) mask[k] = 1.0;
     #pragma omp for schedule(dynamic, 32)
@@ -1826,13 +1860,15 @@ static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir) {
         double irv = pe[0];
         if (!(r2 <= off2 && mask[k] > 0.0 && irv > 0.0)) continue;
         double r = sqrt(r2);
+        int soft = soft_pair(s, prm, i, k);
+        double sc1 = soft ? sf.sc1 : 0.0, sc2 = soft ? sf.sc2 : 1.0;
         double ev = mask[k] * pe[1];
         double rho = r * irv;
         double rho6 = pow(rho, 6.0), rho7 = rho6 * rho;
         double rd = rho + dl, rd6 = pow(rd, 6.0), rd7 = rd6 * rd;
-        double t1d = 1.0 / rd7, t2d = 1.0 / (rho7 + gm);
+        double t1d = 1.0 / (sc1 + rd7), t2d = 1.0 / (sc1 + rho7 + gm);
         double t1 = t1n * t1d, t2a = g1 * t2d, t2 = t2a - 2.0;
-        double eik = ev * t1 * t2;
+        double eik = sc2 * ev * t1 * t2;
         double tp = 1.0, dtp = 0.0;
         if (r2 > cut2) {
           double r3 = r2 * r, r4 = r2 * r2, r5 = r2 * r3;
@@ -1841,30 +1877,55 @@ static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir) {
         }
         e += eik * tp;
         cnt++;
-        if (!grad) continue;
-        double dt1 = t1 * 7.0 * rd6 * irv * t1d;
-        double dt2 = t2a * 7.0 * rho6 * irv * t2d;
-        double dedr = -ev * (dt1 * t2 + t1 * dt2);
-        double ds = eik * dtp + dedr * tp;
+        double dt1 = t1 * 7.0 * rd6 * irv * t1d;   /* -d t1 / dr */
+        double dt2 = t2a * 7.0 * rho6 * irv * t2d; /* -d t2 / dr */
         double ir = 1.0 / r;
+        double dldr = 0.0;
+        if (soft) {
+          /* dU/dL, d2U/dL2 and d(dU/dL)/dr of VanDerWaals.java:1842-1881 */
+          double lt1 = -t1 * t1d * sf.dsc1, lt2 = -t2a * t2d * sf.dsc1;
+          double dedl = ev * (sf.dsc2 * t1 * t2 + sf.sc2 * lt1 * t2 + sf.sc2 * t1 * lt2);
+          el1 += dedl * tp;
+          double t1d2 = -sf.dsc1 * t1d * t1d, t2d2 = -sf.dsc1 * t2d * t2d;
+          double d2t1 = -lt1 * t1d * sf.dsc1 - t1 * t1d * sf.d2sc1 - t1 * t1d2 * sf.dsc1;
+          double d2t2 = -lt2 * t2d * sf.dsc1 - t2a * t2d * sf.d2sc1 - t2a * t2d2 * sf.dsc1;
+          double df1 = sf.d2sc2 * t1 * t2 + sf.dsc2 * lt1 * t2 + sf.dsc2 * t1 * lt2;
+          double df2 = sf.dsc2 * lt1 * t2 + sf.sc2 * d2t1 * t2 + sf.sc2 * lt1 * lt2;
+          double df3 = sf.dsc2 * t1 * lt2 + sf.sc2 * lt1 * lt2 + sf.sc2 * t1 * d2t2;
+          el2 += ev * (df1 + df2 + df3) * tp;
+          double a1 = -sf.dsc2 * t2 * dt1, a2 = -sf.dsc2 * t1 * dt2;
+          double a3 = 2.0 * sf.sc2 * t2 * dt1 * sf.dsc1 * t1d, a4 = 2.0 * sf.sc2 * t1 * dt2 * sf.dsc1 * t2d;
+          double a5 = -sf.sc2 * lt2 * dt1, a6 = -sf.sc2 * lt1 * dt2;
+          double dedldr = ev * (a1 + a2 + a3 + a4 + a5 + a6);
+          dldr = dedl * dtp + dedldr * tp;
+        }
+        if (!grad) continue;
+        double dedr = -sc2 * ev * (dt1 * t2 + t1 * dt2);
+        double ds = eik * dtp + dedr * tp;
         int rk = s->red_index[k];
         double rv_k = s->red_value[k];
         /* pair strain derivative (dE/dr)_a r_b with r = red_k - red_i = -d */
         for (int a2 = 0; a2 < 3; a2++)
           for (int b2 = 0; b2 < 3; b2++) w[3 * a2 + b2] += ds * ir * d[a2] * d[b2];
         for (int q = 0; q < 3; q++) {
-          double de = ds * d[q] * ir;
+          double de = ds * d[q] * ir, dl2 = dldr * d[q] * ir;
           if (ri >= 0 && ri != i && rv_i > 0.0) {
             ag[3 * i + q] += de * rv_i;
             ag[3 * ri + q] += de * (1.0 - rv_i);
+            al[3 * i + q] += dl2 * rv_i;
+            al[3 * ri + q] += dl2 * (1.0 - rv_i);
           } else {
             ag[3 * i + q] += de;
+            al[3 * i + q] += dl2;
           }
           if (rk >= 0 && rk != k && rv_k > 0.0) {
             ag[3 * k + q] -= de * rv_k;
             ag[3 * rk + q] -= de * (1.0 - rv_k);
+            al[3 * k + q] -= dl2 * rv_k;
+            al[3 * rk + q] -= dl2 * (1.0 - rv_k);
           } else {
             ag[3 * k + q] -= de;
+            al[3 * k + q] -= dl2;
           }
         }
       }
@@ -1874,10 +1935,15 @@ static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir) {
   }
   if (grad)
     for (int t = 0; t < nth; t++)
-      for (int q = 0; q < 3 * n; q++) grad[q] += acc[(size_t)t * 3 * n + q];
+      for (int q = 0; q < 3 * n; q++) {
+        grad[q] += acc[(size_t)t * 6 * n + q];
+        if (lgrad) lgrad[q] += acc[(size_t)t * 6 * n + 3 * n + q];
+      }
   free(acc);
   free(red);
   *count = cnt;
+  if (dudl) *dudl += el1;
+  if (d2udl2) *d2udl2 += el2;
   if (vir)
     for (int q = 0; q < 9; q++) vir[q] += w[q];
   return e;
@@ -1926,6 +1992,11 @@ double ora_ewald_recip_exact(const ora_system *s, const ora_params *prm, int kma
 /* ------------------------------------------------------------------------ */
 int ora_evaluate(const ora_system *s, const ora_params *prm, ora_energies *out, double *grad, double *ind,
                  double *indcr) {
+  return ora_evaluate_lambda(s, prm, out, grad, ind, indcr, NULL);
+}
+
+int ora_evaluate_lambda(const ora_system *s, const ora_params *prm, ora_energies *out, double *grad, double *ind,
+                        double *indcr, double *dudx_dl) {
   memset(out, 0, sizeof(*out));
   ctx_t cx;
   memset(&cx, 0, sizeof cx);
@@ -1942,7 +2013,8 @@ int ora_evaluate(const ora_system *s, const ora_params *prm, ora_energies *out, 
   double lcut = (prm->vdw_cutoff > prm->ewald_cutoff ? prm->vdw_cutoff : prm->ewald_cutoff) + prm->list_buffer;
   nlist_build(&c->nl, &c->cell, n, s->xyz, lcut);
 
-  if (prm->do_vdw) out->vdw = vdw_energy(c, c->grad, &out->n_vdw, out->vir);
+  if (dudx_dl) memset(dudx_dl, 0, 3 * (size_t)n * sizeof(double));
+  if (prm->do_vdw) out->vdw = vdw_energy(c, c->grad, &out->n_vdw, out->vir, &out->dudl, &out->d2udl2, dudx_dl);
 
   if (prm->do_pme) {
     c->gmp = malloc(10 * s3 / 3);

@@ -49,6 +49,7 @@ typedef struct {
   const int *mask_atom;     /* [nnz] partner atom */
   const int *mask_rel;      /* [nnz] 2 = 1-2, 3 = 1-3, 4 = 1-4, 5 = 1-5 */
   const double *box;        /* [6] a b c alpha beta gamma (deg); a <= 0: aperiodic */
+  const int *soft;          /* [n] alchemical (softcore) atoms, NULL: none */
 } ora_system;
 
 typedef struct {
@@ -74,6 +75,12 @@ typedef struct {
   int do_pme;
   int nthreads;             /* 0 = OpenMP default */
   int max_scf_iter;         /* 1000 */
+  /* vdW softcore (VanDerWaals.setLambda, VanDerWaals.java:599-653) */
+  double lambda;            /* 1 = fully coupled */
+  double vdw_lambda_alpha;  /* 0.25 */
+  double vdw_lambda_exponent; /* 3 */
+  double vdw_lambda_end;    /* 1 */
+  int intermolecular_softcore, intramolecular_softcore; /* both-soft pairs (0, 0) */
 } ora_params;
 
 typedef struct {
@@ -86,9 +93,13 @@ typedef struct {
   int n_vdw, n_perm;
   int status;               /* 0 ok, 1 nan energy, 2 scf diverged, 3 scf maxiter, 4 bad torque */
   double vir[9];            /* strain derivative W_ab = dE/d(eps_ab), row-major, kcal/mol */
+  double dudl, d2udl2;      /* vdW softcore lambda derivatives */
 } ora_energies;
 
 /* Full evaluation.  grad [3n] (kcal/mol/A), ind, indcr [3n] (e*A) may be NULL. */
+/* As ora_evaluate, plus dU/dX/dlambda [3n] (may be NULL). */
+int ora_evaluate_lambda(const ora_system *sys, const ora_params *prm, ora_energies *out, double *grad,
+                        double *ind, double *indcr, double *dudx_dl);
 int ora_evaluate(const ora_system *sys, const ora_params *prm, ora_energies *out,
                  double *grad, double *ind, double *indcr);
 

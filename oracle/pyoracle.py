@@ -35,7 +35,7 @@ class OraSystem(C.Structure):
                 ("vdw_table", C.POINTER(C.c_double)), ("red_index", C.POINTER(C.c_int)),
                 ("red_value", C.POINTER(C.c_double)), ("mask_off", C.POINTER(C.c_int)),
                 ("mask_atom", C.POINTER(C.c_int)), ("mask_rel", C.POINTER(C.c_int)),
-                ("box", C.POINTER(C.c_double))]
+                ("box", C.POINTER(C.c_double)), ("soft", C.POINTER(C.c_int))]
 
 
 class OraParams(C.Structure):
@@ -46,7 +46,9 @@ class OraParams(C.Structure):
                 ("p_scale", C.c_double * 4), ("intra14", C.c_double), ("d11", C.c_double),
                 ("v_scale", C.c_double * 3), ("halgren_delta", C.c_double), ("halgren_gamma", C.c_double),
                 ("polarization", C.c_int), ("do_vdw", C.c_int), ("do_pme", C.c_int), ("nthreads", C.c_int),
-                ("max_scf_iter", C.c_int)]
+                ("max_scf_iter", C.c_int), ("lambda_", C.c_double), ("vdw_lambda_alpha", C.c_double),
+                ("vdw_lambda_exponent", C.c_double), ("vdw_lambda_end", C.c_double),
+                ("intermolecular_softcore", C.c_int), ("intramolecular_softcore", C.c_int)]
 
 
 class OraEnergies(C.Structure):
@@ -54,7 +56,8 @@ class OraEnergies(C.Structure):
                 ("perm_charge_corr", C.c_double), ("pol_real", C.c_double), ("pol_self", C.c_double),
                 ("pol_recip", C.c_double), ("vdw", C.c_double), ("perm", C.c_double), ("pol", C.c_double),
                 ("total", C.c_double), ("pcg_iters", C.c_int), ("pcg_eps", C.c_double), ("n_vdw", C.c_int),
-                ("n_perm", C.c_int), ("status", C.c_int), ("vir", C.c_double * 9)]
+                ("n_perm", C.c_int), ("status", C.c_int), ("vir", C.c_double * 9),
+                ("dudl", C.c_double), ("d2udl2", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -74,6 +77,9 @@ def lib():
         dp = C.POINTER(C.c_double)
         L.ora_evaluate.argtypes = [C.POINTER(OraSystem), C.POINTER(OraParams), C.POINTER(OraEnergies), dp, dp, dp]
         L.ora_evaluate.restype = C.c_int
+        L.ora_evaluate_lambda.argtypes = [C.POINTER(OraSystem), C.POINTER(OraParams), C.POINTER(OraEnergies), dp, dp,
+                                          dp, dp]
+        L.ora_evaluate_lambda.restype = C.c_int
         L.ora_rotate_multipoles.argtypes = [C.POINTER(OraSystem), dp]
         L.ora_bspline.argtypes = [C.c_double, C.c_int, C.c_int, dp]
         L.ora_perm_pair.argtypes = [dp, dp, dp, C.c_double, C.c_double, dp, dp, dp]
@@ -118,13 +124,14 @@ class System:
             red_index=np.ascontiguousarray(t.red_index, np.int32),
             red_value=np.ascontiguousarray(t.red_value, np.float64),
             mask_off=np.ascontiguousarray(t.mask_off, np.int32), mask_atom=np.ascontiguousarray(t.mask_atom, np.int32),
-            mask_rel=np.ascontiguousarray(t.mask_rel, np.int32))
+            mask_rel=np.ascontiguousarray(t.mask_rel, np.int32),
+            soft=np.ascontiguousarray(t.soft if getattr(t, "soft", None) is not None else np.zeros(t.n), np.int32))
         k = self._keep
         self.c = OraSystem(t.n, _dp(self.xyz), _ip(k["frame"]), _ip(k["axis"]), _dp(k["mpole"]), _dp(k["polarity"]),
                            _dp(k["thole"]), _dp(k["pdamp"]), _ip(k["group"]), _ip(k["molecule"]),
                            _ip(k["vdw_class"]), int(t.vdw_table.shape[0]), _dp(k["vdw_table"]),
                            _ip(k["red_index"]), _dp(k["red_value"]), _ip(k["mask_off"]), _ip(k["mask_atom"]),
-                           _ip(k["mask_rel"]), _dp(self.box))
+                           _ip(k["mask_rel"]), _dp(self.box), _ip(k["soft"]))
 
     def set_xyz(self, xyz):
         self.xyz[:] = np.asarray(xyz, np.float64).reshape(-1)
@@ -137,7 +144,9 @@ def make_params(p, box=None, nthreads=0):
                      (C.c_int * 3)(*grid), p.polar_eps, p.electric, p.precond_cutoff, p.precond_scale,
                      (C.c_double * 4)(*p.m_scale), (C.c_double * 4)(*p.p_scale), p.intra14, p.d11,
                      (C.c_double * 3)(*p.v_scale), p.halgren_delta, p.halgren_gamma, p.polarization,
-                     int(p.do_vdw), int(p.do_pme), nthreads, p.max_scf_iter)
+                     int(p.do_vdw), int(p.do_pme), nthreads, p.max_scf_iter, p.lambda_value, p.vdw_lambda_alpha,
+                     p.vdw_lambda_exponent, p.vdw_lambda_end, int(p.intermolecular_softcore),
+                     int(p.intramolecular_softcore))
 
 
 def evaluate(system, params, nthreads=0):
@@ -150,6 +159,16 @@ def evaluate(system, params, nthreads=0):
     up = np.zeros(3 * n)
     lib().ora_evaluate(C.byref(system.c), C.byref(prm), C.byref(out), _dp(g), _dp(u), _dp(up))
     return out.as_dict(), g.reshape(n, 3), u.reshape(n, 3), up.reshape(n, 3)
+
+
+def evaluate_lambda(system, params, nthreads=0):
+    """As evaluate, plus dU/dX/dlambda (n,3); energies carry dudl, d2udl2."""
+    n = system.top.n
+    prm = make_params(params, system.box, nthreads) if not isinstance(params, OraParams) else params
+    out = OraEnergies()
+    g, u, up, lg = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+    lib().ora_evaluate_lambda(C.byref(system.c), C.byref(prm), C.byref(out), _dp(g), _dp(u), _dp(up), _dp(lg))
+    return out.as_dict(), g.reshape(n, 3), lg.reshape(n, 3)
 
 
 def rotate(system):

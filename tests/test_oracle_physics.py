@@ -284,3 +284,69 @@ def test_virial_central_differences(oracle, pol, shear):
     scale = np.abs(fd).max()
     assert np.abs(W - W.T).max() <= 1e-9 * scale
     assert np.abs(W - fd).max() <= 1e-6 * scale, (W, fd)
+
+
+def soft_water_case(lam):
+    """32 waters with the first two soft (alchemical), vdW only."""
+    xyz, box, top = water_box(32, 9.9, seed=4)
+    top.soft = np.zeros(top.n, np.int32)
+    top.soft[:6] = 1
+    p = NonbondedParams(vdw_cutoff=4.8, ewald_cutoff=4.8, do_pme=False, polarization=0, lambda_value=lam)
+    return xyz.reshape(-1, 3), box, top, p
+
+
+@pytest.mark.parametrize("lam", [0.35, 0.8])
+def test_softcore_vdw_lambda_derivatives(oracle, lam):
+    """dU/dL, d2U/dL2 and dU/dX/dL of the buffered 14-7 softcore against
+    central differences in L and in the coordinates."""
+    xyz, box, top, p = soft_water_case(lam)
+    s = oracle.System(xyz, box, top)
+    e, g, lg = oracle.evaluate_lambda(s, p)
+    h = 1e-5
+    ev = {}
+    for dl in (h, -h):
+        p.lambda_value = lam + dl
+        ev[dl] = oracle.evaluate_lambda(s, p)
+    p.lambda_value = lam
+    fd1 = (ev[h][0]["vdw"] - ev[-h][0]["vdw"]) / (2 * h)
+    fd2 = (ev[h][0]["dudl"] - ev[-h][0]["dudl"]) / (2 * h)
+    assert abs(e["dudl"]) > 1e-2
+    assert abs(e["dudl"] - fd1) <= 1e-6 * max(1.0, abs(e["dudl"]))
+    assert abs(e["d2udl2"] - fd2) <= 1e-5 * max(1.0, abs(e["d2udl2"]))
+    # dU/dX/dL = d/dL of the gradient; the gradient itself by differences in x
+    fdl = (ev[h][1] - ev[-h][1]) / (2 * h)
+    assert np.abs(lg - fdl).max() <= 1e-5 * max(1.0, np.abs(lg).max())
+    assert np.abs(lg[6:]).max() > 0  # hard atoms feel the soft ones
+    rng = np.random.default_rng(5)
+    for idx in rng.choice(3 * top.n, 6, replace=False):
+        x = xyz.reshape(-1).copy()
+        x[idx] += 1e-5
+        s.set_xyz(x)
+        ep = oracle.evaluate_lambda(s, p)[0]["vdw"]
+        x[idx] -= 2e-5
+        s.set_xyz(x)
+        em = oracle.evaluate_lambda(s, p)[0]["vdw"]
+        assert abs((ep - em) / 2e-5 - g.reshape(-1)[idx]) <= 1e-6 * max(1.0, np.abs(g).max())
+    s.set_xyz(xyz)
+
+
+def test_softcore_end_points(oracle):
+    """L = 1 is the plain potential; at L = 0 soft-hard pairs vanish, so the
+    energy equals that of the system with the soft molecules' cross terms
+    removed (evaluated as the hard subsystem plus the isolated soft one)."""
+    xyz, box, top, p = soft_water_case(1.0)
+    e1 = oracle.evaluate_lambda(oracle.System(xyz, box, top), p)[0]["vdw"]
+    top0 = soft_water_case(1.0)[2]
+    top0.soft = None
+    p0 = NonbondedParams(vdw_cutoff=4.8, ewald_cutoff=4.8, do_pme=False, polarization=0)
+    assert abs(e1 - oracle.evaluate(oracle.System(xyz, box, top0), p0)[0]["vdw"]) < 1e-12
+    p.lambda_value = 0.0
+    e0 = oracle.evaluate_lambda(oracle.System(xyz, box, top), p)[0]["vdw"]
+    # soft subsystem alone (both-soft pairs are plain) and hard subsystem alone
+    zs = np.zeros(top.n, bool)
+    zs[:6] = True
+    parts = 0.0
+    for keep in (zs, ~zs):
+        sub = water_topology(int(keep.sum()) // 3)
+        parts += oracle.evaluate(oracle.System(xyz[keep], box, sub), p0)[0]["vdw"]
+    assert abs(e0 - parts) <= 1e-10 * abs(parts)
