@@ -77,7 +77,7 @@ void free_all(NbHandle *h) {
                   h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                  h->d_scal, h->d_part, h->d_dip, h->d_vpart, h->d_vir, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+                  h->d_scal, h->d_part, h->d_dip, h->d_vpart, h->d_vir, h->d_soft0, h->d_mol0, h->d_soft, h->d_mol, h->d_gred_l, h->d_dudx, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -241,6 +241,19 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     upload(&h->d_mask_atom, top->mask_atom, nnz);
     upload(&h->d_mask_rel, top->mask_rel, nnz);
     h->n_class = top->n_vdw_class;
+    h->lambda = params->lambda;
+    if (!(h->lambda >= 0.0 && h->lambda <= 1.0)) throw std::runtime_error("lambda must lie in [0, 1]");
+    if (top->soft) {
+      for (int i = 0; i < n; i++) h->has_soft = h->has_soft || top->soft[i] != 0;
+    }
+    if (h->has_soft) {
+      std::vector<int32_t> mol(n);
+      for (int i = 0; i < n; i++) mol[i] = top->molecule ? top->molecule[i] : i;
+      upload(&h->d_soft0, top->soft, n);
+      upload(&h->d_mol0, mol.data(), n);
+      alloc(&h->d_soft, n);
+      alloc(&h->d_mol, n);
+    }
     upload(&h->d_vdw_table, top->vdw_table, 2 * (size_t)h->n_class * h->n_class);
     size_t n3 = 3 * (size_t)n;
     alloc(&h->d_xyz, n3);
@@ -282,7 +295,7 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     HIPCHECK(hipMemset(h->d_phiup, 0, sizeof(double) * 30 * n));
     double **vecs[] = {&h->d_field, &h->d_fieldp, &h->d_u, &h->d_up, &h->d_udir, &h->d_updir, &h->d_r, &h->d_rp,
                        &h->d_z, &h->d_zp, &h->d_p, &h->d_pp, &h->d_vec, &h->d_vecp, &h->d_grad_s, &h->d_torque_s,
-                       &h->d_gred_s, &h->d_grad, &h->d_grad_r, &h->d_torque_r};
+                       &h->d_gred_s, &h->d_grad, &h->d_grad_r, &h->d_torque_r, &h->d_gred_l, &h->d_dudx};
     for (double **v : vecs) {
       alloc(v, n3);
       HIPCHECK(hipMemset(*v, 0, sizeof(double) * n3));
@@ -355,7 +368,19 @@ int nb_set_positions_device(void *handle, const void *d_xyz) {
 int nb_set_lambda(void *handle, double lambda) {
   NbHandle *h = (NbHandle *)handle;
   if (!h) return NB_ERR_INVALID;
-  if (lambda < 0.0 || lambda > 1.0) return NB_ERR_INVALID;
+  if (!(lambda >= 0.0 && lambda <= 1.0)) {
+    h->err = "lambda must lie in [0, 1]";
+    return NB_ERR_INVALID;
+  }
+  h->lambda = lambda;
+  return NB_OK;
+}
+
+int nb_struct_sizes(int32_t sizes[3]) {
+  if (!sizes) return NB_ERR_INVALID;
+  sizes[0] = (int32_t)sizeof(nb_params);
+  sizes[1] = (int32_t)sizeof(nb_topology);
+  sizes[2] = (int32_t)sizeof(nb_energies);
   return NB_OK;
 }
 
@@ -372,13 +397,15 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
   try {
     HIPCHECK(hipSetDevice(h->device));
     int n = h->n;
-    double en[8];
+    double en[16];
     int64_t cnt[4];
     int32_t fl[8];
     bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
     int rebuilt = 0;
     h->want_vir = (flags & NB_VIRIAL) != 0;
     h->vir_valid = false;
+    h->want_dudl = (flags & NB_DUDL) != 0;
+    h->dudl_valid = false;
     if (h->want_vir) {
       if (h->cell.aperiodic) throw std::runtime_error("the virial needs a periodic cell");
       // 9 partial sums per block of every contributing launch
@@ -445,11 +472,14 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
         HIPCHECK(hipMemcpyAsync(h->h_pinned + 16, h->d_vir, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
       }
       HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipMemcpyAsync(h->h_pinned + 32, h->d_energy + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
       HIPCHECK(hipStreamSynchronize(h->stream));
       if (h->profiling) nb_prof_harvest(h);
-      memcpy(en, h->h_pinned, sizeof(en));
+      memcpy(en, h->h_pinned, 8 * sizeof(double));
+      en[E_DUDL] = h->h_pinned[32];
+      en[E_D2UDL2] = h->h_pinned[33];
       memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
       memcpy(fl, h->h_pinned + 12, sizeof(fl));
       if (!h->prm.do_pme || !nb_grow_filtered_rows(h, fl)) break;
@@ -463,6 +493,9 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
     e.perm_real = en[E_PERM_REAL];
     e.pol_real = en[E_POL_REAL];
     e.vdw = en[E_VDW];
+    e.dudl = en[E_DUDL];
+    e.d2udl2 = en[E_D2UDL2];
+    h->dudl_valid = h->want_dudl;
     if (use_recip && h->prm.do_pme) {
       e.perm_self = en[E_PERM_SELF];
       e.perm_recip = en[E_PERM_RECIP];
@@ -551,8 +584,18 @@ int nb_get_virial(void *handle, double v[9]) {
 int nb_get_dudx_dl(void *handle, double *g) {
   NbHandle *h = (NbHandle *)handle;
   if (!h || !g) return NB_ERR_INVALID;
-  h->err = "dU/dX/dlambda is not implemented";
-  return NB_ERR_INVALID;
+  if (!h->dudl_valid) {
+    h->err = "the last evaluation did not compute lambda derivatives (pass NB_DUDL)";
+    return NB_ERR_INVALID;
+  }
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    HIPCHECK(hipMemcpyAsync(g, h->d_dudx, sizeof(double) * 3 * h->n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    return NB_OK;
+  } catch (std::exception &e) {
+    return fail(h, e, NB_ERR_HIP);
+  }
 }
 
 const char *nb_last_error(void *handle) {

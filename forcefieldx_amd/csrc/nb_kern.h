@@ -6,7 +6,8 @@
 struct ScaleTab { double m[16], p[16], d[16], v[16]; int masked[16]; };
 
 // energy slots of NbHandle::d_energy
-enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT, E_NSLOT };
+enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT, E_DUDL,
+       E_D2UDL2, E_NSLOT };
 // PCG scalar slots of NbHandle::d_scal
 enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_NSLOT };
 
@@ -169,4 +170,26 @@ __device__ __forceinline__ void dip_put_static(void *rec, int mixed, int s, doub
     r->ipd = ipd;
     r->thole = thole;
   }
+}
+
+// vdW softcore factors (VanDerWaals.setLambda, VanDerWaals.java:599-653):
+// lambda is rescaled by the end point, sc1 = alpha (1 - L)^2 shifts both
+// buffered denominators and sc2 = L^beta scales eps; derivatives with respect
+// to the rescaled L
+struct Softcore { double sc1, dsc1, d2sc1, sc2, dsc2, d2sc2; int inter, intra; };
+
+static inline Softcore softcore_of(const NbHandle *h) {
+  Softcore f;
+  double L = h->lambda, end = h->prm.vdw_lambda_end > 0.0 ? h->prm.vdw_lambda_end : 1.0;
+  L = L > end ? 1.0 : L / end;
+  double a = h->prm.vdw_lambda_alpha, b = h->prm.vdw_lambda_exponent;
+  f.sc1 = a * (1.0 - L) * (1.0 - L);
+  f.dsc1 = -2.0 * a * (1.0 - L);
+  f.d2sc1 = 2.0 * a;
+  f.sc2 = pow(L, b);
+  f.dsc2 = b * pow(L, b - 1.0);
+  f.d2sc2 = b >= 2.0 ? b * (b - 1.0) * pow(L, b - 2.0) : 0.0;
+  f.inter = h->prm.intermolecular_softcore;
+  f.intra = h->prm.intramolecular_softcore;
+  return f;
 }

@@ -119,14 +119,19 @@ __global__ void k_iperm(int n, const int32_t *__restrict__ perm, int32_t *__rest
 __global__ void k_gather_static(int n, const int32_t *__restrict__ perm, const double *__restrict__ polar0,
                                 const double *__restrict__ thole0, const double *__restrict__ ipd0,
                                 const int32_t *__restrict__ group0, const int32_t *__restrict__ vclass0,
+                                const int32_t *__restrict__ soft0, const int32_t *__restrict__ mol0,
                                 double *__restrict__ polar, int32_t *__restrict__ group, int32_t *__restrict__ vclass,
-                                AtomMP *__restrict__ mp) {
+                                int32_t *__restrict__ soft, int32_t *__restrict__ mol, AtomMP *__restrict__ mp) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
   polar[s] = polar0[i];
   group[s] = group0[i];
   vclass[s] = vclass0[i];
+  if (soft0) {
+    soft[s] = soft0[i] != 0;
+    mol[s] = mol0[i];
+  }
   mp[s].ipd = ipd0[i];
   mp[s].thole = thole0[i];
 }
@@ -251,7 +256,8 @@ void nb_build_lists(NbHandle *h) {
   NB_LAUNCH(k_cell_rank, dim3(m), B, 0, m, h->d_cellstart, h->d_iperm, h->d_perm);
   NB_LAUNCH(k_iperm, G, B, 0, n, h->d_perm, h->d_iperm);
   NB_LAUNCH(k_gather_static, G, B, 0, n, h->d_perm, h->d_polar0, h->d_thole0, h->d_ipd0,
-                     h->d_group0, h->d_vclass0, h->d_polar, h->d_group, h->d_vclass, h->d_mp);
+                     h->d_group0, h->d_vclass0, h->d_soft0, h->d_mol0, h->d_polar, h->d_group, h->d_vclass, h->d_soft,
+                     h->d_mol, h->d_mp);
   nb_gather_sorted(h);
   // neighbor rows, grown on overflow
   if (h->maxnb == 0) h->maxnb = estimate_neighbors(h, cut);

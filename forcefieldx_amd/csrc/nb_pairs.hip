@@ -315,19 +315,24 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
   if (!POL) block_add_ll(cnt, count + 1);
 }
 
-template <typename T, int LPA, bool VIR>
+// LAM: softcore pairs (exactly one atom soft, or both soft under the inter /
+// intramolecular flags; VanDerWaals.java:1689-1737) with dU/dL, d2U/dL2 and the
+// lambda gradient dU/dX/dL (VanDerWaals.java:1842-1881) on the reduced sites
+template <typename T, int LPA, bool VIR, bool LAM>
 __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ red, const int32_t *__restrict__ vclass,
                                              const uint32_t *__restrict__ nbr, const int32_t *__restrict__ nnb, int maxnb,
                                              CellD cell, const double *__restrict__ vtab, int nclass, ScaleTab sc,
                                              double off, double cut, double dl, double gm, double *__restrict__ gred,
                                              double *__restrict__ energy, int64_t *__restrict__ count,
-                                             double *__restrict__ vpart) {
+                                             double *__restrict__ vpart, const int32_t *__restrict__ soft,
+                                             const int32_t *__restrict__ mol, Softcore sf,
+                                             double *__restrict__ gred_l) {
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
-  double esum = 0.0;
+  double esum = 0.0, el1 = 0.0, el2 = 0.0;
   long long cnt = 0;
-  V3<double> G{0, 0, 0};
+  V3<double> G{0, 0, 0}, GL{0, 0, 0};
   double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (valid) {
     double off2 = off * off, cut2 = cut * cut;
@@ -342,6 +347,7 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
     double4 pi = red[s];
     int ci = vclass[s];
     int m = nnb[s];
+    int softi = LAM ? soft[s] : 0, moli = LAM ? mol[s] : 0;
     const uint32_t *row = nbr + (size_t)s * maxnb;
     for (int t = lane; t < m; t += LPA) {
       uint32_t en = row[t];
@@ -356,14 +362,20 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
       T irv = (T)pe[0];
       if (!(irv > (T)0)) continue;
       cnt++;
+      bool spair = false;
+      if (LAM) {
+        int sk = soft[k];
+        spair = softi != sk || (softi && sk && ((sf.inter && moli != mol[k]) || (sf.intra && moli == mol[k])));
+      }
+      T sc1 = spair ? (T)sf.sc1 : (T)0, sc2 = spair ? (T)sf.sc2 : (T)1;
       T rr = sqrt((T)r2);
       T ev = (T)(vs * pe[1]);
       T rho = rr * irv;
       T rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
       T rd = rho + d1, rd2 = rd * rd, rd6 = rd2 * rd2 * rd2, rd7 = rd6 * rd;
-      T t1d = (T)1 / rd7, t2d = (T)1 / (rho7 + gg);
+      T t1d = (T)1 / (sc1 + rd7), t2d = (T)1 / (sc1 + rho7 + gg);
       T t1 = t1n * t1d, t2a = g1 * t2d, t2 = t2a - (T)2;
-      T eik = ev * t1 * t2;
+      T eik = sc2 * ev * t1 * t2;
       T tp = 1, dtp = 0;
       if (r2 > cut2) {
         T x = (rr - wcut) * iw, x2 = x * x;
@@ -373,9 +385,26 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
       esum += (double)(eik * tp);
       T dt1 = t1 * (T)7 * rd6 * irv * t1d;
       T dt2 = t2a * (T)7 * rho6 * irv * t2d;
-      T dedr = -ev * (dt1 * t2 + t1 * dt2);
+      T dedr = -sc2 * ev * (dt1 * t2 + t1 * dt2);
       T ds = (eik * dtp + dedr * tp) / rr;
       acc3(G, V3<T>{ds * (T)dx, ds * (T)dy, ds * (T)dz});
+      if (LAM && spair) {
+        T dsc1 = (T)sf.dsc1, dsc2 = (T)sf.dsc2, d2sc1 = (T)sf.d2sc1, d2sc2 = (T)sf.d2sc2;
+        T lt1 = -t1 * t1d * dsc1, lt2 = -t2a * t2d * dsc1;
+        T dedl = ev * (dsc2 * t1 * t2 + sc2 * lt1 * t2 + sc2 * t1 * lt2);
+        el1 += (double)(dedl * tp);
+        T t1d2 = -dsc1 * t1d * t1d, t2d2 = -dsc1 * t2d * t2d;
+        T d2t1 = -lt1 * t1d * dsc1 - t1 * t1d * d2sc1 - t1 * t1d2 * dsc1;
+        T d2t2 = -lt2 * t2d * dsc1 - t2a * t2d * d2sc1 - t2a * t2d2 * dsc1;
+        T df1 = d2sc2 * t1 * t2 + dsc2 * lt1 * t2 + dsc2 * t1 * lt2;
+        T df2 = dsc2 * lt1 * t2 + sc2 * d2t1 * t2 + sc2 * lt1 * lt2;
+        T df3 = dsc2 * t1 * lt2 + sc2 * lt1 * lt2 + sc2 * t1 * d2t2;
+        el2 += (double)(ev * (df1 + df2 + df3) * tp);
+        T dedldr = ev * (-dsc2 * t2 * dt1 - dsc2 * t1 * dt2 + (T)2 * sc2 * t2 * dt1 * dsc1 * t1d +
+                         (T)2 * sc2 * t1 * dt2 * dsc1 * t2d - sc2 * lt2 * dt1 - sc2 * lt1 * dt2);
+        T dl2 = (dedl * dtp + dedldr * tp) / rr;
+        acc3(GL, V3<T>{dl2 * (T)dx, dl2 * (T)dy, dl2 * (T)dz});
+      }
       if (VIR) {
         // d = red_i - red_k and the i-side gradient ds d: half of ds d_a d_b per visit
         double dd[3] = {dx, dy, dz}, h = 0.5 * (double)ds;
@@ -387,9 +416,17 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
     }
   }
   group_sum3<LPA>(G);
-  if (valid && lane == 0) st3(gred, s, G);
+  if (LAM) group_sum3<LPA>(GL);
+  if (valid && lane == 0) {
+    st3(gred, s, G);
+    if (LAM) st3(gred_l, s, GL);
+  }
   block_add(0.5 * esum, energy + E_VDW);
   block_add_ll(cnt, count);
+  if (LAM) {
+    block_add(0.5 * el1, energy + E_DUDL);
+    block_add(0.5 * el2, energy + E_D2UDL2);
+  }
   if (VIR) block_store9(w9, vpart);
 }
 
@@ -532,14 +569,20 @@ void nb_vdw(NbHandle *h) {
   double off = h->prm.vdw_cutoff, cut = h->prm.vdw_taper * off;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   bool mixed = h->prm.precision == NB_PREC_MIXED, vir = h->want_vir;
+  // softcore only matters when some atom is soft and lambda < 1 or its derivatives are wanted
+  bool lam = h->has_soft && (h->lambda < 1.0 || h->want_dudl);
   double *vp = vir ? nb_vir_region(h, G.x) : nullptr;
-#define NB_VDW(T, V)                                                                                              \
-  NB_LAUNCH((k_vdw<T, NB_LPA, V>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,       \
+  Softcore sf = softcore_of(h);
+#define NB_VDW(T, V, L)                                                                                           \
+  NB_LAUNCH((k_vdw<T, NB_LPA, V, L>), G, B, 0, n, h->d_red4, h->d_vclass, h->d_nbr, h->d_nnb, h->maxnb, h->cell,    \
             h->d_vdw_table, h->n_class, sc, off, cut, h->prm.halgren_delta, h->prm.halgren_gamma, h->d_gred_s,     \
-            h->d_energy, h->d_count, vp)
-  if (mixed && vir) NB_VDW(float, true);
-  else if (mixed) NB_VDW(float, false);
-  else if (vir) NB_VDW(double, true);
-  else NB_VDW(double, false);
+            h->d_energy, h->d_count, vp, h->d_soft, h->d_mol, sf, h->d_gred_l)
+  if (mixed) {
+    if (lam) { if (vir) NB_VDW(float, true, true); else NB_VDW(float, false, true); }
+    else { if (vir) NB_VDW(float, true, false); else NB_VDW(float, false, false); }
+  } else {
+    if (lam) { if (vir) NB_VDW(double, true, true); else NB_VDW(double, false, true); }
+    else { if (vir) NB_VDW(double, true, false); else NB_VDW(double, false, false); }
+  }
 #undef NB_VDW
 }

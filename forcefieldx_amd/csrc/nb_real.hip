@@ -397,7 +397,8 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
                            const double *__restrict__ grad_s, const double *__restrict__ torque_s,
                            const double *__restrict__ grad_r, const double *__restrict__ torque_r, int recip,
                            const double *__restrict__ gred_s, int do_vdw, int do_pme, double *__restrict__ grad,
-                           int32_t *__restrict__ flags, double *__restrict__ vpart) {
+                           int32_t *__restrict__ flags, double *__restrict__ vpart, const double *__restrict__ gred_l,
+                           double *__restrict__ dudx) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (s < n) {
@@ -408,11 +409,22 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
       V3<double> gv = ld3d(gred_s, s);
       int h = redidx[i];
       double f = redval[i];
-      if (h >= 0 && h != i && f > 0.0) {
+      bool reduced = h >= 0 && h != i && f > 0.0;
+      if (reduced) {
         g += f * gv;
         aadd(grad, h, (1.0 - f) * gv);
       } else {
         g += gv;
+      }
+      if (dudx) {
+        // the lambda gradient follows the same reduced-site chain rule
+        V3<double> gl = ld3d(gred_l, s);
+        if (reduced) {
+          aadd(dudx, i, f * gl);
+          aadd(dudx, h, (1.0 - f) * gl);
+        } else {
+          aadd(dudx, i, gl);
+        }
       }
     }
     aadd(grad, i, g);
@@ -534,9 +546,10 @@ void nb_gather_sorted(NbHandle *h) {
 
 // one launch clears the accumulated outputs of an evaluation
 __global__ void k_zero_outputs(int m, double *__restrict__ a, double *__restrict__ b, double *__restrict__ c,
-                               double *__restrict__ d, double *__restrict__ energy, int64_t *__restrict__ count) {
+                               double *__restrict__ d, double *__restrict__ energy, int64_t *__restrict__ count,
+                               double *__restrict__ e, double *__restrict__ f) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; d[i] = 0.0; }
+  if (i < m) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; d[i] = 0.0; e[i] = 0.0; f[i] = 0.0; }
   if (i < 16) energy[i] = 0.0;
   if (i < 4) count[i] = 0;
 }
@@ -544,7 +557,7 @@ __global__ void k_zero_outputs(int m, double *__restrict__ a, double *__restrict
 void nb_zero_outputs(NbHandle *h) {
   int n3 = 3 * h->n;
   NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_grad_s, h->d_torque_s,
-            h->d_gred_s, h->d_grad, h->d_energy, h->d_count);
+            h->d_gred_s, h->d_grad, h->d_energy, h->d_count, h->d_gred_l, h->d_dudx);
 }
 
 void nb_direct(NbHandle *h) {
@@ -666,7 +679,8 @@ void nb_finalize_gradient(NbHandle *h) {
   NB_LAUNCH(k_finalize, grid_for(n, 128), dim3(128), 0, n, h->d_perm, h->d_xyz, h->d_frame,
                      h->d_axis, h->d_redidx, h->d_redval, h->d_grad_s, h->d_torque_s, h->d_grad_r, h->d_torque_r,
                      recip, h->d_gred_s, (int)h->prm.do_vdw, (int)h->prm.do_pme, h->d_grad, h->d_flags,
-                     h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr);
+                     h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr, h->d_gred_l,
+                     h->want_dudl ? h->d_dudx : (double *)nullptr);
 }
 
 // next free region of nblocks x 9 partial sums in d_vpart (host bookkeeping;

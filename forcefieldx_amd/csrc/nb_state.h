@@ -41,6 +41,13 @@ struct NbHandle {
   int32_t *d_mask_off = nullptr, *d_mask_atom = nullptr, *d_mask_rel = nullptr;
   int n_class = 0;
   double *d_vdw_table = nullptr;
+  int32_t *d_soft0 = nullptr, *d_mol0 = nullptr;   // caller order (soft: null when no atom is soft)
+  int32_t *d_soft = nullptr, *d_mol = nullptr;     // sorted order
+  bool has_soft = false;
+  double lambda = 1.0;
+  double *d_gred_l = nullptr;     // [3n] dU/dX/dlambda on reduced vdW sites (sorted)
+  double *d_dudx = nullptr;       // [3n] dU/dX/dlambda, caller order
+  bool want_dudl = false, dudl_valid = false;
   MaskScales scales;
 
   // cell

@@ -103,6 +103,8 @@ class ForceFieldEnergy:
         self._handle.set_cell(self.box)
         self._x = None
         self._last = None
+        self._lambda = None     # set by setLambda: evaluations then carry dU/dL
+        self._fallback_grad = self._fallback_dudx = None
         self.direct_fallback = direct_fallback
         self.vanDerWaals = VanDerWaals(self)
         self.particleMeshEwald = ParticleMeshEwald(self)
@@ -150,6 +152,30 @@ class ForceFieldEnergy:
     def destroy(self):
         self._handle.close()
 
+    # -- LambdaInterface (ForceFieldEnergy.java setLambda / getdEdL /
+    #    getd2EdL2 / getdEdXdL; the vdW softcore of VanDerWaals.java) --------
+    def setLambda(self, lam):
+        self._handle.set_lambda(lam)
+        self._lambda = float(lam)
+
+    def getLambda(self):
+        return 1.0 if self._lambda is None else self._lambda
+
+    def getdEdL(self):
+        return self._last["dudl"]
+
+    def getd2EdL2(self):
+        return self._last["d2udl2"]
+
+    def getdEdXdL(self, g=None):
+        """Adds dU/dX/dL of the last gradient evaluation into g (allocated
+        when None), as LambdaInterface.getdEdXdL accumulates."""
+        d = (self._fallback_dudx if self._fallback_dudx is not None else self._handle.dudx_dl()).reshape(-1)
+        if g is None:
+            return d.copy()
+        g += d
+        return g
+
     # -- internals -----------------------------------------------------------
     def energyGradientAndVirial(self, x):
         """Energy, gradient and the strain derivative W_ab = dE/d eps_ab
@@ -162,8 +188,9 @@ class ForceFieldEnergy:
         return self._last["total"], self._handle.gradient().reshape(-1), self._handle.virial()
 
     def _evaluate(self, gradient, virial=False):
-        self._fallback_grad = None
-        flags = (ffi.NB_GRAD if gradient else 0) | (ffi.NB_VIRIAL if virial else 0)
+        self._fallback_grad = self._fallback_dudx = None
+        flags = (ffi.NB_GRAD if gradient else 0) | (ffi.NB_VIRIAL if virial else 0) | \
+            (ffi.NB_DUDL if self._lambda is not None else 0)
         e, rc = self._handle.evaluate(flags, check=False)
         if rc in (ffi.NB_ERR_SCF_DIVERGED, ffi.NB_ERR_SCF_MAXITER) and self.direct_fallback:
             # direct polarization for this evaluation only; the mutual handle is kept
@@ -173,9 +200,13 @@ class ForceFieldEnergy:
             try:
                 tmp.set_cell(self.box)
                 tmp.set_positions(self._x)
-                e, rc = tmp.evaluate(ffi.NB_GRAD if gradient else 0, check=False)
+                if self._lambda is not None:
+                    tmp.set_lambda(self._lambda)
+                e, rc = tmp.evaluate(flags & ~ffi.NB_VIRIAL, check=False)
                 if rc == ffi.NB_OK:
                     self._fallback_grad = tmp.gradient()
+                    if self._lambda is not None and gradient:
+                        self._fallback_dudx = tmp.dudx_dl()
             finally:
                 tmp.close()
         if rc != ffi.NB_OK:

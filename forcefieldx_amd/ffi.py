@@ -23,7 +23,7 @@ NB_PREC_FP64, NB_PREC_MIXED = 0, 1
 EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
-            "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry"]
+            "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -37,14 +37,17 @@ class NbParams(C.Structure):
                 ("p_scale", C.c_double * 4), ("intra14", C.c_double), ("d11", C.c_double),
                 ("v_scale", C.c_double * 3), ("halgren_delta", C.c_double), ("halgren_gamma", C.c_double),
                 ("polarization", C.c_int32), ("do_vdw", C.c_int32), ("do_pme", C.c_int32),
-                ("precision", C.c_int32), ("max_scf_iter", C.c_int32), ("device", C.c_int32)]
+                ("precision", C.c_int32), ("max_scf_iter", C.c_int32), ("device", C.c_int32),
+                ("lambda_", C.c_double), ("vdw_lambda_alpha", C.c_double), ("vdw_lambda_exponent", C.c_double),
+                ("vdw_lambda_end", C.c_double), ("intermolecular_softcore", C.c_int32),
+                ("intramolecular_softcore", C.c_int32)]
 
 
 class NbTopology(C.Structure):
     _fields_ = [("n_atoms", C.c_int32), ("frame", _ip), ("axis", _ip), ("multipole", _dp),
                 ("polarizability", _dp), ("thole", _dp), ("pdamp", _dp), ("group", _ip), ("vdw_class", _ip),
                 ("n_vdw_class", C.c_int32), ("vdw_table", _dp), ("red_index", _ip), ("red_value", _dp),
-                ("mask_offset", _ip), ("mask_atom", _ip), ("mask_rel", _ip)]
+                ("mask_offset", _ip), ("mask_atom", _ip), ("mask_rel", _ip), ("soft", _ip), ("molecule", _ip)]
 
 
 class NbEnergies(C.Structure):
@@ -77,6 +80,8 @@ def load(path=LIB_PATH):
     L.nb_set_positions.argtypes = [vp, _dp]
     L.nb_set_positions_device.argtypes = [vp, vp]
     L.nb_set_lambda.argtypes = [vp, C.c_double]
+    L.nb_struct_sizes.argtypes = [_ip]
+    L.nb_struct_sizes.restype = C.c_int
     L.nb_evaluate.argtypes = [vp, C.c_uint32, C.POINTER(NbEnergies)]
     L.nb_get_gradient.argtypes = [vp, _dp]
     L.nb_gradient_device.argtypes = [vp]
@@ -131,13 +136,19 @@ class Handle:
             vtab=np.ascontiguousarray(t.vdw_table, np.float64).reshape(-1),
             redi=np.ascontiguousarray(t.red_index, np.int32), redv=np.ascontiguousarray(t.red_value, np.float64),
             moff=np.ascontiguousarray(t.mask_off, np.int32), matom=np.ascontiguousarray(t.mask_atom, np.int32),
-            mrel=np.ascontiguousarray(t.mask_rel, np.int32))
+            mrel=np.ascontiguousarray(t.mask_rel, np.int32),
+            mol=np.ascontiguousarray(t.molecule, np.int32))
+        soft = getattr(t, "soft", None)
+        has_soft = soft is not None and np.any(np.asarray(soft) != 0)
+        if has_soft:
+            self._arrays["soft"] = np.ascontiguousarray(soft, np.int32)
         a = self._arrays
         top = NbTopology(t.n, _p(a["frame"], C.c_int32), _p(a["axis"], C.c_int32), _p(a["mpole"], C.c_double),
                          _p(a["polarity"], C.c_double), _p(a["thole"], C.c_double), _p(a["pdamp"], C.c_double),
                          _p(a["group"], C.c_int32), _p(a["vclass"], C.c_int32), int(t.vdw_table.shape[0]),
                          _p(a["vtab"], C.c_double), _p(a["redi"], C.c_int32), _p(a["redv"], C.c_double),
-                         _p(a["moff"], C.c_int32), _p(a["matom"], C.c_int32), _p(a["mrel"], C.c_int32))
+                         _p(a["moff"], C.c_int32), _p(a["matom"], C.c_int32), _p(a["mrel"], C.c_int32),
+                         _p(a["soft"], C.c_int32) if has_soft else None, _p(a["mol"], C.c_int32))
         params.device = device
         self.params = params
         self.n = t.n
@@ -173,6 +184,15 @@ class Handle:
     def gradient(self):
         g = np.zeros(3 * self.n)
         self._check(self.L.nb_get_gradient(self.h, _p(g, C.c_double)))
+        return g.reshape(self.n, 3)
+
+    def set_lambda(self, lam):
+        self._check(self.L.nb_set_lambda(self.h, float(lam)))
+
+    def dudx_dl(self):
+        """dU/dX/dlambda (n, 3) of the last evaluation made with NB_DUDL."""
+        g = np.zeros(3 * self.n)
+        self._check(self.L.nb_get_dudx_dl(self.h, _p(g, C.c_double)))
         return g.reshape(self.n, 3)
 
     def virial(self):
@@ -225,7 +245,9 @@ def params_from(p, grid, precision=None, device=0):
                     (C.c_int32 * 3)(*grid), p.polar_eps, p.electric, p.precond_cutoff, p.precond_scale,
                     (C.c_double * 4)(*p.m_scale), (C.c_double * 4)(*p.p_scale), p.intra14, p.d11,
                     (C.c_double * 3)(*p.v_scale), p.halgren_delta, p.halgren_gamma, p.polarization, int(p.do_vdw),
-                    int(p.do_pme), prec, p.max_scf_iter, device)
+                    int(p.do_pme), prec, p.max_scf_iter, device, p.lambda_value, p.vdw_lambda_alpha,
+                    p.vdw_lambda_exponent, p.vdw_lambda_end, int(p.intermolecular_softcore),
+                    int(p.intramolecular_softcore))
 
 
 def device_count():

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 1
+#define NB_ABI_VERSION 2
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -77,6 +77,13 @@ typedef struct {
   int32_t precision;        /* NB_PREC_* */
   int32_t max_scf_iter;     /* 1000 */
   int32_t device;           /* HIP device ordinal */
+  /* alchemical vdW softcore (VanDerWaals.setLambda, VanDerWaals.java:599-653) */
+  double lambda;            /* initial state, 1 = fully coupled (nb_set_lambda changes it) */
+  double vdw_lambda_alpha;  /* softcore shift alpha (0.25) */
+  double vdw_lambda_exponent; /* eps scaling exponent beta (3) */
+  double vdw_lambda_end;    /* lambda at which the vdW coupling is complete (1) */
+  int32_t intermolecular_softcore; /* both-soft pairs of different molecules are soft (0) */
+  int32_t intramolecular_softcore; /* both-soft pairs of one molecule are soft (0) */
 } nb_params;
 
 typedef struct {
@@ -96,6 +103,8 @@ typedef struct {
   const int32_t *mask_offset;  /* [n+1] CSR offsets of bonded relations */
   const int32_t *mask_atom;    /* [nnz] partner */
   const int32_t *mask_rel;     /* [nnz] 2 = 1-2, 3 = 1-3, 4 = 1-4, 5 = 1-5 */
+  const int32_t *soft;         /* [n] alchemical atoms (Atom.applyLambda), NULL: none */
+  const int32_t *molecule;     /* [n] molecule ids, needed with soft (NULL: all distinct) */
 } nb_topology;
 
 typedef struct {
@@ -126,7 +135,9 @@ int nb_set_positions(void *handle, const double *xyz);
 /* Device-resident coordinates [3n] (copied device-to-device). */
 int nb_set_positions_device(void *handle, const void *d_xyz);
 
-/* Alchemical state; replaces LambdaInterface.setLambda (LambdaInterface.java:73). */
+/* Alchemical state in [0, 1]; replaces LambdaInterface.setLambda
+ * (LambdaInterface.java:73).  Softcore applies to vdW pairs with exactly one
+ * soft atom (and to both-soft pairs per the inter/intramolecular flags). */
 int nb_set_lambda(void *handle, double lambda);
 
 /* One energy (+ gradient) evaluation: ForceFieldEnergy.energy(gradient, print)
@@ -148,7 +159,8 @@ int nb_get_induced(void *handle, double *u, double *ucr);
 /* Virial (3x3, row-major, kcal/mol); not provided by the reference. */
 int nb_get_virial(void *handle, double v[9]);
 
-/* dU/dX/dlambda [3n] (ParticleMeshEwald.getdEdXdL, :1199-1242). */
+/* dU/dX/dlambda [3n] of the last evaluation made with NB_DUDL (vdW softcore;
+ * VanDerWaals.getdEdXdL and the lambdaGrad of VanDerWaals.java:1869-1881). */
 int nb_get_dudx_dl(void *handle, double *g);
 
 /* Human-readable message for the last error on this handle. */
@@ -159,6 +171,11 @@ void nb_destroy(void *handle);
 
 /* ABI version (NB_ABI_VERSION). */
 int nb_abi_version(void);
+
+/* sizeof(nb_params), sizeof(nb_topology), sizeof(nb_energies) as compiled,
+ * for bindings to check their struct mirrors (no counterpart in the
+ * reference, whose interface is Java objects). */
+int nb_struct_sizes(int32_t sizes[3]);
 
 /* Number of HIP devices visible (0 without a GPU). */
 int nb_device_count(void);
