@@ -74,6 +74,9 @@ class NonbondedParams:
     pol_lambda_exponent: float = 3.0
     pol_lambda_start: float = 0.75
     pol_lambda_end: float = 1.0
+    elec_lambda: bool = False          # lambda also couples the soft atoms' electrostatics (OST)
+    ligand_vapor_elec: bool = True     # AlchemicalParameters.java:249
+    no_ligand_condensed_scf: bool = True  # AlchemicalParameters.java:251
 
     def resolved_grid(self, box):
         """Grid dimensions: explicit, or the reference density rule."""

@@ -189,3 +189,125 @@ WATER_CONFIGS = {
     "water-20736": (6912, 59.1, (64, 64, 64), 7.0, 9.0, 1.0e-5),
     "water-288000": (96000, 144.3, (144, 144, 144), 7.0, 9.0, 1.0e-5),
 }
+
+
+CHAIN_VDW = (3.8, 0.1)     # chain atom vdW radius (R-min, A) and well depth (kcal/mol)
+
+
+def _random_chain(rng, n, box, bond=1.5, min_dist=2.6, tries=200):
+    """Self-avoiding random walk of ``n`` points with fixed bond length: each
+    new point keeps ``min_dist`` (minimum image) from every earlier point
+    except its bonded predecessor."""
+    box = np.asarray(box, float)[:3]
+    pts = np.empty((n, 3))
+    pts[0] = rng.random(3) * box
+    md2 = min_dist * min_dist
+    for i in range(1, n):
+        for _ in range(tries):
+            v = rng.normal(size=3)
+            c = pts[i - 1] + bond * v / np.linalg.norm(v)
+            if i < 2:
+                break
+            d = pts[:i - 1] - c
+            d -= box * np.round(d / box)
+            if np.einsum("ij,ij->i", d, d).min() >= md2:
+                break
+        else:
+            raise RuntimeError("random chain could not be continued")
+        pts[i] = c
+    return pts
+
+
+def _random_traceless(rng, n, norm):
+    """Random symmetric traceless 3x3 tensors with Frobenius norm ``norm``, as
+    (xx, yy, zz, xy, xz, yz)."""
+    a = rng.normal(size=(n, 3, 3))
+    q = 0.5 * (a + a.transpose(0, 2, 1))
+    q -= np.eye(3) * (np.trace(q, axis1=1, axis2=2) / 3.0)[:, None, None]
+    q *= norm / np.linalg.norm(q, axis=(1, 2))[:, None, None]
+    return np.stack([q[:, 0, 0], q[:, 1, 1], q[:, 2, 2], q[:, 0, 1], q[:, 0, 2], q[:, 1, 2]], axis=1)
+
+
+def chain_in_water(n_chain, n_waters, box_length, seed=0, n_soft=0, soft_start=None, min_dist=2.6):
+    """Random-walk chain with random moments, solvated by AMOEBA waters.
+
+    Chain: bond 1.5 A, self-avoiding at ``min_dist``; charges N(0, 0.3) e made
+    neutral; local dipoles of 0.1 e*A and traceless quadrupoles of 0.3 e*A^2
+    (Frobenius) in Z-then-X frames on the chain neighbours; polarizabilities
+    U(0.3, 1.5) A^3, Thole 0.39; 1-2..1-5 masks from the chain bonds; each
+    atom its own polarization group.  The waters' oxygens are inserted at
+    ``min_dist`` from every chain atom and from each other.  ``n_soft``
+    contiguous chain atoms (from ``soft_start``, default the middle) are
+    marked soft.  Returns (xyz (3n,), box[6], Topology); the chain is
+    molecule 0 and comes first."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    box = np.array([box_length, box_length, box_length, 90.0, 90.0, 90.0])
+    chain = _random_chain(rng, n_chain, box, min_dist=min_dist)
+    q = rng.normal(0.0, 0.3, n_chain)
+    q -= q.mean()
+    dv = rng.normal(size=(n_chain, 3))
+    dv *= 0.1 / np.linalg.norm(dv, axis=1, keepdims=True)
+    quad = _random_traceless(rng, n_chain, 0.3)
+    pol = rng.uniform(0.3, 1.5, n_chain)
+    # waters around the chain
+    L = _sysgen()
+    n_tot = n_chain + n_waters
+    if L is None:
+        raise RuntimeError("libsysgen.so is not built")
+    import ctypes as C
+    dp = C.POINTER(C.c_double)
+    b = np.ascontiguousarray(box[:3])
+    pts = np.zeros((n_tot, 3))
+    pts[:n_chain] = chain
+    count, tries = n_chain, 4000
+    while count < n_tot:
+        cand = np.ascontiguousarray(rng.random((4096, 3)) * b)
+        count = L.rsa_place(cand.ctypes.data_as(dp), 4096, pts.ctypes.data_as(dp), count, n_tot,
+                            b.ctypes.data_as(dp), float(min_dist))
+        tries -= 1
+        if tries <= 0:
+            raise RuntimeError("water insertion did not converge")
+    opos = pts[n_chain:]
+    R = _random_rotations(rng, n_waters)
+    o, h1, h2 = water_geometry()
+    wxyz = np.empty((n_waters, 3, 3))
+    wxyz[:, 0] = opos
+    wxyz[:, 1] = opos + R @ h1
+    wxyz[:, 2] = opos + R @ h2
+    wt = water_topology(n_waters, atom_offset=n_chain, molecule_offset=1)
+    # chain topology
+    frame = np.full(n_chain, FRAME_ZTHENX, np.int32)
+    axis = -np.ones((n_chain, 3), np.int32)
+    for i in range(n_chain):
+        # z toward the next atom (the previous one at the end), x toward the
+        # previous one (the one after z at the start, the one before z at the end)
+        if i + 1 < n_chain:
+            z, x = i + 1, (i - 1 if i > 0 else i + 2)
+        else:
+            z, x = i - 1, i - 2
+        axis[i, 0], axis[i, 1] = z, x
+    mp = np.concatenate([q[:, None], dv, quad], axis=1)
+    bonds = [(i, i + 1) for i in range(n_chain - 1)]
+    off, atoms, rels = masks_from_bonds(n_chain, bonds)
+    # vdW classes: 0 water O, 1 water H, 2 chain
+    tab = vdw_pair_table([WATER_VDW["O"][0], WATER_VDW["H"][0], CHAIN_VDW[0]],
+                         [WATER_VDW["O"][1], WATER_VDW["H"][1], CHAIN_VDW[1]])
+    n = n_tot = n_chain + 3 * n_waters
+    cat = np.concatenate
+    mask_off = cat([off, off[-1] + wt.mask_off[1:]]).astype(np.int32)
+    top = Topology(n=n, frame=cat([frame, wt.frame]).astype(np.int32), axis=cat([axis, wt.axis]).astype(np.int32),
+                   mpole=cat([mp, wt.mpole]), polarity=cat([pol, wt.polarity]),
+                   thole=cat([np.full(n_chain, WATER_THOLE), wt.thole]),
+                   pdamp=cat([pol ** (1.0 / 6.0), wt.pdamp]),
+                   group=cat([np.arange(n_chain, dtype=np.int32) + 10 * n, wt.group]).astype(np.int32),
+                   molecule=cat([np.zeros(n_chain, np.int32), wt.molecule]).astype(np.int32),
+                   vdw_class=cat([np.full(n_chain, 2, np.int32), wt.vdw_class]).astype(np.int32), vdw_table=tab,
+                   red_index=cat([np.arange(n_chain, dtype=np.int32), wt.red_index]).astype(np.int32),
+                   red_value=cat([np.zeros(n_chain), wt.red_value]),
+                   mask_off=mask_off, mask_atom=cat([atoms, wt.mask_atom]).astype(np.int32),
+                   mask_rel=cat([rels, wt.mask_rel]).astype(np.int32))
+    if n_soft:
+        s0 = (n_chain - n_soft) // 2 if soft_start is None else soft_start
+        top.soft[s0:s0 + n_soft] = 1
+    xyz = np.concatenate([chain, wxyz.reshape(-1, 3)]).reshape(-1)
+    return xyz, box, top

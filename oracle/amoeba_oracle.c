@@ -1167,9 +1167,31 @@ static void remove_masks(const ora_system *s, int i, double *m, double *p) {
 /* ------------------------------------------------------------------------ */
 /* evaluation context                                                       */
 /* ------------------------------------------------------------------------ */
+/* Scale factors of one electrostatics evaluation.  A plain evaluation has
+ * every scale 1 and no lambda terms.  Under OST alchemistry the energy is
+ * the sum of three such evaluations (ParticleMeshEwald.java:894-935):
+ * permanent real-space pairs with a soft atom are softcore-shifted
+ * (r -> sqrt(r^2 + lalpha)) and scaled by l2 (RealSpaceEnergyRegion.java:
+ * 875-908); reciprocal, self and charge-correction terms are scaled by perm,
+ * all polarization terms by pol (ReciprocalEnergyRegion.java:254, 390-391,
+ * 555-556; RealSpaceEnergyRegion.java:1818, 2176); d* and d2* are the
+ * first and second lambda derivatives of those scales, sign included. */
+typedef struct {
+  int perm_real;      /* permanent real-space energy on */
+  int polar;          /* polarization on */
+  int soft_pairs;     /* pairs with a soft atom use l2 and the softcore shift */
+  int vapor;          /* skip both-soft pairs per the inter/intramolecular flags */
+  double l2, lalpha, dlalpha, d2lalpha;
+  double perm, dperm, d2perm;
+  double pol, dpol, d2pol;
+} phase_t;
+
+static const phase_t PHASE_PLAIN = {1, 1, 0, 0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+
 typedef struct {
   const ora_system *s;
   const ora_params *prm;
+  const phase_t *ph;
   int n, nth;
   cell_t cell;
   double aewald, off2;
@@ -1186,6 +1208,7 @@ typedef struct {
   double *field, *fieldp;        /* [3n] */
   double *ind, *indp, *dir, *dirp;
   double *grad, *torque;
+  double *lgrad, *ltorque;       /* dU/dX/dlambda and its torque */
   int n_perm;
 } ctx_t;
 
@@ -1507,14 +1530,15 @@ static void recip_grid_virial(ctx_t *c, int polar, int mutual, double *vir) {
         for (int d = 0; d < 3; d++) m[d] = c->cell.A[d][0] * h + c->cell.A[d][1] * k + c->cell.A[d][2] * l;
         double m2 = dot3(m, m);
         double pr = fpm[2 * idx], pi = fpm[2 * idx + 1];
-        double ws = pr * pr + pi * pi;
+        double ws = c->ph->perm * (pr * pr + pi * pi), wq = 0.0;
         if (polar) {
           long nidx = ((nx - kx) % nx) + (long)nx * (((ny - ky) % ny) + (long)ny * ((nz - kz) % nz));
           double ar = fuv[2 * idx], ai = fuv[2 * idx + 1], br = fuv[2 * nidx], bi = -fuv[2 * nidx + 1];
           double ur = 0.5 * (ar + br), ui = 0.5 * (ai + bi);   /* Fu = (F(m) + conj F(-m)) / 2 */
           double vr = 0.5 * (ai - bi), vi = -0.5 * (ar - br);  /* Fv = (F(m) - conj F(-m)) / 2i */
-          ws += pr * (ur + vr) + pi * (ui + vi);
-          if (mutual) ws += ur * vr + ui * vi;
+          wq = pr * (ur + vr) + pi * (ui + vi);
+          if (mutual) wq += ur * vr + ui * vi;
+          ws += c->ph->pol * wq;
         }
         double w = 0.5 * el * r * ws, f = 2.0 * (1.0 + pt * m2) / m2;
         for (int a = 0; a < 3; a++)
@@ -1536,6 +1560,7 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
   pme_t *p = &c->pme;
   double aewald1 = -el * aw / SQRT_PI, aewald2 = 2.0 * aw * aw;
   double aewald3 = -2.0 / 3.0 * el * aw * aw * aw / SQRT_PI, aewald4 = -2.0 * aewald3;
+  const phase_t *phs = c->ph;
   double eself = 0, erecip = 0, qtot = 0;
   for (int i = 0; i < n; i++) {
     const double *m = c->gmp + 10 * i;
@@ -1567,17 +1592,26 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
     tq[0] -= tt * (m[7] * ph[8] + m[5] * ph[9] + m[9] * ph[6] - m[8] * ph[7] - m[9] * ph[5] - m[6] * ph[9]);
     tq[1] -= tt * (m[8] * ph[4] + m[9] * ph[7] + m[6] * ph[8] - m[4] * ph[8] - m[7] * ph[9] - m[8] * ph[6]);
     tq[2] -= tt * (m[4] * ph[7] + m[7] * ph[5] + m[8] * ph[9] - m[7] * ph[4] - m[5] * ph[7] - m[9] * ph[8]);
-    for (int a = 0; a < 3; a++) { c->grad[3 * i + a] += el * df[a]; c->torque[3 * i + a] += el * tq[a]; }
+    for (int a = 0; a < 3; a++) {
+      c->grad[3 * i + a] += phs->perm * el * df[a];
+      c->torque[3 * i + a] += phs->perm * el * tq[a];
+      c->lgrad[3 * i + a] += phs->dperm * el * df[a];
+      c->ltorque[3 * i + a] += phs->dperm * el * tq[a];
+    }
   }
-  out->perm_self = eself;
-  out->perm_recip = 0.5 * el * erecip;
-  out->perm_charge_corr = 0.0;
-  if (qtot != 0.0) out->perm_charge_corr = -0.5 * el * M_PI * qtot * qtot / (c->cell.vol * aw * aw);
+  double ecc = 0.0;
+  if (qtot != 0.0) ecc = -0.5 * el * M_PI * qtot * qtot / (c->cell.vol * aw * aw);
+  double eperm = eself + 0.5 * el * erecip + ecc;
+  out->perm_self += phs->perm * eself;
+  out->perm_recip += phs->perm * 0.5 * el * erecip;
+  out->perm_charge_corr += phs->perm * ecc;
+  out->dudl += phs->dperm * eperm;
+  out->d2udl2 += phs->d2perm * eperm;
   /* strain derivative: the charge correction scales as 1/V; with the global
    * multipoles held fixed, the wave vectors' strain dependence gives
    * -d_b dphi/dx_a - (2/3) Q_bc d2phi/dx_a dx_c per atom (Cartesian
    * multipoles, potentials of the permanent grid) */
-  for (int a = 0; a < 3; a++) out->vir[4 * a] -= out->perm_charge_corr;
+  for (int a = 0; a < 3; a++) out->vir[4 * a] -= phs->perm * ecc;
   for (int i = 0; i < n; i++) {
     const double *m = c->gmp + 10 * i, *ph = c->phi_p_c + 10 * i;
     double Q[3][3], H[3][3];
@@ -1587,7 +1621,7 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
       for (int b = 0; b < 3; b++) {
         double t = m[1 + b] * ph[1 + a];
         for (int cc = 0; cc < 3; cc++) t += 2.0 / 3.0 * Q[b][cc] * H[a][cc];
-        out->vir[3 * a + b] -= el * t;
+        out->vir[3 * a + b] -= phs->perm * el * t;
       }
   }
   if (!polar) return;
@@ -1608,7 +1642,7 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
         double t = (u[b] + v[b]) * pp[1 + a] + m[1 + b] * (cu[1 + a] + cv[1 + a]);
         for (int cc = 0; cc < 3; cc++) t += 2.0 / 3.0 * Q[b][cc] * Hw[a][cc];
         if (mutual) t += v[b] * cu[1 + a] + u[b] * cv[1 + a];
-        out->vir[3 * a + b] -= 0.5 * el * t;
+        out->vir[3 * a + b] -= phs->pol * 0.5 * el * t;
       }
   }
   /* induced dipoles */
@@ -1619,7 +1653,8 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
     double us[3] = {0.5 * (u[0] + up[0]), 0.5 * (u[1] + up[1]), 0.5 * (u[2] + up[2])};
     double t[3];
     cross3(m + 1, us, t);
-    axpy3(aewald4, t, c->torque + 3 * i);
+    axpy3(phs->pol * aewald4, t, c->torque + 3 * i);
+    axpy3(phs->dpol * aewald4, t, c->ltorque + 3 * i);
   }
   for (int i = 0; i < n; i++) {
     const double *m = c->gmp + 10 * i, *fm = c->fmp + 10 * i, *fp = c->phi_p_f + 20 * i;
@@ -1658,10 +1693,17 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
     tq[0] -= tt * (m[7] * sp[8] + m[5] * sp[9] + m[9] * sp[6] - m[8] * sp[7] - m[9] * sp[5] - m[6] * sp[9]);
     tq[1] -= tt * (m[8] * sp[4] + m[9] * sp[7] + m[6] * sp[8] - m[4] * sp[8] - m[7] * sp[9] - m[8] * sp[6]);
     tq[2] -= tt * (m[4] * sp[7] + m[7] * sp[5] + m[8] * sp[9] - m[7] * sp[4] - m[5] * sp[7] - m[9] * sp[8]);
-    for (int a = 0; a < 3; a++) { c->grad[3 * i + a] += df[a]; c->torque[3 * i + a] += el * tq[a]; }
+    for (int a = 0; a < 3; a++) {
+      c->grad[3 * i + a] += phs->pol * df[a];
+      c->torque[3 * i + a] += phs->pol * el * tq[a];
+      c->lgrad[3 * i + a] += phs->dpol * df[a];
+      c->ltorque[3 * i + a] += phs->dpol * el * tq[a];
+    }
   }
-  out->pol_self = eselfi;
-  out->pol_recip = 0.5 * el * erecipi;
+  out->pol_self += phs->pol * eselfi;
+  out->pol_recip += phs->pol * 0.5 * el * erecipi;
+  out->dudl += phs->dpol * (eselfi + 0.5 * el * erecipi);
+  out->d2udl2 += phs->d2pol * (eselfi + 0.5 * el * erecipi);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1672,18 +1714,19 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
   int n = c->n, nth = c->nth;
   const ora_system *s = c->s;
   const ora_params *prm = c->prm;
+  const phase_t *ph = c->ph;
   double el = prm->electric;
-  double *acc = calloc((size_t)nth * 6 * n, sizeof(double));
-  double ep = 0.0, ei = 0.0;
+  double *acc = calloc((size_t)nth * 12 * n, sizeof(double));
+  double ep = 0.0, ei = 0.0, dl1 = 0.0, dl2 = 0.0;
   int cnt = 0;
   double w[9] = {0};
-  #pragma omp parallel num_threads(nth) reduction(+ : ep, ei, cnt, w[:9])
+  #pragma omp parallel num_threads(nth) reduction(+ : ep, ei, dl1, dl2, cnt, w[:9])
   {
     int tid = 0;
 #ifdef _OPENMP
     tid = omp_get_thread_num();
 #endif
-    double *ag = acc + (size_t)tid * 6 * n, *at = ag + 3 * n;
+    double *ag = acc + (size_t)tid * 12 * n, *at = ag + 3 * n, *lg = ag + 6 * n, *lt = ag + 9 * n;
     double *mm = malloc(n * sizeof(double)), *pm = malloc(n * sizeof(double));
     for (int k = 0; k < n; k++) { mm[k] = 1.0; pm[k] = 1.0; }
     #pragma omp for schedule(dynamic, 32)
@@ -1694,6 +1737,13 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
       mp3_load(c->gmp + 10 * i, &I);
       for (int t = c->ew_off[i]; t < c->ew_off[i + 1]; t++) {
         int k = c->ew[t];
+        int soft = ph->soft_pairs && s->soft && (s->soft[i] || s->soft[k]);
+        if (ph->vapor && s->soft && s->soft[i] && s->soft[k]) {
+          /* vapor phase: both-soft pairs that the condensed phase softens
+           * are left out (RealSpaceEnergyRegion.java:864-870) */
+          int same = s->molecule ? s->molecule[i] == s->molecule[k] : 0;
+          if ((prm->intermolecular_softcore && !same) || (prm->intramolecular_softcore && same)) continue;
+        }
         double r[3];
         sub3(s->xyz + 3 * k, xi, r);
         double r2 = cell_image(&c->cell, r);
@@ -1703,25 +1753,63 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
         double bn[7], rr[7];
         ewald_bn(rl, c->aewald, 6, bn);
         coulomb_rr(rl, 6, rr);
-        permgeo_t g;
-        perm_geometry(r, &I, &K, &g);
-        double f1[3], t1[3], u1[3], f2[3], t2[3], u2[3];
-        double e = perm_terms(r, &I, &K, &g, bn, f1, t1, u1);
-        double s1 = 1.0 - mm[k];
-        if (s1 != 0.0) {
-          e -= s1 * perm_terms(r, &I, &K, &g, rr, f2, t2, u2);
-          for (int a = 0; a < 3; a++) { f1[a] -= s1 * f2[a]; t1[a] -= s1 * t2[a]; u1[a] -= s1 * u2[a]; }
-        }
-        if (e != 0.0) cnt++;
-        ep += e;
-        /* pair strain derivative: dE/dr = -(i-side gradient), r = x_k - x_i */
-        for (int a = 0; a < 3; a++)
-          for (int b = 0; b < 3; b++) w[3 * a + b] -= el * f1[a] * r[b];
-        for (int a = 0; a < 3; a++) {
-          ag[3 * i + a] += el * f1[a];
-          ag[3 * k + a] -= el * f1[a];
-          at[3 * i + a] += el * t1[a];
-          at[3 * k + a] += el * u1[a];
+        if (ph->perm_real) {
+          /* permanent pair; a soft pair sees r_eff = sqrt(r^2 + lalpha) in
+           * the radial functions and the unshifted r in the angular ones */
+          double sb[7], sr[7];
+          const double *B = bn, *R = rr;
+          double l2 = 1.0;
+          if (soft) {
+            double re = sqrt(r2 + ph->lalpha);
+            ewald_bn(re, c->aewald, 6, sb);
+            coulomb_rr(re, 6, sr);
+            B = sb; R = sr;
+            l2 = ph->l2;
+          }
+          permgeo_t g;
+          perm_geometry(r, &I, &K, &g);
+          double f1[3], t1[3], u1[3], f2[3], t2[3], u2[3];
+          double e = perm_terms(r, &I, &K, &g, B, f1, t1, u1);
+          double s1 = 1.0 - mm[k];
+          if (s1 != 0.0) {
+            e -= s1 * perm_terms(r, &I, &K, &g, R, f2, t2, u2);
+            for (int a = 0; a < 3; a++) { f1[a] -= s1 * f2[a]; t1[a] -= s1 * t2[a]; u1[a] -= s1 * u2[a]; }
+          }
+          if (e != 0.0) cnt++;
+          ep += l2 * e;
+          /* pair strain derivative: dE/dr = -(i-side gradient), r = x_k - x_i */
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) w[3 * a + b] -= el * l2 * f1[a] * r[b];
+          for (int a = 0; a < 3; a++) {
+            ag[3 * i + a] += el * l2 * f1[a];
+            ag[3 * k + a] -= el * l2 * f1[a];
+            at[3 * i + a] += el * l2 * t1[a];
+            at[3 * k + a] += el * l2 * u1[a];
+          }
+          if (soft) {
+            /* d(radial)/dlambda: bn_j -> bn_{j+1} times dlalpha, since
+             * d bn_j / d(r^2) = -bn_{j+1}/2 and d lalpha/dlambda = -2 dlalpha
+             * (RealSpaceEnergyRegion.java:1477-1690) */
+            double fr[3], tr[3], ur[3], fq[3], tq2[3], uq[3];
+            double dR = perm_terms(r, &I, &K, &g, B + 1, fr, tr, ur);
+            double d2R = perm_terms(r, &I, &K, &g, B + 2, NULL, NULL, NULL);
+            if (s1 != 0.0) {
+              dR -= s1 * perm_terms(r, &I, &K, &g, R + 1, fq, tq2, uq);
+              d2R -= s1 * perm_terms(r, &I, &K, &g, R + 2, NULL, NULL, NULL);
+              for (int a = 0; a < 3; a++) { fr[a] -= s1 * fq[a]; tr[a] -= s1 * tq2[a]; ur[a] -= s1 * uq[a]; }
+            }
+            dl1 += ph->dperm * e + l2 * ph->dlalpha * dR;
+            dl2 += ph->d2perm * e + 2.0 * ph->dperm * ph->dlalpha * dR + l2 * ph->d2lalpha * dR +
+                   l2 * ph->dlalpha * ph->dlalpha * d2R;
+            double ca = ph->dperm, cb = l2 * ph->dlalpha;
+            for (int a = 0; a < 3; a++) {
+              double fl = el * (ca * f1[a] + cb * fr[a]);
+              lg[3 * i + a] += fl;
+              lg[3 * k + a] -= fl;
+              lt[3 * i + a] += el * (ca * t1[a] + cb * tr[a]);
+              lt[3 * k + a] += el * (ca * u1[a] + cb * ur[a]);
+            }
+          }
         }
         if (polar) {
           polctx_t cx;
@@ -1731,15 +1819,22 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
           cx.scaled = (s->group[k] == s->group[i]) ? prm->d11 : 1.0;
           cx.direct = (prm->polarization == ORA_POL_DIRECT);
           double f[3], a2[3], a3[3];
-          ei += pol_terms(r, &I, &K, c->ind + 3 * i, c->indp + 3 * i, c->ind + 3 * k, c->indp + 3 * k, &cx, 1, f,
-                          a2, a3);
+          double e = pol_terms(r, &I, &K, c->ind + 3 * i, c->indp + 3 * i, c->ind + 3 * k, c->indp + 3 * k, &cx, 1,
+                               f, a2, a3);
+          ei += ph->pol * e;
+          dl1 += ph->dpol * e;
+          dl2 += ph->d2pol * e;
           for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) w[3 * a + b] -= el * f[a] * r[b];
+            for (int b = 0; b < 3; b++) w[3 * a + b] -= el * ph->pol * f[a] * r[b];
           for (int a = 0; a < 3; a++) {
-            ag[3 * i + a] += el * f[a];
-            ag[3 * k + a] -= el * f[a];
-            at[3 * i + a] += el * a2[a];
-            at[3 * k + a] += el * a3[a];
+            ag[3 * i + a] += el * ph->pol * f[a];
+            ag[3 * k + a] -= el * ph->pol * f[a];
+            at[3 * i + a] += el * ph->pol * a2[a];
+            at[3 * k + a] += el * ph->pol * a3[a];
+            lg[3 * i + a] += el * ph->dpol * f[a];
+            lg[3 * k + a] -= el * ph->dpol * f[a];
+            lt[3 * i + a] += el * ph->dpol * a2[a];
+            lt[3 * k + a] += el * ph->dpol * a3[a];
           }
         }
       }
@@ -1749,13 +1844,18 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
   }
   for (int t = 0; t < nth; t++)
     for (int q = 0; q < 3 * n; q++) {
-      c->grad[q] += acc[(size_t)t * 6 * n + q];
-      c->torque[q] += acc[(size_t)t * 6 * n + 3 * n + q];
+      const double *a = acc + (size_t)t * 12 * n;
+      c->grad[q] += a[q];
+      c->torque[q] += a[3 * n + q];
+      c->lgrad[q] += a[6 * n + q];
+      c->ltorque[q] += a[9 * n + q];
     }
   free(acc);
-  out->perm_real = el * ep;
-  out->pol_real = el * ei;
-  out->n_perm = cnt;
+  out->perm_real += el * ep;
+  out->pol_real += el * ei;
+  out->dudl += el * dl1;
+  out->d2udl2 += el * dl2;
+  out->n_perm += cnt;
   for (int q = 0; q < 9; q++) out->vir[q] += w[q];
 }
 
@@ -1995,120 +2095,337 @@ int ora_evaluate(const ora_system *s, const ora_params *prm, ora_energies *out, 
   return ora_evaluate_lambda(s, prm, out, grad, ind, indcr, NULL);
 }
 
-int ora_evaluate_lambda(const ora_system *s, const ora_params *prm, ora_energies *out, double *grad, double *ind,
-                        double *indcr, double *dudx_dl) {
-  memset(out, 0, sizeof(*out));
-  ctx_t cx;
-  memset(&cx, 0, sizeof cx);
-  ctx_t *c = &cx;
-  c->s = s; c->prm = prm; c->n = s->n;
+static void ctx_open(ctx_t *c, const ora_system *s, const ora_params *prm, const phase_t *ph) {
+  memset(c, 0, sizeof *c);
+  c->s = s; c->prm = prm; c->ph = ph; c->n = s->n;
   c->nth = nthreads_of(prm);
   int n = s->n;
   cell_init(&c->cell, s->box);
   c->aewald = c->cell.aperiodic ? 0.0 : prm->ewald_alpha;
   c->off2 = prm->ewald_cutoff * prm->ewald_cutoff;
-  size_t s3 = 3 * (size_t)n * sizeof(double);
   c->grad = calloc(3 * n, sizeof(double));
   c->torque = calloc(3 * n, sizeof(double));
+  c->lgrad = calloc(3 * n, sizeof(double));
+  c->ltorque = calloc(3 * n, sizeof(double));
   double lcut = (prm->vdw_cutoff > prm->ewald_cutoff ? prm->vdw_cutoff : prm->ewald_cutoff) + prm->list_buffer;
   nlist_build(&c->nl, &c->cell, n, s->xyz, lcut);
+}
 
-  if (dudx_dl) memset(dudx_dl, 0, 3 * (size_t)n * sizeof(double));
-  if (prm->do_vdw) out->vdw = vdw_energy(c, c->grad, &out->n_vdw, out->vir, &out->dudl, &out->d2udl2, dudx_dl);
+static void ctx_close(ctx_t *c) {
+  free(c->grad); free(c->torque); free(c->lgrad); free(c->ltorque);
+  nlist_free(&c->nl);
+}
 
-  if (prm->do_pme) {
-    c->gmp = malloc(10 * s3 / 3);
-    ora_rotate_multipoles(s, c->gmp);
-    c->ipd = malloc(n * sizeof(double));
-    for (int i = 0; i < n; i++) c->ipd[i] = s->pdamp[i] > 0.0 ? 1.0 / s->pdamp[i] : INFINITY;
-    c->field = malloc(s3); c->fieldp = malloc(s3);
-    c->ind = calloc(3 * n, sizeof(double)); c->indp = calloc(3 * n, sizeof(double));
-    c->dir = calloc(3 * n, sizeof(double)); c->dirp = calloc(3 * n, sizeof(double));
-    c->use_recip = (!c->cell.aperiodic && c->aewald > 0.0);
-    c->phi_p_f = calloc(20 * n, sizeof(double)); c->phi_p_c = calloc(10 * n, sizeof(double));
-    c->phi_u_f = calloc(20 * n, sizeof(double)); c->phi_u_c = calloc(10 * n, sizeof(double));
-    c->phi_pp_f = calloc(20 * n, sizeof(double)); c->phi_pp_c = calloc(10 * n, sizeof(double));
-    c->fmp = calloc(10 * n, sizeof(double));
-    if (c->use_recip) {
-      pme_setup(&c->pme, &c->cell, prm, n, c->nth);
-      ora_influence(s->box, prm->grid, prm->pme_order, c->aewald, c->pme.recip);
-      pme_bsplines(&c->pme, &c->cell, n, s->xyz);
-      for (int i = 0; i < n; i++) frac_multipole(&c->pme, c->gmp + 10 * i, c->fmp + 10 * i);
-      pme_spread(&c->pme, n, c->fmp, NULL, NULL);
-      ora_convolution(c->pme.grid, c->pme.recip, c->pme.K[0], c->pme.K[1], c->pme.K[2]);
-      for (int i = 0; i < n; i++) {
-        pme_gather(&c->pme, i, 0, c->phi_p_f + 20 * i);
-        phi_to_cart(&c->pme, c->phi_p_f + 20 * i, c->phi_p_c + 10 * i);
-      }
-    }
-    permanent_field(c);
-    int polar = prm->polarization != ORA_POL_NONE;
-    int status = 0;
-    if (polar) {
-      double a3 = 4.0 / 3.0 * pow(c->aewald, 3.0) / SQRT_PI;
-      for (int i = 0; i < n; i++) {
-        for (int d = 0; d < 3; d++) {
-          if (c->use_recip) {
-            double fx = a3 * c->gmp[10 * i + 1 + d] - c->phi_p_c[10 * i + 1 + d];
-            c->field[3 * i + d] += fx;
-            c->fieldp[3 * i + d] += fx;
-          }
-          c->ind[3 * i + d] = c->dir[3 * i + d] = s->polarity[i] * c->field[3 * i + d];
-          c->indp[3 * i + d] = c->dirp[3 * i + d] = s->polarity[i] * c->fieldp[3 * i + d];
-        }
-      }
-      if (prm->polarization == ORA_POL_MUTUAL) {
-        status = pcg_solve(c, out);
-      } else {
-        double *f = malloc(s3), *fp = malloc(s3);
-        induced_field(c, c->ind, c->indp, f, fp);
-        free(f); free(fp);
-      }
-    }
-    if (c->use_recip) {
-      recip_energy(c, out, polar);
-      recip_grid_virial(c, polar, prm->polarization == ORA_POL_MUTUAL, out->vir);
-    }
-    real_energy(c, out, polar);
-    out->n_perm += c->use_recip ? n : 0;
-    /* torques to gradients: ReduceRegion.TorqueLoop */
-    for (int i = 0; i < n; i++) {
+/* torques to frame-atom gradients (ReduceRegion.TorqueLoop), for the energy
+ * gradient (with the frame-rotation virial) and for dU/dX/dlambda */
+static int reduce_torques(ctx_t *c, ora_energies *out) {
+  const ora_system *s = c->s;
+  int status = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    double *tqa = pass ? c->ltorque : c->torque, *ga = pass ? c->lgrad : c->grad;
+    for (int i = 0; i < c->n; i++) {
       const double *o = s->xyz + 3 * i;
       const int *ax = s->axis + 3 * i;
       const double *pz = ax[0] >= 0 ? s->xyz + 3 * ax[0] : o;
       const double *px = ax[1] >= 0 ? s->xyz + 3 * ax[1] : o;
       const double *py = ax[2] >= 0 ? s->xyz + 3 * ax[2] : o;
       double g4[4][3];
-      const double *tq = c->torque + 3 * i;
+      const double *tq = tqa + 3 * i;
       if (!isfinite(tq[0]) || !isfinite(tq[1]) || !isfinite(tq[2])) status = status ? status : 4;
       if (torque_to_grad(s->frame[i], o, pz, px, py, tq, g4) == 0 && s->frame[i] != ORA_FRAME_NONE) {
         int idx[4] = {ax[0], ax[1], ax[2], i};
         for (int j = 0; j < 4; j++)
           if (idx[j] >= 0)
-            for (int d = 0; d < 3; d++) c->grad[3 * idx[j] + d] += g4[j][d];
+            for (int d = 0; d < 3; d++) ga[3 * idx[j] + d] += g4[j][d];
         /* frame rotation under strain: the frame-atom gradients against the
          * frame vectors (they sum to zero with the centre's) */
-        for (int j = 0; j < 3; j++)
-          if (idx[j] >= 0)
-            for (int a = 0; a < 3; a++)
-              for (int b = 0; b < 3; b++) out->vir[3 * a + b] += g4[j][a] * (s->xyz[3 * idx[j] + b] - o[b]);
+        if (!pass)
+          for (int j = 0; j < 3; j++)
+            if (idx[j] >= 0)
+              for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) out->vir[3 * a + b] += g4[j][a] * (s->xyz[3 * idx[j] + b] - o[b]);
       }
     }
-    if (ind) memcpy(ind, c->ind, s3);
-    if (indcr) memcpy(indcr, c->indp, s3);
-    out->status = status;
-    free(c->gmp); free(c->ipd); free(c->field); free(c->fieldp); free(c->ind); free(c->indp);
-    free(c->dir); free(c->dirp); free(c->phi_p_f); free(c->phi_p_c); free(c->phi_u_f); free(c->phi_u_c);
-    free(c->phi_pp_f); free(c->phi_pp_c); free(c->fmp);
-    free(c->ew_off); free(c->ew); free(c->pc_off); free(c->pc);
-    if (c->use_recip) pme_free(&c->pme);
   }
+  return status;
+}
+
+/* One electrostatics evaluation (ParticleMeshEwald.computeEnergy) of the
+ * system in c under the phase scales c->ph; energies and the virial are
+ * added into out, gradients into c->grad / c->lgrad. */
+static int elec_phase(ctx_t *c, ora_energies *out, double *ind, double *indcr) {
+  const ora_system *s = c->s;
+  const ora_params *prm = c->prm;
+  int n = c->n;
+  size_t s3 = 3 * (size_t)n * sizeof(double);
+  c->gmp = malloc(10 * s3 / 3);
+  ora_rotate_multipoles(s, c->gmp);
+  c->ipd = malloc(n * sizeof(double));
+  for (int i = 0; i < n; i++) c->ipd[i] = s->pdamp[i] > 0.0 ? 1.0 / s->pdamp[i] : INFINITY;
+  c->field = malloc(s3); c->fieldp = malloc(s3);
+  c->ind = calloc(3 * n, sizeof(double)); c->indp = calloc(3 * n, sizeof(double));
+  c->dir = calloc(3 * n, sizeof(double)); c->dirp = calloc(3 * n, sizeof(double));
+  c->use_recip = (!c->cell.aperiodic && c->aewald > 0.0);
+  c->phi_p_f = calloc(20 * n, sizeof(double)); c->phi_p_c = calloc(10 * n, sizeof(double));
+  c->phi_u_f = calloc(20 * n, sizeof(double)); c->phi_u_c = calloc(10 * n, sizeof(double));
+  c->phi_pp_f = calloc(20 * n, sizeof(double)); c->phi_pp_c = calloc(10 * n, sizeof(double));
+  c->fmp = calloc(10 * n, sizeof(double));
+  if (c->use_recip) {
+    pme_setup(&c->pme, &c->cell, prm, n, c->nth);
+    ora_influence(s->box, prm->grid, prm->pme_order, c->aewald, c->pme.recip);
+    pme_bsplines(&c->pme, &c->cell, n, s->xyz);
+    for (int i = 0; i < n; i++) frac_multipole(&c->pme, c->gmp + 10 * i, c->fmp + 10 * i);
+    pme_spread(&c->pme, n, c->fmp, NULL, NULL);
+    ora_convolution(c->pme.grid, c->pme.recip, c->pme.K[0], c->pme.K[1], c->pme.K[2]);
+    for (int i = 0; i < n; i++) {
+      pme_gather(&c->pme, i, 0, c->phi_p_f + 20 * i);
+      phi_to_cart(&c->pme, c->phi_p_f + 20 * i, c->phi_p_c + 10 * i);
+    }
+  }
+  permanent_field(c);
+  int polar = prm->polarization != ORA_POL_NONE && c->ph->polar;
+  int status = 0;
+  if (polar) {
+    double a3 = 4.0 / 3.0 * pow(c->aewald, 3.0) / SQRT_PI;
+    for (int i = 0; i < n; i++) {
+      for (int d = 0; d < 3; d++) {
+        if (c->use_recip) {
+          double fx = a3 * c->gmp[10 * i + 1 + d] - c->phi_p_c[10 * i + 1 + d];
+          c->field[3 * i + d] += fx;
+          c->fieldp[3 * i + d] += fx;
+        }
+        c->ind[3 * i + d] = c->dir[3 * i + d] = s->polarity[i] * c->field[3 * i + d];
+        c->indp[3 * i + d] = c->dirp[3 * i + d] = s->polarity[i] * c->fieldp[3 * i + d];
+      }
+    }
+    if (prm->polarization == ORA_POL_MUTUAL) {
+      status = pcg_solve(c, out);
+    } else {
+      double *f = malloc(s3), *fp = malloc(s3);
+      induced_field(c, c->ind, c->indp, f, fp);
+      free(f); free(fp);
+    }
+  }
+  if (c->use_recip) {
+    recip_energy(c, out, polar);
+    recip_grid_virial(c, polar, prm->polarization == ORA_POL_MUTUAL, out->vir);
+  }
+  real_energy(c, out, polar);
+  out->n_perm += c->use_recip ? n : 0;
+  int st = reduce_torques(c, out);
+  if (!status) status = st;
+  if (ind) memcpy(ind, c->ind, s3);
+  if (indcr) memcpy(indcr, c->indp, s3);
+  free(c->gmp); free(c->ipd); free(c->field); free(c->fieldp); free(c->ind); free(c->indp);
+  free(c->dir); free(c->dirp); free(c->phi_p_f); free(c->phi_p_c); free(c->phi_u_f); free(c->phi_u_c);
+  free(c->phi_pp_f); free(c->phi_pp_c); free(c->fmp);
+  free(c->ew_off); free(c->ew); free(c->pc_off); free(c->pc);
+  if (c->use_recip) pme_free(&c->pme);
+  return status;
+}
+
+/* AlchemicalParameters.update (AlchemicalParameters.java:316-385), OST mode */
+void ora_alchemical_factors(const ora_params *prm, double lambda, ora_alchemy *f) {
+  memset(f, 0, sizeof *f);
+  f->lpow_perm = 1.0;
+  if (lambda < prm->perm_lambda_start) {
+    f->lpow_perm = 0.0;
+  } else if (lambda <= prm->perm_lambda_end) {
+    double sc = 1.0 / (prm->perm_lambda_end - prm->perm_lambda_start);
+    double L = sc * (lambda - prm->perm_lambda_start), a = prm->perm_lambda_alpha, b = prm->perm_lambda_exponent;
+    f->lalpha = a * (1.0 - L) * (1.0 - L);
+    f->dlalpha = a * (1.0 - L) * sc;
+    f->d2lalpha = -a * sc * sc;
+    f->lpow_perm = pow(L, b);
+    f->dlpow_perm = b * pow(L, b - 1.0) * sc;
+    f->d2lpow_perm = b >= 2.0 ? b * (b - 1.0) * pow(L, b - 2.0) * sc * sc : 0.0;
+  }
+  f->lpow_pol = 1.0;
+  if (lambda < prm->pol_lambda_start) {
+    f->lpow_pol = 0.0;
+  } else if (lambda <= prm->pol_lambda_end) {
+    double sc = 1.0 / (prm->pol_lambda_end - prm->pol_lambda_start);
+    double L = sc * (lambda - prm->pol_lambda_start), b = prm->pol_lambda_exponent;
+    if (b > 0.0) {
+      f->lpow_pol = pow(L, b);
+      if (b >= 1.0) {
+        f->dlpow_pol = b * pow(L, b - 1.0) * sc;
+        if (b >= 2.0) f->d2lpow_pol = b * (b - 1.0) * pow(L, b - 2.0) * sc * sc;
+      }
+    }
+  }
+}
+
+/* The three OST electrostatics phases (ParticleMeshEwald.java:894-935):
+ * (1) condensed phase, all atoms (condensedEnergy, :1609-1643);
+ * (2) condensed phase without the soft atoms, reciprocal and polarization
+ *     only, scaled by 1 - lambda terms (condensedNoLigandSCF, :1650-1697); an
+ *     atom with use[i] = false is the same as one with no multipole and no
+ *     polarizability, which is how it is built here;
+ * (3) the soft atoms in vacuum, no Ewald screening or cutoff (ligandElec,
+ *     :1704-1790), built as the soft atoms plus their frame atoms (the
+ *     latter with no multipole or polarizability) in an aperiodic cell.  The
+ *     reference solves this phase by SOR; PCG to the same tolerance is used
+ *     here. */
+static int ost_electrostatics(ctx_t *c1, ora_energies *out, double *ind, double *indcr) {
+  const ora_system *s = c1->s;
+  const ora_params *prm = c1->prm;
+  int n = s->n;
+  double lam = prm->lambda;
+  ora_alchemy f;
+  ora_alchemical_factors(prm, lam, &f);
+  int status = 0;
+  /* phase 1 */
+  phase_t p1 = PHASE_PLAIN;
+  p1.soft_pairs = 1;
+  p1.l2 = f.lpow_perm; p1.lalpha = f.lalpha; p1.dlalpha = f.dlalpha; p1.d2lalpha = f.d2lalpha;
+  p1.perm = f.lpow_perm; p1.dperm = f.dlpow_perm; p1.d2perm = f.d2lpow_perm;
+  if (lam < prm->pol_lambda_start) { p1.polar = 0; p1.pol = 0.0; }
+  else { p1.pol = f.lpow_pol; p1.dpol = f.dlpow_pol; p1.d2pol = f.d2lpow_pol; }
+  c1->ph = &p1;
+  status = elec_phase(c1, out, ind, indcr);
+
+  /* phase 2: no soft atoms */
+  int nsoft = 0;
+  for (int i = 0; i < n; i++) nsoft += s->soft[i] != 0;
+  if (nsoft < n) {
+    double *mp = malloc(10 * (size_t)n * sizeof(double)), *pol = malloc((size_t)n * sizeof(double));
+    memcpy(mp, s->mpole, 10 * (size_t)n * sizeof(double));
+    memcpy(pol, s->polarity, (size_t)n * sizeof(double));
+    for (int i = 0; i < n; i++)
+      if (s->soft[i]) { memset(mp + 10 * i, 0, 10 * sizeof(double)); pol[i] = 0.0; }
+    ora_system s2 = *s;
+    s2.mpole = mp; s2.polarity = pol;
+    phase_t p2 = PHASE_PLAIN;
+    p2.perm_real = 0;
+    p2.perm = 1.0 - f.lpow_perm; p2.dperm = -f.dlpow_perm; p2.d2perm = -f.d2lpow_perm;
+    if (lam <= prm->pol_lambda_end && prm->no_ligand_condensed_scf) {
+      p2.pol = 1.0 - f.lpow_pol; p2.dpol = -f.dlpow_pol; p2.d2pol = -f.d2lpow_pol;
+    } else {
+      p2.polar = 0; p2.pol = 0.0;
+    }
+    ctx_t c2;
+    ctx_open(&c2, &s2, prm, &p2);
+    ora_energies e2;
+    memset(&e2, 0, sizeof e2);
+    int st = elec_phase(&c2, &e2, NULL, NULL);
+    if (!status) status = st;
+    for (int q = 0; q < 3 * n; q++) { c1->grad[q] += c2.grad[q]; c1->lgrad[q] += c2.lgrad[q]; }
+    out->perm_real += e2.perm_real; out->perm_self += e2.perm_self; out->perm_recip += e2.perm_recip;
+    out->perm_charge_corr += e2.perm_charge_corr;
+    out->pol_real += e2.pol_real; out->pol_self += e2.pol_self; out->pol_recip += e2.pol_recip;
+    out->dudl += e2.dudl; out->d2udl2 += e2.d2udl2;
+    for (int q = 0; q < 9; q++) out->vir[q] += e2.vir[q];
+    ctx_close(&c2);
+    free(mp); free(pol);
+  }
+
+  /* phase 3: soft atoms in vacuum */
+  if (prm->ligand_vapor_elec && nsoft > 0) {
+    int *map = malloc((size_t)n * sizeof(int)), *inv = malloc((size_t)n * sizeof(int));
+    for (int i = 0; i < n; i++) inv[i] = -1;
+    int m = 0;
+    for (int i = 0; i < n; i++)
+      if (s->soft[i]) { inv[i] = m; map[m++] = i; }
+    for (int i = 0; i < n; i++)
+      if (s->soft[i])
+        for (int j = 0; j < 3; j++) {
+          int a = s->axis[3 * i + j];
+          if (a >= 0 && inv[a] < 0) { inv[a] = m; map[m++] = a; }
+        }
+    double *xyz = malloc(3 * (size_t)m * sizeof(double)), *mp = calloc(10 * (size_t)m, sizeof(double));
+    double *pol = calloc((size_t)m, sizeof(double)), *th = malloc((size_t)m * sizeof(double));
+    double *pd = malloc((size_t)m * sizeof(double)), *rv = calloc((size_t)m, sizeof(double));
+    int *fr = calloc((size_t)m, sizeof(int)), *ax = malloc(3 * (size_t)m * sizeof(int));
+    int *grp = malloc((size_t)m * sizeof(int)), *mol = malloc((size_t)m * sizeof(int));
+    int *sf = malloc((size_t)m * sizeof(int)), *ri = malloc((size_t)m * sizeof(int));
+    int *vc = calloc((size_t)m, sizeof(int));
+    int *moff = calloc((size_t)m + 1, sizeof(int));
+    int nnz = s->mask_off[n];
+    int *mat = malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int)), *mrel = malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int));
+    int t = 0;
+    for (int j = 0; j < m; j++) {
+      int i = map[j];
+      memcpy(xyz + 3 * j, s->xyz + 3 * i, 3 * sizeof(double));
+      th[j] = s->thole[i]; pd[j] = s->pdamp[i]; grp[j] = s->group[i];
+      mol[j] = s->molecule ? s->molecule[i] : i;
+      sf[j] = s->soft[i] != 0; ri[j] = j;
+      if (s->soft[i]) {
+        memcpy(mp + 10 * j, s->mpole + 10 * i, 10 * sizeof(double));
+        pol[j] = s->polarity[i];
+        fr[j] = s->frame[i];
+        for (int d = 0; d < 3; d++) ax[3 * j + d] = s->axis[3 * i + d] >= 0 ? inv[s->axis[3 * i + d]] : -1;
+      } else {
+        for (int d = 0; d < 3; d++) ax[3 * j + d] = -1;
+      }
+      for (int q = s->mask_off[i]; q < s->mask_off[i + 1]; q++)
+        if (inv[s->mask_atom[q]] >= 0) { mat[t] = inv[s->mask_atom[q]]; mrel[t] = s->mask_rel[q]; t++; }
+      moff[j + 1] = t;
+    }
+    double vbox[6] = {0, 0, 0, 90, 90, 90};
+    ora_system s3 = *s;
+    s3.n = m; s3.xyz = xyz; s3.frame = fr; s3.axis = ax; s3.mpole = mp; s3.polarity = pol; s3.thole = th;
+    s3.pdamp = pd; s3.group = grp; s3.molecule = mol; s3.vdw_class = vc; s3.red_index = ri; s3.red_value = rv;
+    s3.mask_off = moff; s3.mask_atom = mat; s3.mask_rel = mrel; s3.box = vbox; s3.soft = sf;
+    ora_params q3 = *prm;
+    q3.ewald_cutoff = 1.0e4;   /* setEwaldParameters(Double.MAX_VALUE, 0.0) */
+    q3.vdw_cutoff = 0.0;
+    q3.list_buffer = 0.0;
+    q3.do_vdw = 0;
+    phase_t p3 = PHASE_PLAIN;
+    p3.soft_pairs = 1; p3.vapor = 1;
+    p3.l2 = 1.0 - f.lpow_perm;
+    p3.perm = 1.0 - f.lpow_perm; p3.dperm = -f.dlpow_perm; p3.d2perm = -f.d2lpow_perm;
+    if (lam <= prm->pol_lambda_end) {
+      p3.pol = 1.0 - f.lpow_pol; p3.dpol = -f.dlpow_pol; p3.d2pol = -f.d2lpow_pol;
+    } else {
+      p3.polar = 0; p3.pol = 0.0;
+    }
+    ctx_t c3;
+    ctx_open(&c3, &s3, &q3, &p3);
+    ora_energies e3;
+    memset(&e3, 0, sizeof e3);
+    int st = elec_phase(&c3, &e3, NULL, NULL);
+    if (!status) status = st;
+    for (int j = 0; j < m; j++)
+      for (int d = 0; d < 3; d++) {
+        c1->grad[3 * map[j] + d] += c3.grad[3 * j + d];
+        c1->lgrad[3 * map[j] + d] += c3.lgrad[3 * j + d];
+      }
+    out->perm_real += e3.perm_real;
+    out->pol_real += e3.pol_real;
+    out->dudl += e3.dudl; out->d2udl2 += e3.d2udl2;
+    for (int q = 0; q < 9; q++) out->vir[q] += e3.vir[q];
+    ctx_close(&c3);
+    free(map); free(inv); free(xyz); free(mp); free(pol); free(th); free(pd); free(rv); free(fr); free(ax);
+    free(grp); free(mol); free(sf); free(ri); free(vc); free(moff); free(mat); free(mrel);
+  }
+  return status;
+}
+
+int ora_evaluate_lambda(const ora_system *s, const ora_params *prm, ora_energies *out, double *grad, double *ind,
+                        double *indcr, double *dudx_dl) {
+  memset(out, 0, sizeof(*out));
+  int n = s->n;
+  size_t s3 = 3 * (size_t)n * sizeof(double);
+  ctx_t c;
+  ctx_open(&c, s, prm, &PHASE_PLAIN);
+  if (prm->do_vdw) out->vdw = vdw_energy(&c, c.grad, &out->n_vdw, out->vir, &out->dudl, &out->d2udl2, c.lgrad);
+  int status = 0;
+  if (prm->do_pme) {
+    int nsoft = 0;
+    if (s->soft)
+      for (int i = 0; i < n; i++) nsoft += s->soft[i] != 0;
+    if (prm->elec_lambda && nsoft > 0) status = ost_electrostatics(&c, out, ind, indcr);
+    else status = elec_phase(&c, out, ind, indcr);
+  }
+  out->status = status;
   out->perm = out->perm_real + out->perm_self + out->perm_recip + out->perm_charge_corr;
   out->pol = out->pol_real + out->pol_self + out->pol_recip;
   out->total = out->perm + out->pol + out->vdw;
   if (!isfinite(out->total) && out->status == 0) out->status = 1;
-  if (grad) memcpy(grad, c->grad, s3);
-  free(c->grad); free(c->torque);
-  nlist_free(&c->nl);
+  if (grad) memcpy(grad, c.grad, s3);
+  if (dudx_dl) memcpy(dudx_dl, c.lgrad, s3);
+  ctx_close(&c);
   return out->status;
 }

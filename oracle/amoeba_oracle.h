@@ -81,6 +81,16 @@ typedef struct {
   double vdw_lambda_exponent; /* 3 */
   double vdw_lambda_end;    /* 1 */
   int intermolecular_softcore, intramolecular_softcore; /* both-soft pairs (0, 0) */
+  /* electrostatic alchemistry in OST mode (AlchemicalParameters.java:171-249,
+   * 316-385; ParticleMeshEwald.java:894-935, 1609-1790) */
+  int elec_lambda;          /* apply lambda to the electrostatics of the soft atoms (0) */
+  double perm_lambda_alpha; /* 2 */
+  double perm_lambda_exponent; /* 3 */
+  double perm_lambda_start, perm_lambda_end; /* 0.4, 1 */
+  double pol_lambda_exponent; /* 3 */
+  double pol_lambda_start, pol_lambda_end; /* 0.75, 1 */
+  int ligand_vapor_elec;    /* 1 */
+  int no_ligand_condensed_scf; /* 1 */
 } ora_params;
 
 typedef struct {
@@ -93,7 +103,7 @@ typedef struct {
   int n_vdw, n_perm;
   int status;               /* 0 ok, 1 nan energy, 2 scf diverged, 3 scf maxiter, 4 bad torque */
   double vir[9];            /* strain derivative W_ab = dE/d(eps_ab), row-major, kcal/mol */
-  double dudl, d2udl2;      /* vdW softcore lambda derivatives */
+  double dudl, d2udl2;      /* lambda derivatives (vdW softcore and, with elec_lambda, electrostatics) */
 } ora_energies;
 
 /* Full evaluation.  grad [3n] (kcal/mol/A), ind, indcr [3n] (e*A) may be NULL. */
@@ -102,6 +112,15 @@ int ora_evaluate_lambda(const ora_system *sys, const ora_params *prm, ora_energi
                         double *ind, double *indcr, double *dudx_dl);
 int ora_evaluate(const ora_system *sys, const ora_params *prm, ora_energies *out,
                  double *grad, double *ind, double *indcr);
+
+/* OST lambda factors of AlchemicalParameters.update (lPowPerm, lAlpha,
+ * lPowPol and their first and second lambda derivatives). */
+typedef struct {
+  double lpow_perm, dlpow_perm, d2lpow_perm;
+  double lalpha, dlalpha, d2lalpha;
+  double lpow_pol, dlpow_pol, d2lpow_pol;
+} ora_alchemy;
+void ora_alchemical_factors(const ora_params *prm, double lambda, ora_alchemy *f);
 
 /* Global-frame multipoles [10n] (RotateMultipolesLoop). */
 void ora_rotate_multipoles(const ora_system *sys, double *gmp);

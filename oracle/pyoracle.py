@@ -48,7 +48,16 @@ class OraParams(C.Structure):
                 ("polarization", C.c_int), ("do_vdw", C.c_int), ("do_pme", C.c_int), ("nthreads", C.c_int),
                 ("max_scf_iter", C.c_int), ("lambda_", C.c_double), ("vdw_lambda_alpha", C.c_double),
                 ("vdw_lambda_exponent", C.c_double), ("vdw_lambda_end", C.c_double),
-                ("intermolecular_softcore", C.c_int), ("intramolecular_softcore", C.c_int)]
+                ("intermolecular_softcore", C.c_int), ("intramolecular_softcore", C.c_int),
+                ("elec_lambda", C.c_int), ("perm_lambda_alpha", C.c_double), ("perm_lambda_exponent", C.c_double),
+                ("perm_lambda_start", C.c_double), ("perm_lambda_end", C.c_double),
+                ("pol_lambda_exponent", C.c_double), ("pol_lambda_start", C.c_double),
+                ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int), ("no_ligand_condensed_scf", C.c_int)]
+
+
+class OraAlchemy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("lpow_perm", "dlpow_perm", "d2lpow_perm", "lalpha", "dlalpha", "d2lalpha",
+                                          "lpow_pol", "dlpow_pol", "d2lpow_pol")]
 
 
 class OraEnergies(C.Structure):
@@ -93,6 +102,7 @@ def lib():
         L.ora_ewald_recip_exact.restype = C.c_double
         L.ora_influence.argtypes = [dp, C.POINTER(C.c_int), C.c_int, C.c_double, dp]
         L.ora_reduced_xyz.argtypes = [C.POINTER(OraSystem), dp]
+        L.ora_alchemical_factors.argtypes = [C.POINTER(OraParams), C.c_double, C.POINTER(OraAlchemy)]
         _lib = L
     return _lib
 
@@ -146,7 +156,17 @@ def make_params(p, box=None, nthreads=0):
                      (C.c_double * 3)(*p.v_scale), p.halgren_delta, p.halgren_gamma, p.polarization,
                      int(p.do_vdw), int(p.do_pme), nthreads, p.max_scf_iter, p.lambda_value, p.vdw_lambda_alpha,
                      p.vdw_lambda_exponent, p.vdw_lambda_end, int(p.intermolecular_softcore),
-                     int(p.intramolecular_softcore))
+                     int(p.intramolecular_softcore), int(p.elec_lambda), p.perm_lambda_alpha,
+                     p.perm_lambda_exponent, p.perm_lambda_start, p.perm_lambda_end, p.pol_lambda_exponent,
+                     p.pol_lambda_start, p.pol_lambda_end, int(p.ligand_vapor_elec),
+                     int(p.no_ligand_condensed_scf))
+
+
+def alchemical_factors(params, lam):
+    """lPowPerm, lAlpha, lPowPol and their lambda derivatives (dict)."""
+    f = OraAlchemy()
+    lib().ora_alchemical_factors(C.byref(make_params(params)), float(lam), C.byref(f))
+    return {k: getattr(f, k) for k, _ in OraAlchemy._fields_}
 
 
 def evaluate(system, params, nthreads=0):
