@@ -69,6 +69,11 @@ void alloc(T **dst, size_t count) {
 }
 
 void free_all(NbHandle *h) {
+  if (h->child_env) { free_all(h->child_env); delete h->child_env; h->child_env = nullptr; }
+  if (h->child_vap) { free_all(h->child_vap); delete h->child_vap; h->child_vap = nullptr; }
+  void *lam[] = {h->d_lgrad_s, h->d_ltorque_s, h->d_lgrad_r, h->d_ltorque_r, h->d_vap_map};
+  for (void *p : lam)
+    if (p) hipFree(p);
   void *ptrs[] = {h->d_frame, h->d_axis, h->d_group0, h->d_vclass0, h->d_redidx, h->d_mpole_local, h->d_polar0,
                   h->d_thole0, h->d_ipd0, h->d_redval, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_vdw_table,
                   h->d_xyz, h->d_xyz_list, h->d_perm, h->d_iperm, h->d_cellid, h->d_cellcount, h->d_cellstart,
@@ -95,6 +100,211 @@ void free_all(NbHandle *h) {
 int fail(NbHandle *h, const std::exception &e, int code) {
   if (h) h->err = e.what();
   return code;
+}
+
+// dst[3 map[j] + d] += src[3 j + d] (map null: identity)
+__global__ void k_add_mapped(int m, const int32_t *__restrict__ map, const double *__restrict__ src,
+                             double *__restrict__ dst) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * m) return;
+  int j = t / 3, d = t % 3;
+  int i = map ? map[j] : j;
+  dst[3 * i + d] += src[t];
+}
+
+// dst[3 j + d] = src[3 map[j] + d]
+__global__ void k_gather_mapped(int m, const int32_t *__restrict__ map, const double *__restrict__ src,
+                                double *__restrict__ dst) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * m) return;
+  int j = t / 3, d = t % 3;
+  dst[t] = src[3 * map[j] + d];
+}
+
+void alloc_lambda_arrays(NbHandle *h) {
+  size_t n3 = 3 * (size_t)h->n;
+  double **v[] = {&h->d_lgrad_s, &h->d_ltorque_s, &h->d_lgrad_r, &h->d_ltorque_r};
+  for (double **p : v) {
+    alloc(p, n3);
+    HIPCHECK(hipMemset(*p, 0, sizeof(double) * n3));
+  }
+  h->ost = true;
+}
+
+// OST lambda factors (AlchemicalParameters.update, AlchemicalParameters.java:316-385)
+struct Alchemy {
+  double lpow_perm = 1.0, dlpow_perm = 0.0, d2lpow_perm = 0.0;
+  double lalpha = 0.0, dlalpha = 0.0, d2lalpha = 0.0;
+  double lpow_pol = 1.0, dlpow_pol = 0.0, d2lpow_pol = 0.0;
+};
+
+Alchemy alchemy_of(const nb_params &p, double lam) {
+  Alchemy f;
+  if (lam < p.perm_lambda_start) {
+    f.lpow_perm = 0.0;
+  } else if (lam <= p.perm_lambda_end) {
+    double sc = 1.0 / (p.perm_lambda_end - p.perm_lambda_start);
+    double L = sc * (lam - p.perm_lambda_start), a = p.perm_lambda_alpha, b = p.perm_lambda_exponent;
+    f.lalpha = a * (1.0 - L) * (1.0 - L);
+    f.dlalpha = a * (1.0 - L) * sc;
+    f.d2lalpha = -a * sc * sc;
+    f.lpow_perm = pow(L, b);
+    f.dlpow_perm = b * pow(L, b - 1.0) * sc;
+    f.d2lpow_perm = b >= 2.0 ? b * (b - 1.0) * pow(L, b - 2.0) * sc * sc : 0.0;
+  }
+  if (lam < p.pol_lambda_start) {
+    f.lpow_pol = 0.0;
+  } else if (lam <= p.pol_lambda_end) {
+    double sc = 1.0 / (p.pol_lambda_end - p.pol_lambda_start);
+    double L = sc * (lam - p.pol_lambda_start), b = p.pol_lambda_exponent;
+    if (b > 0.0) {
+      f.lpow_pol = pow(L, b);
+      if (b >= 1.0) {
+        f.dlpow_pol = b * pow(L, b - 1.0) * sc;
+        if (b >= 2.0) f.d2lpow_pol = b * (b - 1.0) * pow(L, b - 2.0) * sc * sc;
+      }
+    }
+  }
+  return f;
+}
+
+// Phase scales of the three OST evaluations at the handle's lambda
+// (condensedEnergy / condensedNoLigandSCF / ligandElec,
+// ParticleMeshEwald.java:1609-1790).
+void set_ost_phases(NbHandle *h) {
+  const nb_params &p = h->prm;
+  double lam = h->lambda;
+  Alchemy f = alchemy_of(p, lam);
+  ElecPhase p1;
+  p1.soft_pairs = 1;
+  p1.l2 = f.lpow_perm; p1.lalpha = f.lalpha; p1.dlalpha = f.dlalpha; p1.d2lalpha = f.d2lalpha;
+  p1.perm = f.lpow_perm; p1.dperm = f.dlpow_perm; p1.d2perm = f.d2lpow_perm;
+  if (lam < p.pol_lambda_start) { p1.polar = 0; p1.pol = 0.0; }
+  else { p1.pol = f.lpow_pol; p1.dpol = f.dlpow_pol; p1.d2pol = f.d2lpow_pol; }
+  h->ph = p1;
+  if (h->child_env) {
+    ElecPhase p2;
+    p2.perm_real = 0;
+    p2.perm = 1.0 - f.lpow_perm; p2.dperm = -f.dlpow_perm; p2.d2perm = -f.d2lpow_perm;
+    if (lam <= p.pol_lambda_end && p.no_ligand_condensed_scf) {
+      p2.pol = 1.0 - f.lpow_pol; p2.dpol = -f.dlpow_pol; p2.d2pol = -f.d2lpow_pol;
+    } else {
+      p2.polar = 0; p2.pol = 0.0;
+    }
+    h->child_env->ph = p2;
+  }
+  if (h->child_vap) {
+    ElecPhase p3;
+    p3.soft_pairs = 1; p3.vapor = 1;
+    p3.inter = p.intermolecular_softcore; p3.intra = p.intramolecular_softcore;
+    p3.l2 = 1.0 - f.lpow_perm;
+    p3.perm = 1.0 - f.lpow_perm; p3.dperm = -f.dlpow_perm; p3.d2perm = -f.d2lpow_perm;
+    if (lam <= p.pol_lambda_end) {
+      p3.pol = 1.0 - f.lpow_pol; p3.dpol = -f.dlpow_pol; p3.d2pol = -f.d2lpow_pol;
+    } else {
+      p3.polar = 0; p3.pol = 0.0;
+    }
+    h->child_vap->ph = p3;
+  }
+}
+
+}  // namespace
+
+extern "C" int nb_create(const nb_params *params, const nb_topology *top, void **handle);
+extern "C" int nb_set_cell(void *handle, const double box[6]);
+
+namespace {
+
+// The no-ligand phase is this system with the soft atoms' multipoles and
+// polarizabilities removed (an atom with use[i] = false contributes nothing
+// else); the vapor phase is the soft atoms plus their frame atoms (the
+// latter without multipole or polarizability) in an aperiodic cell with no
+// Ewald screening and no cutoff (ParticleMeshEwald.java:1704-1790; its SOR
+// solve is replaced by PCG to the same tolerance).
+void create_ost_children(NbHandle *h, const nb_params *params, const nb_topology *top) {
+  int n = top->n_atoms;
+  nb_params cp = *params;
+  cp.elec_lambda = 0;
+  cp.do_vdw = 0;
+  cp.lambda = 1.0;
+  // no-ligand phase
+  std::vector<double> mp(top->multipole, top->multipole + 10 * (size_t)n), pol(top->polarizability,
+                                                                              top->polarizability + n);
+  int nsoft = 0;
+  for (int i = 0; i < n; i++)
+    if (top->soft[i]) {
+      nsoft++;
+      std::fill(mp.begin() + 10 * (size_t)i, mp.begin() + 10 * (size_t)i + 10, 0.0);
+      pol[i] = 0.0;
+    }
+  if (nsoft < n) {
+    nb_topology t2 = *top;
+    t2.multipole = mp.data();
+    t2.polarizability = pol.data();
+    t2.soft = nullptr;
+    void *c = nullptr;
+    int rc = nb_create(&cp, &t2, &c);
+    if (rc != NB_OK) throw std::runtime_error("could not create the no-ligand phase");
+    h->child_env = (NbHandle *)c;
+    alloc_lambda_arrays(h->child_env);
+  }
+  // vapor phase
+  if (params->ligand_vapor_elec) {
+    std::vector<int32_t> map, inv(n, -1);
+    for (int i = 0; i < n; i++)
+      if (top->soft[i]) { inv[i] = (int)map.size(); map.push_back(i); }
+    for (int i = 0; i < n; i++)
+      if (top->soft[i])
+        for (int a = 0; a < 3; a++) {
+          int k = top->axis[3 * i + a];
+          if (k >= 0 && inv[k] < 0) { inv[k] = (int)map.size(); map.push_back(k); }
+        }
+    int m = (int)map.size();
+    std::vector<int32_t> fr(m, NB_FRAME_NONE), ax(3 * (size_t)m, -1), grp(m), vc(m, 0), ri(m), sf(m), mol(m);
+    std::vector<double> vmp(10 * (size_t)m, 0.0), vpol(m, 0.0), th(m), pd(m), rv(m, 0.0);
+    std::vector<int32_t> moff(m + 1, 0), mat, mrel;
+    for (int j = 0; j < m; j++) {
+      int i = map[j];
+      th[j] = top->thole[i];
+      pd[j] = top->pdamp[i];
+      grp[j] = top->group[i];
+      ri[j] = j;
+      sf[j] = top->soft[i] != 0;
+      mol[j] = top->molecule ? top->molecule[i] : i;
+      if (top->soft[i]) {
+        std::copy(top->multipole + 10 * (size_t)i, top->multipole + 10 * (size_t)i + 10, vmp.begin() + 10 * (size_t)j);
+        vpol[j] = top->polarizability[i];
+        fr[j] = top->frame[i];
+        for (int a = 0; a < 3; a++) {
+          int k = top->axis[3 * i + a];
+          ax[3 * j + a] = k >= 0 ? inv[k] : -1;
+        }
+      }
+      for (int q = top->mask_offset[i]; q < top->mask_offset[i + 1]; q++)
+        if (inv[top->mask_atom[q]] >= 0) {
+          mat.push_back(inv[top->mask_atom[q]]);
+          mrel.push_back(top->mask_rel[q]);
+        }
+      moff[j + 1] = (int32_t)mat.size();
+    }
+    nb_topology t3 = *top;
+    t3.n_atoms = m;
+    t3.frame = fr.data(); t3.axis = ax.data(); t3.multipole = vmp.data(); t3.polarizability = vpol.data();
+    t3.thole = th.data(); t3.pdamp = pd.data(); t3.group = grp.data(); t3.vdw_class = vc.data();
+    t3.red_index = ri.data(); t3.red_value = rv.data(); t3.mask_offset = moff.data();
+    t3.mask_atom = mat.data(); t3.mask_rel = mrel.data(); t3.soft = sf.data(); t3.molecule = mol.data();
+    nb_params vp = cp;
+    vp.ewald_cutoff = 1.0e4;   // setEwaldParameters(Double.MAX_VALUE, 0.0)
+    vp.vdw_cutoff = 0.0;
+    void *c = nullptr;
+    int rc = nb_create(&vp, &t3, &c);
+    if (rc != NB_OK) throw std::runtime_error("could not create the vapor phase");
+    h->child_vap = (NbHandle *)c;
+    alloc_lambda_arrays(h->child_vap);
+    if (nb_set_cell(c, nullptr) != NB_OK) throw std::runtime_error("could not set the vapor cell");
+    upload(&h->d_vap_map, map.data(), m);
+    h->n_vap = m;
+  }
 }
 
 }  // namespace
@@ -309,6 +519,13 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_count, 4);
     HIPCHECK(hipHostMalloc((void **)&h->h_pinned, sizeof(double) * 64, hipHostMallocDefault));
     memset(&h->last, 0, sizeof(h->last));
+    if (params->elec_lambda && h->has_soft && params->do_pme) {
+      if (!(params->perm_lambda_end > params->perm_lambda_start && params->pol_lambda_end > params->pol_lambda_start))
+        throw std::runtime_error("invalid alchemical lambda windows");
+      alloc_lambda_arrays(h);
+      h->elec_lambda = true;
+      create_ost_children(h, params, top);
+    }
     *handle = h;
     return NB_OK;
   } catch (std::exception &e) {
@@ -332,6 +549,10 @@ int nb_set_cell(void *handle, const double box[6]) {
     h->cell_set = true;
     h->list_valid = false;
     if (h->prm.do_pme && !h->cell.aperiodic && h->prm.ewald_alpha > 0.0) nb_pme_setup(h);
+    if (h->child_env) {
+      int rc = nb_set_cell(h->child_env, box);
+      if (rc != NB_OK) throw std::runtime_error(h->child_env->err);
+    }
     return NB_OK;
   } catch (std::exception &e) {
     return fail(h, e, NB_ERR_HIP);
@@ -384,7 +605,7 @@ int nb_struct_sizes(int32_t sizes[3]) {
   return NB_OK;
 }
 
-int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
+static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
   NbHandle *h = (NbHandle *)handle;
   if (!h) return NB_ERR_INVALID;
   if (!h->cell_set || !h->positions_set) {
@@ -407,7 +628,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
     h->want_dudl = (flags & NB_DUDL) != 0;
     h->dudl_valid = false;
     if (h->want_vir) {
-      if (h->cell.aperiodic) throw std::runtime_error("the virial needs a periodic cell");
+      // an aperiodic system has pair and frame terms only
       // 9 partial sums per block of every contributing launch
       long pb = xcd_grid((long)n * 16, 256).x, sb = xcd_grid((long)n * 4, 256).x, ab = (n + 127) / 128;
       long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + 64;
@@ -436,7 +657,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
         NbSide side(h);
         nb_vdw(h);
       }
-      bool polar = h->prm.polarization != NB_POL_NONE;
+      bool polar = h->prm.polarization != NB_POL_NONE && h->ph.polar;
       if (h->prm.do_pme) {
         nb_filter_lists(h);
         // real-space permanent field (side) || permanent PME (main)
@@ -502,7 +723,14 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
       e.pol_self = en[E_POL_SELF];
       e.pol_recip = en[E_POL_RECIP];
       double q = en[E_QTOT];
-      if (q != 0.0) e.perm_charge_corr = -0.5 * h->prm.electric * M_PI * q * q / (h->cell.vol * h->prm.ewald_alpha * h->prm.ewald_alpha);
+      if (q != 0.0) {
+        double cc = -0.5 * h->prm.electric * M_PI * q * q / (h->cell.vol * h->prm.ewald_alpha * h->prm.ewald_alpha);
+        e.perm_charge_corr = h->ph.perm * cc;
+        if (h->ost) {
+          e.dudl += h->ph.dperm * cc;
+          e.d2udl2 += h->ph.d2perm * cc;
+        }
+      }
     }
     if (h->want_vir) {
       // the net-charge correction scales as 1 / V
@@ -523,6 +751,77 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
                            "maximum SCF iterations reached", "undefined torque"};
       h->err = status < 5 ? msg[status] : "error";
     }
+    return status;
+  } catch (std::exception &ex) {
+    return fail(h, ex, NB_ERR_HIP);
+  }
+}
+
+// With electrostatic alchemistry the energy is the sum of the condensed phase
+// (this handle) and the no-ligand and vapor phases (its two child handles),
+// ParticleMeshEwald.java:894-935; gradients, dU/dX/dL and the virial add up.
+int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  if (!h->elec_lambda) return nb_evaluate_one(h, flags, out);
+  if (!h->cell_set || !h->positions_set) {
+    h->err = "nb_set_cell and nb_set_positions must be called before nb_evaluate";
+    return NB_ERR_INVALID;
+  }
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    set_ost_phases(h);
+    nb_energies e;
+    int status = nb_evaluate_one(h, flags, &e);
+    if (status != NB_OK) return status;
+    uint32_t cf = flags & (NB_GRAD | NB_VIRIAL | NB_DUDL);
+    NbHandle *kids[2] = {h->child_env, h->child_vap};
+    for (int c = 0; c < 2; c++) {
+      NbHandle *k = kids[c];
+      if (!k) continue;
+      int m = k->n;
+      if (c == 0) {
+        HIPCHECK(hipMemcpyAsync(k->d_xyz, h->d_xyz, sizeof(double) * 3 * m, hipMemcpyDeviceToDevice, k->stream));
+      } else {
+        hipLaunchKernelGGL(k_gather_mapped, dim3((3 * m + 255) / 256), dim3(256), 0, k->stream, m, h->d_vap_map,
+                           h->d_xyz, k->d_xyz);
+        HIPCHECK(hipGetLastError());
+      }
+      k->positions_set = true;
+      nb_energies ek;
+      int st = nb_evaluate_one(k, cf, &ek);
+      if (st != NB_OK) {
+        h->err = std::string(c ? "vapor phase: " : "no-ligand phase: ") + k->err;
+        return st;
+      }
+      e.perm_real += ek.perm_real;
+      e.perm_self += ek.perm_self;
+      e.perm_recip += ek.perm_recip;
+      e.perm_charge_corr += ek.perm_charge_corr;
+      e.pol_real += ek.pol_real;
+      e.pol_self += ek.pol_self;
+      e.pol_recip += ek.pol_recip;
+      e.dudl += ek.dudl;
+      e.d2udl2 += ek.d2udl2;
+      const int32_t *map = c ? h->d_vap_map : nullptr;
+      dim3 G((3 * m + 255) / 256);
+      hipLaunchKernelGGL(k_add_mapped, G, dim3(256), 0, h->stream, m, map, (const double *)k->d_grad, h->d_grad);
+      if (flags & NB_DUDL)
+        hipLaunchKernelGGL(k_add_mapped, G, dim3(256), 0, h->stream, m, map, (const double *)k->d_dudx, h->d_dudx);
+      HIPCHECK(hipGetLastError());
+      if (flags & NB_VIRIAL)
+        for (int q = 0; q < 9; q++) h->vir[q] += k->vir[q];
+    }
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    e.perm = e.perm_real + e.perm_self + e.perm_recip + e.perm_charge_corr;
+    e.pol = e.pol_real + e.pol_self + e.pol_recip;
+    e.total = e.perm + e.pol + e.vdw;
+    if (!std::isfinite(e.total)) {
+      h->err = "NaN energy";
+      status = NB_ERR_NAN_ENERGY;
+    }
+    h->last = e;
+    if (out) *out = e;
     return status;
   } catch (std::exception &ex) {
     return fail(h, ex, NB_ERR_HIP);

@@ -204,9 +204,9 @@ static int estimate_neighbors(const NbHandle *h, double r) {
   double vol = h->cell.aperiodic ? 1.0 : h->cell.vol;
   double rho = h->cell.aperiodic ? 0.12 : h->n / vol;
   double v = 4.0 / 3.0 * M_PI * r * r * r;
-  int est = (int)(rho * v * 1.4) + 64;
+  double est = rho * v * 1.4 + 64.0;
   if (est > h->n) est = h->n;
-  return est < 16 ? 16 : est;
+  return est < 16.0 ? 16 : (int)est;
 }
 
 bool nb_need_rebuild(NbHandle *h) {

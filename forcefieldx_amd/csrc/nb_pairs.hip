@@ -266,39 +266,154 @@ __device__ __forceinline__ void pol_energy_pair(const double4 &pi, const AtomMP 
   if (w9) pair_strain(w9, f, r);
 }
 
+// Permanent pair under an OST phase.  A soft pair sees r_eff = sqrt(r^2 +
+// lalpha) in the radial functions and the unshifted r in the angular ones, and
+// is scaled by l2.  Its lambda derivatives use d bn_j / d(r^2) = -bn_{j+1} / 2
+// with d lalpha / dlambda = -2 dlalpha: the radial set shifted by one index is
+// dE/dlambda / dlalpha, by two the second derivative (RealSpaceEnergyRegion.java
+// :1477-1690); the force of the once-shifted set is its coordinate derivative.
+template <typename U>
+__device__ __forceinline__ void perm_energy_pair_ost(const double4 &pi, const AtomMP &ai, const double4 &pk,
+                                                     const AtomMP &ak, int code, bool soft, const CellD &cell,
+                                                     double aewald, const ScaleTab &sc, const ElecPhase &ph, double &ep,
+                                                     long long &cnt, V3<double> &F, V3<double> &Tq, double *w9,
+                                                     double &dl1, double &dl2, V3<double> &FL, V3<double> &TL) {
+  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+  double r2 = image(cell, dx, dy, dz);
+  MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
+  V3<U> r{(U)dx, (U)dy, (U)dz};
+  U rl = sqrt((U)(soft ? r2 + ph.lalpha : r2)), rinv = (U)1 / rl;
+  U bn[7], rr[7];
+  ewald_bn<U, 6>(rl, rinv, (U)aewald, bn);
+  coulomb_rr<U, 6>(rinv, rr);
+  PermGeo<U> g;
+  perm_geometry(r, I, K, g);
+  V3<U> f{0, 0, 0}, tq{0, 0, 0};
+  U epair = perm_terms<U, true>(r, I, K, g, bn, (U)1, f, tq);
+  U s1 = (U)1 - (U)sc.m[code];
+  if (s1 != (U)0) epair -= s1 * perm_terms<U, true>(r, I, K, g, rr, -s1, f, tq);
+  U l2 = soft ? (U)ph.l2 : (U)1;
+  if (epair != (U)0) cnt++;
+  ep += (double)(l2 * epair);
+  acc3(F, l2 * f);
+  acc3(Tq, l2 * tq);
+  if (w9) pair_strain(w9, l2 * f, r);
+  if (soft) {
+    V3<U> fr{0, 0, 0}, tr{0, 0, 0}, dummy{0, 0, 0};
+    U dR = perm_terms<U, true>(r, I, K, g, bn + 1, (U)1, fr, tr);
+    U d2R = perm_terms<U, false>(r, I, K, g, bn + 2, (U)1, dummy, dummy);
+    if (s1 != (U)0) {
+      dR -= s1 * perm_terms<U, true>(r, I, K, g, rr + 1, -s1, fr, tr);
+      d2R -= s1 * perm_terms<U, false>(r, I, K, g, rr + 2, (U)1, dummy, dummy);
+    }
+    double e = (double)epair, d1 = (double)dR, d2 = (double)d2R, l2d = ph.l2;
+    dl1 += ph.dperm * e + l2d * ph.dlalpha * d1;
+    dl2 += ph.d2perm * e + 2.0 * ph.dperm * ph.dlalpha * d1 + l2d * ph.d2lalpha * d1 +
+           l2d * ph.dlalpha * ph.dlalpha * d2;
+    U ca = (U)ph.dperm, cb = (U)(l2d * ph.dlalpha);
+    acc3(FL, ca * f + cb * fr);
+    acc3(TL, ca * tq + cb * tr);
+  }
+}
+
+// Polarization pair under an OST phase: unshifted, scaled by pol; its lambda
+// terms are dpol / d2pol times the pair.
+template <typename U>
+__device__ __forceinline__ void pol_energy_pair_ost(const double4 &pi, const AtomMP &ai, const double *u,
+                                                    const double *up, int s, const double4 &pk, const AtomMP &ak,
+                                                    int k, int code, const CellD &cell, double aewald,
+                                                    const ScaleTab &sc, int direct, const ElecPhase &ph, double &ei,
+                                                    V3<double> &F, V3<double> &Tq, double *w9, double &dl1,
+                                                    double &dl2, V3<double> &FL, V3<double> &TL) {
+  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+  double r2 = image(cell, dx, dy, dz);
+  MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
+  V3<U> r{(U)dx, (U)dy, (U)dz};
+  U rl = sqrt((U)r2), rinv = (U)1 / rl;
+  U bn[5], rr[5];
+  ewald_bn<U, 4>(rl, rinv, (U)aewald, bn);
+  coulomb_rr<U, 4>(rinv, rr);
+  V3<U> f{0, 0, 0}, tq{0, 0, 0};
+  Thole<U> th = thole<U, true>(rl, r, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
+  double e = (double)pol_terms<U, true>(r, I, K, ld3t<U>(u, s), ld3t<U>(up, s), ld3t<U>(u, k), ld3t<U>(up, k), bn, rr,
+                                        th, (U)sc.p[code], (U)sc.d[code], direct != 0, f, tq);
+  U w = (U)ph.pol, dw = (U)ph.dpol;
+  ei += ph.pol * e;
+  dl1 += ph.dpol * e;
+  dl2 += ph.d2pol * e;
+  acc3(F, w * f);
+  acc3(Tq, w * tq);
+  acc3(FL, dw * f);
+  acc3(TL, dw * tq);
+  if (w9) pair_strain(w9, w * f, r);
+}
+
 // POL = false: permanent energy (slot E_PERM_REAL, pair count); POL = true:
 // polarization energy (slot E_POL_REAL).  LIST 0: the whole Ewald row (FP64
 // form); LIST 1: the Ewald row without the masked pairs (mixed form, T =
 // float); LIST 2: only the masked row, in FP64 (mixed form).
-// VIR: also the pair strain derivative, 9 partial sums per block into vpart
-template <typename T, int LPA, bool POL, int LIST, bool VIR>
+// VIR: also the pair strain derivative, 9 partial sums per block into vpart.
+// OST: the phase scales of ph, with dU/dL, d2U/dL2 (slots E_DUDL, E_D2UDL2)
+// and the lambda gradient and torque added into lgrad / ltorque; soft (sorted,
+// may be null) marks the alchemical atoms and mol their molecules.
+template <typename T, int LPA, bool POL, int LIST, bool VIR, bool OST>
 __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
                                                      const uint32_t *__restrict__ lst, const int32_t *__restrict__ lcnt,
                                                      int maxl, CellD cell, double aewald, ScaleTab sc, int direct,
                                                      const double *__restrict__ u, const double *__restrict__ up,
                                                      double el, double *__restrict__ grad, double *__restrict__ torque,
                                                      double *__restrict__ energy, int64_t *__restrict__ count,
-                                                     double *__restrict__ vpart) {
+                                                     double *__restrict__ vpart, ElecPhase ph,
+                                                     const int32_t *__restrict__ soft, const int32_t *__restrict__ mol,
+                                                     double *__restrict__ lgrad, double *__restrict__ ltorque) {
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
-  double e = 0.0;
+  double e = 0.0, dl1 = 0.0, dl2 = 0.0;
   long long cnt = 0;
-  V3<double> F{0, 0, 0}, Tq{0, 0, 0};
+  V3<double> F{0, 0, 0}, Tq{0, 0, 0}, FL{0, 0, 0}, TL{0, 0, 0};
   double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   double *wp = VIR ? w9 : nullptr;
   if (valid) {
     double4 pi = pos[s];
     AtomMP ai = mp[s];
     int m = lcnt[s];
+    int si = (OST && soft) ? soft[s] : 0, moli = (OST && soft) ? mol[s] : 0;
     const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
       uint32_t en = row[t];
       int k = en & NB_INDEX_MASK, code = en >> NB_CODE_SHIFT;
       if (LIST == 1 && sc.masked[code]) continue;
-      if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, e, F, Tq, wp);
-      else perm_energy_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, e, cnt, F, Tq, wp);
+      if (OST) {
+        int sk = soft ? soft[k] : 0;
+        if (ph.vapor && si && sk) {
+          // vapor phase: both-soft pairs that the condensed phase softens are
+          // left out (RealSpaceEnergyRegion.java:864-870)
+          bool same = moli == mol[k];
+          if ((ph.inter && !same) || (ph.intra && same)) continue;
+        }
+        bool sp = ph.soft_pairs && (si || sk);
+        if (POL)
+          pol_energy_pair_ost<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, ph, e, F, Tq, wp,
+                                 dl1, dl2, FL, TL);
+        else
+          perm_energy_pair_ost<T>(pi, ai, pos[k], mp[k], code, sp, cell, aewald, sc, ph, e, cnt, F, Tq, wp, dl1, dl2,
+                                  FL, TL);
+      } else {
+        if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, e, F, Tq, wp);
+        else perm_energy_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, e, cnt, F, Tq, wp);
+      }
     }
+  }
+  if (OST) {
+    group_sum3<LPA>(FL);
+    group_sum3<LPA>(TL);
+    if (valid && lane == 0) {
+      add3(lgrad, s, el * FL);
+      add3(ltorque, s, el * TL);
+    }
+    block_add(0.5 * el * dl1, energy + E_DUDL);
+    block_add(0.5 * el * dl2, energy + E_D2UDL2);
   }
   if (VIR) {
 #pragma unroll
@@ -524,7 +639,7 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
 
 #define NB_LPA_SP 4
 
-template <bool VIR>
+template <bool VIR, bool OST>
 static void real_energy_t(NbHandle *h, bool polar) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
@@ -533,34 +648,36 @@ static void real_energy_t(NbHandle *h, bool polar) {
   int direct = h->prm.polarization == NB_POL_DIRECT;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256), Gs = xcd_grid((long)n * NB_LPA_SP, 256);
   double el = h->prm.electric;
+  bool perm = !OST || h->ph.perm_real;
   auto vr = [&](dim3 g) { return VIR ? nb_vir_region(h, g.x) : (double *)nullptr; };
+#define NB_RE(T, L, POL, LIST, GRID, ROWS, CNT, MAXR)                                                               \
+  NB_LAUNCH((k_real_energy<T, L, POL, LIST, VIR, OST>), GRID, B, 0, n, h->d_pos4, h->d_mp, ROWS, CNT, MAXR, h->cell,  \
+            aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count, vr(GRID),      \
+            h->ph, h->d_soft, h->d_mol, h->d_lgrad_s, h->d_ltorque_s)
   if (h->prm.precision == NB_PREC_MIXED) {
-    NB_LAUNCH((k_real_energy<float, NB_LPA, false, 1, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
-              h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count, vr(G));
-    NB_LAUNCH((k_real_energy<double, NB_LPA_SP, false, 2, VIR>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
-              h->maxsp, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
-              h->d_count, vr(Gs));
+    if (perm) {
+      NB_RE(float, NB_LPA, false, 1, G, h->d_ew, h->d_new, h->maxew);
+      NB_RE(double, NB_LPA_SP, false, 2, Gs, h->d_sp, h->d_nsp, h->maxsp);
+    }
     if (polar) {
-      NB_LAUNCH((k_real_energy<float, NB_LPA, true, 1, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
-                h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count, vr(G));
-      NB_LAUNCH((k_real_energy<double, NB_LPA_SP, true, 2, VIR>), Gs, B, 0, n, h->d_pos4, h->d_mp, h->d_sp, h->d_nsp,
-                h->maxsp, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
-                h->d_count, vr(Gs));
+      NB_RE(float, NB_LPA, true, 1, G, h->d_ew, h->d_new, h->maxew);
+      NB_RE(double, NB_LPA_SP, true, 2, Gs, h->d_sp, h->d_nsp, h->maxsp);
     }
   } else {
-    NB_LAUNCH((k_real_energy<double, NB_LPA, false, 0, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new,
-              h->maxew, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
-              h->d_count, vr(G));
-    if (polar)
-      NB_LAUNCH((k_real_energy<double, NB_LPA, true, 0, VIR>), G, B, 0, n, h->d_pos4, h->d_mp, h->d_ew, h->d_new,
-                h->maxew, h->cell, aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy,
-                h->d_count, vr(G));
+    if (perm) NB_RE(double, NB_LPA, false, 0, G, h->d_ew, h->d_new, h->maxew);
+    if (polar) NB_RE(double, NB_LPA, true, 0, G, h->d_ew, h->d_new, h->maxew);
   }
+#undef NB_RE
 }
 
 void nb_real_energy(NbHandle *h, bool polar) {
-  if (h->want_vir) real_energy_t<true>(h, polar);
-  else real_energy_t<false>(h, polar);
+  if (h->ost) {
+    if (h->want_vir) real_energy_t<true, true>(h, polar);
+    else real_energy_t<false, true>(h, polar);
+  } else {
+    if (h->want_vir) real_energy_t<true, false>(h, polar);
+    else real_energy_t<false, false>(h, polar);
+  }
 }
 
 void nb_vdw(NbHandle *h) {

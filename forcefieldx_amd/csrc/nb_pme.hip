@@ -1114,7 +1114,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_grid_virial(int K0, int K1, int K2, const CX<T> *__restrict__ fp,
                                                      const CX<T> *__restrict__ fi, int mutual,
                                                      const double *__restrict__ recip, Frac2v A, double pt, double el,
-                                                     double *__restrict__ vpart) {
+                                                     double sperm, double spol, double *__restrict__ vpart) {
   long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long ntot = (long)K0 * K1 * K2;
   double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1128,14 +1128,15 @@ __global__ void __launch_bounds__(256) k_grid_virial(int K0, int K1, int K2, con
     for (int d = 0; d < 3; d++) m[d] = A.a[d][0] * hh + A.a[d][1] * k + A.a[d][2] * l;
     double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
     double pr = (double)fp[g].x, pim = (double)fp[g].y;
-    double ws = pr * pr + pim * pim;
+    double ws = sperm * (pr * pr + pim * pim);
     if (fi) {
       long ng = (long)((K0 - kx) % K0) + (long)K0 * (((K1 - ky) % K1) + (long)K1 * ((K2 - kz) % K2));
       double ar = (double)fi[g].x, ai = (double)fi[g].y, br = (double)fi[ng].x, bi = -(double)fi[ng].y;
       double ur = 0.5 * (ar + br), ui = 0.5 * (ai + bi);
       double vr = 0.5 * (ai - bi), vi = -0.5 * (ar - br);
-      ws += pr * (ur + vr) + pim * (ui + vi);
-      if (mutual) ws += ur * vr + ui * vi;
+      double wq = pr * (ur + vr) + pim * (ui + vi);
+      if (mutual) wq += ur * vr + ui * vi;
+      ws += spol * wq;
     }
     double w = 0.5 * el * recip[g] * ws, f = 2.0 * (1.0 + pt * m2) / m2;
     for (int a = 0; a < 3; a++)
@@ -1166,7 +1167,7 @@ static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
   dim3 G = grid_for(h->ntot, 256);
   NB_LAUNCH((k_grid_virial<T>), G, dim3(256), 0, h->K[0], h->K[1], h->K[2], (const CX<T> *)fpg,
             polar ? (const CX<T> *)g : (const CX<T> *)nullptr, (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_recip,
-            A, pt, h->prm.electric, nb_vir_region(h, G.x));
+            A, pt, h->prm.electric, h->ph.perm, h->ph.pol, nb_vir_region(h, G.x));
 }
 
 static bool mixed_grid(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXED; }

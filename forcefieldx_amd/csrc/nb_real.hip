@@ -273,7 +273,10 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
                                const double *__restrict__ phiup, const double *__restrict__ u,
                                const double *__restrict__ up, Frac2 fr, double el, double aw, int polar, int mutual,
                                double *__restrict__ grad, double *__restrict__ torque, double *__restrict__ energy,
-                               double *__restrict__ vpart) {
+                               double *__restrict__ vpart, ElecPhase phs, double *__restrict__ lgrad,
+                               double *__restrict__ ltorque) {
+  // permanent terms are scaled by phs.perm, polarization terms by phs.pol; with
+  // lgrad set, their lambda derivatives go to lgrad / ltorque and E_DUDL
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double eself = 0, erec = 0, qt = 0, eselfi = 0, ereci = 0;
   double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -302,13 +305,14 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
     tq[1] = -m[3] * ph[1] + m[1] * ph[3] - tt * (m[8] * ph[4] + m[9] * ph[7] + m[6] * ph[8] - m[4] * ph[8] - m[7] * ph[9] - m[8] * ph[6]);
     tq[2] = -m[1] * ph[2] + m[2] * ph[1] - tt * (m[4] * ph[7] + m[7] * ph[5] + m[8] * ph[9] - m[7] * ph[4] - m[5] * ph[7] - m[9] * ph[8]);
     for (int a = 0; a < 3; a++) tq[a] *= el;
+    double dq[3] = {0, 0, 0}, tqq[3] = {0, 0, 0};   // polarization part
     if (polar) {
       const double *ui = u + 3 * s, *upi = up + 3 * s;
       eselfi = aewald3 * (ui[0] * m[1] + ui[1] * m[2] + ui[2] * m[3]);
       double us[3] = {0.5 * (ui[0] + upi[0]), 0.5 * (ui[1] + upi[1]), 0.5 * (ui[2] + upi[2])};
-      tq[0] += aewald4 * (m[2] * us[2] - m[3] * us[1]);
-      tq[1] += aewald4 * (m[3] * us[0] - m[1] * us[2]);
-      tq[2] += aewald4 * (m[1] * us[1] - m[2] * us[0]);
+      tqq[0] = aewald4 * (m[2] * us[2] - m[3] * us[1]);
+      tqq[1] = aewald4 * (m[3] * us[0] - m[1] * us[2]);
+      tqq[2] = aewald4 * (m[1] * us[1] - m[2] * us[0]);
       double fu[3], fpp[3];
       for (int j = 0; j < 3; j++) {
         fu[j] = fr.a[j][0] * ui[0] + fr.a[j][1] * ui[1] + fr.a[j][2] * ui[2];
@@ -336,12 +340,20 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
               fm[6] * sf[18] + fm[7] * sf[15] + fm[8] * sf[19] + fm[9] * sf[16];
       h[2] += fm[0] * sf[3] + fm[1] * sf[8] + fm[2] * sf[9] + fm[3] * sf[6] + fm[4] * sf[14] + fm[5] * sf[16] +
               fm[6] * sf[12] + fm[7] * sf[19] + fm[8] * sf[17] + fm[9] * sf[18];
-      for (int a = 0; a < 3; a++) df[a] += 0.5 * el * (fr.a[0][a] * h[0] + fr.a[1][a] * h[1] + fr.a[2][a] * h[2]);
-      tq[0] += el * (-m[2] * sp[3] + m[3] * sp[2] - tt * (m[7] * sp[8] + m[5] * sp[9] + m[9] * sp[6] - m[8] * sp[7] - m[9] * sp[5] - m[6] * sp[9]));
-      tq[1] += el * (-m[3] * sp[1] + m[1] * sp[3] - tt * (m[8] * sp[4] + m[9] * sp[7] + m[6] * sp[8] - m[4] * sp[8] - m[7] * sp[9] - m[8] * sp[6]));
-      tq[2] += el * (-m[1] * sp[2] + m[2] * sp[1] - tt * (m[4] * sp[7] + m[7] * sp[5] + m[8] * sp[9] - m[7] * sp[4] - m[5] * sp[7] - m[9] * sp[8]));
+      for (int a = 0; a < 3; a++) dq[a] = 0.5 * el * (fr.a[0][a] * h[0] + fr.a[1][a] * h[1] + fr.a[2][a] * h[2]);
+      tqq[0] += el * (-m[2] * sp[3] + m[3] * sp[2] - tt * (m[7] * sp[8] + m[5] * sp[9] + m[9] * sp[6] - m[8] * sp[7] - m[9] * sp[5] - m[6] * sp[9]));
+      tqq[1] += el * (-m[3] * sp[1] + m[1] * sp[3] - tt * (m[8] * sp[4] + m[9] * sp[7] + m[6] * sp[8] - m[4] * sp[8] - m[7] * sp[9] - m[8] * sp[6]));
+      tqq[2] += el * (-m[1] * sp[2] + m[2] * sp[1] - tt * (m[4] * sp[7] + m[7] * sp[5] + m[8] * sp[9] - m[7] * sp[4] - m[5] * sp[7] - m[9] * sp[8]));
     }
-    for (int a = 0; a < 3; a++) { grad[3 * s + a] = df[a]; torque[3 * s + a] = tq[a]; }
+    for (int a = 0; a < 3; a++) {
+      grad[3 * s + a] = phs.perm * df[a] + phs.pol * dq[a];
+      torque[3 * s + a] = phs.perm * tq[a] + phs.pol * tqq[a];
+    }
+    if (lgrad)
+      for (int a = 0; a < 3; a++) {
+        lgrad[3 * s + a] = phs.dperm * df[a] + phs.dpol * dq[a];
+        ltorque[3 * s + a] = phs.dperm * tq[a] + phs.dpol * tqq[a];
+      }
     if (vpart) {
       // Strain derivative with the global multipoles held fixed: the wave
       // vectors' strain dependence gives -d_b dphi_a - (2/3) Q_bc dphi_ac per
@@ -354,7 +366,7 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
         for (int b = 0; b < 3; b++) {
           double t = m[1 + b] * ph[1 + a];
           for (int c = 0; c < 3; c++) t += 2.0 / 3.0 * Q[b][c] * H[a][c];
-          w9[3 * a + b] -= el * t;
+          w9[3 * a + b] -= phs.perm * el * t;
         }
       if (polar) {
         const double *ui = u + 3 * s, *vi = up + 3 * s;
@@ -367,17 +379,22 @@ __global__ void k_recip_energy(int n, const double *__restrict__ gmp, const doub
             double t = (ui[b] + vi[b]) * ph[1 + a] + m[1 + b] * (cu[1 + a] + cv[1 + a]);
             for (int c = 0; c < 3; c++) t += 2.0 / 3.0 * Q[b][c] * Hw[a][c];
             if (mutual) t += vi[b] * cu[1 + a] + ui[b] * cv[1 + a];
-            w9[3 * a + b] -= 0.5 * el * t;
+            w9[3 * a + b] -= phs.pol * 0.5 * el * t;
           }
       }
     }
   }
   if (vpart) block_store9(w9, vpart);
-  block_add(eself, energy + E_PERM_SELF);
-  block_add(0.5 * el * erec, energy + E_PERM_RECIP);
+  block_add(phs.perm * eself, energy + E_PERM_SELF);
+  block_add(phs.perm * 0.5 * el * erec, energy + E_PERM_RECIP);
   block_add(qt, energy + E_QTOT);
-  block_add(eselfi, energy + E_POL_SELF);
-  block_add(0.5 * el * ereci, energy + E_POL_RECIP);
+  block_add(phs.pol * eselfi, energy + E_POL_SELF);
+  block_add(phs.pol * 0.5 * el * ereci, energy + E_POL_RECIP);
+  if (lgrad) {
+    double ep = eself + 0.5 * el * erec, eq = eselfi + 0.5 * el * ereci;
+    block_add(phs.dperm * ep + phs.dpol * eq, energy + E_DUDL);
+    block_add(phs.d2perm * ep + phs.d2pol * eq, energy + E_D2UDL2);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -391,6 +408,83 @@ __device__ __forceinline__ void aadd(double *g, int i, V3<double> v) {
   atomicAdd(&g[3 * i + 2], v.z);
 }
 
+// Torque on atom i to gradients on its frame atoms z, x, y (Torque.torque,
+// Torque.java:111-316); the centre takes -(gz + gx + gy).  vz, vx, vy are the
+// frame vectors (for the strain term).
+__device__ void torque_frame(int fr, int i, int iz, int ix, int iy, const double *__restrict__ xyz, V3<double> trq,
+                             V3<double> &gz, V3<double> &gx, V3<double> &gy, V3<double> &vz, V3<double> &vx,
+                             V3<double> &vy) {
+  V3<double> o = ld3d(xyz, i);
+  V3<double> u = ld3d(xyz, iz) - o;
+  vz = u;
+  vx = V3<double>{0, 0, 0};
+  vy = V3<double>{0, 0, 0};
+  double ru = sqrt(dot(u, u));
+  V3<double> v, w;
+  if (fr != NB_FRAME_ZONLY) {
+    v = ld3d(xyz, ix) - o;
+    vx = v;
+  } else {
+    v = V3<double>{1.0, 0.0, 0.0};
+    if (fabs(u.x / ru) > 0.866) v = V3<double>{0.0, 1.0, 0.0};
+  }
+  double rv = sqrt(dot(v, v));
+  if (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD) {
+    w = ld3d(xyz, iy) - o;
+    vy = w;
+  } else {
+    w = cross(u, v);
+  }
+  double rw = sqrt(dot(w, w));
+  u = (1.0 / ru) * u; v = (1.0 / rv) * v; w = (1.0 / rw) * w;
+  V3<double> uv = unit_d(cross(v, u)), uw = unit_d(cross(w, u)), vw = unit_d(cross(w, v));
+  double cuv = dot(u, v), suv = sqrt(1.0 - cuv * cuv);
+  double dpu = -dot(trq, u), dpv = -dot(trq, v), dpw = -dot(trq, w);
+  gz = gx = gy = V3<double>{0, 0, 0};
+  if (fr == NB_FRAME_ZONLY) {
+    gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
+  } else if (fr == NB_FRAME_ZTHENX) {
+    gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
+    gx = (-dpu / (rv * suv)) * uv;
+  } else if (fr == NB_FRAME_BISECTOR) {
+    gz = (dpv / (ru * suv)) * uv + (0.5 * dpw / ru) * uw;
+    gx = (-dpu / (rv * suv)) * uv + (0.5 * dpw / rv) * vw;
+  } else if (fr == NB_FRAME_ZTHENBISECTOR) {
+    V3<double> r = v + w;
+    V3<double> sv = unit_d(cross(u, r));
+    r = unit_d(r);
+    V3<double> ur = unit_d(cross(r, u)), us = unit_d(cross(sv, u));
+    double cur = dot(u, r), sur = sqrt(1.0 - cur * cur);
+    double cvs = dot(v, sv), svs = sqrt(1.0 - cvs * cvs);
+    double cws = dot(w, sv), sws = sqrt(1.0 - cws * cws);
+    V3<double> t1 = unit_d(v - cvs * sv), t2 = unit_d(w - cws * sv);
+    double ut1c = dot(u, t1), ut1s = sqrt(1.0 - ut1c * ut1c);
+    double ut2c = dot(u, t2), ut2s = sqrt(1.0 - ut2c * ut2c);
+    double dpr = -dot(trq, r), dps = -dot(trq, sv);
+    gz = (dpr / (ru * sur)) * ur + (dps / ru) * us;
+    gx = (dpu / (rv * (ut1s + ut2s))) * (svs * sv - cvs * t1);
+    gy = (dpu / (rw * (ut1s + ut2s))) * (sws * sv - cws * t2);
+  } else if (fr == NB_FRAME_THREEFOLD) {
+    V3<double> sum = u + v + w;
+    double rs = sqrt(dot(sum, sum));
+    sum = (1.0 / rs) * sum;
+    V3<double> ax[3] = {u, v, w};
+    double rr3[3] = {ru, rv, rw};
+    V3<double> out[3];
+    for (int a = 0; a < 3; a++) {
+      V3<double> e = ax[a];
+      V3<double> r = unit_d(ax[(a + 1) % 3] + ax[(a + 2) % 3]);
+      double c = dot(r, e), sn = sqrt(1.0 - c * c);
+      double dpr = -dot(trq, r);
+      V3<double> del = unit_d(cross(r, e));
+      double dpd = -dot(trq, del);
+      V3<double> eps = cross(del, e);
+      out[a] = (dpr / (rr3[a] * sn)) * del + (dpd * dot(e, sum) / (rr3[a] * rs)) * eps;
+    }
+    gz = out[0]; gx = out[1]; gy = out[2];
+  }
+}
+
 __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double *__restrict__ xyz,
                            const int32_t *__restrict__ frame, const int32_t *__restrict__ axis,
                            const int32_t *__restrict__ redidx, const double *__restrict__ redval,
@@ -398,7 +492,9 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
                            const double *__restrict__ grad_r, const double *__restrict__ torque_r, int recip,
                            const double *__restrict__ gred_s, int do_vdw, int do_pme, double *__restrict__ grad,
                            int32_t *__restrict__ flags, double *__restrict__ vpart, const double *__restrict__ gred_l,
-                           double *__restrict__ dudx) {
+                           double *__restrict__ dudx, const double *__restrict__ lgrad_s,
+                           const double *__restrict__ ltorque_s, const double *__restrict__ lgrad_r,
+                           const double *__restrict__ ltorque_r) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (s < n) {
@@ -434,73 +530,8 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
       if (recip) trq += ld3d(torque_r, s);
       if (!isfinite(trq.x) || !isfinite(trq.y) || !isfinite(trq.z)) atomicOr(&flags[4], 1);
       int iz = axis[3 * i], ix = axis[3 * i + 1], iy = axis[3 * i + 2];
-      V3<double> o = ld3d(xyz, i);
-      V3<double> u = ld3d(xyz, iz) - o;
-      V3<double> vz = u, vx{0, 0, 0}, vy{0, 0, 0};  // frame vectors (strain term)
-      double ru = sqrt(dot(u, u));
-      V3<double> v, w;
-      if (fr != NB_FRAME_ZONLY) {
-        v = ld3d(xyz, ix) - o;
-        vx = v;
-      } else {
-        v = V3<double>{1.0, 0.0, 0.0};
-        if (fabs(u.x / ru) > 0.866) v = V3<double>{0.0, 1.0, 0.0};
-      }
-      double rv = sqrt(dot(v, v));
-      if (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD) {
-        w = ld3d(xyz, iy) - o;
-        vy = w;
-      } else {
-        w = cross(u, v);
-      }
-      double rw = sqrt(dot(w, w));
-      u = (1.0 / ru) * u; v = (1.0 / rv) * v; w = (1.0 / rw) * w;
-      V3<double> uv = unit_d(cross(v, u)), uw = unit_d(cross(w, u)), vw = unit_d(cross(w, v));
-      double cuv = dot(u, v), suv = sqrt(1.0 - cuv * cuv);
-      double dpu = -dot(trq, u), dpv = -dot(trq, v), dpw = -dot(trq, w);
-      V3<double> gz{0, 0, 0}, gx{0, 0, 0}, gy{0, 0, 0};
-      if (fr == NB_FRAME_ZONLY) {
-        gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
-      } else if (fr == NB_FRAME_ZTHENX) {
-        gz = (dpv / (ru * suv)) * uv + (dpw / ru) * uw;
-        gx = (-dpu / (rv * suv)) * uv;
-      } else if (fr == NB_FRAME_BISECTOR) {
-        gz = (dpv / (ru * suv)) * uv + (0.5 * dpw / ru) * uw;
-        gx = (-dpu / (rv * suv)) * uv + (0.5 * dpw / rv) * vw;
-      } else if (fr == NB_FRAME_ZTHENBISECTOR) {
-        V3<double> r = v + w;
-        V3<double> sv = unit_d(cross(u, r));
-        r = unit_d(r);
-        V3<double> ur = unit_d(cross(r, u)), us = unit_d(cross(sv, u));
-        double cur = dot(u, r), sur = sqrt(1.0 - cur * cur);
-        double cvs = dot(v, sv), svs = sqrt(1.0 - cvs * cvs);
-        double cws = dot(w, sv), sws = sqrt(1.0 - cws * cws);
-        V3<double> t1 = unit_d(v - cvs * sv), t2 = unit_d(w - cws * sv);
-        double ut1c = dot(u, t1), ut1s = sqrt(1.0 - ut1c * ut1c);
-        double ut2c = dot(u, t2), ut2s = sqrt(1.0 - ut2c * ut2c);
-        double dpr = -dot(trq, r), dps = -dot(trq, sv);
-        gz = (dpr / (ru * sur)) * ur + (dps / ru) * us;
-        gx = (dpu / (rv * (ut1s + ut2s))) * (svs * sv - cvs * t1);
-        gy = (dpu / (rw * (ut1s + ut2s))) * (sws * sv - cws * t2);
-      } else if (fr == NB_FRAME_THREEFOLD) {
-        V3<double> sum = u + v + w;
-        double rs = sqrt(dot(sum, sum));
-        sum = (1.0 / rs) * sum;
-        V3<double> ax[3] = {u, v, w};
-        double rr3[3] = {ru, rv, rw};
-        V3<double> out[3];
-        for (int a = 0; a < 3; a++) {
-          V3<double> e = ax[a];
-          V3<double> r = unit_d(ax[(a + 1) % 3] + ax[(a + 2) % 3]);
-          double c = dot(r, e), sn = sqrt(1.0 - c * c);
-          double dpr = -dot(trq, r);
-          V3<double> del = unit_d(cross(r, e));
-          double dpd = -dot(trq, del);
-          V3<double> eps = cross(del, e);
-          out[a] = (dpr / (rr3[a] * sn)) * del + (dpd * dot(e, sum) / (rr3[a] * rs)) * eps;
-        }
-        gz = out[0]; gx = out[1]; gy = out[2];
-      }
+      V3<double> gz, gx, gy, vz, vx, vy;
+      torque_frame(fr, i, iz, ix, iy, xyz, trq, gz, gx, gy, vz, vx, vy);
       aadd(grad, iz, gz);
       if (ix >= 0 && fr != NB_FRAME_ZONLY) aadd(grad, ix, gx);
       if (iy >= 0 && (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD)) aadd(grad, iy, gy);
@@ -512,6 +543,23 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
         for (int j = 0; j < 3; j++)
           for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) w9[3 * a + b] += G3[j][a] * V3a[j][b];
+      }
+    }
+    if (dudx && lgrad_s) {
+      // electrostatic lambda gradient and torque (OST), same frame rules
+      V3<double> gl = ld3d(lgrad_s, s);
+      if (recip) gl += ld3d(lgrad_r, s);
+      aadd(dudx, i, gl);
+      if (fr != NB_FRAME_NONE) {
+        V3<double> lt = ld3d(ltorque_s, s);
+        if (recip) lt += ld3d(ltorque_r, s);
+        int iz = axis[3 * i], ix = axis[3 * i + 1], iy = axis[3 * i + 2];
+        V3<double> gz, gx, gy, vz, vx, vy;
+        torque_frame(fr, i, iz, ix, iy, xyz, lt, gz, gx, gy, vz, vx, vy);
+        aadd(dudx, iz, gz);
+        if (ix >= 0 && fr != NB_FRAME_ZONLY) aadd(dudx, ix, gx);
+        if (iy >= 0 && (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD)) aadd(dudx, iy, gy);
+        aadd(dudx, i, -(gz + gx + gy));
       }
     }
   }
@@ -558,6 +606,9 @@ void nb_zero_outputs(NbHandle *h) {
   int n3 = 3 * h->n;
   NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_grad_s, h->d_torque_s,
             h->d_gred_s, h->d_grad, h->d_energy, h->d_count, h->d_gred_l, h->d_dudx);
+  if (h->ost)
+    NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_lgrad_s, h->d_ltorque_s,
+              h->d_lgrad_r, h->d_ltorque_r, h->d_energy, h->d_count, h->d_lgrad_s, h->d_ltorque_s);
 }
 
 void nb_direct(NbHandle *h) {
@@ -670,7 +721,8 @@ void nb_recip_energy(NbHandle *h, bool polar) {
   NB_LAUNCH(k_recip_energy, grid_for(n, 128), dim3(128), 0, n, h->d_gmp, h->d_fmp, h->d_phip,
                      h->d_phiu, h->d_phiup, h->d_u, h->d_up, fr, h->prm.electric, h->prm.ewald_alpha, (int)polar,
                      (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_grad_r, h->d_torque_r, h->d_energy,
-                     h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr);
+                     h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr, h->ph,
+                     h->ost ? h->d_lgrad_r : (double *)nullptr, h->d_ltorque_r);
 }
 
 void nb_finalize_gradient(NbHandle *h) {
@@ -680,7 +732,9 @@ void nb_finalize_gradient(NbHandle *h) {
                      h->d_axis, h->d_redidx, h->d_redval, h->d_grad_s, h->d_torque_s, h->d_grad_r, h->d_torque_r,
                      recip, h->d_gred_s, (int)h->prm.do_vdw, (int)h->prm.do_pme, h->d_grad, h->d_flags,
                      h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr, h->d_gred_l,
-                     h->want_dudl ? h->d_dudx : (double *)nullptr);
+                     h->want_dudl ? h->d_dudx : (double *)nullptr,
+                     (h->ost && h->prm.do_pme) ? h->d_lgrad_s : (const double *)nullptr, h->d_ltorque_s, h->d_lgrad_r,
+                     h->d_ltorque_r);
 }
 
 // next free region of nblocks x 9 partial sums in d_vpart (host bookkeeping;

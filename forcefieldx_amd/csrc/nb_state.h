@@ -25,6 +25,25 @@
 
 struct MaskScales { double m[32], p[32], d[32], v[32]; };
 
+// Scales of one electrostatics evaluation.  A plain evaluation has every
+// scale 1 and no lambda terms.  Under OST alchemistry the energy is the sum
+// of three evaluations (ParticleMeshEwald.java:894-935): permanent real-space
+// pairs with a soft atom are softcore-shifted (r -> sqrt(r^2 + lalpha) in the
+// radial functions) and scaled by l2 (RealSpaceEnergyRegion.java:875-908);
+// reciprocal and self terms are scaled by perm, all polarization terms by pol
+// (ReciprocalEnergyRegion.java:254, 390-391, 555-556); d* / d2* are their
+// first and second lambda derivatives, sign included.
+struct ElecPhase {
+  int perm_real = 1;     // permanent real-space energy on
+  int polar = 1;         // polarization on
+  int soft_pairs = 0;    // pairs with a soft atom use l2 and the softcore shift
+  int vapor = 0;         // leave out both-soft pairs per the inter / intramolecular flags
+  int inter = 0, intra = 0;
+  double l2 = 1.0, lalpha = 0.0, dlalpha = 0.0, d2lalpha = 0.0;
+  double perm = 1.0, dperm = 0.0, d2perm = 0.0;
+  double pol = 1.0, dpol = 0.0, d2pol = 0.0;
+};
+
 struct NbHandle {
   nb_params prm;
   int n = 0;
@@ -49,6 +68,17 @@ struct NbHandle {
   double *d_dudx = nullptr;       // [3n] dU/dX/dlambda, caller order
   bool want_dudl = false, dudl_valid = false;
   MaskScales scales;
+
+  // electrostatic alchemistry (OST): this handle's phase scales, the lambda
+  // gradient and torque of its electrostatics (sorted), and the two handles
+  // that evaluate the no-ligand and vapor phases for it
+  ElecPhase ph;
+  bool ost = false;               // this handle's electrostatics carry lambda terms
+  bool elec_lambda = false;       // this handle composes the three OST phases
+  double *d_lgrad_s = nullptr, *d_ltorque_s = nullptr, *d_lgrad_r = nullptr, *d_ltorque_r = nullptr;
+  NbHandle *child_env = nullptr, *child_vap = nullptr;
+  int32_t *d_vap_map = nullptr;   // vapor-handle atom -> atom of this handle (caller order)
+  int n_vap = 0;
 
   // cell
   CellD cell;

@@ -40,7 +40,12 @@ class NbParams(C.Structure):
                 ("precision", C.c_int32), ("max_scf_iter", C.c_int32), ("device", C.c_int32),
                 ("lambda_", C.c_double), ("vdw_lambda_alpha", C.c_double), ("vdw_lambda_exponent", C.c_double),
                 ("vdw_lambda_end", C.c_double), ("intermolecular_softcore", C.c_int32),
-                ("intramolecular_softcore", C.c_int32)]
+                ("intramolecular_softcore", C.c_int32), ("elec_lambda", C.c_int32),
+                ("perm_lambda_alpha", C.c_double), ("perm_lambda_exponent", C.c_double),
+                ("perm_lambda_start", C.c_double), ("perm_lambda_end", C.c_double),
+                ("pol_lambda_exponent", C.c_double), ("pol_lambda_start", C.c_double),
+                ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int32),
+                ("no_ligand_condensed_scf", C.c_int32)]
 
 
 class NbTopology(C.Structure):
@@ -247,7 +252,10 @@ def params_from(p, grid, precision=None, device=0):
                     (C.c_double * 3)(*p.v_scale), p.halgren_delta, p.halgren_gamma, p.polarization, int(p.do_vdw),
                     int(p.do_pme), prec, p.max_scf_iter, device, p.lambda_value, p.vdw_lambda_alpha,
                     p.vdw_lambda_exponent, p.vdw_lambda_end, int(p.intermolecular_softcore),
-                    int(p.intramolecular_softcore))
+                    int(p.intramolecular_softcore), int(p.elec_lambda), p.perm_lambda_alpha,
+                    p.perm_lambda_exponent, p.perm_lambda_start, p.perm_lambda_end, p.pol_lambda_exponent,
+                    p.pol_lambda_start, p.pol_lambda_end, int(p.ligand_vapor_elec),
+                    int(p.no_ligand_condensed_scf))
 
 
 def device_count():

@@ -194,27 +194,36 @@ WATER_CONFIGS = {
 CHAIN_VDW = (3.8, 0.1)     # chain atom vdW radius (R-min, A) and well depth (kcal/mol)
 
 
-def _random_chain(rng, n, box, bond=1.5, min_dist=2.6, tries=200):
+def _random_chain(rng, n, box, bond=1.5, min_dist=2.6, tries=200, max_backtracks=10000):
     """Self-avoiding random walk of ``n`` points with fixed bond length: each
     new point keeps ``min_dist`` (minimum image) from every earlier point
-    except its bonded predecessor."""
+    except its bonded predecessor.  A walk that cannot be continued steps back
+    a few points and grows again."""
     box = np.asarray(box, float)[:3]
     pts = np.empty((n, 3))
     pts[0] = rng.random(3) * box
     md2 = min_dist * min_dist
-    for i in range(1, n):
+    i, backtracks = 1, 0
+    while i < n:
+        placed = False
         for _ in range(tries):
             v = rng.normal(size=3)
             c = pts[i - 1] + bond * v / np.linalg.norm(v)
-            if i < 2:
-                break
-            d = pts[:i - 1] - c
-            d -= box * np.round(d / box)
-            if np.einsum("ij,ij->i", d, d).min() >= md2:
-                break
-        else:
+            if i >= 2:
+                d = pts[:i - 1] - c
+                d -= box * np.round(d / box)
+                if np.einsum("ij,ij->i", d, d).min() < md2:
+                    continue
+            pts[i] = c
+            placed = True
+            break
+        if placed:
+            i += 1
+            continue
+        backtracks += 1
+        if backtracks > max_backtracks:
             raise RuntimeError("random chain could not be continued")
-        pts[i] = c
+        i = max(1, i - int(rng.integers(2, 8)))
     return pts
 
 

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 2
+#define NB_ABI_VERSION 3
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -84,6 +84,17 @@ typedef struct {
   double vdw_lambda_end;    /* lambda at which the vdW coupling is complete (1) */
   int32_t intermolecular_softcore; /* both-soft pairs of different molecules are soft (0) */
   int32_t intramolecular_softcore; /* both-soft pairs of one molecule are soft (0) */
+  /* electrostatic alchemistry in OST mode (AlchemicalParameters.java:171-251):
+   * with elec_lambda set and soft atoms present, nb_evaluate sums the
+   * condensed, no-ligand and vapor phases of ParticleMeshEwald.java:894-935 */
+  int32_t elec_lambda;      /* 0: lambda applies to vdW only */
+  double perm_lambda_alpha; /* permanent softcore shift (2) */
+  double perm_lambda_exponent; /* (3) */
+  double perm_lambda_start, perm_lambda_end; /* permanent window (0.4, 1) */
+  double pol_lambda_exponent; /* (3) */
+  double pol_lambda_start, pol_lambda_end; /* polarization window (0.75, 1) */
+  int32_t ligand_vapor_elec; /* vapor phase of the soft atoms (1) */
+  int32_t no_ligand_condensed_scf; /* polarization of the no-ligand phase (1) */
 } nb_params;
 
 typedef struct {
