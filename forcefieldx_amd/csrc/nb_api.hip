@@ -69,6 +69,10 @@ void alloc(T **dst, size_t count) {
 }
 
 void free_all(NbHandle *h) {
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
   if (h->child_env) { free_all(h->child_env); delete h->child_env; h->child_env = nullptr; }
   if (h->child_vap) { free_all(h->child_vap); delete h->child_vap; h->child_vap = nullptr; }
   void *lam[] = {h->d_lgrad_s, h->d_ltorque_s, h->d_lgrad_r, h->d_ltorque_r, h->d_vap_map};
@@ -95,6 +99,13 @@ void free_all(NbHandle *h) {
   if (h->ev_eps) hipEventDestroy(h->ev_eps);
   if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
+}
+
+void drop_graph(NbHandle *h) {
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
 }
 
 int fail(NbHandle *h, const std::exception &e, int code) {
@@ -545,6 +556,7 @@ int nb_set_cell(void *handle, const double box[6]) {
     double zero[6] = {0, 0, 0, 90, 90, 90};
     const double *b = box ? box : zero;
     for (int d = 0; d < 6; d++) h->box[d] = b[d];
+    drop_graph(h);
     cell_from_box(b, h->cell);
     h->cell_set = true;
     h->list_valid = false;
@@ -593,8 +605,24 @@ int nb_set_lambda(void *handle, double lambda) {
     h->err = "lambda must lie in [0, 1]";
     return NB_ERR_INVALID;
   }
+  if (lambda != h->lambda) drop_graph(h);
   h->lambda = lambda;
   return NB_OK;
+}
+
+int nb_set_graph(void *handle, int on) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  h->graph_on = on != 0;
+  if (!h->graph_on) drop_graph(h);
+  if (h->child_env) nb_set_graph(h->child_env, on);
+  if (h->child_vap) nb_set_graph(h->child_vap, on);
+  return NB_OK;
+}
+
+int nb_graph_replays(void *handle) {
+  NbHandle *h = (NbHandle *)handle;
+  return h ? (int)h->graph_replays : -1;
 }
 
 int nb_struct_sizes(int32_t sizes[3]) {
@@ -603,6 +631,102 @@ int nb_struct_sizes(int32_t sizes[3]) {
   sizes[1] = (int32_t)sizeof(nb_topology);
   sizes[2] = (int32_t)sizeof(nb_energies);
   return NB_OK;
+}
+
+// Enqueue one evaluation on the handle's streams, ending with the energies,
+// counts, flags (and virial, PCG scalars) copied into pinned host memory.
+// capture: the list check only raises flags[3] (a list built for other
+// coordinates is not rebuilt), the PCG runs graph_iters iterations under
+// device-side control, and nothing synchronises with the host, so the
+// sequence can be recorded into a HIP graph.
+static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &rebuilt) {
+  int status = NB_OK;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  h->vpart_used = 0;
+  rebuilt = false;
+  if (capture) {
+    nb_check_rebuild_async(h);
+    nb_gather_sorted(h);
+  } else {
+    rebuilt = nb_need_rebuild(h);
+    if (rebuilt) nb_build_lists(h);
+    else nb_gather_sorted(h);
+  }
+  nb_zero_outputs(h);
+  if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
+  // vdW runs on the side stream, overlapping everything up to the gradient assembly
+  if (h->prm.do_vdw) {
+    nb_fork(h);
+    NbSide side(h);
+    nb_vdw(h);
+  }
+  bool polar = h->prm.polarization != NB_POL_NONE && h->ph.polar;
+  if (h->prm.do_pme) {
+    nb_filter_lists(h);
+    // real-space permanent field (side) || permanent PME (main)
+    nb_fork(h);
+    {
+      NbSide side(h);
+      nb_perm_field(h);
+    }
+    if (use_recip) nb_pme_permanent(h);
+    nb_join(h);
+    if (polar) {
+      nb_direct(h);
+      if (h->prm.polarization == NB_POL_MUTUAL) {
+        status = capture ? nb_pcg_device(h, h->graph_iters) : nb_pcg(h, &e);
+      } else if (use_recip) {
+        // direct polarization: only the reciprocal potentials of u_direct are needed
+        nb_pme_induced(h, h->d_u, h->d_up, true);
+      }
+    }
+    if (h->want_vir && use_recip) nb_grid_virial(h, polar);
+    // real-space energies (side) || reciprocal energy (main)
+    nb_fork(h);
+    {
+      NbSide side(h);
+      nb_real_energy(h, polar);
+    }
+    if (use_recip) nb_recip_energy(h, polar);
+  }
+  nb_join(h);
+  nb_finalize_gradient(h);
+  if (h->want_vir) {
+    nb_vir_reduce(h);
+    HIPCHECK(hipMemcpyAsync(h->h_pinned + 16, h->d_vir, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(h->h_pinned + 32, h->d_energy + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
+  if (capture)
+    HIPCHECK(hipMemcpyAsync(h->h_pinned + 40, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
+  return status;
+}
+
+// Record the evaluation sequence into a graph (not launched here).
+static void capture_graph(NbHandle *h, uint32_t flags, int iters) {
+  drop_graph(h);
+  h->graph_iters = iters;
+  h->capturing = true;
+  nb_energies e;
+  memset(&e, 0, sizeof(e));
+  bool rebuilt = false;
+  hipGraph_t g = nullptr;
+  HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    issue_evaluation(h, true, e, rebuilt);
+  } catch (...) {
+    hipStreamEndCapture(h->stream, &g);
+    if (g) hipGraphDestroy(g);
+    h->capturing = false;
+    throw;
+  }
+  h->capturing = false;
+  HIPCHECK(hipStreamEndCapture(h->stream, &g));
+  h->graph = g;
+  HIPCHECK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  h->graph_flags = flags;
 }
 
 static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
@@ -633,79 +757,62 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       long pb = xcd_grid((long)n * 16, 256).x, sb = xcd_grid((long)n * 4, 256).x, ab = (n + 127) / 128;
       long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + 64;
       if (need > h->vpart_cap) {
+        drop_graph(h);
         if (h->d_vpart) hipFree(h->d_vpart);
         h->d_vpart = nullptr;
         HIPCHECK(hipMalloc(&h->d_vpart, sizeof(double) * 9 * need));
         h->vpart_cap = need;
       }
     }
-    // A filtered (Ewald / preconditioner / masked) row that overflowed its
-    // capacity is only detected when the flags come back with the energies;
-    // the rows are then grown and the evaluation repeated.
-    for (int attempt = 0;; attempt++) {
-      status = NB_OK;
-      h->vpart_used = 0;
-      bool rebuild = nb_need_rebuild(h);
-      if (rebuild) nb_build_lists(h);
-      else nb_gather_sorted(h);
-      rebuilt |= rebuild ? 1 : 0;
-      nb_zero_outputs(h);
-      if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
-      // vdW runs on the side stream, overlapping everything up to the gradient assembly
-      if (h->prm.do_vdw) {
-        nb_fork(h);
-        NbSide side(h);
-        nb_vdw(h);
-      }
-      bool polar = h->prm.polarization != NB_POL_NONE && h->ph.polar;
-      if (h->prm.do_pme) {
-        nb_filter_lists(h);
-        // real-space permanent field (side) || permanent PME (main)
-        nb_fork(h);
-        {
-          NbSide side(h);
-          nb_perm_field(h);
-        }
-        if (use_recip) nb_pme_permanent(h);
-        nb_join(h);
-        if (polar) {
-          nb_direct(h);
-          if (h->prm.polarization == NB_POL_MUTUAL) {
-            status = nb_pcg(h, &e);
-          } else if (use_recip) {
-            // direct polarization: only the reciprocal potentials of u_direct are needed
-            nb_pme_induced(h, h->d_u, h->d_up, true);
-          }
-        }
-        if (h->want_vir && use_recip) nb_grid_virial(h, polar);
-        // real-space energies (side) || reciprocal energy (main)
-        nb_fork(h);
-        {
-          NbSide side(h);
-          nb_real_energy(h, polar);
-        }
-        if (use_recip) nb_recip_energy(h, polar);
-      }
-      nb_join(h);
-      nb_finalize_gradient(h);
-      if (h->want_vir) {
-        nb_vir_reduce(h);
-        HIPCHECK(hipMemcpyAsync(h->h_pinned + 16, h->d_vir, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
-      }
-      HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
-      HIPCHECK(hipMemcpyAsync(h->h_pinned + 32, h->d_energy + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
-      HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
-      HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
+    bool mutual = h->prm.do_pme && h->prm.polarization == NB_POL_MUTUAL && h->ph.polar;
+    bool replayed = false;
+    if (h->graph_on && h->gexec && h->graph_flags == flags && !h->profiling) {
+      // replay; a list that needs rebuilding, an overflowed filtered row or a
+      // solve that did not stop within the recorded iterations sends the
+      // evaluation down the regular path below
+      HIPCHECK(hipGraphLaunch(h->gexec, h->stream));
       HIPCHECK(hipStreamSynchronize(h->stream));
-      if (h->profiling) nb_prof_harvest(h);
+      memcpy(fl, h->h_pinned + 12, sizeof(fl));
+      const double *sc = h->h_pinned + 40;
+      bool ok = fl[3] == 0 && !(h->prm.do_pme && nb_grow_filtered_rows(h, fl)) && (!mutual || sc[S_DONE] != 0.0);
+      if (ok) {
+        replayed = true;
+        h->graph_replays++;
+        if (mutual) {
+          e.pcg_iters = (int)sc[S_ITER];
+          e.pcg_eps = sc[S_EPSV];
+          status = (int)sc[S_STAT];
+        }
+      } else {
+        drop_graph(h);
+      }
+    }
+    if (replayed) {
       memcpy(en, h->h_pinned, 8 * sizeof(double));
       en[E_DUDL] = h->h_pinned[32];
       en[E_D2UDL2] = h->h_pinned[33];
       memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
-      memcpy(fl, h->h_pinned + 12, sizeof(fl));
-      if (!h->prm.do_pme || !nb_grow_filtered_rows(h, fl)) break;
-      if (attempt == 3) throw std::runtime_error("Ewald list capacity could not be established");
-      memset(&e, 0, sizeof(e));
+    } else {
+      // the regular path may grow or rebuild the rows a recorded graph points to
+      drop_graph(h);
+      // A filtered (Ewald / preconditioner / masked) row that overflowed its
+      // capacity is only detected when the flags come back with the energies;
+      // the rows are then grown and the evaluation repeated.
+      for (int attempt = 0;; attempt++) {
+        bool rb = false;
+        status = issue_evaluation(h, false, e, rb);
+        rebuilt |= rb ? 1 : 0;
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        if (h->profiling) nb_prof_harvest(h);
+        memcpy(en, h->h_pinned, 8 * sizeof(double));
+        en[E_DUDL] = h->h_pinned[32];
+        en[E_D2UDL2] = h->h_pinned[33];
+        memcpy(cnt, h->h_pinned + 8, sizeof(cnt));
+        memcpy(fl, h->h_pinned + 12, sizeof(fl));
+        if (!h->prm.do_pme || !nb_grow_filtered_rows(h, fl)) break;
+        if (attempt == 3) throw std::runtime_error("Ewald list capacity could not be established");
+        memset(&e, 0, sizeof(e));
+      }
     }
     e.list_rebuilt = rebuilt;
     if (h->want_vir) {
@@ -744,6 +851,10 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     e.total = e.perm + e.pol + e.vdw;
     if (status == NB_OK && fl[4]) status = NB_ERR_BAD_TORQUE;
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
+    // record a graph for the next evaluations: the PCG budget is this
+    // evaluation's iteration count plus one
+    if (h->graph_on && !replayed && status == NB_OK && !h->profiling)
+      capture_graph(h, flags, mutual ? e.pcg_iters + 1 : 0);
     h->last = e;
     if (out) *out = e;
     if (status != NB_OK) {
@@ -753,6 +864,8 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     }
     return status;
   } catch (std::exception &ex) {
+    h->capturing = false;
+    drop_graph(h);
     return fail(h, ex, NB_ERR_HIP);
   }
 }

@@ -9,7 +9,8 @@ struct ScaleTab { double m[16], p[16], d[16], v[16]; int masked[16]; };
 enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT, E_DUDL,
        E_D2UDL2, E_NSLOT };
 // PCG scalar slots of NbHandle::d_scal
-enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_NSLOT };
+// PCG scalars; S_DONE .. S_EPSV are the device-side control of a recorded solve
+enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_DONE, S_ITER, S_PREV, S_STAT, S_EPSV, S_NSLOT };
 
 static inline ScaleTab scale_tab(const NbHandle *h) {
   ScaleTab t;

@@ -160,7 +160,8 @@ template <typename T, int LPA, bool PRE>
 __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__restrict__ rec,
                                                    const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
                                                    int maxl, CellD cell, double aewald, double *__restrict__ f,
-                                                   double *__restrict__ fp) {
+                                                   double *__restrict__ fp, const double *__restrict__ ctl) {
+  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
@@ -550,8 +551,10 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// flags[3] (list rebuild needed) belongs to the list check and is kept
 __global__ void k_clear_flags(int32_t *__restrict__ flags) {
-  if (threadIdx.x >= 1 && threadIdx.x <= 5) flags[threadIdx.x] = 0;
+  int t = threadIdx.x;
+  if (t == 1 || t == 2 || t == 4 || t == 5) flags[t] = 0;
 }
 
 // Ewald / preconditioner / masked rows of this evaluation from the Verlet list
@@ -631,10 +634,15 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
   bool mixed = h->prm.precision == NB_PREC_MIXED;
   const DipRec<float> *rf = (const DipRec<float> *)h->d_dip;
   const DipRec<double> *rd = (const DipRec<double> *)h->d_dip;
-  if (mixed && precond) NB_LAUNCH((k_dip_field<float, NB_LPA, true>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp);
-  else if (mixed) NB_LAUNCH((k_dip_field<float, NB_LPA, false>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp);
-  else if (precond) NB_LAUNCH((k_dip_field<double, NB_LPA, true>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp);
-  else NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp);
+  const double *ctl = h->ctl;
+  if (mixed && precond)
+    NB_LAUNCH((k_dip_field<float, NB_LPA, true>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+  else if (mixed)
+    NB_LAUNCH((k_dip_field<float, NB_LPA, false>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+  else if (precond)
+    NB_LAUNCH((k_dip_field<double, NB_LPA, true>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+  else
+    NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
 }
 
 #define NB_LPA_SP 4

@@ -224,7 +224,9 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
                                                       const double *__restrict__ theta,
                                                       const int32_t *__restrict__ igrid,
                                                       const double *__restrict__ fmp, const double *__restrict__ u3,
-                                                      const double *__restrict__ up3, Frac fr, T *__restrict__ bbuf) {
+                                                      const double *__restrict__ up3, Frac fr, T *__restrict__ bbuf,
+                                                      const double *__restrict__ ctl) {
+  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
   constexpr int NP = IND ? 2 : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double *lds = (double *)smem_raw;
@@ -358,7 +360,9 @@ __device__ __forceinline__ T group8(T v) {
 template <typename T, int NP, bool FULL>
 __global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
                                                 const CX<T> *__restrict__ grid, Frac fr, int K0, int K1, int K2,
-                                                double *__restrict__ out0, double *__restrict__ out1) {
+                                                double *__restrict__ out0, double *__restrict__ out1,
+                                                const double *__restrict__ ctl) {
+  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
   constexpr int NC = FULL ? 20 : 4;
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
@@ -1153,11 +1157,12 @@ static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
   size_t e3 = (size_t)bk.ext * bk.ext * bk.ext;
   NB_LAUNCH((k_spread_brick<T, false>), dim3(h->nbricks), dim3(256), sizeof(double) * e3, h->n, bk, h->d_bstart,
             h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, (const double *)nullptr, (const double *)nullptr, fr,
-            (T *)h->d_bbuf);
+            (T *)h->d_bbuf, (const double *)nullptr);
   forward_t<T>(h, 1, fpg, tw);
   if (polar) {
     NB_LAUNCH((k_spread_brick<T, true>), dim3(h->nbricks), dim3(256), sizeof(double) * 2 * e3, h->n, bk, h->d_bstart,
-              h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, h->d_u, h->d_up, fr, (T *)h->d_bbuf);
+              h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, h->d_u, h->d_up, fr, (T *)h->d_bbuf,
+              (const double *)nullptr);
     forward_t<T>(h, 2, g, tw);
   }
   Frac2v A;
@@ -1187,14 +1192,14 @@ static void spread_t(NbHandle *h, const double *u, const double *up) {
   size_t e3 = (size_t)bk.ext * bk.ext * bk.ext;
   Frac fr = frac_of(h);
   NB_LAUNCH((k_spread_brick<T, IND>), dim3(h->nbricks), dim3(256), sizeof(double) * NP * e3, h->n, bk, h->d_bstart, h->d_batom,
-            h->d_theta, h->d_igrid, h->d_fmp, u, up, fr, (T *)h->d_bbuf);
+            h->d_theta, h->d_igrid, h->d_fmp, u, up, fr, (T *)h->d_bbuf, h->ctl);
 }
 
 template <typename T, int NP, bool FULL>
 static void gather_t(NbHandle *h, double *o0, double *o1) {
   Frac fr = frac_of(h);
   NB_LAUNCH((k_gather<T, NP, FULL>), xcd_grid((long)h->n * 8, 256), dim3(256), 0, h->n, h->d_theta, h->d_igrid,
-            (const CX<T> *)h->d_grid, fr, h->K[0], h->K[1], h->K[2], o0, o1);
+            (const CX<T> *)h->d_grid, fr, h->K[0], h->K[1], h->K[2], o0, o1, h->ctl);
 }
 
 void nb_pme_permanent(NbHandle *h) {

@@ -206,7 +206,9 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
                                                     const double *__restrict__ pp, double *__restrict__ r,
                                                     double *__restrict__ rp, double *__restrict__ u,
                                                     double *__restrict__ up, double *__restrict__ part, int stride,
-                                                    int nb, void *rec, int mixed) {
+                                                    int nb, void *rec, int mixed, const double *__restrict__ ctl) {
+  // a recorded solve that has stopped keeps its dipoles and residuals
+  if (ctl && ctl[S_DONE] != 0.0) return;
   const int in[4] = {S_PAP, S_PAPP, S_RZ, S_RZP};
   double t[4];
   block_totals<4>(part, in, stride, nb, t);
@@ -231,15 +233,47 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
   block_partials<2>(v, part, out, stride);
 }
 
-// residual norms for the convergence test: scal[S_EPS], scal[S_EPSP]
+// residual norms for the convergence test: scal[S_EPS], scal[S_EPSP].  With
+// control set (a recorded solve), the test of the host loop runs here:
+// eps = 4.80321 sqrt(max(r.r, rCR.rCR) / n); a rise of eps ends the solve as
+// diverged, reaching maxit as out of iterations, eps < polar_eps as converged
+// (PCGSolver.java:543-547); S_DONE then stops the following iterations.
 __global__ void __launch_bounds__(256) k_pcg_eps(const double *__restrict__ part, int stride, int nb,
-                                                 double *__restrict__ scal) {
+                                                 double *__restrict__ scal, int control, int n, double polar_eps,
+                                                 int maxit) {
   const int in[2] = {S_EPS, S_EPSP};
   double t[2];
   block_totals<2>(part, in, stride, nb, t);
   if (threadIdx.x == 0) {
     scal[S_EPS] = t[0];
     scal[S_EPSP] = t[1];
+    if (control && scal[S_DONE] == 0.0) {
+      double e = t[0] > t[1] ? t[0] : t[1];
+      double eps = 4.80321 * sqrt(e / n), prev = scal[S_PREV];
+      double iter = scal[S_ITER] + 1.0;
+      scal[S_ITER] = iter;
+      scal[S_EPSV] = eps;
+      scal[S_PREV] = eps;
+      if (!(eps <= prev)) {
+        scal[S_STAT] = (double)NB_ERR_SCF_DIVERGED;
+        scal[S_DONE] = 1.0;
+      } else if (iter >= (double)maxit) {
+        scal[S_STAT] = (double)NB_ERR_SCF_MAXITER;
+        scal[S_DONE] = 1.0;
+      } else if (eps < polar_eps) {
+        scal[S_DONE] = 1.0;
+      }
+    }
+  }
+}
+
+__global__ void k_pcg_ctl_init(double *__restrict__ scal) {
+  if (threadIdx.x == 0) {
+    scal[S_DONE] = 0.0;
+    scal[S_ITER] = 0.0;
+    scal[S_PREV] = 100.0;
+    scal[S_STAT] = 0.0;
+    scal[S_EPSV] = 0.0;
   }
 }
 
@@ -684,8 +718,8 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
               use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
               h->d_vec, h->d_vecp, part, stride);
     NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
-              h->d_up, part, stride, nb, rec, mx);
-    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal);
+              h->d_up, part, stride, nb, rec, mx, (const double *)nullptr);
+    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal, 0, n, 0.0, 0);
     HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipEventRecord(h->ev_eps, h->stream));
     // The next direction is queued before the convergence test is read back,
@@ -711,6 +745,57 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
   // energy and gradient; their real-space field is not needed
   if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
   return status;
+}
+
+// The same solve with its control on the device, for recording into a graph:
+// a fixed budget of iterations is queued; once k_pcg_eps has stopped the
+// solve, the update and the field, spread and gather kernels of the remaining
+// iterations return at once (the transforms and small vector kernels still
+// run on stale data that nothing reads).  The result equals that of nb_pcg.
+int nb_pcg_device(NbHandle *h, int iters) {
+  int n = h->n;
+  dim3 B(256), G = grid_for(3 * n, 256);
+  const int nb = (int)G.x, stride = h->part_stride;
+  if (nb > stride) throw std::runtime_error("PCG partial-sum buffer too small");
+  double psc = h->prm.precond_scale;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
+  double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
+  double *f = h->d_field, *fp = h->d_fieldp;
+  double *part = h->d_part;
+  int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
+  nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
+  void *rec = (void *)h->d_dip;
+  int mx = mixed_of(h);
+  NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp, h->d_r, h->d_rp,
+            rec, mx);
+  nb_precond_field(h, f, fp);
+  NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, 1, part,
+            stride, (int)S_RZN, rec, mx);
+  NB_LAUNCH(k_pcg_ctl_init, dim3(1), dim3(64), 0, h->d_scal);
+  h->ctl = h->d_scal;
+  for (int it = 0; it < iters; it++) {
+    nb_fork(h);
+    {
+      NbSide side(h);
+      nb_dip_field_real(h, f, fp, false);
+    }
+    if (use_recip) nb_pme_induced(h, h->d_p, h->d_pp, false);
+    nb_join(h);
+    NB_LAUNCH(k_pcg_ap, G, B, 0, n, a3, h->d_polar, h->d_p, h->d_pp, f, fp,
+              use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
+              h->d_vec, h->d_vecp, part, stride);
+    NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
+              h->d_up, part, stride, nb, rec, mx, (const double *)h->d_scal);
+    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal, 1, n, h->prm.polar_eps, maxit);
+    nb_precond_field(h, f, fp);
+    NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, 0,
+              part, stride, (int)S_RZN, rec, mx);
+    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_z, h->d_zp, h->d_p, h->d_pp, part, stride, nb, rec, mx);
+  }
+  h->ctl = nullptr;
+  if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
+  return NB_OK;
 }
 
 void nb_recip_energy(NbHandle *h, bool polar) {

@@ -171,6 +171,17 @@ struct NbHandle {
 
   nb_energies last;
 
+  // graph mode (nb_set_graph): the evaluation sequence is recorded once and
+  // replayed; the PCG then runs graph_iters iterations under device-side
+  // control (kernels that only serve a stopped solve return at once)
+  bool graph_on = false, capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint32_t graph_flags = 0;
+  int graph_iters = 0;
+  long graph_replays = 0;
+  const double *ctl = nullptr;    // PCG control slots while the captured solve is being recorded
+
   // profiling
   bool profiling = false;
   std::string prof_filter;  // when set, only launches whose kernel name contains it are timed
@@ -197,6 +208,8 @@ void nb_direct(NbHandle *h);
 void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f, double *fp, bool full);
 void nb_precond_field(NbHandle *h, double *f, double *fp);
 int nb_pcg(NbHandle *h, nb_energies *out);
+int nb_pcg_device(NbHandle *h, int iters);
+void nb_check_rebuild_async(NbHandle *h);
 void nb_recip_energy(NbHandle *h, bool polar);
 void nb_real_energy(NbHandle *h, bool polar);
 void nb_vdw(NbHandle *h);

@@ -23,7 +23,8 @@ NB_PREC_FP64, NB_PREC_MIXED = 0, 1
 EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
-            "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes"]
+            "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
+            "nb_set_graph", "nb_graph_replays"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -85,6 +86,8 @@ def load(path=LIB_PATH):
     L.nb_set_positions.argtypes = [vp, _dp]
     L.nb_set_positions_device.argtypes = [vp, vp]
     L.nb_set_lambda.argtypes = [vp, C.c_double]
+    L.nb_set_graph.argtypes = [vp, C.c_int]
+    L.nb_graph_replays.argtypes = [vp]
     L.nb_struct_sizes.argtypes = [_ip]
     L.nb_struct_sizes.restype = C.c_int
     L.nb_evaluate.argtypes = [vp, C.c_uint32, C.POINTER(NbEnergies)]
@@ -193,6 +196,13 @@ class Handle:
 
     def set_lambda(self, lam):
         self._check(self.L.nb_set_lambda(self.h, float(lam)))
+
+    def set_graph(self, on=True):
+        """Record evaluations into a HIP graph and replay them."""
+        self._check(self.L.nb_set_graph(self.h, int(on)))
+
+    def graph_replays(self):
+        return self.L.nb_graph_replays(self.h)
 
     def dudx_dl(self):
         """dU/dX/dlambda (n, 3) of the last evaluation made with NB_DUDL."""

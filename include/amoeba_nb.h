@@ -180,6 +180,19 @@ const char *nb_last_error(void *handle);
 /* Release all device memory of the handle. */
 void nb_destroy(void *handle);
 
+/* Graph mode (no counterpart in the reference): with on != 0, an evaluation
+ * records its launch sequence into a HIP graph, and later evaluations with
+ * the same flags replay it.  The PCG of a replay runs under device-side
+ * control with a budget of the recorded evaluation's iterations plus one;
+ * a replay whose list needs rebuilding, whose filtered rows overflow or whose
+ * solve did not stop within the budget is repeated on the regular path and
+ * recorded again.  nb_set_cell, nb_set_lambda and list rebuilds drop the graph.
+ * Results equal those of the regular path. */
+int nb_set_graph(void *handle, int on);
+
+/* Number of evaluations of this handle served by a graph replay. */
+int nb_graph_replays(void *handle);
+
 /* ABI version (NB_ABI_VERSION). */
 int nb_abi_version(void);
 
