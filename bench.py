@@ -128,58 +128,184 @@ def measured_traffic(name):
     return None
 
 
-def time_gpu(top, box, xyz, params, prec, steps, warmup, profile):
+def time_gpu(top, box, xyz, params, prec, steps, warmup, profile, graph=False, lam=None):
     """Warm up, optionally find the dominant kernel with a 2-step fully timed
-    calibration, then time `steps` evaluations.  In the timed region HIP events
-    bracket only the launches of the dominant kernel (on the library's
-    stream), so the kernel's average duration is measured live at negligible
-    overhead."""
+    calibration, then time `steps` evaluations.  Without graph mode, HIP events
+    bracket only the launches of the dominant kernel inside the timed region
+    (on the library's stream), so its average duration is measured live at
+    negligible overhead.  In graph mode the timed region replays the recorded
+    evaluation, and the dominant kernel is timed with HIP events over `steps`
+    regular evaluations right after it (same kernels on the same data)."""
     from forcefieldx_amd.energy import ForceFieldEnergy
     ffe = ForceFieldEnergy(top, box, params, precision=prec)
     ffe.setCoordinates(xyz)
     h = ffe._handle
-    for _ in range(warmup):
-        ffe._evaluate(True)
+    if lam is not None:
+        ffe.setLambda(lam)
     calib = {}
+    dominant = None
     if profile:
+        ffe._evaluate(True)
         h.set_profiling(True)
         for _ in range(2):
             ffe._evaluate(True)
         calib = h.profile()
         dominant = max(calib.items(), key=lambda kv: kv[1][0])[0]
+        h.set_profiling(False)
+    if graph:
+        h.set_graph(True)
+    elif profile:
         h.set_profiling(True, kernel_filter=dominant)
+    for _ in range(warmup):
+        ffe._evaluate(True)
+    replays0 = h.graph_replays()
     t0 = time.perf_counter()
     for _ in range(steps):
         ffe._evaluate(True)
     t1 = time.perf_counter()
+    replays = h.graph_replays() - replays0
     prof = {}
     if profile:
-        prof = {"calibration": calib, "timed": h.profile()}
+        if graph:
+            h.set_graph(False)
+            h.set_profiling(True, kernel_filter=dominant)
+            for _ in range(steps):
+                ffe._evaluate(True)
+        prof = {"calibration": calib, "timed": h.profile(), "graph_replays": replays}
         h.set_profiling(False)
     last = dict(ffe._last)
     g = h.gradient()
     u, up = h.induced()
+    extra = {"graph_replays": replays}
+    if lam is not None:
+        extra["dudl"] = ffe.getdEdL()
     ffe.destroy()
-    return (t1 - t0) / steps, last, g, u, prof
+    return (t1 - t0) / steps, last, g, u, prof, extra
 
 
-def cpu_baseline(top, box, xyz, params, budget_s=12.0):
+def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, graph=False, lam=None):
+    """As time_gpu, but before each evaluation the coordinates, resident on the
+    device, receive a Gaussian step of sigma 0.01 A (torch on the device; the
+    library copies them device-to-device).  Both precisions see the same
+    sequence, so the last gradients are comparable."""
+    import torch
+    from forcefieldx_amd.energy import ForceFieldEnergy
+    ffe = ForceFieldEnergy(top, box, params, precision=prec)
+    h = ffe._handle
+    if lam is not None:
+        ffe.setLambda(lam)
+    x = torch.tensor(np.asarray(xyz, np.float64).reshape(-1), device="cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1234)
+
+    def step():
+        x.add_(torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64), alpha=0.01)
+        torch.cuda.current_stream().synchronize()
+        h.set_positions_device(x.data_ptr())
+        ffe._evaluate(True)
+
+    ffe._x = np.asarray(xyz, np.float64).reshape(-1)
+    h.set_positions(ffe._x)
+    calib, dominant = {}, None
+    if profile:
+        ffe._evaluate(True)
+        h.set_profiling(True)
+        for _ in range(2):
+            step()
+        calib = h.profile()
+        dominant = max(calib.items(), key=lambda kv: kv[1][0])[0]
+        h.set_profiling(False)
+    if graph:
+        h.set_graph(True)
+    elif profile:
+        h.set_profiling(True, kernel_filter=dominant)
+    # both precisions start the timed steps from the same coordinates
+    x.copy_(torch.tensor(np.asarray(xyz, np.float64).reshape(-1), device="cuda"))
+    gen.manual_seed(1234)
+    for _ in range(warmup):
+        step()
+    replays0 = h.graph_replays()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    t1 = time.perf_counter()
+    replays = h.graph_replays() - replays0
+    last = dict(ffe._last)
+    g = h.gradient()
+    u, up = h.induced()
+    prof = {}
+    if profile:
+        if graph:
+            h.set_graph(False)
+            h.set_profiling(True, kernel_filter=dominant)
+            for _ in range(steps):
+                step()
+        prof = {"calibration": calib, "timed": h.profile(), "graph_replays": replays}
+        h.set_profiling(False)
+    extra = {"graph_replays": replays}
+    if lam is not None:
+        extra["dudl"] = ffe.getdEdL()
+    ffe.destroy()
+    return (t1 - t0) / steps, last, g, u, prof, extra
+
+
+def cpu_baseline(top, box, xyz, params, workload, lam=None, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.build()
     nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    import dataclasses
+    if lam is not None:
+        params = dataclasses.replace(params, lambda_value=lam)
     s = pyoracle.System(xyz, box, top)
     t0 = time.perf_counter()
     count = 0
     while True:
-        e, g, u, up = pyoracle.evaluate(s, params, nthreads=nth)
+        e, g, lg = pyoracle.evaluate_lambda(s, params, nthreads=nth)
         count += 1
         if time.perf_counter() - t0 > budget_s or count >= 50:
             break
     dt = time.perf_counter() - t0
     return {"value": count / dt, "unit": "evals/s", "cores": nth, "kind": "port",
-            "sample": f"{count} full FP64 E+G evaluations of the same water-20736 workload "
+            "sample": f"{count} full FP64 E+G evaluations of the same {workload} workload "
                       f"(oracle/amoeba_oracle.c, OpenMP, {dt:.1f} s)"}, e, g
+
+
+WORKLOADS = {
+    # name: (description, builder) -- the first is the headline configuration
+    "water-20736": "6912 AMOEBA-2003 waters, 59.1 A cubic box, PME 64^3 order 5, Ewald 7 A, vdW 9 A, PCG 1e-5 D",
+    "water-288000": "96000 AMOEBA-2003 waters, 144.3 A cubic box, PME 144^3 order 5, Ewald 7 A, vdW 9 A, PCG 1e-5 D, "
+                    "each step after a Gaussian perturbation of sigma 0.01 A applied on the device",
+    "chain-23557": "2500-atom random-walk chain with random multipoles in 7019 waters, 62.2 A box, PME 64^3, "
+                   "20 soft atoms at lambda 0.5: OST electrostatics + vdW softcore with dU/dL",
+    "water-648": "216 AMOEBA-2003 waters, 18.643 A cubic box, PME 24^3, PCG 1e-6 D",
+}
+
+
+def build_workload(name):
+    from forcefieldx_amd.forcefield import NonbondedParams
+    from forcefieldx_amd.systems import chain_in_water, water_box
+    lam = None
+    if name == "water-20736":
+        xyz, box, top = water_box(N_WATERS, BOX, seed=SEED)
+        params, grid = NonbondedParams(vdw_cutoff=9.0, grid=GRID, polar_eps=1e-5), GRID
+    elif name == "water-288000":
+        xyz, box, top = water_box(96000, 144.3, seed=SEED)
+        grid = (144, 144, 144)
+        params = NonbondedParams(vdw_cutoff=9.0, grid=grid, polar_eps=1e-5)
+    elif name == "chain-23557":
+        xyz, box, top = chain_in_water(2500, 7019, 62.2, seed=7, n_soft=20)
+        grid = (64, 64, 64)
+        params = NonbondedParams(vdw_cutoff=9.0, grid=grid, polar_eps=1e-5, elec_lambda=True)
+        lam = 0.5
+    elif name == "water-648":
+        xyz, box, top = water_box(216, 18.643, seed=1)
+        grid = (24, 24, 24)
+        params = NonbondedParams(vdw_cutoff=9.0, grid=grid, polar_eps=1e-6)
+    else:
+        raise SystemExit(f"unknown workload {name}; one of {', '.join(WORKLOADS)}")
+    return xyz, box, top, params, grid, lam
 
 
 def main():
@@ -187,6 +313,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="water-20736", choices=list(WORKLOADS))
+    ap.add_argument("--no-graph", action="store_true", help="time the regular launch path instead of graph replays")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -199,8 +327,6 @@ def main():
         sys.exit(2)
     from forcefieldx_amd import build as nbbuild
     from forcefieldx_amd import ffi
-    from forcefieldx_amd.forcefield import NonbondedParams
-    from forcefieldx_amd.systems import water_box
     if not os.path.exists(ffi.LIB_PATH):
         nbbuild.build()
     if args.copy_calibration:
@@ -210,31 +336,38 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    xyz, box, top = water_box(N_WATERS, BOX, seed=SEED)
-    params = NonbondedParams(vdw_cutoff=9.0, grid=GRID, polar_eps=1e-5)
+    xyz, box, top, params, grid, lam = build_workload(args.workload)
+    graph = not args.no_graph
+    perturb = args.workload == "water-288000"
+    runner = time_gpu_perturbed if perturb else time_gpu
 
-    t64, e64, g64, u64, prof64 = time_gpu(top, box, xyz, params, ffi.NB_PREC_FP64, args.steps, args.warmup, False)
-    tmx, emx, gmx, umx, profmx = time_gpu(top, box, xyz, params, ffi.NB_PREC_MIXED, args.steps, args.warmup,
-                                          not args.no_profile)
+    t64, e64, g64, u64, prof64, x64 = runner(top, box, xyz, params, ffi.NB_PREC_FP64, args.steps, args.warmup, False,
+                                             graph, lam)
+    tmx, emx, gmx, umx, profmx, xmx = runner(top, box, xyz, params, ffi.NB_PREC_MIXED, args.steps, args.warmup,
+                                             not args.no_profile, graph, lam)
     frc_rms = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
     frc_max = float(np.max(np.abs(gmx - g64)) / np.max(np.abs(g64)))
-    ind_rms = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+    ind_rms = float(np.linalg.norm(umx - u64) / max(np.linalg.norm(u64), 1e-300))
 
     roof = None
     if profmx:
         name, (tot, cnt) = max(profmx["timed"].items(), key=lambda kv: kv[1][0])
-        nbk = max(2, (GRID[0] + 7) // 8)
-        w = {"n": top.n, "K": GRID[0], "s": 4, "ewald_rows": 2 * (int(emx["n_perm"]) - top.n),
-             "vdw_rows": 2 * int(emx["n_vdw"]), "nbricks": nbk ** 3, "ext": -(-GRID[0] // nbk) + 4}
+        nbk = max(2, (grid[0] + 7) // 8)
+        w = {"n": top.n, "K": grid[0], "s": 4, "ewald_rows": 2 * (int(emx["n_perm"]) - top.n),
+             "vdw_rows": 2 * int(emx["n_vdw"]), "nbricks": nbk ** 3, "ext": -(-grid[0] // nbk) + 4}
         roof = roofline_for(name, tot / cnt, w, "mixed")
-        roof["share_of_step"] = tot / (tmx * 1e3 * args.steps)
-        roof["launches_per_step"] = cnt / args.steps
-        roof["traffic"] = measured_traffic(name)
+        kernel_steps = args.steps
+        roof["launches_per_step"] = cnt / kernel_steps
+        roof["share_of_step"] = tot / (tmx * 1e3 * kernel_steps)
+        roof["traffic"] = measured_traffic(name) if args.workload == "water-20736" else None
+        roof["timed_with"] = ("HIP events around each launch of this kernel in the timed region" if not graph else
+                              "HIP events around each launch of this kernel over the same number of regular "
+                              "evaluations right after the graph-replay timed region")
 
     cpu = None
     e_cpu = None
     if not args.no_cpu:
-        cpu, e_cpu, g_cpu = cpu_baseline(top, box, xyz, params)
+        cpu, e_cpu, g_cpu = cpu_baseline(top, box, xyz, params, args.workload, lam)
 
     out = {
         "metric": METRIC,
@@ -249,9 +382,8 @@ def main():
         "vs_baseline": None,
         "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG",
         "data": "synthetic",
-        "config": {"workload": "water-20736: 6912 AMOEBA-2003 waters, 59.1 A cubic box, PME 64^3 order 5, "
-                               "Ewald 7 A, vdW 9 A, PCG 1e-5 D", "atoms": top.n, "grid": list(GRID),
-                   "seed": SEED, "parallelism": "single GPU"},
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "atoms": top.n, "grid": list(grid),
+                   "launch": "HIP graph replay" if graph else "regular launches", "parallelism": "single GPU"},
         "fp64_evals_per_s": 1.0 / t64,
         "fp64_ms_per_step": t64 * 1e3,
         "mixed_over_fp64": t64 / tmx,
@@ -262,12 +394,16 @@ def main():
         "energy_fp64": e64["total"],
         "pcg_iters": {"mixed": emx["pcg_iters"], "fp64": e64["pcg_iters"]},
         "pairs": {"ewald": int(e64["n_perm"]) - top.n, "vdw": int(e64["n_vdw"])},
+        "graph_replays": {"mixed": xmx["graph_replays"], "fp64": x64["graph_replays"]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if lam is not None:
+        out["dudl"] = {"mixed": xmx["dudl"], "fp64": x64["dudl"]}
     if cpu:
         out["gpu_over_cpu"] = out["value"] / cpu["value"]
-        out["energy_fp64_minus_oracle"] = e64["total"] - e_cpu["total"]
+        if not perturb:
+            out["energy_fp64_minus_oracle"] = e64["total"] - e_cpu["total"]
     if profmx:
         out["kernel_ms_per_step_calibration"] = {k: round(v[0] / 2, 4) for k, v in
                                                  sorted(profmx["calibration"].items(), key=lambda kv: -kv[1][0])}
