@@ -320,3 +320,126 @@ def chain_in_water(n_chain, n_waters, box_length, seed=0, n_soft=0, soft_start=N
         top.soft[s0:s0 + n_soft] = 1
     xyz = np.concatenate([chain, wxyz.reshape(-1, 3)]).reshape(-1)
     return xyz, box, top
+
+
+def lattice_vectors(box):
+    """Rows a, b, c of a cell given as (a, b, c, alpha, beta, gamma) in degrees,
+    with a along x and b in the xy plane (the convention of Crystal.java)."""
+    a, b, c, al, be, ga = [float(v) for v in box]
+    if al == 90.0 and be == 90.0 and ga == 90.0:
+        return np.diag([a, b, c])
+    ca, cb, cg = np.cos(np.deg2rad([al, be, ga]))
+    sg, sb = np.sin(np.deg2rad(ga)), np.sin(np.deg2rad(be))
+    bt = (ca - cb * cg) / sg
+    gt = np.sqrt(sb * sb - bt * bt)
+    return np.array([[a, 0.0, 0.0], [b * cg, b * sg, 0.0], [c * cb, c * bt, c * gt]])
+
+
+def replicate(xyz, box, top, reps):
+    """Supercell of reps = (l, m, n) copies of the cell (ReplicatesCrystal):
+    replica r = (i, j, k) holds atoms r * n_atoms + 0..n_atoms-1 translated by
+    i a + j b + k c.  Returns (xyz (3 N,), box[6], Topology)."""
+    l, m, nn = reps
+    L = lattice_vectors(box)
+    x = np.asarray(xyz, float).reshape(-1, 3)
+    n = top.n
+    shifts = [i * L[0] + j * L[1] + k * L[2] for k in range(nn) for j in range(m) for i in range(l)]
+    R = len(shifts)
+    X = np.concatenate([x + s for s in shifts])
+    off = lambda a, r: np.where(a >= 0, a + r * n, a)
+    moff = np.concatenate([top.mask_off[:-1] + r * top.mask_off[-1] for r in range(R)] + [[R * top.mask_off[-1]]])
+    gmax = int(max(top.group.max(), top.molecule.max())) + 1
+    t = Topology(n=R * n, frame=np.tile(top.frame, R), axis=np.concatenate([off(top.axis, r) for r in range(R)]),
+                 mpole=np.tile(top.mpole, (R, 1)), polarity=np.tile(top.polarity, R), thole=np.tile(top.thole, R),
+                 pdamp=np.tile(top.pdamp, R), group=np.concatenate([top.group + r * gmax for r in range(R)]),
+                 molecule=np.concatenate([top.molecule + r * gmax for r in range(R)]),
+                 vdw_class=np.tile(top.vdw_class, R), vdw_table=top.vdw_table,
+                 red_index=np.concatenate([top.red_index + r * n for r in range(R)]),
+                 red_value=np.tile(top.red_value, R), mask_off=moff.astype(np.int32),
+                 mask_atom=np.concatenate([top.mask_atom + r * n for r in range(R)]).astype(np.int32),
+                 mask_rel=np.tile(top.mask_rel, R).astype(np.int32),
+                 soft=np.tile(top.soft, R))
+    sbox = np.array([box[0] * l, box[1] * m, box[2] * nn, box[3], box[4], box[5]], float)
+    return X.reshape(-1), sbox, t
+
+
+def small_crystal(n_target, box, seed=0, min_dist=2.6, sizes=(3, 4, 5), tries=4000):
+    """P1 cell of small rigid molecules with random multipoles (the batched
+    small-crystal workload): each molecule is a 3-5 atom chain (bond 1.5 A,
+    bond angles >= 109.5 deg) with charges N(0, 0.3) e made neutral per
+    molecule, local dipoles of 0.1 e*A and traceless quadrupoles of 0.3 e*A^2
+    in Z-then-X frames, polarizabilities U(0.3, 1.5) A^3 and Thole 0.39; one
+    polarization group per molecule.  Molecules are placed at random with every
+    atom at least ``min_dist`` from every image of the atoms already placed,
+    until ``n_target`` atoms or ``tries`` failed attempts.  The cell may be
+    thinner than twice the cutoff (image sums).  Returns (xyz, box, Topology)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    box = np.asarray(box, float)
+    L = lattice_vectors(box)
+    img = np.array([(i, j, k) for i in (-2, -1, 0, 1, 2) for j in (-2, -1, 0, 1, 2) for k in (-2, -1, 0, 1, 2)],
+                   float) @ L
+    placed = np.zeros((0, 3))
+    mols = []
+    fails = 0
+    while placed.shape[0] < n_target and fails < tries:
+        na = int(rng.choice(sizes))
+        if placed.shape[0] + na > n_target + 2:
+            na = max(3, n_target - placed.shape[0])
+        # rigid chain geometry
+        geo = [np.zeros(3)]
+        d = rng.normal(size=3)
+        geo.append(1.5 * d / np.linalg.norm(d))
+        while len(geo) < na:
+            v = rng.normal(size=3)
+            c = geo[-1] + 1.5 * v / np.linalg.norm(v)
+            back = geo[-2] - geo[-1]
+            if np.dot(back, c - geo[-1]) / (1.5 * 1.5) > np.cos(np.deg2rad(109.5)):
+                continue
+            geo.append(c)
+        geo = np.array(geo) - np.mean(geo, axis=0)
+        R = _random_rotations(rng, 1)[0]
+        pos = geo @ R.T + rng.random(3) @ L
+        if placed.shape[0]:
+            ok = True
+            for p in pos:
+                dd = placed[:, None, :] - p[None, None, :] + img[None, :, :]
+                if np.einsum("ijk,ijk->ij", dd, dd).min() < min_dist ** 2:
+                    ok = False
+                    break
+            if not ok:
+                fails += 1
+                continue
+        mols.append(pos)
+        placed = np.concatenate([placed, pos])
+    n = placed.shape[0]
+    frame = np.full(n, FRAME_ZTHENX, np.int32)
+    axis = -np.ones((n, 3), np.int32)
+    mp = np.zeros((n, 10))
+    mol = np.zeros(n, np.int32)
+    bonds = []
+    a0 = 0
+    for m, pos in enumerate(mols):
+        na = len(pos)
+        q = rng.normal(0.0, 0.3, na)
+        q -= q.mean()
+        dv = rng.normal(size=(na, 3))
+        dv *= 0.1 / np.linalg.norm(dv, axis=1, keepdims=True)
+        mp[a0:a0 + na] = np.concatenate([q[:, None], dv, _random_traceless(rng, na, 0.3)], axis=1)
+        for j in range(na):
+            i = a0 + j
+            if j + 1 < na:
+                axis[i, 0], axis[i, 1] = i + 1, (i - 1 if j > 0 else i + 2)
+            else:
+                axis[i, 0], axis[i, 1] = i - 1, i - 2
+            if j + 1 < na:
+                bonds.append((i, i + 1))
+        mol[a0:a0 + na] = m
+        a0 += na
+    pol = rng.uniform(0.3, 1.5, n)
+    off, atoms, rels = masks_from_bonds(n, bonds)
+    tab = vdw_pair_table([CHAIN_VDW[0]], [CHAIN_VDW[1]])
+    top = Topology(n=n, frame=frame, axis=axis, mpole=mp, polarity=pol, thole=np.full(n, WATER_THOLE),
+                   pdamp=pol ** (1.0 / 6.0), group=mol.copy(), molecule=mol, vdw_class=np.zeros(n, np.int32),
+                   vdw_table=tab, red_index=np.arange(n, dtype=np.int32), red_value=np.zeros(n), mask_off=off,
+                   mask_atom=atoms, mask_rel=rels)
+    return placed.reshape(-1), box, top

@@ -17,6 +17,7 @@
 #include "amoeba_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -67,6 +68,7 @@ static int nthreads_of(const ora_params *p) {
 /* ------------------------------------------------------------------------ */
 typedef struct {
   int aperiodic;
+  int multi;       /* cell thinner than twice the cutoff: explicit image sums */
   double Ai[3][3]; /* rows: lattice vectors a, b, c in Cartesian */
   double A[3][3];  /* inverse of Ai: fractional = cart . A */
   double vol;
@@ -127,6 +129,42 @@ static inline double cell_image(const cell_t *c, double *d) {
   d[1] = yf * c->Ai[1][1] + zf * c->Ai[2][1];
   d[2] = zf * c->Ai[2][2];
   return dot3(d, d);
+}
+
+/* Perpendicular width of the cell along lattice direction d. */
+static double cell_width(const cell_t *c, int d) {
+  return 1.0 / sqrt(c->A[0][d] * c->A[0][d] + c->A[1][d] * c->A[1][d] + c->A[2][d] * c->A[2][d]);
+}
+
+/* Image sums for cells thinner than twice the cutoff (the reference builds a
+ * ReplicatesCrystal, ReplicatesCrystal.java:172-202, applied when the
+ * interfacial radius is below the cutoff, ForceFieldEnergy.java:893-901).
+ * Here a list entry names the partner atom k and the lattice translation T of
+ * the image: e = k + n * code(T).  Bonded masks apply only to the pair as
+ * given (T = 0), as the reference masks only the asymmetric unit's own pairs. */
+#define IMG_R 8
+#define IMG_W (2 * IMG_R + 1)
+#define IMG_ORIGIN (IMG_R + IMG_W * (IMG_R + IMG_W * IMG_R))
+
+static inline int img_encode(const int *T) { return (T[0] + IMG_R) + IMG_W * ((T[1] + IMG_R) + IMG_W * (T[2] + IMG_R)); }
+
+/* displacement (x_k + T) - x_i of list entry e of atom i; returns r^2 and
+ * sets *k and *orig (the entry is the pair as given: masks apply) */
+static inline double entry_vec(const cell_t *c, int n, const double *xyz, int i, int e, int *k, double *r,
+                               int *orig) {
+  if (!c->multi) {
+    *k = e;
+    *orig = 1;
+    sub3(xyz + 3 * e, xyz + 3 * i, r);
+    return cell_image(c, r);
+  }
+  int kk = e % n, code = e / n;
+  int T[3] = {code % IMG_W - IMG_R, (code / IMG_W) % IMG_W - IMG_R, code / (IMG_W * IMG_W) - IMG_R};
+  *k = kk;
+  *orig = code == IMG_ORIGIN;
+  sub3(xyz + 3 * kk, xyz + 3 * i, r);
+  for (int a = 0; a < 3; a++) r[a] += T[0] * c->Ai[0][a] + T[1] * c->Ai[1][a] + T[2] * c->Ai[2][a];
+  return dot3(r, r);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -713,7 +751,66 @@ typedef struct {
   int *nbr;   /* pairs */
 } nlist_t;
 
+static void nlist_build_images(nlist_t *L, const cell_t *c, int n, const double *xyz, double cut) {
+  /* half list over atom-image pairs: every k > i with each translation T in
+   * range, and k == i with T in the positive half space */
+  double cut2 = cut * cut;
+  int m[3];
+  for (int d = 0; d < 3; d++) m[d] = (int)ceil(cut / cell_width(c, d)) + 1;
+  L->n = n;
+  L->off = malloc((n + 1) * sizeof(int));
+  int *cnt = calloc(n, sizeof(int));
+  int **lists = malloc(n * sizeof(int *));
+  #pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < n; i++) {
+    int cap = 256, q = 0;
+    int *lst = malloc(cap * sizeof(int));
+    for (int k = i; k < n; k++) {
+      double d0[3];
+      sub3(xyz + 3 * k, xyz + 3 * i, d0);
+      int T0[3];
+      for (int a = 0; a < 3; a++) {
+        double f = d0[0] * c->A[0][a] + d0[1] * c->A[1][a] + d0[2] * c->A[2][a];
+        T0[a] = -(int)floor(f + 0.5);
+      }
+      for (int dz = -m[2]; dz <= m[2]; dz++)
+        for (int dy = -m[1]; dy <= m[1]; dy++)
+          for (int dx = -m[0]; dx <= m[0]; dx++) {
+            int T[3] = {T0[0] + dx, T0[1] + dy, T0[2] + dz};
+            if (k == i) {
+              /* self images: one of each pair T / -T */
+              if (T[0] == 0 && T[1] == 0 && T[2] == 0) continue;
+              if (T[2] < 0 || (T[2] == 0 && (T[1] < 0 || (T[1] == 0 && T[0] < 0)))) continue;
+            }
+            double r[3];
+            for (int a = 0; a < 3; a++) r[a] = d0[a] + T[0] * c->Ai[0][a] + T[1] * c->Ai[1][a] + T[2] * c->Ai[2][a];
+            if (dot3(r, r) > cut2) continue;
+            if (abs(T[0]) > IMG_R || abs(T[1]) > IMG_R || abs(T[2]) > IMG_R) {
+              fprintf(stderr, "oracle: image translation out of range\n");
+              abort();
+            }
+            if (q == cap) { cap *= 2; lst = realloc(lst, cap * sizeof(int)); }
+            lst[q++] = k + n * img_encode(T);
+          }
+    }
+    cnt[i] = q;
+    lists[i] = lst;
+  }
+  L->off[0] = 0;
+  for (int i = 0; i < n; i++) L->off[i + 1] = L->off[i] + cnt[i];
+  L->nbr = malloc((size_t)(L->off[n] > 0 ? L->off[n] : 1) * sizeof(int));
+  for (int i = 0; i < n; i++) {
+    memcpy(L->nbr + L->off[i], lists[i], cnt[i] * sizeof(int));
+    free(lists[i]);
+  }
+  free(lists); free(cnt);
+}
+
 static void nlist_build(nlist_t *L, const cell_t *c, int n, const double *xyz, double cut) {
+  if (c->multi) {
+    nlist_build_images(L, c, n, xyz, cut);
+    return;
+  }
   double cut2 = cut * cut;
   int cap = 1024;
   L->n = n;
@@ -1232,10 +1329,9 @@ static void induced_real_field(ctx_t *c, const int *off, const int *lst, double 
       const double *xi = c->s->xyz + 3 * i;
       double fi[3] = {0, 0, 0}, pi_[3] = {0, 0, 0};
       for (int t = off[i]; t < off[i + 1]; t++) {
-        int k = lst[t];
+        int k, orig;
         double r[3];
-        sub3(c->s->xyz + 3 * k, xi, r);
-        double r2 = cell_image(&c->cell, r);
+        double r2 = entry_vec(&c->cell, n, c->s->xyz, i, lst[t], &k, r, &orig);
         double rl = sqrt(r2);
         double bn[3];
         ewald_bn(rl, aewald, 2, bn);
@@ -1328,23 +1424,22 @@ static void permanent_field(ctx_t *c) {
       int ne = 0, np = 0;
       double fi[3] = {0, 0, 0}, fip[3] = {0, 0, 0};
       for (int t = c->nl.off[i]; t < c->nl.off[i + 1]; t++) {
-        int k = c->nl.nbr[t];
+        int e = c->nl.nbr[t], k, orig;
         double r[3];
-        sub3(s->xyz + 3 * k, xi, r);
-        double r2 = cell_image(&c->cell, r);
+        double r2 = entry_vec(&c->cell, n, s->xyz, i, e, &k, r, &orig);
         if (r2 > c->off2) continue;
         npairs_tot++;
-        ew[ne++] = k;
+        ew[ne++] = e;
         double rl = sqrt(r2);
-        if (rl < pc2) pcq[np++] = k;
+        if (rl < pc2) pcq[np++] = e;
         mp3_t K;
         mp3_load(c->gmp + 10 * k, &K);
         double bn[4];
         ewald_bn(rl, c->aewald, 3, bn);
         thole_t th;
         thole_terms(rl, NULL, c->ipd[i], c->ipd[k], s->thole[i], s->thole[k], &th);
-        double dsc = (s->group[k] == gi) ? prm->d11 : 1.0;
-        double psc = pm[k];
+        double dsc = (orig && s->group[k] == gi) ? prm->d11 : 1.0;
+        double psc = orig ? pm[k] : 1.0;
         double rr3 = 1.0 / (rl * r2), rr5 = 3.0 * rr3 / r2, rr7 = 5.0 * rr5 / r2;
         double D1 = bn[1] - (1.0 - th.s3 * dsc) * rr3, D2 = bn[2] - (1.0 - th.s5 * dsc) * rr5,
                D3 = bn[3] - (1.0 - th.s7 * dsc) * rr7;
@@ -1736,7 +1831,9 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
       mp3_t I;
       mp3_load(c->gmp + 10 * i, &I);
       for (int t = c->ew_off[i]; t < c->ew_off[i + 1]; t++) {
-        int k = c->ew[t];
+        int k, orig;
+        double r[3];
+        double r2 = entry_vec(&c->cell, n, s->xyz, i, c->ew[t], &k, r, &orig);
         int soft = ph->soft_pairs && s->soft && (s->soft[i] || s->soft[k]);
         if (ph->vapor && s->soft && s->soft[i] && s->soft[k]) {
           /* vapor phase: both-soft pairs that the condensed phase softens
@@ -1744,9 +1841,6 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
           int same = s->molecule ? s->molecule[i] == s->molecule[k] : 0;
           if ((prm->intermolecular_softcore && !same) || (prm->intramolecular_softcore && same)) continue;
         }
-        double r[3];
-        sub3(s->xyz + 3 * k, xi, r);
-        double r2 = cell_image(&c->cell, r);
         double rl = sqrt(r2);
         mp3_t K;
         mp3_load(c->gmp + 10 * k, &K);
@@ -1770,7 +1864,7 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
           perm_geometry(r, &I, &K, &g);
           double f1[3], t1[3], u1[3], f2[3], t2[3], u2[3];
           double e = perm_terms(r, &I, &K, &g, B, f1, t1, u1);
-          double s1 = 1.0 - mm[k];
+          double s1 = orig ? 1.0 - mm[k] : 0.0;
           if (s1 != 0.0) {
             e -= s1 * perm_terms(r, &I, &K, &g, R, f2, t2, u2);
             for (int a = 0; a < 3; a++) { f1[a] -= s1 * f2[a]; t1[a] -= s1 * t2[a]; u1[a] -= s1 * u2[a]; }
@@ -1815,8 +1909,8 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
           polctx_t cx;
           cx.bn = bn; cx.rr = rr;
           thole_terms(rl, r, c->ipd[i], c->ipd[k], s->thole[i], s->thole[k], &cx.th);
-          cx.scalep = pm[k];
-          cx.scaled = (s->group[k] == s->group[i]) ? prm->d11 : 1.0;
+          cx.scalep = orig ? pm[k] : 1.0;
+          cx.scaled = (orig && s->group[k] == s->group[i]) ? prm->d11 : 1.0;
           cx.direct = (prm->polarization == ORA_POL_DIRECT);
           double f[3], a2[3], a3[3];
           double e = pol_terms(r, &I, &K, c->ind + 3 * i, c->indp + 3 * i, c->ind + 3 * k, c->indp + 3 * k, &cx, 1,
@@ -1951,18 +2045,19 @@ static double vdw_energy(ctx_t *c, double *grad, int *count, double *vir, double
       int ri = s->red_index[i];
       double rv_i = s->red_value[i];
       for (int tt = c->nl.off[i]; tt < c->nl.off[i + 1]; tt++) {
-        int k = c->nl.nbr[tt];
+        int k, orig;
         double d[3];
-        sub3(red + 3 * i, red + 3 * k, d);
-        double r2 = cell_image(&c->cell, d);
+        double r2 = entry_vec(&c->cell, n, red, i, c->nl.nbr[tt], &k, d, &orig);
+        scale3(-1.0, d);   /* d = red_i - (red_k + T) */
         int ck = s->vdw_class[k];
         const double *pe = s->vdw_table + 2 * (ci * s->n_class + ck);
         double irv = pe[0];
-        if (!(r2 <= off2 && mask[k] > 0.0 && irv > 0.0)) continue;
+        double mk = orig ? mask[k] : 1.0;
+        if (!(r2 <= off2 && mk > 0.0 && irv > 0.0)) continue;
         double r = sqrt(r2);
         int soft = soft_pair(s, prm, i, k);
         double sc1 = soft ? sf.sc1 : 0.0, sc2 = soft ? sf.sc2 : 1.0;
-        double ev = mask[k] * pe[1];
+        double ev = mk * pe[1];
         double rho = r * irv;
         double rho6 = pow(rho, 6.0), rho7 = rho6 * rho;
         double rd = rho + dl, rd6 = pow(rd, 6.0), rd7 = rd6 * rd;
@@ -2101,6 +2196,11 @@ static void ctx_open(ctx_t *c, const ora_system *s, const ora_params *prm, const
   c->nth = nthreads_of(prm);
   int n = s->n;
   cell_init(&c->cell, s->box);
+  if (!c->cell.aperiodic) {
+    double cmax = prm->ewald_cutoff > prm->vdw_cutoff ? prm->ewald_cutoff : prm->vdw_cutoff;
+    for (int d = 0; d < 3; d++)
+      if (0.5 * cell_width(&c->cell, d) < cmax) c->cell.multi = 1;
+  }
   c->aewald = c->cell.aperiodic ? 0.0 : prm->ewald_alpha;
   c->off2 = prm->ewald_cutoff * prm->ewald_cutoff;
   c->grad = calloc(3 * n, sizeof(double));
