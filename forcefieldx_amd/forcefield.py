@@ -77,6 +77,9 @@ class NonbondedParams:
     elec_lambda: bool = False          # lambda also couples the soft atoms' electrostatics (OST)
     ligand_vapor_elec: bool = True     # AlchemicalParameters.java:249
     no_ligand_condensed_scf: bool = True  # AlchemicalParameters.java:251
+    scf_algorithm: int = 0             # 0 PCG, 1 SOR (ParticleMeshEwald.java:2288-2299)
+    polar_sor: float = 0.7             # SORRegion.java:65-72
+    sor_fallback: bool = True          # ParticleMeshEwald.java:563
 
     def resolved_grid(self, box):
         """Grid dimensions: explicit, or the reference density rule."""

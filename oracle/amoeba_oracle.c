@@ -1276,6 +1276,7 @@ static void remove_masks(const ora_system *s, int i, double *m, double *p) {
 typedef struct {
   int perm_real;      /* permanent real-space energy on */
   int polar;          /* polarization on */
+  int sor;            /* solve the dipoles by SOR (the vapor phase, ParticleMeshEwald.java:1744-1746) */
   int soft_pairs;     /* pairs with a soft atom use l2 and the softcore shift */
   int vapor;          /* skip both-soft pairs per the inter/intramolecular flags */
   double l2, lalpha, dlalpha, d2lalpha;
@@ -1283,7 +1284,7 @@ typedef struct {
   double pol, dpol, d2pol;
 } phase_t;
 
-static const phase_t PHASE_PLAIN = {1, 1, 0, 0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
+static const phase_t PHASE_PLAIN = {1, 1, 0, 0, 0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, 0.0};
 
 typedef struct {
   const ora_system *s;
@@ -1578,6 +1579,46 @@ static int pcg_solve(ctx_t *c, ora_energies *out) {
   induced_field(c, c->ind, c->indp, f, fp);
   free(r); free(rp); free(z); free(zp); free(p); free(pp); free(vec); free(vecp); free(f); free(fp); free(au);
   free(aup);
+  return status;
+}
+
+/* SOR (ParticleMeshEwald.scfBySOR, ParticleMeshEwald.java:2331-2423, with
+ * the update of SORRegion.java:236-258): from the direct dipoles,
+ * u <- u + w (u_direct + alpha E(u) - u) for u and uCR; the RMS step
+ * 4.80321 sqrt(max(sum du^2, sum duCR^2) / n) must fall below polar_eps; a
+ * rise fails the solve, and 1000 cycles are the limit. */
+static int sor_solve(ctx_t *c, ora_energies *out, double w) {
+  int n = c->n;
+  const double *pol = c->s->polarity;
+  size_t sz = 3 * (size_t)n * sizeof(double);
+  double *f = malloc(sz), *fp = malloc(sz);
+  memcpy(c->ind, c->dir, sz);
+  memcpy(c->indp, c->dirp, sz);
+  double eps = 100.0, prev;
+  int iter = 0, status = 0;
+  for (;;) {
+    induced_field(c, c->ind, c->indp, f, fp);
+    double e1 = 0.0, e2 = 0.0;
+    for (int q = 0; q < 3 * n; q++) {
+      double a = pol[q / 3];
+      double du = w * (c->dir[q] + a * f[q] - c->ind[q]);
+      double dv = w * (c->dirp[q] + a * fp[q] - c->indp[q]);
+      c->ind[q] += du;
+      c->indp[q] += dv;
+      e1 += du * du;
+      e2 += dv * dv;
+    }
+    iter++;
+    prev = eps;
+    eps = ELEC_ANG_TO_DEBYE * sqrt((e1 > e2 ? e1 : e2) / n);
+    if (eps > prev) { status = 2; break; }
+    if (iter >= 1000) { status = 3; break; }
+    if (eps < c->prm->polar_eps) break;
+  }
+  out->pcg_iters = iter;
+  out->pcg_eps = eps;
+  induced_field(c, c->ind, c->indp, f, fp);
+  free(f); free(fp);
   return status;
 }
 
@@ -2299,7 +2340,14 @@ static int elec_phase(ctx_t *c, ora_energies *out, double *ind, double *indcr) {
       }
     }
     if (prm->polarization == ORA_POL_MUTUAL) {
-      status = pcg_solve(c, out);
+      if (prm->scf_algorithm == 1 || c->ph->sor) {
+        status = sor_solve(c, out, prm->polar_sor);
+      } else {
+        status = pcg_solve(c, out);
+        /* ParticleMeshEwald.java:2300-2312: a failed PCG is repeated by SOR
+         * from the direct dipoles with the conservative factor 0.65 */
+        if (status && prm->sor_fallback) status = sor_solve(c, out, 0.65);
+      }
     } else {
       double *f = malloc(s3), *fp = malloc(s3);
       induced_field(c, c->ind, c->indp, f, fp);
@@ -2364,9 +2412,8 @@ void ora_alchemical_factors(const ora_params *prm, double lambda, ora_alchemy *f
  *     polarizability, which is how it is built here;
  * (3) the soft atoms in vacuum, no Ewald screening or cutoff (ligandElec,
  *     :1704-1790), built as the soft atoms plus their frame atoms (the
- *     latter with no multipole or polarizability) in an aperiodic cell.  The
- *     reference solves this phase by SOR; PCG to the same tolerance is used
- *     here. */
+ *     latter with no multipole or polarizability) in an aperiodic cell,
+ *     whose dipoles are solved by SOR as in the reference. */
 static int ost_electrostatics(ctx_t *c1, ora_energies *out, double *ind, double *indcr) {
   const ora_system *s = c1->s;
   const ora_params *prm = c1->prm;
@@ -2473,7 +2520,7 @@ static int ost_electrostatics(ctx_t *c1, ora_energies *out, double *ind, double 
     q3.list_buffer = 0.0;
     q3.do_vdw = 0;
     phase_t p3 = PHASE_PLAIN;
-    p3.soft_pairs = 1; p3.vapor = 1;
+    p3.soft_pairs = 1; p3.vapor = 1; p3.sor = 1;
     p3.l2 = 1.0 - f.lpow_perm;
     p3.perm = 1.0 - f.lpow_perm; p3.dperm = -f.dlpow_perm; p3.d2perm = -f.d2lpow_perm;
     if (lam <= prm->pol_lambda_end) {

@@ -91,6 +91,10 @@ typedef struct {
   double pol_lambda_start, pol_lambda_end; /* 0.75, 1 */
   int ligand_vapor_elec;    /* 1 */
   int no_ligand_condensed_scf; /* 1 */
+  /* SCF algorithm (ParticleMeshEwald.java:2247-2325, SORRegion.java) */
+  int scf_algorithm;        /* 0 PCG, 1 SOR */
+  double polar_sor;         /* SOR factor (0.7) */
+  int sor_fallback;         /* SOR with factor 0.65 after a failed PCG (1) */
 } ora_params;
 
 typedef struct {

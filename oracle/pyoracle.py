@@ -52,7 +52,8 @@ class OraParams(C.Structure):
                 ("elec_lambda", C.c_int), ("perm_lambda_alpha", C.c_double), ("perm_lambda_exponent", C.c_double),
                 ("perm_lambda_start", C.c_double), ("perm_lambda_end", C.c_double),
                 ("pol_lambda_exponent", C.c_double), ("pol_lambda_start", C.c_double),
-                ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int), ("no_ligand_condensed_scf", C.c_int)]
+                ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int), ("no_ligand_condensed_scf", C.c_int),
+                ("scf_algorithm", C.c_int), ("polar_sor", C.c_double), ("sor_fallback", C.c_int)]
 
 
 class OraAlchemy(C.Structure):
@@ -159,7 +160,7 @@ def make_params(p, box=None, nthreads=0):
                      int(p.intramolecular_softcore), int(p.elec_lambda), p.perm_lambda_alpha,
                      p.perm_lambda_exponent, p.perm_lambda_start, p.perm_lambda_end, p.pol_lambda_exponent,
                      p.pol_lambda_start, p.pol_lambda_end, int(p.ligand_vapor_elec),
-                     int(p.no_ligand_condensed_scf))
+                     int(p.no_ligand_condensed_scf), int(p.scf_algorithm), p.polar_sor, int(p.sor_fallback))
 
 
 def alchemical_factors(params, lam):
