@@ -250,6 +250,81 @@ def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, grap
     return (t1 - t0) / steps, last, g, u, prof, extra
 
 
+def time_batch(systems, params, prec, steps, warmup, graph=True):
+    """One step = nb_evaluate_batch over every cell (graphs of all cells
+    launched together, each on its own stream)."""
+    from forcefieldx_amd import ffi
+    hs = []
+    for xyz, box, top, grid in systems:
+        h = ffi.Handle(ffi.params_from(params, grid, prec), top)
+        h.set_cell(box)
+        h.set_positions(xyz)
+        if graph:
+            h.set_graph(True)
+        hs.append(h)
+    try:
+        for _ in range(warmup):
+            ffi.evaluate_batch(hs, ffi.NB_GRAD)
+        r0 = sum(h.graph_replays() for h in hs)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            es, rc = ffi.evaluate_batch(hs, ffi.NB_GRAD)
+            if rc != ffi.NB_OK:
+                raise RuntimeError(f"batch evaluation failed: {rc}")
+        t1 = time.perf_counter()
+        replays = sum(h.graph_replays() for h in hs) - r0
+        grads = [h.gradient().reshape(-1) for h in hs]
+        iters = [e["pcg_iters"] for e in es]
+    finally:
+        for h in hs:
+            h.close()
+    return (t1 - t0) / steps, es, np.concatenate(grads), replays, iters
+
+
+def batch_main(args, systems, params):
+    from forcefieldx_amd import ffi
+    graph = not args.no_graph
+    t64, e64, g64, r64, it64 = time_batch(systems, params, ffi.NB_PREC_FP64, args.steps, args.warmup, graph)
+    tmx, emx, gmx, rmx, itmx = time_batch(systems, params, ffi.NB_PREC_MIXED, args.steps, args.warmup, graph)
+    natoms = sum(s[2].n for s in systems)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import dataclasses
+        import pyoracle
+        pyoracle.build()
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        done = 0
+        for xyz, box, top, grid in systems:
+            pyoracle.evaluate(pyoracle.System(xyz, box, top), dataclasses.replace(params, grid=grid), nthreads=nth)
+            done += 1
+            if time.perf_counter() - t0 > 12.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {"value": done / dt / len(systems), "unit": "sweeps/s", "cores": nth, "kind": "port",
+               "sample": f"{done} of the {len(systems)} cells evaluated one after the other by the FP64 oracle "
+                         f"({dt:.1f} s), scaled to a full sweep"}
+    out = {
+        "metric": METRIC, "value": 1.0 / tmx, "unit": "sweeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": tmx * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG", "data": "synthetic",
+        "config": {"workload": f"crystals-512: {WORKLOADS['crystals-512']}", "cells": len(systems), "atoms": natoms,
+                   "launch": "HIP graph replay, all cells launched together" if graph else "regular launches",
+                   "parallelism": "single GPU"},
+        "cell_evals_per_s": len(systems) / tmx,
+        "fp64_ms_per_step": t64 * 1e3, "mixed_over_fp64": t64 / tmx,
+        "force_rel_rms_err": float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64)),
+        "graph_replays": {"mixed": rmx, "fp64": r64},
+        "pcg_iters_mean": {"mixed": float(np.mean(itmx)), "fp64": float(np.mean(it64))},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out))
+
+
 def cpu_baseline(top, box, xyz, params, workload, lam=None, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -280,12 +355,15 @@ WORKLOADS = {
     "chain-23557": "2500-atom random-walk chain with random multipoles in 7019 waters, 62.2 A box, PME 64^3, "
                    "20 soft atoms at lambda 0.5: OST electrostatics + vdW softcore with dU/dL",
     "water-648": "216 AMOEBA-2003 waters, 18.643 A cubic box, PME 24^3, PCG 1e-6 D",
+    "crystals-512": "512 independent P1 triclinic cells (lengths 8-20 A, angles 70-110 deg) of small molecules "
+                    "with random multipoles, per-cell PME grids 16-32, Ewald and vdW 6 A, PCG 1e-5 D, image sums "
+                    "in cells thinner than twice the cutoff; a step is one sweep over all cells (nb_evaluate_batch)",
 }
 
 
 def build_workload(name):
     from forcefieldx_amd.forcefield import NonbondedParams
-    from forcefieldx_amd.systems import chain_in_water, water_box
+    from forcefieldx_amd.systems import chain_in_water, crystal_batch, water_box
     lam = None
     if name == "water-20736":
         xyz, box, top = water_box(N_WATERS, BOX, seed=SEED)
@@ -299,6 +377,10 @@ def build_workload(name):
         grid = (64, 64, 64)
         params = NonbondedParams(vdw_cutoff=9.0, grid=grid, polar_eps=1e-5, elec_lambda=True)
         lam = 0.5
+    elif name == "crystals-512":
+        systems = crystal_batch(512, seed=SEED)
+        params = NonbondedParams(ewald_cutoff=6.0, vdw_cutoff=6.0, polar_eps=1e-5)
+        return systems, None, None, params, None, None
     elif name == "water-648":
         xyz, box, top = water_box(216, 18.643, seed=1)
         grid = (24, 24, 24)
@@ -337,6 +419,8 @@ def main():
         torch.cuda.synchronize()
         del a, b
     xyz, box, top, params, grid, lam = build_workload(args.workload)
+    if args.workload == "crystals-512":
+        return batch_main(args, xyz, params)
     graph = not args.no_graph
     perturb = args.workload == "water-288000"
     runner = time_gpu_perturbed if perturb else time_gpu

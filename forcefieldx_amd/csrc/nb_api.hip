@@ -20,6 +20,7 @@ namespace {
 
 void cell_from_box(const double *box, CellD &c) {
   memset(&c, 0, sizeof(c));
+  c.kmask = NB_INDEX_MASK;
   if (!box || box[0] <= 0.0) {
     c.aperiodic = 1;
     return;
@@ -81,7 +82,7 @@ void free_all(NbHandle *h) {
   void *ptrs[] = {h->d_frame, h->d_axis, h->d_group0, h->d_vclass0, h->d_redidx, h->d_mpole_local, h->d_polar0,
                   h->d_thole0, h->d_ipd0, h->d_redval, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_vdw_table,
                   h->d_xyz, h->d_xyz_list, h->d_perm, h->d_iperm, h->d_cellid, h->d_cellcount, h->d_cellstart,
-                  h->d_nbr, h->d_ew, h->d_pc, h->d_sp, h->d_nsp, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_red4, h->d_mp,
+                  h->d_nbr, h->d_ew, h->d_pc, h->d_sp, h->d_nsp, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_wrap, h->d_red4, h->d_mp,
                   h->d_gmp, h->d_polar, h->d_group, h->d_vclass, h->d_grid, h->d_recip, h->d_theta, h->d_igrid,
                   h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
@@ -206,7 +207,7 @@ void set_ost_phases(NbHandle *h) {
   }
   if (h->child_vap) {
     ElecPhase p3;
-    p3.soft_pairs = 1; p3.vapor = 1;
+    p3.soft_pairs = 1; p3.vapor = 1; p3.sor = 1;
     p3.inter = p.intermolecular_softcore; p3.intra = p.intramolecular_softcore;
     p3.l2 = 1.0 - f.lpow_perm;
     p3.perm = 1.0 - f.lpow_perm; p3.dperm = -f.dlpow_perm; p3.d2perm = -f.d2lpow_perm;
@@ -495,6 +496,8 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_flags, 8);
     HIPCHECK(hipMemset(h->d_flags, 0, 8 * sizeof(int32_t)));
     alloc(&h->d_pos4, n);
+    alloc(&h->d_wrap, 3 * (size_t)n);
+    HIPCHECK(hipMemset(h->d_wrap, 0, sizeof(int32_t) * 3 * n));
     alloc(&h->d_red4, n);
     alloc(&h->d_mp, n);
     HIPCHECK(hipMemset(h->d_mp, 0, sizeof(AtomMP) * n));
@@ -558,6 +561,26 @@ int nb_set_cell(void *handle, const double box[6]) {
     for (int d = 0; d < 6; d++) h->box[d] = b[d];
     drop_graph(h);
     cell_from_box(b, h->cell);
+    if (!h->cell.aperiodic) {
+      // image sums when a perpendicular width is below twice the interaction
+      // cutoff (the reference then builds a ReplicatesCrystal,
+      // ForceFieldEnergy.java:893-901)
+      double cmax = std::max(h->prm.ewald_cutoff, h->prm.vdw_cutoff);
+      double lcut = cmax + h->prm.list_buffer;
+      double w[3];
+      for (int d = 0; d < 3; d++) {
+        const double(*A)[3] = h->cell.A;
+        w[d] = 1.0 / sqrt(A[0][d] * A[0][d] + A[1][d] * A[1][d] + A[2][d] * A[2][d]);
+        if (0.5 * w[d] < cmax) h->cell.multi = 1;
+      }
+      for (int d = 0; d < 3; d++)
+        if (h->cell.multi && (int)floor(lcut / w[d]) + 1 > NB_IMG_R)
+          throw std::runtime_error("cell too thin for image sums at this cutoff");
+      if (h->cell.multi) {
+        if (h->n >= (1 << NB_IMG_SHIFT)) throw std::runtime_error("too many atoms for image sums");
+        h->cell.kmask = (1u << NB_IMG_SHIFT) - 1u;
+      }
+    }
     h->cell_set = true;
     h->list_valid = false;
     if (h->prm.do_pme && !h->cell.aperiodic && h->prm.ewald_alpha > 0.0) nb_pme_setup(h);
@@ -643,6 +666,7 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   int status = NB_OK;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   h->vpart_used = 0;
+  h->used_sor = false;
   rebuilt = false;
   if (capture) {
     nb_check_rebuild_async(h);
@@ -674,7 +698,21 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
     if (polar) {
       nb_direct(h);
       if (h->prm.polarization == NB_POL_MUTUAL) {
-        status = capture ? nb_pcg_device(h, h->graph_iters) : nb_pcg(h, &e);
+        if (capture) {
+          status = nb_pcg_device(h, h->graph_iters);
+        } else if (h->prm.scf_algorithm == NB_SCF_SOR || h->ph.sor) {
+          status = nb_sor(h, h->prm.polar_sor, &e);
+          h->used_sor = true;
+        } else {
+          status = nb_pcg(h, &e);
+          if ((status == NB_ERR_SCF_DIVERGED || status == NB_ERR_SCF_MAXITER) && h->prm.sor_fallback) {
+            // ParticleMeshEwald.java:2300-2312: SOR from the direct dipoles with
+            // the conservative factor 0.65
+            nb_reset_dipoles(h);
+            status = nb_sor(h, 0.65, &e);
+            h->used_sor = true;
+          }
+        }
       } else if (use_recip) {
         // direct polarization: only the reciprocal potentials of u_direct are needed
         nb_pme_induced(h, h->d_u, h->d_up, true);
@@ -770,11 +808,13 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       // replay; a list that needs rebuilding, an overflowed filtered row or a
       // solve that did not stop within the recorded iterations sends the
       // evaluation down the regular path below
-      HIPCHECK(hipGraphLaunch(h->gexec, h->stream));
+      if (!h->graph_launched) HIPCHECK(hipGraphLaunch(h->gexec, h->stream));
+      h->graph_launched = false;
       HIPCHECK(hipStreamSynchronize(h->stream));
       memcpy(fl, h->h_pinned + 12, sizeof(fl));
       const double *sc = h->h_pinned + 40;
-      bool ok = fl[3] == 0 && !(h->prm.do_pme && nb_grow_filtered_rows(h, fl)) && (!mutual || sc[S_DONE] != 0.0);
+      bool ok = fl[3] == 0 && !(h->prm.do_pme && nb_grow_filtered_rows(h, fl)) &&
+                (!mutual || (sc[S_DONE] != 0.0 && sc[S_STAT] == 0.0));
       if (ok) {
         replayed = true;
         h->graph_replays++;
@@ -853,7 +893,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
     // record a graph for the next evaluations: the PCG budget is this
     // evaluation's iteration count plus one
-    if (h->graph_on && !replayed && status == NB_OK && !h->profiling)
+    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor)
       capture_graph(h, flags, mutual ? e.pcg_iters + 1 : 0);
     h->last = e;
     if (out) *out = e;
@@ -939,6 +979,35 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
   } catch (std::exception &ex) {
     return fail(h, ex, NB_ERR_HIP);
   }
+}
+
+// Evaluate several independent systems (one handle each) together: the
+// recorded graphs of all handles are launched first, each on its own stream,
+// so that the systems run concurrently, then each is completed (and falls
+// back to the regular path where its replay is not valid).  Handles without
+// a valid graph, and alchemical ones, are evaluated one after the other.
+int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energies *out) {
+  if (!handles || count < 0) return NB_ERR_INVALID;
+  std::vector<char> launched(count, 0);
+  int status = NB_OK;
+  for (int b = 0; b < count; b++) {
+    NbHandle *h = (NbHandle *)handles[b];
+    if (!h) return NB_ERR_INVALID;
+    if (h->elec_lambda || !h->graph_on || !h->gexec || h->graph_flags != flags || h->profiling || !h->cell_set ||
+        !h->positions_set)
+      continue;
+    if (hipSetDevice(h->device) != hipSuccess || hipGraphLaunch(h->gexec, h->stream) != hipSuccess) {
+      h->err = "graph launch failed";
+      return NB_ERR_HIP;
+    }
+    h->graph_launched = true;
+    launched[b] = 1;
+  }
+  for (int b = 0; b < count; b++) {
+    int st = nb_evaluate((void *)handles[b], flags, out ? out + b : nullptr);
+    if (st != NB_OK && status == NB_OK) status = st;
+  }
+  return status;
 }
 
 int nb_get_gradient(void *handle, double *g) {

@@ -19,13 +19,35 @@ struct CellD {
   double A[3][3];
   double vol;
   int aperiodic;
+  // Cells thinner than twice the cutoff use explicit image sums: a list entry
+  // then carries the lattice translation of its partner in bits 18..26 and
+  // the partner index in bits 0..17 (kmask); otherwise the minimum image.
+  int multi;
+  uint32_t kmask;
 };
+
+#define NB_IMG_SHIFT 18
+#define NB_IMG_R 3
+#define NB_IMG_W 7
+#define NB_IMG_ORIGIN (NB_IMG_R + NB_IMG_W * (NB_IMG_R + NB_IMG_W * NB_IMG_R))
+
+// lattice translation of a list entry's partner (image sums only)
+__device__ __forceinline__ void entry_shift(const CellD &c, uint32_t en, double &x, double &y, double &z) {
+  if (!c.multi) return;
+  int code = (int)((en >> NB_IMG_SHIFT) & 511u);
+  int tx = code % NB_IMG_W - NB_IMG_R, ty = (code / NB_IMG_W) % NB_IMG_W - NB_IMG_R;
+  int tz = code / (NB_IMG_W * NB_IMG_W) - NB_IMG_R;
+  x += tx * c.Ai00 + ty * c.Ai10 + tz * c.Ai20;
+  y += ty * c.Ai11 + tz * c.Ai21;
+  z += tz * c.Ai22;
+}
 
 __device__ __forceinline__ double round_half_away(double x) { return copysign(floor(fabs(x) + 0.5), x); }
 
-// minimum image with the reference's independent fractional rounding
+// minimum image with the reference's independent fractional rounding (image
+// sums: the displacement already holds its translation)
 __device__ __forceinline__ double image(const CellD &c, double &dx, double &dy, double &dz) {
-  if (c.aperiodic) return dx * dx + dy * dy + dz * dz;
+  if (c.aperiodic || c.multi) return dx * dx + dy * dy + dz * dz;
   double xf = dx * c.A00 + dy * c.A10 + dz * c.A20;
   double yf = dy * c.A11 + dz * c.A21;
   double zf = dz * c.A22;
@@ -84,6 +106,18 @@ __device__ __forceinline__ float nb_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double nb_rsqrt(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float nb_rsqrt(float x) { return rsqrtf(x); }
 
+// erfc(x) given e^{-x^2}: FP64 calls erfc; the FP32 form uses the rational
+// approximation of Abramowitz & Stegun, Handbook of Mathematical Functions,
+// eq. 7.1.26 (after Hastings), erfc(x) = t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
+// e^{-x^2} with t = 1 / (1 + p x), absolute error <= 1.5e-7 for x >= 0, which
+// reuses the exponential the Ewald recursion needs anyway.
+__device__ __forceinline__ double nb_erfc_ex(double x, double) { return erfc(x); }
+__device__ __forceinline__ float nb_erfc_ex(float x, float ex) {
+  float t = 1.0f / (1.0f + 0.3275911f * x);
+  float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+  return poly * ex;
+}
+
 // B[0..NMAX]: B0 = erfc(a r)/r, Bk = ((2k-1) B(k-1) + an(k-1) e^{-a^2 r^2}) / r^2
 template <typename T, int NMAX>
 __device__ __forceinline__ void ewald_bn(T r, T rinv, T aewald, T *B) {
@@ -93,7 +127,7 @@ __device__ __forceinline__ void ewald_bn(T r, T rinv, T aewald, T *B) {
     T ex = nb_exp(-ra * ra);
     T alsq2 = (T)2 * aewald * aewald;
     T an = alsq2 / ((T)NB_SQRT_PI * aewald);
-    B[0] = nb_erfc(ra) * rinv;
+    B[0] = nb_erfc_ex(ra, ex) * rinv;
 #pragma unroll
     for (int k = 1; k <= NMAX; k++) {
       B[k] = ((T)(2 * k - 1) * B[k - 1] + an * ex) * rr2;

@@ -90,23 +90,50 @@ __global__ void k_cell_scatter(int n, const int32_t *__restrict__ cellid, const 
   perm[start[c] + slot] = i;
 }
 
-// order each cell's segment by caller index (deterministic): one workgroup per
-// cell ranks every member by counting the members with a smaller index
+// Order each cell's segment by an 8x8x8 Morton key of the atom's place inside
+// the cell, then by caller index (deterministic): atoms that follow each other
+// in the sorted order are close in space, so the rows of the atoms one
+// workgroup handles share most of their partners in the caches.  One
+// workgroup per cell ranks every member by counting the smaller keys.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 7u;
+  return (v & 1u) | ((v & 2u) << 2) | ((v & 4u) << 4);
+}
+
 __global__ void __launch_bounds__(256) k_cell_rank(int m, const int32_t *__restrict__ start,
-                                                   const int32_t *__restrict__ unsorted, int32_t *__restrict__ perm) {
-  __shared__ int32_t mem[2048];
+                                                   const int32_t *__restrict__ unsorted, int32_t *__restrict__ perm,
+                                                   const double *__restrict__ xyz, CellD cell, int nc0, int nc1,
+                                                   int nc2) {
+  __shared__ unsigned long long mem[2048];
   int c = blockIdx.x;
   if (c >= m) return;
   int b = start[c], e = start[c + 1], cnt = e - b;
   bool in_lds = cnt <= 2048;
+  auto key_of = [&](int v) -> unsigned long long {
+    uint32_t mc = 0;
+    if (!cell.aperiodic) {
+      double x = xyz[3 * v], y = xyz[3 * v + 1], z = xyz[3 * v + 2];
+      double f[3] = {x * cell.A00 + y * cell.A10 + z * cell.A20, y * cell.A11 + z * cell.A21, z * cell.A22};
+      int ncs[3] = {nc0, nc1, nc2};
+      uint32_t sub[3];
+      for (int d = 0; d < 3; d++) {
+        double w = (f[d] - floor(f[d])) * ncs[d];
+        w -= floor(w);
+        int q = (int)(w * 8.0);
+        sub[d] = (uint32_t)(q < 0 ? 0 : (q > 7 ? 7 : q));
+      }
+      mc = spread3(sub[0]) | (spread3(sub[1]) << 1) | (spread3(sub[2]) << 2);
+    }
+    return ((unsigned long long)mc << 32) | (uint32_t)v;
+  };
   if (in_lds)
-    for (int t = threadIdx.x; t < cnt; t += blockDim.x) mem[t] = unsorted[b + t];
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) mem[t] = key_of(unsorted[b + t]);
   __syncthreads();
-  const int32_t *src = in_lds ? mem : unsorted + b;
   for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-    int v = src[t], rank = 0;
-    for (int j = 0; j < cnt; j++) rank += src[j] < v;
-    perm[b + rank] = v;
+    unsigned long long kv = in_lds ? mem[t] : key_of(unsorted[b + t]);
+    int rank = 0;
+    for (int j = 0; j < cnt; j++) rank += (in_lds ? mem[j] : key_of(unsorted[b + j])) < kv;
+    perm[b + rank] = (int32_t)(kv & 0xffffffffull);
   }
 }
 
@@ -198,6 +225,66 @@ __global__ void __launch_bounds__(256) k_build_list(int n, const double4 *__rest
   }
 }
 
+// Lattice translation that brings each atom into the cell at the list build
+// (image sums): W = floor(fractional coordinates), sorted order.
+__global__ void k_wrap_shift(int n, const int32_t *__restrict__ perm, const double *__restrict__ xyz, CellD cell,
+                             int32_t *__restrict__ wrap) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int i = perm[s];
+  double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  double f0 = x * cell.A00 + y * cell.A10 + z * cell.A20, f1 = y * cell.A11 + z * cell.A21, f2 = z * cell.A22;
+  wrap[3 * s] = (int)floor(f0);
+  wrap[3 * s + 1] = (int)floor(f1);
+  wrap[3 * s + 2] = (int)floor(f2);
+}
+
+// Full rows over atom images for cells thinner than twice the cutoff: every
+// partner t (the atom itself included) at every lattice translation T with
+// |x_t + T - x_s| <= cut, T in [-m, m] per axis (positions wrapped into the
+// cell, so fractional differences lie in (-1, 1)).  Bonded and group codes
+// apply only to the pair as given in the caller's coordinates, T = W_t - W_s.
+// One thread per atom scans partners and translations in a fixed order.
+__global__ void k_build_list_images(int n, const double4 *__restrict__ pos, const int32_t *__restrict__ perm,
+                                    const int32_t *__restrict__ wrap, CellD cell, double cut2, int m0, int m1, int m2,
+                                    const int32_t *__restrict__ group, const int32_t *__restrict__ moff,
+                                    const int32_t *__restrict__ matom, const int32_t *__restrict__ mrel,
+                                    uint32_t *__restrict__ nbr, int maxnb, int32_t *__restrict__ nnb,
+                                    int32_t *__restrict__ flags) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double4 pi = pos[s];
+  int wi0 = wrap[3 * s], wi1 = wrap[3 * s + 1], wi2 = wrap[3 * s + 2];
+  int i = perm[s], gi = group[s];
+  uint32_t *row = nbr + (size_t)s * maxnb;
+  int cnt = 0;
+  for (int t = 0; t < n; t++) {
+    double4 pk = pos[t];
+    double d0x = pk.x - pi.x, d0y = pk.y - pi.y, d0z = pk.z - pi.z;
+    int o0 = wrap[3 * t] - wi0, o1 = wrap[3 * t + 1] - wi1, o2 = wrap[3 * t + 2] - wi2;
+    int rel = -1;
+    for (int tz = -m2; tz <= m2; tz++)
+      for (int ty = -m1; ty <= m1; ty++)
+        for (int tx = -m0; tx <= m0; tx++) {
+          if (t == s && tx == 0 && ty == 0 && tz == 0) continue;
+          double dx = d0x + tx * cell.Ai00 + ty * cell.Ai10 + tz * cell.Ai20;
+          double dy = d0y + ty * cell.Ai11 + tz * cell.Ai21;
+          double dz = d0z + tz * cell.Ai22;
+          if (dx * dx + dy * dy + dz * dz > cut2) continue;
+          int code = 0;
+          if (tx == o0 && ty == o1 && tz == o2) {
+            if (rel < 0) rel = relation(moff, matom, mrel, i, perm[t]);
+            code = rel + (group[t] == gi ? 8 : 0);
+          }
+          uint32_t img = (uint32_t)((tx + NB_IMG_R) + NB_IMG_W * ((ty + NB_IMG_R) + NB_IMG_W * (tz + NB_IMG_R)));
+          if (cnt < maxnb) row[cnt] = (uint32_t)t | (img << NB_IMG_SHIFT) | ((uint32_t)code << NB_CODE_SHIFT);
+          cnt++;
+        }
+  }
+  nnb[s] = cnt;
+  if (cnt > maxnb) atomicMax(&flags[0], cnt);
+}
+
 }  // namespace
 
 static int estimate_neighbors(const NbHandle *h, double r) {
@@ -266,21 +353,44 @@ void nb_build_lists(NbHandle *h) {
   HIPCHECK(hipMemsetAsync(h->d_cellcount, 0, sizeof(int32_t) * (m + 1), h->stream));
   // unsorted cell segments go to iperm (rebuilt just after), then are ranked into perm
   NB_LAUNCH(k_cell_scatter, G, B, 0, n, h->d_cellid, h->d_cellstart, h->d_cellcount, h->d_iperm);
-  NB_LAUNCH(k_cell_rank, dim3(m), B, 0, m, h->d_cellstart, h->d_iperm, h->d_perm);
+  NB_LAUNCH(k_cell_rank, dim3(m), B, 0, m, h->d_cellstart, h->d_iperm, h->d_perm, h->d_xyz, h->cell, nc[0], nc[1],
+            nc[2]);
   NB_LAUNCH(k_iperm, G, B, 0, n, h->d_perm, h->d_iperm);
   NB_LAUNCH(k_gather_static, G, B, 0, n, h->d_perm, h->d_polar0, h->d_thole0, h->d_ipd0,
                      h->d_group0, h->d_vclass0, h->d_soft0, h->d_mol0, h->d_polar, h->d_group, h->d_vclass, h->d_soft,
                      h->d_mol, h->d_mp);
+  int mimg[3] = {0, 0, 0};
+  if (h->cell.multi) {
+    NB_LAUNCH(k_wrap_shift, G, B, 0, n, h->d_perm, h->d_xyz, h->cell, h->d_wrap);
+    for (int d = 0; d < 3; d++) {
+      const double(*A)[3] = h->cell.A;
+      double w = 1.0 / sqrt(A[0][d] * A[0][d] + A[1][d] * A[1][d] + A[2][d] * A[2][d]);
+      mimg[d] = (int)floor(cut / w) + 1;
+      if (mimg[d] > NB_IMG_R) throw std::runtime_error("cell too thin for image sums at this cutoff");
+    }
+  }
   nb_gather_sorted(h);
   // neighbor rows, grown on overflow
-  if (h->maxnb == 0) h->maxnb = estimate_neighbors(h, cut);
+  if (h->maxnb == 0) {
+    h->maxnb = estimate_neighbors(h, cut);
+    if (h->cell.multi) {
+      // every partner may appear at several translations
+      double est = h->n / h->cell.vol * 4.0 / 3.0 * M_PI * cut * cut * cut * 1.4 + 64.0;
+      h->maxnb = (int)std::min(est, (double)n * (2 * mimg[0] + 1) * (2 * mimg[1] + 1) * (2 * mimg[2] + 1));
+    }
+  }
   for (int attempt = 0; attempt < 4; attempt++) {
     if (!h->d_nbr) HIPCHECK(hipMalloc(&h->d_nbr, sizeof(uint32_t) * (size_t)h->maxnb * n));
     int32_t zeros[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(h->d_flags, zeros, sizeof(zeros), hipMemcpyHostToDevice, h->stream));
-    NB_LAUNCH(k_build_list<16>, grid_for((long)n * 16, 256), B, 0, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell, nc[0],
-                       nc[1], nc[2], cut * cut, h->d_group, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_nbr,
-                       h->maxnb, h->d_nnb, h->d_flags);
+    if (h->cell.multi)
+      NB_LAUNCH(k_build_list_images, grid_for(n, 64), dim3(64), 0, n, h->d_pos4, h->d_perm, h->d_wrap, h->cell,
+                cut * cut, mimg[0], mimg[1], mimg[2], h->d_group, h->d_mask_off, h->d_mask_atom, h->d_mask_rel,
+                h->d_nbr, h->maxnb, h->d_nnb, h->d_flags);
+    else
+      NB_LAUNCH(k_build_list<16>, grid_for((long)n * 16, 256), B, 0, n, h->d_pos4, h->d_perm, h->d_cellstart, h->cell,
+                nc[0], nc[1], nc[2], cut * cut, h->d_group, h->d_mask_off, h->d_mask_atom, h->d_mask_rel, h->d_nbr,
+                h->maxnb, h->d_nnb, h->d_flags);
     int32_t f0 = 0;
     HIPCHECK(hipMemcpyAsync(&f0, h->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));

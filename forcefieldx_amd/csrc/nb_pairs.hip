@@ -56,7 +56,8 @@ __global__ void __launch_bounds__(256) k_filter(int n, const double4 *__restrict
     uint32_t e = 0;
     if (t < cnt) {
       e = row[t];
-      double4 pk = pos[e & NB_INDEX_MASK];
+      double4 pk = pos[e & cell.kmask];
+      entry_shift(cell, e, pk.x, pk.y, pk.z);
       double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
       double r2 = image(cell, dx, dy, dz);
       ke = r2 <= off2;
@@ -136,9 +137,11 @@ __global__ void __launch_bounds__(256) k_perm_field(int n, const double4 *__rest
     const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
       uint32_t e = row[t];
-      int k = e & NB_INDEX_MASK, code = e >> NB_CODE_SHIFT;
+      int k = e & cell.kmask, code = e >> NB_CODE_SHIFT;
       if (LIST == 1 && sc.masked[code]) continue;
-      perm_field_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, fd, fp);
+      double4 pk = pos[k];
+      entry_shift(cell, e, pk.x, pk.y, pk.z);
+      perm_field_pair<T>(pi, ai, pk, mp[k], code, cell, aewald, sc, fd, fp);
     }
   }
   group_sum3<LPA>(fd);
@@ -171,9 +174,11 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
     int m = cnt[s];
     const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
-      int k = row[t] & NB_INDEX_MASK;
+      uint32_t e = row[t];
+      int k = e & cell.kmask;
       const DipRec<T> rk = rec[k];
       double dx = rk.x - ri.x, dy = rk.y - ri.y, dz = rk.z - ri.z;
+      entry_shift(cell, e, dx, dy, dz);
       double r2 = image(cell, dx, dy, dz);
       T r = sqrt((T)r2), rinv = (T)1 / r;
       V3<T> R{(T)dx, (T)dy, (T)dz};
@@ -383,8 +388,10 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
     const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
       uint32_t en = row[t];
-      int k = en & NB_INDEX_MASK, code = en >> NB_CODE_SHIFT;
+      int k = en & cell.kmask, code = en >> NB_CODE_SHIFT;
       if (LIST == 1 && sc.masked[code]) continue;
+      double4 pk = pos[k];
+      entry_shift(cell, en, pk.x, pk.y, pk.z);
       if (OST) {
         int sk = soft ? soft[k] : 0;
         if (ph.vapor && si && sk) {
@@ -395,14 +402,14 @@ __global__ void __launch_bounds__(256) k_real_energy(int n, const double4 *__res
         }
         bool sp = ph.soft_pairs && (si || sk);
         if (POL)
-          pol_energy_pair_ost<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, ph, e, F, Tq, wp,
+          pol_energy_pair_ost<T>(pi, ai, u, up, s, pk, mp[k], k, code, cell, aewald, sc, direct, ph, e, F, Tq, wp,
                                  dl1, dl2, FL, TL);
         else
-          perm_energy_pair_ost<T>(pi, ai, pos[k], mp[k], code, sp, cell, aewald, sc, ph, e, cnt, F, Tq, wp, dl1, dl2,
+          perm_energy_pair_ost<T>(pi, ai, pk, mp[k], code, sp, cell, aewald, sc, ph, e, cnt, F, Tq, wp, dl1, dl2,
                                   FL, TL);
       } else {
-        if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pos[k], mp[k], k, code, cell, aewald, sc, direct, e, F, Tq, wp);
-        else perm_energy_pair<T>(pi, ai, pos[k], mp[k], code, cell, aewald, sc, e, cnt, F, Tq, wp);
+        if (POL) pol_energy_pair<T>(pi, ai, u, up, s, pk, mp[k], k, code, cell, aewald, sc, direct, e, F, Tq, wp);
+        else perm_energy_pair<T>(pi, ai, pk, mp[k], code, cell, aewald, sc, e, cnt, F, Tq, wp);
       }
     }
   }
@@ -467,10 +474,11 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
     const uint32_t *row = nbr + (size_t)s * maxnb;
     for (int t = lane; t < m; t += LPA) {
       uint32_t en = row[t];
-      int k = en & NB_INDEX_MASK, code = en >> NB_CODE_SHIFT;
+      int k = en & cell.kmask, code = en >> NB_CODE_SHIFT;
       double vs = sc.v[code];
       if (!(vs > 0.0)) continue;
       double4 pk = red[k];
+      entry_shift(cell, en, pk.x, pk.y, pk.z);
       double dx = pi.x - pk.x, dy = pi.y - pk.y, dz = pi.z - pk.z;
       double r2 = image(cell, dx, dy, dz);
       if (r2 > off2) continue;

@@ -49,7 +49,7 @@ __global__ void k_rotate(int n, const double *__restrict__ xyz, const int32_t *_
                          const int32_t *__restrict__ axis, const double *__restrict__ mloc,
                          const int32_t *__restrict__ iperm, const int32_t *__restrict__ redidx,
                          const double *__restrict__ redval, AtomMP *__restrict__ mp, double *__restrict__ gmp,
-                         double4 *__restrict__ red4) {
+                         double4 *__restrict__ red4, CellD cell, const int32_t *__restrict__ wrap) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3<double> o = ld3(xyz, i);
@@ -114,6 +114,12 @@ __global__ void k_rotate(int n, const double *__restrict__ xyz, const int32_t *_
     rs.x = rv * (o.x - ph.x) + ph.x;
     rs.y = rv * (o.y - ph.y) + ph.y;
     rs.z = rv * (o.z - ph.z) + ph.z;
+  }
+  if (cell.multi) {
+    int wx = wrap[3 * s], wy = wrap[3 * s + 1], wz = wrap[3 * s + 2];
+    rs.x -= wx * cell.Ai00 + wy * cell.Ai10 + wz * cell.Ai20;
+    rs.y -= wy * cell.Ai11 + wz * cell.Ai21;
+    rs.z -= wz * cell.Ai22;
   }
   red4[s] = rs;
 }
@@ -1240,7 +1246,7 @@ void nb_rotate(NbHandle *h) {
   int n = h->n;
   dim3 B(128), G((n + 127) / 128);
   NB_LAUNCH(k_rotate, G, B, 0, n, h->d_xyz, h->d_frame, h->d_axis, h->d_mpole_local, h->d_iperm,
-                     h->d_redidx, h->d_redval, h->d_mp, h->d_gmp, h->d_red4);
+                     h->d_redidx, h->d_redval, h->d_mp, h->d_gmp, h->d_red4, h->cell, (const int32_t *)h->d_wrap);
 }
 
 // standalone convolution entry used by the FFT parity test (precision:

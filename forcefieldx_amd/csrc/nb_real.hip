@@ -28,11 +28,19 @@ namespace {
 
 
 __global__ void k_gather_pos(int n, const int32_t *__restrict__ perm, const double *__restrict__ xyz,
-                             const AtomMP *__restrict__ mp, double4 *__restrict__ pos, void *rec, int mixed) {
+                             const AtomMP *__restrict__ mp, double4 *__restrict__ pos, void *rec, int mixed,
+                             CellD cell, const int32_t *__restrict__ wrap) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
   double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  if (cell.multi) {
+    // the lattice translation fixed at the last list build (image sums)
+    int wx = wrap[3 * s], wy = wrap[3 * s + 1], wz = wrap[3 * s + 2];
+    x -= wx * cell.Ai00 + wy * cell.Ai10 + wz * cell.Ai20;
+    y -= wy * cell.Ai11 + wz * cell.Ai21;
+    z -= wz * cell.Ai22;
+  }
   pos[s] = make_double4(x, y, z, 0.0);
   dip_put_static(rec, mixed, s, x, y, z, mp[s].ipd, mp[s].thole);
 }
@@ -275,6 +283,40 @@ __global__ void k_pcg_ctl_init(double *__restrict__ scal) {
     scal[S_STAT] = 0.0;
     scal[S_EPSV] = 0.0;
   }
+}
+
+// SOR step (SORRegion.java:236-258): u <- u + w (u_direct + polar E(u) - u)
+// for u and uCR; the packed records receive the new dipoles; partial sums of
+// the squared steps go to the residual slots
+__global__ void __launch_bounds__(256) k_sor_update(int n, double w, const double *__restrict__ polar,
+                                                    const double *__restrict__ ud, const double *__restrict__ upd,
+                                                    const double *__restrict__ f, const double *__restrict__ fp,
+                                                    double *__restrict__ u, double *__restrict__ up,
+                                                    double *__restrict__ part, int stride, void *rec, int mixed) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (q < 3 * n) {
+    double a = polar[q / 3];
+    double du = w * (ud[q] + a * f[q] - u[q]), dv = w * (upd[q] + a * fp[q] - up[q]);
+    double nu = u[q] + du, nv = up[q] + dv;
+    u[q] = nu;
+    up[q] = nv;
+    dip_put(rec, mixed, q / 3, q % 3, nu, nv);
+    v[0] = du * du;
+    v[1] = dv * dv;
+  }
+  const int out[2] = {S_EPS, S_EPSP};
+  block_partials<2>(v, part, out, stride);
+}
+
+// u = u_direct (and the packed records), the start of an SOR solve
+__global__ void k_reset_dipoles(int n, const double *__restrict__ ud, const double *__restrict__ upd,
+                                double *__restrict__ u, double *__restrict__ up, void *rec, int mixed) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 3 * n) return;
+  u[q] = ud[q];
+  up[q] = upd[q];
+  dip_put(rec, mixed, q / 3, q % 3, ud[q], upd[q]);
 }
 
 // p = z + beta p; the packed records receive p
@@ -623,7 +665,7 @@ static int mixed_of(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXE
 
 void nb_gather_sorted(NbHandle *h) {
   NB_LAUNCH(k_gather_pos, grid_for(h->n, 256), dim3(256), 0, h->n, h->d_perm, h->d_xyz, h->d_mp, h->d_pos4,
-            (void *)h->d_dip, mixed_of(h));
+            (void *)h->d_dip, mixed_of(h), h->cell, (const int32_t *)h->d_wrap);
 }
 
 // one launch clears the accumulated outputs of an evaluation
@@ -796,6 +838,47 @@ int nb_pcg_device(NbHandle *h, int iters) {
   h->ctl = nullptr;
   if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
   return NB_OK;
+}
+
+void nb_reset_dipoles(NbHandle *h) {
+  int n = h->n;
+  NB_LAUNCH(k_reset_dipoles, grid_for(3 * n, 256), dim3(256), 0, n, h->d_udir, h->d_updir, h->d_u, h->d_up,
+            (void *)h->d_dip, mixed_of(h));
+}
+
+// SOR solve (ParticleMeshEwald.scfBySOR, ParticleMeshEwald.java:2331-2423) from
+// the current dipoles: eps = 4.80321 sqrt(max(sum du^2, sum duCR^2) / n); a rise
+// fails, 1000 cycles are the limit, eps < polar_eps converges.
+int nb_sor(NbHandle *h, double w, nb_energies *out) {
+  int n = h->n;
+  dim3 B(256), G = grid_for(3 * n, 256);
+  const int nb = (int)G.x, stride = h->part_stride;
+  if (nb > stride) throw std::runtime_error("PCG partial-sum buffer too small");
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double *f = h->d_field, *fp = h->d_fieldp;
+  void *rec = (void *)h->d_dip;
+  int mx = mixed_of(h);
+  double eps = 100.0, prev;
+  int iter = 0, status = NB_OK;
+  for (;;) {
+    nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
+    NB_LAUNCH(k_sor_update, G, B, 0, n, w, h->d_polar, h->d_udir, h->d_updir, f, fp, h->d_u, h->d_up, h->d_part,
+              stride, rec, mx);
+    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, h->d_part, stride, nb, h->d_scal, 0, n, 0.0, 0);
+    HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    prev = eps;
+    double e = h->h_pinned[S_EPS] > h->h_pinned[S_EPSP] ? h->h_pinned[S_EPS] : h->h_pinned[S_EPSP];
+    iter++;
+    eps = 4.80321 * sqrt(e / n);
+    if (eps > prev) { status = NB_ERR_SCF_DIVERGED; break; }
+    if (iter >= 1000) { status = NB_ERR_SCF_MAXITER; break; }
+    if (eps < h->prm.polar_eps) break;
+  }
+  out->pcg_iters = iter;
+  out->pcg_eps = eps;
+  if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
+  return status;
 }
 
 void nb_recip_energy(NbHandle *h, bool polar) {

@@ -36,6 +36,7 @@ struct MaskScales { double m[32], p[32], d[32], v[32]; };
 struct ElecPhase {
   int perm_real = 1;     // permanent real-space energy on
   int polar = 1;         // polarization on
+  int sor = 0;           // solve the dipoles by SOR (the vapor phase, ParticleMeshEwald.java:1744-1746)
   int soft_pairs = 0;    // pairs with a soft atom use l2 and the softcore shift
   int vapor = 0;         // leave out both-soft pairs per the inter / intramolecular flags
   int inter = 0, intra = 0;
@@ -104,6 +105,7 @@ struct NbHandle {
 
   // sorted per-atom data
   double4 *d_pos4 = nullptr;
+  int32_t *d_wrap = nullptr;      // [3n] lattice translation subtracted from each sorted atom (image sums)
   double4 *d_red4 = nullptr;      // reduced vdW sites
   AtomMP *d_mp = nullptr;
   double *d_gmp = nullptr;        // [10n] global multipoles (full Q), sorted
@@ -180,6 +182,8 @@ struct NbHandle {
   uint32_t graph_flags = 0;
   int graph_iters = 0;
   long graph_replays = 0;
+  bool graph_launched = false;    // the graph of this evaluation was launched by nb_evaluate_batch
+  bool used_sor = false;          // the last evaluation's dipoles came from SOR (not recorded)
   const double *ctl = nullptr;    // PCG control slots while the captured solve is being recorded
 
   // profiling
@@ -209,6 +213,8 @@ void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f,
 void nb_precond_field(NbHandle *h, double *f, double *fp);
 int nb_pcg(NbHandle *h, nb_energies *out);
 int nb_pcg_device(NbHandle *h, int iters);
+int nb_sor(NbHandle *h, double w, nb_energies *out);
+void nb_reset_dipoles(NbHandle *h);
 void nb_check_rebuild_async(NbHandle *h);
 void nb_recip_energy(NbHandle *h, bool polar);
 void nb_real_energy(NbHandle *h, bool polar);

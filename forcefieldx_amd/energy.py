@@ -92,7 +92,7 @@ class ForceFieldEnergy:
     """Nonbonded part of ForceFieldEnergy on the MI355X library."""
 
     def __init__(self, topology, box, params: NonbondedParams | None = None, precision=None, device=0,
-                 direct_fallback=True):
+                 direct_fallback=None):
         self.params = params or NonbondedParams()
         self.topology = topology
         self.box = np.asarray(box if box is not None else [0, 0, 0, 90, 90, 90], float)
@@ -105,7 +105,9 @@ class ForceFieldEnergy:
         self._last = None
         self._lambda = None     # set by setLambda: evaluations then carry dU/dL
         self._fallback_grad = self._fallback_dudx = None
-        self.direct_fallback = direct_fallback
+        # ParticleMeshEwald.java:563-565: direct dipoles after a failed SCF only
+        # when the SOR fallback (done inside the library) is off
+        self.direct_fallback = (not self.params.sor_fallback) if direct_fallback is None else direct_fallback
         self.vanDerWaals = VanDerWaals(self)
         self.particleMeshEwald = ParticleMeshEwald(self)
 

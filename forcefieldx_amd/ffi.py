@@ -24,7 +24,7 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
-            "nb_set_graph", "nb_graph_replays"]
+            "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -46,7 +46,8 @@ class NbParams(C.Structure):
                 ("perm_lambda_start", C.c_double), ("perm_lambda_end", C.c_double),
                 ("pol_lambda_exponent", C.c_double), ("pol_lambda_start", C.c_double),
                 ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int32),
-                ("no_ligand_condensed_scf", C.c_int32)]
+                ("no_ligand_condensed_scf", C.c_int32), ("scf_algorithm", C.c_int32), ("polar_sor", C.c_double),
+                ("sor_fallback", C.c_int32)]
 
 
 class NbTopology(C.Structure):
@@ -87,6 +88,7 @@ def load(path=LIB_PATH):
     L.nb_set_positions_device.argtypes = [vp, vp]
     L.nb_set_lambda.argtypes = [vp, C.c_double]
     L.nb_set_graph.argtypes = [vp, C.c_int]
+    L.nb_evaluate_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(NbEnergies)]
     L.nb_graph_replays.argtypes = [vp]
     L.nb_struct_sizes.argtypes = [_ip]
     L.nb_struct_sizes.restype = C.c_int
@@ -253,6 +255,15 @@ class Handle:
             pass
 
 
+def evaluate_batch(handles, flags=NB_GRAD):
+    """nb_evaluate_batch over Handle objects; returns (list of energy dicts, rc)."""
+    L = load()
+    arr = (C.c_void_p * len(handles))(*[h.h for h in handles])
+    outs = (NbEnergies * len(handles))()
+    rc = L.nb_evaluate_batch(arr, len(handles), flags, outs)
+    return [o.as_dict() for o in outs], rc
+
+
 def params_from(p, grid, precision=None, device=0):
     """NbParams from forcefieldx_amd.forcefield.NonbondedParams."""
     prec = p.precision if precision is None else precision
@@ -265,7 +276,7 @@ def params_from(p, grid, precision=None, device=0):
                     int(p.intramolecular_softcore), int(p.elec_lambda), p.perm_lambda_alpha,
                     p.perm_lambda_exponent, p.perm_lambda_start, p.perm_lambda_end, p.pol_lambda_exponent,
                     p.pol_lambda_start, p.pol_lambda_end, int(p.ligand_vapor_elec),
-                    int(p.no_ligand_condensed_scf))
+                    int(p.no_ligand_condensed_scf), int(p.scf_algorithm), p.polar_sor, int(p.sor_fallback))
 
 
 def device_count():

@@ -376,8 +376,8 @@ def small_crystal(n_target, box, seed=0, min_dist=2.6, sizes=(3, 4, 5), tries=40
     rng = np.random.Generator(np.random.PCG64(seed))
     box = np.asarray(box, float)
     L = lattice_vectors(box)
-    img = np.array([(i, j, k) for i in (-2, -1, 0, 1, 2) for j in (-2, -1, 0, 1, 2) for k in (-2, -1, 0, 1, 2)],
-                   float) @ L
+    # neighbouring translations suffice for a spacing below every cell width
+    img = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], float) @ L
     placed = np.zeros((0, 3))
     mols = []
     fails = 0
@@ -443,3 +443,40 @@ def small_crystal(n_target, box, seed=0, min_dist=2.6, sizes=(3, 4, 5), tries=40
                    vdw_table=tab, red_index=np.arange(n, dtype=np.int32), red_value=np.zeros(n), mask_off=off,
                    mask_atom=atoms, mask_rel=rels)
     return placed.reshape(-1), box, top
+
+
+def _smooth_at_least(x, lo=16, hi=32):
+    """Smallest 2,3,5-smooth integer >= x, clamped to [lo, hi]."""
+    n = max(lo, int(np.ceil(x)))
+    while True:
+        m = n
+        for f in (2, 3, 5):
+            while m % f == 0:
+                m //= f
+        if m == 1 or n >= hi:
+            return min(n, hi) if m == 1 or n <= hi else hi
+        n += 1
+
+
+def crystal_batch(count, seed=0):
+    """The batched small-crystal workload: ``count`` independent P1 triclinic
+    cells with lengths U(8, 20) A and angles U(70, 110) deg (cells of
+    non-positive volume redrawn), each filled by small_crystal with U(30, 300)
+    atoms (fewer where the cell is full at 2.6 A spacing).  Per-cell PME grids
+    are the next 2,3,5-smooth size >= 1.2 L, clamped to 16..32.  Returns a list
+    of (xyz, box, Topology, grid)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for c in range(count):
+        while True:
+            box = np.concatenate([rng.uniform(8.0, 20.0, 3), rng.uniform(70.0, 110.0, 3)])
+            ca, cb, cg = np.cos(np.deg2rad(box[3:]))
+            v2 = 1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg
+            if v2 > 0.05:
+                break
+        vol = np.prod(box[:3]) * np.sqrt(v2)
+        n_target = int(min(rng.uniform(30, 300), 0.045 * vol))
+        xyz, box, top = small_crystal(max(n_target, 12), box, seed=int(rng.integers(1 << 30)), tries=300)
+        grid = tuple(_smooth_at_least(1.2 * L) for L in box[:3])
+        out.append((xyz, box, top, grid))
+    return out

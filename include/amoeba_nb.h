@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 3
+#define NB_ABI_VERSION 4
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -51,6 +51,9 @@ enum { NB_PREC_FP64 = 0, NB_PREC_MIXED = 1 };
 
 /* nb_evaluate flags */
 enum { NB_GRAD = 1u, NB_VIRIAL = 2u, NB_DUDL = 4u, NB_FINITE_PCG = 8u };
+
+/* SCF algorithms (SCFAlgorithm.java) */
+enum { NB_SCF_PCG = 0, NB_SCF_SOR = 1 };
 
 typedef struct {
   double ewald_cutoff;      /* real-space Ewald cutoff, A (EwaldParameters.java:55) */
@@ -95,6 +98,10 @@ typedef struct {
   double pol_lambda_start, pol_lambda_end; /* polarization window (0.75, 1) */
   int32_t ligand_vapor_elec; /* vapor phase of the soft atoms (1) */
   int32_t no_ligand_condensed_scf; /* polarization of the no-ligand phase (1) */
+  /* SCF algorithm (ParticleMeshEwald.java:2247-2325) */
+  int32_t scf_algorithm;    /* NB_SCF_PCG or NB_SCF_SOR */
+  double polar_sor;         /* SOR factor, polar-sor (0.7; SORRegion.java:65-72) */
+  int32_t sor_fallback;     /* SOR with factor 0.65 after a failed PCG (1; ParticleMeshEwald.java:563) */
 } nb_params;
 
 typedef struct {
@@ -189,6 +196,13 @@ void nb_destroy(void *handle);
  * recorded again.  nb_set_cell, nb_set_lambda and list rebuilds drop the graph.
  * Results equal those of the regular path. */
 int nb_set_graph(void *handle, int on);
+
+/* Evaluate count independent systems, one handle each, with the same flags
+ * (no counterpart in the reference, which evaluates one system at a time):
+ * handles in graph mode with a recorded graph are launched together, each on
+ * its own stream, then completed; the others are evaluated in turn.  out
+ * (may be NULL) receives count records.  Returns the first non-OK status. */
+int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energies *out);
 
 /* Number of evaluations of this handle served by a graph replay. */
 int nb_graph_replays(void *handle);
