@@ -777,6 +777,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
   nb_energies e;
   memset(&e, 0, sizeof(e));
   int status = NB_OK;
+  h->induced_caller = false;
   try {
     HIPCHECK(hipSetDevice(h->device));
     int n = h->n;
@@ -981,18 +982,44 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
   }
 }
 
-// Evaluate several independent systems (one handle each) together: the
-// recorded graphs of all handles are launched first, each on its own stream,
-// so that the systems run concurrently, then each is completed (and falls
-// back to the regular path where its replay is not valid).  Handles without
-// a valid graph, and alchemical ones, are evaluated one after the other.
+// Evaluate several independent systems (one handle each) together.  When
+// every handle qualifies (small periodic systems with the same parameters,
+// no alchemistry, virial or SOR), one persistent launch of the batch kernel
+// evaluates them all; a handle whose solve failed there is evaluated again on
+// the regular path, which applies the SOR fallback.  Otherwise, or with
+// NB_BATCH_SEPARATE, the recorded graphs of the handles are launched
+// together, each on its own stream, then each is completed (falling back to
+// the regular path where its replay is not valid); handles without a valid
+// graph, and alchemical ones, are evaluated one after the other.
 int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energies *out) {
   if (!handles || count < 0) return NB_ERR_INVALID;
-  std::vector<char> launched(count, 0);
+  for (int b = 0; b < count; b++)
+    if (!handles[b]) return NB_ERR_INVALID;
+  bool separate_all = (flags & NB_BATCH_SEPARATE) != 0;
+  flags &= ~(uint32_t)NB_BATCH_SEPARATE;
   int status = NB_OK;
+  if (!separate_all && count > 0) {
+    std::vector<char> again;
+    int st = NB_OK;
+    bool done;
+    try {
+      done = nb_batch_evaluate((NbHandle **)handles, count, flags, out, again, st);
+    } catch (std::exception &ex) {
+      return fail((NbHandle *)handles[0], ex, NB_ERR_HIP);
+    }
+    if (done) {
+      status = st;
+      for (int b = 0; b < count; b++) {
+        if (!again[b]) continue;
+        int s2 = nb_evaluate((void *)handles[b], flags, out ? out + b : nullptr);
+        if (s2 != NB_OK && status == NB_OK) status = s2;
+      }
+      return status;
+    }
+  }
+  std::vector<char> launched(count, 0);
   for (int b = 0; b < count; b++) {
     NbHandle *h = (NbHandle *)handles[b];
-    if (!h) return NB_ERR_INVALID;
     if (h->elec_lambda || !h->graph_on || !h->gexec || h->graph_flags != flags || h->profiling || !h->cell_set ||
         !h->positions_set)
       continue;
@@ -1008,6 +1035,15 @@ int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energi
     if (st != NB_OK && status == NB_OK) status = st;
   }
   return status;
+}
+
+int nb_batch_config(int device, int32_t cfg[4]) {
+  if (!cfg) return NB_ERR_INVALID;
+  try {
+    return nb_batch_last_config(device, cfg);
+  } catch (std::exception &) {
+    return NB_ERR_HIP;
+  }
 }
 
 int nb_get_gradient(void *handle, double *g) {
@@ -1038,6 +1074,8 @@ int nb_get_induced(void *handle, double *u, double *ucr) {
     HIPCHECK(hipMemcpyAsync(ups.data(), h->d_up, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipMemcpyAsync(perm.data(), h->d_perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
+    if (h->induced_caller)
+      for (int s = 0; s < n; s++) perm[s] = s;   // the batch kernel works in caller order
     for (int s = 0; s < n; s++) {
       int i = perm[s];
       for (int d = 0; d < 3; d++) {

@@ -1,0 +1,1382 @@
+// nb_batch.hip -- many small periodic systems evaluated in one persistent
+// launch (gfx950).
+//
+// A workgroup takes one system at a time from a queue (largest first) and
+// carries it through the whole evaluation with workgroup barriers between the
+// phases, so a batch of hundreds of cells costs one launch instead of a few
+// hundred small launches per cell:
+//
+//   atoms   local -> global multipoles (MultipoleType.java:385-540, 1031-1058),
+//           reduced vdW sites (VanDerWaals.java:1510-1540), positions wrapped
+//           into the cell, order-5 B-splines and fractional multipoles
+//           (ReciprocalSpace.java:1344-1397, 2778-2834);
+//   rows    every partner at every lattice translation within the cutoff
+//           (explicit image sums, as ReplicatesCrystal provides for thin
+//           cells; masks only for the pair as given), split into the
+//           preconditioner part (r < 4.5 A) and the rest
+//           (PermanentFieldRegion.java:679-714);
+//   field   permanent real-space field (PermanentFieldRegion.java:704-808)
+//           and the permanent PME potential (spread with atomics into the
+//           system's grid, in-LDS line FFTs with the influence multiply fused,
+//           gather; ReciprocalSpace.java:1445-1505, Complex3DParallel.java:
+//           445-453, ReciprocalSpace.java:2264-2455);
+//   solve   direct dipoles (DirectRegion.java:170-245) and the PCG
+//           (PCGSolver.java:487-1178) with workgroup reductions, the
+//           convergence test of the reference (:543-547) taken in the kernel;
+//   energy  real-space permanent and polarization energy / force / torque
+//           (RealSpaceEnergyRegion.java:803-2177), buffered 14-7 vdW
+//           (VanDerWaals.java:1638-1900), reciprocal energy / gradient /
+//           torque (ReciprocalEnergyRegion.java:381-746), torques to
+//           frame-atom gradients and the reduced-site chain rule
+//           (Torque.java:111-316, VanDerWaals.java:1826-1833).
+//
+// The per-pair and per-atom arithmetic is shared with the per-system kernels
+// (nb_pair.h, nb_atom.h, nb_fft.h); T is double (FP64 form) or float (mixed
+// form: FP32 pair, spread and FFT arithmetic, FP64 accumulation, masked pairs
+// of the permanent terms in FP64).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <stdexcept>
+#include <vector>
+
+#include "nb_kern.h"
+#include "nb_pair.h"
+#include "nb_fft.h"
+
+namespace {
+
+struct BatchSys {
+  int n, K0, K1, K2;
+  int maxa, maxb;           // row capacities: preconditioner part, the rest
+  int Lx, Ly, Lz;           // FFT lines per chunk along x, y, z
+  double lcut;              // list cutoff the image range was checked for
+  Radices rad[3];
+  CellD cell;               // image sums (multi = 1)
+  Frac fr;
+  const double *xyz, *mloc, *polar, *thole, *ipd, *redval, *vtab;
+  const int32_t *frame, *axis, *group, *vclass, *redidx, *moff, *matom, *mrel;
+  int nclass;
+  const void *recip, *tw;   // influence function and twiddles in the grid type
+  double *grad, *u, *up;    // outputs (caller order)
+  char *ws;                 // workspace
+};
+
+struct BatchPrm {
+  ScaleTab sc;
+  double aewald, ewald_cut, vdw_off, vdw_cut, pcut, psc, polar_eps, electric, hdelta, hgamma;
+  int polarization, maxit, do_vdw, do_pme, recip;
+};
+
+// per-system workspace
+template <typename T> struct Ws {
+  double4 *pos, *red;
+  AtomMP *mp;
+  double *gmp, *theta, *fmp, *phip, *phiu, *phiup;
+  double *f, *fp, *ud, *upd, *r, *rp, *z, *zp, *p, *pp, *vec, *vecp, *gacc, *tacc, *gred;
+  int32_t *igrid, *wrap, *na, *nb;
+  uint32_t *ra, *rb;
+  DipRec<T> *rec;
+  CX<T> *grid;
+};
+
+template <typename T>
+__host__ __device__ size_t ws_map(char *base, int n, int maxa, int maxb, long ntot, Ws<T> *w) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> char * {
+    char *q = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  size_t d3 = sizeof(double) * 3 * (size_t)n;
+  Ws<T> v;
+  v.pos = (double4 *)take(sizeof(double4) * n);
+  v.red = (double4 *)take(sizeof(double4) * n);
+  v.mp = (AtomMP *)take(sizeof(AtomMP) * n);
+  v.gmp = (double *)take(sizeof(double) * 10 * n);
+  v.theta = (double *)take(sizeof(double) * 60 * n);
+  v.fmp = (double *)take(sizeof(double) * 10 * n);
+  v.phip = (double *)take(sizeof(double) * 30 * n);
+  v.phiu = (double *)take(sizeof(double) * 30 * n);
+  v.phiup = (double *)take(sizeof(double) * 30 * n);
+  v.f = (double *)take(d3);
+  v.fp = (double *)take(d3);
+  v.ud = (double *)take(d3);
+  v.upd = (double *)take(d3);
+  v.r = (double *)take(d3);
+  v.rp = (double *)take(d3);
+  v.z = (double *)take(d3);
+  v.zp = (double *)take(d3);
+  v.p = (double *)take(d3);
+  v.pp = (double *)take(d3);
+  v.vec = (double *)take(d3);
+  v.vecp = (double *)take(d3);
+  v.gacc = (double *)take(d3);
+  v.tacc = (double *)take(d3);
+  v.gred = (double *)take(d3);
+  v.igrid = (int32_t *)take(sizeof(int32_t) * 3 * n);
+  v.wrap = (int32_t *)take(sizeof(int32_t) * 3 * n);
+  v.na = (int32_t *)take(sizeof(int32_t) * n);
+  v.nb = (int32_t *)take(sizeof(int32_t) * n);
+  v.ra = (uint32_t *)take(sizeof(uint32_t) * (size_t)n * maxa);
+  v.rb = (uint32_t *)take(sizeof(uint32_t) * (size_t)n * maxb);
+  v.rec = (DipRec<T> *)take(sizeof(DipRec<T>) * n);
+  v.grid = (CX<T> *)take(sizeof(CX<T>) * (size_t)ntot);
+  if (w) *w = v;
+  return off;
+}
+
+// per-system outputs: energy slots E_PERM_REAL .. E_QTOT, then these
+enum { O_ITER = 8, O_EPS, O_STAT, O_NVDW, O_NPERM, O_FLAGS, O_ROWA, O_ROWB, O_N };
+// O_FLAGS bits
+enum { F_ROWS = 1, F_TORQUE = 2, F_CUTOFF = 4 };
+
+constexpr int FFT_CAP = 2048;   // LDS line-buffer elements per workgroup
+constexpr int TW_CAP = 3 * 64;  // LDS twiddle entries (three axes of at most 64)
+
+// sums of NV values over the workgroup, returned to every thread (fixed order)
+template <int NT, int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double *red) {
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum_d(v[k]);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k * NW + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int q = 0; q < NW; q++) s += red[k * NW + q];
+    v[k] = s;
+  }
+}
+
+// maxima of NV values over the workgroup, returned to every thread
+template <int NT, int NV>
+__device__ __forceinline__ void block_max(double (&v)[NV], double *red) {
+  constexpr int NW = NT / 64;
+#pragma unroll
+  for (int k = 0; k < NV; k++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v[k] = fmax(v[k], __shfl_xor(v[k], o, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k * NW + w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double m = red[k * NW];
+    for (int q = 1; q < NW; q++) m = fmax(m, red[k * NW + q]);
+    v[k] = m;
+  }
+}
+
+// Phases read what other threads of the workgroup wrote to global memory,
+// partly through atomics performed in L2: the barrier is completed by a
+// device-scope fence.
+__device__ __forceinline__ void phase_sync() {
+  __threadfence();
+  __syncthreads();
+}
+
+// exclusive prefix over the lower lanes of an LPA group
+template <int LPA>
+__device__ __forceinline__ int group_excl_scan(int v) {
+  int lane = threadIdx.x % LPA, x = v;
+#pragma unroll
+  for (int o = 1; o < LPA; o <<= 1) {
+    int y = __shfl_up(x, o, LPA);
+    if (lane >= o) x += y;
+  }
+  return x - v;
+}
+
+// row entry t of atom i: the preconditioner part first, then the rest
+template <typename T>
+__device__ __forceinline__ uint32_t row_entry(const BatchSys &S, const Ws<T> &w, int i, int na, int t) {
+  return t < na ? w.ra[(size_t)i * S.maxa + t] : w.rb[(size_t)i * S.maxb + (t - na)];
+}
+
+// ---------------------------------------------------------------------------
+// atoms: frames, multipoles, reduced sites, wrapped positions, B-splines.
+// Returns the largest reduced-site displacement among this thread's atoms.
+// ---------------------------------------------------------------------------
+template <typename T, int NT>
+__device__ double phase_atoms(const BatchSys &S, const BatchPrm &P, const Ws<T> &w) {
+  const CellD &c = S.cell;
+  double dmax = 0.0;
+  for (int i = threadIdx.x; i < S.n; i += NT) {
+    V3<double> o = ld3(S.xyz, i);
+    double f0 = o.x * c.A00 + o.y * c.A10 + o.z * c.A20, f1 = o.y * c.A11 + o.z * c.A21, f2 = o.z * c.A22;
+    int wx = (int)floor(f0), wy = (int)floor(f1), wz = (int)floor(f2);
+    w.wrap[3 * i] = wx;
+    w.wrap[3 * i + 1] = wy;
+    w.wrap[3 * i + 2] = wz;
+    double sx = wx * c.Ai00 + wy * c.Ai10 + wz * c.Ai20, sy = wy * c.Ai11 + wz * c.Ai21, sz = wz * c.Ai22;
+    double4 p = make_double4(o.x - sx, o.y - sy, o.z - sz, 0.0);
+    w.pos[i] = p;
+    double g[10];
+    rotate_multipole(i, S.xyz, S.frame, S.axis, S.mloc, g);
+    for (int q = 0; q < 10; q++) w.gmp[10 * i + q] = g[q];
+    const double t = 1.0 / 3.0;
+    AtomMP a;
+    a.c = g[0]; a.dx = g[1]; a.dy = g[2]; a.dz = g[3];
+    a.qxx = g[4] * t; a.qyy = g[5] * t; a.qzz = g[6] * t;
+    a.qxy = g[7] * t; a.qxz = g[8] * t; a.qyz = g[9] * t;
+    a.ipd = S.ipd[i];
+    a.thole = S.thole[i];
+    w.mp[i] = a;
+    int h = S.redidx[i];
+    double rv = S.redval[i];
+    double4 rs = make_double4(o.x, o.y, o.z, 0.0);
+    if (h >= 0 && h != i && rv > 0.0) {
+      V3<double> ph = ld3(S.xyz, h);
+      rs.x = rv * (o.x - ph.x) + ph.x;
+      rs.y = rv * (o.y - ph.y) + ph.y;
+      rs.z = rv * (o.z - ph.z) + ph.z;
+    }
+    double ex = rs.x - o.x, ey = rs.y - o.y, ez = rs.z - o.z;
+    dmax = fmax(dmax, sqrt(ex * ex + ey * ey + ez * ez));
+    w.red[i] = make_double4(rs.x - sx, rs.y - sy, rs.z - sz, 0.0);
+    dip_put_static((void *)w.rec, sizeof(T) == 4, i, p.x, p.y, p.z, a.ipd, a.thole);
+    if (P.recip) {
+      int Kd[3] = {S.K0, S.K1, S.K2};
+      for (int d = 0; d < 3; d++) {
+        double wf = p.x * c.A[0][d] + p.y * c.A[1][d] + p.z * c.A[2][d];
+        double u = wf - floor(wf + 0.5) + 0.5;
+        double frx = Kd[d] * u;
+        int ifr = (int)frx;
+        double bx = frx - ifr;
+        w.igrid[3 * i + d] = ifr - 5;
+        double th[5][4];
+        bspline4<5>(bx, th);
+        double *ot = w.theta + ((size_t)i * 3 + d) * 20;
+        for (int k = 0; k < 5; k++)
+          for (int j = 0; j < 4; j++) ot[4 * k + j] = th[k][j];
+      }
+      frac_multipole(S.fr, g, w.fmp + 10 * i);
+    }
+    for (int d = 0; d < 3; d++) {
+      w.gacc[3 * i + d] = 0.0;
+      w.tacc[3 * i + d] = 0.0;
+      w.gred[3 * i + d] = 0.0;
+      S.grad[3 * i + d] = 0.0;
+    }
+  }
+  return dmax;
+}
+
+// ---------------------------------------------------------------------------
+// rows.  Lane l of an atom's group scans the partner block [l n / LPA,
+// (l + 1) n / LPA) and, per partner, the lattice translations whose
+// fractional offset can reach the cutoff.  A counting pass, a group scan for
+// each lane's offset and a writing pass list the partners of every row in
+// ascending order, translations in (z, y, x) order.  Entry: partner | image
+// code << 18 | mask code << 27, the format of the per-system image rows.
+// ---------------------------------------------------------------------------
+template <typename T, bool WRITE>
+__device__ __forceinline__ void scan_partners(const BatchSys &S, const Ws<T> &w, int i, int t0, int t1,
+                                              double lcut2, double ecut2, double pcut, const double cr[3], int &ca,
+                                              int &cb, uint32_t *rowa, uint32_t *rowb, int capa, int capb) {
+  const CellD &c = S.cell;
+  double4 pi = w.pos[i];
+  double fi0 = pi.x * c.A00 + pi.y * c.A10 + pi.z * c.A20, fi1 = pi.y * c.A11 + pi.z * c.A21, fi2 = pi.z * c.A22;
+  int gi = S.group[i];
+  int wi0 = w.wrap[3 * i], wi1 = w.wrap[3 * i + 1], wi2 = w.wrap[3 * i + 2];
+  for (int t = t0; t < t1; t++) {
+    double4 pk = w.pos[t];
+    double f0 = pk.x * c.A00 + pk.y * c.A10 + pk.z * c.A20 - fi0, f1 = pk.y * c.A11 + pk.z * c.A21 - fi1;
+    double f2 = pk.z * c.A22 - fi2;
+    int lo0 = max((int)ceil(-cr[0] - f0), -NB_IMG_R), hi0 = min((int)floor(cr[0] - f0), NB_IMG_R);
+    int lo1 = max((int)ceil(-cr[1] - f1), -NB_IMG_R), hi1 = min((int)floor(cr[1] - f1), NB_IMG_R);
+    int lo2 = max((int)ceil(-cr[2] - f2), -NB_IMG_R), hi2 = min((int)floor(cr[2] - f2), NB_IMG_R);
+    int o0 = 0, o1 = 0, o2 = 0, rel = -1;
+    if (WRITE) {
+      o0 = w.wrap[3 * t] - wi0;
+      o1 = w.wrap[3 * t + 1] - wi1;
+      o2 = w.wrap[3 * t + 2] - wi2;
+    }
+    for (int tz = lo2; tz <= hi2; tz++)
+      for (int ty = lo1; ty <= hi1; ty++)
+        for (int tx = lo0; tx <= hi0; tx++) {
+          if (t == i && tx == 0 && ty == 0 && tz == 0) continue;
+          double x = pk.x + tx * c.Ai00 + ty * c.Ai10 + tz * c.Ai20, y = pk.y + ty * c.Ai11 + tz * c.Ai21;
+          double z = pk.z + tz * c.Ai22;
+          double dx = x - pi.x, dy = y - pi.y, dz = z - pi.z;
+          double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 > lcut2) continue;
+          bool pre = r2 <= ecut2 && sqrt(r2) < pcut;
+          if (WRITE) {
+            int code = 0;
+            if (tx == o0 && ty == o1 && tz == o2) {
+              if (rel < 0) rel = relation(S.moff, S.matom, S.mrel, i, t);
+              code = rel + (S.group[t] == gi ? 8 : 0);
+            }
+            uint32_t img = (uint32_t)((tx + NB_IMG_R) + NB_IMG_W * ((ty + NB_IMG_R) + NB_IMG_W * (tz + NB_IMG_R)));
+            uint32_t e = (uint32_t)t | (img << NB_IMG_SHIFT) | ((uint32_t)code << NB_CODE_SHIFT);
+            if (pre) {
+              if (ca < capa) rowa[ca] = e;
+            } else {
+              if (cb < capb) rowb[cb] = e;
+            }
+          }
+          if (pre) ca++;
+          else cb++;
+        }
+  }
+}
+
+// returns (per thread) the largest true row lengths seen, which exceed the
+// capacities when a row overflowed (its entries past the capacity are dropped)
+template <typename T, int NT, int LPA>
+__device__ void phase_rows(const BatchSys &S, const BatchPrm &P, const Ws<T> &w, double lcut, int &mxa, int &mxb) {
+  const CellD &c = S.cell;
+  double cr[3];
+  for (int d = 0; d < 3; d++)
+    cr[d] = lcut * sqrt(c.A[0][d] * c.A[0][d] + c.A[1][d] * c.A[1][d] + c.A[2][d] * c.A[2][d]);
+  double lcut2 = lcut * lcut, ecut2 = P.ewald_cut * P.ewald_cut;
+  const int groups = NT / LPA, g = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  mxa = 0;
+  mxb = 0;
+  for (int base = 0; base < S.n; base += groups) {
+    int i = base + g;
+    bool valid = i < S.n;
+    int t0 = (int)(((long)lane * S.n) / LPA), t1 = (int)(((long)(lane + 1) * S.n) / LPA);
+    int ca = 0, cb = 0;
+    if (valid) scan_partners<T, false>(S, w, i, t0, t1, lcut2, ecut2, P.pcut, cr, ca, cb, nullptr, nullptr, 0, 0);
+    int oa = group_excl_scan<LPA>(ca), ob = group_excl_scan<LPA>(cb);
+    int ta = __shfl(oa + ca, LPA - 1, LPA), tb = __shfl(ob + cb, LPA - 1, LPA);
+    if (valid) {
+      int wa = 0, wb = 0;
+      uint32_t *rowa = w.ra + (size_t)i * S.maxa, *rowb = w.rb + (size_t)i * S.maxb;
+      // each lane writes from its offset, bounded by what is left of the row
+      if (oa < S.maxa && ob < S.maxb)
+        scan_partners<T, true>(S, w, i, t0, t1, lcut2, ecut2, P.pcut, cr, wa, wb, rowa + oa, rowb + ob,
+                               S.maxa - oa, S.maxb - ob);
+      if (lane == 0) {
+        w.na[i] = min(ta, S.maxa);
+        w.nb[i] = min(tb, S.maxb);
+        mxa = max(mxa, ta);
+        mxb = max(mxb, tb);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// real-space pair loops: LPA lanes per atom walk the row (preconditioner part,
+// then the rest)
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int LPA>
+__device__ void phase_perm_field(const BatchSys &S, const BatchPrm &P, const Ws<T> &w) {
+  const int groups = NT / LPA, g = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  const double ecut2 = P.ewald_cut * P.ewald_cut, aw = P.recip ? P.aewald : 0.0;
+  for (int base = 0; base < S.n; base += groups) {
+    int i = base + g;
+    bool valid = i < S.n;
+    V3<double> fd{0, 0, 0}, fp{0, 0, 0};
+    if (valid) {
+      double4 pi = w.pos[i];
+      AtomMP ai = w.mp[i];
+      int na = w.na[i], m = na + w.nb[i];
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t e = row_entry(S, w, i, na, t);
+        int k = e & S.cell.kmask, code = e >> NB_CODE_SHIFT;
+        double4 pk = w.pos[k];
+        entry_shift(S.cell, e, pk.x, pk.y, pk.z);
+        double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+        if (dx * dx + dy * dy + dz * dz > ecut2) continue;
+        if (sizeof(T) == 4 && P.sc.masked[code])
+          perm_field_pair<double>(pi, ai, pk, w.mp[k], code, S.cell, aw, P.sc, fd, fp);
+        else
+          perm_field_pair<T>(pi, ai, pk, w.mp[k], code, S.cell, aw, P.sc, fd, fp);
+      }
+    }
+    group_sum3<LPA>(fd);
+    group_sum3<LPA>(fp);
+    if (valid && lane == 0) {
+      st3(w.f, i, fd);
+      st3(w.fp, i, fp);
+    }
+  }
+}
+
+// field of the vectors held in the packed records: Ewald-screened over the
+// Ewald part of the rows, or (PRE) bare Coulomb + Thole over the
+// preconditioner part
+template <typename T, int NT, int LPA, bool PRE>
+__device__ void phase_dip_field(const BatchSys &S, const BatchPrm &P, const Ws<T> &w) {
+  const int groups = NT / LPA, g = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  const double ecut2 = P.ewald_cut * P.ewald_cut, aw = (P.recip && !PRE) ? P.aewald : 0.0;
+  for (int base = 0; base < S.n; base += groups) {
+    int i = base + g;
+    bool valid = i < S.n;
+    V3<double> a{0, 0, 0}, b{0, 0, 0};
+    if (valid) {
+      const DipRec<T> ri = w.rec[i];
+      int na = w.na[i], m = PRE ? na : na + w.nb[i];
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t e = row_entry(S, w, i, na, t);
+        const DipRec<T> rk = w.rec[e & S.cell.kmask];
+        if (!PRE && t >= na) {
+          double dx = rk.x - ri.x, dy = rk.y - ri.y, dz = rk.z - ri.z;
+          entry_shift(S.cell, e, dx, dy, dz);
+          if (dx * dx + dy * dy + dz * dz > ecut2) continue;
+        }
+        dip_field_pair<T, PRE>(ri, rk, e, S.cell, aw, a, b);
+      }
+    }
+    group_sum3<LPA>(a);
+    group_sum3<LPA>(b);
+    if (valid && lane == 0) {
+      st3(w.f, i, a);
+      st3(w.fp, i, b);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reciprocal space
+// ---------------------------------------------------------------------------
+template <typename T, int NT>
+__device__ void grid_zero(const BatchSys &S, const Ws<T> &w) {
+  long ntot = (long)S.K0 * S.K1 * S.K2;
+  for (long q = threadIdx.x; q < ntot; q += NT) w.grid[q] = CX<T>{(T)0, (T)0};
+}
+
+// 125-point stencils added to the grid with atomics: the permanent fractional
+// multipoles into the real part (IND false), or the dipoles s0 into the real
+// and s1 into the imaginary part (ReciprocalSpace.java:1445-1643)
+template <typename T, int NT, bool IND>
+__device__ void grid_spread(const BatchSys &S, const Ws<T> &w, const double *__restrict__ s0,
+                            const double *__restrict__ s1) {
+  const int K0 = S.K0, K1 = S.K1, K2 = S.K2;
+  const Frac &fr = S.fr;
+  for (int q = threadIdx.x; q < S.n * 25; q += NT) {
+    int a = q / 25, yz = q - a * 25;
+    int ky = yz % 5, kz = yz / 5;
+    const double *tx = w.theta + (size_t)a * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
+    int gz = wrapg(w.igrid[3 * a + 2] + 1 + kz, K2), gy = wrapg(w.igrid[3 * a + 1] + 1 + ky, K1);
+    int gx0 = w.igrid[3 * a] + 1;
+    CX<T> *row = w.grid + (size_t)K0 * (gy + (size_t)K1 * gz);
+    if (!IND) {
+      const double *m = w.fmp + 10 * (size_t)a;
+      T u0 = (T)uu[0], u1 = (T)uu[1], u2 = (T)uu[2], v0 = (T)vv[0], v1 = (T)vv[1], v2 = (T)vv[2];
+      T t0 = ((T)m[0] * v0 + (T)m[3] * v1 + (T)m[6] * v2) * u0 + ((T)m[2] * v0 + (T)m[9] * v1) * u1 + (T)m[5] * v0 * u2;
+      T t1 = ((T)m[1] * v0 + (T)m[8] * v1) * u0 + (T)m[7] * v0 * u1;
+      T t2 = (T)m[4] * v0 * u0;
+#pragma unroll
+      for (int kx = 0; kx < 5; kx++) {
+        T val = (T)tx[4 * kx] * t0 + (T)tx[4 * kx + 1] * t1 + (T)tx[4 * kx + 2] * t2;
+        unsafeAtomicAdd(&row[wrapg(gx0 + kx, K0)].x, val);
+      }
+    } else {
+      T u0 = (T)uu[0], u1 = (T)uu[1], v0 = (T)vv[0], v1 = (T)vv[1];
+      T t0[2], t1[2];
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        const double *src = (p ? s1 : s0) + 3 * (size_t)a;
+        T d0 = (T)(fr.a[0][0] * src[0] + fr.a[0][1] * src[1] + fr.a[0][2] * src[2]);
+        T d1 = (T)(fr.a[1][0] * src[0] + fr.a[1][1] * src[1] + fr.a[1][2] * src[2]);
+        T d2 = (T)(fr.a[2][0] * src[0] + fr.a[2][1] * src[1] + fr.a[2][2] * src[2]);
+        t0[p] = d2 * v1 * u0 + d1 * v0 * u1;
+        t1[p] = d0 * v0 * u0;
+      }
+#pragma unroll
+      for (int kx = 0; kx < 5; kx++) {
+        T wx0 = (T)tx[4 * kx], wx1 = (T)tx[4 * kx + 1];
+        CX<T> *pt = &row[wrapg(gx0 + kx, K0)];
+        unsafeAtomicAdd(&pt->x, wx0 * t0[0] + wx1 * t1[0]);
+        unsafeAtomicAdd(&pt->y, wx0 * t0[1] + wx1 * t1[1]);
+      }
+    }
+  }
+}
+
+// forward x, forward y, fused forward z / influence / inverse z, inverse y,
+// inverse x (un-normalised inverse, Complex.java:436-470)
+template <typename T, int NT>
+__device__ void grid_convolve(const BatchSys &S, const Ws<T> &w, CX<T> *buf, const CX<T> *tw) {
+  constexpr int EPT = FFT_CAP / NT;
+  const int K0 = S.K0, K1 = S.K1, K2 = S.K2;
+  const CX<T> *twx = tw, *twy = tw + K0, *twz = tw + K0 + K1;
+  const T *recip = (const T *)S.recip;
+  const T *nul = nullptr;
+  for (int pass = 0; pass < 2; pass++) {
+    T s = pass == 0 ? (T)1 : (T)-1;
+    if (pass == 0) {
+      int nch = (K1 * K2 + S.Lx - 1) / S.Lx;
+      for (int c = 0; c < nch; c++) {
+        fft_chunk<T, EPT, 0, false>(w.grid, K0, K1, K2, S.Lx, c, S.rad[0], buf, twx, s, nul);
+        __syncthreads();
+      }
+      phase_sync();
+    }
+    {
+      int nch = ((K0 + S.Ly - 1) / S.Ly) * K2;
+      for (int c = 0; c < nch; c++) {
+        fft_chunk<T, EPT, 1, false>(w.grid, K0, K1, K2, S.Ly, c, S.rad[1], buf, twy, s, nul);
+        __syncthreads();
+      }
+      phase_sync();
+    }
+    if (pass == 0) {
+      int nch = (K0 * K1 + S.Lz - 1) / S.Lz;
+      for (int c = 0; c < nch; c++) {
+        fft_chunk<T, EPT, 2, true>(w.grid, K0, K1, K2, S.Lz, c, S.rad[2], buf, twz, (T)1, recip);
+        __syncthreads();
+      }
+      phase_sync();
+    } else {
+      int nch = (K1 * K2 + S.Lx - 1) / S.Lx;
+      for (int c = 0; c < nch; c++) {
+        fft_chunk<T, EPT, 0, false>(w.grid, K0, K1, K2, S.Lx, c, S.rad[0], buf, twx, s, nul);
+        __syncthreads();
+      }
+      phase_sync();
+    }
+  }
+}
+
+template <typename T, int NT, int NP, bool FULL>
+__device__ void grid_gather(const BatchSys &S, const Ws<T> &w, double *o0, double *o1) {
+  const int groups = NT / 8, g = threadIdx.x / 8, lane = threadIdx.x % 8;
+  for (int base = 0; base < S.n; base += groups) {
+    int i = base + g;
+    gather_atom<T, NP, FULL>(i, lane, i < S.n, w.theta, w.igrid, w.grid, S.fr, S.K0, S.K1, S.K2, o0, o1);
+  }
+}
+
+// reciprocal potential of the dipoles s0 (u) and s1 (uCR): FULL gives the 20
+// derivatives each, otherwise the field only
+template <typename T, int NT, bool FULL>
+__device__ void recip_induced(const BatchSys &S, const Ws<T> &w, const double *s0, const double *s1, CX<T> *buf,
+                              const CX<T> *tw) {
+  grid_zero<T, NT>(S, w);
+  phase_sync();
+  grid_spread<T, NT, true>(S, w, s0, s1);
+  phase_sync();
+  grid_convolve<T, NT>(S, w, buf, tw);
+  grid_gather<T, NT, 2, FULL>(S, w, w.phiu, w.phiup);
+  phase_sync();
+}
+
+// total induced field of (s0, s1) into f / fp: real space over the Ewald rows
+// (the packed records must hold s0 / s1), self term and reciprocal potential
+// (InducedDipoleFieldRegion + InducedDipoleFieldReduceRegion)
+template <typename T, int NT, int LPA>
+__device__ void induced_field(const BatchSys &S, const BatchPrm &P, const Ws<T> &w, const double *s0,
+                              const double *s1, CX<T> *buf, const CX<T> *tw) {
+  phase_dip_field<T, NT, LPA, false>(S, P, w);
+  if (P.recip) {
+    recip_induced<T, NT, false>(S, w, s0, s1, buf, tw);
+    const double a3 = 4.0 / 3.0 * P.aewald * P.aewald * P.aewald / NB_SQRT_PI;
+    for (int q = threadIdx.x; q < 3 * S.n; q += NT) {
+      int s = q / 3, d = q - 3 * s;
+      w.f[q] += a3 * s0[q] - w.phiu[30 * (size_t)s + 21 + d];
+      w.fp[q] += a3 * s1[q] - w.phiup[30 * (size_t)s + 21 + d];
+    }
+  }
+  phase_sync();
+}
+
+// ---------------------------------------------------------------------------
+// PCG (PCGSolver.java:487-604, the iteration of nb_pcg in device form).
+// Returns the status; iterations and final eps through it / eps.
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int LPA>
+__device__ int phase_pcg(const BatchSys &S, const BatchPrm &P, const Ws<T> &w, CX<T> *buf, const CX<T> *tw,
+                         double *red, int &iters, double &eps_out) {
+  const int n = S.n, n3 = 3 * n;
+  const int mixed = sizeof(T) == 4;
+  void *rec = (void *)w.rec;
+  double *u = S.u, *up = S.up;
+  // E(u0) with u0 = u_direct (the records hold it)
+  induced_field<T, NT, LPA>(S, P, w, u, up, buf, tw);
+  for (int q = threadIdx.x; q < n3; q += NT) {
+    double a = S.polar[q / 3];
+    double rr = 0.0, rrp = 0.0;
+    if (a > 0.0) {
+      rr = (w.ud[q] - u[q]) / a + w.f[q];
+      rrp = (w.upd[q] - up[q]) / a + w.fp[q];
+    }
+    w.r[q] = rr;
+    w.rp[q] = rrp;
+    dip_put(rec, mixed, q / 3, q % 3, a * rr, a * rrp);
+  }
+  phase_sync();
+  phase_dip_field<T, NT, LPA, true>(S, P, w);
+  phase_sync();
+  double rz[2] = {0.0, 0.0};
+  for (int q = threadIdx.x; q < n3; q += NT) {
+    double a = S.polar[q / 3];
+    double zz = a * (w.f[q] + P.psc * w.r[q]), zzp = a * (w.fp[q] + P.psc * w.rp[q]);
+    w.z[q] = zz;
+    w.zp[q] = zzp;
+    w.p[q] = zz;
+    w.pp[q] = zzp;
+    dip_put(rec, mixed, q / 3, q % 3, zz, zzp);
+    rz[0] += w.r[q] * zz;
+    rz[1] += w.rp[q] * zzp;
+  }
+  block_sums<NT, 2>(rz, red);
+  phase_sync();
+  double eps = 100.0;
+  int iter = 0, status = NB_OK;
+  const double a3 = P.recip ? 4.0 / 3.0 * P.aewald * P.aewald * P.aewald / NB_SQRT_PI : 0.0;
+  (void)a3;
+  for (;;) {
+    induced_field<T, NT, LPA>(S, P, w, w.p, w.pp, buf, tw);
+    double v4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < n3; q += NT) {
+      double a = S.polar[q / 3];
+      double x = 0.0, xp = 0.0;
+      if (a > 0.0) {
+        x = w.p[q] / a - w.f[q];
+        xp = w.pp[q] / a - w.fp[q];
+      }
+      w.vec[q] = x;
+      w.vecp[q] = xp;
+      v4[0] += w.p[q] * x;
+      v4[1] += w.pp[q] * xp;
+      v4[2] += w.r[q] * w.z[q];
+      v4[3] += w.rp[q] * w.zp[q];
+    }
+    block_sums<NT, 4>(v4, red);
+    double al = v4[2], alp = v4[3];
+    if (v4[0] != 0.0) al /= v4[0];
+    if (v4[1] != 0.0) alp /= v4[1];
+    double rr2[2] = {0.0, 0.0};
+    for (int q = threadIdx.x; q < n3; q += NT) {
+      double rr = w.r[q] - al * w.vec[q], rrp = w.rp[q] - alp * w.vecp[q];
+      w.r[q] = rr;
+      w.rp[q] = rrp;
+      u[q] += al * w.p[q];
+      up[q] += alp * w.pp[q];
+      double a = S.polar[q / 3];
+      dip_put(rec, mixed, q / 3, q % 3, a * rr, a * rrp);
+      rr2[0] += rr * rr;
+      rr2[1] += rrp * rrp;
+    }
+    block_sums<NT, 2>(rr2, red);
+    double prev = eps;
+    double e = rr2[0] > rr2[1] ? rr2[0] : rr2[1];
+    iter++;
+    eps = 4.80321 * sqrt(e / n);
+    if (!(eps <= prev)) { status = NB_ERR_SCF_DIVERGED; break; }
+    if (iter >= P.maxit) { status = NB_ERR_SCF_MAXITER; break; }
+    if (eps < P.polar_eps) break;
+    phase_sync();
+    phase_dip_field<T, NT, LPA, true>(S, P, w);
+    phase_sync();
+    double rzn[2] = {0.0, 0.0};
+    for (int q = threadIdx.x; q < n3; q += NT) {
+      double a = S.polar[q / 3];
+      double zz = a * (w.f[q] + P.psc * w.r[q]), zzp = a * (w.fp[q] + P.psc * w.rp[q]);
+      w.z[q] = zz;
+      w.zp[q] = zzp;
+      rzn[0] += w.r[q] * zz;
+      rzn[1] += w.rp[q] * zzp;
+    }
+    block_sums<NT, 2>(rzn, red);
+    double prev0 = v4[2] != 0.0 ? v4[2] : 1.0, prev1 = v4[3] != 0.0 ? v4[3] : 1.0;
+    double b = rzn[0] / prev0, bp = rzn[1] / prev1;
+    for (int q = threadIdx.x; q < n3; q += NT) {
+      double x = w.z[q] + b * w.p[q], xp = w.zp[q] + bp * w.pp[q];
+      w.p[q] = x;
+      w.pp[q] = xp;
+      dip_put(rec, mixed, q / 3, q % 3, x, xp);
+    }
+    phase_sync();
+  }
+  phase_sync();
+  iters = iter;
+  eps_out = eps;
+  return status;
+}
+
+// ---------------------------------------------------------------------------
+// energies, forces and torques
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int LPA, bool POL>
+__device__ void phase_real_energy(const BatchSys &S, const BatchPrm &P, const Ws<T> &w, double &esum,
+                                  long long &cnt) {
+  const int groups = NT / LPA, g = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  const double ecut2 = P.ewald_cut * P.ewald_cut, aw = P.recip ? P.aewald : 0.0, el = P.electric;
+  const int direct = P.polarization == NB_POL_DIRECT;
+  for (int base = 0; base < S.n; base += groups) {
+    int i = base + g;
+    bool valid = i < S.n;
+    V3<double> F{0, 0, 0}, Tq{0, 0, 0};
+    double e = 0.0;
+    if (valid) {
+      double4 pi = w.pos[i];
+      AtomMP ai = w.mp[i];
+      int na = w.na[i], m = na + w.nb[i];
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t en = row_entry(S, w, i, na, t);
+        int k = en & S.cell.kmask, code = en >> NB_CODE_SHIFT;
+        double4 pk = w.pos[k];
+        entry_shift(S.cell, en, pk.x, pk.y, pk.z);
+        double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+        if (dx * dx + dy * dy + dz * dz > ecut2) continue;
+        bool dbl = sizeof(T) == 4 && P.sc.masked[code];
+        if (POL) {
+          if (dbl)
+            pol_energy_pair<double>(pi, ai, S.u, S.up, i, pk, w.mp[k], k, code, S.cell, aw, P.sc, direct, e, F, Tq,
+                                    nullptr);
+          else
+            pol_energy_pair<T>(pi, ai, S.u, S.up, i, pk, w.mp[k], k, code, S.cell, aw, P.sc, direct, e, F, Tq,
+                               nullptr);
+        } else {
+          if (dbl) perm_energy_pair<double>(pi, ai, pk, w.mp[k], code, S.cell, aw, P.sc, e, cnt, F, Tq, nullptr);
+          else perm_energy_pair<T>(pi, ai, pk, w.mp[k], code, S.cell, aw, P.sc, e, cnt, F, Tq, nullptr);
+        }
+      }
+    }
+    group_sum3<LPA>(F);
+    group_sum3<LPA>(Tq);
+    if (valid && lane == 0) {
+      add3(w.gacc, i, el * F);
+      add3(w.tacc, i, el * Tq);
+    }
+    esum += 0.5 * el * e;
+  }
+}
+
+template <typename T, int NT, int LPA>
+__device__ void phase_vdw(const BatchSys &S, const BatchPrm &P, const Ws<T> &w, double &esum, long long &cnt) {
+  const int groups = NT / LPA, g = threadIdx.x / LPA, lane = threadIdx.x % LPA;
+  double off = P.vdw_off, cut = P.vdw_cut, off2 = off * off, cut2 = cut * cut;
+  T wcut = (T)cut, iw = (T)(1.0 / (off - cut));
+  T d1 = (T)P.hdelta, g1 = (T)1 + (T)P.hgamma, gg = (T)P.hgamma;
+  T t1n = (T)pow(1.0 + P.hdelta, 7.0);
+  for (int base = 0; base < S.n; base += groups) {
+    int i = base + g;
+    bool valid = i < S.n;
+    V3<double> G{0, 0, 0};
+    double e = 0.0;
+    if (valid) {
+      double4 pi = w.red[i];
+      int ci = S.vclass[i];
+      int na = w.na[i], m = na + w.nb[i];
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t en = row_entry(S, w, i, na, t);
+        int k = en & S.cell.kmask, code = en >> NB_CODE_SHIFT;
+        double vs = P.sc.v[code];
+        if (!(vs > 0.0)) continue;
+        double4 pk = w.red[k];
+        entry_shift(S.cell, en, pk.x, pk.y, pk.z);
+        double dx = pi.x - pk.x, dy = pi.y - pk.y, dz = pi.z - pk.z;
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > off2) continue;
+        const double *pe = S.vtab + 2 * (ci * S.nclass + S.vclass[k]);
+        T irv = (T)pe[0];
+        if (!(irv > (T)0)) continue;
+        cnt++;
+        T ev = (T)(vs * pe[1]);
+        Vdw147<T> v = vdw147<T>(r2, irv, ev, (T)0, (T)1, d1, g1, gg, t1n, wcut, iw, r2 > cut2);
+        e += (double)(v.eik * v.tp);
+        acc3(G, V3<T>{v.ds * (T)dx, v.ds * (T)dy, v.ds * (T)dz});
+      }
+    }
+    group_sum3<LPA>(G);
+    if (valid && lane == 0) st3(w.gred, i, G);
+    esum += 0.5 * e;
+  }
+}
+
+// torques to frame-atom gradients, reduced-site chain rule, caller-order
+// gradient (atomics: frame and parent atoms receive from several atoms)
+template <typename T, int NT>
+__device__ int phase_finalize(const BatchSys &S, const BatchPrm &P, const Ws<T> &w) {
+  int bad = 0;
+  for (int i = threadIdx.x; i < S.n; i += NT) {
+    V3<double> g = ld3d(w.gacc, i);
+    if (P.do_vdw) {
+      V3<double> gv = ld3d(w.gred, i);
+      int h = S.redidx[i];
+      double f = S.redval[i];
+      if (h >= 0 && h != i && f > 0.0) {
+        g += f * gv;
+        V3<double> gh = (1.0 - f) * gv;
+        unsafeAtomicAdd(&S.grad[3 * h], gh.x);
+        unsafeAtomicAdd(&S.grad[3 * h + 1], gh.y);
+        unsafeAtomicAdd(&S.grad[3 * h + 2], gh.z);
+      } else {
+        g += gv;
+      }
+    }
+    V3<double> gi = g;
+    int fr = S.frame[i];
+    if (P.do_pme && fr != NB_FRAME_NONE) {
+      V3<double> trq = ld3d(w.tacc, i);
+      if (!isfinite(trq.x) || !isfinite(trq.y) || !isfinite(trq.z)) bad = 1;
+      int iz = S.axis[3 * i], ix = S.axis[3 * i + 1], iy = S.axis[3 * i + 2];
+      V3<double> gz, gx, gy, vz, vx, vy;
+      torque_frame(fr, i, iz, ix, iy, S.xyz, trq, gz, gx, gy, vz, vx, vy);
+      unsafeAtomicAdd(&S.grad[3 * iz], gz.x);
+      unsafeAtomicAdd(&S.grad[3 * iz + 1], gz.y);
+      unsafeAtomicAdd(&S.grad[3 * iz + 2], gz.z);
+      if (ix >= 0 && fr != NB_FRAME_ZONLY) {
+        unsafeAtomicAdd(&S.grad[3 * ix], gx.x);
+        unsafeAtomicAdd(&S.grad[3 * ix + 1], gx.y);
+        unsafeAtomicAdd(&S.grad[3 * ix + 2], gx.z);
+      }
+      if (iy >= 0 && (fr == NB_FRAME_ZTHENBISECTOR || fr == NB_FRAME_THREEFOLD)) {
+        unsafeAtomicAdd(&S.grad[3 * iy], gy.x);
+        unsafeAtomicAdd(&S.grad[3 * iy + 1], gy.y);
+        unsafeAtomicAdd(&S.grad[3 * iy + 2], gy.z);
+      }
+      gi = gi - (gz + gx + gy);
+    }
+    unsafeAtomicAdd(&S.grad[3 * i], gi.x);
+    unsafeAtomicAdd(&S.grad[3 * i + 1], gi.y);
+    unsafeAtomicAdd(&S.grad[3 * i + 2], gi.z);
+  }
+  return bad;
+}
+
+// ---------------------------------------------------------------------------
+// one system
+// ---------------------------------------------------------------------------
+template <typename T, int NT, int LPA>
+__device__ void run_system(const BatchSys &S, const BatchPrm &P, const Ws<T> &w, double *__restrict__ out,
+                           CX<T> *buf, CX<T> *tw, double *red) {
+  const int mixed = sizeof(T) == 4;
+  const bool polar = P.do_pme && P.polarization != NB_POL_NONE;
+  const bool mutual = polar && P.polarization == NB_POL_MUTUAL;
+  int flags = 0;
+  if (P.recip) {
+    const CX<T> *twg = (const CX<T> *)S.tw;
+    for (int t = threadIdx.x; t < S.K0 + S.K1 + S.K2; t += NT) tw[t] = twg[t];
+  }
+  // the list cutoff covers the Ewald cutoff and the vdW cutoff between
+  // reduced sites (each within its largest displacement of the atom)
+  double dmax[1] = {phase_atoms<T, NT>(S, P, w)};
+  block_max<NT, 1>(dmax, red);
+  double lcut = fmax(P.ewald_cut, P.do_vdw ? P.vdw_off + 2.0 * dmax[0] : 0.0);
+  if (lcut > S.lcut) flags |= F_CUTOFF;
+  phase_sync();
+  double rows[2];
+  {
+    int mxa, mxb;
+    phase_rows<T, NT, LPA>(S, P, w, fmin(lcut, S.lcut), mxa, mxb);
+    rows[0] = mxa;
+    rows[1] = mxb;
+    block_max<NT, 2>(rows, red);
+    if (rows[0] > S.maxa || rows[1] > S.maxb) flags |= F_ROWS;
+  }
+  phase_sync();
+  double e_pr = 0.0, e_pol = 0.0, e_vdw = 0.0, e_self = 0.0, e_rec = 0.0, e_q = 0.0, e_selfi = 0.0, e_reci = 0.0;
+  long long c_perm = 0, c_vdw = 0;
+  int iters = 0, status = NB_OK;
+  double eps = 0.0;
+  if (P.do_pme && !(flags & (F_ROWS | F_CUTOFF))) {
+    phase_perm_field<T, NT, LPA>(S, P, w);
+    if (P.recip) {
+      grid_zero<T, NT>(S, w);
+      phase_sync();
+      grid_spread<T, NT, false>(S, w, nullptr, nullptr);
+      phase_sync();
+      grid_convolve<T, NT>(S, w, buf, tw);
+      grid_gather<T, NT, 1, true>(S, w, w.phip, nullptr);
+    }
+    phase_sync();
+    if (polar) {
+      const double a3 = P.recip ? 4.0 / 3.0 * P.aewald * P.aewald * P.aewald / NB_SQRT_PI : 0.0;
+      for (int s = threadIdx.x; s < S.n; s += NT) {
+        double a = S.polar[s];
+        for (int d = 0; d < 3; d++) {
+          double f = w.f[3 * s + d], fp = w.fp[3 * s + d];
+          if (P.recip) {
+            double x = a3 * w.gmp[10 * s + 1 + d] - w.phip[30 * (size_t)s + 21 + d];
+            f += x;
+            fp += x;
+          }
+          S.u[3 * s + d] = w.ud[3 * s + d] = a * f;
+          S.up[3 * s + d] = w.upd[3 * s + d] = a * fp;
+          dip_put((void *)w.rec, mixed, s, d, a * f, a * fp);
+        }
+      }
+      phase_sync();
+      if (mutual) status = phase_pcg<T, NT, LPA>(S, P, w, buf, tw, red, iters, eps);
+      if (P.recip) recip_induced<T, NT, true>(S, w, S.u, S.up, buf, tw);
+    }
+    phase_real_energy<T, NT, LPA, false>(S, P, w, e_pr, c_perm);
+    phase_sync();
+    if (polar) {
+      phase_real_energy<T, NT, LPA, true>(S, P, w, e_pol, c_perm);
+      phase_sync();
+    }
+    if (P.recip) {
+      for (int s = threadIdx.x; s < S.n; s += NT) {
+        RecipAtom o;
+        recip_atom(s, w.gmp, w.fmp, w.phip, w.phiu, w.phiup, S.u, S.up, S.fr, P.electric, P.aewald, (int)polar,
+                   (int)mutual, 1.0, 1.0, nullptr, o);
+        e_self += o.eself;
+        e_rec += o.erec;
+        e_q += o.qt;
+        e_selfi += o.eselfi;
+        e_reci += o.ereci;
+        for (int a = 0; a < 3; a++) {
+          w.gacc[3 * s + a] += o.df[a] + o.dq[a];
+          w.tacc[3 * s + a] += o.tq[a] + o.tqq[a];
+        }
+      }
+    }
+  }
+  if (P.do_vdw && !(flags & (F_ROWS | F_CUTOFF))) phase_vdw<T, NT, LPA>(S, P, w, e_vdw, c_vdw);
+  phase_sync();
+  int bad = 0;
+  if (!(flags & (F_ROWS | F_CUTOFF))) bad = phase_finalize<T, NT>(S, P, w);
+  double v[11] = {e_pr, e_self, 0.5 * P.electric * e_rec, e_pol, e_selfi, 0.5 * P.electric * e_reci, e_vdw, e_q,
+                  (double)c_vdw, (double)c_perm, (double)bad};
+  block_sums<NT, 11>(v, red);
+  if (v[10] != 0.0) flags |= F_TORQUE;
+  if (threadIdx.x == 0) {
+    out[E_PERM_REAL] = v[0];
+    out[E_PERM_SELF] = v[1];
+    out[E_PERM_RECIP] = v[2];
+    out[E_POL_REAL] = v[3];
+    out[E_POL_SELF] = v[4];
+    out[E_POL_RECIP] = v[5];
+    out[E_VDW] = v[6];
+    out[E_QTOT] = v[7];
+    out[O_ITER] = (double)iters;
+    out[O_EPS] = eps;
+    out[O_STAT] = (double)status;
+    out[O_NVDW] = v[8];
+    out[O_NPERM] = v[9];
+    out[O_FLAGS] = (double)flags;
+    out[O_ROWA] = rows[0];
+    out[O_ROWB] = rows[1];
+  }
+}
+
+// Persistent workgroups take systems from the queue (host order: largest
+// first) until it is empty; every wave of a workgroup leaves the loop on the
+// same index.
+template <typename T, int NT, int LPA>
+__global__ void __launch_bounds__(NT) k_batch(const BatchSys *__restrict__ sys, int nsys, BatchPrm P,
+                                               double *__restrict__ out, int *__restrict__ next) {
+  __shared__ __align__(16) CX<T> buf[FFT_CAP];
+  __shared__ __align__(16) CX<T> tw[TW_CAP];
+  __shared__ double red[16 * (NT / 64)];
+  // the system's descriptor, its workspace pointers and the parameters live
+  // in LDS: every thread reads them with uniform addresses, and they do not
+  // occupy registers across the phases
+  __shared__ BatchSys Sl;
+  __shared__ Ws<T> wl;
+  __shared__ BatchPrm Pl;
+  __shared__ int take;
+  if (threadIdx.x == 0) Pl = P;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      take = atomicAdd(next, 1);
+      if (take < nsys) {
+        Sl = sys[take];
+        ws_map<T>(Sl.ws, Sl.n, Sl.maxa, Sl.maxb, (long)Sl.K0 * Sl.K1 * Sl.K2, &wl);
+      }
+    }
+    __syncthreads();
+    int b = take;
+    if (b >= nsys) break;
+    run_system<T, NT, LPA>(Sl, Pl, wl, out + (size_t)O_N * b, buf, tw, red);
+  }
+}
+
+__global__ void k_queue_reset(int *next) {
+  if (threadIdx.x == 0) next[0] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+Radices nb_factor(int N);
+
+namespace {
+
+// kernel shapes the tuner chooses from: threads per workgroup, lanes per atom
+struct Shape { int nt, lpa; };
+constexpr Shape kShapes[] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {256, 4}};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
+template <typename T>
+static const void *kernel_of(int v) {
+  switch (v) {
+    case 0: return (const void *)k_batch<T, 256, 8>;
+    case 1: return (const void *)k_batch<T, 256, 16>;
+    case 2: return (const void *)k_batch<T, 512, 8>;
+    case 3: return (const void *)k_batch<T, 512, 16>;
+    default: return (const void *)k_batch<T, 256, 4>;
+  }
+}
+
+template <typename T>
+static void launch_shape(int v, int blocks, hipStream_t st, const BatchSys *sys, int nsys, const BatchPrm &P,
+                         double *out, int *next) {
+  hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, st, next);
+  dim3 G(blocks);
+  switch (v) {
+    case 0: hipLaunchKernelGGL((k_batch<T, 256, 8>), G, dim3(256), 0, st, sys, nsys, P, out, next); break;
+    case 1: hipLaunchKernelGGL((k_batch<T, 256, 16>), G, dim3(256), 0, st, sys, nsys, P, out, next); break;
+    case 2: hipLaunchKernelGGL((k_batch<T, 512, 8>), G, dim3(512), 0, st, sys, nsys, P, out, next); break;
+    case 3: hipLaunchKernelGGL((k_batch<T, 512, 16>), G, dim3(512), 0, st, sys, nsys, P, out, next); break;
+    default: hipLaunchKernelGGL((k_batch<T, 256, 4>), G, dim3(256), 0, st, sys, nsys, P, out, next); break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// Per-device resources of the batch kernel, shared by all batches on the
+// device and guarded by a mutex: workspace, descriptors, outputs, queue
+// counter, stream, and the tuner's choice per batch signature.
+struct Pool {
+  std::mutex mu;
+  int device = 0, cus = 0;
+  char *ws = nullptr;
+  size_t ws_cap = 0;
+  BatchSys *d_sys = nullptr, *h_sys = nullptr;
+  int sys_cap = 0;
+  double *d_out = nullptr, *h_out = nullptr;
+  int *d_next = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, ready = nullptr;
+  std::map<std::string, int> tuned;
+  int occ[2][kNumShapes] = {};
+  int last[4] = {0, 0, 0, 0};
+};
+
+std::mutex g_pools_mu;
+std::map<int, Pool *> g_pools;
+
+static Pool *pool_of(int device) {
+  std::lock_guard<std::mutex> lk(g_pools_mu);
+  auto it = g_pools.find(device);
+  if (it != g_pools.end()) return it->second;
+  Pool *p = new Pool();
+  p->device = device;
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, device));
+  p->cus = prop.multiProcessorCount;
+  HIPCHECK(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreate(&p->e0));
+  HIPCHECK(hipEventCreate(&p->e1));
+  HIPCHECK(hipEventCreateWithFlags(&p->ready, hipEventDisableTiming));
+  HIPCHECK(hipMalloc(&p->d_next, sizeof(int) * 64));
+  g_pools[device] = p;
+  return p;
+}
+
+static bool same_params(const nb_params &a, const nb_params &b) {
+  bool ok = a.precision == b.precision && a.do_vdw == b.do_vdw && a.do_pme == b.do_pme &&
+            a.polarization == b.polarization && a.ewald_cutoff == b.ewald_cutoff && a.vdw_cutoff == b.vdw_cutoff &&
+            a.ewald_alpha == b.ewald_alpha && a.vdw_taper == b.vdw_taper && a.precond_cutoff == b.precond_cutoff &&
+            a.precond_scale == b.precond_scale && a.polar_eps == b.polar_eps && a.electric == b.electric &&
+            a.halgren_delta == b.halgren_delta && a.halgren_gamma == b.halgren_gamma &&
+            a.max_scf_iter == b.max_scf_iter && a.list_buffer == b.list_buffer && a.d11 == b.d11 &&
+            a.intra14 == b.intra14;
+  for (int q = 0; q < 4; q++) ok = ok && a.m_scale[q] == b.m_scale[q];
+  for (int q = 0; q < 3; q++) ok = ok && a.p_scale[q] == b.p_scale[q] && a.v_scale[q] == b.v_scale[q];
+  return ok;
+}
+
+// a handle the batch kernel can evaluate: a small periodic system without
+// alchemistry, virial or SOR, and a cell whose image range covers the list
+// cutoff
+static bool batchable(const NbHandle *h, uint32_t flags) {
+  if (!h->cell_set || !h->positions_set || h->cell.aperiodic || h->elec_lambda || h->ost || h->profiling)
+    return false;
+  if (flags & ~(uint32_t)NB_GRAD) return false;
+  if (h->has_soft && h->lambda < 1.0) return false;
+  if (h->prm.scf_algorithm != NB_SCF_PCG) return false;
+  if (h->n > 4096) return false;
+  if (h->prm.do_pme) {
+    if (!(h->prm.ewald_alpha > 0.0) || !h->recip_valid) return false;
+    for (int d = 0; d < 3; d++)
+      if (h->K[d] > 64) return false;
+  }
+  double lcut = std::max(h->prm.ewald_cutoff, h->prm.vdw_cutoff) + h->prm.list_buffer;
+  for (int d = 0; d < 3; d++) {
+    const double(*A)[3] = h->cell.A;
+    double w = 1.0 / sqrt(A[0][d] * A[0][d] + A[1][d] * A[1][d] + A[2][d] * A[2][d]);
+    if ((int)floor(lcut / w) + 1 > NB_IMG_R) return false;
+  }
+  return true;
+}
+
+static int row_estimate(const NbHandle *h, double r) {
+  double est = h->n / h->cell.vol * 4.0 / 3.0 * M_PI * r * r * r * 1.5 + 32.0;
+  return (int)std::min(est, (double)h->n * 343.0);
+}
+
+template <typename T>
+static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::vector<int> &order,
+                     std::vector<char> &retry) {
+  int nsys = (int)hs.size();
+  // descriptors in queue order (largest work first)
+  size_t total = 0;
+  std::vector<size_t> off(nsys);
+  for (int q = 0; q < nsys; q++) {
+    NbHandle *h = hs[order[q]];
+    long ntot = h->prm.do_pme ? (long)h->K[0] * h->K[1] * h->K[2] : 1;
+    off[q] = total;
+    total += ws_map<T>(nullptr, h->n, h->fmaxa, h->fmaxb, ntot, nullptr);
+  }
+  if (total > pl->ws_cap) {
+    if (pl->ws) HIPCHECK(hipFree(pl->ws));
+    pl->ws = nullptr;
+    pl->ws_cap = total + total / 4;
+    HIPCHECK(hipMalloc(&pl->ws, pl->ws_cap));
+  }
+  if (nsys > pl->sys_cap) {
+    if (pl->d_sys) HIPCHECK(hipFree(pl->d_sys));
+    if (pl->h_sys) HIPCHECK(hipHostFree(pl->h_sys));
+    if (pl->d_out) HIPCHECK(hipFree(pl->d_out));
+    if (pl->h_out) HIPCHECK(hipHostFree(pl->h_out));
+    pl->sys_cap = nsys + nsys / 4 + 8;
+    HIPCHECK(hipMalloc(&pl->d_sys, sizeof(BatchSys) * pl->sys_cap));
+    HIPCHECK(hipHostMalloc((void **)&pl->h_sys, sizeof(BatchSys) * pl->sys_cap, hipHostMallocDefault));
+    HIPCHECK(hipMalloc(&pl->d_out, sizeof(double) * O_N * pl->sys_cap));
+    HIPCHECK(hipHostMalloc((void **)&pl->h_out, sizeof(double) * O_N * pl->sys_cap, hipHostMallocDefault));
+  }
+  const bool mixed = sizeof(T) == 4;
+  for (int q = 0; q < nsys; q++) {
+    NbHandle *h = hs[order[q]];
+    BatchSys &S = pl->h_sys[q];
+    memset(&S, 0, sizeof(S));
+    S.n = h->n;
+    S.K0 = h->prm.do_pme ? h->K[0] : 1;
+    S.K1 = h->prm.do_pme ? h->K[1] : 1;
+    S.K2 = h->prm.do_pme ? h->K[2] : 1;
+    S.maxa = h->fmaxa;
+    S.maxb = h->fmaxb;
+    if (h->prm.do_pme) {
+      S.Lx = std::max(1, std::min(FFT_CAP / S.K0, S.K1 * S.K2));
+      S.Ly = std::max(1, std::min(FFT_CAP / S.K1, S.K0));
+      S.Lz = std::max(1, std::min(FFT_CAP / S.K2, S.K0 * S.K1));
+      for (int d = 0; d < 3; d++) S.rad[d] = nb_factor(h->K[d]);
+    }
+    S.lcut = std::max(h->prm.ewald_cutoff, h->prm.vdw_cutoff) + h->prm.list_buffer;
+    S.cell = h->cell;
+    S.cell.multi = 1;
+    S.cell.kmask = (1u << NB_IMG_SHIFT) - 1u;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) S.fr.a[a][b] = h->afrac[a][b];
+    S.xyz = h->d_xyz;
+    S.mloc = h->d_mpole_local;
+    S.polar = h->d_polar0;
+    S.thole = h->d_thole0;
+    S.ipd = h->d_ipd0;
+    S.redval = h->d_redval;
+    S.vtab = h->d_vdw_table;
+    S.frame = h->d_frame;
+    S.axis = h->d_axis;
+    S.group = h->d_group0;
+    S.vclass = h->d_vclass0;
+    S.redidx = h->d_redidx;
+    S.moff = h->d_mask_off;
+    S.matom = h->d_mask_atom;
+    S.mrel = h->d_mask_rel;
+    S.nclass = h->n_class;
+    S.recip = mixed ? (const void *)h->d_recipf : (const void *)h->d_recip;
+    S.tw = mixed ? (const void *)h->d_twf : (const void *)h->d_tw;
+    S.grad = h->d_grad;
+    S.u = h->d_u;
+    S.up = h->d_up;
+    S.ws = pl->ws + off[q];
+  }
+  HIPCHECK(hipMemcpyAsync(pl->d_sys, pl->h_sys, sizeof(BatchSys) * nsys, hipMemcpyHostToDevice, pl->stream));
+  // the handles' own streams may still hold position uploads
+  for (NbHandle *h : hs) {
+    hipError_t q = hipStreamQuery(h->stream);
+    if (q == hipErrorNotReady) {
+      HIPCHECK(hipEventRecord(pl->ready, h->stream));
+      HIPCHECK(hipStreamWaitEvent(pl->stream, pl->ready, 0));
+    } else if (q != hipSuccess) {
+      HIPCHECK(q);
+    }
+  }
+  const int ti = mixed ? 1 : 0;
+  auto blocks_for = [&](int v) {
+    if (!pl->occ[ti][v]) {
+      int per = 0;
+      HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel_of<T>(v), kShapes[v].nt, 0));
+      pl->occ[ti][v] = std::max(per, 1);
+    }
+    return std::max(1, std::min(nsys, pl->occ[ti][v] * pl->cus));
+  };
+  // tuner: the first batch of a signature times every shape once (after a
+  // warm-up launch) and keeps the fastest
+  long natoms = 0;
+  for (NbHandle *h : hs) natoms += h->n;
+  std::string sig = std::to_string(ti) + ":" + std::to_string(nsys) + ":" + std::to_string(natoms);
+  auto it = pl->tuned.find(sig);
+  int best;
+  if (it == pl->tuned.end()) {
+    float bt = 0.0f;
+    best = 0;
+    for (int v = 0; v < kNumShapes; v++) {
+      launch_shape<T>(v, blocks_for(v), pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
+      HIPCHECK(hipEventRecord(pl->e0, pl->stream));
+      launch_shape<T>(v, blocks_for(v), pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
+      HIPCHECK(hipEventRecord(pl->e1, pl->stream));
+      HIPCHECK(hipEventSynchronize(pl->e1));
+      float ms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&ms, pl->e0, pl->e1));
+      if (v == 0 || ms < bt) {
+        bt = ms;
+        best = v;
+      }
+    }
+    pl->tuned[sig] = best;
+  } else {
+    best = it->second;
+  }
+  int blocks = blocks_for(best);
+  launch_shape<T>(best, blocks, pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
+  pl->last[0] = kShapes[best].nt;
+  pl->last[1] = kShapes[best].lpa;
+  pl->last[2] = blocks;
+  pl->last[3] = nsys;
+  HIPCHECK(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(double) * O_N * nsys, hipMemcpyDeviceToHost, pl->stream));
+  HIPCHECK(hipStreamSynchronize(pl->stream));
+  // rows that overflowed get larger capacities and the batch is repeated
+  int grow = 0;
+  for (int q = 0; q < nsys; q++) {
+    NbHandle *h = hs[order[q]];
+    const double *o = pl->h_out + (size_t)O_N * q;
+    int fl = (int)o[O_FLAGS];
+    if (fl & F_ROWS) {
+      h->fmaxa = std::max(h->fmaxa, (int)o[O_ROWA] + (int)o[O_ROWA] / 4 + 8);
+      h->fmaxb = std::max(h->fmaxb, (int)o[O_ROWB] + (int)o[O_ROWB] / 4 + 8);
+      grow = 1;
+    }
+    if (fl & F_CUTOFF) retry[order[q]] = 1;
+  }
+  return grow;
+}
+
+}  // namespace
+
+// Evaluate the handles with the batch kernel.  Returns false (nothing done)
+// when some handle does not qualify; otherwise fills out / the handles, and
+// marks in `separate` the handles whose evaluation must be repeated on the
+// regular path (a failed solve, which the regular path hands to SOR, or a
+// list cutoff beyond the checked image range).
+bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
+                       int &status) {
+  if (count <= 0) return false;
+  std::vector<NbHandle *> hs(hv, hv + count);
+  for (NbHandle *h : hs)
+    if (!h || !batchable(h, flags) || h->device != hs[0]->device || !same_params(h->prm, hs[0]->prm)) return false;
+  const nb_params &pr = hs[0]->prm;
+  HIPCHECK(hipSetDevice(hs[0]->device));
+  Pool *pl = pool_of(hs[0]->device);
+  std::lock_guard<std::mutex> lk(pl->mu);
+  BatchPrm P;
+  memset(&P, 0, sizeof(P));
+  P.sc = scale_tab(hs[0]);
+  P.aewald = pr.ewald_alpha;
+  P.ewald_cut = pr.ewald_cutoff;
+  P.vdw_off = pr.vdw_cutoff;
+  P.vdw_cut = pr.vdw_taper * pr.vdw_cutoff;
+  P.pcut = pr.precond_cutoff;
+  P.psc = pr.precond_scale;
+  P.polar_eps = pr.polar_eps;
+  P.electric = pr.electric;
+  P.hdelta = pr.halgren_delta;
+  P.hgamma = pr.halgren_gamma;
+  P.polarization = pr.polarization;
+  P.maxit = pr.max_scf_iter > 0 ? pr.max_scf_iter : 1000;
+  P.do_vdw = pr.do_vdw;
+  P.do_pme = pr.do_pme;
+  P.recip = pr.do_pme && pr.ewald_alpha > 0.0;
+  for (NbHandle *h : hs) {
+    double lcut = std::max(pr.ewald_cutoff, pr.vdw_cutoff) + pr.list_buffer;
+    if (h->fmaxa <= 0) h->fmaxa = row_estimate(h, std::min(pr.precond_cutoff, pr.ewald_cutoff));
+    if (h->fmaxb <= 0) h->fmaxb = row_estimate(h, lcut);
+  }
+  // queue order: largest work first (pairs ~ n^2 / V, grid ~ K^3)
+  std::vector<int> order(count);
+  std::vector<double> cost(count);
+  for (int b = 0; b < count; b++) {
+    const NbHandle *h = hs[b];
+    double ntot = pr.do_pme ? (double)h->K[0] * h->K[1] * h->K[2] : 0.0;
+    cost[b] = (double)h->n * h->n / h->cell.vol * 40.0 + ntot * 2.0;
+    order[b] = b;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  separate.assign(count, 0);
+  for (int attempt = 0;; attempt++) {
+    int grow = pr.precision == NB_PREC_MIXED ? run_batch<float>(pl, hs, P, order, separate)
+                                              : run_batch<double>(pl, hs, P, order, separate);
+    if (!grow) break;
+    if (attempt == 2) throw std::runtime_error("batch row capacity could not be established");
+  }
+  status = NB_OK;
+  for (int q = 0; q < count; q++) {
+    int b = order[q];
+    NbHandle *h = hs[b];
+    const double *o = pl->h_out + (size_t)O_N * q;
+    int st = (int)o[O_STAT], fl = (int)o[O_FLAGS];
+    if (st != NB_OK || (fl & F_CUTOFF)) {
+      separate[b] = 1;
+      continue;
+    }
+    nb_energies e;
+    memset(&e, 0, sizeof(e));
+    e.perm_real = o[E_PERM_REAL];
+    e.pol_real = o[E_POL_REAL];
+    e.vdw = o[E_VDW];
+    if (P.recip) {
+      e.perm_self = o[E_PERM_SELF];
+      e.perm_recip = o[E_PERM_RECIP];
+      e.pol_self = o[E_POL_SELF];
+      e.pol_recip = o[E_POL_RECIP];
+      double qt = o[E_QTOT];
+      if (qt != 0.0) e.perm_charge_corr = -0.5 * pr.electric * M_PI * qt * qt / (h->cell.vol * pr.ewald_alpha * pr.ewald_alpha);
+    }
+    e.n_vdw = (int64_t)(o[O_NVDW] + 0.5) / 2;
+    e.n_perm = (int64_t)(o[O_NPERM] + 0.5) / 2 + (P.recip ? h->n : 0);
+    e.pcg_iters = (int)o[O_ITER];
+    e.pcg_eps = o[O_EPS];
+    e.list_rebuilt = 1;
+    e.perm = e.perm_real + e.perm_self + e.perm_recip + e.perm_charge_corr;
+    e.pol = e.pol_real + e.pol_self + e.pol_recip;
+    e.total = e.perm + e.pol + e.vdw;
+    int s = NB_OK;
+    if (fl & F_TORQUE) s = NB_ERR_BAD_TORQUE;
+    else if (!std::isfinite(e.total)) s = NB_ERR_NAN_ENERGY;
+    if (s != NB_OK) {
+      h->err = s == NB_ERR_BAD_TORQUE ? "undefined torque" : "NaN energy";
+      if (status == NB_OK) status = s;
+    }
+    h->want_vir = false;
+    h->vir_valid = false;
+    h->want_dudl = false;
+    h->dudl_valid = false;
+    h->induced_caller = true;
+    h->last = e;
+    if (out) out[b] = e;
+  }
+  return true;
+}
+
+int nb_batch_last_config(int device, int32_t cfg[4]) {
+  Pool *pl = pool_of(device);
+  std::lock_guard<std::mutex> lk(pl->mu);
+  for (int q = 0; q < 4; q++) cfg[q] = pl->last[q];
+  return NB_OK;
+}

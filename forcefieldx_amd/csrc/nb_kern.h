@@ -120,6 +120,15 @@ template <int LPA> static __device__ __forceinline__ void group_sum3(V3<double> 
   a.z = group_sum<LPA>(a.z);
 }
 
+// bonded relation of caller atoms i,k (0 if none): linear scan of i's CSR row
+static __device__ __forceinline__ int relation(const int32_t *__restrict__ off, const int32_t *__restrict__ atom,
+                                        const int32_t *__restrict__ rel, int i, int k) {
+  int b = off[i], e = off[i + 1];
+  for (int t = b; t < e; t++)
+    if (atom[t] == k) return rel[t];
+  return 0;
+}
+
 static __device__ __forceinline__ V3<double> ld3d(const double *x, int i) {
   return V3<double>{x[3 * i], x[3 * i + 1], x[3 * i + 2]};
 }

@@ -163,15 +163,6 @@ __global__ void k_gather_static(int n, const int32_t *__restrict__ perm, const d
   mp[s].thole = thole0[i];
 }
 
-// bonded relation of caller atoms i,k (0 if none): linear scan of i's CSR row
-__device__ __forceinline__ int relation(const int32_t *__restrict__ off, const int32_t *__restrict__ atom,
-                                        const int32_t *__restrict__ rel, int i, int k) {
-  int b = off[i], e = off[i + 1];
-  for (int t = b; t < e; t++)
-    if (atom[t] == k) return rel[t];
-  return 0;
-}
-
 // LPA lanes per sorted atom scan the neighbouring cells LPA candidates at a
 // time; hits are compacted in candidate order with a ballot, so each row lists
 // its neighbours in (cell, sorted index) order exactly as a serial scan would.

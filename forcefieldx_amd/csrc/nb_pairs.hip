@@ -21,6 +21,7 @@
 #include <stdexcept>
 
 #include "nb_kern.h"
+#include "nb_pair.h"
 
 #define NB_LPA 16
 
@@ -96,26 +97,6 @@ __global__ void __launch_bounds__(256) k_filter(int n, const double4 *__restrict
   }
 }
 
-// one permanent-field pair in arithmetic U, accumulated in FP64
-template <typename U>
-__device__ __forceinline__ void perm_field_pair(const double4 &pi, const AtomMP &ai, const double4 &pk,
-                                                const AtomMP &ak, int code, const CellD &cell, double aewald,
-                                                const ScaleTab &sc, V3<double> &fd, V3<double> &fp) {
-  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
-  double r2 = image(cell, dx, dy, dz);
-  MP<U> K = load_mp<U>(ak);
-  U r = sqrt((U)r2), rinv = (U)1 / r;
-  V3<U> R{(U)-dx, (U)-dy, (U)-dz};  // from k to i
-  U bn[4];
-  ewald_bn<U, 3>(r, rinv, (U)aewald, bn);
-  Thole<U> th = thole<U, false>(r, R, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
-  U rr3 = rinv * rinv * rinv, rr5 = (U)3 * rr3 * rinv * rinv, rr7 = (U)5 * rr5 * rinv * rinv;
-  U ds = (U)sc.d[code], ps = (U)sc.p[code];
-  acc3(fd, mp_field(R, K, bn[1] - ((U)1 - th.s3 * ds) * rr3, bn[2] - ((U)1 - th.s5 * ds) * rr5,
-                    bn[3] - ((U)1 - th.s7 * ds) * rr7));
-  acc3(fp, mp_field(R, K, bn[1] - ((U)1 - th.s3 * ps) * rr3, bn[2] - ((U)1 - th.s5 * ps) * rr5,
-                    bn[3] - ((U)1 - th.s7 * ps) * rr7));
-}
 
 // LIST 0: the whole Ewald row (FP64 form); LIST 1: the row without the masked
 // pairs, whose screened-minus-bare radial differences cancel catastrophically
@@ -175,27 +156,7 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
     const uint32_t *row = lst + (size_t)s * maxl;
     for (int t = lane; t < m; t += LPA) {
       uint32_t e = row[t];
-      int k = e & cell.kmask;
-      const DipRec<T> rk = rec[k];
-      double dx = rk.x - ri.x, dy = rk.y - ri.y, dz = rk.z - ri.z;
-      entry_shift(cell, e, dx, dy, dz);
-      double r2 = image(cell, dx, dy, dz);
-      T r = sqrt((T)r2), rinv = (T)1 / r;
-      V3<T> R{(T)dx, (T)dy, (T)dz};
-      Thole<T> th = thole<T, false>(r, R, ri.ipd, rk.ipd, ri.thole, rk.thole);
-      T rr3 = rinv * rinv * rinv, rr5 = (T)3 * rr3 * rinv * rinv;
-      T B1, B2;
-      if (PRE) {
-        B1 = th.s3 * rr3;
-        B2 = th.s5 * rr5;
-      } else {
-        T bn[3];
-        ewald_bn<T, 2>(r, rinv, (T)aewald, bn);
-        B1 = bn[1] - ((T)1 - th.s3) * rr3;
-        B2 = bn[2] - ((T)1 - th.s5) * rr5;
-      }
-      acc3(a, dip_field(R, V3<T>{rk.u[0], rk.u[1], rk.u[2]}, B1, B2));
-      acc3(b, dip_field(R, V3<T>{rk.v[0], rk.v[1], rk.v[2]}, B1, B2));
+      dip_field_pair<T, PRE>(ri, rec[e & cell.kmask], e, cell, aewald, a, b);
     }
   }
   group_sum3<LPA>(a);
@@ -206,71 +167,6 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
   }
 }
 
-// Strain derivative of one full-row visit: with r = x_k - x_i and f the i-side
-// gradient, dE/dr = -f and the pair term (dE/dr)_a r_b is shared by the two
-// visits of the pair.
-template <typename U>
-__device__ __forceinline__ void pair_strain(double *w9, const V3<U> &f, const V3<U> &r) {
-  double fa[3] = {(double)f.x, (double)f.y, (double)f.z}, rb[3] = {(double)r.x, (double)r.y, (double)r.z};
-#pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++) w9[3 * a + b] -= 0.5 * fa[a] * rb[b];
-}
-
-// Real-space energy, force and torque, split into a permanent kernel and a
-// polarization kernel (the combined pair function needs more than 128 VGPRs
-// and ran one wavefront per SIMD).  Both add into the same sorted gradient and
-// torque arrays, one after the other on the handle's stream.
-template <typename U>
-__device__ __forceinline__ void perm_energy_pair(const double4 &pi, const AtomMP &ai, const double4 &pk,
-                                                 const AtomMP &ak, int code, const CellD &cell, double aewald,
-                                                 const ScaleTab &sc, double &ep, long long &cnt, V3<double> &F,
-                                                 V3<double> &Tq, double *w9) {
-  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
-  double r2 = image(cell, dx, dy, dz);
-  MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
-  V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = sqrt((U)r2), rinv = (U)1 / rl;
-  U bn[6], rr[6];
-  ewald_bn<U, 5>(rl, rinv, (U)aewald, bn);
-  PermGeo<U> g;
-  perm_geometry(r, I, K, g);
-  V3<U> f{0, 0, 0}, tq{0, 0, 0};
-  U epair = perm_terms<U, true>(r, I, K, g, bn, (U)1, f, tq);
-  U s1 = (U)1 - (U)sc.m[code];
-  if (s1 != (U)0) {
-    coulomb_rr<U, 5>(rinv, rr);
-    epair -= s1 * perm_terms<U, true>(r, I, K, g, rr, -s1, f, tq);
-  }
-  if (epair != (U)0) cnt++;
-  ep += (double)epair;
-  acc3(F, f);
-  acc3(Tq, tq);
-  if (w9) pair_strain(w9, f, r);
-}
-
-template <typename U>
-__device__ __forceinline__ void pol_energy_pair(const double4 &pi, const AtomMP &ai, const double *u, const double *up,
-                                                int s, const double4 &pk, const AtomMP &ak, int k, int code,
-                                                const CellD &cell, double aewald, const ScaleTab &sc, int direct,
-                                                double &ei, V3<double> &F, V3<double> &Tq, double *w9) {
-  double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
-  double r2 = image(cell, dx, dy, dz);
-  MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
-  V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = sqrt((U)r2), rinv = (U)1 / rl;
-  U bn[5], rr[5];
-  ewald_bn<U, 4>(rl, rinv, (U)aewald, bn);
-  coulomb_rr<U, 4>(rinv, rr);
-  V3<U> f{0, 0, 0}, tq{0, 0, 0};
-  Thole<U> th = thole<U, true>(rl, r, (U)ai.ipd, (U)ak.ipd, (U)ai.thole, (U)ak.thole);
-  ei += (double)pol_terms<U, true>(r, I, K, ld3t<U>(u, s), ld3t<U>(up, s), ld3t<U>(u, k), ld3t<U>(up, k), bn, rr, th,
-                                   (U)sc.p[code], (U)sc.d[code], direct != 0, f, tq);
-  acc3(F, f);
-  acc3(Tq, tq);
-  if (w9) pair_strain(w9, f, r);
-}
 
 // Permanent pair under an OST phase.  A soft pair sees r_eff = sqrt(r^2 +
 // lalpha) in the radial functions and the unshifted r in the angular ones, and
@@ -492,25 +388,11 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
         spair = softi != sk || (softi && sk && ((sf.inter && moli != mol[k]) || (sf.intra && moli == mol[k])));
       }
       T sc1 = spair ? (T)sf.sc1 : (T)0, sc2 = spair ? (T)sf.sc2 : (T)1;
-      T rr = sqrt((T)r2);
       T ev = (T)(vs * pe[1]);
-      T rho = rr * irv;
-      T rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
-      T rd = rho + d1, rd2 = rd * rd, rd6 = rd2 * rd2 * rd2, rd7 = rd6 * rd;
-      T t1d = (T)1 / (sc1 + rd7), t2d = (T)1 / (sc1 + rho7 + gg);
-      T t1 = t1n * t1d, t2a = g1 * t2d, t2 = t2a - (T)2;
-      T eik = sc2 * ev * t1 * t2;
-      T tp = 1, dtp = 0;
-      if (r2 > cut2) {
-        T x = (rr - wcut) * iw, x2 = x * x;
-        tp = (T)1 + x2 * x * ((T)-10 + x * ((T)15 - (T)6 * x));
-        dtp = x2 * ((T)-30 + x * ((T)60 - (T)30 * x)) * iw;
-      }
+      Vdw147<T> v = vdw147<T>(r2, irv, ev, sc1, sc2, d1, g1, gg, t1n, wcut, iw, r2 > cut2);
+      T rr = v.rr, eik = v.eik, tp = v.tp, dtp = v.dtp, t1 = v.t1, t2 = v.t2, t2a = v.t2a;
+      T t1d = v.t1d, t2d = v.t2d, dt1 = v.dt1, dt2 = v.dt2, ds = v.ds;
       esum += (double)(eik * tp);
-      T dt1 = t1 * (T)7 * rd6 * irv * t1d;
-      T dt2 = t2a * (T)7 * rho6 * irv * t2d;
-      T dedr = -sc2 * ev * (dt1 * t2 + t1 * dt2);
-      T ds = (eik * dtp + dedr * tp) / rr;
       acc3(G, V3<T>{ds * (T)dx, ds * (T)dy, ds * (T)dz});
       if (LAM && spair) {
         T dsc1 = (T)sf.dsc1, dsc2 = (T)sf.dsc2, d2sc1 = (T)sf.d2sc1, d2sc2 = (T)sf.d2sc2;

@@ -31,19 +31,17 @@
 #include <vector>
 
 #include "nb_kern.h"
+#include "nb_atom.h"
+#include "nb_fft.h"
 
 namespace {
 
-struct Frac { double a[3][3]; };  // a[f][c] = K_f A[c][f]
 struct Frac2v { double a[3][3]; };  // cell.A (fractional = Cartesian . A)
 
-template <typename T> struct alignas(2 * sizeof(T)) CX { T x, y; };
 
 // ---------------------------------------------------------------------------
 // local frames
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ V3<double> ld3(const double *x, int i) { return V3<double>{x[3 * i], x[3 * i + 1], x[3 * i + 2]}; }
-__device__ __forceinline__ V3<double> unit(V3<double> a) { return (1.0 / sqrt(dot(a, a))) * a; }
 
 __global__ void k_rotate(int n, const double *__restrict__ xyz, const int32_t *__restrict__ frame,
                          const int32_t *__restrict__ axis, const double *__restrict__ mloc,
@@ -53,58 +51,16 @@ __global__ void k_rotate(int n, const double *__restrict__ xyz, const int32_t *_
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3<double> o = ld3(xyz, i);
-  int f = frame[i];
-  double R[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  if (f != NB_FRAME_NONE) {
-    int az = axis[3 * i], ax = axis[3 * i + 1], ay = axis[3 * i + 2];
-    V3<double> z, x;
-    V3<double> pz = ld3(xyz, az);
-    if (f == NB_FRAME_BISECTOR) {
-      z = unit(pz - o);
-      x = unit(ld3(xyz, ax) - o);
-      z = unit(x + z);
-      x = unit(x - dot(x, z) * z);
-    } else if (f == NB_FRAME_ZTHENBISECTOR) {
-      z = unit(pz - o);
-      V3<double> a = unit(ld3(xyz, ax) - o), b = unit(ld3(xyz, ay) - o);
-      x = unit(a + b);
-      x = unit(x - dot(x, z) * z);
-    } else if (f == NB_FRAME_ZONLY) {
-      z = unit(pz - o);
-      x = V3<double>{1.0, 0.0, 0.0};
-      double dt = z.x;
-      if (fabs(dt) > 0.866) { x = V3<double>{0.0, 1.0, 0.0}; dt = z.y; }
-      x = unit(x - dt * z);
-    } else if (f == NB_FRAME_THREEFOLD) {
-      V3<double> a = unit(pz - o), b = unit(ld3(xyz, ax) - o), c = unit(ld3(xyz, ay) - o);
-      z = unit(a + b + c);
-      x = unit(a - dot(a, z) * z);
-    } else {
-      z = unit(pz - o);
-      x = ld3(xyz, ax) - o;
-      x = unit(x - dot(x, z) * z);
-    }
-    V3<double> y = cross(z, x);
-    R[0][0] = x.x; R[1][0] = x.y; R[2][0] = x.z;
-    R[0][1] = y.x; R[1][1] = y.y; R[2][1] = y.z;
-    R[0][2] = z.x; R[1][2] = z.y; R[2][2] = z.z;
-  }
-  const double *m = mloc + 10 * i;
-  double d[3], Q[3][3] = {{m[4], m[7], m[8]}, {m[7], m[5], m[9]}, {m[8], m[9], m[6]}}, T[3][3], O[3][3];
-  for (int a = 0; a < 3; a++) d[a] = R[a][0] * m[1] + R[a][1] * m[2] + R[a][2] * m[3];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) T[a][b] = R[a][0] * Q[0][b] + R[a][1] * Q[1][b] + R[a][2] * Q[2][b];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) O[a][b] = T[a][0] * R[b][0] + T[a][1] * R[b][1] + T[a][2] * R[b][2];
+  double gl[10];
+  rotate_multipole(i, xyz, frame, axis, mloc, gl);
   int s = iperm[i];
   double *g = gmp + 10 * s;
-  g[0] = m[0]; g[1] = d[0]; g[2] = d[1]; g[3] = d[2];
-  g[4] = O[0][0]; g[5] = O[1][1]; g[6] = O[2][2]; g[7] = O[0][1]; g[8] = O[0][2]; g[9] = O[1][2];
+  for (int q = 0; q < 10; q++) g[q] = gl[q];
   const double t = 1.0 / 3.0;
   AtomMP &a = mp[s];
-  a.c = m[0]; a.dx = d[0]; a.dy = d[1]; a.dz = d[2];
-  a.qxx = O[0][0] * t; a.qyy = O[1][1] * t; a.qzz = O[2][2] * t;
-  a.qxy = O[0][1] * t; a.qxz = O[0][2] * t; a.qyz = O[1][2] * t;
+  a.c = gl[0]; a.dx = gl[1]; a.dy = gl[2]; a.dz = gl[3];
+  a.qxx = gl[4] * t; a.qyy = gl[5] * t; a.qzz = gl[6] * t;
+  a.qxy = gl[7] * t; a.qxz = gl[8] * t; a.qyz = gl[9] * t;
   // reduced vdW site (VanDerWaals.java:1510-1540)
   int h = redidx[i];
   double rv = redval[i];
@@ -132,7 +88,6 @@ struct Bricks { int K[3], nb[3], ext; };
 __host__ __device__ __forceinline__ int brick_of(int p, int K, int nb) { return ((p + 1) * nb - 1) / K; }
 __host__ __device__ __forceinline__ int brick_start(int b, int K, int nb) { return (b * K) / nb; }
 
-__device__ __forceinline__ int wrapg(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 
 // B-splines, grid start, fractional multipoles and brick binning
 __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const double *__restrict__ gmp, CellD cell,
@@ -160,18 +115,7 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
   int b = b3[0] + bk.nb[0] * (b3[1] + bk.nb[1] * b3[2]);
   bid[s] = b;
   brank[s] = atomicAdd(&bcount[b], 1);
-  const double *g = gmp + 10 * s;
-  double *f = fmp + 10 * s;
-  f[0] = g[0];
-  for (int j = 0; j < 3; j++) f[1 + j] = fr.a[j][0] * g[1] + fr.a[j][1] * g[2] + fr.a[j][2] * g[3];
-  double Q[3][3] = {{g[4], g[7], g[8]}, {g[7], g[5], g[9]}, {g[8], g[9], g[6]}}, T[3][3], F[3][3];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) T[a][b] = fr.a[a][0] * Q[0][b] + fr.a[a][1] * Q[1][b] + fr.a[a][2] * Q[2][b];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) F[a][b] = T[a][0] * fr.a[b][0] + T[a][1] * fr.a[b][1] + T[a][2] * fr.a[b][2];
-  const double t = 1.0 / 3.0;
-  f[4] = F[0][0] * t; f[5] = F[1][1] * t; f[6] = F[2][2] * t;
-  f[7] = 2.0 * F[0][1] * t; f[8] = 2.0 * F[0][2] * t; f[9] = 2.0 * F[1][2] * t;
+  frac_multipole(fr, gmp + 10 * s, fmp + 10 * s);
 }
 
 __global__ void k_zero_i(int m, int32_t *__restrict__ x) {
@@ -338,17 +282,6 @@ __global__ void __launch_bounds__(256) k_brick_combine(Bricks bk, const T *__res
   grid[g] = brick_point<T, NP>(bk, bbuf, cx, (int)(r % bk.K[1]), (int)(r / bk.K[1]));
 }
 
-template <typename T>
-__device__ __forceinline__ void phi_cart(const Frac &fr, const T *f, double *c) {
-  c[0] = f[0];
-  for (int j = 0; j < 3; j++) c[1 + j] = fr.a[0][j] * f[1] + fr.a[1][j] * f[2] + fr.a[2][j] * f[3];
-  double F[3][3] = {{f[4], f[7], f[8]}, {f[7], f[5], f[9]}, {f[8], f[9], f[6]}}, T3[3][3], C[3][3];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) T3[a][b] = fr.a[0][a] * F[0][b] + fr.a[1][a] * F[1][b] + fr.a[2][a] * F[2][b];
-  for (int a = 0; a < 3; a++)
-    for (int b = 0; b < 3; b++) C[a][b] = T3[a][0] * fr.a[0][b] + T3[a][1] * fr.a[1][b] + T3[a][2] * fr.a[2][b];
-  c[4] = C[0][0]; c[5] = C[1][1]; c[6] = C[2][2]; c[7] = C[0][1]; c[8] = C[0][2]; c[9] = C[1][2];
-}
 
 // Gather: 8 lanes per atom, lane kz < 5 sums one z plane (25 rows of 5
 // points) into fractional derivatives, then the lanes are combined with
@@ -356,225 +289,18 @@ __device__ __forceinline__ void phi_cart(const Frac &fr, const T *f, double *c) 
 // transform (out[30 s] = 20 fractional + 10 Cartesian).  Otherwise only the
 // first derivatives are formed and the Cartesian field components are
 // written to out[30 s + 21..23] (what the PCG iterations consume).
-template <typename T>
-__device__ __forceinline__ T group8(T v) {
-#pragma unroll
-  for (int o = 4; o > 0; o >>= 1) v += __shfl_xor(v, o, 8);
-  return v;
-}
-
 template <typename T, int NP, bool FULL>
 __global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
                                                 const CX<T> *__restrict__ grid, Frac fr, int K0, int K1, int K2,
                                                 double *__restrict__ out0, double *__restrict__ out1,
                                                 const double *__restrict__ ctl) {
   if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
-  constexpr int NC = FULL ? 20 : 4;
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
-  bool valid = s < n;
-  T phi[NP][NC];
-#pragma unroll
-  for (int p = 0; p < NP; p++)
-#pragma unroll
-    for (int c = 0; c < NC; c++) phi[p][c] = (T)0;
-  if (valid && lane < 5) {
-    const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *vd = tx + 40 + 4 * lane;
-    T v[4] = {(T)vd[0], (T)vd[1], (T)vd[2], (T)vd[3]};
-    T wx[5][4];
-#pragma unroll
-    for (int kx = 0; kx < 5; kx++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) wx[kx][q] = (T)tx[4 * kx + q];
-    int k = wrapg(igrid[3 * s + 2] + 1 + lane, K2);
-    int j0 = igrid[3 * s + 1] + 1, i0 = igrid[3 * s] + 1;
-    int ix[5];
-#pragma unroll
-    for (int kx = 0; kx < 5; kx++) ix[kx] = wrapg(i0 + kx, K0);
-    T tu[NP][10];
-#pragma unroll
-    for (int p = 0; p < NP; p++)
-#pragma unroll
-      for (int c = 0; c < 10; c++) tu[p][c] = (T)0;
-    for (int ky = 0; ky < 5; ky++) {
-      int j = wrapg(j0 + ky, K1);
-      size_t row = (size_t)K0 * (j + (size_t)K1 * k);
-      T t[NP][4];
-#pragma unroll
-      for (int p = 0; p < NP; p++) t[p][0] = t[p][1] = t[p][2] = t[p][3] = (T)0;
-#pragma unroll
-      for (int kx = 0; kx < 5; kx++) {
-        CX<T> q2 = grid[row + ix[kx]];
-        T qv[2] = {q2.x, q2.y};
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
-          T q = qv[p];
-          t[p][0] += q * wx[kx][0]; t[p][1] += q * wx[kx][1];
-          if (FULL) { t[p][2] += q * wx[kx][2]; t[p][3] += q * wx[kx][3]; }
-        }
-      }
-      const double *ud = ty + 4 * ky;
-      T uu[4] = {(T)ud[0], (T)ud[1], (T)ud[2], (T)ud[3]};
-#pragma unroll
-      for (int p = 0; p < NP; p++) {
-        tu[p][0] += t[p][0] * uu[0]; tu[p][1] += t[p][1] * uu[0]; tu[p][2] += t[p][0] * uu[1];
-        if (FULL) {
-          tu[p][3] += t[p][2] * uu[0]; tu[p][4] += t[p][1] * uu[1]; tu[p][5] += t[p][0] * uu[2];
-          tu[p][6] += t[p][3] * uu[0]; tu[p][7] += t[p][2] * uu[1]; tu[p][8] += t[p][1] * uu[2];
-          tu[p][9] += t[p][0] * uu[3];
-        }
-      }
-    }
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-      T *ph = phi[p];
-      const T *a = tu[p];
-      if (FULL) {
-        ph[0] = a[0] * v[0]; ph[1] = a[1] * v[0]; ph[2] = a[2] * v[0]; ph[3] = a[0] * v[1];
-        ph[4] = a[3] * v[0]; ph[5] = a[5] * v[0]; ph[6] = a[0] * v[2];
-        ph[7] = a[4] * v[0]; ph[8] = a[1] * v[1]; ph[9] = a[2] * v[1];
-        ph[10] = a[6] * v[0]; ph[11] = a[9] * v[0]; ph[12] = a[0] * v[3];
-        ph[13] = a[7] * v[0]; ph[14] = a[3] * v[1]; ph[15] = a[8] * v[0];
-        ph[16] = a[5] * v[1]; ph[17] = a[1] * v[2]; ph[18] = a[2] * v[2]; ph[19] = a[4] * v[1];
-      } else {
-        ph[1] = a[1] * v[0]; ph[2] = a[2] * v[0]; ph[3] = a[0] * v[1];
-      }
-    }
-  }
-#pragma unroll
-  for (int p = 0; p < NP; p++)
-#pragma unroll
-    for (int c = 0; c < NC; c++) phi[p][c] = group8(phi[p][c]);
-  if (valid && lane == 0) {
-    double *outs[2] = {out0, out1};
-#pragma unroll
-    for (int p = 0; p < NP; p++) {
-      double *o = outs[p] + 30 * (size_t)s;
-      if (FULL) {
-        for (int c = 0; c < 20; c++) o[c] = (double)phi[p][c];
-        phi_cart<T>(fr, phi[p], o + 20);
-      } else {
-        for (int c = 0; c < 3; c++)
-          o[21 + c] = fr.a[0][c] * (double)phi[p][1] + fr.a[1][c] * (double)phi[p][2] + fr.a[2][c] * (double)phi[p][3];
-      }
-    }
-  }
+  gather_atom<T, NP, FULL>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, out0, out1);
 }
 
-// ---------------------------------------------------------------------------
-// FFT: in-place Stockham autosort in LDS, radix 2/3/4/5.  A pass reads every
-// butterfly input of the workgroup into registers, synchronises, and writes
-// the outputs back to the same buffer, so one LDS buffer suffices.  Lines
-// live at buf[l * sl + e * se]; LINE_FAST puts consecutive threads on
-// consecutive lines (used when lines are the contiguous direction).
-// ---------------------------------------------------------------------------
-struct Radices { int r[12]; int nr; };
-
-template <typename T>
-__device__ __forceinline__ CX<T> cmul(CX<T> a, CX<T> b) {
-  return CX<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-template <typename T> __device__ __forceinline__ CX<T> cadd(CX<T> a, CX<T> b) { return CX<T>{a.x + b.x, a.y + b.y}; }
-template <typename T> __device__ __forceinline__ CX<T> csub(CX<T> a, CX<T> b) { return CX<T>{a.x - b.x, a.y - b.y}; }
-// i * s * a for s = +-1
-template <typename T> __device__ __forceinline__ CX<T> cis(CX<T> a, T s) { return CX<T>{-s * a.y, s * a.x}; }
-
-// y[m] = sum_k x[k] exp(s 2 pi i m k / R), s = +-1
-template <int R, typename T>
-__device__ __forceinline__ void small_dft(CX<T> *v, T s) {
-  if (R == 2) {
-    CX<T> a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  } else if (R == 3) {
-    const T c = (T)-0.5, sn = (T)0.86602540378443864676;  // cos, sin of 2 pi / 3
-    CX<T> t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-    CX<T> m{v[0].x + c * t.x, v[0].y + c * t.y};
-    CX<T> r = cis(CX<T>{sn * d.x, sn * d.y}, s);
-    v[0] = cadd(v[0], t);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
-  } else if (R == 4) {
-    CX<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]), c = cadd(v[1], v[3]), d = cis(csub(v[1], v[3]), s);
-    v[0] = cadd(a, c);
-    v[1] = cadd(b, d);
-    v[2] = csub(a, c);
-    v[3] = csub(b, d);
-  } else {
-    // cos and sin of 2 pi / 5 and 4 pi / 5
-    const T c1 = (T)0.30901699437494742410, s1 = (T)0.95105651629515357212;
-    const T c2 = (T)-0.80901699437494742410, s2 = (T)0.58778525229247312917;
-    CX<T> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]), t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
-    CX<T> m1{v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
-    CX<T> m2{v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
-    CX<T> r1 = cis(CX<T>{s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y}, s);
-    CX<T> r2 = cis(CX<T>{s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y}, s);
-    v[0] = cadd(v[0], cadd(t1, t2));
-    v[1] = cadd(m1, r1);
-    v[4] = csub(m1, r1);
-    v[2] = cadd(m2, r2);
-    v[3] = csub(m2, r2);
-  }
-}
-
-// one in-place pass; EPT bounds the elements per thread (N * nlines <= EPT * blockDim)
-template <int R, int EPT, bool LINE_FAST, typename T>
-__device__ __forceinline__ void lds_pass(CX<T> *buf, int N, int nlines, int sl, int se, int Ns, const CX<T> *tw, T s) {
-  constexpr int MAXB = (EPT + R - 1) / R;
-  const int nb = N / R, total = nlines * nb, stride = N / (Ns * R);
-  CX<T> v[MAXB][R];
-  int where[MAXB];
-#pragma unroll
-  for (int q = 0; q < MAXB; q++) {
-    int b = q * blockDim.x + threadIdx.x;
-    where[q] = -1;
-    if (b < total) {
-      int line, j;
-      if (LINE_FAST) { line = b % nlines; j = b / nlines; }
-      else { j = b % nb; line = b / nb; }
-      int jm = j % Ns;
-      const CX<T> *x = buf + line * sl;
-#pragma unroll
-      for (int k = 0; k < R; k++) {
-        CX<T> a = x[(j + k * nb) * se];
-        if (k > 0 && jm > 0) {
-          CX<T> w = tw[jm * k * stride];  // jm k stride < N
-          w.y *= s;
-          a = cmul(a, w);
-        }
-        v[q][k] = a;
-      }
-      small_dft<R, T>(v[q], s);
-      where[q] = line * sl + ((j / Ns) * Ns * R + jm) * se;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < MAXB; q++) {
-    if (where[q] >= 0) {
-#pragma unroll
-      for (int k = 0; k < R; k++) buf[where[q] + k * Ns * se] = v[q][k];
-    }
-  }
-  __syncthreads();
-}
-
-template <int EPT, bool LINE_FAST, typename T>
-__device__ void lds_fft(CX<T> *buf, int N, int nlines, int sl, int se, const Radices &rad, const CX<T> *tw, T s) {
-  int Ns = 1;
-  for (int p = 0; p < rad.nr; p++) {
-    int R = rad.r[p];
-    if (R == 4) lds_pass<4, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
-    else if (R == 2) lds_pass<2, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
-    else if (R == 3) lds_pass<3, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
-    else lds_pass<5, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
-    Ns *= R;
-  }
-}
-
-// lines along one axis, L lines per workgroup.  AXIS 0: contiguous rows;
-// AXIS 1: L consecutive x at fixed z; AXIS 2: L consecutive (x, y) line
-// indices, with the influence multiply and inverse transform fused (CONV).
+// lines along one axis, L lines per workgroup (fft_chunk of chunk blockIdx.x)
 template <typename T, int EPT, int AXIS, bool CONV>
 __global__ void __launch_bounds__(256) k_fft_lines(CX<T> *__restrict__ grid, int K0, int K1, int K2, int L, Radices rad,
                                                    const CX<T> *__restrict__ twg, T s, const T *__restrict__ recip) {
@@ -582,55 +308,7 @@ __global__ void __launch_bounds__(256) k_fft_lines(CX<T> *__restrict__ grid, int
   const int N = AXIS == 0 ? K0 : (AXIS == 1 ? K1 : K2);
   CX<T> *buf = (CX<T> *)smem_raw, *tw = buf + (size_t)L * N;
   for (int t = threadIdx.x; t < N; t += blockDim.x) tw[t] = twg[t];
-  // line groups: AXIS 0 -> L rows; AXIS 1 -> x block at fixed z; AXIS 2 -> L (x,y) indices
-  int cnt, first = 0, xb = 0, z = 0;
-  if (AXIS == 0) {
-    first = blockIdx.x * L;
-    cnt = min(L, K1 * K2 - first);
-  } else if (AXIS == 1) {
-    int nxb = (K0 + L - 1) / L;
-    xb = (blockIdx.x % nxb) * L;
-    z = blockIdx.x / nxb;
-    cnt = min(L, K0 - xb);
-  } else {
-    first = blockIdx.x * L;
-    cnt = min(L, K0 * K1 - first);
-  }
-  auto gidx = [&](int l, int e) -> size_t {
-    if (AXIS == 0) return (size_t)(first + l) * K0 + e;
-    if (AXIS == 1) return (size_t)(xb + l) + (size_t)K0 * (e + (size_t)K1 * z);
-    return (size_t)(first + l) + (size_t)K0 * K1 * e;
-  };
-  // LDS layout: AXIS 0 lines contiguous (buf[l N + e]); otherwise lines fastest (buf[e L + l])
-  for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
-    int l, e;
-    if (AXIS == 0) { l = t / N; e = t - l * N; }
-    else { e = t / L; l = t - e * L; }
-    CX<T> v{(T)0, (T)0};
-    if (l < cnt) v = grid[gidx(l, e)];
-    buf[t] = v;
-  }
-  __syncthreads();
-  if (AXIS == 0) lds_fft<EPT, false, T>(buf, N, L, N, 1, rad, tw, s);
-  else lds_fft<EPT, true, T>(buf, N, L, 1, L, rad, tw, s);
-  if (CONV) {
-    for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
-      int e = t / L, l = t - e * L;
-      if (l < cnt) {
-        T g = recip[(size_t)(first + l) + (size_t)K0 * K1 * e];
-        buf[t].x *= g;
-        buf[t].y *= g;
-      }
-    }
-    __syncthreads();
-    lds_fft<EPT, true, T>(buf, N, L, 1, L, rad, tw, -s);
-  }
-  for (int t = threadIdx.x; t < L * N; t += blockDim.x) {
-    int l, e;
-    if (AXIS == 0) { l = t / N; e = t - l * N; }
-    else { e = t / L; l = t - e * L; }
-    if (l < cnt) grid[gidx(l, e)] = buf[t];
-  }
+  fft_chunk<T, EPT, AXIS, CONV>(grid, K0, K1, K2, L, blockIdx.x, rad, buf, tw, s, recip);
 }
 
 Radices factor(int N) {
@@ -1306,3 +984,5 @@ void nb_grid_virial(NbHandle *h, bool polar) {
   if (mixed_grid(h)) grid_virial_t<float>(h, polar, (const CX<float> *)h->d_twf);
   else grid_virial_t<double>(h, polar, (const CX<double> *)h->d_tw);
 }
+
+Radices nb_factor(int N) { return factor(N); }

@@ -172,6 +172,8 @@ struct NbHandle {
   double *h_pinned = nullptr;
 
   nb_energies last;
+  int fmaxa = 0, fmaxb = 0;       // row capacities of the batch kernel (preconditioner part, the rest)
+  bool induced_caller = false;    // d_u / d_up hold caller order (written by the batch kernel)
 
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
   // replayed; the PCG then runs graph_iters iterations under device-side
@@ -224,6 +226,9 @@ void nb_zero_outputs(NbHandle *h);
 double *nb_vir_region(NbHandle *h, long nblocks);
 void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
+bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
+                       int &status);
+int nb_batch_last_config(int device, int32_t cfg[4]);
 
 #define HIPCHECK(x)                                                                  \
   do {                                                                               \

@@ -17,14 +17,14 @@ LIB_PATH = os.path.join(HERE, "_build", "libamoeba_nb.so")
 
 NB_OK, NB_ERR_NAN_ENERGY, NB_ERR_SCF_DIVERGED, NB_ERR_SCF_MAXITER, NB_ERR_BAD_TORQUE, NB_ERR_HIP, \
     NB_ERR_INVALID, NB_ERR_NO_DEVICE = range(8)
-NB_GRAD, NB_VIRIAL, NB_DUDL, NB_FINITE_PCG = 1, 2, 4, 8
+NB_GRAD, NB_VIRIAL, NB_DUDL, NB_FINITE_PCG, NB_BATCH_SEPARATE = 1, 2, 4, 8, 16
 NB_PREC_FP64, NB_PREC_MIXED = 0, 1
 
 EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_device", "nb_set_lambda",
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
-            "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch"]
+            "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -89,6 +89,7 @@ def load(path=LIB_PATH):
     L.nb_set_lambda.argtypes = [vp, C.c_double]
     L.nb_set_graph.argtypes = [vp, C.c_int]
     L.nb_evaluate_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(NbEnergies)]
+    L.nb_batch_config.argtypes = [C.c_int, C.POINTER(C.c_int32)]
     L.nb_graph_replays.argtypes = [vp]
     L.nb_struct_sizes.argtypes = [_ip]
     L.nb_struct_sizes.restype = C.c_int
@@ -262,6 +263,17 @@ def evaluate_batch(handles, flags=NB_GRAD):
     outs = (NbEnergies * len(handles))()
     rc = L.nb_evaluate_batch(arr, len(handles), flags, outs)
     return [o.as_dict() for o in outs], rc
+
+
+def batch_config(device=0):
+    """Shape of the last batch-kernel launch: threads per workgroup, lanes per
+    atom, persistent workgroups, systems."""
+    L = load()
+    cfg = (C.c_int32 * 4)()
+    rc = L.nb_batch_config(device, cfg)
+    if rc != NB_OK:
+        raise NbError(rc, "nb_batch_config failed")
+    return dict(threads=cfg[0], lanes=cfg[1], workgroups=cfg[2], systems=cfg[3])
 
 
 def params_from(p, grid, precision=None, device=0):

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 4
+#define NB_ABI_VERSION 5
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -50,7 +50,7 @@ enum { NB_POL_NONE = 0, NB_POL_DIRECT = 1, NB_POL_MUTUAL = 2 };
 enum { NB_PREC_FP64 = 0, NB_PREC_MIXED = 1 };
 
 /* nb_evaluate flags */
-enum { NB_GRAD = 1u, NB_VIRIAL = 2u, NB_DUDL = 4u, NB_FINITE_PCG = 8u };
+enum { NB_GRAD = 1u, NB_VIRIAL = 2u, NB_DUDL = 4u, NB_FINITE_PCG = 8u, NB_BATCH_SEPARATE = 16u };
 
 /* SCF algorithms (SCFAlgorithm.java) */
 enum { NB_SCF_PCG = 0, NB_SCF_SOR = 1 };
@@ -198,11 +198,25 @@ void nb_destroy(void *handle);
 int nb_set_graph(void *handle, int on);
 
 /* Evaluate count independent systems, one handle each, with the same flags
- * (no counterpart in the reference, which evaluates one system at a time):
- * handles in graph mode with a recorded graph are launched together, each on
- * its own stream, then completed; the others are evaluated in turn.  out
- * (may be NULL) receives count records.  Returns the first non-OK status. */
+ * (no counterpart in the reference, which evaluates one system at a time).
+ * When every handle qualifies -- periodic, at most 4096 atoms, grids of at
+ * most 64 per axis, the same parameters, flags of at most NB_GRAD, no
+ * alchemistry, no SOR -- one persistent launch evaluates all of them: a
+ * workgroup takes one system at a time (largest first) through the whole
+ * evaluation, with explicit image sums in every cell.  A handle whose solve
+ * fails there is evaluated again on the regular path (SOR fallback).  The
+ * kernel's workgroup size and lanes per atom are chosen by timing every
+ * candidate on the first batch of a given size (nb_batch_config reports the
+ * choice).  Otherwise, or with NB_BATCH_SEPARATE in flags, handles in graph
+ * mode with a recorded graph are launched together, each on its own stream,
+ * then completed; the others are evaluated in turn.  out (may be NULL)
+ * receives count records.  Returns the first non-OK status. */
 int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energies *out);
+
+/* The shape of the last batch-kernel launch on a device: cfg[0] threads per
+ * workgroup, cfg[1] lanes per atom in the pair loops, cfg[2] persistent
+ * workgroups, cfg[3] systems (no counterpart in the reference). */
+int nb_batch_config(int device, int32_t cfg[4]);
 
 /* Number of evaluations of this handle served by a graph replay. */
 int nb_graph_replays(void *handle);
