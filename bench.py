@@ -250,42 +250,68 @@ def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, grap
     return (t1 - t0) / steps, last, g, u, prof, extra
 
 
-def time_batch(systems, params, prec, steps, warmup, graph=True):
-    """One step = nb_evaluate_batch over every cell (graphs of all cells
-    launched together, each on its own stream)."""
+def time_batch(systems, params, prec, steps, warmup, kernel=True):
+    """One step = nb_evaluate_batch over every cell.  kernel: the single
+    persistent launch of the batch kernel (the default path); otherwise the
+    separate path (NB_BATCH_SEPARATE), whose recorded graphs of all cells are
+    launched together, each on its own stream.  Returns the mean step time,
+    the energies, gradients, graph replays, PCG iterations and the batch
+    kernel's own launch durations (HIP events around each timed launch)."""
     from forcefieldx_amd import ffi
     hs = []
     for xyz, box, top, grid in systems:
         h = ffi.Handle(ffi.params_from(params, grid, prec), top)
         h.set_cell(box)
         h.set_positions(xyz)
-        if graph:
+        if not kernel:
             h.set_graph(True)
         hs.append(h)
+    flags = ffi.NB_GRAD | (0 if kernel else ffi.NB_BATCH_SEPARATE)
+    kms = []
     try:
         for _ in range(warmup):
-            ffi.evaluate_batch(hs, ffi.NB_GRAD)
+            ffi.evaluate_batch(hs, flags)
         r0 = sum(h.graph_replays() for h in hs)
         t0 = time.perf_counter()
         for _ in range(steps):
-            es, rc = ffi.evaluate_batch(hs, ffi.NB_GRAD)
+            es, rc = ffi.evaluate_batch(hs, flags)
             if rc != ffi.NB_OK:
                 raise RuntimeError(f"batch evaluation failed: {rc}")
+            if kernel:
+                kms.append(ffi.batch_config()["kernel_ms"])
         t1 = time.perf_counter()
         replays = sum(h.graph_replays() for h in hs) - r0
         grads = [h.gradient().reshape(-1) for h in hs]
         iters = [e["pcg_iters"] for e in es]
+        cfg = ffi.batch_config() if kernel else None
     finally:
         for h in hs:
             h.close()
-    return (t1 - t0) / steps, es, np.concatenate(grads), replays, iters
+    return (t1 - t0) / steps, es, np.concatenate(grads), replays, iters, kms, cfg
+
+
+def batch_bytes(systems, iters, s):
+    """Algorithmic HBM bytes of one sweep, summed over the cells: per cell
+    12 s K^3 (I + 3) for the grid passes, N (22 s + 48)(2 I + 5) for the atom
+    records and fields of the real-space passes, N (4 s + 24) for vdW,
+    N s (73 + 12 I) for the spline and potential streams and 384 N I for the
+    FP64 PCG vectors, with I the cell's PCG iterations."""
+    tot = 0.0
+    for (xyz, box, top, grid), I in zip(systems, iters):
+        n, K3 = top.n, grid[0] * grid[1] * grid[2]
+        tot += (12 * s * K3 * (I + 3) + n * (22 * s + 48) * (2 * I + 5) + n * (4 * s + 24) + n * s * (73 + 12 * I)
+                + 384 * n * I)
+    return tot
 
 
 def batch_main(args, systems, params):
     from forcefieldx_amd import ffi
-    graph = not args.no_graph
-    t64, e64, g64, r64, it64 = time_batch(systems, params, ffi.NB_PREC_FP64, args.steps, args.warmup, graph)
-    tmx, emx, gmx, rmx, itmx = time_batch(systems, params, ffi.NB_PREC_MIXED, args.steps, args.warmup, graph)
+    t64, e64, g64, _, it64, k64, cfg64 = time_batch(systems, params, ffi.NB_PREC_FP64, args.steps, args.warmup)
+    tmx, emx, gmx, _, itmx, kmx, cfgmx = time_batch(systems, params, ffi.NB_PREC_MIXED, args.steps, args.warmup)
+    tsep = gsep = rsep = None
+    if not args.no_separate:
+        sep_steps = min(args.steps, 3)
+        tsep, _, gsep, rsep, _, _, _ = time_batch(systems, params, ffi.NB_PREC_MIXED, sep_steps, 1, kernel=False)
     natoms = sum(s[2].n for s in systems)
     cpu = None
     if not args.no_cpu:
@@ -305,19 +331,33 @@ def batch_main(args, systems, params):
         cpu = {"value": done / dt / len(systems), "unit": "sweeps/s", "cores": nth, "kind": "port",
                "sample": f"{done} of the {len(systems)} cells evaluated one after the other by the FP64 oracle "
                          f"({dt:.1f} s), scaled to a full sweep"}
+    kavg = float(np.mean(kmx))
+    bytes_sweep = batch_bytes(systems, itmx, 4)
+    gbs = bytes_sweep / (kavg * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+            "traffic": None, "kernel": "k_batch", "avg_launch_ms": kavg, "algorithmic_bytes_per_launch": bytes_sweep,
+            "bytes_model": "sum over cells of the whole-evaluation byte count (grid passes, atom records and "
+                           "fields, vdW, spline and potential streams, FP64 PCG vectors) at 4-byte reals",
+            "timed_with": "HIP events around each batch-kernel launch of the timed region (library stream)"}
     out = {
         "metric": METRIC, "value": 1.0 / tmx, "unit": "sweeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": tmx * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG", "data": "synthetic",
         "config": {"workload": f"crystals-512: {WORKLOADS['crystals-512']}", "cells": len(systems), "atoms": natoms,
-                   "launch": "HIP graph replay, all cells launched together" if graph else "regular launches",
+                   "launch": "one persistent launch of the batch kernel per sweep",
+                   "kernel_shape": {k: cfgmx[k] for k in ("threads", "lanes", "workgroups")},
                    "parallelism": "single GPU"},
         "cell_evals_per_s": len(systems) / tmx,
         "fp64_ms_per_step": t64 * 1e3, "mixed_over_fp64": t64 / tmx,
+        "kernel_ms": {"mixed": kavg, "fp64": float(np.mean(k64))},
+        "separate_path_ms_per_step": tsep * 1e3 if tsep else None,
+        "separate_path_graph_replays": rsep,
+        "kernel_over_separate_path": tsep / tmx if tsep else None,
         "force_rel_rms_err": float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64)),
-        "graph_replays": {"mixed": rmx, "fp64": r64},
+        "separate_vs_kernel_force_rel_err": (float(np.linalg.norm(gsep - gmx) / np.linalg.norm(gmx))
+                                             if gsep is not None else None),
         "pcg_iters_mean": {"mixed": float(np.mean(itmx)), "fp64": float(np.mean(it64))},
-        "roofline": None,
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     if cpu:
@@ -398,6 +438,8 @@ def main():
     ap.add_argument("--workload", default="water-20736", choices=list(WORKLOADS))
     ap.add_argument("--no-graph", action="store_true", help="time the regular launch path instead of graph replays")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-separate", action="store_true",
+                    help="crystals-512: skip timing the separate (graph per cell) path beside the batch kernel")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device-to-device copy of known size (reference point for "

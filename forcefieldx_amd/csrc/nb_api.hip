@@ -1037,10 +1037,10 @@ int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energi
   return status;
 }
 
-int nb_batch_config(int device, int32_t cfg[4]) {
+int nb_batch_config(int device, int32_t cfg[4], double *kernel_ms) {
   if (!cfg) return NB_ERR_INVALID;
   try {
-    return nb_batch_last_config(device, cfg);
+    return nb_batch_last_config(device, cfg, kernel_ms);
   } catch (std::exception &) {
     return NB_ERR_HIP;
   }

@@ -134,7 +134,124 @@ enum { O_ITER = 8, O_EPS, O_STAT, O_NVDW, O_NPERM, O_FLAGS, O_ROWA, O_ROWB, O_N 
 // O_FLAGS bits
 enum { F_ROWS = 1, F_TORQUE = 2, F_CUTOFF = 4 };
 
-constexpr int FFT_CAP = 2048;   // LDS line-buffer elements per workgroup
+// LDS line-buffer elements per workgroup: two 28 KB buffers (ping-pong), so
+// that the workgroup's LDS stays below 64 KB.  (A 135 KB layout, within the
+// 160 KB the device reports per workgroup, gave wrong results: the variables
+// placed after the buffers read back altered.)
+template <typename T> struct FftCap { static constexpr int value = 28672 / sizeof(CX<T>); };
+
+// One out-of-place Stockham pass of radix R over nlines lines of length N
+// (line l at l * sl, element e at e * se), src -> dst.  Every butterfly is
+// read, combined and written by one thread with nothing held across the
+// pass, so a chunk of any size costs no registers beyond one butterfly.
+template <int R, bool LINE_FAST, typename T>
+__device__ __forceinline__ void pp_pass(const CX<T> *src, CX<T> *dst, int N, int nlines, int sl, int se, int Ns,
+                                        const CX<T> *tw, T s) {
+  const int nb = N / R, total = nlines * nb, stride = N / (Ns * R);
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    int line, j;
+    if (LINE_FAST) { line = b % nlines; j = b / nlines; }
+    else { j = b % nb; line = b / nb; }
+    int jm = j % Ns;
+    const CX<T> *x = src + line * sl;
+    CX<T> v[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      CX<T> a = x[(j + k * nb) * se];
+      if (k > 0 && jm > 0) {
+        CX<T> w = tw[jm * k * stride];
+        w.y *= s;
+        a = cmul(a, w);
+      }
+      v[k] = a;
+    }
+    small_dft<R, T>(v, s);
+    CX<T> *y = dst + line * sl + ((j / Ns) * Ns * R + jm) * se;
+#pragma unroll
+    for (int k = 0; k < R; k++) y[k * Ns * se] = v[k];
+  }
+  __syncthreads();
+}
+
+// all passes of one transform; returns the buffer that holds the result
+template <bool LINE_FAST, typename T>
+__device__ CX<T> *pp_fft(CX<T> *a, CX<T> *b, int N, int nlines, int sl, int se, const Radices &rad,
+                         const CX<T> *tw, T s) {
+  int Ns = 1;
+  for (int p = 0; p < rad.nr; p++) {
+    int R = rad.r[p];
+    if (R == 4) pp_pass<4, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
+    else if (R == 2) pp_pass<2, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
+    else if (R == 3) pp_pass<3, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
+    else pp_pass<5, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
+    Ns *= R;
+    CX<T> *t = a;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// One chunk of L lines of the system's grid through LDS (AXIS 0: rows; AXIS
+// 3: lines along y over consecutive (x, z) indices; AXIS 2: lines along z over
+// consecutive (x, y) indices, with the influence multiply and the inverse
+// transform fused when CONV).
+template <typename T, int NT, int AXIS, bool CONV>
+__device__ void pp_chunk(CX<T> *__restrict__ grid, int K0, int K1, int K2, int L, int chunk, const Radices &rad,
+                         CX<T> *b0, CX<T> *b1, const CX<T> *tw, T s, const T *__restrict__ recip) {
+  const int N = AXIS == 0 ? K0 : (AXIS == 3 ? K1 : K2);
+  const int nlt = AXIS == 0 ? K1 * K2 : (AXIS == 3 ? K0 * K2 : K0 * K1);
+  const int first = chunk * L, cnt = min(L, nlt - first);
+  auto gidx = [&](int l, int e) -> size_t {
+    int q = first + l;
+    if (AXIS == 0) return (size_t)q * K0 + e;
+    if (AXIS == 3) return (size_t)(q % K0) + (size_t)K0 * (e + (size_t)K1 * (q / K0));
+    return (size_t)q + (size_t)K0 * K1 * e;
+  };
+  // every load of the chunk is issued before the first is consumed (one
+  // memory latency per chunk, not one per element)
+  {
+    constexpr int MAXE = (FftCap<T>::value + NT - 1) / NT;
+    CX<T> v[MAXE];
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      int t = q * NT + threadIdx.x;
+      if (t < cnt * N) {
+        int l, e;
+        if (AXIS == 0) { l = t / N; e = t - l * N; }
+        else { e = t / cnt; l = t - e * cnt; }
+        v[q] = grid[gidx(l, e)];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < MAXE; q++) {
+      int t = q * NT + threadIdx.x;
+      if (t < cnt * N) b0[t] = v[q];
+    }
+  }
+  __syncthreads();
+  CX<T> *r;
+  if (AXIS == 0) r = pp_fft<false, T>(b0, b1, N, cnt, N, 1, rad, tw, s);
+  else r = pp_fft<true, T>(b0, b1, N, cnt, 1, cnt, rad, tw, s);
+  if (CONV) {
+    for (int t = threadIdx.x; t < cnt * N; t += blockDim.x) {
+      int e = t / cnt, l = t - e * cnt;
+      T g = recip[(size_t)(first + l) + (size_t)K0 * K1 * e];
+      r[t].x *= g;
+      r[t].y *= g;
+    }
+    __syncthreads();
+    CX<T> *o = r == b0 ? b1 : b0;
+    r = pp_fft<true, T>(r, o, N, cnt, 1, cnt, rad, tw, -s);
+  }
+  for (int t = threadIdx.x; t < cnt * N; t += blockDim.x) {
+    int l, e;
+    if (AXIS == 0) { l = t / N; e = t - l * N; }
+    else { e = t / cnt; l = t - e * cnt; }
+    grid[gidx(l, e)] = r[AXIS == 0 ? l * N + e : e * cnt + l];
+  }
+  __syncthreads();
+}
 constexpr int TW_CAP = 3 * 64;  // LDS twiddle entries (three axes of at most 64)
 
 // sums of NV values over the workgroup, returned to every thread (fixed order)
@@ -179,10 +296,15 @@ __device__ __forceinline__ void block_max(double (&v)[NV], double *red) {
   }
 }
 
-// Phases read what other threads of the workgroup wrote to global memory,
-// partly through atomics performed in L2: the barrier is completed by a
-// device-scope fence.
-__device__ __forceinline__ void phase_sync() {
+// Phases read what other threads of the workgroup wrote to global memory
+// with plain stores: the workgroup barrier orders and publishes them.
+__device__ __forceinline__ void phase_sync() { __syncthreads(); }
+
+// The grid is spread with atomics, which are performed in L2 and leave any
+// copy of the line in the CU's vector L1 as it was: the device-scope fence
+// waits for the cleared grid to reach L2 before the atomics, and for the
+// atomics, with L1 invalidated, before the transform reads the grid.
+__device__ __forceinline__ void atomic_sync() {
   __threadfence();
   __syncthreads();
 }
@@ -504,46 +626,17 @@ __device__ void grid_spread(const BatchSys &S, const Ws<T> &w, const double *__r
 // forward x, forward y, fused forward z / influence / inverse z, inverse y,
 // inverse x (un-normalised inverse, Complex.java:436-470)
 template <typename T, int NT>
-__device__ void grid_convolve(const BatchSys &S, const Ws<T> &w, CX<T> *buf, const CX<T> *tw) {
-  constexpr int EPT = FFT_CAP / NT;
+__device__ void grid_convolve(const BatchSys &S, const Ws<T> &w, CX<T> *b0, CX<T> *b1, const CX<T> *tw) {
   const int K0 = S.K0, K1 = S.K1, K2 = S.K2;
   const CX<T> *twx = tw, *twy = tw + K0, *twz = tw + K0 + K1;
   const T *recip = (const T *)S.recip;
   const T *nul = nullptr;
-  for (int pass = 0; pass < 2; pass++) {
-    T s = pass == 0 ? (T)1 : (T)-1;
-    if (pass == 0) {
-      int nch = (K1 * K2 + S.Lx - 1) / S.Lx;
-      for (int c = 0; c < nch; c++) {
-        fft_chunk<T, EPT, 0, false>(w.grid, K0, K1, K2, S.Lx, c, S.rad[0], buf, twx, s, nul);
-        __syncthreads();
-      }
-      phase_sync();
-    }
-    {
-      int nch = ((K0 + S.Ly - 1) / S.Ly) * K2;
-      for (int c = 0; c < nch; c++) {
-        fft_chunk<T, EPT, 1, false>(w.grid, K0, K1, K2, S.Ly, c, S.rad[1], buf, twy, s, nul);
-        __syncthreads();
-      }
-      phase_sync();
-    }
-    if (pass == 0) {
-      int nch = (K0 * K1 + S.Lz - 1) / S.Lz;
-      for (int c = 0; c < nch; c++) {
-        fft_chunk<T, EPT, 2, true>(w.grid, K0, K1, K2, S.Lz, c, S.rad[2], buf, twz, (T)1, recip);
-        __syncthreads();
-      }
-      phase_sync();
-    } else {
-      int nch = (K1 * K2 + S.Lx - 1) / S.Lx;
-      for (int c = 0; c < nch; c++) {
-        fft_chunk<T, EPT, 0, false>(w.grid, K0, K1, K2, S.Lx, c, S.rad[0], buf, twx, s, nul);
-        __syncthreads();
-      }
-      phase_sync();
-    }
-  }
+  const int nx = (K1 * K2 + S.Lx - 1) / S.Lx, ny = (K0 * K2 + S.Ly - 1) / S.Ly, nz = (K0 * K1 + S.Lz - 1) / S.Lz;
+  for (int c = 0; c < nx; c++) pp_chunk<T, NT, 0, false>(w.grid, K0, K1, K2, S.Lx, c, S.rad[0], b0, b1, twx, (T)1, nul);
+  for (int c = 0; c < ny; c++) pp_chunk<T, NT, 3, false>(w.grid, K0, K1, K2, S.Ly, c, S.rad[1], b0, b1, twy, (T)1, nul);
+  for (int c = 0; c < nz; c++) pp_chunk<T, NT, 2, true>(w.grid, K0, K1, K2, S.Lz, c, S.rad[2], b0, b1, twz, (T)1, recip);
+  for (int c = 0; c < ny; c++) pp_chunk<T, NT, 3, false>(w.grid, K0, K1, K2, S.Ly, c, S.rad[1], b0, b1, twy, (T)-1, nul);
+  for (int c = 0; c < nx; c++) pp_chunk<T, NT, 0, false>(w.grid, K0, K1, K2, S.Lx, c, S.rad[0], b0, b1, twx, (T)-1, nul);
 }
 
 template <typename T, int NT, int NP, bool FULL>
@@ -551,7 +644,9 @@ __device__ void grid_gather(const BatchSys &S, const Ws<T> &w, double *o0, doubl
   const int groups = NT / 8, g = threadIdx.x / 8, lane = threadIdx.x % 8;
   for (int base = 0; base < S.n; base += groups) {
     int i = base + g;
-    gather_atom<T, NP, FULL>(i, lane, i < S.n, w.theta, w.igrid, w.grid, S.fr, S.K0, S.K1, S.K2, o0, o1);
+    // the FP32 form prefetches the stencil plane (in FP64 the extra registers spill)
+    gather_atom<T, NP, FULL, sizeof(T) == 4>(i, lane, i < S.n, w.theta, w.igrid, w.grid, S.fr, S.K0, S.K1, S.K2, o0,
+                                             o1);
   }
 }
 
@@ -561,10 +656,10 @@ template <typename T, int NT, bool FULL>
 __device__ void recip_induced(const BatchSys &S, const Ws<T> &w, const double *s0, const double *s1, CX<T> *buf,
                               const CX<T> *tw) {
   grid_zero<T, NT>(S, w);
-  phase_sync();
+  atomic_sync();
   grid_spread<T, NT, true>(S, w, s0, s1);
-  phase_sync();
-  grid_convolve<T, NT>(S, w, buf, tw);
+  atomic_sync();
+  grid_convolve<T, NT>(S, w, buf, buf + FftCap<T>::value, tw);
   grid_gather<T, NT, 2, FULL>(S, w, w.phiu, w.phiup);
   phase_sync();
 }
@@ -884,10 +979,10 @@ __device__ void run_system(const BatchSys &S, const BatchPrm &P, const Ws<T> &w,
     phase_perm_field<T, NT, LPA>(S, P, w);
     if (P.recip) {
       grid_zero<T, NT>(S, w);
-      phase_sync();
+      atomic_sync();
       grid_spread<T, NT, false>(S, w, nullptr, nullptr);
-      phase_sync();
-      grid_convolve<T, NT>(S, w, buf, tw);
+      atomic_sync();
+      grid_convolve<T, NT>(S, w, buf, buf + FftCap<T>::value, tw);
       grid_gather<T, NT, 1, true>(S, w, w.phip, nullptr);
     }
     phase_sync();
@@ -968,7 +1063,7 @@ __device__ void run_system(const BatchSys &S, const BatchPrm &P, const Ws<T> &w,
 template <typename T, int NT, int LPA>
 __global__ void __launch_bounds__(NT) k_batch(const BatchSys *__restrict__ sys, int nsys, BatchPrm P,
                                                double *__restrict__ out, int *__restrict__ next) {
-  __shared__ __align__(16) CX<T> buf[FFT_CAP];
+  __shared__ __align__(16) CX<T> buf[2 * FftCap<T>::value];
   __shared__ __align__(16) CX<T> tw[TW_CAP];
   __shared__ double red[16 * (NT / 64)];
   // the system's descriptor, its workspace pointers and the parameters live
@@ -1056,6 +1151,7 @@ struct Pool {
   std::map<std::string, int> tuned;
   int occ[2][kNumShapes] = {};
   int last[4] = {0, 0, 0, 0};
+  double last_ms = 0.0;   // duration of the last launch (HIP events on the pool stream)
 };
 
 std::mutex g_pools_mu;
@@ -1163,9 +1259,10 @@ static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::ve
     S.maxa = h->fmaxa;
     S.maxb = h->fmaxb;
     if (h->prm.do_pme) {
-      S.Lx = std::max(1, std::min(FFT_CAP / S.K0, S.K1 * S.K2));
-      S.Ly = std::max(1, std::min(FFT_CAP / S.K1, S.K0));
-      S.Lz = std::max(1, std::min(FFT_CAP / S.K2, S.K0 * S.K1));
+      const int cap = FftCap<T>::value;
+      S.Lx = std::max(1, std::min(cap / S.K0, S.K1 * S.K2));
+      S.Ly = std::max(1, std::min(cap / S.K1, S.K0 * S.K2));
+      S.Lz = std::max(1, std::min(cap / S.K2, S.K0 * S.K1));
       for (int d = 0; d < 3; d++) S.rad[d] = nb_factor(h->K[d]);
     }
     S.lcut = std::max(h->prm.ewald_cutoff, h->prm.vdw_cutoff) + h->prm.list_buffer;
@@ -1245,13 +1342,20 @@ static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::ve
     best = it->second;
   }
   int blocks = blocks_for(best);
+  HIPCHECK(hipEventRecord(pl->e0, pl->stream));
   launch_shape<T>(best, blocks, pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
+  HIPCHECK(hipEventRecord(pl->e1, pl->stream));
   pl->last[0] = kShapes[best].nt;
   pl->last[1] = kShapes[best].lpa;
   pl->last[2] = blocks;
   pl->last[3] = nsys;
   HIPCHECK(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(double) * O_N * nsys, hipMemcpyDeviceToHost, pl->stream));
   HIPCHECK(hipStreamSynchronize(pl->stream));
+  {
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, pl->e0, pl->e1));
+    pl->last_ms = ms;
+  }
   // rows that overflowed get larger capacities and the batch is repeated
   int grow = 0;
   for (int q = 0; q < nsys; q++) {
@@ -1374,9 +1478,10 @@ bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *ou
   return true;
 }
 
-int nb_batch_last_config(int device, int32_t cfg[4]) {
+int nb_batch_last_config(int device, int32_t cfg[4], double *kernel_ms) {
   Pool *pl = pool_of(device);
   std::lock_guard<std::mutex> lk(pl->mu);
   for (int q = 0; q < 4; q++) cfg[q] = pl->last[q];
+  if (kernel_ms) *kernel_ms = pl->last_ms;
   return NB_OK;
 }

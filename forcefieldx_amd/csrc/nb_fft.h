@@ -122,16 +122,21 @@ __device__ void lds_fft(CX<T> *buf, int N, int nlines, int sl, int se, const Rad
 // One chunk of L lines along one axis, processed by the whole workgroup:
 // AXIS 0 -> L consecutive rows; AXIS 1 -> an x block of L at fixed z; AXIS 2 ->
 // L consecutive (x, y) line indices, with the influence multiply and inverse
-// transform fused (CONV).  buf holds L * N values, tw the N twiddles.
+// transform fused (CONV); AXIS 3 -> lines along y over L consecutive (x, z)
+// line indices x + K0 z (several z-planes per chunk).  buf holds L * N
+// values, tw the N twiddles.
 template <typename T, int EPT, int AXIS, bool CONV>
 __device__ __forceinline__ void fft_chunk(CX<T> *__restrict__ grid, int K0, int K1, int K2, int L, int chunk,
                                           const Radices &rad, CX<T> *buf, const CX<T> *tw, T s,
                                           const T *__restrict__ recip) {
-  const int N = AXIS == 0 ? K0 : (AXIS == 1 ? K1 : K2);
+  const int N = (AXIS == 0) ? K0 : ((AXIS == 1 || AXIS == 3) ? K1 : K2);
   int cnt, first = 0, xb = 0, z = 0;
   if (AXIS == 0) {
     first = chunk * L;
     cnt = min(L, K1 * K2 - first);
+  } else if (AXIS == 3) {
+    first = chunk * L;
+    cnt = min(L, K0 * K2 - first);
   } else if (AXIS == 1) {
     int nxb = (K0 + L - 1) / L;
     xb = (chunk % nxb) * L;
@@ -143,6 +148,10 @@ __device__ __forceinline__ void fft_chunk(CX<T> *__restrict__ grid, int K0, int 
   }
   auto gidx = [&](int l, int e) -> size_t {
     if (AXIS == 0) return (size_t)(first + l) * K0 + e;
+    if (AXIS == 3) {
+      int q = first + l, x = q % K0, zz = q / K0;
+      return (size_t)x + (size_t)K0 * (e + (size_t)K1 * zz);
+    }
     if (AXIS == 1) return (size_t)(xb + l) + (size_t)K0 * (e + (size_t)K1 * z);
     return (size_t)(first + l) + (size_t)K0 * K1 * e;
   };
@@ -191,7 +200,7 @@ __device__ __forceinline__ T group8(T v) {
 // transform (out[30 s ..]), otherwise only the Cartesian field gradient
 // (out[30 s + 21 ..]).  NP = 2 reads u from the real and uCR from the
 // imaginary part.  Every lane of the 8 must call (cross-lane sums).
-template <typename T, int NP, bool FULL>
+template <typename T, int NP, bool FULL, bool PREFETCH = false>
 __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const double *__restrict__ theta,
                                             const int32_t *__restrict__ igrid, const CX<T> *__restrict__ grid,
                                             const Frac &fr, int K0, int K1, int K2, double *__restrict__ out0,
@@ -220,6 +229,18 @@ __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const d
     for (int p = 0; p < NP; p++)
 #pragma unroll
       for (int c = 0; c < 10; c++) tu[p][c] = (T)0;
+    // PREFETCH: the 25 grid values of the lane's plane are loaded before the
+    // first is used (one memory latency instead of five)
+    CX<T> pre[PREFETCH ? 5 : 1][PREFETCH ? 5 : 1];
+    if (PREFETCH) {
+#pragma unroll
+      for (int ky = 0; ky < 5; ky++) {
+        size_t row = (size_t)K0 * (wrapg(j0 + ky, K1) + (size_t)K1 * k);
+#pragma unroll
+        for (int kx = 0; kx < 5; kx++) pre[PREFETCH ? ky : 0][PREFETCH ? kx : 0] = grid[row + ix[kx]];
+      }
+    }
+#pragma unroll
     for (int ky = 0; ky < 5; ky++) {
       int j = wrapg(j0 + ky, K1);
       size_t row = (size_t)K0 * (j + (size_t)K1 * k);
@@ -228,7 +249,7 @@ __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const d
       for (int p = 0; p < NP; p++) t[p][0] = t[p][1] = t[p][2] = t[p][3] = (T)0;
 #pragma unroll
       for (int kx = 0; kx < 5; kx++) {
-        CX<T> q2 = grid[row + ix[kx]];
+        CX<T> q2 = PREFETCH ? pre[PREFETCH ? ky : 0][PREFETCH ? kx : 0] : grid[row + ix[kx]];
         T qv[2] = {q2.x, q2.y};
 #pragma unroll
         for (int p = 0; p < NP; p++) {

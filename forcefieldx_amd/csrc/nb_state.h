@@ -228,7 +228,7 @@ void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
 bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
                        int &status);
-int nb_batch_last_config(int device, int32_t cfg[4]);
+int nb_batch_last_config(int device, int32_t cfg[4], double *kernel_ms);
 
 #define HIPCHECK(x)                                                                  \
   do {                                                                               \

@@ -89,7 +89,7 @@ def load(path=LIB_PATH):
     L.nb_set_lambda.argtypes = [vp, C.c_double]
     L.nb_set_graph.argtypes = [vp, C.c_int]
     L.nb_evaluate_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(NbEnergies)]
-    L.nb_batch_config.argtypes = [C.c_int, C.POINTER(C.c_int32)]
+    L.nb_batch_config.argtypes = [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.nb_graph_replays.argtypes = [vp]
     L.nb_struct_sizes.argtypes = [_ip]
     L.nb_struct_sizes.restype = C.c_int
@@ -267,13 +267,14 @@ def evaluate_batch(handles, flags=NB_GRAD):
 
 def batch_config(device=0):
     """Shape of the last batch-kernel launch: threads per workgroup, lanes per
-    atom, persistent workgroups, systems."""
+    atom, persistent workgroups, systems, and its duration in ms."""
     L = load()
     cfg = (C.c_int32 * 4)()
-    rc = L.nb_batch_config(device, cfg)
+    ms = C.c_double(0.0)
+    rc = L.nb_batch_config(device, cfg, C.byref(ms))
     if rc != NB_OK:
         raise NbError(rc, "nb_batch_config failed")
-    return dict(threads=cfg[0], lanes=cfg[1], workgroups=cfg[2], systems=cfg[3])
+    return dict(threads=cfg[0], lanes=cfg[1], workgroups=cfg[2], systems=cfg[3], kernel_ms=ms.value)
 
 
 def params_from(p, grid, precision=None, device=0):

@@ -215,8 +215,9 @@ int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energi
 
 /* The shape of the last batch-kernel launch on a device: cfg[0] threads per
  * workgroup, cfg[1] lanes per atom in the pair loops, cfg[2] persistent
- * workgroups, cfg[3] systems (no counterpart in the reference). */
-int nb_batch_config(int device, int32_t cfg[4]);
+ * workgroups, cfg[3] systems; kernel_ms (may be NULL) its duration from HIP
+ * events around the launch (no counterpart in the reference). */
+int nb_batch_config(int device, int32_t cfg[4], double *kernel_ms);
 
 /* Number of evaluations of this handle served by a graph replay. */
 int nb_graph_replays(void *handle);
