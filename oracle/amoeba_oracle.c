@@ -1308,7 +1308,43 @@ typedef struct {
   double *grad, *torque;
   double *lgrad, *ltorque;       /* dU/dX/dlambda and its torque */
   int n_perm;
+  struct pcg_tape *tape;         /* the u-system iterations of the PCG (finite_pcg) */
+  int finite_grad;               /* gradient terms of direct polarization with uCR -> adjoint */
 } ctx_t;
+
+/* The PCG iterations of the u system, for the adjoint of the truncated solve:
+ * at iteration k the residual r_k, the preconditioned residual z_k, the
+ * direction p_k, q_k = A p_k, and the scalars a_k = gam_k / del_k
+ * (gam_k = r_k.z_k, del_k = p_k.q_k) and beta_k. */
+typedef struct pcg_tape {
+  int K, cap;
+  double *r, *z, *p, *q;   /* [cap][3n] */
+  double *a, *b, *gam, *del;
+} pcg_tape_t;
+
+static void tape_push(pcg_tape_t *t, int n3, const double *r, const double *z, const double *p, const double *q,
+                      double a, double gam, double del) {
+  if (t->K == t->cap) {
+    t->cap = t->cap ? 2 * t->cap : 8;
+    double **vs[4] = {&t->r, &t->z, &t->p, &t->q};
+    for (int v = 0; v < 4; v++) *vs[v] = realloc(*vs[v], (size_t)t->cap * n3 * sizeof(double));
+    double **ss[4] = {&t->a, &t->b, &t->gam, &t->del};
+    for (int v = 0; v < 4; v++) *ss[v] = realloc(*ss[v], (size_t)t->cap * sizeof(double));
+  }
+  int k = t->K++;
+  memcpy(t->r + (size_t)k * n3, r, n3 * sizeof(double));
+  memcpy(t->z + (size_t)k * n3, z, n3 * sizeof(double));
+  memcpy(t->p + (size_t)k * n3, p, n3 * sizeof(double));
+  memcpy(t->q + (size_t)k * n3, q, n3 * sizeof(double));
+  t->a[k] = a;
+  t->b[k] = 0.0;
+  t->gam[k] = gam;
+  t->del[k] = del;
+}
+
+static void tape_free(pcg_tape_t *t) {
+  free(t->r); free(t->z); free(t->p); free(t->q); free(t->a); free(t->b); free(t->gam); free(t->del);
+}
 
 /* induced dipole real-space field over a (half) list with Thole screening,
  * no masks: InducedDipoleFieldRegion.java:308-450 and the preconditioner of
@@ -1545,6 +1581,7 @@ static int pcg_solve(ctx_t *c, ora_energies *out) {
       rz += r[q] * z[q]; rzp += rp[q] * zp[q];
     }
     double al = pap != 0.0 ? rz / pap : rz, alp = papp != 0.0 ? rzp / papp : rzp;
+    if (c->tape) tape_push(c->tape, 3 * n, r, z, p, vec, al, rz, pap);
     double e1 = 0, e2 = 0;
     for (int q = 0; q < 3 * n; q++) {
       r[q] -= al * vec[q]; rp[q] -= alp * vecp[q];
@@ -1567,6 +1604,7 @@ static int pcg_solve(ctx_t *c, ora_energies *out) {
       rz2 += r[q] * z[q]; rzp2 += rp[q] * zp[q];
     }
     double b = rz2 / (rz != 0.0 ? rz : 1.0), bp = rzp2 / (rzp != 0.0 ? rzp : 1.0);
+    if (c->tape) c->tape->b[c->tape->K - 1] = b;
     for (int q = 0; q < 3 * n; q++) {
       p[q] = z[q] + b * p[q];
       pp[q] = zp[q] + bp * pp[q];
@@ -1580,6 +1618,223 @@ static int pcg_solve(ctx_t *c, ora_energies *out) {
   free(r); free(rp); free(z); free(zp); free(p); free(pp); free(vec); free(vecp); free(f); free(fp); free(au);
   free(aup);
   return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* gradient of the truncated PCG (finite_pcg)                               */
+/* ------------------------------------------------------------------------ */
+/* The polarization energy is linear in the dipoles the solve returns,
+ * E = -(el/2) u_K . F_p.  With u_K the K-th PCG iterate (K fixed), the
+ * reverse sweep through the recorded iterations, seeded with g = F_p, gives
+ * the adjoint of the direct field, Fhat = alpha (g + T rbar_0), and pairs of
+ * vectors (a, b) whose tensor derivatives a . dT b (Ewald field operator T,
+ * A = 1/alpha - T) and a . dM b (preconditioner M = alpha (T_pre alpha + s))
+ * complete the gradient:
+ *   dE = DirectGrad(u_K, Fhat) - (el/2) [sum_k -Qbar_k . dT p_k
+ *        + rbar_0 . dT u_0 + sum_k Zbar_k . dM r_k],
+ * where DirectGrad is the gradient of direct polarization with uCR replaced
+ * by Fhat (u . dF_p + Fhat . dF_d, no mutual term). */
+typedef struct { double *a, *b; double w; int pre; } bipair_t;
+
+static void apply_T(ctx_t *c, const double *v, double *out) {
+  size_t sz = 3 * (size_t)c->n * sizeof(double);
+  double *fp = malloc(sz);
+  induced_field(c, v, v, out, fp);
+  free(fp);
+}
+
+/* M v = alpha (T_pre (alpha v) + s v) */
+static void apply_M(ctx_t *c, const double *v, double *out) {
+  int n = c->n;
+  const double *pol = c->s->polarity;
+  size_t sz = 3 * (size_t)n * sizeof(double);
+  double *av = malloc(sz), *f = calloc(3 * n, sizeof(double)), *fp = calloc(3 * n, sizeof(double));
+  for (int q = 0; q < 3 * n; q++) av[q] = pol[q / 3] * v[q];
+  induced_real_field(c, c->pc_off, c->pc, 0.0, av, av, f, fp);
+  for (int q = 0; q < 3 * n; q++) out[q] = pol[q / 3] * (f[q] + c->prm->precond_scale * v[q]);
+  free(av); free(f); free(fp);
+}
+
+/* A v = v / alpha - T v on the polarizable atoms, 0 elsewhere */
+static void apply_A(ctx_t *c, const double *v, double *out) {
+  const double *pol = c->s->polarity;
+  apply_T(c, v, out);
+  for (int q = 0; q < 3 * c->n; q++) out[q] = pol[q / 3] > 0.0 ? v[q] / pol[q / 3] - out[q] : 0.0;
+}
+
+static double vdot(int m, const double *a, const double *b) {
+  double s = 0.0;
+  for (int q = 0; q < m; q++) s += a[q] * b[q];
+  return s;
+}
+
+static double *vdup(int m, const double *a) {
+  double *o = malloc(m * sizeof(double));
+  memcpy(o, a, m * sizeof(double));
+  return o;
+}
+
+/* reverse sweep; returns the pairs (count in *np) and writes Fhat */
+static bipair_t *pcg_adjoint(ctx_t *c, const double *g, double *fhat, int *np) {
+  const pcg_tape_t *t = c->tape;
+  int n3 = 3 * c->n, K = t->K;
+  const double *pol = c->s->polarity;
+  size_t vs = (size_t)n3;
+  double *Pb = calloc((size_t)K * vs, sizeof(double)), *Zb = calloc((size_t)K * vs, sizeof(double));
+  double *Rb = calloc((size_t)(K + 1) * vs, sizeof(double)), *Gb = calloc(K + 1, sizeof(double));
+  double *tmp = malloc(vs * sizeof(double)), *Qb = malloc(vs * sizeof(double));
+  bipair_t *pairs = calloc(2 * K + 2, sizeof(bipair_t));
+  int m = 0;
+#define TV(arr, k) (t->arr + (size_t)(k) * vs)
+  for (int k = K - 1; k >= 0; k--) {
+    double *Rk1 = Rb + (size_t)(k + 1) * vs;
+    if (k < K - 1) {
+      double *Pk1 = Pb + (size_t)(k + 1) * vs, *Zk1 = Zb + (size_t)(k + 1) * vs;
+      /* p_{k+1} = z_{k+1} + beta_k p_k */
+      for (int q = 0; q < n3; q++) Zk1[q] += Pk1[q];
+      double bb = vdot(n3, Pk1, TV(p, k));
+      for (int q = 0; q < n3; q++) Pb[(size_t)k * vs + q] += t->b[k] * Pk1[q];
+      /* beta_k = gam_{k+1} / gam_k */
+      Gb[k + 1] += bb / t->gam[k];
+      Gb[k] -= bb * t->gam[k + 1] / (t->gam[k] * t->gam[k]);
+      /* gam_{k+1} = r_{k+1} . z_{k+1} */
+      for (int q = 0; q < n3; q++) {
+        Rk1[q] += Gb[k + 1] * TV(z, k + 1)[q];
+        Zk1[q] += Gb[k + 1] * TV(r, k + 1)[q];
+      }
+      /* z_{k+1} = M r_{k+1} */
+      apply_M(c, Zk1, tmp);
+      for (int q = 0; q < n3; q++) Rk1[q] += tmp[q];
+      pairs[m++] = (bipair_t){vdup(n3, Zk1), vdup(n3, TV(r, k + 1)), 1.0, 1};
+    }
+    /* r_{k+1} = r_k - a_k q_k;  u_{k+1} = u_k + a_k p_k */
+    double *Rk = Rb + (size_t)k * vs, *Pk = Pb + (size_t)k * vs;
+    for (int q = 0; q < n3; q++) Rk[q] += Rk1[q];
+    double ab = -vdot(n3, Rk1, TV(q, k)) + vdot(n3, g, TV(p, k));
+    for (int q = 0; q < n3; q++) {
+      Qb[q] = -t->a[k] * Rk1[q];
+      Pk[q] += t->a[k] * g[q];
+    }
+    /* a_k = gam_k / del_k,  del_k = p_k . q_k */
+    Gb[k] += ab / t->del[k];
+    double db = -ab * t->gam[k] / (t->del[k] * t->del[k]);
+    for (int q = 0; q < n3; q++) {
+      Pk[q] += db * TV(q, k)[q];
+      Qb[q] += db * TV(p, k)[q];
+    }
+    /* q_k = A p_k */
+    apply_A(c, Qb, tmp);
+    for (int q = 0; q < n3; q++) Pk[q] += tmp[q];
+    pairs[m++] = (bipair_t){vdup(n3, Qb), vdup(n3, TV(p, k)), -1.0, 0};
+  }
+  /* gam_0 = r_0 . z_0;  p_0 = z_0;  z_0 = M r_0 */
+  double *R0 = Rb, *Z0 = Zb;
+  for (int q = 0; q < n3; q++) {
+    R0[q] += Gb[0] * TV(z, 0)[q];
+    Z0[q] += Gb[0] * TV(r, 0)[q] + Pb[q];
+  }
+  apply_M(c, Z0, tmp);
+  for (int q = 0; q < n3; q++) R0[q] += tmp[q];
+  pairs[m++] = (bipair_t){vdup(n3, Z0), vdup(n3, TV(r, 0)), 1.0, 1};
+  /* r_0 = T u_0;  u_0 = alpha F_d */
+  apply_T(c, R0, tmp);
+  for (int q = 0; q < n3; q++) fhat[q] = pol[q / 3] * (g[q] + tmp[q]);
+  pairs[m++] = (bipair_t){vdup(n3, R0), vdup(n3, c->dir), 1.0, 0};
+#undef TV
+  free(Pb); free(Zb); free(Rb); free(Gb); free(tmp); free(Qb);
+  *np = m;
+  return pairs;
+}
+
+/* sum over the pairs of w grad(a . T b) (pre: w grad(alpha a . T_pre alpha b)),
+ * scaled by coef, into grad: real space over the Ewald (preconditioner) list,
+ * reciprocal space through the PME potentials of a and b */
+static void bilinear_grad(ctx_t *c, const bipair_t *pairs, int np, double coef, double *grad) {
+  int n = c->n;
+  const double *pol = c->s->polarity;
+  const ora_system *s = c->s;
+  double **A = malloc(np * sizeof(double *)), **B = malloc(np * sizeof(double *));
+  for (int j = 0; j < np; j++) {
+    if (pairs[j].pre) {
+      A[j] = malloc(3 * n * sizeof(double));
+      B[j] = malloc(3 * n * sizeof(double));
+      for (int q = 0; q < 3 * n; q++) { A[j][q] = pol[q / 3] * pairs[j].a[q]; B[j][q] = pol[q / 3] * pairs[j].b[q]; }
+    } else {
+      A[j] = pairs[j].a;
+      B[j] = pairs[j].b;
+    }
+  }
+  for (int pre = 0; pre < 2; pre++) {
+    const int *off = pre ? c->pc_off : c->ew_off, *lst = pre ? c->pc : c->ew;
+    double aw = pre ? 0.0 : c->aewald;
+    for (int i = 0; i < n; i++)
+      for (int e = off[i]; e < off[i + 1]; e++) {
+        int k, orig;
+        double R[3];
+        double r2 = entry_vec(&c->cell, n, s->xyz, i, lst[e], &k, R, &orig);
+        double rl = sqrt(r2);
+        double bn[4], rr[5];
+        ewald_bn(rl, aw, 3, bn);
+        coulomb_rr(rl, 4, rr);
+        thole_t th;
+        thole_terms(rl, R, c->ipd[i], c->ipd[k], s->thole[i], s->thole[k], &th);
+        double B1, B2, C1, C2;
+        if (pre) {
+          B1 = th.s3 * rr[1]; B2 = th.s5 * rr[2]; C1 = th.s3 * rr[2]; C2 = th.s5 * rr[3];
+        } else {
+          B1 = bn[1] - (1.0 - th.s3) * rr[1]; B2 = bn[2] - (1.0 - th.s5) * rr[2];
+          C1 = bn[2] - (1.0 - th.s3) * rr[2]; C2 = bn[3] - (1.0 - th.s5) * rr[3];
+        }
+        double gW[3] = {0, 0, 0};
+        for (int j = 0; j < np; j++) {
+          if (pairs[j].pre != pre) continue;
+          const double *ai = A[j] + 3 * i, *ak = A[j] + 3 * k, *bi = B[j] + 3 * i, *bk = B[j] + 3 * k;
+          double S1 = dot3(ai, bk) + dot3(ak, bi);
+          double air = dot3(ai, R), akr = dot3(ak, R), bir = dot3(bi, R), bkr = dot3(bk, R);
+          double S2 = air * bkr + akr * bir;
+          for (int d = 0; d < 3; d++) {
+            double dB1 = -C1 * R[d] + rr[1] * th.d3[d], dB2 = -C2 * R[d] + rr[2] * th.d5[d];
+            gW[d] += pairs[j].w * (-dB1 * S1 + dB2 * S2 + B2 * (bkr * ai[d] + air * bk[d] + bir * ak[d] + akr * bi[d]));
+          }
+        }
+        for (int d = 0; d < 3; d++) {
+          grad[3 * i + d] -= coef * gW[d];
+          grad[3 * k + d] += coef * gW[d];
+        }
+      }
+  }
+  if (c->use_recip) {
+    pme_t *p = &c->pme;
+    double *fa = malloc(3 * n * sizeof(double)), *fb = malloc(3 * n * sizeof(double));
+    double *pa = malloc(20 * n * sizeof(double)), *pb = malloc(20 * n * sizeof(double));
+    for (int j = 0; j < np; j++) {
+      if (pairs[j].pre) continue;
+      for (int i = 0; i < n; i++) { frac_dipole(p, A[j] + 3 * i, fa + 3 * i); frac_dipole(p, B[j] + 3 * i, fb + 3 * i); }
+      pme_spread(p, n, NULL, fa, fb);
+      ora_convolution(p->grid, p->recip, p->K[0], p->K[1], p->K[2]);
+      for (int i = 0; i < n; i++) {
+        pme_gather(p, i, 0, pa + 20 * i);
+        pme_gather(p, i, 1, pb + 20 * i);
+      }
+      /* the mutual term of the reciprocal gradient with (u, uCR) -> (a, b):
+       * it is the reciprocal part of -(el/2) a . dT b */
+      for (int i = 0; i < n; i++) {
+        const double *u = fa + 3 * i, *v = fb + 3 * i, *fi = pa + 20 * i, *fip = pb + 20 * i;
+        double gg[3];
+        gg[0] = u[0] * fip[4] + v[0] * fi[4] + u[1] * fip[7] + v[1] * fi[7] + u[2] * fip[8] + v[2] * fi[8];
+        gg[1] = u[0] * fip[7] + v[0] * fi[7] + u[1] * fip[5] + v[1] * fi[5] + u[2] * fip[9] + v[2] * fi[9];
+        gg[2] = u[0] * fip[8] + v[0] * fi[8] + u[1] * fip[9] + v[1] * fi[9] + u[2] * fip[6] + v[2] * fi[6];
+        for (int a = 0; a < 3; a++) {
+          double df = 0.5 * c->prm->electric * (p->a[0][a] * gg[0] + p->a[1][a] * gg[1] + p->a[2][a] * gg[2]);
+          grad[3 * i + a] += pairs[j].w * (coef / (-0.5 * c->prm->electric)) * df;
+        }
+      }
+    }
+    free(fa); free(fb); free(pa); free(pb);
+  }
+  for (int j = 0; j < np; j++)
+    if (pairs[j].pre) { free(A[j]); free(B[j]); }
+  free(A); free(B);
 }
 
 /* SOR (ParticleMeshEwald.scfBySOR, ParticleMeshEwald.java:2331-2423, with
@@ -1772,7 +2027,7 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
     sym_full(m + 4, Q);
     for (int t = 0; t < 6; t++) hw[t] = cu[4 + t] + cv[4 + t];
     sym_full(hw, Hw);
-    int mutual = c->prm->polarization == ORA_POL_MUTUAL;
+    int mutual = c->prm->polarization == ORA_POL_MUTUAL && !c->finite_grad;
     for (int a = 0; a < 3; a++)
       for (int b = 0; b < 3; b++) {
         double t = (u[b] + v[b]) * pp[1 + a] + m[1 + b] * (cu[1 + a] + cv[1 + a]);
@@ -1808,7 +2063,7 @@ static void recip_energy(ctx_t *c, ora_energies *out, int polar) {
     g[0] = s[0] * fp[4] + s[1] * fp[7] + s[2] * fp[8];
     g[1] = s[0] * fp[7] + s[1] * fp[5] + s[2] * fp[9];
     g[2] = s[0] * fp[8] + s[1] * fp[9] + s[2] * fp[6];
-    if (c->prm->polarization == ORA_POL_MUTUAL) {
+    if (c->prm->polarization == ORA_POL_MUTUAL && !c->finite_grad) {
       g[0] += fu[0] * fip[4] + fpp[0] * fi[4] + fu[1] * fip[7] + fpp[1] * fi[7] + fu[2] * fip[8] + fpp[2] * fi[8];
       g[1] += fu[0] * fip[7] + fpp[0] * fi[7] + fu[1] * fip[5] + fpp[1] * fi[5] + fu[2] * fip[9] + fpp[2] * fi[9];
       g[2] += fu[0] * fip[8] + fpp[0] * fi[8] + fu[1] * fip[9] + fpp[1] * fi[9] + fu[2] * fip[6] + fpp[2] * fi[6];
@@ -1952,7 +2207,7 @@ static void real_energy(ctx_t *c, ora_energies *out, int polar) {
           thole_terms(rl, r, c->ipd[i], c->ipd[k], s->thole[i], s->thole[k], &cx.th);
           cx.scalep = orig ? pm[k] : 1.0;
           cx.scaled = (orig && s->group[k] == s->group[i]) ? prm->d11 : 1.0;
-          cx.direct = (prm->polarization == ORA_POL_DIRECT);
+          cx.direct = (prm->polarization == ORA_POL_DIRECT) || c->finite_grad;
           double f[3], a2[3], a3[3];
           double e = pol_terms(r, &I, &K, c->ind + 3 * i, c->indp + 3 * i, c->ind + 3 * k, c->indp + 3 * k, &cx, 1,
                                f, a2, a3);
@@ -2343,10 +2598,29 @@ static int elec_phase(ctx_t *c, ora_energies *out, double *ind, double *indcr) {
       if (prm->scf_algorithm == 1 || c->ph->sor) {
         status = sor_solve(c, out, prm->polar_sor);
       } else {
+        pcg_tape_t tape;
+        memset(&tape, 0, sizeof tape);
+        if (prm->finite_pcg) c->tape = &tape;
         status = pcg_solve(c, out);
         /* ParticleMeshEwald.java:2300-2312: a failed PCG is repeated by SOR
          * from the direct dipoles with the conservative factor 0.65 */
         if (status && prm->sor_fallback) status = sor_solve(c, out, 0.65);
+        else if (!status && prm->finite_pcg && tape.K > 0) {
+          /* gradient of the truncated solve: uCR is replaced by the adjoint
+           * of the direct field, the mutual terms by the tensor pairs */
+          double *g = malloc(s3), *fhat = malloc(s3), *f = malloc(s3), *fp = malloc(s3);
+          for (int q = 0; q < 3 * n; q++) g[q] = s->polarity[q / 3] > 0.0 ? c->fieldp[q] : 0.0;
+          int np = 0;
+          bipair_t *pairs = pcg_adjoint(c, g, fhat, &np);
+          memcpy(c->indp, fhat, s3);
+          induced_field(c, c->ind, c->indp, f, fp);
+          bilinear_grad(c, pairs, np, -0.5 * prm->electric * c->ph->pol, c->grad);
+          c->finite_grad = 1;
+          for (int j = 0; j < np; j++) { free(pairs[j].a); free(pairs[j].b); }
+          free(pairs); free(g); free(fhat); free(f); free(fp);
+        }
+        c->tape = NULL;
+        tape_free(&tape);
       }
     } else {
       double *f = malloc(s3), *fp = malloc(s3);

@@ -95,6 +95,10 @@ typedef struct {
   int scf_algorithm;        /* 0 PCG, 1 SOR */
   double polar_sor;         /* SOR factor (0.7) */
   int sor_fallback;         /* SOR with factor 0.65 after a failed PCG (1) */
+  /* gradient of the energy of the dipoles the PCG actually returned (its
+   * iteration count held fixed): the adjoint of the truncated iterations
+   * replaces the stationary-dipole terms (0) */
+  int finite_pcg;
 } ora_params;
 
 typedef struct {

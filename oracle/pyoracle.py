@@ -53,7 +53,8 @@ class OraParams(C.Structure):
                 ("perm_lambda_start", C.c_double), ("perm_lambda_end", C.c_double),
                 ("pol_lambda_exponent", C.c_double), ("pol_lambda_start", C.c_double),
                 ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int), ("no_ligand_condensed_scf", C.c_int),
-                ("scf_algorithm", C.c_int), ("polar_sor", C.c_double), ("sor_fallback", C.c_int)]
+                ("scf_algorithm", C.c_int), ("polar_sor", C.c_double), ("sor_fallback", C.c_int),
+                ("finite_pcg", C.c_int)]
 
 
 class OraAlchemy(C.Structure):
@@ -148,8 +149,9 @@ class System:
         self.xyz[:] = np.asarray(xyz, np.float64).reshape(-1)
 
 
-def make_params(p, box=None, nthreads=0):
-    """OraParams from a forcefieldx_amd.forcefield.NonbondedParams."""
+def make_params(p, box=None, nthreads=0, finite_pcg=False):
+    """OraParams from a forcefieldx_amd.forcefield.NonbondedParams; finite_pcg
+    asks for the gradient of the truncated PCG's dipoles (adjoint terms)."""
     grid = p.resolved_grid(box) if box is not None and box[0] > 0 else (16, 16, 16)
     return OraParams(p.ewald_cutoff, p.ewald_alpha, p.vdw_cutoff, p.vdw_taper, p.list_buffer, p.pme_order,
                      (C.c_int * 3)(*grid), p.polar_eps, p.electric, p.precond_cutoff, p.precond_scale,
@@ -160,7 +162,8 @@ def make_params(p, box=None, nthreads=0):
                      int(p.intramolecular_softcore), int(p.elec_lambda), p.perm_lambda_alpha,
                      p.perm_lambda_exponent, p.perm_lambda_start, p.perm_lambda_end, p.pol_lambda_exponent,
                      p.pol_lambda_start, p.pol_lambda_end, int(p.ligand_vapor_elec),
-                     int(p.no_ligand_condensed_scf), int(p.scf_algorithm), p.polar_sor, int(p.sor_fallback))
+                     int(p.no_ligand_condensed_scf), int(p.scf_algorithm), p.polar_sor, int(p.sor_fallback),
+                     int(finite_pcg))
 
 
 def alchemical_factors(params, lam):
@@ -170,10 +173,12 @@ def alchemical_factors(params, lam):
     return {k: getattr(f, k) for k, _ in OraAlchemy._fields_}
 
 
-def evaluate(system, params, nthreads=0):
-    """Run the oracle; returns (energies dict, gradient (n,3), u (n,3), uCR (n,3))."""
+def evaluate(system, params, nthreads=0, finite_pcg=False):
+    """Run the oracle; returns (energies dict, gradient (n,3), u (n,3), uCR (n,3)).
+    With finite_pcg the gradient is that of the energy of the dipoles the PCG
+    returned (iteration count fixed) and uCR is replaced by the adjoint dipoles."""
     n = system.top.n
-    prm = make_params(params, system.box, nthreads) if not isinstance(params, OraParams) else params
+    prm = make_params(params, system.box, nthreads, finite_pcg) if not isinstance(params, OraParams) else params
     out = OraEnergies()
     g = np.zeros(3 * n)
     u = np.zeros(3 * n)
