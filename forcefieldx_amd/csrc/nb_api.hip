@@ -76,7 +76,8 @@ void free_all(NbHandle *h) {
   h->graph = nullptr;
   if (h->child_env) { free_all(h->child_env); delete h->child_env; h->child_env = nullptr; }
   if (h->child_vap) { free_all(h->child_vap); delete h->child_vap; h->child_vap = nullptr; }
-  void *lam[] = {h->d_lgrad_s, h->d_ltorque_s, h->d_lgrad_r, h->d_ltorque_r, h->d_vap_map};
+  void *lam[] = {h->d_lgrad_s, h->d_ltorque_s, h->d_lgrad_r, h->d_ltorque_r, h->d_vap_map, h->d_tape, h->d_tape_sc,
+                 h->d_fin, h->d_gfin};
   for (void *p : lam)
     if (p) hipFree(p);
   void *ptrs[] = {h->d_frame, h->d_axis, h->d_group0, h->d_vclass0, h->d_redidx, h->d_mpole_local, h->d_polar0,
@@ -667,6 +668,7 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   h->vpart_used = 0;
   h->used_sor = false;
+  h->finite_active = false;
   rebuilt = false;
   if (capture) {
     nb_check_rebuild_async(h);
@@ -704,13 +706,20 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
           status = nb_sor(h, h->prm.polar_sor, &e);
           h->used_sor = true;
         } else {
+          h->tape_on = h->want_finite;
+          h->tape_K = 0;
           status = nb_pcg(h, &e);
+          h->tape_on = false;
           if ((status == NB_ERR_SCF_DIVERGED || status == NB_ERR_SCF_MAXITER) && h->prm.sor_fallback) {
             // ParticleMeshEwald.java:2300-2312: SOR from the direct dipoles with
             // the conservative factor 0.65
             nb_reset_dipoles(h);
             status = nb_sor(h, 0.65, &e);
             h->used_sor = true;
+          } else if (status == NB_OK && h->want_finite) {
+            // gradient of the dipoles this solve returned: uCR -> adjoint
+            nb_finite_adjoint(h);
+            h->finite_active = true;
           }
         }
       } else if (use_recip) {
@@ -728,6 +737,7 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
     if (use_recip) nb_recip_energy(h, polar);
   }
   nb_join(h);
+  if (h->finite_active) nb_finite_add(h);
   nb_finalize_gradient(h);
   if (h->want_vir) {
     nb_vir_reduce(h);
@@ -790,6 +800,12 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     h->vir_valid = false;
     h->want_dudl = (flags & NB_DUDL) != 0;
     h->dudl_valid = false;
+    h->want_finite = (flags & NB_FINITE_PCG) != 0 && h->prm.polarization == NB_POL_MUTUAL;
+    if (h->want_finite && (h->want_vir || h->ost || h->elec_lambda)) {
+      h->err = "NB_FINITE_PCG does not combine with the virial or electrostatic alchemistry";
+      return NB_ERR_INVALID;
+    }
+    if (h->want_finite && !h->d_gfin) HIPCHECK(hipMalloc(&h->d_gfin, sizeof(double) * 3 * n));
     if (h->want_vir) {
       // an aperiodic system has pair and frame terms only
       // 9 partial sums per block of every contributing launch
@@ -805,7 +821,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     }
     bool mutual = h->prm.do_pme && h->prm.polarization == NB_POL_MUTUAL && h->ph.polar;
     bool replayed = false;
-    if (h->graph_on && h->gexec && h->graph_flags == flags && !h->profiling) {
+    if (h->graph_on && h->gexec && h->graph_flags == flags && !h->profiling && !h->want_finite) {
       // replay; a list that needs rebuilding, an overflowed filtered row or a
       // solve that did not stop within the recorded iterations sends the
       // evaluation down the regular path below
@@ -894,7 +910,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
     // record a graph for the next evaluations: the PCG budget is this
     // evaluation's iteration count plus one
-    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor)
+    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite)
       capture_graph(h, flags, mutual ? e.pcg_iters + 1 : 0);
     h->last = e;
     if (out) *out = e;

@@ -543,7 +543,8 @@ static void real_energy_t(NbHandle *h, bool polar) {
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   ScaleTab sc = scale_tab(h);
-  int direct = h->prm.polarization == NB_POL_DIRECT;
+  // with the finite-PCG gradient, uCR holds the adjoint and the mutual terms are replaced
+  int direct = h->prm.polarization == NB_POL_DIRECT || h->finite_active;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256), Gs = xcd_grid((long)n * NB_LPA_SP, 256);
   double el = h->prm.electric;
   bool perm = !OST || h->ph.perm_real;

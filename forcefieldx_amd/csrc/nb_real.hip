@@ -592,6 +592,7 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
     NB_LAUNCH(k_pcg_ap, G, B, 0, n, a3, h->d_polar, h->d_p, h->d_pp, f, fp,
               use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
               h->d_vec, h->d_vecp, part, stride);
+    if (h->tape_on) nb_tape_push(h);
     NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
               h->d_up, part, stride, nb, rec, mx, (const double *)nullptr);
     NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal, 0, n, 0.0, 0);
@@ -721,7 +722,8 @@ void nb_recip_energy(NbHandle *h, bool polar) {
     for (int b = 0; b < 3; b++) fr.a[a][b] = h->afrac[a][b];
   NB_LAUNCH(k_recip_energy, grid_for(n, 128), dim3(128), 0, n, h->d_gmp, h->d_fmp, h->d_phip,
                      h->d_phiu, h->d_phiup, h->d_u, h->d_up, fr, h->prm.electric, h->prm.ewald_alpha, (int)polar,
-                     (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_grad_r, h->d_torque_r, h->d_energy,
+                     (int)(h->prm.polarization == NB_POL_MUTUAL && !h->finite_active), h->d_grad_r, h->d_torque_r,
+                     h->d_energy,
                      h->want_vir ? nb_vir_region(h, grid_for(n, 128).x) : (double *)nullptr, h->ph,
                      h->ost ? h->d_lgrad_r : (double *)nullptr, h->d_ltorque_r);
 }

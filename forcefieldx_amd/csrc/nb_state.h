@@ -173,6 +173,16 @@ struct NbHandle {
 
   nb_energies last;
   int fmaxa = 0, fmaxb = 0;       // row capacities of the batch kernel (preconditioner part, the rest)
+  // gradient of the truncated PCG (NB_FINITE_PCG)
+  bool want_finite = false;       // this evaluation asks for it
+  bool tape_on = false;           // the PCG records its iterations
+  bool finite_active = false;     // the energy kernels use the adjoint terms
+  double *d_tape = nullptr;       // [tape_cap][4][3n] r, z, p, q of each iteration (sorted)
+  double *d_tape_sc = nullptr;    // [tape_cap][2] r.z, p.q
+  int tape_cap = 0, tape_K = 0;
+  double *d_fin = nullptr;        // adjoint work vectors and tensor pairs
+  size_t fin_cap = 0;
+  double *d_gfin = nullptr;       // [3n] tensor-derivative gradient terms (sorted)
   bool induced_caller = false;    // d_u / d_up hold caller order (written by the batch kernel)
 
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
@@ -226,6 +236,9 @@ void nb_zero_outputs(NbHandle *h);
 double *nb_vir_region(NbHandle *h, long nblocks);
 void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
+void nb_tape_push(NbHandle *h);
+void nb_finite_adjoint(NbHandle *h);
+void nb_finite_add(NbHandle *h);
 bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
                        int &status);
 int nb_batch_last_config(int device, int32_t cfg[4], double *kernel_ms);

@@ -49,7 +49,14 @@ enum { NB_POL_NONE = 0, NB_POL_DIRECT = 1, NB_POL_MUTUAL = 2 };
 /* Arithmetic form. */
 enum { NB_PREC_FP64 = 0, NB_PREC_MIXED = 1 };
 
-/* nb_evaluate flags */
+/* nb_evaluate flags.  NB_FINITE_PCG (no counterpart in the reference, whose
+ * gradient assumes stationary dipoles, RealSpaceEnergyRegion.java:1700-2177):
+ * the gradient of the energy of the dipoles the PCG actually returned, with
+ * its iteration count held fixed -- the adjoint of the recorded iterations
+ * replaces uCR and the mutual terms.  Mutual polarization solved by PCG only
+ * (no effect otherwise); not with NB_VIRIAL or electrostatic alchemistry
+ * (NB_ERR_INVALID); evaluated on the regular path (no graph replay).  After
+ * it, nb_get_induced returns the adjoint dipoles as ucr. */
 enum { NB_GRAD = 1u, NB_VIRIAL = 2u, NB_DUDL = 4u, NB_FINITE_PCG = 8u, NB_BATCH_SEPARATE = 16u };
 
 /* SCF algorithms (SCFAlgorithm.java) */
