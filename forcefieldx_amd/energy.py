@@ -92,8 +92,11 @@ class ForceFieldEnergy:
     """Nonbonded part of ForceFieldEnergy on the MI355X library."""
 
     def __init__(self, topology, box, params: NonbondedParams | None = None, precision=None, device=0,
-                 direct_fallback=None):
+                 direct_fallback=None, finite_pcg_gradient=False):
+        """finite_pcg_gradient: gradients of the energy of the dipoles the PCG
+        returned (NB_FINITE_PCG) instead of the stationary-dipole gradient."""
         self.params = params or NonbondedParams()
+        self.finite_pcg_gradient = bool(finite_pcg_gradient)
         self.topology = topology
         self.box = np.asarray(box if box is not None else [0, 0, 0, 90, 90, 90], float)
         grid = self.params.resolved_grid(self.box) if self.box[0] > 0 else (16, 16, 16)
@@ -192,7 +195,8 @@ class ForceFieldEnergy:
     def _evaluate(self, gradient, virial=False):
         self._fallback_grad = self._fallback_dudx = None
         flags = (ffi.NB_GRAD if gradient else 0) | (ffi.NB_VIRIAL if virial else 0) | \
-            (ffi.NB_DUDL if self._lambda is not None else 0)
+            (ffi.NB_DUDL if self._lambda is not None else 0) | \
+            (ffi.NB_FINITE_PCG if gradient and self.finite_pcg_gradient else 0)
         e, rc = self._handle.evaluate(flags, check=False)
         if rc in (ffi.NB_ERR_SCF_DIVERGED, ffi.NB_ERR_SCF_MAXITER) and self.direct_fallback:
             # direct polarization for this evaluation only; the mutual handle is kept
