@@ -1,0 +1,438 @@
+"""Tinker-format input for the nonbonded path, read as Force Field X reads it.
+
+* Coordinate files (.xyz): atom count and title, an optional line of six cell
+  parameters, then one line per atom: index, name, x, y, z, atom type, bonded
+  atoms (parsers/XYZFilter.java).
+* Keyword files (.key / .properties): `keyword value ...` lines, `#` comments
+  (parsers/KeyFilter.java; keyword semantics ForceField.java:238-763).
+* AMOEBA parameter files: the `atom`, `vdw`, `vdwpr`, `multipole` (five-line
+  records), `polarize` records and the scalar keywords of the vdW form, the
+  mask scales and the polarization model (parsers/ForceFieldFilter.java;
+  MultipoleType.parse, MultipoleType.java:787-898).
+
+`build` assembles a Topology and NonbondedParams from them:
+
+* multipole types by the reference's matching order (MultipoleType.
+  multipoleTypeFactory, MultipoleType.java:547-725): the atom's type alone,
+  then 1, 2 and 3 reference atoms among its 1-2 neighbours, then a 1-2 and
+  one or two 1-3 atoms, neighbours sorted by index; the matched atoms are the
+  frame's z, x, y atoms; Bohr -> Angstrom units (MultipoleType.java:265);
+* polarization groups grown over bonds between atoms whose types the
+  `polarize` records list (PolarizeType.assignPolarizationGroups,
+  PolarizeType.java:174-215, 360-390), pdamp = alpha^(1/6);
+* reduced vdW sites on atoms with one bond and a reduction factor
+  (VanDerWaals.java:944-956), pair table by the file's combining rules with
+  `vdwpr` overrides;
+* the cell from the keywords (a-axis .. gamma), PME grid by the density rule,
+  cutoffs, Ewald coefficient, convergence and SCF algorithm.
+"""
+from __future__ import annotations
+
+import gzip
+import io
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .forcefield import (BOHR, FRAME_BISECTOR, FRAME_NONE, FRAME_THREEFOLD, FRAME_ZONLY, FRAME_ZTHENBISECTOR,
+                         FRAME_ZTHENX, NonbondedParams, Topology, combined_eps, combined_radius, masks_from_bonds)
+
+
+def _open(src):
+    if hasattr(src, "read"):
+        return src
+    if str(src).endswith(".gz"):
+        return io.TextIOWrapper(gzip.open(src, "rb"), encoding="utf-8")
+    return open(src, "r", encoding="utf-8", errors="replace")
+
+
+# --------------------------------------------------------------------------
+# coordinates
+# --------------------------------------------------------------------------
+@dataclass
+class XYZFile:
+    title: str
+    names: list
+    xyz: np.ndarray        # (n, 3)
+    types: np.ndarray      # (n,) atom types
+    bonds: list            # (i, j) pairs, 0-based, i < j
+    box: list | None = None
+
+
+def read_xyz(src) -> XYZFile:
+    with _open(src) as f:
+        lines = [ln.rstrip("\n") for ln in f]
+    head = lines[0].split()
+    n = int(head[0])
+    title = lines[0].split(None, 1)[1] if len(head) > 1 else ""
+    k = 1
+    box = None
+    toks = lines[1].split() if len(lines) > 1 else []
+    if len(toks) == 6:
+        try:
+            vals = [float(t) for t in toks]
+            if not toks[0].lstrip("-").isdigit():
+                box = vals
+                k = 2
+        except ValueError:
+            pass
+    names, xyz, types, bonds = [], np.zeros((n, 3)), np.zeros(n, np.int64), set()
+    for a in range(n):
+        t = lines[k + a].split()
+        idx = int(t[0]) - 1
+        if idx != a:
+            raise ValueError(f"atom {a + 1}: index {t[0]} out of order")
+        names.append(t[1])
+        xyz[a] = [float(t[2]), float(t[3]), float(t[4])]
+        types[a] = int(t[5])
+        for b in t[6:]:
+            j = int(b) - 1
+            if j != a:
+                bonds.add((min(a, j), max(a, j)))
+    return XYZFile(title, names, xyz, types, sorted(bonds), box)
+
+
+def write_xyz(path, xf: XYZFile):
+    adj = [[] for _ in range(len(xf.names))]
+    for i, j in xf.bonds:
+        adj[i].append(j)
+        adj[j].append(i)
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "wt") as f:
+        f.write(f"{len(xf.names):6d}  {xf.title}\n")
+        if xf.box is not None:
+            f.write(" ".join(f"{v:.6f}" for v in xf.box) + "\n")
+        for a, nm in enumerate(xf.names):
+            x, y, z = xf.xyz[a]
+            nb = "".join(f"{j + 1:6d}" for j in sorted(adj[a]))
+            f.write(f"{a + 1:6d}  {nm:<3s} {x:11.6f} {y:11.6f} {z:11.6f} {int(xf.types[a]):5d}{nb}\n")
+
+
+# --------------------------------------------------------------------------
+# keywords and parameters
+# --------------------------------------------------------------------------
+def read_keys(src) -> dict:
+    """keyword (lower case) -> list of value strings (the last occurrence wins)."""
+    keys = {}
+    with _open(src) as f:
+        for ln in f:
+            ln = ln.split("#")[0].strip()
+            if not ln:
+                continue
+            t = ln.split()
+            keys[t[0].lower()] = t[1:]
+    return keys
+
+
+@dataclass
+class ForceFieldParams:
+    keys: dict = field(default_factory=dict)          # scalar keywords
+    atom: dict = field(default_factory=dict)          # type -> (class, name)
+    vdw: dict = field(default_factory=dict)           # class -> (radius, eps, reduction)
+    vdwpr: dict = field(default_factory=dict)         # (c1, c2) -> (radius, eps)
+    multipole: dict = field(default_factory=dict)     # (types...) -> (frame, [q, d(3), Q(6)] in Bohr units)
+    polarize: dict = field(default_factory=dict)      # type -> (alpha, thole, (group types))
+
+    def key(self, name, default=None):
+        v = self.keys.get(name)
+        return default if v is None else v[0]
+
+
+def _frame_of(signed):
+    """Frame from the signs of the reference types (MultipoleType.java:818-842)."""
+    n = len(signed)
+    if n == 1:
+        return FRAME_NONE
+    if n == 2:
+        return FRAME_ZONLY
+    if n == 3:
+        return FRAME_BISECTOR if (signed[1] < 0 or signed[2] < 0) else FRAME_ZTHENX
+    if n == 4 and signed[2] < 0 and signed[3] < 0:
+        return FRAME_THREEFOLD if signed[1] < 0 else FRAME_ZTHENBISECTOR
+    return FRAME_ZTHENX
+
+
+def read_prm(src) -> ForceFieldParams:
+    ff = ForceFieldParams()
+    with _open(src) as f:
+        lines = [ln.split("#")[0].rstrip() for ln in f]
+    i = 0
+    while i < len(lines):
+        raw = lines[i]
+        i += 1
+        t = raw.replace("\\", " ").split()
+        if not t:
+            continue
+        kw = t[0].lower()
+        if kw == "atom":
+            ff.atom[int(t[1])] = (int(t[2]), t[3])
+        elif kw == "vdw":
+            red = float(t[4]) if len(t) > 4 else 0.0
+            ff.vdw[int(t[1])] = (float(t[2]), float(t[3]), red)
+        elif kw in ("vdwpr", "vdwpair"):
+            a, b = int(t[1]), int(t[2])
+            ff.vdwpr[(min(a, b), max(a, b))] = (float(t[3]), float(t[4]))
+        elif kw == "multipole":
+            signed = [int(x) for x in t[1:-1] if int(x) != 0]
+            charge = float(t[-1])
+            vals = []
+            while len(vals) < 9:
+                vals += [float(x) for x in lines[i].replace("\\", " ").split()]
+                i += 1
+            d = vals[0:3]
+            qxx, qyx, qyy, qzx, qzy, qzz = vals[3:9]
+            ff.multipole[tuple(abs(x) for x in signed)] = (_frame_of(signed),
+                                                           [charge] + d + [qxx, qyy, qzz, qyx, qzx, qzy])
+        elif kw == "polarize":
+            ff.polarize[int(t[1])] = (float(t[2]), float(t[3]), tuple(int(x) for x in t[4:]))
+        elif len(t) >= 2 and kw in _SCALARS:
+            ff.keys[kw] = t[1:]
+    return ff
+
+
+_SCALARS = {"vdwtype", "radiusrule", "radiustype", "radiussize", "epsilonrule", "polarization", "dielectric",
+            "vdw-12-scale", "vdw-13-scale", "vdw-14-scale", "vdw-15-scale", "mpole-12-scale", "mpole-13-scale",
+            "mpole-14-scale", "mpole-15-scale", "polar-12-scale", "polar-13-scale", "polar-14-scale",
+            "polar-15-scale", "polar-12-intra", "polar-13-intra", "polar-14-intra", "polar-15-intra",
+            "direct-11-scale", "direct-12-scale", "direct-13-scale", "direct-14-scale", "mutual-11-scale",
+            "mutual-12-scale", "mutual-13-scale", "mutual-14-scale", "forcefield"}
+
+
+def write_prm(path, ff: ForceFieldParams, types=None):
+    """Parameter file in the same record formats (only the records of the
+    given atom types and their classes when types is set)."""
+    tset = None if types is None else set(int(t) for t in types)
+    cset = None if tset is None else {ff.atom[t][0] for t in tset}
+    opener = gzip.open if str(path).endswith(".gz") else open
+    with opener(path, "wt") as f:
+        for k, v in ff.keys.items():
+            f.write(f"{k:<24s}{' '.join(v)}\n")
+        for t, (c, nm) in sorted(ff.atom.items()):
+            if tset is None or t in tset:
+                f.write(f"atom {t:6d} {c:6d}  {nm}\n")
+        for c, (r, e, red) in sorted(ff.vdw.items()):
+            if cset is None or c in cset:
+                f.write(f"vdw {c:6d} {r:.9f} {e:.9f}" + (f" {red:.3f}" if red > 0 else "") + "\n")
+        for (a, b), (r, e) in sorted(ff.vdwpr.items()):
+            if cset is None or (a in cset and b in cset):
+                f.write(f"vdwpr {a:6d} {b:6d} {r:.9f} {e:.9f}\n")
+        for key, (fr, v) in sorted(ff.multipole.items()):
+            if tset is not None and key[0] not in tset:
+                continue
+            signed = list(key)
+            if fr == FRAME_BISECTOR:
+                signed[1], signed[2] = -signed[1], -signed[2]
+            elif fr == FRAME_ZTHENBISECTOR:
+                signed[2], signed[3] = -signed[2], -signed[3]
+            elif fr == FRAME_THREEFOLD:
+                signed[1], signed[2], signed[3] = -signed[1], -signed[2], -signed[3]
+            q, d, Q = v[0], v[1:4], v[4:10]   # Qxx Qyy Qzz Qxy Qxz Qyz
+            f.write("multipole " + " ".join(f"{x:6d}" for x in signed) + f" {q:12.5f}\n")
+            f.write(f"    {d[0]:12.5f} {d[1]:12.5f} {d[2]:12.5f}\n")
+            f.write(f"    {Q[0]:12.5f}\n    {Q[3]:12.5f} {Q[1]:12.5f}\n    {Q[4]:12.5f} {Q[5]:12.5f} {Q[2]:12.5f}\n")
+        for t, (a, th, grp) in sorted(ff.polarize.items()):
+            if tset is None or t in tset:
+                f.write(f"polarize {t:6d} {a:.5f} {th:.5f}" + "".join(f" {g:6d}" for g in grp) + "\n")
+
+
+# --------------------------------------------------------------------------
+# topology
+# --------------------------------------------------------------------------
+def _neighbours(n, bonds):
+    n12 = [[] for _ in range(n)]
+    for i, j in bonds:
+        n12[i].append(j)
+        n12[j].append(i)
+    n12 = [sorted(a) for a in n12]
+    n13 = []
+    for i in range(n):
+        s = set(n12[i])
+        lst = [b for a in n12[i] for b in n12[a] if b != i and b not in s]
+        n13.append(sorted(lst))
+    return n12, n13
+
+
+def assign_multipoles(types, n12, n13, ff: ForceFieldParams):
+    """Frame, axis atoms and local multipole (Bohr units) per atom, in the
+    matching order of MultipoleType.multipoleTypeFactory."""
+    n = len(types)
+    frame = np.zeros(n, np.int32)
+    axis = -np.ones((n, 3), np.int32)
+    mp = np.zeros((n, 10))
+    mt = ff.multipole
+    for i in range(n):
+        t = int(types[i])
+        hit = None
+        if (t,) in mt:
+            hit = ((t,), ())
+        if hit is None:
+            for a in n12[i]:
+                key = (t, int(types[a]))
+                if key in mt:
+                    hit = (key, (a,))
+                    break
+        if hit is None:
+            for a in n12[i]:
+                for b in n12[i]:
+                    if a == b:
+                        continue
+                    key = (t, int(types[a]), int(types[b]))
+                    if key in mt:
+                        hit = (key, (a, b))
+                        break
+                if hit:
+                    break
+        if hit is None:
+            for a in n12[i]:
+                for b in n12[i]:
+                    if a == b:
+                        continue
+                    for c in n12[i]:
+                        if c == a or c == b:
+                            continue
+                        key = (t, int(types[a]), int(types[b]), int(types[c]))
+                        if key in mt:
+                            hit = (key, (a, b, c))
+                            break
+                    if hit:
+                        break
+                if hit:
+                    break
+        if hit is None:
+            for a in n12[i]:
+                for b in n13[i]:
+                    key = (t, int(types[a]), int(types[b]))
+                    if key in mt:
+                        hit = (key, (a, b))
+                        break
+                    for c in n13[i]:
+                        if c != b:
+                            key = (t, int(types[a]), int(types[b]), int(types[c]))
+                            if key in mt:
+                                hit = (key, (a, b, c))
+                                break
+                    if hit:
+                        break
+                if hit:
+                    break
+        if hit is None:
+            raise ValueError(f"no multipole type for atom {i + 1} (type {t})")
+        key, atoms = hit
+        fr, vals = mt[key]
+        frame[i] = fr
+        for k, a in enumerate(atoms):
+            axis[i, k] = a
+        mp[i] = vals
+    return frame, axis, mp
+
+
+def polarization_groups(types, n12, ff: ForceFieldParams):
+    """Group id per atom from PolarizeType.assignPolarizationGroups (the
+    group of each atom grown over bonds to atoms of the listed types)."""
+    n = len(types)
+    gid = -np.ones(n, np.int64)
+    for i in range(n):
+        pol = ff.polarize.get(int(types[i]))
+        allowed = [int(types[i])] + ([g for g in pol[2]] if pol else [])
+        group = [i]
+        stack = [i]
+        if pol and pol[2]:
+            while stack:
+                seed = stack.pop()
+                for j in n12[seed]:
+                    if int(types[j]) in allowed and j not in group:
+                        group.append(j)
+                        pj = ff.polarize.get(int(types[j]))
+                        for g in (pj[2] if pj else ()):
+                            if g not in allowed:
+                                allowed.append(g)
+                        stack.append(j)
+        gid[i] = min(group)
+    return gid
+
+
+def _f(keys, name, default):
+    v = keys.get(name)
+    return float(v[0]) if v else default
+
+
+def build(xf: XYZFile, ff: ForceFieldParams, keys: dict):
+    """(xyz (n,3), box or None, Topology, NonbondedParams)."""
+    n = len(xf.names)
+    types = xf.types
+    n12, n13 = _neighbours(n, xf.bonds)
+    frame, axis, mpb = assign_multipoles(types, n12, n13, ff)
+    mpole = np.zeros((n, 10))
+    mpole[:, 0] = mpb[:, 0]
+    mpole[:, 1:4] = mpb[:, 1:4] * BOHR
+    mpole[:, 4:10] = mpb[:, 4:10] * BOHR * BOHR
+    polarity = np.array([ff.polarize.get(int(t), (0.0, 0.0, ()))[0] for t in types])
+    thole = np.array([ff.polarize.get(int(t), (0.0, 0.0, ()))[1] for t in types])
+    pdamp = np.where(polarity > 0, polarity ** (1.0 / 6.0), 0.0)
+    group = polarization_groups(types, n12, ff)
+    # vdW classes (compacted) and reduced sites
+    cls = np.array([ff.atom[int(t)][0] for t in types])
+    used = sorted(set(int(c) for c in cls))
+    cidx = {c: k for k, c in enumerate(used)}
+    vclass = np.array([cidx[int(c)] for c in cls], np.int32)
+    red_index = np.arange(n, dtype=np.int32)
+    red_value = np.zeros(n)
+    for i in range(n):
+        r = ff.vdw[int(cls[i])][2]
+        if r > 0.0 and len(n12[i]) == 1:
+            red_index[i] = n12[i][0]
+            red_value[i] = r
+    rule_r = ff.key("radiusrule", "CUBIC-MEAN").upper()
+    rule_e = ff.key("epsilonrule", "HHG").upper()
+    size = ff.key("radiussize", "DIAMETER").upper()
+    rtype = ff.key("radiustype", "R-MIN").upper()
+    scale = 0.5 if size == "DIAMETER" else 1.0
+    if rtype == "SIGMA":
+        scale *= 1.122462048309372981
+    nc = len(used)
+    tab = np.zeros((nc, nc, 2))
+    for a, ca in enumerate(used):
+        ra, ea = ff.vdw[ca][0] * scale, ff.vdw[ca][1]
+        for b, cb in enumerate(used):
+            rb, eb = ff.vdw[cb][0] * scale, ff.vdw[cb][1]
+            pr = ff.vdwpr.get((min(ca, cb), max(ca, cb)))
+            if pr is not None:
+                # a pair record gives Rmin itself (VanDerWaalsForm.java:442-460)
+                rm, e = pr[0], pr[1]
+            else:
+                rm = combined_radius(ra, rb, rule_r)
+                e = combined_eps(ea, eb, ra, rb, rule_e)
+            tab[a, b] = (1.0 / rm if rm > 0 else 0.0, e)
+    off, atoms, rels = masks_from_bonds(n, xf.bonds)
+    top = Topology(n=n, frame=frame, axis=axis, mpole=mpole, polarity=polarity, thole=thole, pdamp=pdamp,
+                   group=group.astype(np.int32), molecule=np.zeros(n, np.int32), vdw_class=vclass, vdw_table=tab,
+                   red_index=red_index, red_value=red_value, mask_off=off, mask_atom=atoms, mask_rel=rels)
+    # cell and settings
+    box = xf.box
+    if "a-axis" in keys:
+        a = _f(keys, "a-axis", 0.0)
+        box = [a, _f(keys, "b-axis", a), _f(keys, "c-axis", a), _f(keys, "alpha", 90.0), _f(keys, "beta", 90.0),
+               _f(keys, "gamma", 90.0)]
+    fk = ff.keys
+    pol = (keys.get("polarization") or fk.get("polarization") or ["MUTUAL"])[0].upper()
+    p = NonbondedParams(
+        ewald_cutoff=_f(keys, "ewald-cutoff", 7.0), ewald_alpha=_f(keys, "ewald-alpha", 0.545),
+        vdw_cutoff=_f(keys, "vdw-cutoff", 12.0), pme_mesh_density=_f(keys, "pme-mesh-density", 1.2),
+        polar_eps=_f(keys, "polar-eps", 1e-6),
+        m_scale=tuple(_f(fk, f"mpole-1{k}-scale", d) for k, d in zip("2345", (0.0, 0.0, 0.4, 0.8))),
+        p_scale=tuple(_f(fk, f"polar-1{k}-scale", d) for k, d in zip("2345", (0.0, 0.0, 1.0, 1.0))),
+        intra14=_f(fk, "polar-14-intra", 0.5), d11=_f(fk, "direct-11-scale", 0.0),
+        v_scale=tuple(_f(fk, f"vdw-1{k}-scale", d) for k, d in zip("234", (0.0, 0.0, 1.0))),
+        polarization={"NONE": 0, "DIRECT": 1, "MUTUAL": 2}.get(pol, 2),
+        scf_algorithm=1 if (keys.get("scf-algorithm") or ["CG"])[0].upper() == "SOR" else 0)
+    if "pme-grid" in keys:
+        g = [int(v) for v in keys["pme-grid"]]
+        p = replace(p, grid=tuple(g if len(g) == 3 else g * 3))
+    if box is not None and box[0] > 0:
+        p = replace(p, grid=p.resolved_grid(box))
+    return xf.xyz.copy(), box, top, p
+
+
+def load(xyz_path, key_path, prm_path):
+    """Read the three files and build (xyz, box, Topology, NonbondedParams)."""
+    return build(read_xyz(xyz_path), read_prm(prm_path), read_keys(key_path))

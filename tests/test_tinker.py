@@ -1,0 +1,78 @@
+"""Tinker-format input (forcefieldx_amd.tinker) and the reference's DHFR
+EnergyTest (EnergyTest.java:148-180): 23,558 atoms of AMOEBA-BIO-2009 protein
+and water read from the committed fixture (tests/golden/dhfr, written by
+tests/golden/make_dhfr_fixture.py from the reference's own test inputs).  The
+oracle reproduces the test's vdW, permanent and polarization energies within
+its 1e-2 kcal/mol tolerance and its interaction counts exactly."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from forcefieldx_amd import tinker
+from forcefieldx_amd.forcefield import FRAME_BISECTOR, FRAME_ZTHENX
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "dhfr")
+REF = "/root/reference/modules/potential/src"
+
+
+def load_fixture():
+    return tinker.load(os.path.join(GOLD, "dhfr.xyz.gz"), os.path.join(GOLD, "dhfr.key"),
+                       os.path.join(GOLD, "amoeba_bio_2009_dhfr.prm.gz"))
+
+
+def expected():
+    with open(os.path.join(GOLD, "expected.json")) as f:
+        return json.load(f)
+
+
+def test_dhfr_fixture_topology():
+    xyz, box, top, p = load_fixture()
+    assert top.n == 23558 and xyz.shape == (23558, 3)
+    assert box == [62.23, 62.23, 62.23, 90.0, 90.0, 90.0]
+    assert p.grid == (80, 80, 80)   # 62.23 * 1.2 -> 75 -> 76 -> 80 (preferred dimension)
+    assert (p.ewald_cutoff, p.vdw_cutoff, p.ewald_alpha, p.polar_eps, p.scf_algorithm) == (6.7, 9.0, 0.57, 1e-6, 1)
+    assert abs(top.mpole[:, 0].sum() + 11.0) < 1e-9          # DHFR net charge
+    assert np.count_nonzero(top.frame == FRAME_ZTHENX) == 16499
+    assert np.count_nonzero(top.frame == FRAME_BISECTOR) == 7059
+    assert int((top.red_value > 0).sum()) == 15267
+    # the 7,023 waters (the last 21,069 atoms) are polarization groups of three
+    g = top.group[-3 * 7023:]
+    assert len(np.unique(g)) == 7023 and np.all(np.bincount(g - g.min())[np.unique(g) - g.min()] == 3)
+
+
+def test_xyz_round_trip(tmp_path):
+    xf = tinker.read_xyz(os.path.join(GOLD, "dhfr.xyz.gz"))
+    sub = tinker.XYZFile("part", xf.names[:30], xf.xyz[:30], xf.types[:30],
+                         [(i, j) for i, j in xf.bonds if j < 30], [10.0, 11.0, 12.0, 90.0, 95.0, 90.0])
+    path = tmp_path / "part.xyz"
+    tinker.write_xyz(path, sub)
+    back = tinker.read_xyz(path)
+    assert back.names == sub.names and back.bonds == sub.bonds and back.box == sub.box
+    assert np.array_equal(back.types, sub.types) and np.abs(back.xyz - sub.xyz).max() < 1e-6
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="the reference's files are not in this checkout")
+def test_fixture_equals_reference_files():
+    xyz, box, top, p = load_fixture()
+    x2, b2, t2, p2 = tinker.load(REF + "/test/resources/dhfr.xyz", REF + "/test/resources/dhfr.key",
+                                 REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_BIO_2009")
+    assert np.abs(xyz - x2).max() < 1e-6 and box == b2 and p == p2
+    for k in ("frame", "axis", "polarity", "thole", "pdamp", "group", "vdw_class", "red_index", "red_value",
+              "mask_off", "mask_atom", "mask_rel"):
+        assert np.array_equal(getattr(top, k), getattr(t2, k)), k
+    assert np.abs(top.mpole - t2.mpole).max() < 1e-12 and np.array_equal(top.vdw_table, t2.vdw_table)
+
+
+def test_oracle_reproduces_dhfr_energy_test(oracle):
+    """vdW, permanent and polarization energy and interaction counts of the
+    reference's DHFR benchmark (SOR to 1e-6 D, as its key file asks)."""
+    xyz, box, top, p = load_fixture()
+    e = oracle.evaluate(oracle.System(xyz, box, top), p)[0]
+    ref = expected()
+    for k in ("vdw", "perm", "pol"):
+        assert abs(e[k] - ref[k]) <= ref["tolerance"], (k, e[k], ref[k])
+    assert e["n_vdw"] == ref["n_vdw"] and e["n_perm"] == ref["n_perm"]
+    # the permanent and vdW terms hold far tighter than the test's tolerance
+    assert abs(e["vdw"] - ref["vdw"]) <= 1e-6 and abs(e["perm"] - ref["perm"]) <= 1e-6
