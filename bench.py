@@ -395,6 +395,13 @@ WORKLOADS = {
     "chain-23557": "2500-atom random-walk chain with random multipoles in 7019 waters, 62.2 A box, PME 64^3, "
                    "20 soft atoms at lambda 0.5: OST electrostatics + vdW softcore with dU/dL",
     "water-648": "216 AMOEBA-2003 waters, 18.643 A cubic box, PME 24^3, PCG 1e-6 D",
+    "dhfr-23558": "the reference's DHFR benchmark (EnergyTest): 23,558 atoms of AMOEBA-BIO-2009 protein and water, "
+                  "62.23 A cubic box, PME 80^3 order 5, Ewald 6.7 A (alpha 0.57), vdW 9 A, SOR to 1e-6 D as its "
+                  "key file asks; coordinates and parameters from the Tinker fixture in tests/golden/dhfr",
+    "ubiquitin-9737": "the reference's ubiquitin benchmark (EnergyTest): 9,737 atoms of AMOEBA-BIO-2009 protein and "
+                      "water, 54.99 x 41.91 x 41.91 A cell, PME 70 x 54 x 54 order 5, Ewald 6.7 A (alpha 0.57), "
+                      "vdW 9 A, PCG to 1e-6 D; coordinates and parameters from the Tinker fixture in "
+                      "tests/golden/ubiquitin",
     "crystals-512": "512 independent P1 triclinic cells (lengths 8-20 A, angles 70-110 deg) of small molecules "
                     "with random multipoles, per-cell PME grids 16-32, Ewald and vdW 6 A, PCG 1e-5 D, image sums "
                     "in cells thinner than twice the cutoff; a step is one sweep over all cells (nb_evaluate_batch)",
@@ -421,6 +428,13 @@ def build_workload(name):
         systems = crystal_batch(512, seed=SEED)
         params = NonbondedParams(ewald_cutoff=6.0, vdw_cutoff=6.0, polar_eps=1e-5)
         return systems, None, None, params, None, None
+    elif name in ("dhfr-23558", "ubiquitin-9737"):
+        from forcefieldx_amd import tinker
+        stem = name.split("-")[0]
+        gold = os.path.join(ROOT, "tests", "golden", stem)
+        xyz, box, top, params = tinker.load(os.path.join(gold, f"{stem}.xyz.gz"), os.path.join(gold, f"{stem}.key"),
+                                            os.path.join(gold, f"amoeba_bio_2009_{stem}.prm.gz"))
+        grid = params.grid
     elif name == "water-648":
         xyz, box, top = water_box(216, 18.643, seed=1)
         grid = (24, 24, 24)
@@ -507,9 +521,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG",
-        "data": "synthetic",
+        "data": (f"the reference's own test structure (tests/golden/{args.workload.split('-')[0]})"
+                 if args.workload in ("dhfr-23558", "ubiquitin-9737") else "synthetic"),
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "atoms": top.n, "grid": list(grid),
-                   "launch": "HIP graph replay" if graph else "regular launches", "parallelism": "single GPU"},
+                   "launch": ("regular launches (SOR solves are not recorded)" if params.scf_algorithm == 1 else
+                              "HIP graph replay" if graph else "regular launches"),
+                   "parallelism": "single GPU"},
         "fp64_evals_per_s": 1.0 / t64,
         "fp64_ms_per_step": t64 * 1e3,
         "mixed_over_fp64": t64 / tmx,

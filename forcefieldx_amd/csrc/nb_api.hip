@@ -427,7 +427,9 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
       while (m > 1 && m % 2 == 0) m /= 2;
       while (m > 1 && m % 3 == 0) m /= 3;
       while (m > 1 && m % 5 == 0) m /= 5;
-      if (params->do_pme && m != 1) throw std::runtime_error("PME grid dimensions must factor over 2, 3 and 5");
+      while (m > 1 && m % 7 == 0) m /= 7;
+      // the factors Complex.preferredDimension admits (2, 3, 5, 6, 7)
+      if (params->do_pme && m != 1) throw std::runtime_error("PME grid dimensions must factor over 2, 3, 5 and 7");
     }
     if (params->pme_order != 5) throw std::runtime_error("only B-spline order 5 is implemented");
     // every index a kernel dereferences is validated here

@@ -183,7 +183,8 @@ __device__ CX<T> *pp_fft(CX<T> *a, CX<T> *b, int N, int nlines, int sl, int se, 
     if (R == 4) pp_pass<4, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
     else if (R == 2) pp_pass<2, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
     else if (R == 3) pp_pass<3, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
-    else pp_pass<5, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
+    else if (R == 5) pp_pass<5, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
+    else pp_pass<7, LINE_FAST, T>(a, b, N, nlines, sl, se, Ns, tw, s);
     Ns *= R;
     CX<T> *t = a;
     a = b;

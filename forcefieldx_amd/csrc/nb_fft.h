@@ -1,5 +1,5 @@
 // nb_fft.h -- grid routines shared by the per-system PME kernels and the
-// batched small-cell kernel: in-LDS Stockham line FFTs (radix 2/3/4/5) over a
+// batched small-cell kernel: in-LDS Stockham line FFTs (radix 2/3/4/5/7) over a
 // chunk of grid lines, and the potential gather of one atom.
 #pragma once
 #include "nb_atom.h"
@@ -9,7 +9,7 @@ template <typename T> struct alignas(2 * sizeof(T)) CX { T x, y; };
 __device__ __forceinline__ int wrapg(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 
 // ---------------------------------------------------------------------------
-// FFT: in-place Stockham autosort in LDS, radix 2/3/4/5.  A pass reads every
+// FFT: in-place Stockham autosort in LDS, radix 2/3/4/5/7.  A pass reads every
 // butterfly input of the workgroup into registers, synchronises, and writes
 // the outputs back to the same buffer, so one LDS buffer suffices.  Lines
 // live at buf[l * sl + e * se]; LINE_FAST puts consecutive threads on
@@ -47,6 +47,27 @@ __device__ __forceinline__ void small_dft(CX<T> *v, T s) {
     v[1] = cadd(b, d);
     v[2] = csub(a, c);
     v[3] = csub(b, d);
+  } else if (R == 7) {
+    // X_m = v0 + sum_k cos(2 pi k m / 7) t_k + i s sum_k sin(2 pi k m / 7) d_k,
+    // t_k = v_k + v_(7-k), d_k = v_k - v_(7-k); X_(7-m) takes the minus sign
+    const T c1 = (T)0.62348980185873353, c2 = (T)-0.22252093395631440, c3 = (T)-0.90096886790241913;
+    const T s1 = (T)0.78183148246802981, s2 = (T)0.97492791218182361, s3 = (T)0.43388373911755812;
+    CX<T> t1 = cadd(v[1], v[6]), d1 = csub(v[1], v[6]);
+    CX<T> t2 = cadd(v[2], v[5]), d2 = csub(v[2], v[5]);
+    CX<T> t3 = cadd(v[3], v[4]), d3 = csub(v[3], v[4]);
+    CX<T> m1{v[0].x + c1 * t1.x + c2 * t2.x + c3 * t3.x, v[0].y + c1 * t1.y + c2 * t2.y + c3 * t3.y};
+    CX<T> m2{v[0].x + c2 * t1.x + c3 * t2.x + c1 * t3.x, v[0].y + c2 * t1.y + c3 * t2.y + c1 * t3.y};
+    CX<T> m3{v[0].x + c3 * t1.x + c1 * t2.x + c2 * t3.x, v[0].y + c3 * t1.y + c1 * t2.y + c2 * t3.y};
+    CX<T> r1 = cis(CX<T>{s1 * d1.x + s2 * d2.x + s3 * d3.x, s1 * d1.y + s2 * d2.y + s3 * d3.y}, s);
+    CX<T> r2 = cis(CX<T>{s2 * d1.x - s3 * d2.x - s1 * d3.x, s2 * d1.y - s3 * d2.y - s1 * d3.y}, s);
+    CX<T> r3 = cis(CX<T>{s3 * d1.x - s1 * d2.x + s2 * d3.x, s3 * d1.y - s1 * d2.y + s2 * d3.y}, s);
+    v[0] = cadd(v[0], cadd(t1, cadd(t2, t3)));
+    v[1] = cadd(m1, r1);
+    v[6] = csub(m1, r1);
+    v[2] = cadd(m2, r2);
+    v[5] = csub(m2, r2);
+    v[3] = cadd(m3, r3);
+    v[4] = csub(m3, r3);
   } else {
     // cos and sin of 2 pi / 5 and 4 pi / 5
     const T c1 = (T)0.30901699437494742410, s1 = (T)0.95105651629515357212;
@@ -114,7 +135,8 @@ __device__ void lds_fft(CX<T> *buf, int N, int nlines, int sl, int se, const Rad
     if (R == 4) lds_pass<4, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
     else if (R == 2) lds_pass<2, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
     else if (R == 3) lds_pass<3, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
-    else lds_pass<5, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
+    else if (R == 5) lds_pass<5, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
+    else lds_pass<7, EPT, LINE_FAST, T>(buf, N, nlines, sl, se, Ns, tw, s);
     Ns *= R;
   }
 }

@@ -15,7 +15,7 @@
 //    brick and the spill of up to 7 neighbours into the interleaved complex
 //    grid (real part: permanent / u, imaginary part: uCR), inside the load of
 //    the forward x transform.  No global atomics.
-//  * k_ct_* / k_fft_lines: in-place in-LDS Stockham FFT (radix 2/3/4/5) along
+//  * k_ct_* / k_fft_lines: in-place in-LDS Stockham FFT (radix 2/3/4/5/7) along
 //    x (fused with the brick combine on the forward pass) and y, and a fused
 //    forward-z / influence multiply / inverse-z kernel
 //    (Complex3DParallel.convolution, Complex3DParallel.java:445-453, 667-798);
@@ -319,7 +319,8 @@ Radices factor(int N) {
   while (m % 2 == 0) { r.r[r.nr++] = 2; m /= 2; }
   while (m % 3 == 0) { r.r[r.nr++] = 3; m /= 3; }
   while (m % 5 == 0) { r.r[r.nr++] = 5; m /= 5; }
-  if (m != 1) throw std::runtime_error("PME grid dimension must factor over 2, 3, 5");
+  while (m % 7 == 0) { r.r[r.nr++] = 7; m /= 7; }
+  if (m != 1) throw std::runtime_error("PME grid dimension must factor over 2, 3, 5, 7");
   return r;
 }
 
@@ -342,7 +343,7 @@ int lines_per_block(int N) {
 constexpr int CT_BS = 256;
 
 template <int M> struct RadixOf {
-  static constexpr int v = (M % 4 == 0) ? 4 : (M % 2 == 0) ? 2 : (M % 3 == 0) ? 3 : 5;
+  static constexpr int v = (M % 4 == 0) ? 4 : (M % 2 == 0) ? 2 : (M % 3 == 0) ? 3 : (M % 5 == 0) ? 5 : 7;
 };
 template <int N> struct CtLines {
   static constexpr int raw = 1024 / N;

@@ -92,13 +92,20 @@ def read_xyz(src) -> XYZFile:
     return XYZFile(title, names, xyz, types, sorted(bonds), box)
 
 
+def _create(path):
+    """Text output; a .gz path is compressed with a fixed header time, so the
+    same content gives the same bytes."""
+    if str(path).endswith(".gz"):
+        return io.TextIOWrapper(gzip.GzipFile(path, "wb", mtime=0), encoding="utf-8")
+    return open(path, "w")
+
+
 def write_xyz(path, xf: XYZFile):
     adj = [[] for _ in range(len(xf.names))]
     for i, j in xf.bonds:
         adj[i].append(j)
         adj[j].append(i)
-    opener = gzip.open if str(path).endswith(".gz") else open
-    with opener(path, "wt") as f:
+    with _create(path) as f:
         f.write(f"{len(xf.names):6d}  {xf.title}\n")
         if xf.box is not None:
             f.write(" ".join(f"{v:.6f}" for v in xf.box) + "\n")
@@ -203,8 +210,7 @@ def write_prm(path, ff: ForceFieldParams, types=None):
     given atom types and their classes when types is set)."""
     tset = None if types is None else set(int(t) for t in types)
     cset = None if tset is None else {ff.atom[t][0] for t in tset}
-    opener = gzip.open if str(path).endswith(".gz") else open
-    with opener(path, "wt") as f:
+    with _create(path) as f:
         for k, v in ff.keys.items():
             f.write(f"{k:<24s}{' '.join(v)}\n")
         for t, (c, nm) in sorted(ff.atom.items()):

@@ -69,7 +69,7 @@ typedef struct {
   double vdw_taper;         /* taper start as a fraction of the cutoff (0.9) */
   double list_buffer;       /* neighbor-list buffer, A (ForceFieldEnergy.java:244) */
   int32_t pme_order;        /* B-spline order (5) */
-  int32_t grid[3];          /* PME grid */
+  int32_t grid[3];          /* PME grid, each a product of 2, 3, 5 and 7 (Complex.java preferredDimension) */
   double polar_eps;         /* PCG convergence, RMS Debye (ParticleMeshEwald.java:171) */
   double electric;          /* 332.0637133 (Constants.java:192) */
   double precond_cutoff;    /* 4.5 A (PCGSolver.java:300) */

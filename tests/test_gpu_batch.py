@@ -71,6 +71,27 @@ def test_batch_kernel_matches_oracle(oracle, pol):
         close_all(hb)
 
 
+def test_batch_kernel_radix7_grids(oracle):
+    """Per-cell grids with factors of 7 (and 49) through the batch kernel's
+    line transforms."""
+    grids = [(21, 28, 14), (35, 14, 28), (28, 28, 21), (42, 21, 14), (49, 14, 16), (14, 56, 21)]
+    systems = [(xyz, box, top, grids[b]) for b, (xyz, box, top, _) in enumerate(crystal_batch(6, seed=4))]
+    hb = handles(systems, ffi.NB_PREC_FP64, False)
+    try:
+        eb, rc = ffi.evaluate_batch(hb, ffi.NB_GRAD)
+        assert rc == ffi.NB_OK
+        assert ffi.batch_config()["systems"] == len(systems)
+        for b, (xyz, box, top, grid) in enumerate(systems):
+            print(f"radix-7 batch cell {b}: grid {grid}, {top.n} atoms", flush=True)
+            eo, go, _, _ = oracle.evaluate(oracle.System(xyz, box, top), dataclasses.replace(PARAMS, grid=grid))
+            scale = max(abs(eo[k]) for k in ("perm", "pol", "vdw"))
+            for k in ("perm_recip", "pol", "total"):
+                assert abs(eb[b][k] - eo[k]) <= 1e-8 * scale, (b, k, eb[b][k], eo[k])
+            assert rel(hb[b].gradient().reshape(-1, 3), go) <= 1e-8, b
+    finally:
+        close_all(hb)
+
+
 def test_batch_kernel_mixed_against_fp64():
     systems = crystal_batch(16, seed=8)
     p = dataclasses.replace(PARAMS, polar_eps=1e-6)

@@ -49,8 +49,10 @@ def test_water648_fp64_parity(oracle, water648, pol):
 
 # grids that take the per-plane 2D kernel and grids whose planes exceed it
 # (line kernels along x and y): (96, 96, 12) in both precisions, (64, 64, 16)
-# in FP64 only
-CONV_DIMS = ((32, 32, 32), (24, 24, 24), (20, 18, 10), (144, 16, 24), (96, 96, 12), (64, 64, 16), (30, 45, 8))
+# in FP64 only; grids with factors of 7, which the reference's density rule
+# picks (70 x 54 x 54 for a 54.99 x 41.91 x 41.91 A cell)
+CONV_DIMS = ((32, 32, 32), (24, 24, 24), (20, 18, 10), (144, 16, 24), (96, 96, 12), (64, 64, 16), (30, 45, 8),
+             (70, 54, 54), (28, 21, 49), (98, 14, 56))
 
 
 @pytest.mark.parametrize("prec,tol", [(ffi.NB_PREC_FP64, 1e-13), (ffi.NB_PREC_MIXED, 2e-6)])
@@ -67,7 +69,7 @@ def test_gpu_convolution_identity(prec, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [(ffi.NB_PREC_FP64, 1e-12), (ffi.NB_PREC_MIXED, 1e-5)])
-@pytest.mark.parametrize("dims", [(24, 20, 18), (96, 96, 12)])
+@pytest.mark.parametrize("dims", [(24, 20, 18), (96, 96, 12), (28, 21, 35)])
 def test_gpu_convolution_matches_oracle(oracle, prec, tol, dims):
     nx, ny, nz = dims
     rng = np.random.default_rng(11)

@@ -1,9 +1,12 @@
-"""Tinker-format input (forcefieldx_amd.tinker) and the reference's DHFR
-EnergyTest (EnergyTest.java:148-180): 23,558 atoms of AMOEBA-BIO-2009 protein
-and water read from the committed fixture (tests/golden/dhfr, written by
-tests/golden/make_dhfr_fixture.py from the reference's own test inputs).  The
-oracle reproduces the test's vdW, permanent and polarization energies within
-its 1e-2 kcal/mol tolerance and its interaction counts exactly."""
+"""Tinker-format input (forcefieldx_amd.tinker) and two periodic
+AMOEBA-BIO-2009 systems of the reference's EnergyTest: "DHFR Benchmark"
+(EnergyTest.java:148-180, 23,558 atoms) and "Ubiquitin Benchmark"
+(:363-394, 9,737 atoms, PME grid 70 x 54 x 54 with factors 7 and 6), read
+from the committed fixtures (tests/golden/<name>, written by
+tests/golden/make_energy_test_fixtures.py from the reference's own test
+inputs).  The oracle reproduces each test's vdW, permanent and polarization
+energies within its 1e-2 kcal/mol tolerance and its interaction counts
+exactly."""
 import json
 import os
 
@@ -13,17 +16,19 @@ import pytest
 from forcefieldx_amd import tinker
 from forcefieldx_amd.forcefield import FRAME_BISECTOR, FRAME_ZTHENX
 
-GOLD = os.path.join(os.path.dirname(__file__), "golden", "dhfr")
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 REF = "/root/reference/modules/potential/src"
+SYSTEMS = ("dhfr", "ubiquitin")
 
 
-def load_fixture():
-    return tinker.load(os.path.join(GOLD, "dhfr.xyz.gz"), os.path.join(GOLD, "dhfr.key"),
-                       os.path.join(GOLD, "amoeba_bio_2009_dhfr.prm.gz"))
+def load_fixture(name="dhfr"):
+    d = os.path.join(GOLDEN, name)
+    return tinker.load(os.path.join(d, f"{name}.xyz.gz"), os.path.join(d, f"{name}.key"),
+                       os.path.join(d, f"amoeba_bio_2009_{name}.prm.gz"))
 
 
-def expected():
-    with open(os.path.join(GOLD, "expected.json")) as f:
+def expected(name="dhfr"):
+    with open(os.path.join(GOLDEN, name, "expected.json")) as f:
         return json.load(f)
 
 
@@ -42,8 +47,19 @@ def test_dhfr_fixture_topology():
     assert len(np.unique(g)) == 7023 and np.all(np.bincount(g - g.min())[np.unique(g) - g.min()] == 3)
 
 
+def test_ubiquitin_fixture_topology():
+    xyz, box, top, p = load_fixture("ubiquitin")
+    assert top.n == 9737 and box == [54.99, 41.91, 41.91, 90.0, 90.0, 90.0]
+    assert p.grid == (70, 54, 54)   # 66.0 -> 66 -> 70 and 50.3 -> 52 -> 54 (preferred dimensions)
+    assert (p.ewald_cutoff, p.vdw_cutoff, p.ewald_alpha, p.polar_eps, p.scf_algorithm) == (6.7, 9.0, 0.57, 1e-6, 0)
+    # the 2,835 waters (the last 8,505 atoms) are neutral polarization groups of three
+    g = top.group[-3 * 2835:]
+    assert len(np.unique(g)) == 2835
+    assert abs(top.mpole[-3 * 2835:, 0].sum()) < 1e-9
+
+
 def test_xyz_round_trip(tmp_path):
-    xf = tinker.read_xyz(os.path.join(GOLD, "dhfr.xyz.gz"))
+    xf = tinker.read_xyz(os.path.join(GOLDEN, "dhfr", "dhfr.xyz.gz"))
     sub = tinker.XYZFile("part", xf.names[:30], xf.xyz[:30], xf.types[:30],
                          [(i, j) for i, j in xf.bonds if j < 30], [10.0, 11.0, 12.0, 90.0, 95.0, 90.0])
     path = tmp_path / "part.xyz"
@@ -54,9 +70,10 @@ def test_xyz_round_trip(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="the reference's files are not in this checkout")
-def test_fixture_equals_reference_files():
-    xyz, box, top, p = load_fixture()
-    x2, b2, t2, p2 = tinker.load(REF + "/test/resources/dhfr.xyz", REF + "/test/resources/dhfr.key",
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_fixture_equals_reference_files(name):
+    xyz, box, top, p = load_fixture(name)
+    x2, b2, t2, p2 = tinker.load(REF + f"/test/resources/{name}.xyz", REF + f"/test/resources/{name}.key",
                                  REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_BIO_2009")
     assert np.abs(xyz - x2).max() < 1e-6 and box == b2 and p == p2
     for k in ("frame", "axis", "polarity", "thole", "pdamp", "group", "vdw_class", "red_index", "red_value",
@@ -65,12 +82,14 @@ def test_fixture_equals_reference_files():
     assert np.abs(top.mpole - t2.mpole).max() < 1e-12 and np.array_equal(top.vdw_table, t2.vdw_table)
 
 
-def test_oracle_reproduces_dhfr_energy_test(oracle):
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_oracle_reproduces_energy_test(oracle, name):
     """vdW, permanent and polarization energy and interaction counts of the
-    reference's DHFR benchmark (SOR to 1e-6 D, as its key file asks)."""
-    xyz, box, top, p = load_fixture()
+    reference's benchmark (DHFR: SOR to 1e-6 D, as its key file asks;
+    ubiquitin: PCG to 1e-6 D, the default)."""
+    xyz, box, top, p = load_fixture(name)
     e = oracle.evaluate(oracle.System(xyz, box, top), p)[0]
-    ref = expected()
+    ref = expected(name)
     for k in ("vdw", "perm", "pol"):
         assert abs(e[k] - ref[k]) <= ref["tolerance"], (k, e[k], ref[k])
     assert e["n_vdw"] == ref["n_vdw"] and e["n_perm"] == ref["n_perm"]
