@@ -70,6 +70,7 @@ void alloc(T **dst, size_t count) {
 }
 
 void free_all(NbHandle *h) {
+  nb_valence_free(h);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
   h->gexec = nullptr;
@@ -222,6 +223,9 @@ void set_ost_phases(NbHandle *h) {
 }
 
 }  // namespace
+
+// for the other translation units (nb_set_valence)
+void nb_drop_graph(NbHandle *h) { drop_graph(h); }
 
 extern "C" int nb_create(const nb_params *params, const nb_topology *top, void **handle);
 extern "C" int nb_set_cell(void *handle, const double box[6]);
@@ -682,11 +686,13 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   }
   nb_zero_outputs(h);
   if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
-  // vdW runs on the side stream, overlapping everything up to the gradient assembly
-  if (h->prm.do_vdw) {
+  // vdW and the valence terms run on the side stream, overlapping everything
+  // up to the gradient assembly
+  if (h->prm.do_vdw || h->has_valence) {
     nb_fork(h);
     NbSide side(h);
-    nb_vdw(h);
+    if (h->prm.do_vdw) nb_vdw(h);
+    nb_valence_terms(h);
   }
   bool polar = h->prm.polarization != NB_POL_NONE && h->ph.polar;
   if (h->prm.do_pme) {
@@ -741,6 +747,11 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   nb_join(h);
   if (h->finite_active) nb_finite_add(h);
   nb_finalize_gradient(h);
+  if (h->has_valence) {
+    nb_valence_gather(h);
+    HIPCHECK(hipMemcpyAsync(h->h_pinned + 54, h->d_val_energy, sizeof(double) * 8, hipMemcpyDeviceToHost,
+                            h->stream));
+  }
   if (h->want_vir) {
     nb_vir_reduce(h);
     HIPCHECK(hipMemcpyAsync(h->h_pinned + 16, h->d_vir, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
@@ -812,7 +823,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       // an aperiodic system has pair and frame terms only
       // 9 partial sums per block of every contributing launch
       long pb = xcd_grid((long)n * 16, 256).x, sb = xcd_grid((long)n * 4, 256).x, ab = (n + 127) / 128;
-      long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + 64;
+      long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + (n + 255) / 256 + 64;
       if (need > h->vpart_cap) {
         drop_graph(h);
         if (h->d_vpart) hipFree(h->d_vpart);
@@ -908,6 +919,11 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     e.perm = e.perm_real + e.perm_self + e.perm_recip + e.perm_charge_corr;
     e.pol = e.pol_real + e.pol_self + e.pol_recip;
     e.total = e.perm + e.pol + e.vdw;
+    if (h->has_valence)
+      for (int k = 0; k < 8; k++) {
+        e.valence[k] = h->h_pinned[54 + k];
+        e.total += e.valence[k];
+      }
     if (status == NB_OK && fl[4]) status = NB_ERR_BAD_TORQUE;
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
     // record a graph for the next evaluations: the PCG budget is this
@@ -988,6 +1004,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
     e.perm = e.perm_real + e.perm_self + e.perm_recip + e.perm_charge_corr;
     e.pol = e.pol_real + e.pol_self + e.pol_recip;
     e.total = e.perm + e.pol + e.vdw;
+    for (int k = 0; k < 8; k++) e.total += e.valence[k];
     if (!std::isfinite(e.total)) {
       h->err = "NaN energy";
       status = NB_ERR_NAN_ENERGY;

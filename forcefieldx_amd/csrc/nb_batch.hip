@@ -1193,7 +1193,8 @@ static bool same_params(const nb_params &a, const nb_params &b) {
 // alchemistry, virial or SOR, and a cell whose image range covers the list
 // cutoff
 static bool batchable(const NbHandle *h, uint32_t flags) {
-  if (!h->cell_set || !h->positions_set || h->cell.aperiodic || h->elec_lambda || h->ost || h->profiling)
+  if (!h->cell_set || !h->positions_set || h->cell.aperiodic || h->elec_lambda || h->ost || h->profiling ||
+      h->has_valence)
     return false;
   if (flags & ~(uint32_t)NB_GRAD) return false;
   if (h->has_soft && h->lambda < 1.0) return false;

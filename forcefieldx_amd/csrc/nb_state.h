@@ -185,6 +185,19 @@ struct NbHandle {
   double *d_gfin = nullptr;       // [3n] tensor-derivative gradient terms (sorted)
   bool induced_caller = false;    // d_u / d_up hold caller order (written by the batch kernel)
 
+  // valence terms (nb_set_valence): per-term gradient contributions go to
+  // slots, each atom sums its slots in a fixed order (no atomics)
+  bool has_valence = false;
+  long val_terms = 0, val_slots = 0;
+  void *val_dev = nullptr;        // device-side term table (VDev, nb_valence.hip)
+  int32_t *d_val_i = nullptr;     // atom indices and grid integers
+  double *d_val_d = nullptr;      // parameters and grids
+  double *d_val_slot = nullptr;   // [3 val_slots]
+  int32_t *d_val_off = nullptr;   // [n + 1] slots of each atom
+  int32_t *d_val_list = nullptr;  // [val_slots]
+  double *d_val_eterm = nullptr;  // [val_terms]
+  double *d_val_energy = nullptr; // [8]
+
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
   // replayed; the PCG then runs graph_iters iterations under device-side
   // control (kernels that only serve a stopped solve return at once)
@@ -237,6 +250,9 @@ double *nb_vir_region(NbHandle *h, long nblocks);
 void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
 void nb_tape_push(NbHandle *h);
+void nb_valence_terms(NbHandle *h);
+void nb_valence_gather(NbHandle *h);
+void nb_valence_free(NbHandle *h);
 void nb_finite_adjoint(NbHandle *h);
 void nb_finite_add(NbHandle *h);
 bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,

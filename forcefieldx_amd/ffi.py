@@ -24,7 +24,9 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_evaluate", "nb_get_gradient", "nb_gradient_device", "nb_get_induced", "nb_get_virial",
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
-            "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config"]
+            "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config", "nb_set_valence",
+            "nb_valence_struct_size"]
+VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -63,10 +65,32 @@ class NbEnergies(C.Structure):
                 ("pol_recip", C.c_double), ("vdw", C.c_double), ("perm", C.c_double), ("pol", C.c_double),
                 ("total", C.c_double), ("dudl", C.c_double), ("d2udl2", C.c_double), ("pcg_iters", C.c_int32),
                 ("pcg_eps", C.c_double), ("n_vdw", C.c_int64), ("n_perm", C.c_int64),
-                ("list_rebuilt", C.c_int32)]
+                ("list_rebuilt", C.c_int32), ("valence", C.c_double * 8)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "valence"}
+        d.update({"valence_" + k: self.valence[i] for i, k in enumerate(VALENCE_KINDS)})
+        d["valence"] = sum(self.valence)
+        return d
+
+
+class NbValence(C.Structure):
+    _fields_ = [("n_bond", C.c_int32), ("bond", _ip), ("bond_k", _dp), ("bond_r0", _dp),
+                ("bond_cubic", C.c_double), ("bond_quartic", C.c_double),
+                ("n_angle", C.c_int32), ("angle", _ip), ("angle_k", _dp), ("angle_t0", _dp),
+                ("angle_poly", C.c_double * 4),
+                ("n_strbnd", C.c_int32), ("strbnd", _ip), ("strbnd_k", _dp), ("strbnd_r0", _dp),
+                ("strbnd_t0", _dp),
+                ("n_urey", C.c_int32), ("urey", _ip), ("urey_k", _dp), ("urey_r0", _dp),
+                ("urey_cubic", C.c_double), ("urey_quartic", C.c_double),
+                ("n_opbend", C.c_int32), ("opbend", _ip), ("opbend_k", _dp), ("opbend_poly", C.c_double * 4),
+                ("n_torsion", C.c_int32), ("torsion", _ip), ("torsion_amp", _dp), ("torsion_cos", _dp),
+                ("torsion_sin", _dp),
+                ("n_pitors", C.c_int32), ("pitors", _ip), ("pitors_k", _dp),
+                ("n_tortor", C.c_int32), ("tortor", _ip), ("tortor_grid", _ip),
+                ("n_grid", C.c_int32), ("grid_nx", _ip), ("grid_ny", _ip), ("grid_off_t", _ip),
+                ("grid_off_e", _ip), ("grid_t", _dp), ("grid_e", _dp), ("grid_dx", _dp), ("grid_dy", _dp),
+                ("grid_dxy", _dp)]
 
 
 _lib = None
@@ -116,6 +140,9 @@ def load(path=LIB_PATH):
     L.nb_profile_count.restype = C.c_int
     L.nb_profile_entry.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_int64)]
     L.nb_profile_entry.restype = C.c_int
+    L.nb_set_valence.argtypes = [vp, C.POINTER(NbValence)]
+    L.nb_set_valence.restype = C.c_int
+    L.nb_valence_struct_size.restype = C.c_int
     L.nb_test_convolution.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
     L.nb_test_convolution.restype = C.c_int
     _lib = L
@@ -196,6 +223,26 @@ class Handle:
         g = np.zeros(3 * self.n)
         self._check(self.L.nb_get_gradient(self.h, _p(g, C.c_double)))
         return g.reshape(self.n, 3)
+
+    def set_valence(self, terms):
+        """Valence terms (forcefieldx_amd.bonded.ValenceTerms) evaluated with
+        every later evaluation; None removes them."""
+        if terms is None:
+            self._check(self.L.nb_set_valence(self.h, None))
+            return
+        f = terms.flat()
+        c = terms.counts()
+        v = NbValence()
+        for name, arr in f.items():
+            setattr(v, name, _p(arr, C.c_int32) if arr.dtype == np.int32 else _p(arr, C.c_double))
+        for k in VALENCE_KINDS:
+            setattr(v, "n_" + k, c[k])
+        v.n_grid = len(terms.grids)
+        v.bond_cubic, v.bond_quartic = terms.bond_poly
+        v.urey_cubic, v.urey_quartic = terms.urey_poly
+        v.angle_poly = (C.c_double * 4)(*terms.angle_poly)
+        v.opbend_poly = (C.c_double * 4)(*terms.opbend_poly)
+        self._check(self.L.nb_set_valence(self.h, C.byref(v)))   # the library copies the arrays
 
     def set_lambda(self, lam):
         self._check(self.L.nb_set_lambda(self.h, float(lam)))

@@ -8,7 +8,10 @@
 * AMOEBA parameter files: the `atom`, `vdw`, `vdwpr`, `multipole` (five-line
   records), `polarize` records and the scalar keywords of the vdW form, the
   mask scales and the polarization model (parsers/ForceFieldFilter.java;
-  MultipoleType.parse, MultipoleType.java:787-898).
+  MultipoleType.parse, MultipoleType.java:787-898); the valence records
+  `bond`, `angle`, `anglep`, `strbnd`, `ureybrad`, `opbend`, `torsion`,
+  `pitors` and `tortors` (with its grid lines) and their unit and polynomial
+  keywords, kept as records for forcefieldx_amd.bonded.
 
 `build` assembles a Topology and NonbondedParams from them:
 
@@ -135,6 +138,9 @@ def read_keys(src) -> dict:
 class ForceFieldParams:
     keys: dict = field(default_factory=dict)          # scalar keywords
     atom: dict = field(default_factory=dict)          # type -> (class, name)
+    element: dict = field(default_factory=dict)       # type -> (description, atomic number, mass, valence)
+    valence: dict = field(default_factory=dict)       # record keyword -> [(classes, values)]
+    tortors: list = field(default_factory=list)       # [(5 classes, nx, ny, [(t1, t2, e)] in file order)]
     vdw: dict = field(default_factory=dict)           # class -> (radius, eps, reduction)
     vdwpr: dict = field(default_factory=dict)         # (c1, c2) -> (radius, eps)
     multipole: dict = field(default_factory=dict)     # (types...) -> (frame, [q, d(3), Q(6)] in Bohr units)
@@ -173,6 +179,22 @@ def read_prm(src) -> ForceFieldParams:
         kw = t[0].lower()
         if kw == "atom":
             ff.atom[int(t[1])] = (int(t[2]), t[3])
+            q = raw.split('"')
+            if len(q) >= 3:
+                rest = q[2].split()
+                if len(rest) >= 3:
+                    ff.element[int(t[1])] = (q[1], int(rest[0]), float(rest[1]), int(rest[2]))
+        elif kw in _VALENCE:
+            nc = _VALENCE[kw]
+            ff.valence.setdefault(kw, []).append((tuple(int(x) for x in t[1:1 + nc]),
+                                                  [float(x) for x in t[1 + nc:]]))
+        elif kw == "tortors":
+            cls, nx, ny = tuple(int(x) for x in t[1:6]), int(t[6]), int(t[7])
+            vals = []
+            while len(vals) < 3 * nx * ny:
+                vals += [float(x) for x in lines[i].replace("\\", " ").split()]
+                i += 1
+            ff.tortors.append((cls, nx, ny, [tuple(vals[3 * k:3 * k + 3]) for k in range(nx * ny)]))
         elif kw == "vdw":
             red = float(t[4]) if len(t) > 4 else 0.0
             ff.vdw[int(t[1])] = (float(t[2]), float(t[3]), red)
@@ -202,7 +224,15 @@ _SCALARS = {"vdwtype", "radiusrule", "radiustype", "radiussize", "epsilonrule", 
             "mpole-14-scale", "mpole-15-scale", "polar-12-scale", "polar-13-scale", "polar-14-scale",
             "polar-15-scale", "polar-12-intra", "polar-13-intra", "polar-14-intra", "polar-15-intra",
             "direct-11-scale", "direct-12-scale", "direct-13-scale", "direct-14-scale", "mutual-11-scale",
-            "mutual-12-scale", "mutual-13-scale", "mutual-14-scale", "forcefield"}
+            "mutual-12-scale", "mutual-13-scale", "mutual-14-scale", "forcefield",
+            "bondunit", "bond-cubic", "bond-quartic", "angleunit", "angle-cubic", "angle-quartic", "angle-pentic",
+            "angle-sextic", "strbndunit", "ureyunit", "urey-cubic", "urey-quartic", "opbendtype", "opbendunit",
+            "opbend-cubic", "opbend-quartic", "opbend-pentic", "opbend-sextic", "torsionunit", "pitorsunit",
+            "tortorunit", "torsion-scale"}
+
+# valence records and their number of atom classes (parameters/*Type.parse)
+_VALENCE = {"bond": 2, "angle": 3, "anglep": 3, "strbnd": 3, "ureybrad": 3, "opbend": 4, "torsion": 4,
+            "pitors": 2}
 
 
 def write_prm(path, ff: ForceFieldParams, types=None):
@@ -215,7 +245,19 @@ def write_prm(path, ff: ForceFieldParams, types=None):
             f.write(f"{k:<24s}{' '.join(v)}\n")
         for t, (c, nm) in sorted(ff.atom.items()):
             if tset is None or t in tset:
-                f.write(f"atom {t:6d} {c:6d}  {nm}\n")
+                el = ff.element.get(t)
+                tail = f'  "{el[0]}" {el[1]:4d} {el[2]:9.4f} {el[3]:3d}' if el else ""
+                f.write(f"atom {t:6d} {c:6d}  {nm}{tail}\n")
+        for kw in _VALENCE:
+            for cls, vals in ff.valence.get(kw, []):
+                if cset is None or all(c in cset or c == 0 for c in cls):
+                    f.write(f"{kw:<8s}" + "".join(f" {c:6d}" for c in cls) + "".join(f" {v:.9g}" for v in vals)
+                            + "\n")
+        for cls, nx, ny, grid in ff.tortors:
+            if cset is None or all(c in cset for c in cls):
+                f.write("tortors " + "".join(f" {c:6d}" for c in cls) + f" {nx:3d} {ny:3d}\n")
+                for t1, t2, e in grid:
+                    f.write(f"  {t1:.9g} {t2:.9g} {e:.9g}\n")
         for c, (r, e, red) in sorted(ff.vdw.items()):
             if cset is None or c in cset:
                 f.write(f"vdw {c:6d} {r:.9f} {e:.9f}" + (f" {red:.3f}" if red > 0 else "") + "\n")
@@ -439,6 +481,13 @@ def build(xf: XYZFile, ff: ForceFieldParams, keys: dict):
     return xf.xyz.copy(), box, top, p
 
 
-def load(xyz_path, key_path, prm_path):
-    """Read the three files and build (xyz, box, Topology, NonbondedParams)."""
-    return build(read_xyz(xyz_path), read_prm(prm_path), read_keys(key_path))
+def load(xyz_path, key_path, prm_path, valence=False):
+    """Read the three files and build (xyz, box, Topology, NonbondedParams),
+    with the valence terms (forcefieldx_amd.bonded.ValenceTerms) appended
+    when asked."""
+    xf, ff, keys = read_xyz(xyz_path), read_prm(prm_path), read_keys(key_path)
+    out = build(xf, ff, keys)
+    if not valence:
+        return out
+    from .bonded import assign
+    return out + (assign(xf.types, xf.bonds, ff, keys),)

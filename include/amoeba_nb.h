@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 5
+#define NB_ABI_VERSION 6
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -142,7 +142,50 @@ typedef struct {
   double pcg_eps;
   int64_t n_vdw, n_perm;
   int32_t list_rebuilt;
+  /* valence terms (nb_set_valence), NB_VAL_* order; total includes them */
+  double valence[8];
 } nb_energies;
+
+/* Valence term kinds, the order of nb_energies.valence (the reference's
+ * EnergyTest columns: bond, angle, stretch-bend, Urey-Bradley, out-of-plane
+ * bend, torsion, pi-orbital torsion, torsion-torsion). */
+enum { NB_VAL_BOND = 0, NB_VAL_ANGLE, NB_VAL_STRBND, NB_VAL_UREY, NB_VAL_OPBEND, NB_VAL_TORSION, NB_VAL_PITORS,
+       NB_VAL_TORTOR };
+
+/* AMOEBA valence terms, assigned by the caller (atom indices 0-based; force
+ * constants already multiplied by their unit keywords; ideal lengths in A,
+ * ideal angles in degrees).  Forms follow Bond.java:242-283,
+ * Angle.java:240-345, StretchBend.java:117-160, UreyBradley.java:123-153,
+ * OutOfPlaneBend.java:172-250, Torsion.java:247-330,
+ * PiOrbitalTorsion.java:145-240 and TorsionTorsion.java:285-430. */
+typedef struct {
+  int32_t n_bond;    const int32_t *bond;     /* [2 n] */
+  const double *bond_k, *bond_r0;
+  double bond_cubic, bond_quartic;
+  int32_t n_angle;   const int32_t *angle;    /* [4 n] end, centre, end, projection atom of an in-plane angle or -1 */
+  const double *angle_k, *angle_t0;
+  double angle_poly[4];                       /* cubic, quartic, pentic, sextic */
+  int32_t n_strbnd;  const int32_t *strbnd;   /* [3 n] */
+  const double *strbnd_k, *strbnd_r0;         /* [2 n] each: first and second bond */
+  const double *strbnd_t0;
+  int32_t n_urey;    const int32_t *urey;     /* [2 n] the angle's end atoms */
+  const double *urey_k, *urey_r0;
+  double urey_cubic, urey_quartic;
+  int32_t n_opbend;  const int32_t *opbend;   /* [4 n] end, trigonal centre, end, out-of-plane atom */
+  const double *opbend_k;
+  double opbend_poly[4];
+  int32_t n_torsion; const int32_t *torsion;  /* [4 n] */
+  const double *torsion_amp, *torsion_cos, *torsion_sin;  /* [6 n]: periodicity 1..6, cos / sin of the phase */
+  int32_t n_pitors;  const int32_t *pitors;   /* [6 n] two neighbours of c, c, d, two neighbours of d */
+  const double *pitors_k;
+  int32_t n_tortor;  const int32_t *tortor;   /* [6 n] five chain atoms, chirality atom of the centre or -1 */
+  const int32_t *tortor_grid;                 /* [n] grid of each term */
+  int32_t n_grid;    const int32_t *grid_nx, *grid_ny;     /* [n_grid] */
+  const int32_t *grid_off_t;                  /* [n_grid] offset into grid_t of tx (nx values) then ty (ny) */
+  const int32_t *grid_off_e;                  /* [n_grid] offset into the value arrays (nx * ny each) */
+  const double *grid_t;                       /* degrees, ascending */
+  const double *grid_e, *grid_dx, *grid_dy, *grid_dxy;   /* e[j nx + i] at (tx[i], ty[j]); spline derivatives */
+} nb_valence;
 
 /* Build a handle: uploads topology, allocates device buffers.
  * Replaces the construction of ParticleMeshEwald/VanDerWaals
@@ -187,6 +230,19 @@ int nb_get_virial(void *handle, double v[9]);
 /* dU/dX/dlambda [3n] of the last evaluation made with NB_DUDL (vdW softcore;
  * VanDerWaals.getdEdXdL and the lambdaGrad of VanDerWaals.java:1869-1881). */
 int nb_get_dudx_dl(void *handle, double *g);
+
+/* Valence terms evaluated with every later nb_evaluate, beside the
+ * nonbonded terms: their energies in nb_energies.valence (and total), their
+ * gradient and strain derivative added to the handle's.  Replaces the
+ * bonded-term setup and region of ForceFieldEnergy (ForceFieldEnergy.java:
+ * 935-1100, 1670-1677).  Arrays are copied; NULL or all counts zero removes
+ * the terms.  Coordinates are used as given (no periodic images), as the
+ * reference's bonded terms do.  NB_ERR_INVALID for an index out of range or
+ * a periodicity table beyond 6 terms. */
+int nb_set_valence(void *handle, const nb_valence *terms);
+
+/* sizeof(nb_valence) as compiled. */
+int nb_valence_struct_size(void);
 
 /* Human-readable message for the last error on this handle. */
 const char *nb_last_error(void *handle);

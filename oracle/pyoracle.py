@@ -18,10 +18,13 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 def build(force=False):
     """Compile the oracle with gcc (-O2, OpenMP)."""
     os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
-    src = os.path.join(HERE, "amoeba_oracle.c")
-    if not force and os.path.exists(LIB_PATH) and os.path.getmtime(LIB_PATH) >= os.path.getmtime(src):
+    srcs = [os.path.join(HERE, f) for f in ("amoeba_oracle.c", "valence_oracle.c", "amoeba_oracle.h",
+                                            "valence_oracle.h")]
+    if not force and os.path.exists(LIB_PATH) and all(os.path.getmtime(LIB_PATH) >= os.path.getmtime(f)
+                                                      for f in srcs):
         return LIB_PATH
-    cmd = ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=gnu11", "-o", LIB_PATH, src, "-lm"]
+    cmd = ["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=gnu11", "-o", LIB_PATH] + \
+        [f for f in srcs if f.endswith(".c")] + ["-lm"]
     subprocess.check_call(cmd)
     return LIB_PATH
 
@@ -105,6 +108,7 @@ def lib():
         L.ora_influence.argtypes = [dp, C.POINTER(C.c_int), C.c_int, C.c_double, dp]
         L.ora_reduced_xyz.argtypes = [C.POINTER(OraSystem), dp]
         L.ora_alchemical_factors.argtypes = [C.POINTER(OraParams), C.c_double, C.POINTER(OraAlchemy)]
+        L.ora_valence_evaluate.argtypes = [C.POINTER(OraValence), dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -250,3 +254,56 @@ def influence(box, grid, order, aewald):
     out = np.zeros(grid[0] * grid[1] * grid[2])
     lib().ora_influence(_dp(np.ascontiguousarray(box, float)), g, order, aewald, _dp(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# valence terms (valence_oracle.c)
+# ---------------------------------------------------------------------------
+_PI, _PD = C.POINTER(C.c_int), C.POINTER(C.c_double)
+
+
+class OraValence(C.Structure):
+    _fields_ = [("n_bond", C.c_int), ("bond", _PI), ("bond_k", _PD), ("bond_r0", _PD),
+                ("bond_cubic", C.c_double), ("bond_quartic", C.c_double),
+                ("n_angle", C.c_int), ("angle", _PI), ("angle_k", _PD), ("angle_t0", _PD),
+                ("angle_poly", C.c_double * 4),
+                ("n_strbnd", C.c_int), ("strbnd", _PI), ("strbnd_k", _PD), ("strbnd_r0", _PD), ("strbnd_t0", _PD),
+                ("n_urey", C.c_int), ("urey", _PI), ("urey_k", _PD), ("urey_r0", _PD),
+                ("urey_cubic", C.c_double), ("urey_quartic", C.c_double),
+                ("n_opbend", C.c_int), ("opbend", _PI), ("opbend_k", _PD), ("opbend_poly", C.c_double * 4),
+                ("n_torsion", C.c_int), ("torsion", _PI), ("torsion_amp", _PD), ("torsion_cos", _PD),
+                ("torsion_sin", _PD),
+                ("n_pitors", C.c_int), ("pitors", _PI), ("pitors_k", _PD),
+                ("n_tortor", C.c_int), ("tortor", _PI), ("tortor_grid", _PI),
+                ("n_grid", C.c_int), ("grid_nx", _PI), ("grid_ny", _PI), ("grid_off_t", _PI), ("grid_off_e", _PI),
+                ("grid_t", _PD), ("grid_e", _PD), ("grid_dx", _PD), ("grid_dy", _PD), ("grid_dxy", _PD)]
+
+
+VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
+
+
+def valence(terms, xyz, virial=False):
+    """Energies of the eight valence kinds (dict), gradient (n, 3) and, when
+    asked, the strain derivative sum_i g_i x_i^T (3, 3)."""
+    f = terms.flat()
+    c = terms.counts()
+    s = OraValence()
+    keep = []
+    for name, arr in f.items():
+        keep.append(arr)
+        setattr(s, name, _ip(arr) if arr.dtype == np.int32 else _dp(arr))
+    for k in VALENCE_KINDS:
+        setattr(s, "n_" + k, c[k])
+    s.n_grid = len(terms.grids)
+    s.bond_cubic, s.bond_quartic = terms.bond_poly
+    s.urey_cubic, s.urey_quartic = terms.urey_poly
+    s.angle_poly = (C.c_double * 4)(*terms.angle_poly)
+    s.opbend_poly = (C.c_double * 4)(*terms.opbend_poly)
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+    e = np.zeros(8)
+    g = np.zeros(terms.n * 3)
+    vir = np.zeros(9) if virial else None
+    lib().ora_valence_evaluate(C.byref(s), _dp(x), _dp(e), _dp(g), _dp(vir) if virial else None)
+    out = dict(zip(VALENCE_KINDS, e))
+    out["total"] = float(e.sum())
+    return (out, g.reshape(-1, 3), vir.reshape(3, 3)) if virial else (out, g.reshape(-1, 3))

@@ -25,11 +25,12 @@ def test_library_exports_all_declared_symbols():
 
 def test_abi_version_and_struct_sizes():
     L = ffi.load()
-    assert L.nb_abi_version() == 5
+    assert L.nb_abi_version() == 6
     # the ctypes mirrors have the sizes the C compiler gives the header structs
     sizes = (ctypes.c_int32 * 3)()
     assert L.nb_struct_sizes(sizes) == ffi.NB_OK
     assert list(sizes) == [ctypes.sizeof(ffi.NbParams), ctypes.sizeof(ffi.NbTopology), ctypes.sizeof(ffi.NbEnergies)]
+    assert L.nb_valence_struct_size() == ctypes.sizeof(ffi.NbValence)
 
 
 def test_invalid_arguments_are_rejected_without_device_work():
@@ -38,4 +39,5 @@ def test_invalid_arguments_are_rejected_without_device_work():
     assert L.nb_create(None, None, ctypes.byref(h)) == ffi.NB_ERR_INVALID
     assert L.nb_set_cell(None, None) == ffi.NB_ERR_INVALID
     assert L.nb_evaluate(None, 0, None) == ffi.NB_ERR_INVALID
+    assert L.nb_set_valence(None, None) == ffi.NB_ERR_INVALID
     assert L.nb_last_error(None) == b"invalid handle"

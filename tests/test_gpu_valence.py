@@ -1,0 +1,105 @@
+"""Valence terms on the GPU (nb_set_valence) against the oracle and the
+reference's EnergyTest values for DHFR (EnergyTest.java:148-180) and
+ubiquitin (:363-394): every kind's energy, the gradient and the strain
+derivative at 1e-10 of the oracle on the reference geometry and on a
+distorted one; the total of valence and nonbonded terms within the test's
+1e-2 kcal/mol of the sum of its columns; graph replays equal regular
+evaluations."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from forcefieldx_amd import ffi
+from test_tinker import SYSTEMS, expected, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ffi.VALENCE_KINDS
+
+
+def handle(name, nonbonded, graph=False, **kw):
+    xyz, box, top, p, terms = load_fixture(name, valence=True)
+    p = dataclasses.replace(p, do_vdw=nonbonded, do_pme=nonbonded, **kw)
+    h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+    h.set_cell(box)
+    h.set_positions(xyz.reshape(-1))
+    h.set_valence(terms)
+    if graph:
+        h.set_graph(True)
+    return h, xyz, terms
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+@pytest.mark.parametrize("distort", [False, True], ids=["reference", "distorted"])
+def test_valence_matches_oracle(oracle, name, distort):
+    print(f"valence {name} distort={distort}", flush=True)
+    h, xyz, terms = handle(name, False)
+    try:
+        if distort:
+            xyz = xyz + np.random.default_rng(4).normal(0.0, 0.08, xyz.shape)
+            h.set_positions(xyz.reshape(-1))
+        e, rc = h.evaluate(ffi.NB_GRAD | ffi.NB_VIRIAL)
+        eo, go, vo = oracle.valence(terms, xyz, virial=True)
+        for k in KINDS:
+            assert abs(e["valence_" + k] - eo[k]) <= 1e-10 * max(1.0, abs(eo[k])), (k, e["valence_" + k], eo[k])
+        assert abs(e["total"] - eo["total"]) <= 1e-10 * abs(eo["total"])
+        g = h.gradient()
+        assert np.abs(g - go).max() <= 1e-10 * np.abs(go).max()
+        assert np.abs(h.virial() - vo).max() <= 1e-10 * np.abs(vo).max()
+        if not distort:
+            ref = expected(name)["valence"]
+            for k in KINDS:
+                assert abs(e["valence_" + k] - ref[k][0]) <= 1e-7, k
+    finally:
+        h.close()
+
+
+@pytest.mark.parametrize("name", SYSTEMS)
+def test_total_energy_matches_energy_test(name):
+    """Valence + vdW + permanent + polarization against the sum of the
+    reference test's columns (its tolerance, 1e-2 kcal/mol, per column)."""
+    h, xyz, terms = handle(name, True)
+    try:
+        e, rc = h.evaluate(ffi.NB_GRAD)
+        ref = expected(name)
+        want = sum(v[0] for v in ref["valence"].values()) + ref["vdw"] + ref["perm"] + ref["pol"]
+        print(f"{name}: total {e['total']:.8f}, reference columns {want:.8f}", flush=True)
+        assert abs(e["total"] - want) <= 4 * ref["tolerance"]
+        assert abs(e["valence"] - sum(v[0] for v in ref["valence"].values())) <= 1e-6
+    finally:
+        h.close()
+
+
+def test_graph_replays_equal_regular_evaluations():
+    hg, xyz, terms = handle("ubiquitin", True, graph=True)
+    hr, _, _ = handle("ubiquitin", True)
+    rng = np.random.default_rng(7)
+    try:
+        for step in range(4):
+            x = xyz + rng.normal(0.0, 0.01, xyz.shape)
+            hg.set_positions(x.reshape(-1))
+            hr.set_positions(x.reshape(-1))
+            eg, _ = hg.evaluate(ffi.NB_GRAD)
+            er, _ = hr.evaluate(ffi.NB_GRAD)
+            for k in KINDS:
+                assert eg["valence_" + k] == er["valence_" + k], (step, k)
+            assert abs(eg["total"] - er["total"]) <= 1e-10 * abs(er["total"])
+            assert np.abs(hg.gradient() - hr.gradient()).max() <= 1e-10 * np.abs(hr.gradient()).max()
+        assert hg.graph_replays() >= 2
+    finally:
+        hg.close()
+        hr.close()
+
+
+def test_removing_valence_terms():
+    h, xyz, terms = handle("ubiquitin", False)
+    try:
+        e1, _ = h.evaluate(ffi.NB_GRAD)
+        assert e1["valence"] != 0.0
+        h.set_valence(None)
+        e2, _ = h.evaluate(ffi.NB_GRAD)
+        assert e2["valence"] == 0.0 and e2["total"] == 0.0
+        assert np.abs(h.gradient()).max() == 0.0
+    finally:
+        h.close()

@@ -21,10 +21,10 @@ REF = "/root/reference/modules/potential/src"
 SYSTEMS = ("dhfr", "ubiquitin")
 
 
-def load_fixture(name="dhfr"):
+def load_fixture(name="dhfr", valence=False):
     d = os.path.join(GOLDEN, name)
     return tinker.load(os.path.join(d, f"{name}.xyz.gz"), os.path.join(d, f"{name}.key"),
-                       os.path.join(d, f"amoeba_bio_2009_{name}.prm.gz"))
+                       os.path.join(d, f"amoeba_bio_2009_{name}.prm.gz"), valence=valence)
 
 
 def expected(name="dhfr"):
@@ -73,9 +73,12 @@ def test_xyz_round_trip(tmp_path):
 @pytest.mark.parametrize("name", SYSTEMS)
 def test_fixture_equals_reference_files(name):
     xyz, box, top, p = load_fixture(name)
-    x2, b2, t2, p2 = tinker.load(REF + f"/test/resources/{name}.xyz", REF + f"/test/resources/{name}.key",
-                                 REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_BIO_2009")
+    x2, b2, t2, p2, v2 = tinker.load(REF + f"/test/resources/{name}.xyz", REF + f"/test/resources/{name}.key",
+                                     REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_BIO_2009", valence=True)
     assert np.abs(xyz - x2).max() < 1e-6 and box == b2 and p == p2
+    v1 = load_fixture(name, valence=True)[4]
+    f1, f2 = v1.flat(), v2.flat()
+    assert f1.keys() == f2.keys() and all(np.array_equal(f1[k], f2[k]) for k in f1)
     for k in ("frame", "axis", "polarity", "thole", "pdamp", "group", "vdw_class", "red_index", "red_value",
               "mask_off", "mask_atom", "mask_rel"):
         assert np.array_equal(getattr(top, k), getattr(t2, k)), k
