@@ -128,7 +128,7 @@ def measured_traffic(name):
     return None
 
 
-def time_gpu(top, box, xyz, params, prec, steps, warmup, profile, graph=False, lam=None):
+def time_gpu(top, box, xyz, params, prec, steps, warmup, profile, graph=False, lam=None, valence=None):
     """Warm up, optionally find the dominant kernel with a 2-step fully timed
     calibration, then time `steps` evaluations.  Without graph mode, HIP events
     bracket only the launches of the dominant kernel inside the timed region
@@ -137,7 +137,7 @@ def time_gpu(top, box, xyz, params, prec, steps, warmup, profile, graph=False, l
     evaluation, and the dominant kernel is timed with HIP events over `steps`
     regular evaluations right after it (same kernels on the same data)."""
     from forcefieldx_amd.energy import ForceFieldEnergy
-    ffe = ForceFieldEnergy(top, box, params, precision=prec)
+    ffe = ForceFieldEnergy(top, box, params, precision=prec, valence=valence)
     ffe.setCoordinates(xyz)
     h = ffe._handle
     if lam is not None:
@@ -365,7 +365,7 @@ def batch_main(args, systems, params):
     print(json.dumps(out))
 
 
-def cpu_baseline(top, box, xyz, params, workload, lam=None, budget_s=12.0):
+def cpu_baseline(top, box, xyz, params, workload, lam=None, budget_s=12.0, valence=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.build()
@@ -378,13 +378,19 @@ def cpu_baseline(top, box, xyz, params, workload, lam=None, budget_s=12.0):
     count = 0
     while True:
         e, g, lg = pyoracle.evaluate_lambda(s, params, nthreads=nth)
+        if valence is not None:
+            ev, gv = pyoracle.valence(valence, xyz)
+            e = dict(e, total=e["total"] + ev["total"])
+            g = g + gv
         count += 1
         if time.perf_counter() - t0 > budget_s or count >= 50:
             break
     dt = time.perf_counter() - t0
     return {"value": count / dt, "unit": "evals/s", "cores": nth, "kind": "port",
             "sample": f"{count} full FP64 E+G evaluations of the same {workload} workload "
-                      f"(oracle/amoeba_oracle.c, OpenMP, {dt:.1f} s)"}, e, g
+                      f"(oracle/amoeba_oracle.c, OpenMP" + (", valence terms by oracle/valence_oracle.c"
+                                                            if valence is not None else "") +
+                      f", {dt:.1f} s)"}, e, g
 
 
 WORKLOADS = {
@@ -397,11 +403,12 @@ WORKLOADS = {
     "water-648": "216 AMOEBA-2003 waters, 18.643 A cubic box, PME 24^3, PCG 1e-6 D",
     "dhfr-23558": "the reference's DHFR benchmark (EnergyTest): 23,558 atoms of AMOEBA-BIO-2009 protein and water, "
                   "62.23 A cubic box, PME 80^3 order 5, Ewald 6.7 A (alpha 0.57), vdW 9 A, SOR to 1e-6 D as its "
-                  "key file asks; coordinates and parameters from the Tinker fixture in tests/golden/dhfr",
+                  "key file asks, plus the 47,913 valence terms; coordinates and parameters from the Tinker "
+                  "fixture in tests/golden/dhfr",
     "ubiquitin-9737": "the reference's ubiquitin benchmark (EnergyTest): 9,737 atoms of AMOEBA-BIO-2009 protein and "
                       "water, 54.99 x 41.91 x 41.91 A cell, PME 70 x 54 x 54 order 5, Ewald 6.7 A (alpha 0.57), "
-                      "vdW 9 A, PCG to 1e-6 D; coordinates and parameters from the Tinker fixture in "
-                      "tests/golden/ubiquitin",
+                      "vdW 9 A, PCG to 1e-6 D, plus the 20,920 valence terms; coordinates and parameters from "
+                      "the Tinker fixture in tests/golden/ubiquitin",
     "crystals-512": "512 independent P1 triclinic cells (lengths 8-20 A, angles 70-110 deg) of small molecules "
                     "with random multipoles, per-cell PME grids 16-32, Ewald and vdW 6 A, PCG 1e-5 D, image sums "
                     "in cells thinner than twice the cutoff; a step is one sweep over all cells (nb_evaluate_batch)",
@@ -427,21 +434,22 @@ def build_workload(name):
     elif name == "crystals-512":
         systems = crystal_batch(512, seed=SEED)
         params = NonbondedParams(ewald_cutoff=6.0, vdw_cutoff=6.0, polar_eps=1e-5)
-        return systems, None, None, params, None, None
+        return systems, None, None, params, None, None, None
     elif name in ("dhfr-23558", "ubiquitin-9737"):
         from forcefieldx_amd import tinker
         stem = name.split("-")[0]
         gold = os.path.join(ROOT, "tests", "golden", stem)
-        xyz, box, top, params = tinker.load(os.path.join(gold, f"{stem}.xyz.gz"), os.path.join(gold, f"{stem}.key"),
-                                            os.path.join(gold, f"amoeba_bio_2009_{stem}.prm.gz"))
-        grid = params.grid
+        xyz, box, top, params, valence = tinker.load(
+            os.path.join(gold, f"{stem}.xyz.gz"), os.path.join(gold, f"{stem}.key"),
+            os.path.join(gold, f"amoeba_bio_2009_{stem}.prm.gz"), valence=True)
+        return xyz, box, top, params, params.grid, lam, valence
     elif name == "water-648":
         xyz, box, top = water_box(216, 18.643, seed=1)
         grid = (24, 24, 24)
         params = NonbondedParams(vdw_cutoff=9.0, grid=grid, polar_eps=1e-6)
     else:
         raise SystemExit(f"unknown workload {name}; one of {', '.join(WORKLOADS)}")
-    return xyz, box, top, params, grid, lam
+    return xyz, box, top, params, grid, lam, None
 
 
 def main():
@@ -474,17 +482,18 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    xyz, box, top, params, grid, lam = build_workload(args.workload)
+    xyz, box, top, params, grid, lam, valence = build_workload(args.workload)
     if args.workload == "crystals-512":
         return batch_main(args, xyz, params)
     graph = not args.no_graph
     perturb = args.workload == "water-288000"
     runner = time_gpu_perturbed if perturb else time_gpu
 
+    vk = {"valence": valence} if valence is not None else {}
     t64, e64, g64, u64, prof64, x64 = runner(top, box, xyz, params, ffi.NB_PREC_FP64, args.steps, args.warmup, False,
-                                             graph, lam)
+                                             graph, lam, **vk)
     tmx, emx, gmx, umx, profmx, xmx = runner(top, box, xyz, params, ffi.NB_PREC_MIXED, args.steps, args.warmup,
-                                             not args.no_profile, graph, lam)
+                                             not args.no_profile, graph, lam, **vk)
     frc_rms = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
     frc_max = float(np.max(np.abs(gmx - g64)) / np.max(np.abs(g64)))
     ind_rms = float(np.linalg.norm(umx - u64) / max(np.linalg.norm(u64), 1e-300))
@@ -507,7 +516,7 @@ def main():
     cpu = None
     e_cpu = None
     if not args.no_cpu:
-        cpu, e_cpu, g_cpu = cpu_baseline(top, box, xyz, params, args.workload, lam)
+        cpu, e_cpu, g_cpu = cpu_baseline(top, box, xyz, params, args.workload, lam, valence=valence)
 
     out = {
         "metric": METRIC,

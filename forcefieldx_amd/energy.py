@@ -88,13 +88,30 @@ class ParticleMeshEwald:
         return self._ffe._handle.induced()
 
 
+class EnergyTerm:
+    """One valence kind's share of the last evaluation (terms/EnergyTerm.java:
+    getEnergy :182, getNumberOfTerms :114)."""
+
+    def __init__(self, ffe, kind, count):
+        self._ffe, self._kind, self._count = ffe, kind, count
+
+    def getEnergy(self):
+        return self._ffe._last["valence_" + self._kind]
+
+    def getNumberOfTerms(self):
+        return self._count
+
+
 class ForceFieldEnergy:
-    """Nonbonded part of ForceFieldEnergy on the MI355X library."""
+    """ForceFieldEnergy on the MI355X library: the nonbonded terms and, when
+    given, the AMOEBA valence terms."""
 
     def __init__(self, topology, box, params: NonbondedParams | None = None, precision=None, device=0,
-                 direct_fallback=None, finite_pcg_gradient=False):
+                 direct_fallback=None, finite_pcg_gradient=False, valence=None):
         """finite_pcg_gradient: gradients of the energy of the dipoles the PCG
-        returned (NB_FINITE_PCG) instead of the stationary-dipole gradient."""
+        returned (NB_FINITE_PCG) instead of the stationary-dipole gradient.
+        valence: forcefieldx_amd.bonded.ValenceTerms evaluated with every
+        energy (the bonded-term region of ForceFieldEnergy.java:1670-1677)."""
         self.params = params or NonbondedParams()
         self.finite_pcg_gradient = bool(finite_pcg_gradient)
         self.topology = topology
@@ -113,6 +130,48 @@ class ForceFieldEnergy:
         self.direct_fallback = (not self.params.sor_fallback) if direct_fallback is None else direct_fallback
         self.vanDerWaals = VanDerWaals(self)
         self.particleMeshEwald = ParticleMeshEwald(self)
+        self.valence = valence
+        counts = valence.counts() if valence is not None else {k: 0 for k in ffi.VALENCE_KINDS}
+        self._terms = {k: EnergyTerm(self, k, counts[k]) for k in ffi.VALENCE_KINDS}
+        if valence is not None:
+            self._handle.set_valence(valence)
+
+    # -- valence terms (ForceFieldEnergy.java:2955-3045) --------------------
+    def getBondPotentialEnergy(self):
+        return self._terms["bond"]
+
+    def getAnglePotentialEnergy(self):
+        return self._terms["angle"]
+
+    def getStretchBendPotentialEnergy(self):
+        return self._terms["strbnd"]
+
+    def getUreyBradleyPotentialEnergy(self):
+        return self._terms["urey"]
+
+    def getOutOfPlaneBendPotentialEnergy(self):
+        return self._terms["opbend"]
+
+    def getTorsionPotentialEnergy(self):
+        return self._terms["torsion"]
+
+    def getPiOrbitalTorsionPotentialEnergy(self):
+        return self._terms["pitors"]
+
+    def getTorsionTorsionPotentialEnergy(self):
+        return self._terms["tortor"]
+
+    def getVanDerWaalsEnergy(self):
+        return self._last["vdw"]
+
+    def getPermanentMultipoleEnergy(self):
+        return self._last["perm"]
+
+    def getPolarizationEnergy(self):
+        return self._last["pol"]
+
+    def getTotalEnergy(self):
+        return self._last["total"]
 
     # -- Potential interface ------------------------------------------------
     def getNumberOfVariables(self):
@@ -206,6 +265,8 @@ class ForceFieldEnergy:
             try:
                 tmp.set_cell(self.box)
                 tmp.set_positions(self._x)
+                if self.valence is not None:
+                    tmp.set_valence(self.valence)
                 if self._lambda is not None:
                     tmp.set_lambda(self._lambda)
                 e, rc = tmp.evaluate(flags & ~ffi.NB_VIRIAL, check=False)

@@ -103,3 +103,26 @@ def test_removing_valence_terms():
         assert np.abs(h.gradient()).max() == 0.0
     finally:
         h.close()
+
+
+def test_force_field_energy_interface():
+    """The ForceFieldEnergy mirror with valence terms: per-kind getters and
+    term counts (ForceFieldEnergy.java:2955-3045, EnergyTerm.java:114, 182)."""
+    from forcefieldx_amd.energy import ForceFieldEnergy
+    xyz, box, top, p, terms = load_fixture("ubiquitin", valence=True)
+    ffe = ForceFieldEnergy(top, box, p, precision=ffi.NB_PREC_FP64, valence=terms)
+    try:
+        e, g = ffe.energyAndGradient(xyz.reshape(-1))
+        ref = expected("ubiquitin")
+        getters = {"bond": ffe.getBondPotentialEnergy, "angle": ffe.getAnglePotentialEnergy,
+                   "strbnd": ffe.getStretchBendPotentialEnergy, "urey": ffe.getUreyBradleyPotentialEnergy,
+                   "opbend": ffe.getOutOfPlaneBendPotentialEnergy, "torsion": ffe.getTorsionPotentialEnergy,
+                   "pitors": ffe.getPiOrbitalTorsionPotentialEnergy,
+                   "tortor": ffe.getTorsionTorsionPotentialEnergy}
+        for k, get in getters.items():
+            assert get().getNumberOfTerms() == ref["valence"][k][1], k
+            assert abs(get().getEnergy() - ref["valence"][k][0]) <= 1e-7, k
+        assert e == ffe.getTotalEnergy()
+        assert abs(ffe.getVanDerWaalsEnergy() - ref["vdw"]) <= ref["tolerance"]
+    finally:
+        ffe.destroy()
