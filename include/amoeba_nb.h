@@ -264,7 +264,7 @@ int nb_set_graph(void *handle, int on);
  * (no counterpart in the reference, which evaluates one system at a time).
  * When every handle qualifies -- periodic, at most 4096 atoms, grids of at
  * most 64 per axis, the same parameters, flags of at most NB_GRAD, no
- * alchemistry, no SOR -- one persistent launch evaluates all of them: a
+ * alchemistry, no SOR, no valence terms -- one persistent launch evaluates all of them: a
  * workgroup takes one system at a time (largest first) through the whole
  * evaluation, with explicit image sums in every cell.  A handle whose solve
  * fails there is evaluated again on the regular path (SOR fallback).  The
