@@ -106,6 +106,23 @@ __device__ __forceinline__ float nb_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double nb_rsqrt(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float nb_rsqrt(float x) { return rsqrtf(x); }
 
+// r and 1/r from r^2, and reciprocals inside the pair loops: FP64 by the
+// correctly rounded sqrt and division; FP32 by the hardware reciprocal square
+// root and reciprocal (v_rsq_f32 / v_rcp_f32, about 1 ulp), which replace the
+// ~30-instruction correctly rounded sqrt + division sequence of the FP32 form
+// (the mixed form's pair error budget is 1e-5, its pair terms err ~1e-7)
+__device__ __forceinline__ void r_rinv(double r2, double &r, double &rinv) {
+  r = sqrt(r2);
+  rinv = 1.0 / r;
+}
+__device__ __forceinline__ void r_rinv(double r2, float &r, float &rinv) {
+  float x = (float)r2;
+  rinv = __builtin_amdgcn_rsqf(x);
+  r = x * rinv;
+}
+__device__ __forceinline__ double nb_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float nb_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // erfc(x) given e^{-x^2}: FP64 calls erfc; the FP32 form uses the rational
 // approximation of Abramowitz & Stegun, Handbook of Mathematical Functions,
 // eq. 7.1.26 (after Hastings), erfc(x) = t (a1 + t (a2 + t (a3 + t (a4 + t a5))))
@@ -113,7 +130,7 @@ __device__ __forceinline__ float nb_rsqrt(float x) { return rsqrtf(x); }
 // reuses the exponential the Ewald recursion needs anyway.
 __device__ __forceinline__ double nb_erfc_ex(double x, double) { return erfc(x); }
 __device__ __forceinline__ float nb_erfc_ex(float x, float ex) {
-  float t = 1.0f / (1.0f + 0.3275911f * x);
+  float t = nb_rcp(1.0f + 0.3275911f * x);
   float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
   return poly * ex;
 }

@@ -13,7 +13,8 @@ __device__ __forceinline__ void perm_field_pair(const double4 &pi, const AtomMP 
   double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
   double r2 = image(cell, dx, dy, dz);
   MP<U> K = load_mp<U>(ak);
-  U r = sqrt((U)r2), rinv = (U)1 / r;
+  U r, rinv;
+  r_rinv(r2, r, rinv);
   V3<U> R{(U)-dx, (U)-dy, (U)-dz};  // from k to i
   U bn[4];
   ewald_bn<U, 3>(r, rinv, (U)aewald, bn);
@@ -51,7 +52,8 @@ __device__ __forceinline__ void perm_energy_pair(const double4 &pi, const AtomMP
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
   V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = sqrt((U)r2), rinv = (U)1 / rl;
+  U rl, rinv;
+  r_rinv(r2, rl, rinv);
   U bn[6], rr[6];
   ewald_bn<U, 5>(rl, rinv, (U)aewald, bn);
   PermGeo<U> g;
@@ -79,7 +81,8 @@ __device__ __forceinline__ void pol_energy_pair(const double4 &pi, const AtomMP 
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
   V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = sqrt((U)r2), rinv = (U)1 / rl;
+  U rl, rinv;
+  r_rinv(r2, rl, rinv);
   U bn[5], rr[5];
   ewald_bn<U, 4>(rl, rinv, (U)aewald, bn);
   coulomb_rr<U, 4>(rinv, rr);
@@ -106,12 +109,13 @@ template <typename T>
 __device__ __forceinline__ Vdw147<T> vdw147(double r2, T irv, T ev, T sc1, T sc2, T d1, T g1, T gg, T t1n, T wcut,
                                             T iw, bool tapered) {
   Vdw147<T> v;
-  v.rr = sqrt((T)r2);
+  T rinv;
+  r_rinv(r2, v.rr, rinv);
   T rho = v.rr * irv;
   T rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
   T rd = rho + d1, rd2 = rd * rd, rd6 = rd2 * rd2 * rd2, rd7 = rd6 * rd;
-  v.t1d = (T)1 / (sc1 + rd7);
-  v.t2d = (T)1 / (sc1 + rho7 + gg);
+  v.t1d = nb_rcp(sc1 + rd7);
+  v.t2d = nb_rcp(sc1 + rho7 + gg);
   v.t1 = t1n * v.t1d;
   v.t2a = g1 * v.t2d;
   v.t2 = v.t2a - (T)2;
@@ -126,7 +130,7 @@ __device__ __forceinline__ Vdw147<T> vdw147(double r2, T irv, T ev, T sc1, T sc2
   v.dt1 = v.t1 * (T)7 * rd6 * irv * v.t1d;
   v.dt2 = v.t2a * (T)7 * rho6 * irv * v.t2d;
   T dedr = -sc2 * ev * (v.dt1 * v.t2 + v.t1 * v.dt2);
-  v.ds = (v.eik * v.dtp + dedr * v.tp) / v.rr;
+  v.ds = (v.eik * v.dtp + dedr * v.tp) * rinv;
   return v;
 }
 
@@ -140,7 +144,8 @@ __device__ __forceinline__ void dip_field_pair(const DipRec<T> &ri, const DipRec
   double dx = rk.x - ri.x, dy = rk.y - ri.y, dz = rk.z - ri.z;
   entry_shift(cell, e, dx, dy, dz);
   double r2 = image(cell, dx, dy, dz);
-  T r = sqrt((T)r2), rinv = (T)1 / r;
+  T r, rinv;
+  r_rinv(r2, r, rinv);
   V3<T> R{(T)dx, (T)dy, (T)dz};
   Thole<T> th = thole<T, false>(r, R, ri.ipd, rk.ipd, ri.thole, rk.thole);
   T rr3 = rinv * rinv * rinv, rr5 = (T)3 * rr3 * rinv * rinv;

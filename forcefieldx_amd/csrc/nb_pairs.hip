@@ -51,39 +51,49 @@ __global__ void __launch_bounds__(256) k_filter(int n, const double4 *__restrict
   uint32_t *erow = ew + (size_t)(valid ? s : 0) * maxew;
   uint32_t *prow = pc + (size_t)(valid ? s : 0) * maxpc;
   uint32_t *srow = sp + (size_t)(valid ? s : 0) * maxsp;
-  for (int base = 0; base < cnt; base += LPA) {
-    int t = base + lane;
-    bool ke = false, kp = false, ks = false;
-    uint32_t e = 0;
-    if (t < cnt) {
-      e = row[t];
-      double4 pk = pos[e & cell.kmask];
-      entry_shift(cell, e, pk.x, pk.y, pk.z);
-      double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
-      double r2 = image(cell, dx, dy, dz);
-      ke = r2 <= off2;
-      kp = ke && sqrt(r2) < pcut;
-      ks = ke && sc.masked[e >> NB_CODE_SHIFT];
+  const double pc2 = pcut * pcut, plo = pc2 * (1.0 - 1e-12), phi = pc2 * (1.0 + 1e-12);
+  // the entry's class: kept by the Ewald cutoff, the preconditioner cutoff
+  // (r < pcut, by sqrt only within rounding of the boundary), masked
+  auto classify = [&](uint32_t e, double4 pk, bool &ke, bool &kp, bool &ks) {
+    entry_shift(cell, e, pk.x, pk.y, pk.z);
+    double dx = pk.x - pi.x, dy = pk.y - pi.y, dz = pk.z - pi.z;
+    double r2 = image(cell, dx, dy, dz);
+    ke = r2 <= off2;
+    kp = ke && (r2 < plo || (r2 <= phi && sqrt(r2) < pcut));
+    ks = ke && sc.masked[e >> NB_CODE_SHIFT];
+  };
+  for (int base = 0; base < cnt; base += 2 * LPA) {
+    // two chunks of the row per trip: both row loads and both position
+    // gathers are issued before either is used
+    const int t0 = base + lane, t1 = base + LPA + lane;
+    const uint32_t e0 = t0 < cnt ? row[t0] : 0u, e1 = t1 < cnt ? row[t1] : 0u;
+    const double4 p0 = t0 < cnt ? pos[e0 & cell.kmask] : pi, p1 = t1 < cnt ? pos[e1 & cell.kmask] : pi;
+    for (int half = 0; half < 2; half++) {
+      if (half == 1 && base + LPA >= cnt) break;   // uniform across the atom's lanes
+      const int t = half ? t1 : t0;
+      const uint32_t e = half ? e1 : e0;
+      bool ke = false, kp = false, ks = false;
+      if (t < cnt) classify(e, half ? p1 : p0, ke, kp, ks);
+      unsigned long long me = group_mask<LPA>(__ballot(ke));
+      unsigned long long mp = group_mask<LPA>(__ballot(kp));
+      unsigned long long ms = group_mask<LPA>(__ballot(ks));
+      unsigned long long below = (1ull << lane) - 1ull;
+      if (ke) {
+        int q = ne + __popcll(me & below);
+        if (q < maxew) erow[q] = e;
+      }
+      if (kp) {
+        int q = np + __popcll(mp & below);
+        if (q < maxpc) prow[q] = e;
+      }
+      if (ks) {
+        int q = ns + __popcll(ms & below);
+        if (q < maxsp) srow[q] = e;
+      }
+      ne += __popcll(me);
+      np += __popcll(mp);
+      ns += __popcll(ms);
     }
-    unsigned long long me = group_mask<LPA>(__ballot(ke));
-    unsigned long long mp = group_mask<LPA>(__ballot(kp));
-    unsigned long long ms = group_mask<LPA>(__ballot(ks));
-    unsigned long long below = (1ull << lane) - 1ull;
-    if (ke) {
-      int q = ne + __popcll(me & below);
-      if (q < maxew) erow[q] = e;
-    }
-    if (kp) {
-      int q = np + __popcll(mp & below);
-      if (q < maxpc) prow[q] = e;
-    }
-    if (ks) {
-      int q = ns + __popcll(ms & below);
-      if (q < maxsp) srow[q] = e;
-    }
-    ne += __popcll(me);
-    np += __popcll(mp);
-    ns += __popcll(ms);
   }
   if (valid && lane == 0) {
     // counts are clamped to the row capacity (rows are never read past it);
@@ -154,9 +164,23 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
     const DipRec<T> ri = rec[s];
     int m = cnt[s];
     const uint32_t *row = lst + (size_t)s * maxl;
-    for (int t = lane; t < m; t += LPA) {
-      uint32_t e = row[t];
-      dip_field_pair<T, PRE>(ri, rec[e & cell.kmask], e, cell, aewald, a, b);
+    if constexpr (sizeof(T) == 4) {
+      // two entries per trip: both row loads, then both record gathers, are in
+      // flight together, which halves the dependent memory round trips of the
+      // latency-bound loop (71 VGPRs, 7 waves per SIMD; the FP64 form would
+      // drop to 3 waves and keeps one entry per trip)
+      for (int t = lane; t < m; t += 2 * LPA) {
+        const bool two = t + LPA < m;
+        const uint32_t e0 = row[t], e1 = two ? row[t + LPA] : e0;
+        const DipRec<T> r0 = rec[e0 & cell.kmask], r1 = rec[e1 & cell.kmask];
+        dip_field_pair<T, PRE>(ri, r0, e0, cell, aewald, a, b);
+        if (two) dip_field_pair<T, PRE>(ri, r1, e1, cell, aewald, a, b);
+      }
+    } else {
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t e = row[t];
+        dip_field_pair<T, PRE>(ri, rec[e & cell.kmask], e, cell, aewald, a, b);
+      }
     }
   }
   group_sum3<LPA>(a);
@@ -184,7 +208,8 @@ __device__ __forceinline__ void perm_energy_pair_ost(const double4 &pi, const At
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
   V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = sqrt((U)(soft ? r2 + ph.lalpha : r2)), rinv = (U)1 / rl;
+  U rl, rinv;
+  r_rinv(soft ? r2 + ph.lalpha : r2, rl, rinv);
   U bn[7], rr[7];
   ewald_bn<U, 6>(rl, rinv, (U)aewald, bn);
   coulomb_rr<U, 6>(rinv, rr);
@@ -231,7 +256,8 @@ __device__ __forceinline__ void pol_energy_pair_ost(const double4 &pi, const Ato
   double r2 = image(cell, dx, dy, dz);
   MP<U> I = load_mp<U>(ai), K = load_mp<U>(ak);
   V3<U> r{(U)dx, (U)dy, (U)dz};
-  U rl = sqrt((U)r2), rinv = (U)1 / rl;
+  U rl, rinv;
+  r_rinv(r2, rl, rinv);
   U bn[5], rr[5];
   ewald_bn<U, 4>(rl, rinv, (U)aewald, bn);
   coulomb_rr<U, 4>(rinv, rr);
@@ -368,19 +394,18 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
     int m = nnb[s];
     int softi = LAM ? soft[s] : 0, moli = LAM ? mol[s] : 0;
     const uint32_t *row = nbr + (size_t)s * maxnb;
-    for (int t = lane; t < m; t += LPA) {
-      uint32_t en = row[t];
+    // one pair; pk and ck are the partner's reduced site and vdW class
+    auto pair = [&](uint32_t en, double4 pk, int ck) {
       int k = en & cell.kmask, code = en >> NB_CODE_SHIFT;
       double vs = sc.v[code];
-      if (!(vs > 0.0)) continue;
-      double4 pk = red[k];
+      if (!(vs > 0.0)) return;
       entry_shift(cell, en, pk.x, pk.y, pk.z);
       double dx = pi.x - pk.x, dy = pi.y - pk.y, dz = pi.z - pk.z;
       double r2 = image(cell, dx, dy, dz);
-      if (r2 > off2) continue;
-      const double *pe = vtab + 2 * (ci * nclass + vclass[k]);
+      if (r2 > off2) return;
+      const double *pe = vtab + 2 * (ci * nclass + ck);
       T irv = (T)pe[0];
-      if (!(irv > (T)0)) continue;
+      if (!(irv > (T)0)) return;
       cnt++;
       bool spair = false;
       if (LAM) {
@@ -418,6 +443,26 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
         for (int a = 0; a < 3; a++)
 #pragma unroll
           for (int b = 0; b < 3; b++) w9[3 * a + b] += h * dd[a] * dd[b];
+      }
+    };
+    if constexpr (sizeof(T) == 4 && !LAM) {
+      // two entries per trip: the row loads, then the site and class gathers
+      // of both partners, are in flight together (the loop is bound by these
+      // dependent loads, not by arithmetic)
+      for (int t = lane; t < m; t += 2 * LPA) {
+        const bool two = t + LPA < m;
+        const uint32_t e0 = row[t], e1 = two ? row[t + LPA] : e0;
+        const int k0 = e0 & cell.kmask, k1 = e1 & cell.kmask;
+        const double4 p0 = red[k0], p1 = red[k1];
+        const int c0 = vclass[k0], c1 = vclass[k1];
+        pair(e0, p0, c0);
+        if (two) pair(e1, p1, c1);
+      }
+    } else {
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t en = row[t];
+        int k = en & cell.kmask;
+        pair(en, red[k], vclass[k]);
       }
     }
   }

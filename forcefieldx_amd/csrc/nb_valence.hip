@@ -10,7 +10,7 @@
 //  * k_valence: one thread per term over all kinds (kinds in contiguous
 //    ranges, so a wave diverges only at a range boundary); writes the term's
 //    energy and one gradient contribution per atom into its own slots.
-//  * k_valence_gather: one thread per atom sums its slots (a CSR list in slot
+//  * k_valence_gather: eight lanes per atom sum its slots (a CSR list in slot
 //    order, built at nb_set_valence) into the gradient, and its strain
 //    derivative g x^T into per-block partials; no atomics, so the result is
 //    the same on every run and every graph replay.
@@ -370,26 +370,39 @@ __global__ void __launch_bounds__(256) k_valence(VDev v, const double *__restric
   }
 }
 
+// GL lanes per atom: lane j sums slots j, j + GL, ... of the atom's list,
+// then a fixed butterfly over the GL lanes (the same order on every run)
+constexpr int GL = 8;
+
 __global__ void __launch_bounds__(256) k_valence_gather(int n, const int32_t *__restrict__ off,
                                                         const int32_t *__restrict__ list,
                                                         const double *__restrict__ slot, const double *__restrict__ xyz,
                                                         double *__restrict__ grad, double *__restrict__ vpart) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = g / GL, lane = g % GL;
   double gx = 0.0, gy = 0.0, gz = 0.0;
   if (i < n) {
-    for (int k = off[i]; k < off[i + 1]; k++) {
+    for (int k = off[i] + lane; k < off[i + 1]; k += GL) {
       const double *s = slot + 3 * (long)list[k];
       gx += s[0];
       gy += s[1];
       gz += s[2];
     }
+  }
+#pragma unroll
+  for (int o = GL / 2; o > 0; o >>= 1) {
+    gx += __shfl_xor(gx, o, 64);
+    gy += __shfl_xor(gy, o, 64);
+    gz += __shfl_xor(gz, o, 64);
+  }
+  if (i < n && lane == 0) {
     grad[3 * i] += gx;
     grad[3 * i + 1] += gy;
     grad[3 * i + 2] += gz;
   }
   if (vpart) {
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < n) {
+    if (i < n && lane == 0) {
       const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
       w[0] = gx * x; w[1] = gx * y; w[2] = gx * z;
       w[3] = gy * x; w[4] = gy * y; w[5] = gy * z;
@@ -427,7 +440,7 @@ void nb_valence_terms(NbHandle *h) {
 void nb_valence_gather(NbHandle *h) {
   if (!h->has_valence) return;
   const VDev &v = *(const VDev *)h->val_dev;
-  const int nb = (h->n + 255) / 256;
+  const int nb = (int)(((long)h->n * GL + 255) / 256);
   double *vp = h->want_vir ? nb_vir_region(h, nb) : nullptr;
   NB_LAUNCH(k_valence_gather, dim3(nb), dim3(256), 0, h->n, (const int32_t *)h->d_val_off,
             (const int32_t *)h->d_val_list, (const double *)h->d_val_slot, (const double *)h->d_xyz, h->d_grad, vp);

@@ -147,11 +147,14 @@ def test_separate_batch_equals_single_evaluations(precision):
             eb, rc = ffi.evaluate_batch(hb, flags)
             assert rc == ffi.NB_OK
         assert all(h.graph_replays() >= 2 for h in hb)
+        # FP64: only the order of FP64 atomics differs; mixed: an FP32 grid
+        # value can round one ulp apart after them (see test_gpu_graph.py)
+        tol = 1e-12 if precision == ffi.NB_PREC_FP64 else 1e-8
         for b, (h1, h2) in enumerate(zip(hb, hs)):
             e2, _ = h2.evaluate(ffi.NB_GRAD)
-            assert abs(eb[b]["total"] - e2["total"]) <= 1e-12 * max(abs(e2["perm"]), abs(e2["vdw"])), b
+            assert abs(eb[b]["total"] - e2["total"]) <= tol * max(abs(e2["perm"]), abs(e2["vdw"])), b
             g1, g2 = h1.gradient(), h2.gradient()
-            assert np.abs(g1 - g2).max() <= 1e-12 * np.abs(g2).max(), b
+            assert np.abs(g1 - g2).max() <= tol * np.abs(g2).max(), b
     finally:
         close_all(hb + hs)
 

@@ -13,6 +13,11 @@ from forcefieldx_amd.systems import chain_in_water, water_box
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12   # same kernels; only the order of floating-point atomics differs
+# The mixed form rounds the FP64 sums of the spread's LDS atomics to FP32 grid
+# values; when the atomics' order moves such a sum across a rounding boundary
+# the grid point differs by one FP32 ulp, and the solve carries that into the
+# dipoles at ~1e-9 relative.  Seen about once in 70 evaluations, on either path.
+TOL_MIXED = 1e-8
 
 
 def make(top, box, p, precision, graph):
@@ -44,6 +49,7 @@ def test_graph_replay_matches_regular_path(water648, precision):
     xyz, box, top = water648
     p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-6)
     ref, gr = make(top, box, p, precision, False), make(top, box, p, precision, True)
+    tol = TOL if precision == ffi.NB_PREC_FP64 else TOL_MIXED
     rng = np.random.default_rng(11)
     x = xyz.reshape(-1).copy()
     try:
@@ -56,11 +62,11 @@ def test_graph_replay_matches_regular_path(water648, precision):
             flags = ffi.NB_GRAD | (ffi.NB_VIRIAL if step >= 20 else 0)
             a, b = run(ref, x, flags), run(gr, x, flags)
             for k in ("total", "perm", "pol", "vdw"):
-                assert abs(a["e"][k] - b["e"][k]) <= TOL * abs(a["e"][k]), (step, k)
+                assert abs(a["e"][k] - b["e"][k]) <= tol * abs(a["e"][k]), (step, k)
             assert a["e"]["pcg_iters"] == b["e"]["pcg_iters"]
-            assert close(b["g"], a["g"])
+            assert close(b["g"], a["g"], tol)
             if "W" in a:
-                assert close(b["W"], a["W"])
+                assert close(b["W"], a["W"], tol)
         assert gr.graph_replays() >= 18
     finally:
         ref.close()
