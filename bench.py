@@ -116,7 +116,7 @@ def measured_traffic(name):
     path = os.path.join(ROOT, "profiles", "water-20736_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    key = name.strip("()").split("(")[0]
+    key = name.strip("()").split("(")[0].replace("NB_LPA", "16")
     for line in open(path):
         if line.startswith("| `") and "|" in line[3:]:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
