@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(BUILD, "libamoeba_nb.so")
-SOURCES = ["nb_api.hip", "nb_lists.hip", "nb_pme.hip", "nb_real.hip", "nb_pairs.hip", "nb_batch.hip", "nb_finite.hip", "nb_valence.hip"]
+SOURCES = ["nb_api.hip", "nb_lists.hip", "nb_pme.hip", "nb_real.hip", "nb_pairs.hip", "nb_batch.hip", "nb_finite.hip", "nb_valence.hip", "nb_md.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -71,6 +71,7 @@ void alloc(T **dst, size_t count) {
 
 void free_all(NbHandle *h) {
   nb_valence_free(h);
+  nb_md_free(h);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
   h->gexec = nullptr;
@@ -708,11 +709,16 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
     if (polar) {
       nb_direct(h);
       if (h->prm.polarization == NB_POL_MUTUAL) {
+        // the SCF predictor's extrapolation is the solve's starting point
+        // (not for the finite-PCG gradient, which starts from the direct dipoles)
+        if (!h->want_finite) nb_pred_apply(h);
         if (capture) {
           status = nb_pcg_device(h, h->graph_iters);
+          nb_pred_save(h);
         } else if (h->prm.scf_algorithm == NB_SCF_SOR || h->ph.sor) {
           status = nb_sor(h, h->prm.polar_sor, &e);
           h->used_sor = true;
+          if (status == NB_OK) nb_pred_save(h);
         } else {
           h->tape_on = h->want_finite;
           h->tape_K = 0;
@@ -729,6 +735,7 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
             nb_finite_adjoint(h);
             h->finite_active = true;
           }
+          if (status == NB_OK && !h->finite_active) nb_pred_save(h);
         }
       } else if (use_recip) {
         // direct polarization: only the reciprocal potentials of u_direct are needed

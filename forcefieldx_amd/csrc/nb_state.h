@@ -198,6 +198,13 @@ struct NbHandle {
   double *d_val_eterm = nullptr;  // [val_terms]
   double *d_val_energy = nullptr; // [8]
 
+  // SCF predictor (nb_set_scf_predictor) and dynamics (nb_md_*)
+  int predictor = 0;              // NB_PRED_*
+  double *d_pred = nullptr;       // [6][6n] u - u_direct history (u then uCR), caller order
+  int32_t *d_pred_ctl = nullptr;  // [2] ring start, fill count (advanced on the device)
+  double *d_md_v = nullptr, *d_md_a = nullptr, *d_md_mass = nullptr, *d_md_part = nullptr, *d_md_ke = nullptr;
+  bool md_ready = false, md_accel = false;
+
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
   // replayed; the PCG then runs graph_iters iterations under device-side
   // control (kernels that only serve a stopped solve return at once)
@@ -250,6 +257,9 @@ double *nb_vir_region(NbHandle *h, long nblocks);
 void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
 void nb_tape_push(NbHandle *h);
+void nb_pred_apply(NbHandle *h);
+void nb_pred_save(NbHandle *h);
+void nb_md_free(NbHandle *h);
 void nb_valence_terms(NbHandle *h);
 void nb_valence_gather(NbHandle *h);
 void nb_valence_free(NbHandle *h);

@@ -25,7 +25,8 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
             "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config", "nb_set_valence",
-            "nb_valence_struct_size"]
+            "nb_valence_struct_size", "nb_set_scf_predictor", "nb_md_init", "nb_md_step", "nb_md_get_state"]
+NB_PRED_NONE, NB_PRED_ASPC = 0, 1
 VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
 
 _dp = C.POINTER(C.c_double)
@@ -72,6 +73,11 @@ class NbEnergies(C.Structure):
         d.update({"valence_" + k: self.valence[i] for i, k in enumerate(VALENCE_KINDS)})
         d["valence"] = sum(self.valence)
         return d
+
+
+class NbMdReport(C.Structure):
+    _fields_ = [("kinetic", C.c_double), ("potential", C.c_double), ("total", C.c_double), ("steps", C.c_int32),
+                ("pcg_iters", C.c_int32), ("last", NbEnergies)]
 
 
 class NbValence(C.Structure):
@@ -140,6 +146,14 @@ def load(path=LIB_PATH):
     L.nb_profile_count.restype = C.c_int
     L.nb_profile_entry.argtypes = [vp, C.c_int, C.c_char_p, C.c_int, _dp, C.POINTER(C.c_int64)]
     L.nb_profile_entry.restype = C.c_int
+    L.nb_set_scf_predictor.argtypes = [vp, C.c_int]
+    L.nb_set_scf_predictor.restype = C.c_int
+    L.nb_md_init.argtypes = [vp, _dp, _dp]
+    L.nb_md_init.restype = C.c_int
+    L.nb_md_step.argtypes = [vp, C.c_int, C.c_double, C.POINTER(NbMdReport)]
+    L.nb_md_step.restype = C.c_int
+    L.nb_md_get_state.argtypes = [vp, _dp, _dp]
+    L.nb_md_get_state.restype = C.c_int
     L.nb_set_valence.argtypes = [vp, C.POINTER(NbValence)]
     L.nb_set_valence.restype = C.c_int
     L.nb_valence_struct_size.restype = C.c_int
@@ -243,6 +257,27 @@ class Handle:
         v.angle_poly = (C.c_double * 4)(*terms.angle_poly)
         v.opbend_poly = (C.c_double * 4)(*terms.opbend_poly)
         self._check(self.L.nb_set_valence(self.h, C.byref(v)))   # the library copies the arrays
+
+    def set_scf_predictor(self, kind=NB_PRED_ASPC):
+        self._check(self.L.nb_set_scf_predictor(self.h, int(kind)))
+
+    def md_init(self, mass, v0=None):
+        m = np.ascontiguousarray(mass, np.float64).reshape(-1)
+        v = None if v0 is None else np.ascontiguousarray(v0, np.float64).reshape(-1)
+        self._check(self.L.nb_md_init(self.h, _p(m, C.c_double), None if v is None else _p(v, C.c_double)))
+
+    def md_step(self, nsteps, dt_ps):
+        """nsteps velocity-Verlet steps of dt_ps; returns the report as a dict."""
+        r = NbMdReport()
+        self._check(self.L.nb_md_step(self.h, int(nsteps), float(dt_ps), C.byref(r)))
+        return dict(kinetic=r.kinetic, potential=r.potential, total=r.total, steps=r.steps,
+                    pcg_iters=r.pcg_iters, last=r.last.as_dict())
+
+    def md_state(self):
+        x = np.zeros(3 * self.n)
+        v = np.zeros(3 * self.n)
+        self._check(self.L.nb_md_get_state(self.h, _p(x, C.c_double), _p(v, C.c_double)))
+        return x.reshape(-1, 3), v.reshape(-1, 3)
 
     def set_lambda(self, lam):
         self._check(self.L.nb_set_lambda(self.h, float(lam)))

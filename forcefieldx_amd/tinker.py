@@ -481,6 +481,11 @@ def build(xf: XYZFile, ff: ForceFieldParams, keys: dict):
     return xf.xyz.copy(), box, top, p
 
 
+def masses(types, ff: ForceFieldParams):
+    """Atomic masses (g/mol) of the atoms' types from the `atom` records."""
+    return np.array([ff.element[int(t)][2] for t in types])
+
+
 def load(xyz_path, key_path, prm_path, valence=False):
     """Read the three files and build (xyz, box, Topology, NonbondedParams),
     with the valence terms (forcefieldx_amd.bonded.ValenceTerms) appended

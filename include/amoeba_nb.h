@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 6
+#define NB_ABI_VERSION 7
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -152,6 +152,17 @@ typedef struct {
 enum { NB_VAL_BOND = 0, NB_VAL_ANGLE, NB_VAL_STRBND, NB_VAL_UREY, NB_VAL_OPBEND, NB_VAL_TORSION, NB_VAL_PITORS,
        NB_VAL_TORTOR };
 
+/* SCF predictors (SCFPredictor.java) */
+enum { NB_PRED_NONE = 0, NB_PRED_ASPC = 1 };
+
+/* Result of nb_md_step: the energies after the last step. */
+typedef struct {
+  double kinetic, potential, total;   /* kcal/mol */
+  int32_t steps;                      /* steps taken */
+  int32_t pcg_iters;                  /* PCG iterations summed over the steps */
+  nb_energies last;                   /* the last step's evaluation */
+} nb_md_report;
+
 /* AMOEBA valence terms, assigned by the caller (atom indices 0-based; force
  * constants already multiplied by their unit keywords; ideal lengths in A,
  * ideal angles in degrees).  Forms follow Bond.java:242-283,
@@ -243,6 +254,27 @@ int nb_set_valence(void *handle, const nb_valence *terms);
 
 /* sizeof(nb_valence) as compiled. */
 int nb_valence_struct_size(void);
+
+/* Initial guess of the mutual solve from earlier solutions: NB_PRED_ASPC is
+ * the always-stable predictor-corrector of SCFPredictorParameters.
+ * aspcPredictor (SCFPredictorParameters.java:79-121; wired at
+ * ParticleMeshEwald.java:2272-2281, saved at :1952-1955): after six converged
+ * solves, the next starts from the direct dipoles plus sum_k c_k (u -
+ * u_direct) of the last six, c = 22/7, -55/14, 55/21, -22/21, 5/21, -1/42, for
+ * u and uCR alike.  The history is kept per handle in caller order and reset
+ * by this call; graph replays carry it.  Not applied with NB_FINITE_PCG. */
+int nb_set_scf_predictor(void *handle, int kind);
+
+/* Velocity-Verlet dynamics on the device (VelocityVerlet.java:66-113, driven
+ * as MolecularDynamics.java:1285-1299): nb_md_init takes the masses [n]
+ * (g/mol) and velocities [3n] (A/ps, NULL = zero); nb_md_step advances nsteps
+ * steps of dt ps from the handle's current positions, one evaluation per step
+ * (replayed from a graph in graph mode), positions and velocities staying in
+ * device memory; nb_md_get_state copies them out (either may be NULL).  The
+ * first step after nb_md_init evaluates the starting accelerations. */
+int nb_md_init(void *handle, const double *mass, const double *v0);
+int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out);
+int nb_md_get_state(void *handle, double *x, double *v);
 
 /* Human-readable message for the last error on this handle. */
 const char *nb_last_error(void *handle);

@@ -2594,6 +2594,16 @@ static int elec_phase(ctx_t *c, ora_energies *out, double *ind, double *indcr) {
         c->indp[3 * i + d] = c->dirp[3 * i + d] = s->polarity[i] * c->fieldp[3 * i + d];
       }
     }
+    if (prm->direct_out) {
+      memcpy(prm->direct_out, c->dir, s3);
+      memcpy(prm->direct_out + 3 * n, c->dirp, s3);
+    }
+    if (prm->polarization == ORA_POL_MUTUAL && prm->scf_guess) {
+      for (int q = 0; q < 3 * n; q++) {
+        c->ind[q] += prm->scf_guess[q];
+        c->indp[q] += prm->scf_guess[3 * n + q];
+      }
+    }
     if (prm->polarization == ORA_POL_MUTUAL) {
       if (prm->scf_algorithm == 1 || c->ph->sor) {
         status = sor_solve(c, out, prm->polar_sor);

@@ -99,6 +99,12 @@ typedef struct {
    * iteration count held fixed): the adjoint of the truncated iterations
    * replaces the stationary-dipole terms (0) */
   int finite_pcg;
+  /* initial guess of a mutual solve (ParticleMeshEwald.java:2272-2281): when
+   * set, [6n] (u then uCR) added to the direct dipoles before the solve, as an
+   * SCF predictor adds its extrapolation (NULL: start from the direct dipoles) */
+  const double *scf_guess;
+  /* when set, [6n] receives the direct dipoles (u then uCR) */
+  double *direct_out;
 } ora_params;
 
 typedef struct {

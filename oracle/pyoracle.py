@@ -57,7 +57,8 @@ class OraParams(C.Structure):
                 ("pol_lambda_exponent", C.c_double), ("pol_lambda_start", C.c_double),
                 ("pol_lambda_end", C.c_double), ("ligand_vapor_elec", C.c_int), ("no_ligand_condensed_scf", C.c_int),
                 ("scf_algorithm", C.c_int), ("polar_sor", C.c_double), ("sor_fallback", C.c_int),
-                ("finite_pcg", C.c_int)]
+                ("finite_pcg", C.c_int), ("scf_guess", C.POINTER(C.c_double)),
+                ("direct_out", C.POINTER(C.c_double))]
 
 
 class OraAlchemy(C.Structure):
@@ -177,18 +178,29 @@ def alchemical_factors(params, lam):
     return {k: getattr(f, k) for k, _ in OraAlchemy._fields_}
 
 
-def evaluate(system, params, nthreads=0, finite_pcg=False):
+def evaluate(system, params, nthreads=0, finite_pcg=False, scf_guess=None, want_direct=False):
     """Run the oracle; returns (energies dict, gradient (n,3), u (n,3), uCR (n,3)).
     With finite_pcg the gradient is that of the energy of the dipoles the PCG
-    returned (iteration count fixed) and uCR is replaced by the adjoint dipoles."""
+    returned (iteration count fixed) and uCR is replaced by the adjoint dipoles.
+    scf_guess (2, n, 3): added to the direct dipoles as the solve's starting
+    point; want_direct appends the direct dipoles (2, n, 3) to the result."""
     n = system.top.n
     prm = make_params(params, system.box, nthreads, finite_pcg) if not isinstance(params, OraParams) else params
+    keep = []
+    if scf_guess is not None:
+        g6 = np.ascontiguousarray(scf_guess, np.float64).reshape(-1)
+        keep.append(g6)
+        prm.scf_guess = _dp(g6)
+    d6 = np.zeros(6 * n) if want_direct else None
+    if want_direct:
+        prm.direct_out = _dp(d6)
     out = OraEnergies()
     g = np.zeros(3 * n)
     u = np.zeros(3 * n)
     up = np.zeros(3 * n)
     lib().ora_evaluate(C.byref(system.c), C.byref(prm), C.byref(out), _dp(g), _dp(u), _dp(up))
-    return out.as_dict(), g.reshape(n, 3), u.reshape(n, 3), up.reshape(n, 3)
+    res = (out.as_dict(), g.reshape(n, 3), u.reshape(n, 3), up.reshape(n, 3))
+    return res + (d6.reshape(2, n, 3),) if want_direct else res
 
 
 def evaluate_lambda(system, params, nthreads=0):

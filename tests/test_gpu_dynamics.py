@@ -1,0 +1,156 @@
+"""The ASPC SCF predictor and velocity-Verlet dynamics on the device.
+
+* Predictor (SCFPredictorParameters.java:79-121, 269-302): on a water box
+  moving along a smooth path, every solve after the sixth starts from the
+  direct dipoles plus Kolafa's extrapolation of the last six (u - u_direct);
+  the oracle, given the same starting point (built here from its own earlier
+  solutions), returns the same iteration counts, energies, gradients and
+  dipoles at 1e-8, and the predicted solves take fewer iterations.
+* Velocity Verlet (VelocityVerlet.java:66-113) on the reference's ubiquitin
+  structure with its valence terms: three device steps equal three numpy steps
+  driven by the oracle's gradients at 1e-9; energy is conserved over a short
+  NVE run; graph replays give the regular path's trajectory."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from forcefieldx_amd import ffi, tinker
+from forcefieldx_amd.forcefield import NonbondedParams
+from forcefieldx_amd.systems import water_box
+from test_tinker import GOLDEN, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+ASPC = np.array([22.0 / 7.0, -55.0 / 14.0, 55.0 / 21.0, -22.0 / 21.0, 5.0 / 21.0, -1.0 / 42.0])
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_aspc_predictor_matches_oracle(oracle):
+    xyz, box, top = water_box(216, 18.643, seed=1)
+    p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-6)
+    rng = np.random.default_rng(3)
+    vel = rng.normal(0.0, 0.004, xyz.shape)          # A per step, a straight path
+    h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+    cold = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+    hist = []
+    try:
+        h.set_cell(box)
+        cold.set_cell(box)
+        h.set_scf_predictor(ffi.NB_PRED_ASPC)
+        for t in range(10):
+            x = xyz + t * vel
+            guess = np.tensordot(ASPC, np.array(hist), axes=1) if len(hist) == 6 else None
+            eo, go, uo, upo, direct = oracle.evaluate(oracle.System(x, box, top), p, scf_guess=guess,
+                                                      want_direct=True)
+            hist = ([np.stack([uo - direct[0], upo - direct[1]])] + hist)[:6]
+            h.set_positions(x.reshape(-1))
+            e, _ = h.evaluate(ffi.NB_GRAD)
+            u, up = h.induced()
+            cold.set_positions(x.reshape(-1))
+            ec, _ = cold.evaluate(ffi.NB_GRAD)
+            print(f"step {t}: iterations predicted {e['pcg_iters']} oracle {eo['pcg_iters']} cold {ec['pcg_iters']}",
+                  flush=True)
+            assert e["pcg_iters"] == eo["pcg_iters"], t
+            assert abs(e["total"] - eo["total"]) <= 1e-8 * abs(eo["total"]), t
+            assert rel(h.gradient(), go) <= 1e-8, t
+            assert rel(u, uo) <= 1e-8 and rel(up, upo) <= 1e-8, t
+            if t >= 6:
+                assert e["pcg_iters"] < ec["pcg_iters"], t
+    finally:
+        h.close()
+        cold.close()
+
+
+def ubiquitin(polar_eps=None):
+    xyz, box, top, p, terms = load_fixture("ubiquitin", valence=True)
+    if polar_eps is not None:
+        p = dataclasses.replace(p, polar_eps=polar_eps)
+    d = GOLDEN + "/ubiquitin/"
+    ff = tinker.read_prm(d + "amoeba_bio_2009_ubiquitin.prm.gz")
+    mass = tinker.masses(tinker.read_xyz(d + "ubiquitin.xyz.gz").types, ff)
+    return xyz, box, top, p, terms, mass
+
+
+def md_handle(top, box, p, terms, graph=False, predictor=ffi.NB_PRED_NONE):
+    h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+    h.set_cell(box)
+    h.set_valence(terms)
+    if predictor != ffi.NB_PRED_NONE:
+        h.set_scf_predictor(predictor)
+    if graph:
+        h.set_graph(True)
+    return h
+
+
+def velocities(mass, seed=5, temperature=298.15):
+    rng = np.random.default_rng(seed)
+    sd = np.sqrt(0.0019872042586408316 * temperature * 418.4 / mass)
+    return rng.normal(size=(len(mass), 3)) * sd[:, None]
+
+
+def test_velocity_verlet_matches_oracle(oracle):
+    xyz, box, top, p, terms, mass = ubiquitin()
+    v0 = velocities(mass)
+    dt = 0.001
+    h = md_handle(top, box, p, terms)
+    try:
+        h.set_positions(xyz.reshape(-1))
+        h.md_init(mass, v0.reshape(-1))
+        rep = h.md_step(3, dt)
+        xg, vg = h.md_state()
+    finally:
+        h.close()
+
+    def grad(x):
+        e, g, _, _ = oracle.evaluate(oracle.System(x, box, top), p)
+        ev, gv = oracle.valence(terms, x)
+        return e["total"] + ev["total"], g + gv
+
+    x, v = xyz.copy(), v0.copy()
+    m = mass[:, None]
+    a = -418.4 * grad(x)[1] / m
+    for _ in range(3):
+        v = v + a * (0.5 * dt)
+        x = x + v * dt
+        pe, g = grad(x)
+        a = -418.4 * g / m
+        v = v + a * (0.5 * dt)
+    ke = 0.5 * float((m * v * v).sum()) / 418.4
+    print(f"after 3 steps: KE {rep['kinetic']:.8f} / {ke:.8f}, PE {rep['potential']:.8f} / {pe:.8f}", flush=True)
+    assert np.abs(xg - x).max() <= 1e-9
+    assert rel(vg, v) <= 1e-9
+    assert abs(rep["kinetic"] - ke) <= 1e-9 * ke
+    assert abs(rep["potential"] - pe) <= 1e-8 * abs(pe)
+
+
+def test_nve_energy_conservation_and_graph_replay():
+    """40 steps of 0.5 fs with the predictor and graph replays: the total
+    energy stays within 0.1 % of the kinetic energy of its start, and the
+    trajectory equals the regular path's."""
+    xyz, box, top, p, terms, mass = ubiquitin(polar_eps=1e-7)
+    v0 = velocities(mass, seed=9)
+    runs = []
+    for graph in (True, False):
+        h = md_handle(top, box, p, terms, graph=graph, predictor=ffi.NB_PRED_ASPC)
+        try:
+            h.set_positions(xyz.reshape(-1))
+            h.md_init(mass, v0.reshape(-1))
+            r0 = h.md_step(0, 0.0005)
+            reps = [h.md_step(10, 0.0005) for _ in range(4)]
+            x, v = h.md_state()
+            runs.append((r0, reps, x, v, h.graph_replays()))
+        finally:
+            h.close()
+    (r0, reps, x, v, replays), (s0, sreps, sx, sv, _) = runs
+    drift = max(abs(r["total"] - r0["total"]) for r in reps)
+    iters = [r["pcg_iters"] / r["steps"] for r in reps]
+    print(f"E0 {r0['total']:.4f} KE0 {r0['kinetic']:.2f} max drift {drift:.4f}; PCG iterations per step {iters}; "
+          f"graph replays {replays}", flush=True)
+    assert drift <= 1e-3 * r0["kinetic"]
+    assert replays >= 30
+    assert np.abs(x - sx).max() <= 1e-9 and rel(v, sv) <= 1e-9
