@@ -409,6 +409,9 @@ WORKLOADS = {
                       "water, 54.99 x 41.91 x 41.91 A cell, PME 70 x 54 x 54 order 5, Ewald 6.7 A (alpha 0.57), "
                       "vdW 9 A, PCG to 1e-6 D, plus the 20,920 valence terms; coordinates and parameters from "
                       "the Tinker fixture in tests/golden/ubiquitin",
+    "ubiquitin-md": "velocity-Verlet dynamics of the reference's ubiquitin benchmark structure (9,737 atoms, "
+                    "AMOEBA-BIO-2009 valence and nonbonded terms, PME 70 x 54 x 54, PCG to 1e-6 D) at 1 fs steps, "
+                    "ASPC SCF predictor, each step's evaluation replayed from a HIP graph; a step is one MD step",
     "crystals-512": "512 independent P1 triclinic cells (lengths 8-20 A, angles 70-110 deg) of small molecules "
                     "with random multipoles, per-cell PME grids 16-32, Ewald and vdW 6 A, PCG 1e-5 D, image sums "
                     "in cells thinner than twice the cutoff; a step is one sweep over all cells (nb_evaluate_batch)",
@@ -452,6 +455,62 @@ def build_workload(name):
     return xyz, box, top, params, grid, lam, None
 
 
+def md_main(args):
+    """Steps per second of device-resident velocity-Verlet dynamics."""
+    from forcefieldx_amd import ffi, tinker
+    from forcefieldx_amd.dynamics import MolecularDynamics
+    from forcefieldx_amd.energy import ForceFieldEnergy
+    gold = os.path.join(ROOT, "tests", "golden", "ubiquitin")
+    xyz, box, top, params, valence = tinker.load(os.path.join(gold, "ubiquitin.xyz.gz"),
+                                                 os.path.join(gold, "ubiquitin.key"),
+                                                 os.path.join(gold, "amoeba_bio_2009_ubiquitin.prm.gz"), valence=True)
+    ff = tinker.read_prm(os.path.join(gold, "amoeba_bio_2009_ubiquitin.prm.gz"))
+    mass = tinker.masses(tinker.read_xyz(os.path.join(gold, "ubiquitin.xyz.gz")).types, ff)
+    dt_fs = 1.0
+    res = {}
+    for name, prec in (("fp64", ffi.NB_PREC_FP64), ("mixed", ffi.NB_PREC_MIXED)):
+        ffe = ForceFieldEnergy(top, box, params, precision=prec, valence=valence)
+        md = MolecularDynamics(ffe, mass)
+        md.init(xyz, md.initVelocities(298.15, seed=SEED))
+        r0 = md.dynamic(0, dt_fs)
+        md.dynamic(max(args.warmup, 8), dt_fs)     # fills the predictor's history
+        t0 = time.perf_counter()
+        r = md.dynamic(args.steps, dt_fs)
+        t = (time.perf_counter() - t0) / args.steps
+        res[name] = dict(s_per_step=t, pcg_iters_per_step=r["pcg_iters"] / max(r["steps"], 1),
+                         total_drift=r["total"] - r0["total"], kinetic=r["kinetic"],
+                         replays=ffe._handle.graph_replays())
+        ffe.destroy()
+    tm = res["mixed"]["s_per_step"]
+    cpu = None
+    if not args.no_cpu:
+        cpu, _, _ = cpu_baseline(top, box, xyz, params, "ubiquitin-md", valence=valence)
+        cpu = dict(cpu, unit="steps/s", sample=cpu["sample"] + "; one energy and gradient per MD step, the "
+                   "integrator's own work negligible beside it")
+    out = {
+        "metric": "AMOEBA MD steps/s (velocity Verlet, 1 fs)",
+        "value": 1.0 / tm,
+        "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": tm * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG/integration",
+        "data": "the reference's own test structure (tests/golden/ubiquitin), Maxwell-Boltzmann velocities at 298 K",
+        "config": {"workload": f"ubiquitin-md: {WORKLOADS['ubiquitin-md']}", "atoms": top.n, "grid": list(params.grid),
+                   "launch": "HIP graph replay per step", "parallelism": "single GPU"},
+        "ns_per_day": 86400.0 / tm * dt_fs * 1e-6,
+        "fp64_steps_per_s": 1.0 / res["fp64"]["s_per_step"],
+        "fp64_ns_per_day": 86400.0 / res["fp64"]["s_per_step"] * dt_fs * 1e-6,
+        "pcg_iters_per_step": {k: v["pcg_iters_per_step"] for k, v in res.items()},
+        "total_energy_drift": {k: v["total_drift"] for k, v in res.items()},
+        "graph_replays": {k: v["replays"] for k, v in res.items()},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = out["value"] / cpu["value"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -482,6 +541,8 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.workload == "ubiquitin-md":
+        return md_main(args)
     xyz, box, top, params, grid, lam, valence = build_workload(args.workload)
     if args.workload == "crystals-512":
         return batch_main(args, xyz, params)

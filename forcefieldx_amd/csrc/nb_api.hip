@@ -840,7 +840,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       }
     }
     bool mutual = h->prm.do_pme && h->prm.polarization == NB_POL_MUTUAL && h->ph.polar;
-    bool replayed = false;
+    bool replayed = false, shrink_graph = false;
     if (h->graph_on && h->gexec && h->graph_flags == flags && !h->profiling && !h->want_finite) {
       // replay; a list that needs rebuilding, an overflowed filtered row or a
       // solve that did not stop within the recorded iterations sends the
@@ -859,6 +859,10 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
           e.pcg_iters = (int)sc[S_ITER];
           e.pcg_eps = sc[S_EPSV];
           status = (int)sc[S_STAT];
+          // the recorded budget's surplus iterations still run their
+          // transforms; when solves have become cheaper (an SCF predictor
+          // warming up) the graph is recorded again at the next evaluation
+          if (e.pcg_iters + 2 < h->graph_iters) shrink_graph = true;
         }
       } else {
         drop_graph(h);
@@ -937,6 +941,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     // evaluation's iteration count plus one
     if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite)
       capture_graph(h, flags, mutual ? e.pcg_iters + 1 : 0);
+    if (shrink_graph) drop_graph(h);
     h->last = e;
     if (out) *out = e;
     if (status != NB_OK) {

@@ -288,7 +288,9 @@ void nb_destroy(void *handle);
  * control with a budget of the recorded evaluation's iterations plus one;
  * a replay whose list needs rebuilding, whose filtered rows overflow or whose
  * solve did not stop within the budget is repeated on the regular path and
- * recorded again.  nb_set_cell, nb_set_lambda and list rebuilds drop the graph.
+ * recorded again; so is one whose solve stopped two or more iterations short of
+ * the budget (the surplus iterations' transforms would still run).
+ * nb_set_cell, nb_set_lambda and list rebuilds drop the graph.
  * Results equal those of the regular path. */
 int nb_set_graph(void *handle, int on);
 
