@@ -103,7 +103,7 @@ def _create(path):
     return open(path, "w")
 
 
-def write_xyz(path, xf: XYZFile):
+def write_xyz(path, xf: XYZFile, digits=6):
     adj = [[] for _ in range(len(xf.names))]
     for i, j in xf.bonds:
         adj[i].append(j)
@@ -115,7 +115,8 @@ def write_xyz(path, xf: XYZFile):
         for a, nm in enumerate(xf.names):
             x, y, z = xf.xyz[a]
             nb = "".join(f"{j + 1:6d}" for j in sorted(adj[a]))
-            f.write(f"{a + 1:6d}  {nm:<3s} {x:11.6f} {y:11.6f} {z:11.6f} {int(xf.types[a]):5d}{nb}\n")
+            f.write(f"{a + 1:6d}  {nm:<3s} {x:{digits + 5}.{digits}f} {y:{digits + 5}.{digits}f} {z:{digits + 5}.{digits}f}"
+                    f" {int(xf.types[a]):5d}{nb}\n")
 
 
 # --------------------------------------------------------------------------
@@ -496,3 +497,23 @@ def load(xyz_path, key_path, prm_path, valence=False):
         return out
     from .bonded import assign
     return out + (assign(xf.types, xf.bonds, ff, keys),)
+
+
+def load_crystal(xyz_path, key_path, prm_path):
+    """A crystal of the key file's space group expanded to its P1 cell
+    (forcefieldx_amd.crystal): (xyz, box, Topology, NonbondedParams,
+    ValenceTerms, nsymm).  Per-asymmetric-unit energies are the P1 energies
+    divided by nsymm; the cell comes from the keywords."""
+    from .bonded import assign
+    from .crystal import expand_to_p1
+    xf, ff, keys = read_xyz(xyz_path), read_prm(prm_path), read_keys(key_path)
+    box = [_f(keys, "a-axis", 0.0), _f(keys, "b-axis", 0.0), _f(keys, "c-axis", 0.0), _f(keys, "alpha", 90.0),
+           _f(keys, "beta", 90.0), _f(keys, "gamma", 90.0)]
+    box[1] = box[1] or box[0]
+    box[2] = box[2] or box[0]
+    sg = (keys.get("spacegroup") or ["P1"])[0]
+    x, t, b, names, nsymm = expand_to_p1(xf.xyz, xf.types, xf.bonds, xf.names, box, sg)
+    p1 = XYZFile(xf.title, names, x, t, b, box)
+    keys = {k: v for k, v in keys.items() if k != "spacegroup"}
+    xyz, bx, top, params = build(p1, ff, keys)
+    return xyz, bx, top, params, assign(p1.types, p1.bonds, ff, keys), nsymm

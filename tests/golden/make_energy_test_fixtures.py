@@ -9,6 +9,11 @@ system uses, in the same Tinker formats, and the test's expected valence and
 nonbonded energies and term / interaction counts (tolerance 1.0e-2 kcal/mol,
 ParentEnergyTest.java:101).
 
+It also writes tests/golden/crystals: the asymmetric units, keywords and
+parameter patches of the five crystal benchmarks (EnergyTest.java:435-614,
+modules/potential/src/test/resources/<name>.xyz, .properties, .patch) with
+the tests' per-asymmetric-unit energies and counts.
+
 Run in a checkout that has the reference at /root/reference:
     python tests/golden/make_energy_test_fixtures.py
 """
@@ -50,6 +55,60 @@ SYSTEMS = {
 }
 
 
+# P1 crystals of the reference's EnergyTest (EnergyTest.java:435-615): the
+# asymmetric unit, its keywords (space group and cell) and its parameter file
+CRYSTALS = {
+    "acetanilide": (":435-470 (Acetanilide Benchmark)", "acetanilide.patch",
+                    {"bond": [0.67814832, 19], "angle": [1.06817134, 30], "strbnd": [-0.06290501, 26],
+                     "urey": [0.0, 0], "opbend": [0.13033970, 24], "torsion": [-8.01262794, 38],
+                     "pitors": [0.0, 0], "tortor": [0.0, 0]},
+                    (-3.62572228, 7295), (-26.0720131242538, 2231), (-1.9856761645305, 2231)),
+    "ethylparaben": (":471-506 (Ethylparaben Benchmark)", "ethylparaben.patch",
+                     {"bond": [1.37991170, 44], "angle": [2.43600964, 70], "strbnd": [0.20006782, 58],
+                      "urey": [0.0, 0], "opbend": [0.04434686, 42], "torsion": [-7.14298308, 88],
+                      "pitors": [0.0, 0], "tortor": [0.0, 0]},
+                     (-4.816366774263829, 32644), (-45.55012417320137, 5012), None),
+    "methylparaben": (":507-542 (Methylparaben Benchmark)", "methylparaben.patch",
+                      {"bond": [0.42501694, 19], "angle": [1.23611203, 29], "strbnd": [0.00351228, 24],
+                       "urey": [0.0, 0], "opbend": [0.01058196, 21], "torsion": [-3.72750990, 35],
+                       "pitors": [0.0, 0], "tortor": [0.0, 0]},
+                      (0.20573979, 7639), (-26.558577471708134, 2314), (-2.35904592, 2314)),
+    "paracetamol": (":543-578 (Paracetamol Benchmark)", "paracetamol.patch",
+                    {"bond": [0.55316710, 20], "angle": [1.51848706, 31], "strbnd": [0.05096337, 27],
+                     "urey": [0.0, 0], "opbend": [0.46651316, 24], "torsion": [-7.40125527, 40],
+                     "pitors": [0.0, 0], "tortor": [0.0, 0]},
+                    (0.76621272, 7832), (-32.02011297249507, 2357), (-4.96649992387851, 2357)),
+    "phenacetin": (":579-614 (Phenacetin Benchmark)", "phenacetin.patch",
+                   {"bond": [0.56492623, 26], "angle": [2.68273715, 43], "strbnd": [0.00409248, 35],
+                    "urey": [0.0, 0], "opbend": [0.33385711, 24], "torsion": [-6.49787814, 52],
+                    "pitors": [0.0, 0], "tortor": [0.0, 0]},
+                   (-5.792072251098926, 20384), (-25.42173449818894, 3138), None),
+}
+
+
+def write_crystals():
+    out = os.path.join(HERE, "crystals")
+    os.makedirs(out, exist_ok=True)
+    expected = {}
+    for name, (lines, patch, valence, vdw, perm, pol) in CRYSTALS.items():
+        xf = tinker.read_xyz(REF + f"/test/resources/{name}.xyz")
+        keys = tinker.read_keys(REF + f"/test/resources/{name}.properties")
+        ff = tinker.read_prm(REF + f"/test/resources/{patch}")
+        tinker.write_xyz(os.path.join(out, f"{name}.xyz"), xf, digits=8)
+        with open(os.path.join(out, f"{name}.key"), "w") as f:
+            for k, v in keys.items():
+                if k != "parameters":
+                    f.write(f"{k:<20s}{' '.join(v)}\n")
+        tinker.write_prm(os.path.join(out, f"{name}.prm"), ff)
+        e = {"source": "EnergyTest.java" + lines + ", tolerance 1.0e-2 (ParentEnergyTest.java:101)",
+             "atoms": len(xf.names), "valence": valence, "vdw": list(vdw), "perm": list(perm), "tolerance": 1.0e-2}
+        if pol is not None:
+            e["pol"] = list(pol)
+        expected[name] = e
+    with open(os.path.join(out, "expected.json"), "w") as f:
+        json.dump(expected, f, indent=1)
+
+
 def main():
     ff = tinker.read_prm(REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_BIO_2009")
     for name, exp in SYSTEMS.items():
@@ -69,3 +128,4 @@ def main():
 
 if __name__ == "__main__":
     main()
+    write_crystals()
