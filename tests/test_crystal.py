@@ -98,3 +98,31 @@ def test_cell_matrix_reproduces_lengths_and_angles():
     ang = [np.degrees(np.arccos(m[i] @ m[j] / (ln[i] * ln[j]))) for i, j in ((1, 2), (0, 2), (0, 1))]
     assert np.allclose(ang, box[3:])
     assert m[0, 1] == m[0, 2] == m[1, 2] == 0.0
+
+
+def test_folded_gradient_central_differences(oracle):
+    """The per-unit gradient of the whole nonbonded energy with respect to
+    asymmetric-unit coordinates (all images moving with it), against central
+    differences of the P1 energy / nsymm, on a monoclinic P21/c cell."""
+    import dataclasses
+    from forcefieldx_amd.crystal import fold_gradient
+    name = "paracetamol"
+    files = [os.path.join(CRYSTALS, name + ext) for ext in (".xyz", ".key", ".prm")]
+    xf = tinker.read_xyz(files[0])
+    xyz, box, top, p, terms, ns = tinker.load_crystal(*files)
+    p = dataclasses.replace(p, polar_eps=1e-11)
+    assert box[4] != 90.0
+
+    def energy(asu):
+        x = expand_to_p1(asu, xf.types, xf.bonds, xf.names, box, "P21/c")[0]
+        return oracle.evaluate(oracle.System(x, box, top), p)[0]["total"] / ns
+
+    e, g = oracle.evaluate(oracle.System(xyz, box, top), p)[:2]
+    gu = fold_gradient(g, box, "P21/c")
+    h = 1e-5
+    for i in (0, 7, 13, 19):
+        for d in range(3):
+            xp = xf.xyz.copy(); xp[i, d] += h
+            xm = xf.xyz.copy(); xm[i, d] -= h
+            fd = (energy(xp) - energy(xm)) / (2 * h)
+            assert abs(gu[i, d] - fd) <= 1e-5 * max(1.0, abs(fd)), (i, d, gu[i, d], fd)
