@@ -128,7 +128,49 @@ __global__ void k_md_ke(int nb, const double *__restrict__ part, double *__restr
   if (threadIdx.x == 0) ke[0] = 0.5 * (red[0] + red[1] + red[2] + red[3]) / KCAL_TO_GRAM_ANG2_PER_PS2;
 }
 
+// Berendsen weak coupling (Berendsen.java:99-101) after the full step: T from
+// the kinetic energy over 3n - 3 degrees of freedom (Thermostat.java:167,
+// 262-274), v *= sqrt(1 + dt/tau (T0/T - 1)); a system at rest is left as it is.
+constexpr double KB_GRAM_ANG2_PER_PS2_K = 0.831446261815324;  // Boltzmann constant, g A^2 / (ps^2 mol K)
+
+__device__ __forceinline__ double berendsen_scale(double ke, double dof, double t0, double dt_tau) {
+  if (!(ke > 0.0)) return 1.0;
+  const double temp = 2.0 * ke * KCAL_TO_GRAM_ANG2_PER_PS2 / (KB_GRAM_ANG2_PER_PS2_K * dof);
+  return sqrt(1.0 + dt_tau * (t0 / temp - 1.0));
+}
+
+__global__ void __launch_bounds__(256) k_md_berendsen(int n3, double dof, double t0, double dt_tau,
+                                                      const double *__restrict__ ke, double *__restrict__ v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n3) v[q] *= berendsen_scale(ke[0], dof, t0, dt_tau);
+}
+
+// the kinetic energy after scaling (every block above has read the old value)
+__global__ void k_md_ke_scale(double dof, double t0, double dt_tau, double *__restrict__ ke) {
+  if (threadIdx.x == 0) {
+    const double s = berendsen_scale(ke[0], dof, t0, dt_tau);
+    ke[0] *= s * s;
+  }
+}
+
 }  // namespace
+
+int nb_md_set_thermostat(void *handle, int kind, double temperature, double tau) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  if (kind == NB_THERMO_NONE) {
+    h->md_thermo = NB_THERMO_NONE;
+    return NB_OK;
+  }
+  if (kind != NB_THERMO_BERENDSEN || !(temperature > 0.0) || !(tau > 0.0) || h->n < 2) {
+    h->err = "thermostat: kind NB_THERMO_BERENDSEN with temperature > 0 K and tau > 0 ps, at least two atoms";
+    return NB_ERR_INVALID;
+  }
+  h->md_thermo = kind;
+  h->md_t0 = temperature;
+  h->md_tau = tau;
+  return NB_OK;
+}
 
 // ---------------------------------------------------------------------------
 // predictor, called from the evaluation sequence
@@ -222,6 +264,10 @@ int nb_md_init(void *handle, const double *mass, const double *v0) {
 int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out) {
   NbHandle *h = (NbHandle *)handle;
   if (!h || nsteps < 0 || !(dt > 0.0)) return NB_ERR_INVALID;
+  if (h->md_thermo == NB_THERMO_BERENDSEN && dt > h->md_tau) {
+    h->err = "thermostat: the time step must not exceed tau";
+    return NB_ERR_INVALID;
+  }
   if (!h->md_ready || !h->cell_set || !h->positions_set) {
     h->err = "nb_md_init, nb_set_cell and nb_set_positions must be called before nb_md_step";
     return NB_ERR_INVALID;
@@ -254,6 +300,13 @@ int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out) {
       if (rc != NB_OK) return rc;
       iters += e.pcg_iters;
       kick(0);
+      if (h->md_thermo == NB_THERMO_BERENDSEN) {
+        const double dof = 3.0 * h->n - 3.0, dt_tau = dt / h->md_tau;
+        hipLaunchKernelGGL(k_md_berendsen, dim3(nb), dim3(256), 0, h->stream, n3, dof, h->md_t0, dt_tau,
+                           (const double *)h->d_md_ke, h->d_md_v);
+        hipLaunchKernelGGL(k_md_ke_scale, dim3(1), dim3(64), 0, h->stream, dof, h->md_t0, dt_tau, h->d_md_ke);
+        HIPCHECK(hipGetLastError());
+      }
     }
     double ke = 0.0;
     HIPCHECK(hipMemcpyAsync(&ke, h->d_md_ke, sizeof(double), hipMemcpyDeviceToHost, h->stream));

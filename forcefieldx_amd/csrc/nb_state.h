@@ -204,6 +204,8 @@ struct NbHandle {
   int32_t *d_pred_ctl = nullptr;  // [2] ring start, fill count (advanced on the device)
   double *d_md_v = nullptr, *d_md_a = nullptr, *d_md_mass = nullptr, *d_md_part = nullptr, *d_md_ke = nullptr;
   bool md_ready = false, md_accel = false;
+  int md_thermo = 0;                 // NB_THERMO_*
+  double md_t0 = 0.0, md_tau = 0.0;  // K, ps
 
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
   // replayed; the PCG then runs graph_iters iterations under device-side

@@ -18,7 +18,9 @@ KB = 0.0019872042586408316             # kcal/mol/K (Constants.java, R / 1000 in
 class MolecularDynamics:
     """Velocity Verlet, adiabatic (no thermostat), on a ForceFieldEnergy."""
 
-    def __init__(self, ffe, masses, scf_predictor=ffi.NB_PRED_ASPC, graph=True):
+    def __init__(self, ffe, masses, scf_predictor=ffi.NB_PRED_ASPC, graph=True, berendsen=None):
+        """berendsen: (temperature K, tau ps) for the Berendsen thermostat
+        (Berendsen.java:99-101), None for NVE."""
         self.ffe = ffe
         self.mass = np.ascontiguousarray(masses, np.float64)
         self._h = ffe._handle
@@ -26,6 +28,8 @@ class MolecularDynamics:
             self._h.set_scf_predictor(scf_predictor)
         if graph:
             self._h.set_graph(True)
+        if berendsen is not None:
+            self._h.md_set_thermostat(ffi.NB_THERMO_BERENDSEN, *berendsen)
         self._report = None
 
     def initVelocities(self, temperature, seed=0):

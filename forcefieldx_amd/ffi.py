@@ -25,8 +25,10 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_get_dudx_dl", "nb_last_error", "nb_destroy", "nb_abi_version", "nb_device_count",
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
             "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config", "nb_set_valence",
-            "nb_valence_struct_size", "nb_set_scf_predictor", "nb_md_init", "nb_md_step", "nb_md_get_state"]
+            "nb_valence_struct_size", "nb_set_scf_predictor", "nb_md_init", "nb_md_step", "nb_md_get_state",
+            "nb_md_set_thermostat"]
 NB_PRED_NONE, NB_PRED_ASPC = 0, 1
+NB_THERMO_NONE, NB_THERMO_BERENDSEN = 0, 1
 VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
 
 _dp = C.POINTER(C.c_double)
@@ -154,6 +156,8 @@ def load(path=LIB_PATH):
     L.nb_md_step.restype = C.c_int
     L.nb_md_get_state.argtypes = [vp, _dp, _dp]
     L.nb_md_get_state.restype = C.c_int
+    L.nb_md_set_thermostat.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+    L.nb_md_set_thermostat.restype = C.c_int
     L.nb_set_valence.argtypes = [vp, C.POINTER(NbValence)]
     L.nb_set_valence.restype = C.c_int
     L.nb_valence_struct_size.restype = C.c_int
@@ -272,6 +276,11 @@ class Handle:
         self._check(self.L.nb_md_step(self.h, int(nsteps), float(dt_ps), C.byref(r)))
         return dict(kinetic=r.kinetic, potential=r.potential, total=r.total, steps=r.steps,
                     pcg_iters=r.pcg_iters, last=r.last.as_dict())
+
+    def md_set_thermostat(self, kind, temperature=298.15, tau_ps=0.1):
+        """Berendsen.java:99-101 after each full step (NB_THERMO_BERENDSEN),
+        or none (NB_THERMO_NONE)."""
+        self._check(self.L.nb_md_set_thermostat(self.h, int(kind), float(temperature), float(tau_ps)))
 
     def md_state(self):
         x = np.zeros(3 * self.n)

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 7
+#define NB_ABI_VERSION 8
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -275,6 +275,17 @@ int nb_set_scf_predictor(void *handle, int kind);
 int nb_md_init(void *handle, const double *mass, const double *v0);
 int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out);
 int nb_md_get_state(void *handle, double *x, double *v);
+
+/* Thermostat of nb_md_step (replaces the Berendsen class,
+ * modules/algorithms/.../dynamics/thermostats/Berendsen.java:99-101, applied
+ * after each full step as MolecularDynamics.java:1577-1583).  kind
+ * NB_THERMO_BERENDSEN: after each step's second half kick the velocities are
+ * scaled by sqrt(1 + dt/tau (T0/T - 1)), T = sum m v^2 / (kB (3n - 3)),
+ * temperature T0 in K, tau in ps (dt <= tau, else nb_md_step returns
+ * NB_ERR_INVALID); the reported kinetic energy is the scaled one.
+ * NB_THERMO_NONE (the default) gives NVE. */
+enum { NB_THERMO_NONE = 0, NB_THERMO_BERENDSEN = 1 };
+int nb_md_set_thermostat(void *handle, int kind, double temperature, double tau);
 
 /* Human-readable message for the last error on this handle. */
 const char *nb_last_error(void *handle);

@@ -154,3 +154,36 @@ def test_nve_energy_conservation_and_graph_replay():
     assert drift <= 1e-3 * r0["kinetic"]
     assert replays >= 30
     assert np.abs(x - sx).max() <= 1e-9 and rel(v, sv) <= 1e-9
+
+
+def test_berendsen_scales_velocities_after_the_full_step():
+    """One step under the Berendsen thermostat (Berendsen.java:99-101) equals
+    one NVE step with the velocities scaled by sqrt(1 + dt/tau (T0/T - 1)),
+    T = sum m v^2 / (kB (3n - 3)) from the NVE step's kinetic energy
+    (Thermostat.java:167, 262-274); a time step above tau is refused."""
+    xyz, box, top, p, terms, mass = ubiquitin()
+    v0 = velocities(mass)
+    dt, tau, t0 = 0.001, 0.02, 350.0
+    out = []
+    for kind in (ffi.NB_THERMO_NONE, ffi.NB_THERMO_BERENDSEN):
+        h = md_handle(top, box, p, terms)
+        try:
+            h.set_positions(xyz.reshape(-1))
+            h.md_set_thermostat(kind, t0, tau)
+            h.md_init(mass, v0.reshape(-1))
+            rep = h.md_step(1, dt)
+            out.append((rep, *h.md_state()))
+            if kind == ffi.NB_THERMO_BERENDSEN:
+                with pytest.raises(ffi.NbError):
+                    h.md_step(1, 2 * tau)
+        finally:
+            h.close()
+    (r1, x1, v1), (r2, x2, v2) = out
+    kb, dof = 0.831446261815324, 3 * top.n - 3
+    temp = lambda ke: 2.0 * ke * 418.4 / (kb * dof)
+    s = np.sqrt(1.0 + dt / tau * (t0 / temp(r1["kinetic"]) - 1.0))
+    assert s > 1.0
+    # equal but for the summation order of the grid's atomic adds
+    assert np.abs(x2 - x1).max() <= 1e-12 * np.abs(x1).max()
+    assert np.abs(v2 - s * v1).max() <= 1e-12 * np.abs(v1).max()
+    assert abs(r2["kinetic"] - s * s * r1["kinetic"]) <= 1e-12 * r1["kinetic"]
