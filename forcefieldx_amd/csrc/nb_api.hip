@@ -714,11 +714,11 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
         if (!h->want_finite) nb_pred_apply(h);
         if (capture) {
           status = nb_pcg_device(h, h->graph_iters);
-          nb_pred_save(h);
+          nb_pred_save(h, true);
         } else if (h->prm.scf_algorithm == NB_SCF_SOR || h->ph.sor) {
           status = nb_sor(h, h->prm.polar_sor, &e);
           h->used_sor = true;
-          if (status == NB_OK) nb_pred_save(h);
+          if (status == NB_OK) nb_pred_save(h, false);
         } else {
           h->tape_on = h->want_finite;
           h->tape_K = 0;
@@ -735,7 +735,7 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
             nb_finite_adjoint(h);
             h->finite_active = true;
           }
-          if (status == NB_OK && !h->finite_active) nb_pred_save(h);
+          if (status == NB_OK && !h->finite_active) nb_pred_save(h, false);
         }
       } else if (use_recip) {
         // direct polarization: only the reciprocal potentials of u_direct are needed

@@ -1196,6 +1196,8 @@ static bool batchable(const NbHandle *h, uint32_t flags) {
   if (!h->cell_set || !h->positions_set || h->cell.aperiodic || h->elec_lambda || h->ost || h->profiling ||
       h->has_valence)
     return false;
+  // the batch kernel neither applies nor extends an SCF predictor's history
+  if (h->predictor != NB_PRED_NONE) return false;
   if (flags & ~(uint32_t)NB_GRAD) return false;
   if (h->has_soft && h->lambda < 1.0) return false;
   if (h->prm.scf_algorithm != NB_SCF_PCG) return false;

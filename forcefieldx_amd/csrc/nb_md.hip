@@ -61,13 +61,24 @@ __global__ void k_pred_apply(int n, const int32_t *__restrict__ perm, const doub
   }
 }
 
+// A solve's dipoles enter the history only when the evaluation that produced
+// them is kept: its filtered rows did not overflow (flags 1, 2, 5: the
+// evaluation is repeated with larger rows), no list rebuild was pending
+// (flag 3: a replay is repeated on the regular path), and, for a solve under
+// device control (scal non-null), the solve stopped converged.
+__device__ __forceinline__ bool pred_keep(const int32_t *__restrict__ flags, const double *__restrict__ scal) {
+  if (flags[1] | flags[2] | flags[3] | flags[5]) return false;
+  return !scal || (scal[S_DONE] != 0.0 && scal[S_STAT] == 0.0);
+}
+
 // the converged u - u_direct into the slot before the current start
 __global__ void k_pred_save(int n, const int32_t *__restrict__ perm, const double *__restrict__ u,
                             const double *__restrict__ up, const double *__restrict__ ud,
                             const double *__restrict__ upd, const int32_t *__restrict__ ctl,
-                            double *__restrict__ hist) {
+                            double *__restrict__ hist, const int32_t *__restrict__ flags,
+                            const double *__restrict__ scal) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n || !pred_keep(flags, scal)) return;
   const int slot = (ctl[0] + 5) % 6, i = perm[s];
   double *h = hist + (size_t)slot * 6 * (size_t)n;
   for (int d = 0; d < 3; d++) {
@@ -76,8 +87,8 @@ __global__ void k_pred_save(int n, const int32_t *__restrict__ perm, const doubl
   }
 }
 
-__global__ void k_pred_advance(int32_t *ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ void k_pred_advance(int32_t *ctl, const int32_t *__restrict__ flags, const double *__restrict__ scal) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && pred_keep(flags, scal)) {
     ctl[0] = (ctl[0] + 5) % 6;
     ctl[1] = ctl[1] < 6 ? ctl[1] + 1 : 6;
   }
@@ -129,8 +140,9 @@ __global__ void k_md_ke(int nb, const double *__restrict__ part, double *__restr
 }
 
 // Berendsen weak coupling (Berendsen.java:99-101) after the full step: T from
-// the kinetic energy over 3n - 3 degrees of freedom (Thermostat.java:167,
-// 262-274), v *= sqrt(1 + dt/tau (T0/T - 1)); a system at rest is left as it is.
+// the kinetic energy over 3m - 3 degrees of freedom, m the atoms with mass > 0
+// (the active variables of Thermostat.java:167, 262-274),
+// v *= sqrt(1 + dt/tau (T0/T - 1)); a system at rest is left as it is.
 constexpr double KB_GRAM_ANG2_PER_PS2_K = 0.831446261815324;  // Boltzmann constant, g A^2 / (ps^2 mol K)
 
 __device__ __forceinline__ double berendsen_scale(double ke, double dof, double t0, double dt_tau) {
@@ -139,10 +151,13 @@ __device__ __forceinline__ double berendsen_scale(double ke, double dof, double 
   return sqrt(1.0 + dt_tau * (t0 / temp - 1.0));
 }
 
+// only atoms with mass > 0 move (Berendsen.fullStep skips the others, as the
+// drift and kick do)
 __global__ void __launch_bounds__(256) k_md_berendsen(int n3, double dof, double t0, double dt_tau,
-                                                      const double *__restrict__ ke, double *__restrict__ v) {
+                                                      const double *__restrict__ ke, const double *__restrict__ mass,
+                                                      double *__restrict__ v) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n3) v[q] *= berendsen_scale(ke[0], dof, t0, dt_tau);
+  if (q < n3 && mass[q / 3] > 0.0) v[q] *= berendsen_scale(ke[0], dof, t0, dt_tau);
 }
 
 // the kinetic energy after scaling (every block above has read the old value)
@@ -184,12 +199,13 @@ void nb_pred_apply(NbHandle *h) {
             mixed_flag(h));
 }
 
-void nb_pred_save(NbHandle *h) {
+void nb_pred_save(NbHandle *h, bool device_ctl) {
   if (h->predictor == NB_PRED_NONE || !h->d_pred) return;
+  const double *scal = device_ctl ? h->d_scal : nullptr;
   NB_LAUNCH(k_pred_save, grid_for(h->n, 256), dim3(256), 0, h->n, (const int32_t *)h->d_perm,
             (const double *)h->d_u, (const double *)h->d_up, (const double *)h->d_udir, (const double *)h->d_updir,
-            (const int32_t *)h->d_pred_ctl, h->d_pred);
-  NB_LAUNCH(k_pred_advance, dim3(1), dim3(64), 0, h->d_pred_ctl);
+            (const int32_t *)h->d_pred_ctl, h->d_pred, (const int32_t *)h->d_flags, scal);
+  NB_LAUNCH(k_pred_advance, dim3(1), dim3(64), 0, h->d_pred_ctl, (const int32_t *)h->d_flags, scal);
 }
 
 void nb_md_free(NbHandle *h) {
@@ -246,6 +262,8 @@ int nb_md_init(void *handle, const double *mass, const double *v0) {
       HIPCHECK(hipMalloc(&h->d_md_ke, sizeof(double)));
     }
     HIPCHECK(hipMemcpy(h->d_md_mass, mass, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->md_active = 0;
+    for (int i = 0; i < n; i++) h->md_active += mass[i] > 0.0 ? 1 : 0;
     if (v0) {
       HIPCHECK(hipMemcpy(h->d_md_v, v0, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     } else {
@@ -270,6 +288,10 @@ int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out) {
   }
   if (!h->md_ready || !h->cell_set || !h->positions_set) {
     h->err = "nb_md_init, nb_set_cell and nb_set_positions must be called before nb_md_step";
+    return NB_ERR_INVALID;
+  }
+  if (h->md_thermo == NB_THERMO_BERENDSEN && h->md_active < 2) {
+    h->err = "thermostat: at least two atoms with mass > 0";
     return NB_ERR_INVALID;
   }
   try {
@@ -301,9 +323,9 @@ int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out) {
       iters += e.pcg_iters;
       kick(0);
       if (h->md_thermo == NB_THERMO_BERENDSEN) {
-        const double dof = 3.0 * h->n - 3.0, dt_tau = dt / h->md_tau;
+        const double dof = 3.0 * h->md_active - 3.0, dt_tau = dt / h->md_tau;
         hipLaunchKernelGGL(k_md_berendsen, dim3(nb), dim3(256), 0, h->stream, n3, dof, h->md_t0, dt_tau,
-                           (const double *)h->d_md_ke, h->d_md_v);
+                           (const double *)h->d_md_ke, (const double *)h->d_md_mass, h->d_md_v);
         hipLaunchKernelGGL(k_md_ke_scale, dim3(1), dim3(64), 0, h->stream, dof, h->md_t0, dt_tau, h->d_md_ke);
         HIPCHECK(hipGetLastError());
       }

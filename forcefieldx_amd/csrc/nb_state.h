@@ -205,6 +205,7 @@ struct NbHandle {
   double *d_md_v = nullptr, *d_md_a = nullptr, *d_md_mass = nullptr, *d_md_part = nullptr, *d_md_ke = nullptr;
   bool md_ready = false, md_accel = false;
   int md_thermo = 0;                 // NB_THERMO_*
+  int md_active = 0;                 // atoms with mass > 0 (degrees of freedom 3 md_active - 3)
   double md_t0 = 0.0, md_tau = 0.0;  // K, ps
 
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
@@ -260,7 +261,7 @@ void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
 void nb_tape_push(NbHandle *h);
 void nb_pred_apply(NbHandle *h);
-void nb_pred_save(NbHandle *h);
+void nb_pred_save(NbHandle *h, bool device_ctl);
 void nb_md_free(NbHandle *h);
 void nb_valence_terms(NbHandle *h);
 void nb_valence_gather(NbHandle *h);

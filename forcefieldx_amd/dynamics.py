@@ -16,7 +16,8 @@ KB = 0.0019872042586408316             # kcal/mol/K (Constants.java, R / 1000 in
 
 
 class MolecularDynamics:
-    """Velocity Verlet, adiabatic (no thermostat), on a ForceFieldEnergy."""
+    """Velocity Verlet on a ForceFieldEnergy: adiabatic (NVE) by default, or
+    with the Berendsen thermostat."""
 
     def __init__(self, ffe, masses, scf_predictor=ffi.NB_PRED_ASPC, graph=True, berendsen=None):
         """berendsen: (temperature K, tau ps) for the Berendsen thermostat

@@ -458,25 +458,153 @@ def _smooth_at_least(x, lo=16, hi=32):
         n += 1
 
 
+MOLECULE_BOND = 1.1        # centre-substituent bond of the packed small molecules (A)
+
+
+def _star_geometry(m):
+    """Body-frame geometry of a compact m-atom molecule (3 <= m <= 5): a centre
+    at the origin and m - 1 substituents at MOLECULE_BOND along tetrahedral
+    directions (m = 3: bent at 109.47 deg, m = 4: pyramidal, m = 5:
+    tetrahedral)."""
+    t = np.array([[1.0, 1.0, 1.0], [1.0, -1.0, -1.0], [-1.0, 1.0, -1.0], [-1.0, -1.0, 1.0]]) / np.sqrt(3.0)
+    return np.concatenate([np.zeros((1, 3)), MOLECULE_BOND * t[:m - 1]])
+
+
+def _molecule_sizes(rng, n):
+    """Sizes in {3, 4, 5} that add up to n (n >= 6 or n in {3, 4, 5})."""
+    sizes = []
+    rem = n
+    while rem > 5:
+        choices = [k for k in (3, 4, 5) if rem - k not in (1, 2)]
+        k = int(rng.choice(choices))
+        sizes.append(k)
+        rem -= k
+    sizes.append(rem)
+    return sizes
+
+
+def molecular_crystal(n_atoms, box, seed=0, min_dist=1.9, orient_tries=60):
+    """P1 cell of exactly ``n_atoms`` atoms in compact rigid molecules of 3-5
+    atoms (a centre with substituents at 1.1 A) packed on a lattice: molecule
+    centres sit on a g0 x g1 x g2 grid of fractional sites (spacing as even as
+    the cell allows, about 3.4 A at 0.1 atoms/A^3) with a small jitter, each
+    in a random orientation drawn until every atom is at least ``min_dist``
+    from every image of the atoms of other molecules already placed; a site
+    that admits no orientation is left empty and the molecule takes a spare
+    site.  Charges N(0, 0.3) e made neutral per molecule, local dipoles of
+    0.1 e*A and traceless quadrupoles of 0.3 e*A^2 in Z-then-X frames,
+    polarizabilities U(0.3, 1.5) A^3, Thole 0.39, one polarization group per
+    molecule; centres take AMOEBA water oxygen vdW parameters and
+    substituents water hydrogen parameters with their 0.91 reduction toward
+    the centre.  Raises RuntimeError when the atoms do not fit.  Returns
+    (xyz, box, Topology)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    box = np.asarray(box, float)
+    L = lattice_vectors(box)
+    vol = abs(np.linalg.det(L))
+    sizes = _molecule_sizes(rng, n_atoms)
+    M = len(sizes)
+    # site grid: about M * 1.15 sites, spaced in proportion to the cell lengths
+    lens = box[:3]
+    s = (vol / (1.15 * M)) ** (1.0 / 3.0)
+    g = np.maximum(1, np.round(lens / s)).astype(int)
+    while np.prod(g) < M + max(2, M // 10):
+        g[np.argmax(lens / g)] += 1
+    sites = np.array([(i, j, k) for i in range(g[0]) for j in range(g[1]) for k in range(g[2])], float)
+    sites = (sites + 0.5) / g
+    rng.shuffle(sites)
+    img = np.array([(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)], float) @ L
+    placed = np.zeros((0, 3))
+    mols = []
+    site = 0
+    for m in sizes:
+        geo = _star_geometry(m)
+        ok = False
+        while not ok and site < len(sites):
+            centre = (sites[site] + rng.uniform(-0.08, 0.08, 3) / g) @ L
+            site += 1
+            # images of placed atoms that a molecule at this centre can reach
+            near = np.zeros((0, 3))
+            if placed.shape[0]:
+                cand = (placed[:, None, :] + img[None, :, :]).reshape(-1, 3)
+                reach = MOLECULE_BOND + min_dist
+                near = cand[np.einsum("ij,ij->i", cand - centre, cand - centre) < reach * reach]
+            for _ in range(orient_tries):
+                pos = geo @ _random_rotations(rng, 1)[0].T + centre
+                if near.shape[0]:
+                    dd = near[:, None, :] - pos[None, :, :]
+                    if np.einsum("ijk,ijk->ij", dd, dd).min() < min_dist ** 2:
+                        continue
+                ok = True
+                break
+        if not ok:
+            raise RuntimeError(f"molecular_crystal: {n_atoms} atoms do not fit the cell {box.tolist()}")
+        mols.append(pos)
+        placed = np.concatenate([placed, pos])
+    n = placed.shape[0]
+    frame = np.full(n, FRAME_ZTHENX, np.int32)
+    axis = -np.ones((n, 3), np.int32)
+    mp = np.zeros((n, 10))
+    mol = np.zeros(n, np.int32)
+    vcls = np.zeros(n, np.int32)
+    red_i = np.arange(n, dtype=np.int32)
+    red_v = np.zeros(n)
+    bonds = []
+    a0 = 0
+    for k, pos in enumerate(mols):
+        na = len(pos)
+        q = rng.normal(0.0, 0.3, na)
+        q -= q.mean()
+        dv = rng.normal(size=(na, 3))
+        dv *= 0.1 / np.linalg.norm(dv, axis=1, keepdims=True)
+        mp[a0:a0 + na] = np.concatenate([q[:, None], dv, _random_traceless(rng, na, 0.3)], axis=1)
+        c = a0
+        axis[c, 0], axis[c, 1] = c + 1, c + 2
+        for j in range(1, na):
+            i = a0 + j
+            axis[i, 0], axis[i, 1] = c, (a0 + 1 if j != 1 else a0 + 2)
+            bonds.append((c, i))
+            vcls[i] = 1
+            red_i[i] = c
+            red_v[i] = WATER_VDW["H"][2]
+        mol[a0:a0 + na] = k
+        a0 += na
+    pol = rng.uniform(0.3, 1.5, n)
+    off, atoms, rels = masks_from_bonds(n, bonds)
+    tab = vdw_pair_table([WATER_VDW["O"][0], WATER_VDW["H"][0]], [WATER_VDW["O"][1], WATER_VDW["H"][1]])
+    top = Topology(n=n, frame=frame, axis=axis, mpole=mp, polarity=pol, thole=np.full(n, WATER_THOLE),
+                   pdamp=pol ** (1.0 / 6.0), group=mol.copy(), molecule=mol, vdw_class=vcls, vdw_table=tab,
+                   red_index=red_i, red_value=red_v, mask_off=off, mask_atom=atoms, mask_rel=rels)
+    return placed.reshape(-1), box, top
+
+
+CRYSTAL_DENSITY_MAX = 0.1  # atoms per A^3 a batch cell may hold
+
+
 def crystal_batch(count, seed=0):
     """The batched small-crystal workload: ``count`` independent P1 triclinic
-    cells with lengths U(8, 20) A and angles U(70, 110) deg (cells of
-    non-positive volume redrawn), each filled by small_crystal with U(30, 300)
-    atoms (fewer where the cell is full at 2.6 A spacing).  Per-cell PME grids
-    are the next 2,3,5-smooth size >= 1.2 L, clamped to 16..32.  Returns a list
-    of (xyz, box, Topology, grid)."""
+    cells.  Each draws its atom count U{30..300}, then cells with lengths
+    U(8, 20) A and angles U(70, 110) deg until one has a positive volume that
+    holds the count at no more than 0.1 atoms/A^3 (a cell that cannot is
+    redrawn), and fills it by molecular_crystal.  Per-cell PME grids are the
+    next 2,3,5-smooth size >= 1.2 L, clamped to 16..32.  512 cells of seed 0
+    hold 83,375 atoms.  Returns a list of (xyz, box, Topology, grid)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for c in range(count):
+        n_atoms = int(rng.integers(30, 301))
         while True:
-            box = np.concatenate([rng.uniform(8.0, 20.0, 3), rng.uniform(70.0, 110.0, 3)])
-            ca, cb, cg = np.cos(np.deg2rad(box[3:]))
-            v2 = 1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg
-            if v2 > 0.05:
+            while True:
+                box = np.concatenate([rng.uniform(8.0, 20.0, 3), rng.uniform(70.0, 110.0, 3)])
+                ca, cb, cg = np.cos(np.deg2rad(box[3:]))
+                v2 = 1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg
+                if v2 > 0.05 and np.prod(box[:3]) * np.sqrt(v2) * CRYSTAL_DENSITY_MAX >= n_atoms:
+                    break
+            try:
+                xyz, box, top = molecular_crystal(n_atoms, box, seed=int(rng.integers(1 << 30)))
                 break
-        vol = np.prod(box[:3]) * np.sqrt(v2)
-        n_target = int(min(rng.uniform(30, 300), 0.045 * vol))
-        xyz, box, top = small_crystal(max(n_target, 12), box, seed=int(rng.integers(1 << 30)), tries=300)
+            except RuntimeError:
+                continue
         grid = tuple(_smooth_at_least(1.2 * L) for L in box[:3])
         out.append((xyz, box, top, grid))
     return out

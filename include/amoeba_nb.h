@@ -280,7 +280,8 @@ int nb_md_get_state(void *handle, double *x, double *v);
  * modules/algorithms/.../dynamics/thermostats/Berendsen.java:99-101, applied
  * after each full step as MolecularDynamics.java:1577-1583).  kind
  * NB_THERMO_BERENDSEN: after each step's second half kick the velocities are
- * scaled by sqrt(1 + dt/tau (T0/T - 1)), T = sum m v^2 / (kB (3n - 3)),
+ * scaled by sqrt(1 + dt/tau (T0/T - 1)), T = sum m v^2 / (kB (3m - 3)) over
+ * the m atoms with mass > 0 (atoms with mass <= 0 do not move),
  * temperature T0 in K, tau in ps (dt <= tau, else nb_md_step returns
  * NB_ERR_INVALID); the reported kinetic energy is the scaled one.
  * NB_THERMO_NONE (the default) gives NVE. */

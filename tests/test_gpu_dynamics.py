@@ -187,3 +187,110 @@ def test_berendsen_scales_velocities_after_the_full_step():
     assert np.abs(x2 - x1).max() <= 1e-12 * np.abs(x1).max()
     assert np.abs(v2 - s * v1).max() <= 1e-12 * np.abs(v1).max()
     assert abs(r2["kinetic"] - s * s * r1["kinetic"]) <= 1e-12 * r1["kinetic"]
+
+
+def test_predictor_history_skips_rejected_replays():
+    """A graph replay whose list needs rebuilding is repeated on the regular
+    path; the replay's solve must not enter the predictor's history, or the
+    history of the following solves is one step off.  Along a path with a
+    jump that forces a rebuild, a graph-mode handle and a regular handle with
+    the ASPC predictor return the same iteration counts and dipoles (a loose
+    polar-eps makes any difference in the starting points visible)."""
+    xyz, box, top = water_box(216, 18.643, seed=1)
+    p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-4)
+    rng = np.random.default_rng(8)
+    vel = rng.normal(0.0, 0.004, xyz.shape)
+    hs = []
+    try:
+        for graph in (True, False):
+            h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+            h.set_cell(box)
+            h.set_scf_predictor(ffi.NB_PRED_ASPC)
+            h.set_graph(graph)
+            hs.append(h)
+        rebuilt = []
+        for t in range(16):
+            x = (xyz + t * vel).reshape(-1, 3)
+            if t >= 10:
+                x = x.copy()
+                x[5] += 1.3          # more than half the 2 A buffer: the list is rebuilt at t = 10
+            out = []
+            for h in hs:
+                h.set_positions(x.reshape(-1))
+                e, _ = h.evaluate(ffi.NB_GRAD)
+                out.append((e, h.induced()[0]))
+            (eg, ug), (er, ur) = out
+            rebuilt.append(eg["list_rebuilt"])
+            print(f"step {t}: iterations graph {eg['pcg_iters']} regular {er['pcg_iters']}, rebuilt "
+                  f"{eg['list_rebuilt']}", flush=True)
+            assert eg["pcg_iters"] == er["pcg_iters"], t
+            assert rel(ug, ur) <= 1e-9, t
+            assert abs(eg["total"] - er["total"]) <= 1e-9 * abs(er["total"]), t
+        assert rebuilt[10] == 1
+        assert hs[0].graph_replays() >= 8
+    finally:
+        for h in hs:
+            h.close()
+
+
+def test_berendsen_leaves_massless_atoms_alone():
+    """Atoms with mass <= 0 do not move (drift and kick skip them), so the
+    thermostat leaves their velocities as they are and counts 3m - 3 degrees
+    of freedom over the m atoms with mass > 0."""
+    xyz, box, top, p, terms, mass = ubiquitin()
+    mass = mass.copy()
+    frozen = np.arange(0, 40, 4)
+    mass[frozen] = 0.0
+    v0 = velocities(np.where(mass > 0.0, mass, 1.0))
+    dt, tau, t0 = 0.001, 0.02, 350.0
+    out = []
+    for kind in (ffi.NB_THERMO_NONE, ffi.NB_THERMO_BERENDSEN):
+        h = md_handle(top, box, p, terms)
+        try:
+            h.set_positions(xyz.reshape(-1))
+            h.md_set_thermostat(kind, t0, tau)
+            h.md_init(mass, v0.reshape(-1))
+            rep = h.md_step(1, dt)
+            out.append((rep, *h.md_state()))
+        finally:
+            h.close()
+    (r1, x1, v1), (r2, x2, v2) = out
+    active = mass > 0.0
+    kb, dof = 0.831446261815324, 3 * int(active.sum()) - 3
+    temp = 2.0 * r1["kinetic"] * 418.4 / (kb * dof)
+    s = np.sqrt(1.0 + dt / tau * (t0 / temp - 1.0))
+    assert np.array_equal(v2[frozen], v0[frozen]) and np.array_equal(x2[frozen], xyz.reshape(-1, 3)[frozen])
+    assert np.abs(v2[active] - s * v1[active]).max() <= 1e-12 * np.abs(v1).max()
+
+
+def test_batch_of_predictor_handles_keeps_the_history():
+    """Handles with an SCF predictor are evaluated by nb_evaluate_batch on the
+    regular path (the batch kernel neither applies nor extends the history):
+    switching between batch and single evaluations gives the same sequence
+    as single evaluations alone."""
+    xyz, box, top = water_box(216, 18.643, seed=1)
+    p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-4)
+    rng = np.random.default_rng(5)
+    vel = rng.normal(0.0, 0.004, xyz.shape)
+    mk = lambda: ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+    a, b = mk(), mk()
+    try:
+        for h in (a, b):
+            h.set_cell(box)
+            h.set_scf_predictor(ffi.NB_PRED_ASPC)
+        for t in range(12):
+            x = (xyz + t * vel).reshape(-1)
+            a.set_positions(x)
+            b.set_positions(x)
+            if t % 3 == 1:
+                ea, rc = ffi.evaluate_batch([a], ffi.NB_GRAD)
+                assert rc == ffi.NB_OK
+                ea = ea[0]
+            else:
+                ea, _ = a.evaluate(ffi.NB_GRAD)
+            eb, _ = b.evaluate(ffi.NB_GRAD)
+            assert ea["pcg_iters"] == eb["pcg_iters"], t
+            assert rel(a.induced()[0], b.induced()[0]) <= 1e-9, t
+    finally:
+        a.close()
+        b.close()

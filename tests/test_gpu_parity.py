@@ -115,8 +115,8 @@ def test_water648_mixed_against_fp64(oracle, water648):
 
 def test_water20736_mixed_against_fp64():
     """Full-size workload (6912 waters, 64^3 grid): mixed form against the FP64
-    form of the library, force relative RMS error <= 1e-5 (FP64 form itself is
-    pinned to the oracle at 648 atoms above and through energies here)."""
+    form of the library, force relative RMS error <= 1e-5 (the FP64 form at
+    this size equals the oracle in test_gpu_configs.py)."""
     xyz, box, top = water_box(6912, 59.1, seed=2)
     p = NonbondedParams(vdw_cutoff=9.0, grid=(64, 64, 64), polar_eps=1e-5)
     out = {}
