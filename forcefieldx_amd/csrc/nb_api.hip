@@ -90,7 +90,7 @@ void free_all(NbHandle *h) {
                   h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                  h->d_scal, h->d_part, h->d_dip, h->d_vpart, h->d_vir, h->d_soft0, h->d_mol0, h->d_soft, h->d_mol, h->d_gred_l, h->d_dudx, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+                  h->d_scal, h->d_part, h->d_dip, h->d_dip2, h->d_vpart, h->d_vir, h->d_soft0, h->d_mol0, h->d_soft, h->d_mol, h->d_gred_l, h->d_dudx, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -102,6 +102,9 @@ void free_all(NbHandle *h) {
   if (h->ev_join) hipEventDestroy(h->ev_join);
   if (h->ev_eps) hipEventDestroy(h->ev_eps);
   if (h->stream2) hipStreamDestroy(h->stream2);
+  if (h->stream3) hipStreamDestroy(h->stream3);
+  if (h->ev_fork3) hipEventDestroy(h->ev_fork3);
+  if (h->ev_join3) hipEventDestroy(h->ev_join3);
   if (h->stream) hipStreamDestroy(h->stream);
 }
 
@@ -415,8 +418,11 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     HIPCHECK(hipSetDevice(h->device));
     HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHECK(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&h->stream3, hipStreamNonBlocking));
     HIPCHECK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_fork3, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&h->ev_join3, hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&h->ev_eps, hipEventDisableTiming));
     h->cur = h->stream;
     {
@@ -535,11 +541,15 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_scal, 16);
     alloc(&h->d_vir, 16);
     alloc(&h->d_dip, 12 * (size_t)n);
-    h->part_stride = (3 * n + 255) / 256;
+    alloc(&h->d_dip2, 12 * (size_t)n);
+    // producers of PCG partial sums: vector kernels (3n threads), the gather
+    // (8 lanes per atom) and the field kernels (NB_LPA lanes per atom)
+    h->part_stride = (int)std::max({grid_for(3 * (long)n, 256).x, xcd_grid((long)n * 8, 256).x,
+                                    xcd_grid((long)n * 16, 256).x});
     alloc(&h->d_part, 16 * (size_t)h->part_stride);
     alloc(&h->d_energy, 16);
     alloc(&h->d_count, 4);
-    HIPCHECK(hipHostMalloc((void **)&h->h_pinned, sizeof(double) * 64, hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void **)&h->h_pinned, sizeof(double) * P_NSLOT, hipHostMallocDefault));
     memset(&h->last, 0, sizeof(h->last));
     if (params->elec_lambda && h->has_soft && params->do_pme) {
       if (!(params->perm_lambda_end > params->perm_lambda_start && params->pol_lambda_end > params->pol_lambda_start))
@@ -687,21 +697,23 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   }
   nb_zero_outputs(h);
   if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
-  // vdW and the valence terms run on the side stream, overlapping everything
+  // vdW and the valence terms run on the third stream, overlapping everything
   // up to the gradient assembly
-  if (h->prm.do_vdw || h->has_valence) {
-    nb_fork(h);
-    NbSide side(h);
+  const bool third = h->prm.do_vdw || h->has_valence;
+  if (third) {
+    nb_fork3(h);
+    NbThird t3(h);
     if (h->prm.do_vdw) nb_vdw(h);
     nb_valence_terms(h);
   }
   bool polar = h->prm.polarization != NB_POL_NONE && h->ph.polar;
   if (h->prm.do_pme) {
-    nb_filter_lists(h);
-    // real-space permanent field (side) || permanent PME (main)
+    // Ewald / preconditioner / masked rows and the real-space permanent
+    // field (side) || permanent PME (main)
     nb_fork(h);
     {
       NbSide side(h);
+      nb_filter_lists(h);
       nb_perm_field(h);
     }
     if (use_recip) nb_pme_permanent(h);
@@ -752,11 +764,12 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
     if (use_recip) nb_recip_energy(h, polar);
   }
   nb_join(h);
+  if (third) nb_join3(h);
   if (h->finite_active) nb_finite_add(h);
   nb_finalize_gradient(h);
   if (h->has_valence) {
     nb_valence_gather(h);
-    HIPCHECK(hipMemcpyAsync(h->h_pinned + 54, h->d_val_energy, sizeof(double) * 8, hipMemcpyDeviceToHost,
+    HIPCHECK(hipMemcpyAsync(h->h_pinned + P_VALENCE, h->d_val_energy, sizeof(double) * 8, hipMemcpyDeviceToHost,
                             h->stream));
   }
   if (h->want_vir) {
@@ -932,7 +945,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     e.total = e.perm + e.pol + e.vdw;
     if (h->has_valence)
       for (int k = 0; k < 8; k++) {
-        e.valence[k] = h->h_pinned[54 + k];
+        e.valence[k] = h->h_pinned[P_VALENCE + k];
         e.total += e.valence[k];
       }
     if (status == NB_OK && fl[4]) status = NB_ERR_BAD_TORQUE;

@@ -222,8 +222,8 @@ __device__ __forceinline__ T group8(T v) {
 // transform (out[30 s ..]), otherwise only the Cartesian field gradient
 // (out[30 s + 21 ..]).  NP = 2 reads u from the real and uCR from the
 // imaginary part.  Every lane of the 8 must call (cross-lane sums).
-template <typename T, int NP, bool FULL, bool PREFETCH = false>
-__device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const double *__restrict__ theta,
+template <typename T, int NP, bool FULL, bool PREFETCH = false, typename TH = double>
+__device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const TH *__restrict__ theta,
                                             const int32_t *__restrict__ igrid, const CX<T> *__restrict__ grid,
                                             const Frac &fr, int K0, int K1, int K2, double *__restrict__ out0,
                                             double *__restrict__ out1) {
@@ -234,7 +234,7 @@ __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const d
 #pragma unroll
     for (int c = 0; c < NC; c++) phi[p][c] = (T)0;
   if (valid && lane < 5) {
-    const double *tx = theta + (size_t)s * 60, *ty = tx + 20, *vd = tx + 40 + 4 * lane;
+    const TH *tx = theta + (size_t)s * 60, *ty = tx + 20, *vd = tx + 40 + 4 * lane;
     T v[4] = {(T)vd[0], (T)vd[1], (T)vd[2], (T)vd[3]};
     T wx[5][4];
 #pragma unroll
@@ -280,7 +280,7 @@ __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const d
           if (FULL) { t[p][2] += q * wx[kx][2]; t[p][3] += q * wx[kx][3]; }
         }
       }
-      const double *ud = ty + 4 * ky;
+      const TH *ud = ty + 4 * ky;
       T uu[4] = {(T)ud[0], (T)ud[1], (T)ud[2], (T)ud[3]};
 #pragma unroll
       for (int p = 0; p < NP; p++) {

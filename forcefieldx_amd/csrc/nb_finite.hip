@@ -32,14 +32,12 @@ namespace {
 
 constexpr int FIN_LPA = 16;
 
-// totals of the r.z and p.q partial sums of k_pcg_ap (slots S_RZ, S_PAP)
-__global__ void k_tape_scal(const double *__restrict__ part, int stride, int nb, double *__restrict__ out) {
+// totals of the r.z and p.q partial sums of an iteration (slots S_RZN, S_PAP)
+__global__ void k_tape_scal(const double *__restrict__ part, int stride, int nrz, int npq, double *__restrict__ out) {
   __shared__ double red[2][4];
   double v[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    v[0] += part[(size_t)S_RZ * stride + i];
-    v[1] += part[(size_t)S_PAP * stride + i];
-  }
+  for (int i = threadIdx.x; i < nrz; i += blockDim.x) v[0] += part[(size_t)S_RZN * stride + i];
+  for (int i = threadIdx.x; i < npq; i += blockDim.x) v[1] += part[(size_t)S_PAP * stride + i];
   for (int k = 0; k < 2; k++) v[k] = wave_sum_d(v[k]);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) { red[0][w] = v[0]; red[1][w] = v[1]; }
@@ -224,8 +222,9 @@ static void ensure(double **p, size_t &cap, size_t need) {
   cap = need;
 }
 
-// record iteration k of the u system (called after k_pcg_ap, before the update)
-void nb_tape_push(NbHandle *h) {
+// record iteration k of the u system (called after the conjugation, before
+// the update); nrz / npq: partial sums of r.z and p.q
+void nb_tape_push(NbHandle *h, int nrz, int npq) {
   size_t n3 = 3 * (size_t)h->n;
   int k = h->tape_K;
   if (k + 1 > h->tape_cap) {
@@ -250,8 +249,7 @@ void nb_tape_push(NbHandle *h) {
   const double *src[4] = {h->d_r, h->d_z, h->d_p, h->d_vec};
   for (int v = 0; v < 4; v++)
     HIPCHECK(hipMemcpyAsync(slot + v * n3, src[v], sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
-  int nb = (int)grid_for(3 * h->n, 256).x;
-  NB_LAUNCH(k_tape_scal, dim3(1), dim3(256), 0, h->d_part, h->part_stride, nb, h->d_tape_sc + 2 * k);
+  NB_LAUNCH(k_tape_scal, dim3(1), dim3(256), 0, h->d_part, h->part_stride, nrz, npq, h->d_tape_sc + 2 * k);
   h->tape_K = k + 1;
 }
 

@@ -9,8 +9,14 @@ struct ScaleTab { double m[16], p[16], d[16], v[16]; int masked[16]; };
 enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL_RECIP, E_VDW, E_QTOT, E_DUDL,
        E_D2UDL2, E_NSLOT };
 // PCG scalar slots of NbHandle::d_scal
-// PCG scalars; S_DONE .. S_EPSV are the device-side control of a recorded solve
-enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_DONE, S_ITER, S_PREV, S_STAT, S_EPSV, S_NSLOT };
+// PCG scalars; S_DONE .. S_EPSV are the device-side control of a recorded solve;
+// S_RZOLD / S_RZOLDP hold the r.z totals of the last update (the denominators
+// of the next conjugation)
+enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_DONE, S_ITER, S_PREV, S_STAT, S_EPSV,
+       S_RZOLD, S_RZOLDP, S_NSLOT };
+// pinned host slots of an evaluation's read-back (doubles)
+enum { P_ENERGY = 0, P_COUNT = 8, P_FLAGS = 12, P_VIR = 16, P_DUDL = 32, P_SCAL = 40, P_VALENCE = 64, P_NSLOT = 80 };
+static_assert(P_SCAL + S_NSLOT <= P_VALENCE, "PCG scalars overlap the valence energies in the read-back");
 
 static inline ScaleTab scale_tab(const NbHandle *h) {
   ScaleTab t;
@@ -31,6 +37,9 @@ static inline ScaleTab scale_tab(const NbHandle *h) {
 }
 
 static inline dim3 grid_for(long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
+
+// lanes per atom of the real-space pair kernels
+#define NB_LPA 16
 
 // Workgroups are dispatched to the 8 XCDs of the device round-robin, and each
 // XCD has its own L2.  xcd_block() gives XCD x the contiguous logical blocks
@@ -100,6 +109,99 @@ static __device__ __forceinline__ void block_store9(double (&w)[9], double *part
     double t = 0.0;
     for (int i = 0; i < nw; i++) t += red9[threadIdx.x][i];
     part[9 * (size_t)blockIdx.x + threadIdx.x] = t;
+  }
+}
+
+// block sums of NV values -> part[slot_k * stride + blockIdx.x]
+template <int NV>
+static __device__ __forceinline__ void block_partials(double (&v)[NV], double *part, const int (&slot)[NV], int stride) {
+  __shared__ double red[NV][4];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = wave_sum_d(v[k]);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    int k = threadIdx.x;
+    part[(size_t)slot[k] * stride + blockIdx.x] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+  }
+}
+
+// totals of NV partial rows (each of nb entries); every thread receives them
+template <int NV>
+static __device__ __forceinline__ void block_totals(const double *part, const int (&slot)[NV], int stride, int nb,
+                                             double (&tot)[NV]) {
+  __shared__ double red[NV][4];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    v[k] = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v[k] += part[(size_t)slot[k] * stride + i];
+    v[k] = wave_sum_d(v[k]);
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k][w] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) tot[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+}
+
+// The PCG's conjugate direction and its operator image, formed from the field
+// of the preconditioned residual z instead of the field of p (the operator is
+// linear): p = z + beta p and q = A p = A z + beta q, with A z = z / polar -
+// E(z); beta = 0 on the first iteration (PCGSolver.java:991-1178 forms E(p)
+// directly; the two agree to rounding).
+static __device__ __forceinline__ void pcg_direction(double a, double zq, double ez, double beta, int first,
+                                                     double &p, double &q) {
+  const double az = a > 0.0 ? zq / a - ez : 0.0;
+  if (first) {
+    p = zq;
+    q = az;
+  } else {
+    p = zq + beta * p;
+    q = az + beta * q;
+  }
+}
+
+// beta of the two systems: this iteration's r.z totals over the last
+// update's (a zero denominator counts as 1, as the reference's guard)
+static __device__ __forceinline__ void pcg_beta(const double *__restrict__ part, int stride, int nrz,
+                                                const double *__restrict__ scal, int first, double (&beta)[2]) {
+  beta[0] = beta[1] = 0.0;
+  if (first) return;
+  const int in[2] = {S_RZN, S_RZPN};
+  double t[2];
+  block_totals<2>(part, in, stride, nrz, t);
+  double d0 = scal[S_RZOLD], d1 = scal[S_RZOLDP];
+  beta[0] = t[0] / (d0 == 0.0 ? 1.0 : d0);
+  beta[1] = t[1] / (d1 == 0.0 ? 1.0 : d1);
+}
+
+// The convergence test of PCGSolver.java:543-547 on the residual totals t:
+// eps = 4.80321 sqrt(max(r.r, rCR.rCR) / n); a rise of eps ends the solve as
+// diverged, reaching maxit as out of iterations, eps < polar_eps as converged;
+// S_DONE then stops the following iterations of a recorded solve.
+static __device__ __forceinline__ void pcg_control(double *__restrict__ scal, const double (&t)[2], int n, double polar_eps,
+                                            int maxit) {
+  if (scal[S_DONE] != 0.0) return;
+  double e = t[0] > t[1] ? t[0] : t[1];
+  double eps = 4.80321 * sqrt(e / n), prev = scal[S_PREV];
+  double iter = scal[S_ITER] + 1.0;
+  scal[S_ITER] = iter;
+  scal[S_EPSV] = eps;
+  scal[S_PREV] = eps;
+  if (!(eps <= prev)) {
+    scal[S_STAT] = (double)NB_ERR_SCF_DIVERGED;
+    scal[S_DONE] = 1.0;
+  } else if (iter >= (double)maxit) {
+    scal[S_STAT] = (double)NB_ERR_SCF_MAXITER;
+    scal[S_DONE] = 1.0;
+  } else if (eps < polar_eps) {
+    scal[S_DONE] = 1.0;
   }
 }
 

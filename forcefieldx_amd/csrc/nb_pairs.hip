@@ -23,8 +23,6 @@
 #include "nb_kern.h"
 #include "nb_pair.h"
 
-#define NB_LPA 16
-
 namespace {
 
 template <int LPA>
@@ -191,6 +189,77 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
   }
 }
 
+
+// The preconditioner field of polar * r (records recA) fused with the
+// preconditioned residual z = polar (E_pre + scale r) (PCGSolver.java:
+// 1183-1571, the z of UpdateConjugate): lane 0 of each atom writes z, its
+// record in recB (the dipoles whose Ewald field comes next) and the partial
+// r.z into S_RZN / S_RZPN.
+template <typename T, int LPA>
+__global__ void __launch_bounds__(256) k_precond_z(int n, const DipRec<T> *__restrict__ rec,
+                                                   const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
+                                                   int maxl, CellD cell, const double *__restrict__ polar, double psc,
+                                                   const double *__restrict__ r, const double *__restrict__ rp,
+                                                   double *__restrict__ z, double *__restrict__ zp, void *recb,
+                                                   int mixed, double *__restrict__ part, int stride, int neps,
+                                                   double *__restrict__ scal, int control, double polar_eps,
+                                                   int maxit, const double *__restrict__ ctl) {
+  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
+  int s = gt / LPA, lane = gt % LPA;
+  bool valid = s < n;
+  V3<double> a{0, 0, 0}, b{0, 0, 0};
+  if (valid) {
+    const DipRec<T> ri = rec[s];
+    int m = cnt[s];
+    const uint32_t *row = lst + (size_t)s * maxl;
+    if constexpr (sizeof(T) == 4) {
+      for (int t = lane; t < m; t += 2 * LPA) {
+        const bool two = t + LPA < m;
+        const uint32_t e0 = row[t], e1 = two ? row[t + LPA] : e0;
+        const DipRec<T> r0 = rec[e0 & cell.kmask], r1 = rec[e1 & cell.kmask];
+        dip_field_pair<T, true>(ri, r0, e0, cell, 0.0, a, b);
+        if (two) dip_field_pair<T, true>(ri, r1, e1, cell, 0.0, a, b);
+      }
+    } else {
+      for (int t = lane; t < m; t += LPA) {
+        uint32_t e = row[t];
+        dip_field_pair<T, true>(ri, rec[e & cell.kmask], e, cell, 0.0, a, b);
+      }
+    }
+  }
+  group_sum3<LPA>(a);
+  group_sum3<LPA>(b);
+  double v[2] = {0.0, 0.0};
+  if (valid && lane == 0) {
+    const double al = polar[s];
+    const double fa[3] = {a.x, a.y, a.z}, fb[3] = {b.x, b.y, b.z};
+    for (int d = 0; d < 3; d++) {
+      const size_t q = 3 * (size_t)s + d;
+      const double zz = al * (fa[d] + psc * r[q]), zzp = al * (fb[d] + psc * rp[q]);
+      z[q] = zz;
+      zp[q] = zzp;
+      dip_put(recb, mixed, s, d, zz, zzp);
+      v[0] += r[q] * zz;
+      v[1] += rp[q] * zzp;
+    }
+  }
+  const int sl[2] = {S_RZN, S_RZPN};
+  block_partials<2>(v, part, sl, stride);
+  // block 0 reduces the residual norms of the update before it (neps partial
+  // sums) and, for a recorded solve, applies the convergence test; the solve
+  // stops after this kernel, whose z is then not used
+  if (blockIdx.x == 0 && neps > 0) {
+    const int in[2] = {S_EPS, S_EPSP};
+    double t[2];
+    block_totals<2>(part, in, stride, neps, t);
+    if (threadIdx.x == 0) {
+      scal[S_EPS] = t[0];
+      scal[S_EPSP] = t[1];
+      if (control) pcg_control(scal, t, n, polar_eps, maxit);
+    }
+  }
+}
 
 // Permanent pair under an OST phase.  A soft pair sees r_eff = sqrt(r^2 +
 // lalpha) in the radial functions and the unshifted r in the angular ones, and
@@ -578,6 +647,28 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
     NB_LAUNCH((k_dip_field<double, NB_LPA, true>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
   else
     NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+}
+
+// z of the PCG from the preconditioner field of the records d_dip2 (polar *
+// r); z goes to d_z / d_zp and the records d_dip; neps > 0: also the
+// residual norms of the update before (its partial sums), with the
+// convergence test when control is set.  Returns the number of partial sums
+// of r.z.
+int nb_precond_z(NbHandle *h, int neps, bool control) {
+  int n = h->n;
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
+  int mixed = h->prm.precision == NB_PREC_MIXED ? 1 : 0;
+  double psc = h->prm.precond_scale;
+  int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
+  if (mixed)
+    NB_LAUNCH((k_precond_z<float, NB_LPA>), G, B, 0, n, (const DipRec<float> *)h->d_dip2, h->d_pc, h->d_npc,
+              h->maxpc, h->cell, h->d_polar, psc, h->d_r, h->d_rp, h->d_z, h->d_zp, (void *)h->d_dip, mixed,
+              h->d_part, h->part_stride, neps, h->d_scal, (int)control, h->prm.polar_eps, maxit, h->ctl);
+  else
+    NB_LAUNCH((k_precond_z<double, NB_LPA>), G, B, 0, n, (const DipRec<double> *)h->d_dip2, h->d_pc, h->d_npc,
+              h->maxpc, h->cell, h->d_polar, psc, h->d_r, h->d_rp, h->d_z, h->d_zp, (void *)h->d_dip, mixed,
+              h->d_part, h->part_stride, neps, h->d_scal, (int)control, h->prm.polar_eps, maxit, h->ctl);
+  return (int)G.x;
 }
 
 #define NB_LPA_SP 4

@@ -89,9 +89,12 @@ __host__ __device__ __forceinline__ int brick_of(int p, int K, int nb) { return 
 __host__ __device__ __forceinline__ int brick_start(int b, int K, int nb) { return (b * K) / nb; }
 
 
-// B-splines, grid start, fractional multipoles and brick binning
+// B-splines, grid start, fractional multipoles and brick binning.  The
+// weights are stored in the grid's real type TH: the spread and the gather
+// compute in that type.
+template <typename TH>
 __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const double *__restrict__ gmp, CellD cell,
-                               Bricks bk, Frac fr, double *__restrict__ theta, int32_t *__restrict__ igrid,
+                               Bricks bk, Frac fr, TH *__restrict__ theta, int32_t *__restrict__ igrid,
                                double *__restrict__ fmp, int32_t *__restrict__ bcount, int32_t *__restrict__ bid,
                                int32_t *__restrict__ brank) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,9 +111,9 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
     b3[d] = brick_of(wrapg(ifr - 4, bk.K[d]), bk.K[d], bk.nb[d]);
     double th[5][4];
     bspline4<5>(bx, th);
-    double *o = theta + ((size_t)s * 3 + d) * 20;
+    TH *o = theta + ((size_t)s * 3 + d) * 20;
     for (int k = 0; k < 5; k++)
-      for (int j = 0; j < 4; j++) o[4 * k + j] = th[k][j];
+      for (int j = 0; j < 4; j++) o[4 * k + j] = (TH)th[k][j];
   }
   int b = b3[0] + bk.nb[0] * (b3[1] + bk.nb[1] * b3[2]);
   bid[s] = b;
@@ -171,7 +174,7 @@ __device__ __forceinline__ void brick_local(const Bricks &bk, int b, const int32
 template <typename T, bool IND>
 __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const int32_t *__restrict__ bstart,
                                                       const int32_t *__restrict__ batom,
-                                                      const double *__restrict__ theta,
+                                                      const T *__restrict__ theta,
                                                       const int32_t *__restrict__ igrid,
                                                       const double *__restrict__ fmp, const double *__restrict__ u3,
                                                       const double *__restrict__ up3, Frac fr, T *__restrict__ bbuf,
@@ -191,17 +194,17 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
     int s = batom[a0 + a];
     int o[3];
     brick_local(bk, b, igrid + 3 * s, o);
-    const double *tx = theta + (size_t)s * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
+    const T *tx = theta + (size_t)s * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
     int base = ((o[2] + kz) * e + (o[1] + ky)) * e + o[0];
     // every global load is issued before the first LDS atomic: the compiler
     // does not move loads across atomics, so interleaving them serialises
     // five load latencies per thread
-    double wx[5][3];
+    T wx[5][3];
 #pragma unroll
     for (int kx = 0; kx < 5; kx++) {
       wx[kx][0] = tx[4 * kx];
       wx[kx][1] = tx[4 * kx + 1];
-      wx[kx][2] = IND ? 0.0 : tx[4 * kx + 2];
+      wx[kx][2] = IND ? (T)0 : tx[4 * kx + 2];
     }
     T val[NP][5];
     if (!IND) {
@@ -290,14 +293,58 @@ __global__ void __launch_bounds__(256) k_brick_combine(Bricks bk, const T *__res
 // first derivatives are formed and the Cartesian field components are
 // written to out[30 s + 21..23] (what the PCG iterations consume).
 template <typename T, int NP, bool FULL>
-__global__ void __launch_bounds__(256) k_gather(int n, const double *__restrict__ theta, const int32_t *__restrict__ igrid,
+__global__ void __launch_bounds__(256) k_gather(int n, const T *__restrict__ theta, const int32_t *__restrict__ igrid,
                                                 const CX<T> *__restrict__ grid, Frac fr, int K0, int K1, int K2,
                                                 double *__restrict__ out0, double *__restrict__ out1,
                                                 const double *__restrict__ ctl) {
   if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
-  gather_atom<T, NP, FULL>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, out0, out1);
+  gather_atom<T, NP, FULL, false, T>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, out0, out1);
+}
+
+// The gather of a PCG iteration fused with the conjugation: after the field
+// of z is gathered (the real-space part f is complete when this runs), lane 0
+// of each atom forms E(z) = f + a3 z - phi, A z = z / polar - E(z), and the
+// new direction p = z + beta p with its image q = A z + beta q
+// (pcg_direction); partial p.q into S_PAP / S_PAPP.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_ap(int n, const T *__restrict__ theta,
+                                                   const int32_t *__restrict__ igrid, const CX<T> *__restrict__ grid,
+                                                   Frac fr, int K0, int K1, int K2, double *__restrict__ phiu,
+                                                   double *__restrict__ phiup, double a3,
+                                                   const double *__restrict__ polar, const double *__restrict__ z,
+                                                   const double *__restrict__ zp, const double *__restrict__ f,
+                                                   const double *__restrict__ fp, double *__restrict__ p,
+                                                   double *__restrict__ pp, double *__restrict__ vec,
+                                                   double *__restrict__ vecp, double *__restrict__ part, int stride,
+                                                   int nrz, const double *__restrict__ scal, int first,
+                                                   const double *__restrict__ ctl) {
+  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  double beta[2];
+  pcg_beta(part, stride, nrz, scal, first, beta);
+  int gt = xcd_block() * blockDim.x + threadIdx.x;
+  int s = gt >> 3, lane = gt & 7;
+  gather_atom<T, 2, false, false, T>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, phiu, phiup);
+  double v[2] = {0.0, 0.0};
+  if (s < n && lane == 0) {
+    const double a = polar[s];
+    for (int d = 0; d < 3; d++) {
+      const size_t q = 3 * (size_t)s + d;
+      double pn = p[q], qn = vec[q], pnp = pp[q], qnp = vecp[q];
+      pcg_direction(a, z[q], f[q] + a3 * z[q] - phiu[30 * (size_t)s + 21 + d], beta[0], first, pn, qn);
+      pcg_direction(a, zp[q], fp[q] + a3 * zp[q] - phiup[30 * (size_t)s + 21 + d], beta[1], first, pnp, qnp);
+      p[q] = pn;
+      vec[q] = qn;
+      pp[q] = pnp;
+      vecp[q] = qnp;
+      v[0] += pn * qn;
+      v[1] += pnp * qnp;
+    }
+  }
+  __syncthreads();
+  const int sl[2] = {S_PAP, S_PAPP};
+  block_partials<2>(v, part, sl, stride);
 }
 
 // lines along one axis, L lines per workgroup (fft_chunk of chunk blockIdx.x)
@@ -841,12 +888,12 @@ static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
   Bricks bk = bricks_of(h);
   size_t e3 = (size_t)bk.ext * bk.ext * bk.ext;
   NB_LAUNCH((k_spread_brick<T, false>), dim3(h->nbricks), dim3(256), sizeof(double) * e3, h->n, bk, h->d_bstart,
-            h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, (const double *)nullptr, (const double *)nullptr, fr,
+            h->d_batom, (const T *)h->d_theta, h->d_igrid, h->d_fmp, (const double *)nullptr, (const double *)nullptr, fr,
             (T *)h->d_bbuf, (const double *)nullptr);
   forward_t<T>(h, 1, fpg, tw);
   if (polar) {
     NB_LAUNCH((k_spread_brick<T, true>), dim3(h->nbricks), dim3(256), sizeof(double) * 2 * e3, h->n, bk, h->d_bstart,
-              h->d_batom, h->d_theta, h->d_igrid, h->d_fmp, h->d_u, h->d_up, fr, (T *)h->d_bbuf,
+              h->d_batom, (const T *)h->d_theta, h->d_igrid, h->d_fmp, h->d_u, h->d_up, fr, (T *)h->d_bbuf,
               (const double *)nullptr);
     forward_t<T>(h, 2, g, tw);
   }
@@ -877,13 +924,13 @@ static void spread_t(NbHandle *h, const double *u, const double *up) {
   size_t e3 = (size_t)bk.ext * bk.ext * bk.ext;
   Frac fr = frac_of(h);
   NB_LAUNCH((k_spread_brick<T, IND>), dim3(h->nbricks), dim3(256), sizeof(double) * NP * e3, h->n, bk, h->d_bstart, h->d_batom,
-            h->d_theta, h->d_igrid, h->d_fmp, u, up, fr, (T *)h->d_bbuf, h->ctl);
+            (const T *)h->d_theta, h->d_igrid, h->d_fmp, u, up, fr, (T *)h->d_bbuf, h->ctl);
 }
 
 template <typename T, int NP, bool FULL>
 static void gather_t(NbHandle *h, double *o0, double *o1) {
   Frac fr = frac_of(h);
-  NB_LAUNCH((k_gather<T, NP, FULL>), xcd_grid((long)h->n * 8, 256), dim3(256), 0, h->n, h->d_theta, h->d_igrid,
+  NB_LAUNCH((k_gather<T, NP, FULL>), xcd_grid((long)h->n * 8, 256), dim3(256), 0, h->n, (const T *)h->d_theta, h->d_igrid,
             (const CX<T> *)h->d_grid, fr, h->K[0], h->K[1], h->K[2], o0, o1, h->ctl);
 }
 
@@ -892,8 +939,12 @@ void nb_pme_permanent(NbHandle *h) {
   dim3 B(128), G((n + 127) / 128);
   Frac fr = frac_of(h);
   NB_LAUNCH(k_zero_i, grid_for(h->nbricks, 256), dim3(256), 0, h->nbricks, h->d_bcount);
-  NB_LAUNCH(k_bspline_frac, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, bricks_of(h), fr, h->d_theta, h->d_igrid,
-            h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
+  if (mixed_grid(h))
+    NB_LAUNCH(k_bspline_frac<float>, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, bricks_of(h), fr, (float *)h->d_theta,
+              h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
+  else
+    NB_LAUNCH(k_bspline_frac<double>, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, bricks_of(h), fr, h->d_theta,
+              h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
   NB_LAUNCH(k_brick_scan, dim3(1), dim3(1024), 0, h->nbricks, h->d_bcount, h->d_bstart);
   NB_LAUNCH(k_brick_fill, grid_for(n, 256), dim3(256), 0, n, h->d_bid, h->d_brank, h->d_bstart, h->d_batom);
   if (mixed_grid(h)) {
@@ -919,6 +970,27 @@ void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full) {
     if (full) gather_t<double, 2, true>(h, h->d_phiu, h->d_phiup);
     else gather_t<double, 2, false>(h, h->d_phiu, h->d_phiup);
   }
+}
+
+// PME of the PCG iteration's z (spread, convolution), then the join with the
+// side stream's real-space field of z, then the gather fused with the
+// conjugation (k_gather_ap); returns the number of partial sums of p.q.
+int nb_pme_pcg_direction(NbHandle *h, double a3, int nrz, int first) {
+  Frac fr = frac_of(h);
+  dim3 G = xcd_grid((long)h->n * 8, 256);
+  auto gap = [&](auto tag) {
+    using T = decltype(tag);
+    spread_t<T, true>(h, h->d_z, h->d_zp);
+    convolution(h, 2);
+    nb_join(h);
+    NB_LAUNCH((k_gather_ap<T>), G, dim3(256), 0, h->n, (const T *)h->d_theta, h->d_igrid, (const CX<T> *)h->d_grid, fr,
+              h->K[0], h->K[1], h->K[2], h->d_phiu, h->d_phiup, a3, h->d_polar, h->d_z, h->d_zp, h->d_field,
+              h->d_fieldp, h->d_p, h->d_pp, h->d_vec, h->d_vecp, h->d_part, h->part_stride, nrz, h->d_scal, first,
+              h->ctl);
+  };
+  if (mixed_grid(h)) gap(0.0f);
+  else gap(0.0);
+  return (int)G.x;
 }
 
 void nb_rotate(NbHandle *h) {

@@ -29,8 +29,8 @@ namespace {
 
 
 __global__ void k_gather_pos(int n, const int32_t *__restrict__ perm, const double *__restrict__ xyz,
-                             const AtomMP *__restrict__ mp, double4 *__restrict__ pos, void *rec, int mixed,
-                             CellD cell, const int32_t *__restrict__ wrap) {
+                             const AtomMP *__restrict__ mp, double4 *__restrict__ pos, void *rec, void *rec2,
+                             int mixed, CellD cell, const int32_t *__restrict__ wrap) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int i = perm[s];
@@ -44,6 +44,7 @@ __global__ void k_gather_pos(int n, const int32_t *__restrict__ perm, const doub
   }
   pos[s] = make_double4(x, y, z, 0.0);
   dip_put_static(rec, mixed, s, x, y, z, mp[s].ipd, mp[s].thole);
+  dip_put_static(rec2, mixed, s, x, y, z, mp[s].ipd, mp[s].thole);
 }
 
 // self + reciprocal permanent field and the direct dipoles
@@ -91,44 +92,6 @@ __global__ void k_fill(int m, double *x, double v) {
   if (i < m) x[i] = v;
 }
 
-// block sums of NV values -> part[slot_k * stride + blockIdx.x]
-template <int NV>
-__device__ __forceinline__ void block_partials(double (&v)[NV], double *part, const int (&slot)[NV], int stride) {
-  __shared__ double red[NV][4];
-#pragma unroll
-  for (int k = 0; k < NV; k++) v[k] = wave_sum_d(v[k]);
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; k++) red[k][w] = v[k];
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    int k = threadIdx.x;
-    part[(size_t)slot[k] * stride + blockIdx.x] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
-  }
-}
-
-// totals of NV partial rows (each of nb entries); every thread receives them
-template <int NV>
-__device__ __forceinline__ void block_totals(const double *part, const int (&slot)[NV], int stride, int nb,
-                                             double (&tot)[NV]) {
-  __shared__ double red[NV][4];
-  double v[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    v[k] = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) v[k] += part[(size_t)slot[k] * stride + i];
-    v[k] = wave_sum_d(v[k]);
-  }
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < NV; k++) red[k][w] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) tot[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
-}
-
 // r0 = (u_direct - u0) / polar + E(u0); the packed records receive polar * r0
 __global__ void k_pcg_init_r(int n, const double *__restrict__ polar, const double *__restrict__ ud,
                              const double *__restrict__ upd, const double *__restrict__ u, const double *__restrict__ up,
@@ -147,83 +110,59 @@ __global__ void k_pcg_init_r(int n, const double *__restrict__ polar, const doub
   dip_put(rec, mixed, q / 3, q % 3, a * rr, a * rrp);
 }
 
-// z = polar (E_pre + scale r); optionally p = z; partial r.z into slots (slot, slot + 1)
-__global__ void __launch_bounds__(256) k_pcg_z(int n, const double *__restrict__ polar, double psc,
-                                               const double *__restrict__ f, const double *__restrict__ fp,
-                                               const double *__restrict__ r, const double *__restrict__ rp,
-                                               double *__restrict__ z, double *__restrict__ zp, double *__restrict__ p,
-                                               double *__restrict__ pp, int setp, double *__restrict__ part, int stride,
-                                               int slot, void *rec, int mixed) {
+// Conjugate direction without reciprocal space (aperiodic systems): E(z) is
+// the real-space field f alone; partial p.q into S_PAP / S_PAPP.
+__global__ void __launch_bounds__(256) k_pcg_dir(int n, const double *__restrict__ polar, const double *__restrict__ z,
+                                                 const double *__restrict__ zp, const double *__restrict__ f,
+                                                 const double *__restrict__ fp, double *__restrict__ p,
+                                                 double *__restrict__ pp, double *__restrict__ vec,
+                                                 double *__restrict__ vecp, double *__restrict__ part, int stride,
+                                                 int nrz, const double *__restrict__ scal, int first,
+                                                 const double *__restrict__ ctl) {
+  if (ctl && ctl[S_DONE] != 0.0) return;
+  double beta[2];
+  pcg_beta(part, stride, nrz, scal, first, beta);
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < 3 * n) {
     double a = polar[q / 3];
-    double zz = a * (f[q] + psc * r[q]), zzp = a * (fp[q] + psc * rp[q]);
-    z[q] = zz;
-    zp[q] = zzp;
-    if (setp) {
-      p[q] = zz;
-      pp[q] = zzp;
-      dip_put(rec, mixed, q / 3, q % 3, zz, zzp);
-    }
-    v[0] = r[q] * zz;
-    v[1] = rp[q] * zzp;
+    double pn = p[q], qn = vec[q], pnp = pp[q], qnp = vecp[q];
+    pcg_direction(a, z[q], f[q], beta[0], first, pn, qn);
+    pcg_direction(a, zp[q], fp[q], beta[1], first, pnp, qnp);
+    p[q] = pn;
+    vec[q] = qn;
+    pp[q] = pnp;
+    vecp[q] = qnp;
+    v[0] = pn * qn;
+    v[1] = pnp * qnp;
   }
-  const int sl[2] = {slot, slot + 1};
+  __syncthreads();
+  const int sl[2] = {S_PAP, S_PAPP};
   block_partials<2>(v, part, sl, stride);
 }
 
-// total induced field of p (real space f + self a3 p - reciprocal phi),
-// vec = A p = p / polar - E(p); partial p.Ap and r.z
-__global__ void __launch_bounds__(256) k_pcg_ap(int n, double a3, const double *__restrict__ polar,
-                                                const double *__restrict__ p, const double *__restrict__ pp,
-                                                const double *__restrict__ f, const double *__restrict__ fp,
-                                                const double *__restrict__ phiu, const double *__restrict__ phiup,
-                                                const double *__restrict__ r, const double *__restrict__ rp,
-                                                const double *__restrict__ z, const double *__restrict__ zp,
-                                                double *__restrict__ vec, double *__restrict__ vecp,
-                                                double *__restrict__ part, int stride) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  if (q < 3 * n) {
-    int s = q / 3, d = q - 3 * s;
-    double a = polar[s];
-    double ft = f[q], fpt = fp[q];
-    if (phiu) {
-      ft += a3 * p[q] - phiu[30 * (size_t)s + 21 + d];
-      fpt += a3 * pp[q] - phiup[30 * (size_t)s + 21 + d];
-    }
-    double w = 0.0, wp = 0.0;
-    if (a > 0.0) {
-      w = p[q] / a - ft;
-      wp = pp[q] / a - fpt;
-    }
-    vec[q] = w;
-    vecp[q] = wp;
-    v[0] = p[q] * w;
-    v[1] = pp[q] * wp;
-    v[2] = r[q] * z[q];
-    v[3] = rp[q] * zp[q];
-  }
-  const int sl[4] = {S_PAP, S_PAPP, S_RZ, S_RZP};
-  block_partials<4>(v, part, sl, stride);
-}
-
-// the packed records receive polar * r for the preconditioner field
-__global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restrict__ polar, const double *__restrict__ vec,
-                                                    const double *__restrict__ vecp, const double *__restrict__ p,
-                                                    const double *__restrict__ pp, double *__restrict__ r,
-                                                    double *__restrict__ rp, double *__restrict__ u,
-                                                    double *__restrict__ up, double *__restrict__ part, int stride,
-                                                    int nb, void *rec, int mixed, const double *__restrict__ ctl) {
+// r -= alpha q, u += alpha p with alpha = r.z / p.q per system; the records
+// of the preconditioner field receive polar * r; partial r.r into S_EPS /
+// S_EPSP (reduced by the next k_precond_z); block 0 keeps the r.z totals as
+// the denominators of the next conjugation.
+__global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restrict__ polar,
+                                                    const double *__restrict__ vec, const double *__restrict__ vecp,
+                                                    const double *__restrict__ p, const double *__restrict__ pp,
+                                                    double *__restrict__ r, double *__restrict__ rp,
+                                                    double *__restrict__ u, double *__restrict__ up,
+                                                    double *__restrict__ part, int stride, int npq, int nrz,
+                                                    void *rec, int mixed, double *__restrict__ scal,
+                                                    const double *__restrict__ ctl) {
   // a recorded solve that has stopped keeps its dipoles and residuals
   if (ctl && ctl[S_DONE] != 0.0) return;
-  const int in[4] = {S_PAP, S_PAPP, S_RZ, S_RZP};
-  double t[4];
-  block_totals<4>(part, in, stride, nb, t);
-  double al = t[2], alp = t[3];
-  if (t[0] != 0.0) al /= t[0];
-  if (t[1] != 0.0) alp /= t[1];
+  const int in_pq[2] = {S_PAP, S_PAPP}, in_rz[2] = {S_RZN, S_RZPN};
+  double pq[2], rz[2];
+  block_totals<2>(part, in_pq, stride, npq, pq);
+  __syncthreads();
+  block_totals<2>(part, in_rz, stride, nrz, rz);
+  double al = rz[0], alp = rz[1];
+  if (pq[0] != 0.0) al /= pq[0];
+  if (pq[1] != 0.0) alp /= pq[1];
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < 3 * n) {
@@ -237,42 +176,24 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
     v[0] = rr * rr;
     v[1] = rrp * rrp;
   }
-  __syncthreads();  // the LDS of block_totals is reused by block_partials
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[S_RZOLD] = rz[0];
+    scal[S_RZOLDP] = rz[1];
+  }
+  __syncthreads();
   const int out[2] = {S_EPS, S_EPSP};
   block_partials<2>(v, part, out, stride);
 }
 
-// residual norms for the convergence test: scal[S_EPS], scal[S_EPSP].  With
-// control set (a recorded solve), the test of the host loop runs here:
-// eps = 4.80321 sqrt(max(r.r, rCR.rCR) / n); a rise of eps ends the solve as
-// diverged, reaching maxit as out of iterations, eps < polar_eps as converged
-// (PCGSolver.java:543-547); S_DONE then stops the following iterations.
+// residual norms of an SOR cycle: scal[S_EPS], scal[S_EPSP]
 __global__ void __launch_bounds__(256) k_pcg_eps(const double *__restrict__ part, int stride, int nb,
-                                                 double *__restrict__ scal, int control, int n, double polar_eps,
-                                                 int maxit) {
+                                                 double *__restrict__ scal) {
   const int in[2] = {S_EPS, S_EPSP};
   double t[2];
   block_totals<2>(part, in, stride, nb, t);
   if (threadIdx.x == 0) {
     scal[S_EPS] = t[0];
     scal[S_EPSP] = t[1];
-    if (control && scal[S_DONE] == 0.0) {
-      double e = t[0] > t[1] ? t[0] : t[1];
-      double eps = 4.80321 * sqrt(e / n), prev = scal[S_PREV];
-      double iter = scal[S_ITER] + 1.0;
-      scal[S_ITER] = iter;
-      scal[S_EPSV] = eps;
-      scal[S_PREV] = eps;
-      if (!(eps <= prev)) {
-        scal[S_STAT] = (double)NB_ERR_SCF_DIVERGED;
-        scal[S_DONE] = 1.0;
-      } else if (iter >= (double)maxit) {
-        scal[S_STAT] = (double)NB_ERR_SCF_MAXITER;
-        scal[S_DONE] = 1.0;
-      } else if (eps < polar_eps) {
-        scal[S_DONE] = 1.0;
-      }
-    }
   }
 }
 
@@ -318,26 +239,6 @@ __global__ void k_reset_dipoles(int n, const double *__restrict__ ud, const doub
   u[q] = ud[q];
   up[q] = upd[q];
   dip_put(rec, mixed, q / 3, q % 3, ud[q], upd[q]);
-}
-
-// p = z + beta p; the packed records receive p
-__global__ void __launch_bounds__(256) k_pcg_p(int n, const double *__restrict__ z, const double *__restrict__ zp,
-                                               double *__restrict__ p, double *__restrict__ pp,
-                                               const double *__restrict__ part, int stride, int nb, void *rec,
-                                               int mixed) {
-  const int in[4] = {S_RZ, S_RZP, S_RZN, S_RZPN};
-  double t[4];
-  block_totals<4>(part, in, stride, nb, t);
-  double prev = t[0], prevp = t[1];
-  if (prev == 0.0) prev = 1.0;
-  if (prevp == 0.0) prevp = 1.0;
-  double b = t[2] / prev, bp = t[3] / prevp;
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 3 * n) return;
-  double a = z[q] + b * p[q], ap = zp[q] + bp * pp[q];
-  p[q] = a;
-  pp[q] = ap;
-  dip_put(rec, mixed, q / 3, q % 3, a, ap);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +399,7 @@ static int mixed_of(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXE
 
 void nb_gather_sorted(NbHandle *h) {
   NB_LAUNCH(k_gather_pos, grid_for(h->n, 256), dim3(256), 0, h->n, h->d_perm, h->d_xyz, h->d_mp, h->d_pos4,
-            (void *)h->d_dip, mixed_of(h), h->cell, (const int32_t *)h->d_wrap);
+            (void *)h->d_dip, (void *)h->d_dip2, mixed_of(h), h->cell, (const int32_t *)h->d_wrap);
 }
 
 // one launch clears the accumulated outputs of an evaluation
@@ -557,55 +458,68 @@ void nb_induced_field(NbHandle *h, const double *u, const double *up, double *f,
 // preconditioner field of polar * r (the packed records must hold it)
 void nb_precond_field(NbHandle *h, double *f, double *fp) { nb_dip_field_real(h, f, fp, true); }
 
-int nb_pcg(NbHandle *h, nb_energies *out) {
+int nb_precond_z(NbHandle *h, int neps, bool control);
+int nb_pme_pcg_direction(NbHandle *h, double a3, int nrz, int first);
+
+// One PCG solve (PCGSolver.scfByPCG, PCGSolver.java:487-604) from the dipoles
+// u0 held in d_u / d_up and the records d_dip:
+//   r0 = (u_direct - u0) / polar + E(u0)                    k_pcg_init_r
+//   z0 = polar (E_pre(polar r0) + scale r0)                  k_precond_z
+//   per iteration:
+//     E(z): real space (side stream) || spread + transforms (main stream),
+//     joined before the gather, which forms p = z + beta p and q = A p
+//     (A z + beta q)                                         k_gather_ap
+//     alpha = r.z / p.q; r -= alpha q; u += alpha p         k_pcg_update
+//     z = polar (E_pre(polar r) + scale r), residual norms  k_precond_z
+// control: the convergence test runs in k_precond_z (a recorded solve of
+// `iters` iterations); otherwise the host reads the norms back each
+// iteration.
+static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
   int n = h->n;
   dim3 B(256), G = grid_for(3 * n, 256);
   const int nb = (int)G.x, stride = h->part_stride;
-  if (nb > stride) throw std::runtime_error("PCG partial-sum buffer too small");
-  double psc = h->prm.precond_scale;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
   double *f = h->d_field, *fp = h->d_fieldp;  // permanent fields are no longer needed
   double *part = h->d_part;
-  nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
-  void *rec = (void *)h->d_dip;
-  int mx = mixed_of(h);
-  NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp,
-                     h->d_r, h->d_rp, rec, mx);
-  nb_precond_field(h, f, fp);
-  NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
-                     h->d_pp, 1, part, stride, (int)S_RZN, rec, mx);
   int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
+  int mx = mixed_of(h);
+  nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
+  NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp, h->d_r, h->d_rp,
+            (void *)h->d_dip2, mx);
+  if (control) {
+    NB_LAUNCH(k_pcg_ctl_init, dim3(1), dim3(64), 0, h->d_scal);
+    h->ctl = h->d_scal;
+  }
+  int nrz = nb_precond_z(h, 0, control);
   double eps = 100.0, prev;
   int iter = 0, status = NB_OK;
-  for (;;) {
-    // E(p): real space on the side stream || reciprocal space on the main
-    // stream; the reciprocal and self parts are folded into k_pcg_ap
+  for (int it = 0; control ? it < iters : true; it++) {
+    // E(z): real space on the side stream || reciprocal space on the main stream
     nb_fork(h);
     {
       NbSide side(h);
       nb_dip_field_real(h, f, fp, false);
     }
-    if (use_recip) nb_pme_induced(h, h->d_p, h->d_pp, false);
-    nb_join(h);
-    NB_LAUNCH(k_pcg_ap, G, B, 0, n, a3, h->d_polar, h->d_p, h->d_pp, f, fp,
-              use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
-              h->d_vec, h->d_vecp, part, stride);
-    if (h->tape_on) nb_tape_push(h);
+    int npq;
+    if (use_recip) {
+      npq = nb_pme_pcg_direction(h, a3, nrz, it == 0);
+    } else {
+      nb_join(h);
+      NB_LAUNCH(k_pcg_dir, G, B, 0, n, h->d_polar, h->d_z, h->d_zp, f, fp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
+                part, stride, nrz, h->d_scal, (int)(it == 0), h->ctl);
+      npq = nb;
+    }
+    if (h->tape_on) nb_tape_push(h, nrz, npq);
     NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
-              h->d_up, part, stride, nb, rec, mx, (const double *)nullptr);
-    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal, 0, n, 0.0, 0);
+              h->d_up, part, stride, npq, nrz, (void *)h->d_dip2, mx, h->d_scal, h->ctl);
+    // the next z and the norms of this update; when the test ends the loop,
+    // this z, its records and partial sums are not read after the solve
+    nrz = nb_precond_z(h, nb, control);
+    if (control) continue;
     HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipEventRecord(h->ev_eps, h->stream));
-    // The next direction is queued before the convergence test is read back,
-    // so the GPU stays busy during the host round trip.  If the test ends the
-    // loop, these kernels have only rewritten the field, z, p and the packed
-    // records, none of which is read after the solve.
-    nb_precond_field(h, f, fp);
-    NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p,
-                       h->d_pp, 0, part, stride, (int)S_RZN, rec, mx);
-    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_z, h->d_zp, h->d_p, h->d_pp, part, stride, nb, rec, mx);
     HIPCHECK(hipEventSynchronize(h->ev_eps));
     prev = eps;
     double e = h->h_pinned[S_EPS] > h->h_pinned[S_EPSP] ? h->h_pinned[S_EPS] : h->h_pinned[S_EPSP];
@@ -615,64 +529,25 @@ int nb_pcg(NbHandle *h, nb_energies *out) {
     if (iter >= maxit) { status = NB_ERR_SCF_MAXITER; break; }
     if (eps < h->prm.polar_eps) break;
   }
-  out->pcg_iters = iter;
-  out->pcg_eps = eps;
+  h->ctl = nullptr;
+  if (!control) {
+    out->pcg_iters = iter;
+    out->pcg_eps = eps;
+  }
   // reciprocal potentials of the final dipoles (20 derivatives each) for the
   // energy and gradient; their real-space field is not needed
   if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
   return status;
 }
 
+int nb_pcg(NbHandle *h, nb_energies *out) { return pcg_solve(h, out, false, 0); }
+
 // The same solve with its control on the device, for recording into a graph:
-// a fixed budget of iterations is queued; once k_pcg_eps has stopped the
-// solve, the update and the field, spread and gather kernels of the remaining
-// iterations return at once (the transforms and small vector kernels still
-// run on stale data that nothing reads).  The result equals that of nb_pcg.
-int nb_pcg_device(NbHandle *h, int iters) {
-  int n = h->n;
-  dim3 B(256), G = grid_for(3 * n, 256);
-  const int nb = (int)G.x, stride = h->part_stride;
-  if (nb > stride) throw std::runtime_error("PCG partial-sum buffer too small");
-  double psc = h->prm.precond_scale;
-  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
-  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
-  double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
-  double *f = h->d_field, *fp = h->d_fieldp;
-  double *part = h->d_part;
-  int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
-  nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
-  void *rec = (void *)h->d_dip;
-  int mx = mixed_of(h);
-  NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp, h->d_r, h->d_rp,
-            rec, mx);
-  nb_precond_field(h, f, fp);
-  NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, 1, part,
-            stride, (int)S_RZN, rec, mx);
-  NB_LAUNCH(k_pcg_ctl_init, dim3(1), dim3(64), 0, h->d_scal);
-  h->ctl = h->d_scal;
-  for (int it = 0; it < iters; it++) {
-    nb_fork(h);
-    {
-      NbSide side(h);
-      nb_dip_field_real(h, f, fp, false);
-    }
-    if (use_recip) nb_pme_induced(h, h->d_p, h->d_pp, false);
-    nb_join(h);
-    NB_LAUNCH(k_pcg_ap, G, B, 0, n, a3, h->d_polar, h->d_p, h->d_pp, f, fp,
-              use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r, h->d_rp, h->d_z, h->d_zp,
-              h->d_vec, h->d_vecp, part, stride);
-    NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
-              h->d_up, part, stride, nb, rec, mx, (const double *)h->d_scal);
-    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, part, stride, nb, h->d_scal, 1, n, h->prm.polar_eps, maxit);
-    nb_precond_field(h, f, fp);
-    NB_LAUNCH(k_pcg_z, G, B, 0, n, h->d_polar, psc, f, fp, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, 0,
-              part, stride, (int)S_RZN, rec, mx);
-    NB_LAUNCH(k_pcg_p, G, B, 0, n, h->d_z, h->d_zp, h->d_p, h->d_pp, part, stride, nb, rec, mx);
-  }
-  h->ctl = nullptr;
-  if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
-  return NB_OK;
-}
+// a fixed budget of iterations is queued; once k_pcg_update has stopped the
+// solve, the update and the field, spread, gather and z kernels of the
+// remaining iterations return at once (the transforms still run on stale data
+// that nothing reads).  The result equals that of nb_pcg.
+int nb_pcg_device(NbHandle *h, int iters) { return pcg_solve(h, nullptr, true, iters); }
 
 void nb_reset_dipoles(NbHandle *h) {
   int n = h->n;
@@ -698,7 +573,7 @@ int nb_sor(NbHandle *h, double w, nb_energies *out) {
     nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
     NB_LAUNCH(k_sor_update, G, B, 0, n, w, h->d_polar, h->d_udir, h->d_updir, f, fp, h->d_u, h->d_up, h->d_part,
               stride, rec, mx);
-    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, h->d_part, stride, nb, h->d_scal, 0, n, 0.0, 0);
+    NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, h->d_part, stride, nb, h->d_scal);
     HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
     HIPCHECK(hipStreamSynchronize(h->stream));
     prev = eps;

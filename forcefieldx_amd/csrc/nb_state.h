@@ -51,8 +51,9 @@ struct NbHandle {
   int device = 0;
   hipStream_t stream = nullptr;   // main stream (PME chain, PCG vectors, bookkeeping)
   hipStream_t stream2 = nullptr;  // side stream (real-space pair kernels that overlap the PME chain)
+  hipStream_t stream3 = nullptr;  // third stream (vdW and valence terms, joined at the gradient assembly)
   hipStream_t cur = nullptr;      // stream that NB_LAUNCH uses
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_eps = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_eps = nullptr, ev_fork3 = nullptr, ev_join3 = nullptr;
   std::string err;
 
   // topology (caller order)
@@ -147,7 +148,8 @@ struct NbHandle {
   double *d_r = nullptr, *d_rp = nullptr, *d_z = nullptr, *d_zp = nullptr, *d_p = nullptr, *d_pp = nullptr;
   double *d_vec = nullptr, *d_vecp = nullptr;
   double *d_scal = nullptr;       // PCG scalars
-  double *d_dip = nullptr;        // [n] packed dipole records of the field kernels (96 B each)
+  double *d_dip = nullptr;        // [n] packed dipole records of the Ewald field kernel (96 B each)
+  double *d_dip2 = nullptr;       // [n] packed records of polar * r, read by the preconditioner field
   double *d_part = nullptr;       // [S_NSLOT][part_stride] per-block partial dot products
   // strain derivative (NB_VIRIAL): each contributing kernel writes 9 partial
   // sums per block into its own region of d_vpart; k_vir_reduce sums them
@@ -157,7 +159,7 @@ struct NbHandle {
   double *d_fgrid = nullptr;      // forward transform of the permanent grid (virial)
   bool want_vir = false, vir_valid = false;
   double vir[9];
-  int part_stride = 0;
+  int part_stride = 0;            // partial sums per slot: the largest block count of their producers
 
   // outputs
   double *d_grad_s = nullptr;     // [3n] sorted direct gradient
@@ -259,7 +261,7 @@ void nb_zero_outputs(NbHandle *h);
 double *nb_vir_region(NbHandle *h, long nblocks);
 void nb_grid_virial(NbHandle *h, bool polar);
 void nb_vir_reduce(NbHandle *h);
-void nb_tape_push(NbHandle *h);
+void nb_tape_push(NbHandle *h, int nrz, int npq);
 void nb_pred_apply(NbHandle *h);
 void nb_pred_save(NbHandle *h, bool device_ctl);
 void nb_md_free(NbHandle *h);
@@ -312,4 +314,18 @@ struct NbSide {
   NbHandle *h;
   explicit NbSide(NbHandle *hh) : h(hh) { h->cur = h->stream2; }
   ~NbSide() { h->cur = h->stream; }
+};
+// The same for the third stream.
+static inline void nb_fork3(NbHandle *h) {
+  HIPCHECK(hipEventRecord(h->ev_fork3, h->stream));
+  HIPCHECK(hipStreamWaitEvent(h->stream3, h->ev_fork3, 0));
+}
+static inline void nb_join3(NbHandle *h) {
+  HIPCHECK(hipEventRecord(h->ev_join3, h->stream3));
+  HIPCHECK(hipStreamWaitEvent(h->stream, h->ev_join3, 0));
+}
+struct NbThird {
+  NbHandle *h;
+  explicit NbThird(NbHandle *hh) : h(hh) { h->cur = h->stream3; }
+  ~NbThird() { h->cur = h->stream; }
 };
