@@ -121,16 +121,15 @@ struct NbHandle {
   float *d_recipf = nullptr;      // influence function, FP32 copy (mixed form)
   float *d_twf = nullptr;         // twiddle tables, FP32 copy
   // grid bricks: bricks along axis d cover [b K_d / nbk_d, (b+1) K_d / nbk_d);
-  // each atom is binned by its first B-spline point, and a brick's buffer holds
-  // its points plus the 4-point spill into the next brick (edge bext)
+  // each atom is binned by its first B-spline point, and the spread workgroup
+  // of a brick sums the stencils of its own bin and the previous bins
   int nbk[3] = {0, 0, 0};
-  int bext = 0, nbricks = 0;
+  int nbricks = 0;
   int32_t *d_bcount = nullptr;    // [nbricks] atoms per brick
   int32_t *d_bstart = nullptr;    // [nbricks + 1] exclusive scan
   int32_t *d_bid = nullptr;       // [n] brick of each sorted atom
   int32_t *d_brank = nullptr;     // [n] rank of the atom within its brick
   int32_t *d_batom = nullptr;     // [n] sorted atoms grouped by brick
-  double *d_bbuf = nullptr;       // [nbricks][2][bext^3] brick partial sums (FP64 or FP32)
   size_t lds_max = 65536;         // dynamic LDS available to one workgroup
   double *d_theta = nullptr;      // [n][3][5][4]
   int32_t *d_igrid = nullptr;     // [n][3]
