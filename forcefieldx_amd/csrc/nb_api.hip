@@ -90,7 +90,7 @@ void free_all(NbHandle *h) {
                   h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                  h->d_scal, h->d_part, h->d_dip, h->d_dip2, h->d_vpart, h->d_vir, h->d_soft0, h->d_mol0, h->d_soft, h->d_mol, h->d_gred_l, h->d_dudx, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
+                  h->d_scal, h->d_part, h->d_dip, h->d_dip2, h->d_dip64, h->d_theta64, h->d_vpart, h->d_vir, h->d_soft0, h->d_mol0, h->d_soft, h->d_mol, h->d_gred_l, h->d_dudx, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
@@ -542,6 +542,12 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_vir, 16);
     alloc(&h->d_dip, 12 * (size_t)n);
     alloc(&h->d_dip2, 12 * (size_t)n);
+    if (params->precision == NB_PREC_MIXED) {
+      // FP64 iterative refinement of the solve (on by default in the mixed form)
+      alloc(&h->d_dip64, 12 * (size_t)n);
+      alloc(&h->d_theta64, 60 * (size_t)n);
+      h->refine = true;
+    }
     // producers of PCG partial sums: vector kernels (3n threads), the gather
     // (8 lanes per atom) and the field kernels (NB_LPA lanes per atom)
     h->part_stride = (int)std::max({grid_for(3 * (long)n, 256).x, xcd_grid((long)n * 8, 256).x,
@@ -658,6 +664,20 @@ int nb_set_graph(void *handle, int on) {
   if (!h->graph_on) drop_graph(h);
   if (h->child_env) nb_set_graph(h->child_env, on);
   if (h->child_vap) nb_set_graph(h->child_vap, on);
+  return NB_OK;
+}
+
+int nb_set_refinement(void *handle, int on) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  if (on && !h->d_dip64) {
+    h->err = "FP64 refinement applies to the mixed form only";
+    return NB_ERR_INVALID;
+  }
+  if ((on != 0) != h->refine) drop_graph(h);
+  h->refine = on != 0;
+  if (h->child_env) nb_set_refinement(h->child_env, on);
+  if (h->child_vap) nb_set_refinement(h->child_vap, on);
   return NB_OK;
 }
 
@@ -952,7 +972,10 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
     // record a graph for the next evaluations: the PCG budget is this
     // evaluation's iteration count plus one
-    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite)
+    // (a solve that needed FP64 corrections is not recorded: its replays
+    // would be repeated on the regular path each time)
+    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite &&
+        h->refine_cycles == 0)
       capture_graph(h, flags, mutual ? e.pcg_iters + 1 : 0);
     if (shrink_graph) drop_graph(h);
     h->last = e;

@@ -671,6 +671,17 @@ int nb_precond_z(NbHandle *h, int neps, bool control) {
   return (int)G.x;
 }
 
+// FP64 Ewald real-space field of the FP64 records d_dip64 (the FP64 residual
+// of the mixed form's refinement)
+void nb_dip_field64(NbHandle *h, double *f, double *fp) {
+  int n = h->n;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
+  NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, (const DipRec<double> *)h->d_dip64, h->d_ew, h->d_new,
+            h->maxew, h->cell, aw, f, fp, (const double *)nullptr);
+}
+
 #define NB_LPA_SP 4
 
 template <bool VIR, bool OST>

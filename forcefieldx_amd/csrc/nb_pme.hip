@@ -94,7 +94,7 @@ __host__ __device__ __forceinline__ int brick_start(int b, int K, int nb) { retu
 // B-splines, grid start, fractional multipoles and brick binning.  The
 // weights are stored in the grid's real type TH: the spread and the gather
 // compute in that type.
-template <typename TH>
+template <typename TH, bool BIN = true>
 __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const double *__restrict__ gmp, CellD cell,
                                Bricks bk, Frac fr, TH *__restrict__ theta, int32_t *__restrict__ igrid,
                                double *__restrict__ fmp, int32_t *__restrict__ bcount, int32_t *__restrict__ bid,
@@ -109,7 +109,7 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
     double frx = bk.K[d] * u;
     int ifr = (int)frx;
     double bx = frx - ifr;
-    igrid[3 * s + d] = ifr - 5;
+    if (BIN) igrid[3 * s + d] = ifr - 5;
     b3[d] = brick_of(wrapg(ifr - 4, bk.K[d]), bk.K[d], bk.nb[d]);
     double th[5][4];
     bspline4<5>(bx, th);
@@ -117,10 +117,12 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
     for (int k = 0; k < 5; k++)
       for (int j = 0; j < 4; j++) o[4 * k + j] = (TH)th[k][j];
   }
-  int b = b3[0] + bk.nb[0] * (b3[1] + bk.nb[1] * b3[2]);
-  bid[s] = b;
-  brank[s] = atomicAdd(&bcount[b], 1);
-  frac_multipole(fr, gmp + 10 * s, fmp + 10 * s);
+  if (BIN) {
+    int b = b3[0] + bk.nb[0] * (b3[1] + bk.nb[1] * b3[2]);
+    bid[s] = b;
+    brank[s] = atomicAdd(&bcount[b], 1);
+    frac_multipole(fr, gmp + 10 * s, fmp + 10 * s);
+  }
 }
 
 __global__ void k_zero_i(int m, int32_t *__restrict__ x) {
@@ -889,6 +891,7 @@ void nb_pme_permanent(NbHandle *h) {
   NB_LAUNCH(k_brick_scan, dim3(1), dim3(1024), 0, h->nbricks, h->d_bcount, h->d_bstart);
   NB_LAUNCH(k_brick_fill, grid_for(n, 256), dim3(256), 0, n, h->d_bid, h->d_brank, h->d_bstart, h->d_batom);
   if (mixed_grid(h)) {
+    if (h->refine && h->prm.polarization == NB_POL_MUTUAL) nb_pme_theta64(h);
     spread_t<float, false>(h, nullptr, nullptr);
     convolution(h);
     gather_t<float, 1, true>(h, h->d_phip, nullptr);
@@ -897,6 +900,26 @@ void nb_pme_permanent(NbHandle *h) {
     convolution(h);
     gather_t<double, 1, true>(h, h->d_phip, nullptr);
   }
+}
+
+// FP64 B-spline weights of the sorted atoms (same grid starts and bins as the
+// weights of the mixed form), for the FP64 refinement of the mixed solve
+void nb_pme_theta64(NbHandle *h) {
+  int n = h->n;
+  NB_LAUNCH((k_bspline_frac<double, false>), dim3((n + 127) / 128), dim3(128), 0, n, h->d_pos4, h->d_gmp, h->cell,
+            bricks_of(h), frac_of(h), h->d_theta64, h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
+}
+
+// FP64 reciprocal potentials (all 20 derivatives and the Cartesian field) of
+// the dipoles u, up in the mixed form: FP64 spread with the FP64 weights,
+// FP64 transforms and influence function, FP64 gather
+void nb_pme_induced64(NbHandle *h, const double *u, const double *up) {
+  CX<double> *g = (CX<double> *)h->d_grid;
+  spread_into<double, true>(h, g, u, up, h->d_theta64);
+  transform_t<double>(h, g, (const CX<double> *)h->d_tw, h->d_recip);
+  NB_LAUNCH((k_gather<double, 2, true>), xcd_grid((long)h->n * 8, 256), dim3(256), 0, h->n,
+            (const double *)h->d_theta64, h->d_igrid, (const CX<double> *)g, frac_of(h), h->K[0], h->K[1], h->K[2],
+            h->d_phiu, h->d_phiup, h->ctl);
 }
 
 void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full) {

@@ -185,6 +185,83 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
   block_partials<2>(v, part, out, stride);
 }
 
+// FP64 records of the dipoles u, up for the FP64 residual of the refinement
+__global__ void k_rec64(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
+                        const double *__restrict__ u, const double *__restrict__ up, DipRec<double> *__restrict__ rec) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  DipRec<double> r;
+  double4 p = pos[s];
+  r.x = p.x; r.y = p.y; r.z = p.z;
+  r.ipd = mp[s].ipd;
+  r.thole = mp[s].thole;
+  for (int d = 0; d < 3; d++) {
+    r.u[d] = u[3 * s + d];
+    r.v[d] = up[3 * s + d];
+  }
+  r.pad = 0.0;
+  rec[s] = r;
+}
+
+// FP64 residual of the solve (the r of PCGSolver.java:609-663 with every
+// field term in FP64): r = (u_direct - u) / polar + f + a3 u - phi, f the
+// FP64 real-space field, phi the FP64 reciprocal field (phiu null: none); the
+// records of the preconditioner field receive polar * r (a restarted solve
+// starts from them); partial r.r into S_EPS / S_EPSP.
+__global__ void __launch_bounds__(256) k_refine_residual(int n, const double *__restrict__ polar, double a3,
+                                                         const double *__restrict__ ud, const double *__restrict__ upd,
+                                                         const double *__restrict__ u, const double *__restrict__ up,
+                                                         const double *__restrict__ f, const double *__restrict__ fp,
+                                                         const double *__restrict__ phiu,
+                                                         const double *__restrict__ phiup, double *__restrict__ r,
+                                                         double *__restrict__ rp, void *rec, int mixed,
+                                                         double *__restrict__ part, int stride) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (q < 3 * n) {
+    int s = q / 3, d = q - 3 * s;
+    double a = polar[s];
+    double ft = f[q] + a3 * u[q], fpt = fp[q] + a3 * up[q];
+    if (phiu) {
+      ft -= phiu[30 * (size_t)s + 21 + d];
+      fpt -= phiup[30 * (size_t)s + 21 + d];
+    }
+    double rr = 0.0, rrp = 0.0;
+    if (a > 0.0) {
+      rr = (ud[q] - u[q]) / a + ft;
+      rrp = (upd[q] - up[q]) / a + fpt;
+    }
+    r[q] = rr;
+    rp[q] = rrp;
+    dip_put(rec, mixed, s, d, a * rr, a * rrp);
+    v[0] = rr * rr;
+    v[1] = rrp * rrp;
+  }
+  const int out[2] = {S_EPS, S_EPSP};
+  block_partials<2>(v, part, out, stride);
+}
+
+// eps of the FP64 residual (the reference's measure, PCGSolver.java:543-547)
+// for a recorded solve: a solve whose FP64 residual is not below polar_eps is
+// marked unfinished (S_DONE cleared), so the replay is repeated on the
+// regular path, which takes the corrections
+__global__ void __launch_bounds__(256) k_refine_check(const double *__restrict__ part, int stride, int nb,
+                                                      double *__restrict__ scal, int n, double polar_eps) {
+  const int in[2] = {S_EPS, S_EPSP};
+  double t[2];
+  block_totals<2>(part, in, stride, nb, t);
+  if (threadIdx.x == 0) {
+    double e = t[0] > t[1] ? t[0] : t[1];
+    double eps = 4.80321 * sqrt(e / n);
+    scal[S_EPS] = t[0];
+    scal[S_EPSP] = t[1];
+    if (scal[S_STAT] == 0.0) {
+      scal[S_EPSV] = eps;
+      if (!(eps < polar_eps)) scal[S_DONE] = 0.0;
+    }
+  }
+}
+
 // residual norms of an SOR cycle: scal[S_EPS], scal[S_EPSP]
 __global__ void __launch_bounds__(256) k_pcg_eps(const double *__restrict__ part, int stride, int nb,
                                                  double *__restrict__ scal) {
@@ -461,42 +538,30 @@ void nb_precond_field(NbHandle *h, double *f, double *fp) { nb_dip_field_real(h,
 int nb_precond_z(NbHandle *h, int neps, bool control);
 int nb_pme_pcg_direction(NbHandle *h, double a3, int nrz, int first);
 
-// One PCG solve (PCGSolver.scfByPCG, PCGSolver.java:487-604) from the dipoles
-// u0 held in d_u / d_up and the records d_dip:
-//   r0 = (u_direct - u0) / polar + E(u0)                    k_pcg_init_r
-//   z0 = polar (E_pre(polar r0) + scale r0)                  k_precond_z
-//   per iteration:
-//     E(z): real space (side stream) || spread + transforms (main stream),
-//     joined before the gather, which forms p = z + beta p and q = A p
-//     (A z + beta q)                                         k_gather_ap
-//     alpha = r.z / p.q; r -= alpha q; u += alpha p         k_pcg_update
-//     z = polar (E_pre(polar r) + scale r), residual norms  k_precond_z
-// control: the convergence test runs in k_precond_z (a recorded solve of
-// `iters` iterations); otherwise the host reads the norms back each
-// iteration.
-static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
+// PCG iterations (PCGSolver.scfByPCG, PCGSolver.java:487-604) from r, z (and
+// the records of z) already in place; nrz: partial sums of the current r.z.
+//   E(z): real space (side stream) || spread + transforms (main stream),
+//   joined before the gather, which forms p = z + beta p and q = A p
+//   (A z + beta q)                                         k_gather_ap
+//   alpha = r.z / p.q; r -= alpha q; u += alpha p         k_pcg_update
+//   z = polar (E_pre(polar r) + scale r), residual norms  k_precond_z
+// control: `iters` iterations are queued and the convergence test runs in
+// k_precond_z (a recorded solve); otherwise the host reads the norms back
+// each iteration.  iter / eps receive the count and the last eps.
+static int pcg_iterations(NbHandle *h, bool control, int iters, int nrz, int &iter, double &eps) {
   int n = h->n;
   dim3 B(256), G = grid_for(3 * n, 256);
-  const int nb = (int)G.x, stride = h->part_stride;
+  const int nb = (int)G.x;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
-  double *f = h->d_field, *fp = h->d_fieldp;  // permanent fields are no longer needed
-  double *part = h->d_part;
+  double *f = h->d_field, *fp = h->d_fieldp;
   int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
   int mx = mixed_of(h);
-  nb_induced_field(h, h->d_u, h->d_up, f, fp, false);
-  NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, f, fp, h->d_r, h->d_rp,
-            (void *)h->d_dip2, mx);
-  if (control) {
-    NB_LAUNCH(k_pcg_ctl_init, dim3(1), dim3(64), 0, h->d_scal);
-    h->ctl = h->d_scal;
-  }
-  int nrz = nb_precond_z(h, 0, control);
-  double eps = 100.0, prev;
-  int iter = 0, status = NB_OK;
+  double prev;
+  eps = 100.0;
+  iter = 0;
   for (int it = 0; control ? it < iters : true; it++) {
-    // E(z): real space on the side stream || reciprocal space on the main stream
     nb_fork(h);
     {
       NbSide side(h);
@@ -508,12 +573,12 @@ static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
     } else {
       nb_join(h);
       NB_LAUNCH(k_pcg_dir, G, B, 0, n, h->d_polar, h->d_z, h->d_zp, f, fp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
-                part, stride, nrz, h->d_scal, (int)(it == 0), h->ctl);
+                h->d_part, h->part_stride, nrz, h->d_scal, (int)(it == 0), h->ctl);
       npq = nb;
     }
     if (h->tape_on) nb_tape_push(h, nrz, npq);
     NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
-              h->d_up, part, stride, npq, nrz, (void *)h->d_dip2, mx, h->d_scal, h->ctl);
+              h->d_up, h->d_part, h->part_stride, npq, nrz, (void *)h->d_dip2, mx, h->d_scal, h->ctl);
     // the next z and the norms of this update; when the test ends the loop,
     // this z, its records and partial sums are not read after the solve
     nrz = nb_precond_z(h, nb, control);
@@ -525,18 +590,98 @@ static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
     double e = h->h_pinned[S_EPS] > h->h_pinned[S_EPSP] ? h->h_pinned[S_EPS] : h->h_pinned[S_EPSP];
     iter++;
     eps = 4.80321 * sqrt(e / n);
-    if (!(eps <= prev)) { status = NB_ERR_SCF_DIVERGED; break; }
-    if (iter >= maxit) { status = NB_ERR_SCF_MAXITER; break; }
-    if (eps < h->prm.polar_eps) break;
+    if (!(eps <= prev)) return NB_ERR_SCF_DIVERGED;
+    if (iter >= maxit) return NB_ERR_SCF_MAXITER;
+    if (eps < h->prm.polar_eps) return NB_OK;
   }
+  return NB_OK;
+}
+
+// FP64 residual of the current dipoles and their FP64 reciprocal potentials
+// (all derivatives, which are then the final potentials): FP64 records and
+// real-space field (side stream) || FP64 spread, transforms and gather (main
+// stream), then k_refine_residual; partial r.r are left in S_EPS / S_EPSP.
+static void refine_residual(NbHandle *h) {
+  int n = h->n;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  double aw = use_recip ? h->prm.ewald_alpha : 0.0;
+  double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
+  NB_LAUNCH(k_rec64, grid_for(n, 256), dim3(256), 0, n, h->d_pos4, h->d_mp, h->d_u, h->d_up,
+            (DipRec<double> *)h->d_dip64);
+  nb_fork(h);
+  {
+    NbSide side(h);
+    nb_dip_field64(h, h->d_field, h->d_fieldp);
+  }
+  if (use_recip) nb_pme_induced64(h, h->d_u, h->d_up);
+  nb_join(h);
+  NB_LAUNCH(k_refine_residual, grid_for(3 * n, 256), dim3(256), 0, n, h->d_polar, a3, h->d_udir, h->d_updir, h->d_u,
+            h->d_up, h->d_field, h->d_fieldp, use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r,
+            h->d_rp, (void *)h->d_dip2, mixed_of(h), h->d_part, h->part_stride);
+}
+
+constexpr int NB_REFINE_MAX = 3;   // corrections of one solve
+
+// One induced-dipole solve from the dipoles u0 held in d_u / d_up and the
+// records d_dip: r0 = (u_direct - u0) / polar + E(u0) (k_pcg_init_r), z0, then
+// the iterations.  In the mixed form with refinement (h->refine) the FP32
+// operator's solve is followed by FP64 iterative refinement: the residual is
+// formed with the FP64 operator and its eps (PCGSolver.java:543-547) must be
+// below polar_eps; otherwise the solve is restarted from the current dipoles
+// with r = the FP64 residual (FP32 corrections), up to NB_REFINE_MAX times.
+// The FP64 residual's reciprocal potentials are the final ones.
+static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
+  int n = h->n;
+  dim3 B(256), G = grid_for(3 * n, 256);
+  const int nb = (int)G.x;
+  bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
+  int mx = mixed_of(h);
+  nb_induced_field(h, h->d_u, h->d_up, h->d_field, h->d_fieldp, false);
+  NB_LAUNCH(k_pcg_init_r, G, B, 0, n, h->d_polar, h->d_udir, h->d_updir, h->d_u, h->d_up, h->d_field, h->d_fieldp,
+            h->d_r, h->d_rp, (void *)h->d_dip2, mx);
+  if (control) {
+    NB_LAUNCH(k_pcg_ctl_init, dim3(1), dim3(64), 0, h->d_scal);
+    h->ctl = h->d_scal;
+  }
+  int nrz = nb_precond_z(h, 0, control);
+  int iter = 0;
+  double eps = 100.0;
+  int status = pcg_iterations(h, control, iters, nrz, iter, eps);
   h->ctl = nullptr;
-  if (!control) {
+  h->refine_cycles = 0;
+  const bool refine = h->refine && mx && !h->tape_on;
+  if (refine && control) {
+    refine_residual(h);
+    NB_LAUNCH(k_refine_check, dim3(1), B, 0, h->d_part, h->part_stride, nb, h->d_scal, n, h->prm.polar_eps);
+    return NB_OK;
+  }
+  if (refine && status == NB_OK) {
+    for (int cycle = 0;; cycle++) {
+      refine_residual(h);
+      NB_LAUNCH(k_pcg_eps, dim3(1), B, 0, h->d_part, h->part_stride, nb, h->d_scal);
+      HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
+      HIPCHECK(hipStreamSynchronize(h->stream));
+      double e = h->h_pinned[S_EPS] > h->h_pinned[S_EPSP] ? h->h_pinned[S_EPS] : h->h_pinned[S_EPSP];
+      eps = 4.80321 * sqrt(e / n);
+      if (eps < h->prm.polar_eps || cycle == NB_REFINE_MAX) break;
+      h->refine_cycles = cycle + 1;
+      // restart from the current dipoles with the FP64 residual (its records
+      // of polar * r were written by k_refine_residual)
+      nrz = nb_precond_z(h, 0, false);
+      int it2 = 0;
+      double eps2 = 100.0;
+      status = pcg_iterations(h, false, 0, nrz, it2, eps2);
+      iter += it2;
+      if (status != NB_OK) break;
+    }
+  }
+  if (out) {
     out->pcg_iters = iter;
     out->pcg_eps = eps;
   }
   // reciprocal potentials of the final dipoles (20 derivatives each) for the
-  // energy and gradient; their real-space field is not needed
-  if (use_recip) nb_pme_induced(h, h->d_u, h->d_up, true);
+  // energy and gradient, unless the FP64 residual of the refinement gave them
+  if (use_recip && !(refine && status == NB_OK)) nb_pme_induced(h, h->d_u, h->d_up, true);
   return status;
 }
 

@@ -149,6 +149,12 @@ struct NbHandle {
   double *d_scal = nullptr;       // PCG scalars
   double *d_dip = nullptr;        // [n] packed dipole records of the Ewald field kernel (96 B each)
   double *d_dip2 = nullptr;       // [n] packed records of polar * r, read by the preconditioner field
+  // FP64 iterative refinement of the mixed form's solve: FP64 records of the
+  // dipoles and FP64 B-spline weights for the FP64 residual
+  bool refine = false;
+  double *d_dip64 = nullptr;      // [n] DipRec<double>
+  double *d_theta64 = nullptr;    // [n][3][5][4]
+  int refine_cycles = 0;          // corrections taken by the last solve
   double *d_part = nullptr;       // [S_NSLOT][part_stride] per-block partial dot products
   // strain derivative (NB_VIRIAL): each contributing kernel writes 9 partial
   // sums per block into its own region of d_vpart; k_vir_reduce sums them
@@ -241,6 +247,9 @@ void nb_rotate(NbHandle *h);
 void nb_pme_setup(NbHandle *h);
 void nb_pme_permanent(NbHandle *h);
 void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full);
+void nb_pme_theta64(NbHandle *h);
+void nb_pme_induced64(NbHandle *h, const double *u, const double *up);
+void nb_dip_field64(NbHandle *h, double *f, double *fp);
 void nb_filter_lists(NbHandle *h);
 bool nb_grow_filtered_rows(NbHandle *h, const int32_t *fl);
 void nb_perm_field(NbHandle *h);

@@ -26,7 +26,7 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
             "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config", "nb_set_valence",
             "nb_valence_struct_size", "nb_set_scf_predictor", "nb_md_init", "nb_md_step", "nb_md_get_state",
-            "nb_md_set_thermostat"]
+            "nb_md_set_thermostat", "nb_set_refinement"]
 NB_PRED_NONE, NB_PRED_ASPC = 0, 1
 NB_THERMO_NONE, NB_THERMO_BERENDSEN = 0, 1
 VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
@@ -120,6 +120,8 @@ def load(path=LIB_PATH):
     L.nb_set_positions_device.argtypes = [vp, vp]
     L.nb_set_lambda.argtypes = [vp, C.c_double]
     L.nb_set_graph.argtypes = [vp, C.c_int]
+    L.nb_set_refinement.argtypes = [vp, C.c_int]
+    L.nb_set_refinement.restype = C.c_int
     L.nb_evaluate_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(NbEnergies)]
     L.nb_batch_config.argtypes = [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.nb_graph_replays.argtypes = [vp]
@@ -294,6 +296,10 @@ class Handle:
     def set_graph(self, on=True):
         """Record evaluations into a HIP graph and replay them."""
         self._check(self.L.nb_set_graph(self.h, int(on)))
+
+    def set_refinement(self, on=True):
+        """FP64 iterative refinement of the mixed form's solve (default on)."""
+        self._check(self.L.nb_set_refinement(self.h, int(on)))
 
     def graph_replays(self):
         return self.L.nb_graph_replays(self.h)

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define NB_ABI_VERSION 8
+#define NB_ABI_VERSION 9
 
 /* Return codes.  Map onto the reference's failure behaviour:
  * EnergyException on NaN energy (RealSpaceEnergyRegion.java:384-391),
@@ -327,6 +327,20 @@ int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energi
  * workgroups, cfg[3] systems; kernel_ms (may be NULL) its duration from HIP
  * events around the launch (no counterpart in the reference). */
 int nb_batch_config(int device, int32_t cfg[4], double *kernel_ms);
+
+/* FP64 iterative refinement of the mixed form's induced-dipole solve (on by
+ * default in NB_PREC_MIXED; no counterpart in the reference, whose solve is
+ * FP64 throughout).  After the PCG with the FP32 field operator converges, the
+ * residual (PCGSolver.java:609-663) is formed again with the FP64 operator
+ * (FP64 real-space field and FP64 PME) and its eps (PCGSolver.java:543-547)
+ * must be below polar_eps; otherwise the PCG restarts from the current dipoles
+ * with the FP64 residual, up to three times.  The FP64 reciprocal potentials
+ * of that last residual are the ones the energy and gradient use;
+ * nb_energies.pcg_eps reports the FP64 eps and pcg_iters counts the restarts'
+ * iterations too.  A solve that needed restarts is not recorded into a graph.
+ * on = 0 turns it off (the FP32-operator solve stands); NB_ERR_INVALID for an
+ * FP64 handle. */
+int nb_set_refinement(void *handle, int on);
 
 /* Number of evaluations of this handle served by a graph replay. */
 int nb_graph_replays(void *handle);

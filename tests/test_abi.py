@@ -25,7 +25,7 @@ def test_library_exports_all_declared_symbols():
 
 def test_abi_version_and_struct_sizes():
     L = ffi.load()
-    assert L.nb_abi_version() == 8
+    assert L.nb_abi_version() == 9
     # the ctypes mirrors have the sizes the C compiler gives the header structs
     sizes = (ctypes.c_int32 * 3)()
     assert L.nb_struct_sizes(sizes) == ffi.NB_OK

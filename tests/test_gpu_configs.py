@@ -58,6 +58,50 @@ def test_c2_water20736_fp64_matches_oracle(oracle):
     assert rel_err(u, uo) <= REL and rel_err(up, upo) <= REL
 
 
+def test_c2_refinement_brings_mixed_dipoles_to_fp64():
+    """FP64 iterative refinement of the mixed solve (nb_set_refinement): at a
+    tolerance where the FP32 field operator limits the mixed dipoles, the
+    FP64 residual test restarts the solve, and the refined dipoles lie closer
+    to the converged FP64 ones than the FP32-operator solve's.  At the C2
+    tolerance (1e-5 D) the refined mixed form stays within 1e-5 of FP64."""
+    xyz, box, top = water_box(6912, 59.1, seed=2)
+    ref = NonbondedParams(vdw_cutoff=9.0, grid=(64, 64, 64), polar_eps=1e-10)
+    fp64 = {}
+    for eps in (1e-10, 1e-5):
+        print(f"C2 refinement: FP64 at {eps} D", flush=True)
+        p = dataclasses.replace(ref, polar_eps=eps)
+        h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+        try:
+            h.set_cell(box)
+            h.set_positions(xyz)
+            h.evaluate(ffi.NB_GRAD)
+            fp64[eps] = (h.induced()[0], h.gradient())
+        finally:
+            h.close()
+    u64, g64 = fp64[1e-10]
+    res = {}
+    for eps in (1e-7, 1e-5):
+        p = dataclasses.replace(ref, polar_eps=eps)
+        for on in (False, True):
+            h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_MIXED), top)
+            try:
+                h.set_cell(box)
+                h.set_refinement(on)
+                h.set_positions(xyz)
+                e, _ = h.evaluate(ffi.NB_GRAD)
+                res[eps, on] = (e, h.induced()[0], h.gradient())
+            finally:
+                h.close()
+            e, u, g = res[eps, on]
+            print(f"polar_eps {eps} refinement {on}: PCG {e['pcg_iters']}, eps {e['pcg_eps']:.3e}, dipole err "
+                  f"{rel_err(u, u64):.3e}, force err {rel_err(g, g64):.3e}", flush=True)
+    (e_off, u_off, _), (e_on, u_on, _) = res[1e-7, False], res[1e-7, True]
+    assert e_on["pcg_eps"] < 1e-7
+    assert rel_err(u_on, u64) < rel_err(u_off, u64)
+    for on in (False, True):
+        assert rel_err(res[1e-5, on][2], fp64[1e-5][1]) <= 1e-5
+
+
 C3_LAMBDAS = (0.0, 0.25, 0.5, 0.75, 1.0)
 
 
