@@ -184,27 +184,37 @@ def time_gpu(top, box, xyz, params, prec, steps, warmup, profile, graph=False, l
 
 
 def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, graph=False, lam=None):
-    """As time_gpu, but before each evaluation the coordinates, resident on the
-    device, receive a Gaussian step of sigma 0.01 A (torch on the device; the
-    library copies them device-to-device).  Both precisions see the same
-    sequence, so the last gradients are comparable."""
-    import torch
+    """As time_gpu, but every evaluation sees new coordinates: a random walk
+    with Gaussian steps of sigma 0.01 A per coordinate, whose frames are
+    written to device memory before the timed region; each evaluation hands
+    the library the next frame's device address (nb_set_positions_device, a
+    device-to-device copy).  Both precisions see the same sequence, so the
+    last gradients are comparable."""
+    from forcefieldx_amd import ffi
     from forcefieldx_amd.energy import ForceFieldEnergy
     ffe = ForceFieldEnergy(top, box, params, precision=prec)
     h = ffe._handle
     if lam is not None:
         ffe.setLambda(lam)
-    x = torch.tensor(np.asarray(xyz, np.float64).reshape(-1), device="cuda")
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(1234)
+    x0 = np.asarray(xyz, np.float64).reshape(-1)
+    nprof = 2 if profile else 0
+    nframes = nprof + warmup + steps
+    rng = np.random.default_rng(1234)
+    frames = ffi.DeviceArray(x0.size, frames=nframes)
+    x = x0.copy()
+    for k in range(nframes):
+        if k == nprof:
+            x = x0.copy()   # the timed sequence starts from the given coordinates
+        x += rng.normal(0.0, 0.01, x.shape)
+        frames.upload(x, k)
+    cur = [0]
 
     def step():
-        x.add_(torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64), alpha=0.01)
-        torch.cuda.current_stream().synchronize()
-        h.set_positions_device(x.data_ptr())
+        h.set_positions_device(frames.ptr(cur[0]))
+        cur[0] += 1
         ffe._evaluate(True)
 
-    ffe._x = np.asarray(xyz, np.float64).reshape(-1)
+    ffe._x = x0.copy()
     h.set_positions(ffe._x)
     calib, dominant = {}, None
     if profile:
@@ -219,13 +229,10 @@ def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, grap
         h.set_graph(True)
     elif profile:
         h.set_profiling(True, kernel_filter=dominant)
-    # both precisions start the timed steps from the same coordinates
-    x.copy_(torch.tensor(np.asarray(xyz, np.float64).reshape(-1), device="cuda"))
-    gen.manual_seed(1234)
     for _ in range(warmup):
         step()
     replays0 = h.graph_replays()
-    torch.cuda.synchronize()
+    ffi.device_synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
@@ -239,6 +246,7 @@ def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, grap
         if graph:
             h.set_graph(False)
             h.set_profiling(True, kernel_filter=dominant)
+            cur[0] = nframes - steps
             for _ in range(steps):
                 step()
         prof = {"calibration": calib, "timed": h.profile(), "graph_replays": replays}
@@ -247,6 +255,7 @@ def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, grap
     if lam is not None:
         extra["dudl"] = ffe.getdEdL()
     ffe.destroy()
+    frames.close()
     return (t1 - t0) / steps, last, g, u, prof, extra
 
 

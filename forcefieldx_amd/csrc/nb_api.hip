@@ -538,7 +538,7 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
       alloc(v, n3);
       HIPCHECK(hipMemset(*v, 0, sizeof(double) * n3));
     }
-    alloc(&h->d_scal, 16);
+    alloc(&h->d_scal, S_NSLOT);
     alloc(&h->d_vir, 16);
     alloc(&h->d_dip, 12 * (size_t)n);
     alloc(&h->d_dip2, 12 * (size_t)n);
@@ -892,9 +892,11 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
           e.pcg_iters = (int)sc[S_ITER];
           e.pcg_eps = sc[S_EPSV];
           status = (int)sc[S_STAT];
-          // the recorded budget's surplus iterations still run their
-          // transforms; when solves have become cheaper (an SCF predictor
-          // warming up) the graph is recorded again at the next evaluation
+          // the recorded budget's surplus iterations still cost their
+          // (early-returning) launches; when solves have become cheaper (an
+          // SCF predictor warming up) the graph is recorded again at the next
+          // evaluation (against the first solve's budget: the recorded
+          // refinement corrections only run when a residual asks for them)
           if (e.pcg_iters + 2 < h->graph_iters) shrink_graph = true;
         }
       } else {
@@ -971,12 +973,25 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     if (status == NB_OK && fl[4]) status = NB_ERR_BAD_TORQUE;
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
     // record a graph for the next evaluations: the PCG budget is this
-    // evaluation's iteration count plus one
-    // (a solve that needed FP64 corrections is not recorded: its replays
-    // would be repeated on the regular path each time)
-    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite &&
-        h->refine_cycles == 0)
-      capture_graph(h, flags, mutual ? e.pcg_iters + 1 : 0);
+    // evaluation's iteration count plus one, and so is the budget of each
+    // FP64 refinement correction it took; with refinement at least one
+    // correction of two iterations is recorded, so a replay whose FP32 solve
+    // stops just short of the FP64 test is corrected in the graph instead of
+    // being repeated on the regular path
+    if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite) {
+      int first = e.pcg_iters;
+      const bool refining = mutual && h->refine && h->prm.precision == NB_PREC_MIXED;
+      h->graph_refine_n = 0;
+      if (refining) {
+        for (int c = 0; c < h->refine_cycles; c++) {
+          h->graph_refine_iters[c] = h->refine_iters[c] + 1;
+          first -= h->refine_iters[c];
+        }
+        h->graph_refine_n = h->refine_cycles;
+        if (h->graph_refine_n == 0) h->graph_refine_iters[h->graph_refine_n++] = 2;
+      }
+      capture_graph(h, flags, mutual ? first + 1 : 0);
+    }
     if (shrink_graph) drop_graph(h);
     h->last = e;
     if (out) *out = e;

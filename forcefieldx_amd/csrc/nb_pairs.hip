@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_dip_field(int n, const DipRec<T> *__res
                                                    const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
                                                    int maxl, CellD cell, double aewald, double *__restrict__ f,
                                                    double *__restrict__ fp, const double *__restrict__ ctl) {
-  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) k_precond_z(int n, const DipRec<T> *__res
                                                    int mixed, double *__restrict__ part, int stride, int neps,
                                                    double *__restrict__ scal, int control, double polar_eps,
                                                    int maxit, const double *__restrict__ ctl) {
-  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
@@ -679,7 +679,7 @@ void nb_dip_field64(NbHandle *h, double *f, double *fp) {
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, (const DipRec<double> *)h->d_dip64, h->d_ew, h->d_new,
-            h->maxew, h->cell, aw, f, fp, (const double *)nullptr);
+            h->maxew, h->cell, aw, f, fp, h->ctl);
 }
 
 #define NB_LPA_SP 4

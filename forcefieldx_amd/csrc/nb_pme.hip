@@ -165,11 +165,42 @@ __global__ void k_brick_fill(int n, const int32_t *__restrict__ bid, const int32
 // One workgroup per brick of the grid (the bricks partition the grid).  It
 // sums, for every point it owns, the stencils of the atoms binned in its
 // brick and in the previous brick along each axis (an atom's 5-point stencil
-// starts in its bin and ends in the next), clipped to the owned points, with
-// FP64 LDS atomics, and writes its points straight into the interleaved
-// complex grid (real: permanent / u, imaginary: uCR), so every point is
-// written once and no other pass assembles the grid.  Thread (atom, ky, kz)
-// forms the y-z weight and adds the owned x points of that row.
+// starts in its bin and ends in the next), clipped to the owned points, and
+// writes its points straight into the interleaved complex grid (real:
+// permanent / u, imaginary: uCR), so every point is written once and no other
+// pass assembles the grid.  Thread (atom, ky, kz) forms the y-z weight and
+// adds the owned x points of that row.
+//
+// The sums are fixed point in 64-bit integers (LDS integer atomics), so they
+// do not depend on the order in which the atomics land: every evaluation of
+// the same input writes the same grid, in the FP32 grid of the mixed form as
+// in the FP64 form (a floating-point sum would round differently with the
+// order, and one FP32 ulp of a grid point moves the solved dipoles by up to
+// ~1e-7).  The scale is chosen per brick and per part before the sums: the
+// order-5 B-spline values, first and second derivatives are at most 0.599,
+// 0.546 and 1.25, so every product of three factors in a stencil value is
+// below 1 and an atom's values are bounded by b = sum|m| (permanent) or
+// sum|d| (dipoles); a point receives at most one value per atom, so with na
+// atoms reaching the brick and bmax their largest bound, every partial sum
+// stays below na bmax < 2^e, and the quantum 2^(e-62) keeps the sums within
+// int64 (a factor 2 to spare for the rounding of the values) with
+// 62 - log2(na) significant bits against the largest value.
+__device__ __forceinline__ double spread_bound(const double *__restrict__ fmp, const double *__restrict__ v3,
+                                               const Frac &fr, int s, bool ind) {
+  if (!ind) {
+    const double *m = fmp + 10 * (size_t)s;
+    double b = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; k++) b += fabs(m[k]);
+    return b;
+  }
+  const double *d = v3 + 3 * (size_t)s;
+  double b = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; a++) b += fabs(fr.a[a][0] * d[0] + fr.a[a][1] * d[1] + fr.a[a][2] * d[2]);
+  return b;
+}
+
 template <typename T, bool IND>
 __global__ void __launch_bounds__(256) k_spread_own(int n, Bricks bk, const int32_t *__restrict__ bstart,
                                                     const int32_t *__restrict__ batom, const T *__restrict__ theta,
@@ -177,22 +208,53 @@ __global__ void __launch_bounds__(256) k_spread_own(int n, Bricks bk, const int3
                                                     const double *__restrict__ u3, const double *__restrict__ up3,
                                                     Frac fr, CX<T> *__restrict__ grid,
                                                     const double *__restrict__ ctl) {
-  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   constexpr int NP = IND ? 2 : 1, E = NB_BRICK_MAX;
-  __shared__ double lds[NP][E * E * E];
+  __shared__ unsigned long long lds[NP][E * E * E];
+  __shared__ double bmax_w[NP][4];
   const int b = blockIdx.x;
   const int bx = b % bk.nb[0], r = b / bk.nb[0], by = r % bk.nb[1], bz = r / bk.nb[1];
   const int s0 = brick_start(bx, bk.K[0], bk.nb[0]), s1 = brick_start(by, bk.K[1], bk.nb[1]),
             s2 = brick_start(bz, bk.K[2], bk.nb[2]);
   const int L0 = brick_start(bx + 1, bk.K[0], bk.nb[0]) - s0, L1 = brick_start(by + 1, bk.K[1], bk.nb[1]) - s1,
             L2 = brick_start(bz + 1, bk.K[2], bk.nb[2]) - s2;
-  for (int t = threadIdx.x; t < NP * E * E * E; t += blockDim.x) (&lds[0][0])[t] = 0.0;
-  __syncthreads();
+  for (int t = threadIdx.x; t < NP * E * E * E; t += blockDim.x) (&lds[0][0])[t] = 0ull;
+  // the 8 source bins and the largest per-atom bound of each part
+  int sa0[8], sna[8], natot = 0;
+  double bm[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) bm[p] = 0.0;
+#pragma unroll
   for (int src = 0; src < 8; src++) {
     const int sx = (bx - (src & 1) + bk.nb[0]) % bk.nb[0], sy = (by - ((src >> 1) & 1) + bk.nb[1]) % bk.nb[1],
               sz = (bz - (src >> 2) + bk.nb[2]) % bk.nb[2];
     const int sb = sx + bk.nb[0] * (sy + bk.nb[1] * sz);
-    const int a0 = bstart[sb], na = min(bstart[sb + 1], n) - a0;
+    sa0[src] = bstart[sb];
+    sna[src] = min(bstart[sb + 1], n) - sa0[src];
+    natot += sna[src];
+    for (int a = threadIdx.x; a < sna[src]; a += blockDim.x) {
+      const int s = batom[sa0[src] + a];
+#pragma unroll
+      for (int p = 0; p < NP; p++) bm[p] = fmax(bm[p], spread_bound(fmp, p ? up3 : u3, fr, s, IND));
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bm[p] = fmax(bm[p], __shfl_xor(bm[p], o, 64));
+    if ((threadIdx.x & 63) == 0) bmax_w[p][threadIdx.x >> 6] = bm[p];
+  }
+  __syncthreads();
+  double qs[NP];   // 2^(62 - e): the quantum's inverse
+#pragma unroll
+  for (int p = 0; p < NP; p++) {
+    const double m = fmax(fmax(bmax_w[p][0], bmax_w[p][1]), fmax(bmax_w[p][2], bmax_w[p][3])) * (double)natot;
+    int e = 0;
+    if (m > 0.0) frexp(m, &e);
+    qs[p] = ldexp(1.0, 62 - e);
+  }
+  for (int src = 0; src < 8; src++) {
+    const int a0 = sa0[src], na = sna[src];
     for (int q = threadIdx.x; q < na * 25; q += blockDim.x) {
       const int a = q / 25, yz = q - a * 25, ky = yz % 5, kz = yz / 5;
       const int s = batom[a0 + a];
@@ -239,7 +301,8 @@ __global__ void __launch_bounds__(256) k_spread_own(int n, Bricks bk, const int3
         lx -= s0;
         if (lx < 0 || lx >= L0) continue;
 #pragma unroll
-        for (int p = 0; p < NP; p++) atomicAdd(&lds[p][row + lx], (double)val[p][kx]);
+        for (int p = 0; p < NP; p++)
+          atomicAdd(&lds[p][row + lx], (unsigned long long)__double2ll_rn((double)val[p][kx] * qs[p]));
       }
     }
   }
@@ -248,8 +311,8 @@ __global__ void __launch_bounds__(256) k_spread_own(int n, Bricks bk, const int3
     const int lx = t % L0, rr = t / L0, ly = rr % L1, lz = rr / L1;
     const int l = (lz * E + ly) * E + lx;
     CX<T> v;
-    v.x = (T)lds[0][l];
-    v.y = NP == 2 ? (T)lds[NP - 1][l] : (T)0;
+    v.x = (T)((double)(long long)lds[0][l] / qs[0]);
+    v.y = NP == 2 ? (T)((double)(long long)lds[NP - 1][l] / qs[NP - 1]) : (T)0;
     grid[(size_t)(s0 + lx) + (size_t)bk.K[0] * ((size_t)(s1 + ly) + (size_t)bk.K[1] * (s2 + lz))] = v;
   }
 }
@@ -265,7 +328,7 @@ __global__ void __launch_bounds__(256) k_gather(int n, const T *__restrict__ the
                                                 const CX<T> *__restrict__ grid, Frac fr, int K0, int K1, int K2,
                                                 double *__restrict__ out0, double *__restrict__ out1,
                                                 const double *__restrict__ ctl) {
-  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
   gather_atom<T, NP, FULL, false, T>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, out0, out1);
@@ -288,7 +351,7 @@ __global__ void __launch_bounds__(256) k_gather_ap(int n, const T *__restrict__ 
                                                    double *__restrict__ vecp, double *__restrict__ part, int stride,
                                                    int nrz, const double *__restrict__ scal, int first,
                                                    const double *__restrict__ ctl) {
-  if (ctl && ctl[S_DONE] != 0.0) return;  // recorded solve already stopped
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   double beta[2];
   pcg_beta(part, stride, nrz, scal, first, beta);
   int gt = xcd_block() * blockDim.x + threadIdx.x;
@@ -318,7 +381,9 @@ __global__ void __launch_bounds__(256) k_gather_ap(int n, const T *__restrict__ 
 // lines along one axis, L lines per workgroup (fft_chunk of chunk blockIdx.x)
 template <typename T, int EPT, int AXIS, bool CONV>
 __global__ void __launch_bounds__(256) k_fft_lines(CX<T> *__restrict__ grid, int K0, int K1, int K2, int L, Radices rad,
-                                                   const CX<T> *__restrict__ twg, T s, const T *__restrict__ recip) {
+                                                   const CX<T> *__restrict__ twg, T s, const T *__restrict__ recip,
+                                                   const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int N = AXIS == 0 ? K0 : (AXIS == 1 ? K1 : K2);
   CX<T> *buf = (CX<T> *)smem_raw, *tw = buf + (size_t)L * N;
@@ -418,7 +483,8 @@ __device__ __forceinline__ void ct_fft(CX<T> *buf, const CX<T> *tw, T s) {
 // x lines (contiguous), rows padded by one element in LDS
 template <typename T, int N>
 __global__ void __launch_bounds__(CT_BS) k_ct_x(CX<T> *__restrict__ grid, int K1, int K2,
-                                                const CX<T> *__restrict__ twg, T s) {
+                                                const CX<T> *__restrict__ twg, T s, const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   constexpr int NL = CtLines<N>::x, SL = N + 1;
   __shared__ CX<T> buf[NL * SL];
   __shared__ CX<T> tw[N];
@@ -453,7 +519,9 @@ __global__ void __launch_bounds__(CT_BS) k_ct_x(CX<T> *__restrict__ grid, int K1
 
 // y lines: NL consecutive x at one z
 template <typename T, int N>
-__global__ void __launch_bounds__(CT_BS) k_ct_y(CX<T> *__restrict__ grid, int K0, const CX<T> *__restrict__ twg, T s) {
+__global__ void __launch_bounds__(CT_BS) k_ct_y(CX<T> *__restrict__ grid, int K0, const CX<T> *__restrict__ twg, T s,
+                                                const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   constexpr int NL = CtLines<N>::yz;
   __shared__ CX<T> buf[NL * N];
   __shared__ CX<T> tw[N];
@@ -490,7 +558,8 @@ __global__ void __launch_bounds__(CT_BS) k_ct_y(CX<T> *__restrict__ grid, int K0
 // forward transform alone
 template <typename T, int N, bool CONV>
 __global__ void __launch_bounds__(CT_BS) k_ct_z(CX<T> *__restrict__ grid, int K01, const CX<T> *__restrict__ twg,
-                                                const T *__restrict__ recip) {
+                                                const T *__restrict__ recip, const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   constexpr int NL = CtLines<N>::yz;
   __shared__ CX<T> buf[NL * N];
   __shared__ CX<T> tw[N];
@@ -701,7 +770,7 @@ static bool ct_x(NbHandle *h, CX<T> *g, const CX<T> *tw, T s) {
 #define NB_X(N)                                                                                      \
   case N: {                                                                                          \
     dim3 G((K1 * K2 + CtLines<N>::x - 1) / CtLines<N>::x);                                           \
-    NB_LAUNCH((k_ct_x<T, N>), G, dim3(CT_BS), 0, g, K1, K2, tw, s);                                  \
+    NB_LAUNCH((k_ct_x<T, N>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, h->ctl);                                \
     return true;                                                                                     \
   }
     NB_CT_SIZES(NB_X)
@@ -717,7 +786,7 @@ static bool ct_y(NbHandle *h, CX<T> *g, const CX<T> *tw, T s) {
 #define NB_Y(N)                                                                                      \
   case N: {                                                                                          \
     dim3 G(((K0 + CtLines<N>::yz - 1) / CtLines<N>::yz) * K2);                                       \
-    NB_LAUNCH((k_ct_y<T, N>), G, dim3(CT_BS), 0, g, K0, tw, s);                                      \
+    NB_LAUNCH((k_ct_y<T, N>), G, dim3(CT_BS), 0, g, K0, tw, s, h->ctl);                                \
     return true;                                                                                     \
   }
     NB_CT_SIZES(NB_Y)
@@ -733,8 +802,8 @@ static bool ct_z(NbHandle *h, CX<T> *g, const CX<T> *tw, const T *recip) {
 #define NB_Z(N)                                                                                      \
   case N: {                                                                                          \
     dim3 G((K01 + CtLines<N>::yz - 1) / CtLines<N>::yz);                                             \
-    if (recip) NB_LAUNCH((k_ct_z<T, N, true>), G, dim3(CT_BS), 0, g, K01, tw, recip);                \
-    else NB_LAUNCH((k_ct_z<T, N, false>), G, dim3(CT_BS), 0, g, K01, tw, recip);                     \
+    if (recip) NB_LAUNCH((k_ct_z<T, N, true>), G, dim3(CT_BS), 0, g, K01, tw, recip, h->ctl);                \
+    else NB_LAUNCH((k_ct_z<T, N, false>), G, dim3(CT_BS), 0, g, K01, tw, recip, h->ctl);                    \
     return true;                                                                                     \
   }
     NB_CT_SIZES(NB_Z)
@@ -758,14 +827,14 @@ static void transform_t(NbHandle *h, CX<T> *g, const CX<T> *tw0, const T *recip)
   for (int pass = 0; pass < 2; pass++) {
     T s = pass == 0 ? (T)1 : (T)-1;
     if (pass == 0 && !ct_x<T>(h, g, twx, s))
-      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul);
+      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul, h->ctl);
     if (!ct_y<T>(h, g, twy, s))
-      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, s, nul);
+      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, s, nul, h->ctl);
     if (pass == 0) {
       if (!ct_z<T>(h, g, twz, recip))
-        NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, true>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, recip);
+        NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, true>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, recip, h->ctl);
     } else if (!ct_x<T>(h, g, twx, s)) {
-      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul);
+      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul, h->ctl);
     }
   }
 }
@@ -781,11 +850,11 @@ static void forward_t(NbHandle *h, CX<T> *g, const CX<T> *tw0) {
   dim3 B(256), Gx((K1 * K2 + Lx - 1) / Lx), Gy(((K0 + Ly - 1) / Ly) * K2), Gz((K0 * K1 + Lz - 1) / Lz);
   const T *nul = nullptr;
   if (!ct_x<T>(h, g, twx, (T)1))
-    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, (T)1, nul);
+    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, (T)1, nul, h->ctl);
   if (!ct_y<T>(h, g, twy, (T)1))
-    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, (T)1, nul);
+    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, (T)1, nul, h->ctl);
   if (!ct_z<T>(h, g, twz, nul))
-    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, false>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, nul);
+    NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, false>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, nul, h->ctl);
 }
 
 // Spread into a grid (the owned points of every brick; IND: u and uCR of the

@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(256) k_pcg_dir(int n, const double *__restrict
                                                  double *__restrict__ vecp, double *__restrict__ part, int stride,
                                                  int nrz, const double *__restrict__ scal, int first,
                                                  const double *__restrict__ ctl) {
-  if (ctl && ctl[S_DONE] != 0.0) return;
+  if (ctl && ctl[0] != 0.0) return;
   double beta[2];
   pcg_beta(part, stride, nrz, scal, first, beta);
   int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
                                                     void *rec, int mixed, double *__restrict__ scal,
                                                     const double *__restrict__ ctl) {
   // a recorded solve that has stopped keeps its dipoles and residuals
-  if (ctl && ctl[S_DONE] != 0.0) return;
+  if (ctl && ctl[0] != 0.0) return;
   const int in_pq[2] = {S_PAP, S_PAPP}, in_rz[2] = {S_RZN, S_RZPN};
   double pq[2], rz[2];
   block_totals<2>(part, in_pq, stride, npq, pq);
@@ -187,7 +187,9 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
 
 // FP64 records of the dipoles u, up for the FP64 residual of the refinement
 __global__ void k_rec64(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
-                        const double *__restrict__ u, const double *__restrict__ up, DipRec<double> *__restrict__ rec) {
+                        const double *__restrict__ u, const double *__restrict__ up, DipRec<double> *__restrict__ rec,
+                        const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded refinement already finished
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   DipRec<double> r;
@@ -215,7 +217,9 @@ __global__ void __launch_bounds__(256) k_refine_residual(int n, const double *__
                                                          const double *__restrict__ phiu,
                                                          const double *__restrict__ phiup, double *__restrict__ r,
                                                          double *__restrict__ rp, void *rec, int mixed,
-                                                         double *__restrict__ part, int stride) {
+                                                         double *__restrict__ part, int stride,
+                                                         const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded refinement already finished
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (q < 3 * n) {
@@ -242,22 +246,36 @@ __global__ void __launch_bounds__(256) k_refine_residual(int n, const double *__
 }
 
 // eps of the FP64 residual (the reference's measure, PCGSolver.java:543-547)
-// for a recorded solve: a solve whose FP64 residual is not below polar_eps is
-// marked unfinished (S_DONE cleared), so the replay is repeated on the
-// regular path, which takes the corrections
+// for a recorded solve.  restart = 1 (before a recorded correction): a solve
+// whose FP64 residual is not below polar_eps is reopened for the correction's
+// iterations (S_DONE cleared, the eps rise test restarted); one that passes,
+// or has failed, ends the refinement (S_RDONE), and the later corrections
+// and residuals return at once.  restart = 0 (the last check, unless the
+// refinement ended before): a solve whose FP64 residual is not below
+// polar_eps is marked unfinished (S_DONE cleared), so the replay is repeated
+// on the regular path, which takes further corrections.
 __global__ void __launch_bounds__(256) k_refine_check(const double *__restrict__ part, int stride, int nb,
-                                                      double *__restrict__ scal, int n, double polar_eps) {
+                                                      double *__restrict__ scal, int n, double polar_eps,
+                                                      int restart) {
   const int in[2] = {S_EPS, S_EPSP};
   double t[2];
   block_totals<2>(part, in, stride, nb, t);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && scal[S_RDONE] == 0.0) {
     double e = t[0] > t[1] ? t[0] : t[1];
     double eps = 4.80321 * sqrt(e / n);
+    const bool pass = eps < polar_eps, failed = scal[S_STAT] != 0.0;
     scal[S_EPS] = t[0];
     scal[S_EPSP] = t[1];
-    if (scal[S_STAT] == 0.0) {
-      scal[S_EPSV] = eps;
-      if (!(eps < polar_eps)) scal[S_DONE] = 0.0;
+    if (!failed) scal[S_EPSV] = eps;
+    if (restart) {
+      if (pass || failed) {
+        scal[S_RDONE] = 1.0;
+      } else {
+        scal[S_DONE] = 0.0;
+        scal[S_PREV] = 100.0;
+      }
+    } else if (!failed && !pass) {
+      scal[S_DONE] = 0.0;
     }
   }
 }
@@ -281,6 +299,7 @@ __global__ void k_pcg_ctl_init(double *__restrict__ scal) {
     scal[S_PREV] = 100.0;
     scal[S_STAT] = 0.0;
     scal[S_EPSV] = 0.0;
+    scal[S_RDONE] = 0.0;
   }
 }
 
@@ -607,7 +626,7 @@ static void refine_residual(NbHandle *h) {
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
   double a3 = 4.0 / 3.0 * aw * aw * aw / NB_SQRT_PI;
   NB_LAUNCH(k_rec64, grid_for(n, 256), dim3(256), 0, n, h->d_pos4, h->d_mp, h->d_u, h->d_up,
-            (DipRec<double> *)h->d_dip64);
+            (DipRec<double> *)h->d_dip64, h->ctl);
   nb_fork(h);
   {
     NbSide side(h);
@@ -617,7 +636,7 @@ static void refine_residual(NbHandle *h) {
   nb_join(h);
   NB_LAUNCH(k_refine_residual, grid_for(3 * n, 256), dim3(256), 0, n, h->d_polar, a3, h->d_udir, h->d_updir, h->d_u,
             h->d_up, h->d_field, h->d_fieldp, use_recip ? h->d_phiu : (const double *)nullptr, h->d_phiup, h->d_r,
-            h->d_rp, (void *)h->d_dip2, mixed_of(h), h->d_part, h->part_stride);
+            h->d_rp, (void *)h->d_dip2, mixed_of(h), h->d_part, h->part_stride, h->ctl);
 }
 
 constexpr int NB_REFINE_MAX = 3;   // corrections of one solve
@@ -641,7 +660,7 @@ static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
             h->d_r, h->d_rp, (void *)h->d_dip2, mx);
   if (control) {
     NB_LAUNCH(k_pcg_ctl_init, dim3(1), dim3(64), 0, h->d_scal);
-    h->ctl = h->d_scal;
+    h->ctl = h->d_scal + S_DONE;
   }
   int nrz = nb_precond_z(h, 0, control);
   int iter = 0;
@@ -651,8 +670,23 @@ static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
   h->refine_cycles = 0;
   const bool refine = h->refine && mx && !h->tape_on;
   if (refine && control) {
+    // the corrections the recorded evaluation took (at least one), each with
+    // a budget of its iteration count + 1; once a residual passes, the
+    // remaining corrections and residuals return at once (S_RDONE)
+    for (int c = 0; c < h->graph_refine_n; c++) {
+      h->ctl = h->d_scal + S_RDONE;
+      refine_residual(h);
+      NB_LAUNCH(k_refine_check, dim3(1), B, 0, h->d_part, h->part_stride, nb, h->d_scal, n, h->prm.polar_eps, 1);
+      h->ctl = h->d_scal + S_DONE;
+      int nrz2 = nb_precond_z(h, 0, true);
+      int it2 = 0;
+      double eps2 = 100.0;
+      pcg_iterations(h, true, h->graph_refine_iters[c], nrz2, it2, eps2);
+    }
+    h->ctl = h->d_scal + S_RDONE;
     refine_residual(h);
-    NB_LAUNCH(k_refine_check, dim3(1), B, 0, h->d_part, h->part_stride, nb, h->d_scal, n, h->prm.polar_eps);
+    h->ctl = nullptr;
+    NB_LAUNCH(k_refine_check, dim3(1), B, 0, h->d_part, h->part_stride, nb, h->d_scal, n, h->prm.polar_eps, 0);
     return NB_OK;
   }
   if (refine && status == NB_OK) {
@@ -671,6 +705,7 @@ static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
       int it2 = 0;
       double eps2 = 100.0;
       status = pcg_iterations(h, false, 0, nrz, it2, eps2);
+      h->refine_iters[cycle] = it2;
       iter += it2;
       if (status != NB_OK) break;
     }

@@ -155,6 +155,7 @@ struct NbHandle {
   double *d_dip64 = nullptr;      // [n] DipRec<double>
   double *d_theta64 = nullptr;    // [n][3][5][4]
   int refine_cycles = 0;          // corrections taken by the last solve
+  int refine_iters[4] = {0, 0, 0, 0};  // PCG iterations of each of those corrections
   double *d_part = nullptr;       // [S_NSLOT][part_stride] per-block partial dot products
   // strain derivative (NB_VIRIAL): each contributing kernel writes 9 partial
   // sums per block into its own region of d_vpart; k_vir_reduce sums them
@@ -223,6 +224,9 @@ struct NbHandle {
   hipGraphExec_t gexec = nullptr;
   uint32_t graph_flags = 0;
   int graph_iters = 0;
+  // recorded FP64 refinement: corrections and the PCG budget of each
+  int graph_refine_n = 0;
+  int graph_refine_iters[4] = {0, 0, 0, 0};
   long graph_replays = 0;
   bool graph_launched = false;    // the graph of this evaluation was launched by nb_evaluate_batch
   bool used_sor = false;          // the last evaluation's dipoles came from SOR (not recorded)

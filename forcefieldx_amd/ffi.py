@@ -396,6 +396,85 @@ def device_count():
         return 0
 
 
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime the library itself links (loaded once per process, so
+    buffers made here live in the same device context as the library's)."""
+    global _hip
+    if _hip is None:
+        load()
+        H = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipSetDevice.argtypes = [C.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipSetDevice", "hipDeviceSynchronize"):
+            getattr(H, f).restype = C.c_int
+        _hip = H
+    return _hip
+
+
+def device_synchronize(device=0):
+    """Wait for all work on the device (the caller's timing boundary)."""
+    H = _hip_runtime()
+    H.hipSetDevice(device)
+    rc = H.hipDeviceSynchronize()
+    if rc != 0:
+        raise NbError(NB_ERR_HIP, f"hipDeviceSynchronize returned {rc}")
+
+
+class DeviceArray:
+    """A caller-owned FP64 buffer in device memory, for the *_device entry
+    points (coordinates already resident in HBM).  Frames of equal size can be
+    addressed by index: ptr(k) is the address of frame k."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, count, frames=1, device=0):
+        H = _hip_runtime()
+        self.count, self.frames, self.device = int(count), int(frames), device
+        self._p = C.c_void_p()
+        self._chk(H.hipSetDevice(device), "hipSetDevice")
+        self._chk(H.hipMalloc(C.byref(self._p), 8 * self.count * self.frames), "hipMalloc")
+
+    @staticmethod
+    def _chk(rc, what):
+        if rc != 0:
+            raise NbError(NB_ERR_HIP, f"{what} returned {rc}")
+
+    def ptr(self, frame=0):
+        if not 0 <= frame < self.frames:
+            raise IndexError(frame)
+        return self._p.value + 8 * self.count * frame
+
+    def upload(self, a, frame=0):
+        a = np.ascontiguousarray(a, np.float64).reshape(-1)
+        if a.size != self.count:
+            raise ValueError(f"expected {self.count} values, got {a.size}")
+        self._chk(_hip.hipMemcpy(C.c_void_p(self.ptr(frame)), a.ctypes.data_as(C.c_void_p), 8 * a.size, self.H2D),
+                  "hipMemcpy")
+
+    def download(self, frame=0):
+        out = np.empty(self.count, np.float64)
+        self._chk(_hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr(frame)), 8 * out.size, self.D2H),
+                  "hipMemcpy")
+        return out
+
+    def close(self):
+        if self._p.value:
+            _hip.hipFree(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def test_convolution(data, recip, nx, ny, nz, precision=NB_PREC_FP64):
     """GPU convolution of the library's FFT kernels (test entry); precision
     NB_PREC_MIXED runs the FP32 transform."""

@@ -212,15 +212,29 @@ def test_c4_crystals512_batch_kernel(oracle):
     err = rel_err(np.concatenate(gmx), np.concatenate(g64))
     print(f"C4 mixed against FP64: force rel RMS error {err:.2e}", flush=True)
     assert err <= 1e-5
-    for a, b in zip(emx, e64):
-        assert abs(a["total"] - b["total"]) <= 1e-5 * max(abs(b["perm"]), abs(b["pol"]), abs(b["vdw"]))
+    # energies against the largest term summed into them: the per-cell totals
+    # of these small cells cancel to well below their reciprocal and self terms
+    worst_e = 0.0
+    for c, (a, b) in enumerate(zip(emx, e64)):
+        scale = max(abs(b[k]) for k in ("perm_real", "perm_self", "perm_recip", "pol_real", "pol_self",
+                                        "pol_recip", "vdw"))
+        worst_e = max(worst_e, abs(a["total"] - b["total"]) / scale)
+        assert abs(a["total"] - b["total"]) <= 1e-5 * scale, (c, a["total"], b["total"], scale)
+    print(f"C4 mixed against FP64: worst energy error {worst_e:.2e} of the largest term", flush=True)
 
 
 def test_c5_water288000_graph_replays_under_device_steps():
-    torch = pytest.importorskip("torch")
     xyz, box, top = water_box(96000, 144.3, seed=2)
     p = NonbondedParams(vdw_cutoff=9.0, grid=(144, 144, 144), polar_eps=1e-5)
     steps = 20
+    # a random walk of sigma 0.01 A steps, every frame resident on the device
+    rng = np.random.default_rng(4321)
+    x0 = np.asarray(xyz, np.float64).reshape(-1)
+    frames = ffi.DeviceArray(x0.size, frames=steps)
+    x = x0.copy()
+    for k in range(steps):
+        x += rng.normal(0.0, 0.01, x.shape)
+        frames.upload(x, k)
     res = {}
     for prec in (ffi.NB_PREC_FP64, ffi.NB_PREC_MIXED):
         print(f"C5 water-288000 precision={prec}: {steps} graph replays", flush=True)
@@ -228,20 +242,15 @@ def test_c5_water288000_graph_replays_under_device_steps():
         try:
             h.set_cell(box)
             h.set_graph(True)
-            x = torch.tensor(np.asarray(xyz, np.float64).reshape(-1), device="cuda")
-            gen = torch.Generator(device="cuda")
-            gen.manual_seed(4321)
-            for _ in range(steps):
-                x.add_(torch.randn(x.shape, generator=gen, device="cuda", dtype=torch.float64), alpha=0.01)
-                torch.cuda.synchronize()
-                h.set_positions_device(x.data_ptr())
+            for k in range(steps):
+                h.set_positions_device(frames.ptr(k))
                 e, _ = h.evaluate(ffi.NB_GRAD)
             replays = h.graph_replays()
             g_replay = h.gradient()
             u_replay = h.induced()[0]
             # the same coordinates on the regular path
             h.set_graph(False)
-            h.set_positions_device(x.data_ptr())
+            h.set_positions_device(frames.ptr(steps - 1))
             e_reg, _ = h.evaluate(ffi.NB_GRAD)
             g_reg = h.gradient()
             res[prec] = (e, g_replay, u_replay, e_reg, g_reg, replays)
@@ -249,7 +258,7 @@ def test_c5_water288000_graph_replays_under_device_steps():
             h.close()
         print(f"C5 precision={prec}: {replays} replays, total {e['total']:.6f}, PCG {e['pcg_iters']}", flush=True)
         assert replays >= steps - 2
-        tol = 1e-12 if prec == ffi.NB_PREC_FP64 else 1e-8
+        tol = 1e-12
         assert np.abs(g_replay - g_reg).max() <= tol * np.abs(g_reg).max()
         assert abs(e["total"] - e_reg["total"]) <= tol * abs(e_reg["perm_self"])
     e64, g64, u64 = res[ffi.NB_PREC_FP64][:3]
@@ -258,3 +267,4 @@ def test_c5_water288000_graph_replays_under_device_steps():
     assert rel_err(gmx, g64) <= 1e-5
     assert rel_err(umx, u64) <= 1e-5
     assert abs(emx["total"] - e64["total"]) <= 1e-6 * abs(e64["perm_self"])
+    frames.close()

@@ -43,25 +43,33 @@ def test_crystal_matches_oracle(oracle, name):
 @pytest.mark.parametrize("name", NAMES)
 def test_crystal_mixed_against_fp64(name):
     """The mixed form on the thin P1 cells: nonbonded force relative RMS
-    error within 1e-5 of the FP64 form, the same PCG iteration count.  The
-    nonbonded gradient is the one compared: these geometries are minimized,
-    so the net force (valence included) is 13-63 times smaller than either
-    part, and its relative error is that much larger (3.6e-5 on acetanilide
-    with the valence terms included)."""
+    error within 1e-5 of the FP64 form.  Without the FP64 refinement the
+    FP32-operator solve takes the FP64 solve's iteration count; with it (the
+    default) the count can only grow, by the corrections the FP64 residual
+    asks for.  The nonbonded gradient is the one compared: these geometries
+    are minimized, so the net force (valence included) is 13-63 times smaller
+    than either part, and its relative error is that much larger (3.6e-5 on
+    acetanilide with the valence terms included)."""
     print(f"crystal mixed {name}", flush=True)
     xyz, box, top, p, terms, ns = load_crystal(name)
     out = {}
-    for prec in (ffi.NB_PREC_FP64, ffi.NB_PREC_MIXED):
+    for key, prec, refine in (("fp64", ffi.NB_PREC_FP64, False), ("mixed", ffi.NB_PREC_MIXED, True),
+                              ("mixed-unrefined", ffi.NB_PREC_MIXED, False)):
         h = ffi.Handle(ffi.params_from(p, p.grid, prec), top)
         try:
+            if prec == ffi.NB_PREC_MIXED:
+                h.set_refinement(refine)
             h.set_cell(box)
             h.set_positions(xyz.reshape(-1))
             e, rc = h.evaluate(ffi.NB_GRAD)
-            out[prec] = (e, h.gradient().reshape(-1, 3))
+            out[key] = (e, h.gradient().reshape(-1, 3))
         finally:
             h.close()
-    (e64, g64), (emx, gmx) = out[ffi.NB_PREC_FP64], out[ffi.NB_PREC_MIXED]
+    (e64, g64), (emx, gmx), (eun, gun) = out["fp64"], out["mixed"], out["mixed-unrefined"]
     err = np.sqrt(((gmx - g64) ** 2).sum() / (g64 ** 2).sum())
+    print(f"{name}: PCG FP64 {e64['pcg_iters']}, mixed {emx['pcg_iters']} (unrefined {eun['pcg_iters']}), "
+          f"force err {err:.2e}", flush=True)
     assert err <= 1e-5, err
-    assert emx["pcg_iters"] == e64["pcg_iters"]
+    assert eun["pcg_iters"] == e64["pcg_iters"]
+    assert emx["pcg_iters"] >= e64["pcg_iters"]
     assert abs(emx["total"] - e64["total"]) <= 1e-5 * max(1.0, abs(e64["perm"]))

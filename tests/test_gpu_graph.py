@@ -12,12 +12,10 @@ from forcefieldx_amd.systems import chain_in_water, water_box
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-12   # same kernels; only the order of floating-point atomics differs
-# The mixed form rounds the FP64 sums of the spread's LDS atomics to FP32 grid
-# values; when the atomics' order moves such a sum across a rounding boundary
-# the grid point differs by one FP32 ulp, and the solve carries that into the
-# dipoles at ~1e-9 relative.  Seen about once in 70 evaluations, on either path.
-TOL_MIXED = 1e-8
+TOL = 1e-12   # same kernels; only the order of the FP64 energy and gradient atomics differs
+# The mixed form's grids are fixed-point sums, independent of the order of the
+# spread's atomics, so its replays hold the same bar
+TOL_MIXED = 1e-12
 
 
 def make(top, box, p, precision, graph):
