@@ -87,7 +87,7 @@ void free_all(NbHandle *h) {
                   h->d_xyz, h->d_xyz_list, h->d_perm, h->d_iperm, h->d_cellid, h->d_cellcount, h->d_cellstart,
                   h->d_nbr, h->d_ew, h->d_pc, h->d_sp, h->d_nsp, h->d_nnb, h->d_new, h->d_npc, h->d_flags, h->d_pos4, h->d_wrap, h->d_red4, h->d_mp,
                   h->d_gmp, h->d_polar, h->d_group, h->d_vclass, h->d_grid, h->d_recip, h->d_theta, h->d_igrid,
-                  h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom,
+                  h->d_recipf, h->d_twf, h->d_bcount, h->d_bstart, h->d_bid, h->d_brank, h->d_batom, h->d_bbuf,
                   h->d_fmp, h->d_phip, h->d_phiu, h->d_phiup, h->d_tw, h->d_field, h->d_fieldp, h->d_u, h->d_up,
                   h->d_udir, h->d_updir, h->d_r, h->d_rp, h->d_z, h->d_zp, h->d_p, h->d_pp, h->d_vec, h->d_vecp,
                   h->d_scal, h->d_part, h->d_dip, h->d_dip2, h->d_dip64, h->d_theta64, h->d_vpart, h->d_vir, h->d_soft0, h->d_mol0, h->d_soft, h->d_mol, h->d_gred_l, h->d_dudx, h->d_grad_r, h->d_torque_r, h->d_grad_s, h->d_torque_s, h->d_gred_s, h->d_grad, h->d_energy, h->d_count};

@@ -6,16 +6,18 @@
 //  * k_bspline_frac: order-5 B-splines with 3 derivatives and fractional
 //    multipoles (ReciprocalSpace.java:1344-1397, 2778-2834), plus the binning
 //    of atoms into grid bricks.
-//  * k_spread_own: the SpatialPermanentLoop / SpatialInducedLoop density
+//  * k_spread_brick: the SpatialPermanentLoop / SpatialInducedLoop density
 //    (ReciprocalSpace.java:1445-1643).  Instead of the reference's 8-colour
-//    octant decomposition, the grid is cut into bricks and one workgroup owns
-//    one brick: it sums the 125-point stencils of the atoms that reach its
-//    points (those binned in its brick and in the previous brick along each
-//    axis) in LDS with LDS atomics and writes its points once into the
-//    interleaved complex grid (real part: permanent / u, imaginary part:
-//    uCR).  No global atomics, no second pass over the grid.
+//    octant decomposition, one workgroup owns one brick of the grid: it
+//    accumulates the 125-point stencils of its atoms in LDS (64-bit
+//    fixed-point sums, independent of the order of the LDS atomics) and
+//    writes the brick, including its 4-point spill into the next bricks, to a
+//    brick buffer; the forward x transform sums, for every grid point, its
+//    own brick and the spill of up to 7 neighbours into the interleaved
+//    complex grid (real part: permanent / u, imaginary part: uCR).  No global
+//    atomics.
 //  * k_ct_* / k_fft_lines: in-place in-LDS Stockham FFT (radix 2/3/4/5/7) along
-//    x and y, and a fused
+//    x (fused with the brick combine on the forward pass) and y, and a fused
 //    forward-z / influence multiply / inverse-z kernel
 //    (Complex3DParallel.convolution, Complex3DParallel.java:445-453, 667-798);
 //    the inverse is un-normalised as in the reference.
@@ -82,10 +84,11 @@ __global__ void k_rotate(int n, const double *__restrict__ xyz, const int32_t *_
 // ---------------------------------------------------------------------------
 // bricks: axis d is cut into nb bricks, brick b = [b K / nb, (b+1) K / nb)
 // ---------------------------------------------------------------------------
-struct Bricks { int K[3], nb[3]; };
-// longest brick edge (the owned region of a spread workgroup is at most this
-// cube; nb = max(2, ceil(K / 8)) keeps every edge at or below it)
+struct Bricks { int K[3], nb[3], ext; };
+// longest brick edge (nb = max(2, ceil(K / 8)) keeps every edge at or below
+// it); a spread workgroup's LDS brick holds the edge plus the 4-point spill
 #define NB_BRICK_MAX 8
+#define NB_BRICK_EXT (NB_BRICK_MAX + 4)
 
 __host__ __device__ __forceinline__ int brick_of(int p, int K, int nb) { return ((p + 1) * nb - 1) / K; }
 __host__ __device__ __forceinline__ int brick_start(int b, int K, int nb) { return (b * K) / nb; }
@@ -162,29 +165,19 @@ __global__ void k_brick_fill(int n, const int32_t *__restrict__ bid, const int32
   if (q < n) batom[q] = s;
 }
 
-// One workgroup per brick of the grid (the bricks partition the grid).  It
-// sums, for every point it owns, the stencils of the atoms binned in its
-// brick and in the previous brick along each axis (an atom's 5-point stencil
-// starts in its bin and ends in the next), clipped to the owned points, and
-// writes its points straight into the interleaved complex grid (real:
-// permanent / u, imaginary: uCR), so every point is written once and no other
-// pass assembles the grid.  Thread (atom, ky, kz) forms the y-z weight and
-// adds the owned x points of that row.
-//
-// The sums are fixed point in 64-bit integers (LDS integer atomics), so they
-// do not depend on the order in which the atomics land: every evaluation of
-// the same input writes the same grid, in the FP32 grid of the mixed form as
-// in the FP64 form (a floating-point sum would round differently with the
-// order, and one FP32 ulp of a grid point moves the solved dipoles by up to
-// ~1e-7).  The scale is chosen per brick and per part before the sums: the
-// order-5 B-spline values, first and second derivatives are at most 0.599,
-// 0.546 and 1.25, so every product of three factors in a stencil value is
-// below 1 and an atom's values are bounded by b = sum|m| (permanent) or
-// sum|d| (dipoles); a point receives at most one value per atom, so with na
-// atoms reaching the brick and bmax their largest bound, every partial sum
-// stays below na bmax < 2^e, and the quantum 2^(e-62) keeps the sums within
-// int64 (a factor 2 to spare for the rounding of the values) with
-// 62 - log2(na) significant bits against the largest value.
+// local offset of an atom's first spline point inside its brick, per axis
+__device__ __forceinline__ void brick_local(const Bricks &bk, int b, const int32_t *ig, int o[3]) {
+  int bx = b % bk.nb[0], r = b / bk.nb[0];
+  int by = r % bk.nb[1], bz = r / bk.nb[1];
+  int bb[3] = {bx, by, bz};
+#pragma unroll
+  for (int d = 0; d < 3; d++) o[d] = wrapg(ig[d] + 1, bk.K[d]) - brick_start(bb[d], bk.K[d], bk.nb[d]);
+}
+
+// Bound of the stencil values of atom s: the order-5 B-spline values, first
+// and second derivatives are at most 0.599, 0.546 and 1.25, so every product
+// of three factors in a stencil value is below 1 and the values are bounded by
+// sum|m| (permanent fractional multipole) or sum|d| (fractional dipole).
 __device__ __forceinline__ double spread_bound(const double *__restrict__ fmp, const double *__restrict__ v3,
                                                const Frac &fr, int s, bool ind) {
   if (!ind) {
@@ -201,42 +194,47 @@ __device__ __forceinline__ double spread_bound(const double *__restrict__ fmp, c
   return b;
 }
 
+// One workgroup per brick: it sums the full 125-point stencils of the atoms
+// binned in its brick into an LDS brick of edge ext (its points plus the
+// 4-point spill into the next bricks) and stores it, in the grid's real type,
+// to the brick buffer; the forward x transform assembles each grid point from
+// its own brick and the spills of up to 7 previous bricks (brick_point), in a
+// fixed order.  Thread (atom, ky, kz) forms the y-z weight and adds 5
+// consecutive x points.
+//
+// The LDS sums are fixed point in 64-bit integers (integer LDS atomics), so
+// they do not depend on the order in which the atomics land and every
+// evaluation of the same input gives the same grid, in the FP32 grid of the
+// mixed form as in the FP64 form (a floating-point sum rounds differently
+// with the order, and one FP32 ulp of a grid point moves the solved dipoles
+// by up to ~1e-7).  The scale is chosen per brick and part: a point receives
+// at most one value per atom, so with na atoms in the bin and bmax the
+// largest bound of their values (spread_bound) every partial sum stays below
+// na bmax < 2^e; the quantum 2^(e-62) keeps the sums within int64 (a factor 2
+// to spare for the rounding of the values) with 62 - log2(na) significant
+// bits against the largest value.
 template <typename T, bool IND>
-__global__ void __launch_bounds__(256) k_spread_own(int n, Bricks bk, const int32_t *__restrict__ bstart,
-                                                    const int32_t *__restrict__ batom, const T *__restrict__ theta,
-                                                    const int32_t *__restrict__ igrid, const double *__restrict__ fmp,
-                                                    const double *__restrict__ u3, const double *__restrict__ up3,
-                                                    Frac fr, CX<T> *__restrict__ grid,
-                                                    const double *__restrict__ ctl) {
+__global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const int32_t *__restrict__ bstart,
+                                                      const int32_t *__restrict__ batom, const T *__restrict__ theta,
+                                                      const int32_t *__restrict__ igrid,
+                                                      const double *__restrict__ fmp, const double *__restrict__ u3,
+                                                      const double *__restrict__ up3, Frac fr, T *__restrict__ bbuf,
+                                                      const double *__restrict__ ctl) {
   if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
-  constexpr int NP = IND ? 2 : 1, E = NB_BRICK_MAX;
-  __shared__ unsigned long long lds[NP][E * E * E];
+  constexpr int NP = IND ? 2 : 1, EM = NB_BRICK_EXT * NB_BRICK_EXT * NB_BRICK_EXT;
+  __shared__ unsigned long long lds[NP][EM];
   __shared__ double bmax_w[NP][4];
+  const int e = bk.ext, e3 = e * e * e;
   const int b = blockIdx.x;
-  const int bx = b % bk.nb[0], r = b / bk.nb[0], by = r % bk.nb[1], bz = r / bk.nb[1];
-  const int s0 = brick_start(bx, bk.K[0], bk.nb[0]), s1 = brick_start(by, bk.K[1], bk.nb[1]),
-            s2 = brick_start(bz, bk.K[2], bk.nb[2]);
-  const int L0 = brick_start(bx + 1, bk.K[0], bk.nb[0]) - s0, L1 = brick_start(by + 1, bk.K[1], bk.nb[1]) - s1,
-            L2 = brick_start(bz + 1, bk.K[2], bk.nb[2]) - s2;
-  for (int t = threadIdx.x; t < NP * E * E * E; t += blockDim.x) (&lds[0][0])[t] = 0ull;
-  // the 8 source bins and the largest per-atom bound of each part
-  int sa0[8], sna[8], natot = 0;
+  const int a0 = bstart[b], na = min(bstart[b + 1], n) - a0;
+  for (int t = threadIdx.x; t < NP * EM; t += blockDim.x) (&lds[0][0])[t] = 0ull;
   double bm[NP];
 #pragma unroll
   for (int p = 0; p < NP; p++) bm[p] = 0.0;
+  for (int a = threadIdx.x; a < na; a += blockDim.x) {
+    const int s = batom[a0 + a];
 #pragma unroll
-  for (int src = 0; src < 8; src++) {
-    const int sx = (bx - (src & 1) + bk.nb[0]) % bk.nb[0], sy = (by - ((src >> 1) & 1) + bk.nb[1]) % bk.nb[1],
-              sz = (bz - (src >> 2) + bk.nb[2]) % bk.nb[2];
-    const int sb = sx + bk.nb[0] * (sy + bk.nb[1] * sz);
-    sa0[src] = bstart[sb];
-    sna[src] = min(bstart[sb + 1], n) - sa0[src];
-    natot += sna[src];
-    for (int a = threadIdx.x; a < sna[src]; a += blockDim.x) {
-      const int s = batom[sa0[src] + a];
-#pragma unroll
-      for (int p = 0; p < NP; p++) bm[p] = fmax(bm[p], spread_bound(fmp, p ? up3 : u3, fr, s, IND));
-    }
+    for (int p = 0; p < NP; p++) bm[p] = fmax(bm[p], spread_bound(fmp, p ? up3 : u3, fr, s, IND));
   }
 #pragma unroll
   for (int p = 0; p < NP; p++) {
@@ -245,76 +243,116 @@ __global__ void __launch_bounds__(256) k_spread_own(int n, Bricks bk, const int3
     if ((threadIdx.x & 63) == 0) bmax_w[p][threadIdx.x >> 6] = bm[p];
   }
   __syncthreads();
-  double qs[NP];   // 2^(62 - e): the quantum's inverse
+  double qs[NP];   // 2^(62 - e): the inverse of the quantum
 #pragma unroll
   for (int p = 0; p < NP; p++) {
-    const double m = fmax(fmax(bmax_w[p][0], bmax_w[p][1]), fmax(bmax_w[p][2], bmax_w[p][3])) * (double)natot;
-    int e = 0;
-    if (m > 0.0) frexp(m, &e);
-    qs[p] = ldexp(1.0, 62 - e);
+    const double m = fmax(fmax(bmax_w[p][0], bmax_w[p][1]), fmax(bmax_w[p][2], bmax_w[p][3])) * (double)na;
+    int ex = 0;
+    if (m > 0.0) frexp(m, &ex);
+    qs[p] = ldexp(1.0, 62 - ex);
   }
-  for (int src = 0; src < 8; src++) {
-    const int a0 = sa0[src], na = sna[src];
-    for (int q = threadIdx.x; q < na * 25; q += blockDim.x) {
-      const int a = q / 25, yz = q - a * 25, ky = yz % 5, kz = yz / 5;
-      const int s = batom[a0 + a];
-      const int32_t *ig = igrid + 3 * s;
-      int ly = wrapg(ig[1] + 1 + ky, bk.K[1]) - s1, lz = wrapg(ig[2] + 1 + kz, bk.K[2]) - s2;
-      if (ly < 0 || ly >= L1 || lz < 0 || lz >= L2) continue;
-      const int fx = wrapg(ig[0] + 1, bk.K[0]);
-      const T *tx = theta + (size_t)s * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
-      // every global load is issued before the first LDS atomic
-      T wx[5][3];
+  for (int q = threadIdx.x; q < na * 25; q += blockDim.x) {
+    const int a = q / 25, yz = q - a * 25, ky = yz % 5, kz = yz / 5;
+    const int s = batom[a0 + a];
+    int o[3];
+    brick_local(bk, b, igrid + 3 * s, o);
+    const T *tx = theta + (size_t)s * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
+    const int base = ((o[2] + kz) * e + (o[1] + ky)) * e + o[0];
+    // every global load is issued before the first LDS atomic: the compiler
+    // does not move loads across atomics, so interleaving them serialises
+    // five load latencies per thread
+    T wx[5][3];
 #pragma unroll
-      for (int kx = 0; kx < 5; kx++) {
-        wx[kx][0] = tx[4 * kx];
-        wx[kx][1] = tx[4 * kx + 1];
-        wx[kx][2] = IND ? (T)0 : tx[4 * kx + 2];
-      }
-      T val[NP][5];
-      if (!IND) {
-        const double *m = fmp + 10 * (size_t)s;
-        T u0 = uu[0], u1 = uu[1], u2 = uu[2], v0 = vv[0], v1 = vv[1], v2 = vv[2];
-        T t0 = ((T)m[0] * v0 + (T)m[3] * v1 + (T)m[6] * v2) * u0 + ((T)m[2] * v0 + (T)m[9] * v1) * u1 + (T)m[5] * v0 * u2;
-        T t1 = ((T)m[1] * v0 + (T)m[8] * v1) * u0 + (T)m[7] * v0 * u1;
-        T t2 = (T)m[4] * v0 * u0;
+    for (int kx = 0; kx < 5; kx++) {
+      wx[kx][0] = tx[4 * kx];
+      wx[kx][1] = tx[4 * kx + 1];
+      wx[kx][2] = IND ? (T)0 : tx[4 * kx + 2];
+    }
+    T val[NP][5];
+    if (!IND) {
+      const double *m = fmp + 10 * (size_t)s;
+      T u0 = uu[0], u1 = uu[1], u2 = uu[2], v0 = vv[0], v1 = vv[1], v2 = vv[2];
+      T t0 = ((T)m[0] * v0 + (T)m[3] * v1 + (T)m[6] * v2) * u0 + ((T)m[2] * v0 + (T)m[9] * v1) * u1 + (T)m[5] * v0 * u2;
+      T t1 = ((T)m[1] * v0 + (T)m[8] * v1) * u0 + (T)m[7] * v0 * u1;
+      T t2 = (T)m[4] * v0 * u0;
 #pragma unroll
-        for (int kx = 0; kx < 5; kx++) val[0][kx] = wx[kx][0] * t0 + wx[kx][1] * t1 + wx[kx][2] * t2;
-      } else {
-        T u0 = uu[0], u1 = uu[1], v0 = vv[0], v1 = vv[1];
+      for (int kx = 0; kx < 5; kx++) val[0][kx] = wx[kx][0] * t0 + wx[kx][1] * t1 + wx[kx][2] * t2;
+    } else {
+      T u0 = uu[0], u1 = uu[1], v0 = vv[0], v1 = vv[1];
 #pragma unroll
-        for (int p = 0; p < NP; p++) {
-          const double *src3 = (p ? up3 : u3) + 3 * (size_t)s;
-          T d0 = (T)(fr.a[0][0] * src3[0] + fr.a[0][1] * src3[1] + fr.a[0][2] * src3[2]);
-          T d1 = (T)(fr.a[1][0] * src3[0] + fr.a[1][1] * src3[1] + fr.a[1][2] * src3[2]);
-          T d2 = (T)(fr.a[2][0] * src3[0] + fr.a[2][1] * src3[1] + fr.a[2][2] * src3[2]);
-          T t0 = d2 * v1 * u0 + d1 * v0 * u1, t1 = d0 * v0 * u0;
+      for (int p = 0; p < NP; p++) {
+        const double *src3 = (p ? up3 : u3) + 3 * (size_t)s;
+        T d0 = (T)(fr.a[0][0] * src3[0] + fr.a[0][1] * src3[1] + fr.a[0][2] * src3[2]);
+        T d1 = (T)(fr.a[1][0] * src3[0] + fr.a[1][1] * src3[1] + fr.a[1][2] * src3[2]);
+        T d2 = (T)(fr.a[2][0] * src3[0] + fr.a[2][1] * src3[1] + fr.a[2][2] * src3[2]);
+        T t0 = d2 * v1 * u0 + d1 * v0 * u1, t1 = d0 * v0 * u0;
 #pragma unroll
-          for (int kx = 0; kx < 5; kx++) val[p][kx] = wx[kx][0] * t0 + wx[kx][1] * t1;
-        }
-      }
-      const int row = (lz * E + ly) * E;
-#pragma unroll
-      for (int kx = 0; kx < 5; kx++) {
-        int lx = fx + kx;
-        if (lx >= bk.K[0]) lx -= bk.K[0];
-        lx -= s0;
-        if (lx < 0 || lx >= L0) continue;
-#pragma unroll
-        for (int p = 0; p < NP; p++)
-          atomicAdd(&lds[p][row + lx], (unsigned long long)__double2ll_rn((double)val[p][kx] * qs[p]));
+        for (int kx = 0; kx < 5; kx++) val[p][kx] = wx[kx][0] * t0 + wx[kx][1] * t1;
       }
     }
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int kx = 0; kx < 5; kx++)
+        atomicAdd(&lds[p][base + kx], (unsigned long long)__double2ll_rn((double)val[p][kx] * qs[p]));
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < L0 * L1 * L2; t += blockDim.x) {
-    const int lx = t % L0, rr = t / L0, ly = rr % L1, lz = rr / L1;
-    const int l = (lz * E + ly) * E + lx;
-    CX<T> v;
-    v.x = (T)((double)(long long)lds[0][l] / qs[0]);
-    v.y = NP == 2 ? (T)((double)(long long)lds[NP - 1][l] / qs[NP - 1]) : (T)0;
-    grid[(size_t)(s0 + lx) + (size_t)bk.K[0] * ((size_t)(s1 + ly) + (size_t)bk.K[1] * (s2 + lz))] = v;
+  T *out = bbuf + (size_t)b * NP * e3;
+  for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) {
+    const int p = t / e3, l = t - p * e3;
+    out[t] = (T)((double)(long long)lds[p][l] / qs[p]);
   }
+}
+
+// grid point (x, y, z) = own brick at local offset o plus, along each axis
+// where o < 4, the spill of the previous brick at local offset len_prev + o
+template <typename T, int NP>
+__device__ __forceinline__ CX<T> brick_point(const Bricks &bk, const T *__restrict__ bbuf, int cx, int cy, int cz) {
+  int c[3] = {cx, cy, cz};
+  int bo[3], lo[3], bp[3], lp[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    int K = bk.K[d], nb = bk.nb[d];
+    bo[d] = brick_of(c[d], K, nb);
+    int st = brick_start(bo[d], K, nb);
+    lo[d] = c[d] - st;
+    bp[d] = bo[d] == 0 ? nb - 1 : bo[d] - 1;
+    int lenp = (bo[d] == 0 ? K : st) - brick_start(bp[d], K, nb);
+    lp[d] = lenp + lo[d];
+  }
+  const int e = bk.ext, e3 = e * e * e;
+  T acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; p++) acc[p] = (T)0;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
+    if ((dx && lo[0] >= 4) || (dy && lo[1] >= 4) || (dz && lo[2] >= 4)) continue;
+    int bx = dx ? bp[0] : bo[0], by = dy ? bp[1] : bo[1], bz = dz ? bp[2] : bo[2];
+    int x = dx ? lp[0] : lo[0], y = dy ? lp[1] : lo[1], z = dz ? lp[2] : lo[2];
+    size_t bi = (size_t)bx + (size_t)bk.nb[0] * (by + (size_t)bk.nb[1] * bz);
+    const T *src = bbuf + bi * NP * e3 + (z * e + y) * e + x;
+#pragma unroll
+    for (int p = 0; p < NP; p++) acc[p] += src[(size_t)p * e3];
+  }
+  CX<T> v;
+  v.x = acc[0];
+  v.y = NP == 2 ? acc[NP - 1] : (T)0;
+  return v;
+}
+
+// the grid assembled from the brick buffer (for lengths without a fused x
+// transform)
+template <typename T, int NP>
+__global__ void __launch_bounds__(256) k_brick_combine(Bricks bk, const T *__restrict__ bbuf, CX<T> *__restrict__ grid,
+                                                       const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ntot = (long)bk.K[0] * bk.K[1] * bk.K[2];
+  if (g >= ntot) return;
+  int cx = (int)(g % bk.K[0]);
+  long r = g / bk.K[0];
+  grid[g] = brick_point<T, NP>(bk, bbuf, cx, (int)(r % bk.K[1]), (int)(r / bk.K[1]));
 }
 
 // Gather: 8 lanes per atom, lane kz < 5 sums one z plane (25 rows of 5
@@ -480,10 +518,13 @@ __device__ __forceinline__ void ct_fft(CX<T> *buf, const CX<T> *tw, T s) {
   }
 }
 
-// x lines (contiguous), rows padded by one element in LDS
-template <typename T, int N>
+// x lines (contiguous), rows padded by one element in LDS.  MODE 0 reads the
+// grid; MODE 1 / 2 assembles each point from the brick buffer (1 or 2 parts),
+// which fuses the brick combine into the forward transform.
+template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(CT_BS) k_ct_x(CX<T> *__restrict__ grid, int K1, int K2,
-                                                const CX<T> *__restrict__ twg, T s, const double *__restrict__ ctl) {
+                                                const CX<T> *__restrict__ twg, T s, Bricks bk,
+                                                const T *__restrict__ bbuf, const double *__restrict__ ctl) {
   if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   constexpr int NL = CtLines<N>::x, SL = N + 1;
   __shared__ CX<T> buf[NL * SL];
@@ -499,7 +540,9 @@ __global__ void __launch_bounds__(CT_BS) k_ct_x(CX<T> *__restrict__ grid, int K1
     re[q] = (T)0;
     im[q] = (T)0;
     if (t < TOT && line < nlines) {
-      CX<T> v = grid[(size_t)line * N + e];
+      CX<T> v;
+      if constexpr (MODE == 0) v = grid[(size_t)line * N + e];
+      else v = brick_point<T, MODE>(bk, bbuf, e, line % K1, line / K1);
       re[q] = v.x;
       im[q] = v.y;
     }
@@ -623,6 +666,7 @@ static Frac frac_of(const NbHandle *h) {
 static Bricks bricks_of(const NbHandle *h) {
   Bricks b;
   for (int d = 0; d < 3; d++) { b.K[d] = h->K[d]; b.nb[d] = h->nbk[d]; }
+  b.ext = h->bext;
   return b;
 }
 
@@ -685,12 +729,12 @@ static void twiddles(const int K[3], std::vector<double> &tw) {
 
 static void free_grid(NbHandle *h) {
   void *ptrs[] = {h->d_grid, h->d_recip, h->d_recipf, h->d_tw, h->d_twf, h->d_bcount, h->d_bstart,
-                  h->d_fgrid};
+                  h->d_bbuf, h->d_fgrid};
   for (void *p : ptrs)
     if (p) hipFree(p);
   h->d_fgrid = nullptr;
   h->d_grid = nullptr; h->d_recip = nullptr; h->d_recipf = nullptr; h->d_tw = nullptr; h->d_twf = nullptr;
-  h->d_bcount = nullptr; h->d_bstart = nullptr;
+  h->d_bcount = nullptr; h->d_bstart = nullptr; h->d_bbuf = nullptr;
 }
 
 void nb_pme_setup(NbHandle *h) {
@@ -702,17 +746,20 @@ void nb_pme_setup(NbHandle *h) {
   long ntot = (long)K0 * K1 * K2;
   if (ntot != h->ntot || !h->d_grid) {
     free_grid(h);
-    // bricks: at least 2 per axis (an atom's stencil spans its bin and the
-    // next), edges of at most NB_BRICK_MAX and at least 4 points
-    int nbk[3];
+    // bricks: at least 2 per axis, edges of at most NB_BRICK_MAX and at
+    // least 4 points (the 4-point spill of a brick lands in the next brick
+    // only); the LDS brick of the spread holds the longest edge + 4
+    int nbk[3], ext = 0;
     for (int d = 0; d < 3; d++) {
       nbk[d] = std::max(2, (K[d] + NB_BRICK_MAX - 1) / NB_BRICK_MAX);
       for (int b = 0; b < nbk[d]; b++) {
         int len = brick_start(b + 1, K[d], nbk[d]) - brick_start(b, K[d], nbk[d]);
         if (len > NB_BRICK_MAX || len < 4) throw std::runtime_error("PME brick edge out of range");
+        ext = std::max(ext, len + 4);
       }
       h->nbk[d] = nbk[d];
     }
+    h->bext = ext;
     h->nbricks = nbk[0] * nbk[1] * nbk[2];
     HIPCHECK(hipMalloc(&h->d_grid, sizeof(double) * 2 * ntot));
     HIPCHECK(hipMalloc(&h->d_recip, sizeof(double) * ntot));
@@ -721,6 +768,7 @@ void nb_pme_setup(NbHandle *h) {
     HIPCHECK(hipMalloc(&h->d_twf, sizeof(float) * 2 * (K0 + K1 + K2)));
     HIPCHECK(hipMalloc(&h->d_bcount, sizeof(int32_t) * h->nbricks));
     HIPCHECK(hipMalloc(&h->d_bstart, sizeof(int32_t) * (h->nbricks + 1)));
+    HIPCHECK(hipMalloc(&h->d_bbuf, sizeof(double) * 2 * (size_t)h->nbricks * ext * ext * ext));
   }
   h->K[0] = K0; h->K[1] = K1; h->K[2] = K2;
   h->ntot = ntot;
@@ -764,14 +812,18 @@ void nb_pme_setup(NbHandle *h) {
 // runtime-N line kernels otherwise.  Each returns false when it has no
 // specialisation for the length.
 template <typename T>
-static bool ct_x(NbHandle *h, CX<T> *g, const CX<T> *tw, T s) {
+static bool ct_x(NbHandle *h, int mode, CX<T> *g, const CX<T> *tw, T s) {
   int K1 = h->K[1], K2 = h->K[2];
+  const Bricks bk = bricks_of(h);
+  const T *bb = (const T *)h->d_bbuf;
   switch (h->K[0]) {
-#define NB_X(N)                                                                                      \
-  case N: {                                                                                          \
-    dim3 G((K1 * K2 + CtLines<N>::x - 1) / CtLines<N>::x);                                           \
-    NB_LAUNCH((k_ct_x<T, N>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, h->ctl);                                \
-    return true;                                                                                     \
+#define NB_X(N)                                                                                       \
+  case N: {                                                                                           \
+    dim3 G((K1 * K2 + CtLines<N>::x - 1) / CtLines<N>::x);                                            \
+    if (mode == 0) NB_LAUNCH((k_ct_x<T, N, 0>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, bk, bb, h->ctl); \
+    else if (mode == 1) NB_LAUNCH((k_ct_x<T, N, 1>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, bk, bb, h->ctl); \
+    else NB_LAUNCH((k_ct_x<T, N, 2>), G, dim3(CT_BS), 0, g, K1, K2, tw, s, bk, bb, h->ctl); \
+    return true;                                                                                      \
   }
     NB_CT_SIZES(NB_X)
 #undef NB_X
@@ -812,12 +864,14 @@ static bool ct_z(NbHandle *h, CX<T> *g, const CX<T> *tw, const T *recip) {
   }
 }
 
-// forward x, forward y, fused z (forward * influence * inverse), inverse y,
-// inverse x
+// forward x (assembling the grid from the bricks when np > 0), forward y,
+// fused z (forward * influence * inverse), inverse y, inverse x
 template <typename T>
-static void transform_t(NbHandle *h, CX<T> *g, const CX<T> *tw0, const T *recip) {
+static void transform_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0, const T *recip) {
   int K0 = h->K[0], K1 = h->K[1], K2 = h->K[2];
   const CX<T> *twx = tw0, *twy = twx + K0, *twz = twy + K1;
+  const Bricks bk = bricks_of(h);
+  const T *bb = (const T *)h->d_bbuf;
   int Lx = lines_per_block(K0), Ly = lines_per_block(K1), Lz = lines_per_block(K2);
   size_t shx = sizeof(CX<T>) * ((size_t)Lx * K0 + K0), shy = sizeof(CX<T>) * ((size_t)Ly * K1 + K1);
   size_t shz = sizeof(CX<T>) * ((size_t)Lz * K2 + K2);
@@ -826,44 +880,53 @@ static void transform_t(NbHandle *h, CX<T> *g, const CX<T> *tw0, const T *recip)
   const T *nul = nullptr;
   for (int pass = 0; pass < 2; pass++) {
     T s = pass == 0 ? (T)1 : (T)-1;
-    if (pass == 0 && !ct_x<T>(h, g, twx, s))
+    if (pass == 0 && !ct_x<T>(h, np, g, twx, s)) {
+      if (np == 1) NB_LAUNCH((k_brick_combine<T, 1>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g, h->ctl);
+      if (np == 2) NB_LAUNCH((k_brick_combine<T, 2>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g, h->ctl);
       NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul, h->ctl);
+    }
     if (!ct_y<T>(h, g, twy, s))
       NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, s, nul, h->ctl);
     if (pass == 0) {
       if (!ct_z<T>(h, g, twz, recip))
         NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, true>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, recip, h->ctl);
-    } else if (!ct_x<T>(h, g, twx, s)) {
+    } else if (!ct_x<T>(h, 0, g, twx, s)) {
       NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, s, nul, h->ctl);
     }
   }
 }
 
-// forward transform only, used for the strain derivative
+// forward transform only (x assembled from the bricks when np > 0), used for
+// the strain derivative
 template <typename T>
-static void forward_t(NbHandle *h, CX<T> *g, const CX<T> *tw0) {
+static void forward_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0) {
   int K0 = h->K[0], K1 = h->K[1], K2 = h->K[2];
   const CX<T> *twx = tw0, *twy = twx + K0, *twz = twy + K1;
+  const Bricks bk = bricks_of(h);
+  const T *bb = (const T *)h->d_bbuf;
   int Lx = lines_per_block(K0), Ly = lines_per_block(K1), Lz = lines_per_block(K2);
   size_t shx = sizeof(CX<T>) * ((size_t)Lx * K0 + K0), shy = sizeof(CX<T>) * ((size_t)Ly * K1 + K1);
   size_t shz = sizeof(CX<T>) * ((size_t)Lz * K2 + K2);
   dim3 B(256), Gx((K1 * K2 + Lx - 1) / Lx), Gy(((K0 + Ly - 1) / Ly) * K2), Gz((K0 * K1 + Lz - 1) / Lz);
   const T *nul = nullptr;
-  if (!ct_x<T>(h, g, twx, (T)1))
+  if (!ct_x<T>(h, np, g, twx, (T)1)) {
+    if (np == 1) NB_LAUNCH((k_brick_combine<T, 1>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g, h->ctl);
+    if (np == 2) NB_LAUNCH((k_brick_combine<T, 2>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g, h->ctl);
     NB_LAUNCH((k_fft_lines<T, LINE_EPT, 0, false>), Gx, B, shx, g, K0, K1, K2, Lx, factor(K0), twx, (T)1, nul, h->ctl);
+  }
   if (!ct_y<T>(h, g, twy, (T)1))
     NB_LAUNCH((k_fft_lines<T, LINE_EPT, 1, false>), Gy, B, shy, g, K0, K1, K2, Ly, factor(K1), twy, (T)1, nul, h->ctl);
   if (!ct_z<T>(h, g, twz, nul))
     NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, false>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, nul, h->ctl);
 }
 
-// Spread into a grid (the owned points of every brick; IND: u and uCR of the
-// dipoles u3 / up3, otherwise the permanent fractional multipoles); theta of
-// type T unless given
+// Spread into the brick buffer (IND: u and uCR of the dipoles u3 / up3,
+// otherwise the permanent fractional multipoles); theta of type T unless
+// given.  The next forward transform (np = 1, or 2 with IND) assembles the grid.
 template <typename T, bool IND>
-static void spread_into(NbHandle *h, CX<T> *grid, const double *u, const double *up, const T *theta = nullptr) {
-  NB_LAUNCH((k_spread_own<T, IND>), dim3(h->nbricks), dim3(256), 0, h->n, bricks_of(h), h->d_bstart, h->d_batom,
-            theta ? theta : (const T *)h->d_theta, h->d_igrid, h->d_fmp, u, up, frac_of(h), grid, h->ctl);
+static void spread_into(NbHandle *h, const double *u, const double *up, const T *theta = nullptr) {
+  NB_LAUNCH((k_spread_brick<T, IND>), dim3(h->nbricks), dim3(256), 0, h->n, bricks_of(h), h->d_bstart, h->d_batom,
+            theta ? theta : (const T *)h->d_theta, h->d_igrid, h->d_fmp, u, up, frac_of(h), (T *)h->d_bbuf, h->ctl);
 }
 
 // Reciprocal strain derivative, grid part: at every wave vector m the energy
@@ -909,11 +972,11 @@ __global__ void __launch_bounds__(256) k_grid_virial(int K0, int K1, int K2, con
 template <typename T>
 static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
   CX<T> *g = (CX<T> *)h->d_grid, *fpg = (CX<T> *)h->d_fgrid;
-  spread_into<T, false>(h, fpg, nullptr, nullptr);
-  forward_t<T>(h, fpg, tw);
+  spread_into<T, false>(h, nullptr, nullptr);
+  forward_t<T>(h, 1, fpg, tw);
   if (polar) {
-    spread_into<T, true>(h, g, h->d_u, h->d_up);
-    forward_t<T>(h, g, tw);
+    spread_into<T, true>(h, h->d_u, h->d_up);
+    forward_t<T>(h, 2, g, tw);
   }
   Frac2v A;
   for (int a = 0; a < 3; a++)
@@ -927,16 +990,17 @@ static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
 
 static bool mixed_grid(const NbHandle *h) { return h->prm.precision == NB_PREC_MIXED; }
 
-static void convolution(NbHandle *h) {
+// np = 0: the grid already holds the data; np = 1, 2: it is assembled from the bricks
+static void convolution(NbHandle *h, int np) {
   if (mixed_grid(h))
-    transform_t<float>(h, (CX<float> *)h->d_grid, (const CX<float> *)h->d_twf, h->d_recipf);
+    transform_t<float>(h, np, (CX<float> *)h->d_grid, (const CX<float> *)h->d_twf, h->d_recipf);
   else
-    transform_t<double>(h, (CX<double> *)h->d_grid, (const CX<double> *)h->d_tw, h->d_recip);
+    transform_t<double>(h, np, (CX<double> *)h->d_grid, (const CX<double> *)h->d_tw, h->d_recip);
 }
 
 template <typename T, bool IND>
 static void spread_t(NbHandle *h, const double *u, const double *up) {
-  spread_into<T, IND>(h, (CX<T> *)h->d_grid, u, up);
+  spread_into<T, IND>(h, u, up);
 }
 
 template <typename T, int NP, bool FULL>
@@ -962,11 +1026,11 @@ void nb_pme_permanent(NbHandle *h) {
   if (mixed_grid(h)) {
     if (h->refine && h->prm.polarization == NB_POL_MUTUAL) nb_pme_theta64(h);
     spread_t<float, false>(h, nullptr, nullptr);
-    convolution(h);
+    convolution(h, 1);
     gather_t<float, 1, true>(h, h->d_phip, nullptr);
   } else {
     spread_t<double, false>(h, nullptr, nullptr);
-    convolution(h);
+    convolution(h, 1);
     gather_t<double, 1, true>(h, h->d_phip, nullptr);
   }
 }
@@ -984,8 +1048,8 @@ void nb_pme_theta64(NbHandle *h) {
 // FP64 transforms and influence function, FP64 gather
 void nb_pme_induced64(NbHandle *h, const double *u, const double *up) {
   CX<double> *g = (CX<double> *)h->d_grid;
-  spread_into<double, true>(h, g, u, up, h->d_theta64);
-  transform_t<double>(h, g, (const CX<double> *)h->d_tw, h->d_recip);
+  spread_into<double, true>(h, u, up, h->d_theta64);
+  transform_t<double>(h, 2, g, (const CX<double> *)h->d_tw, h->d_recip);
   NB_LAUNCH((k_gather<double, 2, true>), xcd_grid((long)h->n * 8, 256), dim3(256), 0, h->n,
             (const double *)h->d_theta64, h->d_igrid, (const CX<double> *)g, frac_of(h), h->K[0], h->K[1], h->K[2],
             h->d_phiu, h->d_phiup, h->ctl);
@@ -994,12 +1058,12 @@ void nb_pme_induced64(NbHandle *h, const double *u, const double *up) {
 void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full) {
   if (mixed_grid(h)) {
     spread_t<float, true>(h, u, up);
-    convolution(h);
+    convolution(h, 2);
     if (full) gather_t<float, 2, true>(h, h->d_phiu, h->d_phiup);
     else gather_t<float, 2, false>(h, h->d_phiu, h->d_phiup);
   } else {
     spread_t<double, true>(h, u, up);
-    convolution(h);
+    convolution(h, 2);
     if (full) gather_t<double, 2, true>(h, h->d_phiu, h->d_phiup);
     else gather_t<double, 2, false>(h, h->d_phiu, h->d_phiup);
   }
@@ -1014,7 +1078,7 @@ int nb_pme_pcg_direction(NbHandle *h, double a3, int nrz, int first) {
   auto gap = [&](auto tag) {
     using T = decltype(tag);
     spread_t<T, true>(h, h->d_z, h->d_zp);
-    convolution(h);
+    convolution(h, 2);
     nb_join(h);
     NB_LAUNCH((k_gather_ap<T>), G, dim3(256), 0, h->n, (const T *)h->d_theta, h->d_igrid, (const CX<T> *)h->d_grid, fr,
               h->K[0], h->K[1], h->K[2], h->d_phiu, h->d_phiup, a3, h->d_polar, h->d_z, h->d_zp, h->d_field,
@@ -1062,14 +1126,14 @@ extern "C" int nb_test_convolution(double *data, const double *recip, int nx, in
       std::vector<float> gf(data, data + 2 * ntot), rf(recip, recip + ntot);
       HIPCHECK(hipMemcpy(h.d_grid, gf.data(), sizeof(float) * 2 * ntot, hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(h.d_recipf, rf.data(), sizeof(float) * ntot, hipMemcpyHostToDevice));
-      convolution(&h);
+      convolution(&h, 0);
       HIPCHECK(hipStreamSynchronize(h.stream));
       HIPCHECK(hipMemcpy(gf.data(), h.d_grid, sizeof(float) * 2 * ntot, hipMemcpyDeviceToHost));
       for (long i = 0; i < 2 * ntot; i++) data[i] = gf[i];
     } else {
       HIPCHECK(hipMemcpy(h.d_grid, data, sizeof(double) * 2 * ntot, hipMemcpyHostToDevice));
       HIPCHECK(hipMemcpy(h.d_recip, recip, sizeof(double) * ntot, hipMemcpyHostToDevice));
-      convolution(&h);
+      convolution(&h, 0);
       HIPCHECK(hipStreamSynchronize(h.stream));
       HIPCHECK(hipMemcpy(data, h.d_grid, sizeof(double) * 2 * ntot, hipMemcpyDeviceToHost));
     }

@@ -121,10 +121,13 @@ struct NbHandle {
   float *d_recipf = nullptr;      // influence function, FP32 copy (mixed form)
   float *d_twf = nullptr;         // twiddle tables, FP32 copy
   // grid bricks: bricks along axis d cover [b K_d / nbk_d, (b+1) K_d / nbk_d);
-  // each atom is binned by its first B-spline point, and the spread workgroup
-  // of a brick sums the stencils of its own bin and the previous bins
+  // each atom is binned by its first B-spline point; the spread workgroup of a
+  // brick sums the stencils of its bin into the brick and its 4-point spill
+  // (edge bext), and the x transform assembles every point from its brick
+  // and the spills of the previous bricks
   int nbk[3] = {0, 0, 0};
-  int nbricks = 0;
+  int nbricks = 0, bext = 0;
+  double *d_bbuf = nullptr;       // [nbricks][2][bext^3] brick buffer (grid real type)
   int32_t *d_bcount = nullptr;    // [nbricks] atoms per brick
   int32_t *d_bstart = nullptr;    // [nbricks + 1] exclusive scan
   int32_t *d_bid = nullptr;       // [n] brick of each sorted atom
