@@ -9,8 +9,8 @@
 //  * k_spread_brick: the SpatialPermanentLoop / SpatialInducedLoop density
 //    (ReciprocalSpace.java:1445-1643).  Instead of the reference's 8-colour
 //    octant decomposition, one workgroup owns one brick of the grid: it
-//    accumulates the 125-point stencils of its atoms in LDS (64-bit
-//    fixed-point sums, independent of the order of the LDS atomics) and
+//    accumulates the 125-point stencils of its atoms in LDS (exact sums of
+//    quantized values, independent of the order of the LDS atomics) and
 //    writes the brick, including its 4-point spill into the next bricks, to a
 //    brick buffer; the forward x transform sums, for every grid point, its
 //    own brick and the spill of up to 7 neighbours into the interleaved
@@ -202,17 +202,18 @@ __device__ __forceinline__ double spread_bound(const double *__restrict__ fmp, c
 // fixed order.  Thread (atom, ky, kz) forms the y-z weight and adds 5
 // consecutive x points.
 //
-// The LDS sums are fixed point in 64-bit integers (integer LDS atomics), so
-// they do not depend on the order in which the atomics land and every
-// evaluation of the same input gives the same grid, in the FP32 grid of the
-// mixed form as in the FP64 form (a floating-point sum rounds differently
-// with the order, and one FP32 ulp of a grid point moves the solved dipoles
-// by up to ~1e-7).  The scale is chosen per brick and part: a point receives
-// at most one value per atom, so with na atoms in the bin and bmax the
-// largest bound of their values (spread_bound) every partial sum stays below
-// na bmax < 2^e; the quantum 2^(e-62) keeps the sums within int64 (a factor 2
-// to spare for the rounding of the values) with 62 - log2(na) significant
-// bits against the largest value.
+// The LDS sums are exact, so they do not depend on the order in which the
+// atomics land and every evaluation of the same input gives the same grid,
+// in the FP32 grid of the mixed form as in the FP64 form (a rounded
+// floating-point sum changes with the order, and one FP32 ulp of a grid point
+// moves the solved dipoles by up to ~1e-7).  Every stencil value is rounded
+// to a multiple of a quantum before its FP64 LDS atomic add; while all
+// partial sums stay below 2^53 quanta, FP64 adds of such multiples are exact.
+// The quantum is chosen per brick and part: a point receives at most one
+// value per atom, so with na atoms in the bin and bmax the largest bound of
+// their values (spread_bound) every partial sum stays below na bmax < 2^e,
+// and the quantum 2^(e-52) keeps it below 2^52 (+ na / 2 of rounding) quanta,
+// with 52 - log2(na) significant bits against the largest value.
 template <typename T, bool IND>
 __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const int32_t *__restrict__ bstart,
                                                       const int32_t *__restrict__ batom, const T *__restrict__ theta,
@@ -222,12 +223,12 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
                                                       const double *__restrict__ ctl) {
   if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
   constexpr int NP = IND ? 2 : 1, EM = NB_BRICK_EXT * NB_BRICK_EXT * NB_BRICK_EXT;
-  __shared__ unsigned long long lds[NP][EM];
+  __shared__ double lds[NP][EM];
   __shared__ double bmax_w[NP][4];
   const int e = bk.ext, e3 = e * e * e;
   const int b = blockIdx.x;
   const int a0 = bstart[b], na = min(bstart[b + 1], n) - a0;
-  for (int t = threadIdx.x; t < NP * EM; t += blockDim.x) (&lds[0][0])[t] = 0ull;
+  for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) lds[t / e3][t % e3] = 0.0;
   double bm[NP];
 #pragma unroll
   for (int p = 0; p < NP; p++) bm[p] = 0.0;
@@ -243,13 +244,14 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
     if ((threadIdx.x & 63) == 0) bmax_w[p][threadIdx.x >> 6] = bm[p];
   }
   __syncthreads();
-  double qs[NP];   // 2^(62 - e): the inverse of the quantum
+  double qs[NP], qi[NP];   // 2^(52 - e) and the quantum 2^(e - 52)
 #pragma unroll
   for (int p = 0; p < NP; p++) {
     const double m = fmax(fmax(bmax_w[p][0], bmax_w[p][1]), fmax(bmax_w[p][2], bmax_w[p][3])) * (double)na;
     int ex = 0;
     if (m > 0.0) frexp(m, &ex);
-    qs[p] = ldexp(1.0, 62 - ex);
+    qs[p] = ldexp(1.0, 52 - ex);
+    qi[p] = ldexp(1.0, ex - 52);
   }
   for (int q = threadIdx.x; q < na * 25; q += blockDim.x) {
     const int a = q / 25, yz = q - a * 25, ky = yz % 5, kz = yz / 5;
@@ -294,13 +296,13 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
     for (int p = 0; p < NP; p++)
 #pragma unroll
       for (int kx = 0; kx < 5; kx++)
-        atomicAdd(&lds[p][base + kx], (unsigned long long)__double2ll_rn((double)val[p][kx] * qs[p]));
+        atomicAdd(&lds[p][base + kx], rint((double)val[p][kx] * qs[p]));
   }
   __syncthreads();
   T *out = bbuf + (size_t)b * NP * e3;
   for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) {
     const int p = t / e3, l = t - p * e3;
-    out[t] = (T)((double)(long long)lds[p][l] / qs[p]);
+    out[t] = (T)(lds[p][l] * qi[p]);
   }
 }
 

@@ -301,7 +301,8 @@ void nb_destroy(void *handle);
  * a replay whose list needs rebuilding, whose filtered rows overflow or whose
  * solve did not stop within the budget is repeated on the regular path and
  * recorded again; so is one whose solve stopped two or more iterations short of
- * the budget (the surplus iterations' transforms would still run).
+ * the budget (the surplus iterations' kernels still launch, and return at
+ * once).
  * nb_set_cell, nb_set_lambda and list rebuilds drop the graph.
  * Results equal those of the regular path. */
 int nb_set_graph(void *handle, int on);
@@ -337,7 +338,9 @@ int nb_batch_config(int device, int32_t cfg[4], double *kernel_ms);
  * with the FP64 residual, up to three times.  The FP64 reciprocal potentials
  * of that last residual are the ones the energy and gradient use;
  * nb_energies.pcg_eps reports the FP64 eps and pcg_iters counts the restarts'
- * iterations too.  A solve that needed restarts is not recorded into a graph.
+ * iterations too.  In graph mode the recorded evaluation carries the restarts
+ * its solve took (at least one), each with its iteration count + 1 as budget;
+ * a replay whose FP64 residual passes earlier skips the rest.
  * on = 0 turns it off (the FP32-operator solve stands); NB_ERR_INVALID for an
  * FP64 handle. */
 int nb_set_refinement(void *handle, int on);
