@@ -1135,6 +1135,22 @@ int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energi
   return status;
 }
 
+int nb_evaluate_batch_static(void *const *handles, int count, uint32_t flags, nb_energies *out) {
+  if (!handles || count <= 0) return NB_ERR_INVALID;
+  for (int b = 0; b < count; b++)
+    if (!handles[b]) return NB_ERR_INVALID;
+  NbHandle *h0 = (NbHandle *)handles[0];
+  if (flags & ~(uint32_t)NB_GRAD) {
+    h0->err = "nb_evaluate_batch_static: flags may hold at most NB_GRAD";
+    return NB_ERR_INVALID;
+  }
+  try {
+    return nb_batch_evaluate_static((NbHandle **)handles, count, flags, out);
+  } catch (std::exception &ex) {
+    return fail(h0, ex, NB_ERR_HIP);
+  }
+}
+
 int nb_batch_config(int device, int32_t cfg[4], double *kernel_ms) {
   if (!cfg) return NB_ERR_INVALID;
   try {

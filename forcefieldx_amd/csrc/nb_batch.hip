@@ -1158,6 +1158,35 @@ struct Pool {
 std::mutex g_pools_mu;
 std::map<int, Pool *> g_pools;
 
+// The device buffers one batch launch uses (the pool's, or the static scratch
+// of the allocation-free entry point) and its host mirrors.
+struct BatchBuf {
+  char *ws = nullptr;
+  BatchSys *d_sys = nullptr, *h_sys = nullptr;
+  double *d_out = nullptr, *h_out = nullptr;
+  int *d_next = nullptr;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  int (*occ)[kNumShapes] = nullptr;   // occupancy cache [form][shape]
+  int shape = -1;                     // >= 0: this shape, no tuner
+  Pool *tune = nullptr;               // the tuner's choices and timing events
+  int used_shape = 0, used_blocks = 0;
+};
+
+// workspace bytes of the systems in queue order, and each one's offset
+template <typename T>
+static size_t batch_ws_bytes(std::vector<NbHandle *> &hs, const std::vector<int> &order, std::vector<size_t> &off) {
+  size_t total = 0;
+  off.assign(hs.size(), 0);
+  for (size_t q = 0; q < hs.size(); q++) {
+    NbHandle *h = hs[order[q]];
+    long ntot = h->prm.do_pme ? (long)h->K[0] * h->K[1] * h->K[2] : 1;
+    off[q] = total;
+    total += ws_map<T>(nullptr, h->n, h->fmaxa, h->fmaxb, ntot, nullptr);
+  }
+  return total;
+}
+
 static Pool *pool_of(int device) {
   std::lock_guard<std::mutex> lk(g_pools_mu);
   auto it = g_pools.find(device);
@@ -1221,19 +1250,12 @@ static int row_estimate(const NbHandle *h, double r) {
   return (int)std::min(est, (double)h->n * 343.0);
 }
 
+// grow the pool's buffers to the batch and point b at them
 template <typename T>
-static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::vector<int> &order,
-                     std::vector<char> &retry) {
+static void pool_buffers(Pool *pl, std::vector<NbHandle *> &hs, const std::vector<int> &order,
+                         std::vector<size_t> &off, BatchBuf &b) {
   int nsys = (int)hs.size();
-  // descriptors in queue order (largest work first)
-  size_t total = 0;
-  std::vector<size_t> off(nsys);
-  for (int q = 0; q < nsys; q++) {
-    NbHandle *h = hs[order[q]];
-    long ntot = h->prm.do_pme ? (long)h->K[0] * h->K[1] * h->K[2] : 1;
-    off[q] = total;
-    total += ws_map<T>(nullptr, h->n, h->fmaxa, h->fmaxb, ntot, nullptr);
-  }
+  size_t total = batch_ws_bytes<T>(hs, order, off);
   if (total > pl->ws_cap) {
     if (pl->ws) HIPCHECK(hipFree(pl->ws));
     pl->ws = nullptr;
@@ -1251,10 +1273,29 @@ static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::ve
     HIPCHECK(hipMalloc(&pl->d_out, sizeof(double) * O_N * pl->sys_cap));
     HIPCHECK(hipHostMalloc((void **)&pl->h_out, sizeof(double) * O_N * pl->sys_cap, hipHostMallocDefault));
   }
+  b.ws = pl->ws;
+  b.d_sys = pl->d_sys;
+  b.h_sys = pl->h_sys;
+  b.d_out = pl->d_out;
+  b.h_out = pl->h_out;
+  b.d_next = pl->d_next;
+  b.stream = pl->stream;
+  b.cus = pl->cus;
+  b.occ = pl->occ;
+  b.tune = pl;
+}
+
+// One launch of the batch kernel over the systems in queue order, with the
+// buffers of b (workspace offsets off).  Returns 1 when some rows overflowed
+// (their capacities are raised; the batch must be repeated).
+template <typename T>
+static int run_batch(BatchBuf &bf, std::vector<NbHandle *> &hs, BatchPrm &P, std::vector<int> &order,
+                     const std::vector<size_t> &off, std::vector<char> &retry) {
+  int nsys = (int)hs.size();
   const bool mixed = sizeof(T) == 4;
   for (int q = 0; q < nsys; q++) {
     NbHandle *h = hs[order[q]];
-    BatchSys &S = pl->h_sys[q];
+    BatchSys &S = bf.h_sys[q];
     memset(&S, 0, sizeof(S));
     S.n = h->n;
     S.K0 = h->prm.do_pme ? h->K[0] : 1;
@@ -1296,75 +1337,85 @@ static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::ve
     S.grad = h->d_grad;
     S.u = h->d_u;
     S.up = h->d_up;
-    S.ws = pl->ws + off[q];
+    S.ws = bf.ws + off[q];
   }
-  HIPCHECK(hipMemcpyAsync(pl->d_sys, pl->h_sys, sizeof(BatchSys) * nsys, hipMemcpyHostToDevice, pl->stream));
+  HIPCHECK(hipMemcpyAsync(bf.d_sys, bf.h_sys, sizeof(BatchSys) * nsys, hipMemcpyHostToDevice, bf.stream));
   // the handles' own streams may still hold position uploads
   for (NbHandle *h : hs) {
     hipError_t q = hipStreamQuery(h->stream);
     if (q == hipErrorNotReady) {
-      HIPCHECK(hipEventRecord(pl->ready, h->stream));
-      HIPCHECK(hipStreamWaitEvent(pl->stream, pl->ready, 0));
+      HIPCHECK(hipEventRecord(h->ev_join, h->stream));
+      HIPCHECK(hipStreamWaitEvent(bf.stream, h->ev_join, 0));
     } else if (q != hipSuccess) {
       HIPCHECK(q);
     }
   }
   const int ti = mixed ? 1 : 0;
   auto blocks_for = [&](int v) {
-    if (!pl->occ[ti][v]) {
+    if (!bf.occ[ti][v]) {
       int per = 0;
       HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel_of<T>(v), kShapes[v].nt, 0));
-      pl->occ[ti][v] = std::max(per, 1);
+      bf.occ[ti][v] = std::max(per, 1);
     }
-    return std::max(1, std::min(nsys, pl->occ[ti][v] * pl->cus));
+    return std::max(1, std::min(nsys, bf.occ[ti][v] * bf.cus));
   };
-  // tuner: the first batch of a signature times every shape once (after a
-  // warm-up launch) and keeps the fastest
-  long natoms = 0;
-  for (NbHandle *h : hs) natoms += h->n;
-  std::string sig = std::to_string(ti) + ":" + std::to_string(nsys) + ":" + std::to_string(natoms);
-  auto it = pl->tuned.find(sig);
-  int best;
-  if (it == pl->tuned.end()) {
-    float bt = 0.0f;
-    best = 0;
-    for (int v = 0; v < kNumShapes; v++) {
-      launch_shape<T>(v, blocks_for(v), pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
-      HIPCHECK(hipEventRecord(pl->e0, pl->stream));
-      launch_shape<T>(v, blocks_for(v), pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
-      HIPCHECK(hipEventRecord(pl->e1, pl->stream));
-      HIPCHECK(hipEventSynchronize(pl->e1));
-      float ms = 0.0f;
-      HIPCHECK(hipEventElapsedTime(&ms, pl->e0, pl->e1));
-      if (v == 0 || ms < bt) {
-        bt = ms;
-        best = v;
+  int best = bf.shape;
+  if (best < 0) {
+    // tuner: the first batch of a signature times every shape once (after a
+    // warm-up launch) and keeps the fastest
+    Pool *pl = bf.tune;
+    long natoms = 0;
+    for (NbHandle *h : hs) natoms += h->n;
+    std::string sig = std::to_string(ti) + ":" + std::to_string(nsys) + ":" + std::to_string(natoms);
+    auto it = pl->tuned.find(sig);
+    if (it == pl->tuned.end()) {
+      float bt = 0.0f;
+      best = 0;
+      for (int v = 0; v < kNumShapes; v++) {
+        launch_shape<T>(v, blocks_for(v), bf.stream, bf.d_sys, nsys, P, bf.d_out, bf.d_next);
+        HIPCHECK(hipEventRecord(pl->e0, bf.stream));
+        launch_shape<T>(v, blocks_for(v), bf.stream, bf.d_sys, nsys, P, bf.d_out, bf.d_next);
+        HIPCHECK(hipEventRecord(pl->e1, bf.stream));
+        HIPCHECK(hipEventSynchronize(pl->e1));
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, pl->e0, pl->e1));
+        if (v == 0 || ms < bt) {
+          bt = ms;
+          best = v;
+        }
       }
+      pl->tuned[sig] = best;
+    } else {
+      best = it->second;
     }
-    pl->tuned[sig] = best;
-  } else {
-    best = it->second;
   }
   int blocks = blocks_for(best);
-  HIPCHECK(hipEventRecord(pl->e0, pl->stream));
-  launch_shape<T>(best, blocks, pl->stream, pl->d_sys, nsys, P, pl->d_out, pl->d_next);
-  HIPCHECK(hipEventRecord(pl->e1, pl->stream));
-  pl->last[0] = kShapes[best].nt;
-  pl->last[1] = kShapes[best].lpa;
-  pl->last[2] = blocks;
-  pl->last[3] = nsys;
-  HIPCHECK(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(double) * O_N * nsys, hipMemcpyDeviceToHost, pl->stream));
-  HIPCHECK(hipStreamSynchronize(pl->stream));
-  {
+  bf.used_shape = best;
+  bf.used_blocks = blocks;
+  if (bf.tune) {
+    Pool *pl = bf.tune;
+    HIPCHECK(hipEventRecord(pl->e0, bf.stream));
+    launch_shape<T>(best, blocks, bf.stream, bf.d_sys, nsys, P, bf.d_out, bf.d_next);
+    HIPCHECK(hipEventRecord(pl->e1, bf.stream));
+    pl->last[0] = kShapes[best].nt;
+    pl->last[1] = kShapes[best].lpa;
+    pl->last[2] = blocks;
+    pl->last[3] = nsys;
+  } else {
+    launch_shape<T>(best, blocks, bf.stream, bf.d_sys, nsys, P, bf.d_out, bf.d_next);
+  }
+  HIPCHECK(hipMemcpyAsync(bf.h_out, bf.d_out, sizeof(double) * O_N * nsys, hipMemcpyDeviceToHost, bf.stream));
+  HIPCHECK(hipStreamSynchronize(bf.stream));
+  if (bf.tune) {
     float ms = 0.0f;
-    HIPCHECK(hipEventElapsedTime(&ms, pl->e0, pl->e1));
-    pl->last_ms = ms;
+    HIPCHECK(hipEventElapsedTime(&ms, bf.tune->e0, bf.tune->e1));
+    bf.tune->last_ms = ms;
   }
   // rows that overflowed get larger capacities and the batch is repeated
   int grow = 0;
   for (int q = 0; q < nsys; q++) {
     NbHandle *h = hs[order[q]];
-    const double *o = pl->h_out + (size_t)O_N * q;
+    const double *o = bf.h_out + (size_t)O_N * q;
     int fl = (int)o[O_FLAGS];
     if (fl & F_ROWS) {
       h->fmaxa = std::max(h->fmaxa, (int)o[O_ROWA] + (int)o[O_ROWA] / 4 + 8);
@@ -1378,22 +1429,12 @@ static int run_batch(Pool *pl, std::vector<NbHandle *> &hs, BatchPrm &P, std::ve
 
 }  // namespace
 
-// Evaluate the handles with the batch kernel.  Returns false (nothing done)
-// when some handle does not qualify; otherwise fills out / the handles, and
-// marks in `separate` the handles whose evaluation must be repeated on the
-// regular path (a failed solve, which the regular path hands to SOR, or a
-// list cutoff beyond the checked image range).
-bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
-                       int &status) {
-  if (count <= 0) return false;
-  std::vector<NbHandle *> hs(hv, hv + count);
-  for (NbHandle *h : hs)
-    if (!h || !batchable(h, flags) || h->device != hs[0]->device || !same_params(h->prm, hs[0]->prm)) return false;
+namespace {
+
+// parameters of a batch, row capacities of new handles, and the queue order
+// (largest work first: pairs ~ n^2 / V, grid ~ K^3)
+static void batch_setup(std::vector<NbHandle *> &hs, BatchPrm &P, std::vector<int> &order) {
   const nb_params &pr = hs[0]->prm;
-  HIPCHECK(hipSetDevice(hs[0]->device));
-  Pool *pl = pool_of(hs[0]->device);
-  std::lock_guard<std::mutex> lk(pl->mu);
-  BatchPrm P;
   memset(&P, 0, sizeof(P));
   P.sc = scale_tab(hs[0]);
   P.aewald = pr.ewald_alpha;
@@ -1416,8 +1457,8 @@ bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *ou
     if (h->fmaxa <= 0) h->fmaxa = row_estimate(h, std::min(pr.precond_cutoff, pr.ewald_cutoff));
     if (h->fmaxb <= 0) h->fmaxb = row_estimate(h, lcut);
   }
-  // queue order: largest work first (pairs ~ n^2 / V, grid ~ K^3)
-  std::vector<int> order(count);
+  const int count = (int)hs.size();
+  order.resize(count);
   std::vector<double> cost(count);
   for (int b = 0; b < count; b++) {
     const NbHandle *h = hs[b];
@@ -1426,18 +1467,21 @@ bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *ou
     order[b] = b;
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+}
+
+// energies of the systems from the launch's outputs (queue order); systems
+// the batch kernel hands back (a failed solve, a cutoff beyond the checked
+// image range) are marked in `separate`
+static void batch_results(std::vector<NbHandle *> &hs, const BatchPrm &P, const std::vector<int> &order,
+                          const double *h_out, nb_energies *out, std::vector<char> &separate, int &status) {
+  const nb_params &pr = hs[0]->prm;
+  const int count = (int)hs.size();
   separate.assign(count, 0);
-  for (int attempt = 0;; attempt++) {
-    int grow = pr.precision == NB_PREC_MIXED ? run_batch<float>(pl, hs, P, order, separate)
-                                              : run_batch<double>(pl, hs, P, order, separate);
-    if (!grow) break;
-    if (attempt == 2) throw std::runtime_error("batch row capacity could not be established");
-  }
   status = NB_OK;
   for (int q = 0; q < count; q++) {
     int b = order[q];
     NbHandle *h = hs[b];
-    const double *o = pl->h_out + (size_t)O_N * q;
+    const double *o = h_out + (size_t)O_N * q;
     int st = (int)o[O_STAT], fl = (int)o[O_FLAGS];
     if (st != NB_OK || (fl & F_CUTOFF)) {
       separate[b] = 1;
@@ -1479,7 +1523,141 @@ bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *ou
     h->last = e;
     if (out) out[b] = e;
   }
+}
+
+static bool batch_qualifies(std::vector<NbHandle *> &hs, uint32_t flags) {
+  for (NbHandle *h : hs)
+    if (!h || !batchable(h, flags) || h->device != hs[0]->device || !same_params(h->prm, hs[0]->prm)) return false;
   return true;
+}
+
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free entry point (nb_evaluate_batch_static):
+// one launch's workspace, system descriptors, outputs and queue counter.  Each
+// GPU the library runs on holds its own copy until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress gives on that
+// device (scratch_base), never through the array's name.
+// ---------------------------------------------------------------------------
+constexpr size_t NB_SCRATCH_BYTES = (size_t)NB_BATCH_SCRATCH_MIB << 20;
+static __device__ __align__(256) char scratch[NB_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;          // guards g_scratch
+std::mutex g_scratch_run_mu;      // one allocation-free batch at a time in the process
+std::map<int, char *> g_scratch;   // device -> address of its scratch
+
+// the device's scratch address (looked up once per device, with it current)
+static char *scratch_base(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  if (it != g_scratch.end()) return it->second;
+  char *ptr = nullptr;
+  HIPCHECK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw std::runtime_error("scratch symbol has no device address");
+  g_scratch[device] = ptr;
+  return ptr;
+}
+
+static size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+// Evaluate the handles with the batch kernel.  Returns false (nothing done)
+// when some handle does not qualify; otherwise fills out / the handles, and
+// marks in `separate` the handles whose evaluation must be repeated on the
+// regular path (a failed solve, which the regular path hands to SOR, or a
+// list cutoff beyond the checked image range).
+bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
+                       int &status) {
+  if (count <= 0) return false;
+  std::vector<NbHandle *> hs(hv, hv + count);
+  if (!batch_qualifies(hs, flags)) return false;
+  HIPCHECK(hipSetDevice(hs[0]->device));
+  Pool *pl = pool_of(hs[0]->device);
+  std::lock_guard<std::mutex> lk(pl->mu);
+  BatchPrm P;
+  std::vector<int> order;
+  batch_setup(hs, P, order);
+  const bool mixed = hs[0]->prm.precision == NB_PREC_MIXED;
+  std::vector<char> retry(count, 0);
+  std::vector<size_t> off;
+  BatchBuf bf;
+  for (int attempt = 0;; attempt++) {
+    int grow;
+    if (mixed) {
+      pool_buffers<float>(pl, hs, order, off, bf);
+      grow = run_batch<float>(bf, hs, P, order, off, retry);
+    } else {
+      pool_buffers<double>(pl, hs, order, off, bf);
+      grow = run_batch<double>(bf, hs, P, order, off, retry);
+    }
+    if (!grow) break;
+    if (attempt == 2) throw std::runtime_error("batch row capacity could not be established");
+  }
+  batch_results(hs, P, order, pl->h_out, out, separate, status);
+  return true;
+}
+
+// The batch kernel with its buffers in the static scratch; no device or
+// pinned allocation.  See nb_evaluate_batch_static in amoeba_nb.h.
+int nb_batch_evaluate_static(NbHandle **hv, int count, uint32_t flags, nb_energies *out) {
+  if (count <= 0) return NB_ERR_INVALID;
+  std::vector<NbHandle *> hs(hv, hv + count);
+  if (!batch_qualifies(hs, flags)) {
+    hs[0]->err = "nb_evaluate_batch_static: every handle must qualify for the batch kernel (small periodic "
+                 "systems with the same parameters on one device, NB_GRAD only, no alchemistry, valence terms, "
+                 "SCF predictor or profiling)";
+    return NB_ERR_INVALID;
+  }
+  const int device = hs[0]->device;
+  HIPCHECK(hipSetDevice(device));
+  char *base = scratch_base(device);
+  std::lock_guard<std::mutex> run_lock(g_scratch_run_mu);
+  BatchPrm P;
+  std::vector<int> order;
+  batch_setup(hs, P, order);
+  const bool mixed = hs[0]->prm.precision == NB_PREC_MIXED;
+  std::vector<BatchSys> h_sys(count);
+  std::vector<double> h_out((size_t)O_N * count);
+  std::vector<char> retry(count, 0);
+  std::vector<size_t> off;
+  BatchBuf bf;
+  int cus = 0;
+  HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  static int occ[2][kNumShapes] = {};
+  for (int attempt = 0;; attempt++) {
+    // layout: queue counter | descriptors | outputs | workspace
+    size_t ws = mixed ? batch_ws_bytes<float>(hs, order, off) : batch_ws_bytes<double>(hs, order, off);
+    size_t o_sys = 256, o_out = o_sys + up256(sizeof(BatchSys) * count), o_ws = o_out + up256(sizeof(double) * O_N * count);
+    if (o_ws + ws > NB_SCRATCH_BYTES) {
+      hs[0]->err = "nb_evaluate_batch_static: the batch needs more device memory than the static scratch holds "
+                   "(NB_BATCH_SCRATCH_MIB); use nb_evaluate_batch";
+      return NB_ERR_INVALID;
+    }
+    bf.d_next = (int *)base;
+    bf.d_sys = (BatchSys *)(base + o_sys);
+    bf.d_out = (double *)(base + o_out);
+    bf.ws = base + o_ws;
+    bf.h_sys = h_sys.data();
+    bf.h_out = h_out.data();
+    bf.stream = hs[0]->stream;
+    bf.cus = cus;
+    bf.occ = occ;
+    bf.shape = kNumShapes - 1;   // 256 threads, 4 lanes per atom
+    bf.tune = nullptr;
+    int grow = mixed ? run_batch<float>(bf, hs, P, order, off, retry) : run_batch<double>(bf, hs, P, order, off, retry);
+    if (!grow) break;
+    if (attempt == 2) throw std::runtime_error("batch row capacity could not be established");
+  }
+  std::vector<char> separate;
+  int status = NB_OK;
+  batch_results(hs, P, order, h_out.data(), out, separate, status);
+  for (int b = 0; b < count; b++)
+    if (separate[b]) {
+      hs[b]->err = "nb_evaluate_batch_static: the batch kernel handed this system back (failed solve or a "
+                   "cutoff beyond the image range); evaluate it with nb_evaluate";
+      if (status == NB_OK) status = NB_ERR_SCF_DIVERGED;
+    }
+  return status;
 }
 
 int nb_batch_last_config(int device, int32_t cfg[4], double *kernel_ms) {

@@ -288,6 +288,7 @@ void nb_finite_add(NbHandle *h);
 bool nb_batch_evaluate(NbHandle **hv, int count, uint32_t flags, nb_energies *out, std::vector<char> &separate,
                        int &status);
 int nb_batch_last_config(int device, int32_t cfg[4], double *kernel_ms);
+int nb_batch_evaluate_static(NbHandle **hv, int count, uint32_t flags, nb_energies *out);
 
 #define HIPCHECK(x)                                                                  \
   do {                                                                               \

@@ -26,7 +26,8 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
             "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config", "nb_set_valence",
             "nb_valence_struct_size", "nb_set_scf_predictor", "nb_md_init", "nb_md_step", "nb_md_get_state",
-            "nb_md_set_thermostat", "nb_set_refinement"]
+            "nb_md_set_thermostat", "nb_set_refinement", "nb_evaluate_batch_static"]
+NB_BATCH_SCRATCH_MIB = 64
 NB_PRED_NONE, NB_PRED_ASPC = 0, 1
 NB_THERMO_NONE, NB_THERMO_BERENDSEN = 0, 1
 VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
@@ -123,6 +124,8 @@ def load(path=LIB_PATH):
     L.nb_set_refinement.argtypes = [vp, C.c_int]
     L.nb_set_refinement.restype = C.c_int
     L.nb_evaluate_batch.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(NbEnergies)]
+    L.nb_evaluate_batch_static.argtypes = [C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(NbEnergies)]
+    L.nb_evaluate_batch_static.restype = C.c_int
     L.nb_batch_config.argtypes = [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     L.nb_graph_replays.argtypes = [vp]
     L.nb_struct_sizes.argtypes = [_ip]
@@ -359,6 +362,17 @@ def evaluate_batch(handles, flags=NB_GRAD):
     arr = (C.c_void_p * len(handles))(*[h.h for h in handles])
     outs = (NbEnergies * len(handles))()
     rc = L.nb_evaluate_batch(arr, len(handles), flags, outs)
+    return [o.as_dict() for o in outs], rc
+
+
+def evaluate_batch_static(handles, flags=NB_GRAD):
+    """nb_evaluate_batch_static (the batch kernel with its buffers in the
+    library's static device scratch, no allocation) over Handle objects;
+    returns (list of energy dicts, rc)."""
+    L = load()
+    arr = (C.c_void_p * len(handles))(*[h.h for h in handles])
+    outs = (NbEnergies * len(handles))()
+    rc = L.nb_evaluate_batch_static(arr, len(handles), flags, outs)
     return [o.as_dict() for o in outs], rc
 
 

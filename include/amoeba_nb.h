@@ -323,6 +323,26 @@ int nb_set_graph(void *handle, int on);
  * receives count records.  Returns the first non-OK status. */
 int nb_evaluate_batch(void *const *handles, int count, uint32_t flags, nb_energies *out);
 
+/* Allocation-free batch evaluation: the batch kernel of nb_evaluate_batch
+ * with its workspace, system descriptors, outputs and queue counter in a
+ * static device buffer of NB_BATCH_SCRATCH_MIB MiB that each GPU holds for the
+ * life of the process, instead of device and pinned memory the library
+ * allocates.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory during evaluation (code under HIP graph capture,
+ * tight serving loops); the handles themselves are created beforehand.
+ * Every handle must qualify for the batch kernel as in nb_evaluate_batch and
+ * flags may hold at most NB_GRAD; the kernel's shape is fixed (256 threads,
+ * 4 lanes per atom; no tuner).  NB_ERR_INVALID when a handle does not
+ * qualify or the batch needs more than the static buffer (nb_last_error of
+ * handles[0] says which); a system the batch kernel hands back (failed solve,
+ * list cutoff beyond the checked image range) is not re-evaluated: its out
+ * record is left untouched, its nb_last_error says so, and the call returns
+ * NB_ERR_SCF_DIVERGED.  Calls on one GPU share its buffer and are serialised
+ * within the process; calls that overlap from different host threads wait
+ * for each other, and nothing else may use the buffer. */
+#define NB_BATCH_SCRATCH_MIB 64
+int nb_evaluate_batch_static(void *const *handles, int count, uint32_t flags, nb_energies *out);
+
 /* The shape of the last batch-kernel launch on a device: cfg[0] threads per
  * workgroup, cfg[1] lanes per atom in the pair loops, cfg[2] persistent
  * workgroups, cfg[3] systems; kernel_ms (may be NULL) its duration from HIP
