@@ -775,13 +775,17 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
       }
     }
     if (h->want_vir && use_recip) nb_grid_virial(h, polar);
-    // real-space energies (side) || reciprocal energy (main)
+    // permanent real-space energy (side) || reciprocal energy, then the
+    // polarization real-space energy added into the reciprocal arrays (main)
     nb_fork(h);
     {
       NbSide side(h);
-      nb_real_energy(h, polar);
+      nb_real_energy(h, polar, use_recip ? NB_RE_PERM : NB_RE_ALL);
     }
-    if (use_recip) nb_recip_energy(h, polar);
+    if (use_recip) {
+      nb_recip_energy(h, polar);
+      if (polar) nb_real_energy(h, polar, NB_RE_POL);
+    }
   }
   nb_join(h);
   if (third) nb_join3(h);

@@ -684,8 +684,12 @@ void nb_dip_field64(NbHandle *h, double *f, double *fp) {
 
 #define NB_LPA_SP 4
 
+// which: NB_RE_ALL both passes into the real-space arrays (grad_s, torque_s);
+// NB_RE_PERM the permanent pass alone; NB_RE_POL the polarization pass alone,
+// added into the reciprocal arrays (grad_r, torque_r, which k_recip_energy
+// stores first on the same stream), so it can run beside the permanent pass
 template <bool VIR, bool OST>
-static void real_energy_t(NbHandle *h, bool polar) {
+static void real_energy_t(NbHandle *h, bool polar, int which) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
@@ -694,12 +698,16 @@ static void real_energy_t(NbHandle *h, bool polar) {
   int direct = h->prm.polarization == NB_POL_DIRECT || h->finite_active;
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256), Gs = xcd_grid((long)n * NB_LPA_SP, 256);
   double el = h->prm.electric;
-  bool perm = !OST || h->ph.perm_real;
-  auto vr = [&](dim3 g) { return VIR ? nb_vir_region(h, g.x) : (double *)nullptr; };
+  bool perm = (!OST || h->ph.perm_real) && which != NB_RE_POL;
+  polar = polar && which != NB_RE_PERM;
+  const bool to_r = which == NB_RE_POL;
+  double *g = to_r ? h->d_grad_r : h->d_grad_s, *tq = to_r ? h->d_torque_r : h->d_torque_s;
+  double *lg = to_r ? h->d_lgrad_r : h->d_lgrad_s, *ltq = to_r ? h->d_ltorque_r : h->d_ltorque_s;
+  auto vr = [&](dim3 gr) { return VIR ? nb_vir_region(h, gr.x) : (double *)nullptr; };
 #define NB_RE(T, L, POL, LIST, GRID, ROWS, CNT, MAXR)                                                               \
   NB_LAUNCH((k_real_energy<T, L, POL, LIST, VIR, OST>), GRID, B, 0, n, h->d_pos4, h->d_mp, ROWS, CNT, MAXR, h->cell,  \
-            aw, sc, direct, h->d_u, h->d_up, el, h->d_grad_s, h->d_torque_s, h->d_energy, h->d_count, vr(GRID),      \
-            h->ph, h->d_soft, h->d_mol, h->d_lgrad_s, h->d_ltorque_s)
+            aw, sc, direct, h->d_u, h->d_up, el, g, tq, h->d_energy, h->d_count, vr(GRID), h->ph, h->d_soft, h->d_mol, \
+            lg, ltq)
   if (h->prm.precision == NB_PREC_MIXED) {
     if (perm) {
       NB_RE(float, NB_LPA, false, 1, G, h->d_ew, h->d_new, h->maxew);
@@ -716,13 +724,13 @@ static void real_energy_t(NbHandle *h, bool polar) {
 #undef NB_RE
 }
 
-void nb_real_energy(NbHandle *h, bool polar) {
+void nb_real_energy(NbHandle *h, bool polar, int which) {
   if (h->ost) {
-    if (h->want_vir) real_energy_t<true, true>(h, polar);
-    else real_energy_t<false, true>(h, polar);
+    if (h->want_vir) real_energy_t<true, true>(h, polar, which);
+    else real_energy_t<false, true>(h, polar, which);
   } else {
-    if (h->want_vir) real_energy_t<true, false>(h, polar);
-    else real_energy_t<false, false>(h, polar);
+    if (h->want_vir) real_energy_t<true, false>(h, polar, which);
+    else real_energy_t<false, false>(h, polar, which);
   }
 }
 

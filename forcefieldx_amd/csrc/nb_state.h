@@ -269,7 +269,9 @@ int nb_sor(NbHandle *h, double w, nb_energies *out);
 void nb_reset_dipoles(NbHandle *h);
 void nb_check_rebuild_async(NbHandle *h);
 void nb_recip_energy(NbHandle *h, bool polar);
-void nb_real_energy(NbHandle *h, bool polar);
+// passes of nb_real_energy
+enum { NB_RE_ALL = 0, NB_RE_PERM = 1, NB_RE_POL = 2 };
+void nb_real_energy(NbHandle *h, bool polar, int which);
 void nb_vdw(NbHandle *h);
 void nb_finalize_gradient(NbHandle *h);
 void nb_zero_outputs(NbHandle *h);
