@@ -471,15 +471,15 @@ template <int N> struct CtLines {
   static constexpr int yz = raw < 1 ? 1 : (raw > 16 ? 16 : raw);
 };
 
-template <int R, int N, int Ns, int NL, int SL, int SE, bool LF, typename T>
+template <int R, int N, int Ns, int NL, int SL, int SE, bool LF, typename T, int BS = CT_BS>
 __device__ __forceinline__ void ct_pass(CX<T> *buf, const CX<T> *tw, T s) {
   constexpr int nb = N / R, total = NL * nb, stride = N / (Ns * R);
-  constexpr int MAXB = (total + CT_BS - 1) / CT_BS;
+  constexpr int MAXB = (total + BS - 1) / BS;
   CX<T> v[MAXB][R];
   int where[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; q++) {
-    const int b = q * CT_BS + threadIdx.x;
+    const int b = q * BS + threadIdx.x;
     where[q] = -1;
     if (b < total) {
       const int line = LF ? b % NL : b / nb;
@@ -511,13 +511,58 @@ __device__ __forceinline__ void ct_pass(CX<T> *buf, const CX<T> *tw, T s) {
   __syncthreads();
 }
 
-template <int N, int Ns, int NL, int SL, int SE, bool LF, typename T>
+template <int N, int Ns, int NL, int SL, int SE, bool LF, typename T, int BS = CT_BS>
 __device__ __forceinline__ void ct_fft(CX<T> *buf, const CX<T> *tw, T s) {
   if constexpr (Ns < N) {
     constexpr int R = RadixOf<N / Ns>::v;
-    ct_pass<R, N, Ns, NL, SL, SE, LF, T>(buf, tw, s);
-    ct_fft<N, Ns * R, NL, SL, SE, LF, T>(buf, tw, s);
+    ct_pass<R, N, Ns, NL, SL, SE, LF, T, BS>(buf, tw, s);
+    ct_fft<N, Ns * R, NL, SL, SE, LF, T, BS>(buf, tw, s);
   }
+}
+
+// Whole XY planes of the FP32 grid in LDS (rows padded by one element), one
+// workgroup of PL_BS threads per z: the forward kernel (INV false) assembles
+// the plane from the brick buffer (MODE 1 / 2) or reads it (MODE 0) and
+// transforms x then y; the inverse kernel transforms y then x.  With the
+// fused z kernel between them a convolution takes three launches instead of
+// five (x, y, z, y, x), for planes of up to 64 KB of LDS (FP32 complex).
+constexpr int PL_BS = 1024;
+template <typename T, int N0, int N1, int MODE, bool INV>
+__global__ void __launch_bounds__(PL_BS) k_plane(CX<T> *__restrict__ grid, const CX<T> *__restrict__ twx,
+                                                 const CX<T> *__restrict__ twy, T s, Bricks bk,
+                                                 const T *__restrict__ bbuf, const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
+  constexpr int SL = N0 + 1, TOT = N0 * N1, PER = (TOT + PL_BS - 1) / PL_BS;
+  __shared__ CX<T> buf[N1 * SL];
+  __shared__ CX<T> tx[N0], ty[N1];
+  const int z = blockIdx.x;
+  CX<T> *plane = grid + (size_t)TOT * z;
+  for (int t = threadIdx.x; t < N0; t += PL_BS) tx[t] = twx[t];
+  for (int t = threadIdx.x; t < N1; t += PL_BS) ty[t] = twy[t];
+  // all loads of the thread are issued before the first LDS store
+  CX<T> v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int t = q * PL_BS + threadIdx.x;
+    if (t < TOT) {
+      if constexpr (MODE == 0) v[q] = plane[t];
+      else v[q] = brick_point<T, MODE>(bk, bbuf, t % N0, t / N0, z);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int t = q * PL_BS + threadIdx.x;
+    if (t < TOT) buf[(t / N0) * SL + t % N0] = v[q];
+  }
+  __syncthreads();
+  if (!INV) {
+    ct_fft<N0, 1, N1, SL, 1, false, T, PL_BS>(buf, tx, s);
+    ct_fft<N1, 1, N0, 1, SL, true, T, PL_BS>(buf, ty, s);
+  } else {
+    ct_fft<N1, 1, N0, 1, SL, true, T, PL_BS>(buf, ty, s);
+    ct_fft<N0, 1, N1, SL, 1, false, T, PL_BS>(buf, tx, s);
+  }
+  for (int t = threadIdx.x; t < TOT; t += PL_BS) plane[t] = buf[(t / N0) * SL + t % N0];
 }
 
 // x lines (contiguous), rows padded by one element in LDS.  MODE 0 reads the
@@ -866,8 +911,32 @@ static bool ct_z(NbHandle *h, CX<T> *g, const CX<T> *tw, const T *recip) {
   }
 }
 
+// XY-plane kernels for square FP32 planes of these edges; returns false
+// (nothing launched) for other grids and for FP64 grids
+#define NB_PLANE_SIZES(X) X(24) X(32) X(40) X(48) X(60) X(64) X(72) X(80)
+template <typename T>
+static bool plane_t(NbHandle *h, int np, bool inv, CX<T> *g, const CX<T> *twx, const CX<T> *twy, T s) {
+  if (sizeof(T) != sizeof(float) || h->K[0] != h->K[1]) return false;
+  const Bricks bk = bricks_of(h);
+  const T *bb = (const T *)h->d_bbuf;
+  dim3 G(h->K[2]);
+  switch (h->K[0]) {
+#define NB_P(N)                                                                                               \
+  case N:                                                                                                     \
+    if (inv) NB_LAUNCH((k_plane<T, N, N, 0, true>), G, dim3(PL_BS), 0, g, twx, twy, s, bk, bb, h->ctl);        \
+    else if (np == 0) NB_LAUNCH((k_plane<T, N, N, 0, false>), G, dim3(PL_BS), 0, g, twx, twy, s, bk, bb, h->ctl); \
+    else if (np == 1) NB_LAUNCH((k_plane<T, N, N, 1, false>), G, dim3(PL_BS), 0, g, twx, twy, s, bk, bb, h->ctl); \
+    else NB_LAUNCH((k_plane<T, N, N, 2, false>), G, dim3(PL_BS), 0, g, twx, twy, s, bk, bb, h->ctl);           \
+    return true;
+    NB_PLANE_SIZES(NB_P)
+#undef NB_P
+    default: return false;
+  }
+}
+
 // forward x (assembling the grid from the bricks when np > 0), forward y,
-// fused z (forward * influence * inverse), inverse y, inverse x
+// fused z (forward * influence * inverse), inverse y, inverse x; with XY-plane
+// kernels: forward plane, fused z, inverse plane
 template <typename T>
 static void transform_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0, const T *recip) {
   int K0 = h->K[0], K1 = h->K[1], K2 = h->K[2];
@@ -880,6 +949,12 @@ static void transform_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0, const T
   dim3 B(256);
   dim3 Gx((K1 * K2 + Lx - 1) / Lx), Gy(((K0 + Ly - 1) / Ly) * K2), Gz((K0 * K1 + Lz - 1) / Lz);
   const T *nul = nullptr;
+  if (plane_t<T>(h, np, false, g, twx, twy, (T)1)) {
+    if (!ct_z<T>(h, g, twz, recip))
+      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, true>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, recip, h->ctl);
+    plane_t<T>(h, 0, true, g, twx, twy, (T)-1);
+    return;
+  }
   for (int pass = 0; pass < 2; pass++) {
     T s = pass == 0 ? (T)1 : (T)-1;
     if (pass == 0 && !ct_x<T>(h, np, g, twx, s)) {
@@ -911,6 +986,11 @@ static void forward_t(NbHandle *h, int np, CX<T> *g, const CX<T> *tw0) {
   size_t shz = sizeof(CX<T>) * ((size_t)Lz * K2 + K2);
   dim3 B(256), Gx((K1 * K2 + Lx - 1) / Lx), Gy(((K0 + Ly - 1) / Ly) * K2), Gz((K0 * K1 + Lz - 1) / Lz);
   const T *nul = nullptr;
+  if (plane_t<T>(h, np, false, g, twx, twy, (T)1)) {
+    if (!ct_z<T>(h, g, twz, nul))
+      NB_LAUNCH((k_fft_lines<T, LINE_EPT, 2, false>), Gz, B, shz, g, K0, K1, K2, Lz, factor(K2), twz, (T)1, nul, h->ctl);
+    return;
+  }
   if (!ct_x<T>(h, np, g, twx, (T)1)) {
     if (np == 1) NB_LAUNCH((k_brick_combine<T, 1>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g, h->ctl);
     if (np == 2) NB_LAUNCH((k_brick_combine<T, 2>), grid_for(h->ntot, 256), dim3(256), 0, bk, bb, g, h->ctl);
