@@ -39,7 +39,7 @@ PEAK_HBM_GBS = 8000.0
 
 def kernel_family(name):
     for k in ("k_real_energy", "k_perm_field", "k_dip_field", "k_vdw", "k_ct_x", "k_ct_y", "k_ct_z",
-              "k_spread_brick", "k_gather", "k_filter"):
+              "k_spread_brick", "k_gather_ap", "k_gather", "k_filter", "k_plane"):
         if k in name:
             return k
     return name
@@ -79,14 +79,35 @@ def work_model(name, w):
         nbuf = w["nbricks"] * w["ext"] ** 3 * (2 if "2>" in name else 1) * s
         return nbuf + K3 * cx, "brick buffer read + complex grid write", None
     if fam == "k_spread_brick":
-        parts = 2 if "true" in name else 1
-        per_atom = 480 + 12 + 4 + (48 if parts == 2 else 80)
+        # the generic name covers 1 permanent and ~8 induced launches per
+        # evaluation: the induced (two-part) launch is the model
+        parts = 1 if "false" in name else 2
+        per_atom = 60 * s + 12 + 4 + (48 if parts == 2 else 80)
         return n * per_atom + w["nbricks"] * w["ext"] ** 3 * parts * s, \
-            "B-spline weights, grid start, dipoles / multipoles per atom + brick buffer write", None
+            "B-spline weights (grid type), grid start, atom index, dipoles / multipoles per atom + brick buffer write", \
+            None
+    if fam == "k_plane":
+        if "true>" in name:
+            return 2 * K3 * cx, "complex grid read + write (inverse y, x)", None
+        nbuf = w["nbricks"] * w["ext"] ** 3 * (1 if ", 1, false" in name else 2) * s
+        return nbuf + K3 * cx, "brick buffer read + complex grid write (combine, x, y)", None
+    if fam == "k_gather_ap":
+        return n * (60 * s + 12) + K3 * cx + n * 3 * 8 * 2 * 7, \
+            "B-spline weights per atom + grid read + z, f, p, q (read / write) per atom", None
     if fam == "k_gather":
-        return n * (480 + 12) + K3 * cx + n * 2 * 8 * (30 if "true" in name else 3), \
+        return n * (60 * s + 12) + K3 * cx + n * 2 * 8 * (30 if "true" in name else 3), \
             "B-spline weights per atom + grid read + potentials written", None
     return None, None, None
+
+
+def profile_key(name, prec="mixed"):
+    """The kernel's name in the rocprofv3 summaries for a name the library's
+    profiler reports (the launch expression, with template parameters by
+    name): the mixed form's instantiation."""
+    key = name.strip("()").split("(")[0].replace("NB_LPA", "16")
+    real = "float" if prec == "mixed" else "double"
+    key = key.replace("<T, IND>", f"<{real}, true>").replace("<T>", f"<{real}>").replace("<T, ", f"<{real}, ")
+    return key
 
 
 def roofline_for(name, avg_ms, w, prec):
@@ -116,7 +137,7 @@ def measured_traffic(name):
     path = os.path.join(ROOT, "profiles", "water-20736_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    key = name.strip("()").split("(")[0].replace("NB_LPA", "16")
+    key = profile_key(name)
     for line in open(path):
         if line.startswith("| `") and "|" in line[3:]:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
@@ -582,6 +603,18 @@ def main():
         roof["timed_with"] = ("HIP events around each launch of this kernel in the timed region" if not graph else
                               "HIP events around each launch of this kernel over the same number of regular "
                               "evaluations right after the graph-replay timed region")
+        # the other kernels of the PCG iteration's critical path (spread, FFT
+        # planes, gather) and the real-space field, from the calibration pass
+        fams = []
+        for kname, (ktot, kcnt) in sorted(profmx["calibration"].items(), key=lambda kv: -kv[1][0]):
+            if kernel_family(kname) in ("k_spread_brick", "k_plane", "k_ct_x", "k_ct_y", "k_ct_z", "k_gather_ap",
+                                        "k_dip_field") and kcnt > 0:
+                r = roofline_for(kname, ktot / kcnt, w, "mixed")
+                r["traffic"] = measured_traffic(kname) if args.workload == "water-20736" else None
+                r["launches_per_step"] = kcnt / 2
+                fams.append({k: r.get(k) for k in ("kernel", "avg_launch_ms", "achieved", "frac", "traffic",
+                                                   "launches_per_step", "valu_frac")})
+        roof["path_kernels"] = fams[:8]
 
     cpu = None
     e_cpu = None
