@@ -220,14 +220,15 @@ def time_gpu_perturbed(top, box, xyz, params, prec, steps, warmup, profile, grap
     x0 = np.asarray(xyz, np.float64).reshape(-1)
     nprof = 2 if profile else 0
     nframes = nprof + warmup + steps
-    rng = np.random.default_rng(1234)
     frames = ffi.DeviceArray(x0.size, frames=nframes)
-    x = x0.copy()
-    for k in range(nframes):
-        if k == nprof:
-            x = x0.copy()   # the timed sequence starts from the given coordinates
-        x += rng.normal(0.0, 0.01, x.shape)
-        frames.upload(x, k)
+    # frames [0, nprof): the profiling evaluations' own walk; then warm-up and
+    # timed frames from the same seed in both forms
+    for seed, first, count in ((4321, 0, nprof), (1234, nprof, warmup + steps)):
+        rng = np.random.default_rng(seed)
+        x = x0.copy()
+        for k in range(first, first + count):
+            x += rng.normal(0.0, 0.01, x.shape)
+            frames.upload(x, k)
     cur = [0]
 
     def step():
@@ -395,6 +396,37 @@ def batch_main(args, systems, params):
     print(json.dumps(out))
 
 
+def accuracy_probe(top, box, xyz, params, grid, lam=None, valence=None):
+    """Errors of the mixed form against the FP64 form on one evaluation of the
+    given coordinates with the strain derivative (outside any timed region):
+    max and RMS relative errors of forces, induced dipoles and virial, and the
+    energy difference."""
+    from forcefieldx_amd import ffi
+    res = {}
+    for prec in (ffi.NB_PREC_FP64, ffi.NB_PREC_MIXED):
+        h = ffi.Handle(ffi.params_from(params, grid, prec), top)
+        try:
+            h.set_cell(box)
+            if valence is not None:
+                h.set_valence(valence)
+            if lam is not None:
+                h.set_lambda(lam)
+            h.set_positions(np.asarray(xyz, np.float64).reshape(-1))
+            e, _ = h.evaluate(ffi.NB_GRAD | ffi.NB_VIRIAL)
+            res[prec] = (e, h.gradient().reshape(-1), h.induced()[0].reshape(-1), h.virial().reshape(-1))
+        finally:
+            h.close()
+    (e64, g64, u64, w64), (emx, gmx, umx, wmx) = res[ffi.NB_PREC_FP64], res[ffi.NB_PREC_MIXED]
+
+    def errs(a, b):
+        return {"rms": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)),
+                "max": float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))}
+    return {"force": errs(gmx, g64), "induced": errs(umx, u64), "virial": errs(wmx, w64),
+            "energy_abs": float(emx["total"] - e64["total"]),
+            "energy_rel": float(abs(emx["total"] - e64["total"]) / max(abs(e64["total"]), 1e-300)),
+            "note": "one evaluation with NB_GRAD | NB_VIRIAL of the workload's starting coordinates"}
+
+
 def cpu_baseline(top, box, xyz, params, workload, lam=None, budget_s=12.0, valence=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -427,7 +459,8 @@ WORKLOADS = {
     # name: (description, builder) -- the first is the headline configuration
     "water-20736": "6912 AMOEBA-2003 waters, 59.1 A cubic box, PME 64^3 order 5, Ewald 7 A, vdW 9 A, PCG 1e-5 D",
     "water-288000": "96000 AMOEBA-2003 waters, 144.3 A cubic box, PME 144^3 order 5, Ewald 7 A, vdW 9 A, PCG 1e-5 D, "
-                    "each step after a Gaussian perturbation of sigma 0.01 A applied on the device",
+                    "each step on the next frame of a random walk of sigma 0.01 A steps, the frames resident in "
+                    "device memory before the timed region",
     "chain-23557": "2500-atom random-walk chain with random multipoles in 7019 waters, 62.2 A box, PME 64^3, "
                    "20 soft atoms at lambda 0.5: OST electrostatics + vdW softcore with dU/dL",
     "water-648": "216 AMOEBA-2003 waters, 18.643 A cubic box, PME 24^3, PCG 1e-6 D",
@@ -588,6 +621,7 @@ def main():
     frc_rms = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
     frc_max = float(np.max(np.abs(gmx - g64)) / np.max(np.abs(g64)))
     ind_rms = float(np.linalg.norm(umx - u64) / max(np.linalg.norm(u64), 1e-300))
+    accuracy = accuracy_probe(top, box, xyz, params, grid, lam, valence)
 
     roof = None
     if profmx:
@@ -646,6 +680,7 @@ def main():
         "force_rel_max_err": frc_max,
         "induced_rel_rms_err": ind_rms,
         "energy_mixed_minus_fp64": emx["total"] - e64["total"],
+        "mixed_vs_fp64": accuracy,
         "energy_fp64": e64["total"],
         "pcg_iters": {"mixed": emx["pcg_iters"], "fp64": e64["pcg_iters"]},
         "pairs": {"ewald": int(e64["n_perm"]) - top.n, "vdw": int(e64["n_vdw"])},
