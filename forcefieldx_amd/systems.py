@@ -608,3 +608,77 @@ def crystal_batch(count, seed=0):
         grid = tuple(_smooth_at_least(1.2 * L) for L in box[:3])
         out.append((xyz, box, top, grid))
     return out
+
+
+# --------------------------------------------------------------------------
+# the named workloads
+# --------------------------------------------------------------------------
+def elements_of(top):
+    """(symbol, atomic number, mass) per atom of the synthetic systems by vdW
+    class: water oxygens and molecule centres (class 0) O, water hydrogens
+    and substituents (class 1) H, chain atoms (class 2) C."""
+    table = {0: ("O", 8, 15.999), 1: ("H", 1, 1.008), 2: ("C", 6, 12.011)}
+    return [table[int(c)] for c in top.vdw_class]
+
+
+NAMED_CONFIGS = ("water-648", "water-20736", "chain-23557", "crystals-512", "water-288000")
+
+
+def named_config(name, seed=None):
+    """The systems of a named workload as a list of (label, xyz (3n,), box,
+    Topology, NonbondedParams with the grid and state set):
+
+    * water-648: 216 waters, 18.643 A, PME 24^3, vdW 9 A, PCG 1e-6 D (seed 1);
+    * water-20736: 6912 waters, 59.1 A, 64^3, vdW 9 A, 1e-5 D (seed 2);
+    * chain-23557: a 2500-atom chain in 7019 waters, 62.2 A, 64^3, 20 soft
+      atoms, lambda 0.5 with the soft atoms' electrostatics coupled (seed 7);
+    * crystals-512: 512 triclinic cells of 30-300 atoms, Ewald and vdW 6 A,
+      per-cell grids, 1e-5 D (seed 2), one entry per cell;
+    * water-288000: 96000 waters, 144.3 A, 144^3, vdW 9 A, 1e-5 D (seed 2)."""
+    from .forcefield import NonbondedParams
+    if name == "water-648":
+        xyz, box, top = water_box(216, 18.643, seed=1 if seed is None else seed)
+        return [(name, xyz, box, top, NonbondedParams(vdw_cutoff=9.0, grid=(24, 24, 24), polar_eps=1e-6))]
+    if name == "water-20736":
+        xyz, box, top = water_box(6912, 59.1, seed=2 if seed is None else seed)
+        return [(name, xyz, box, top, NonbondedParams(vdw_cutoff=9.0, grid=(64, 64, 64), polar_eps=1e-5))]
+    if name == "chain-23557":
+        xyz, box, top = chain_in_water(2500, 7019, 62.2, seed=7 if seed is None else seed, n_soft=20)
+        return [(name, xyz, box, top, NonbondedParams(vdw_cutoff=9.0, grid=(64, 64, 64), polar_eps=1e-5,
+                                                      elec_lambda=True, lambda_value=0.5))]
+    if name == "crystals-512":
+        cells = crystal_batch(512, seed=2 if seed is None else seed)
+        return [(f"cell{k:03d}", xyz, box, top, NonbondedParams(ewald_cutoff=6.0, vdw_cutoff=6.0, polar_eps=1e-5,
+                                                                grid=tuple(grid)))
+                for k, (xyz, box, top, grid) in enumerate(cells)]
+    if name == "water-288000":
+        xyz, box, top = water_box(96000, 144.3, seed=2 if seed is None else seed)
+        return [(name, xyz, box, top, NonbondedParams(vdw_cutoff=9.0, grid=(144, 144, 144), polar_eps=1e-5))]
+    raise ValueError(f"unknown workload {name}; one of {', '.join(NAMED_CONFIGS)}")
+
+
+def write_named_config(name, folder, seed=None, digits=8):
+    """Write a named workload as Tinker files under ``folder``: one
+    xyz/key/prm set per system (``folder/<label>.*``).  Returns the list of
+    forcefieldx_amd.tinker.write_system results; their alchemical_atoms and
+    lambda_value are the options that tinker.load takes for the soft atoms
+    and the state."""
+    import os
+
+    from .tinker import write_system
+    os.makedirs(folder, exist_ok=True)
+    return [write_system(os.path.join(folder, label), xyz, box, top, p, title=f"{name} {label}".strip(),
+                         elements=elements_of(top), digits=digits)
+            for label, xyz, box, top, p in named_config(name, seed)]
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="Write a named workload as Tinker xyz/key/prm files.")
+    ap.add_argument("name", choices=NAMED_CONFIGS)
+    ap.add_argument("folder")
+    ap.add_argument("--seed", type=int, default=None)
+    a = ap.parse_args()
+    for r in write_named_config(a.name, a.folder, a.seed):
+        opts = "".join(f" --ac {r['alchemical_atoms']}" for _ in [0] if r["alchemical_atoms"])
+        print(f"{r['xyz']} {r['key']} {r['prm']} (lambda {r['lambda_value']}{opts})")
