@@ -902,6 +902,16 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
           // evaluation (against the first solve's budget: the recorded
           // refinement corrections only run when a residual asks for them)
           if (e.pcg_iters + 2 < h->graph_iters) shrink_graph = true;
+          // recorded refinement corrections that replays keep skipping
+          // (S_RDONE after the first residual) still cost their launches:
+          // after eight such replays in a row the graph is recorded again
+          if (h->graph_refine_n > 0) {
+            if ((int)sc[S_RCYC] < h->graph_refine_n) {
+              if (++h->refine_idle >= 8) shrink_graph = true;
+            } else {
+              h->refine_idle = 0;
+            }
+          }
         }
       } else {
         drop_graph(h);
@@ -978,10 +988,11 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
     // record a graph for the next evaluations: the PCG budget is this
     // evaluation's iteration count plus one, and so is the budget of each
-    // FP64 refinement correction it took; with refinement at least one
-    // correction of two iterations is recorded, so a replay whose FP32 solve
-    // stops just short of the FP64 test is corrected in the graph instead of
-    // being repeated on the regular path
+    // FP64 refinement correction it took.  A replay whose FP64 residual then
+    // fails with no correction left is repeated on the regular path (which
+    // takes the corrections and records them); no spare correction is
+    // recorded, since its early-returning launches cost about as much as
+    // two PCG iterations on every replay
     if (h->graph_on && !replayed && status == NB_OK && !h->profiling && !h->used_sor && !h->want_finite) {
       int first = e.pcg_iters;
       const bool refining = mutual && h->refine && h->prm.precision == NB_PREC_MIXED;
@@ -992,8 +1003,8 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
           first -= h->refine_iters[c];
         }
         h->graph_refine_n = h->refine_cycles;
-        if (h->graph_refine_n == 0) h->graph_refine_iters[h->graph_refine_n++] = 2;
       }
+      h->refine_idle = 0;
       capture_graph(h, flags, mutual ? first + 1 : 0);
     }
     if (shrink_graph) drop_graph(h);

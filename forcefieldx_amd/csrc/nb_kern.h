@@ -13,9 +13,10 @@ enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL
 // S_RZOLD / S_RZOLDP hold the r.z totals of the last update (the denominators
 // of the next conjugation)
 // S_RDONE stops the FP64 refinement residuals recorded after a solve once
-// one of them has passed the test
+// one of them has passed the test; S_RCYC counts the recorded corrections a
+// replay reopened the solve for
 enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_DONE, S_ITER, S_PREV, S_STAT, S_EPSV,
-       S_RZOLD, S_RZOLDP, S_RDONE, S_NSLOT };
+       S_RZOLD, S_RZOLDP, S_RDONE, S_RCYC, S_NSLOT };
 // pinned host slots of an evaluation's read-back (doubles)
 enum { P_ENERGY = 0, P_COUNT = 8, P_FLAGS = 12, P_VIR = 16, P_DUDL = 32, P_SCAL = 40, P_VALENCE = 64, P_NSLOT = 80 };
 static_assert(P_SCAL + S_NSLOT <= P_VALENCE, "PCG scalars overlap the valence energies in the read-back");

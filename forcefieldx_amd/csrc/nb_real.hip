@@ -273,6 +273,7 @@ __global__ void __launch_bounds__(256) k_refine_check(const double *__restrict__
       } else {
         scal[S_DONE] = 0.0;
         scal[S_PREV] = 100.0;
+        scal[S_RCYC] += 1.0;
       }
     } else if (!failed && !pass) {
       scal[S_DONE] = 0.0;
@@ -300,6 +301,7 @@ __global__ void k_pcg_ctl_init(double *__restrict__ scal) {
     scal[S_STAT] = 0.0;
     scal[S_EPSV] = 0.0;
     scal[S_RDONE] = 0.0;
+    scal[S_RCYC] = 0.0;
   }
 }
 
@@ -670,9 +672,11 @@ static int pcg_solve(NbHandle *h, nb_energies *out, bool control, int iters) {
   h->refine_cycles = 0;
   const bool refine = h->refine && mx && !h->tape_on;
   if (refine && control) {
-    // the corrections the recorded evaluation took (at least one), each with
-    // a budget of its iteration count + 1; once a residual passes, the
-    // remaining corrections and residuals return at once (S_RDONE)
+    // the corrections the recorded evaluation took, each with a budget of its
+    // iteration count + 1; once a residual passes, the remaining corrections
+    // and residuals return at once (S_RDONE); a last residual that fails
+    // leaves the solve unfinished, and the evaluation is repeated on the
+    // regular path
     for (int c = 0; c < h->graph_refine_n; c++) {
       h->ctl = h->d_scal + S_RDONE;
       refine_residual(h);

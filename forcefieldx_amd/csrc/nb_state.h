@@ -229,6 +229,7 @@ struct NbHandle {
   int graph_iters = 0;
   // recorded FP64 refinement: corrections and the PCG budget of each
   int graph_refine_n = 0;
+  int refine_idle = 0;            // replays in a row that skipped a recorded correction
   int graph_refine_iters[4] = {0, 0, 0, 0};
   long graph_replays = 0;
   bool graph_launched = false;    // the graph of this evaluation was launched by nb_evaluate_batch

@@ -71,6 +71,34 @@ def test_graph_replay_matches_regular_path(water648, precision):
         gr.close()
 
 
+def test_graph_replay_with_refinement_corrections(water648):
+    """Mixed form at 1e-8 D, where the FP32-operator solve needs FP64
+    refinement corrections: the graph records the corrections the regular
+    path took, replays run them under device-side control, and a replay whose
+    last residual fails is repeated on the regular path; every evaluation
+    equals the regular path's."""
+    xyz, box, top = water648
+    p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-8)
+    ref, gr = make(top, box, p, ffi.NB_PREC_MIXED, False), make(top, box, p, ffi.NB_PREC_MIXED, True)
+    rng = np.random.default_rng(5)
+    x = xyz.reshape(-1).copy()
+    try:
+        for step in range(16):
+            print(f"graph refinement step={step}", flush=True)
+            x = x + rng.normal(0.0, 0.01, x.shape)
+            a, b = run(ref, x, ffi.NB_GRAD), run(gr, x, ffi.NB_GRAD)
+            assert a["e"]["pcg_eps"] < 1e-8 and b["e"]["pcg_eps"] < 1e-8
+            for k in ("total", "perm", "pol", "vdw"):
+                assert abs(a["e"][k] - b["e"][k]) <= TOL_MIXED * abs(a["e"][k]), (step, k)
+            assert a["e"]["pcg_iters"] == b["e"]["pcg_iters"]
+            assert close(b["g"], a["g"], TOL_MIXED)
+        print(f"replays {gr.graph_replays()} of 16", flush=True)
+        assert gr.graph_replays() >= 8
+    finally:
+        ref.close()
+        gr.close()
+
+
 def test_graph_replay_with_ost_children(oracle):
     """OST handles record one graph per phase; dU/dL and dU/dX/dL included."""
     xyz, box, top = chain_in_water(24, 180, 18.0, seed=5, n_soft=6)
