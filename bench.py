@@ -474,7 +474,8 @@ WORKLOADS = {
                       "the Tinker fixture in tests/golden/ubiquitin",
     "ubiquitin-md": "velocity-Verlet dynamics of the reference's ubiquitin benchmark structure (9,737 atoms, "
                     "AMOEBA-BIO-2009 valence and nonbonded terms, PME 70 x 54 x 54, PCG to 1e-6 D) at 1 fs steps, "
-                    "ASPC SCF predictor, each step's evaluation replayed from a HIP graph; a step is one MD step",
+                    "ASPC SCF predictor, each step (drift, evaluation, kick) replayed as one HIP graph; a step is one "
+                    "MD step",
     "crystals-512": "512 independent P1 triclinic cells (lengths 8-20 A, angles 70-110 deg) of small molecules "
                     "with random multipoles, per-cell PME grids 16-32, Ewald and vdW 6 A, PCG 1e-5 D, image sums "
                     "in cells thinner than twice the cutoff; a step is one sweep over all cells (nb_evaluate_batch)",
@@ -559,7 +560,7 @@ def md_main(args):
         "dtype": "mixed: f32 pair arithmetic, f64 accumulation/PME/PCG/integration",
         "data": "the reference's own test structure (tests/golden/ubiquitin), Maxwell-Boltzmann velocities at 298 K",
         "config": {"workload": f"ubiquitin-md: {WORKLOADS['ubiquitin-md']}", "atoms": top.n, "grid": list(params.grid),
-                   "launch": "HIP graph replay per step", "parallelism": "single GPU"},
+                   "launch": "one HIP graph replay per MD step", "parallelism": "single GPU"},
         "ns_per_day": 86400.0 / tm * dt_fs * 1e-6,
         "fp64_steps_per_s": 1.0 / res["fp64"]["s_per_step"],
         "fp64_ns_per_day": 86400.0 / res["fp64"]["s_per_step"] * dt_fs * 1e-6,

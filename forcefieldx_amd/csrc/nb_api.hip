@@ -818,9 +818,14 @@ static void capture_graph(NbHandle *h, uint32_t flags, int iters) {
   memset(&e, 0, sizeof(e));
   bool rebuilt = false;
   hipGraph_t g = nullptr;
+  const bool md = (flags & NB_FLAG_MD_STEP) != 0;
   HIPCHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   try {
+    // a recorded MD step: the drift before the evaluation, the guarded kick,
+    // kinetic energy and thermostat after it
+    if (md) nb_md_drift(h);
     issue_evaluation(h, true, e, rebuilt);
+    if (md) nb_md_kick(h, true);
   } catch (...) {
     hipStreamEndCapture(h->stream, &g);
     if (g) hipGraphDestroy(g);
@@ -832,6 +837,12 @@ static void capture_graph(NbHandle *h, uint32_t flags, int iters) {
   h->graph = g;
   HIPCHECK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
   h->graph_flags = flags;
+  if (md) {
+    h->md_graph_dt = h->md_dt;
+    h->md_graph_thermo = h->md_thermo;
+    h->md_graph_t0 = h->md_t0;
+    h->md_graph_tau = h->md_tau;
+  }
 }
 
 static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
@@ -877,13 +888,21 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       }
     }
     bool mutual = h->prm.do_pme && h->prm.polarization == NB_POL_MUTUAL && h->ph.polar;
-    bool replayed = false, shrink_graph = false;
+    bool replayed = false, shrink_graph = false, drifted = false;
+    const bool md = (flags & NB_FLAG_MD_STEP) != 0;
+    if (md && h->gexec && (h->graph_flags & NB_FLAG_MD_STEP) &&
+        (h->md_graph_dt != h->md_dt || h->md_graph_thermo != h->md_thermo || h->md_graph_t0 != h->md_t0 ||
+         h->md_graph_tau != h->md_tau))
+      drop_graph(h);
     if (h->graph_on && h->gexec && h->graph_flags == flags && !h->profiling && !h->want_finite) {
       // replay; a list that needs rebuilding, an overflowed filtered row or a
       // solve that did not stop within the recorded iterations sends the
       // evaluation down the regular path below
       if (!h->graph_launched) HIPCHECK(hipGraphLaunch(h->gexec, h->stream));
       h->graph_launched = false;
+      // a recorded MD step has drifted the positions whether or not its
+      // evaluation is kept
+      drifted = md;
       HIPCHECK(hipStreamSynchronize(h->stream));
       memcpy(fl, h->h_pinned + 12, sizeof(fl));
       const double *sc = h->h_pinned + 40;
@@ -925,6 +944,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     } else {
       // the regular path may grow or rebuild the rows a recorded graph points to
       drop_graph(h);
+      if (md && !drifted) nb_md_drift(h);
       // A filtered (Ewald / preconditioner / masked) row that overflowed its
       // capacity is only detected when the flags come back with the energies;
       // the rows are then grown and the evaluation repeated.
@@ -986,6 +1006,12 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       }
     if (status == NB_OK && fl[4]) status = NB_ERR_BAD_TORQUE;
     if (status == NB_OK && !std::isfinite(e.total)) status = NB_ERR_NAN_ENERGY;
+    // the step's kick on the regular path (the recorded kick of a replay that
+    // was not kept returned at once)
+    if (md && !replayed && status == NB_OK) {
+      nb_md_kick(h, false);
+      HIPCHECK(hipStreamSynchronize(h->stream));
+    }
     // record a graph for the next evaluations: the PCG budget is this
     // evaluation's iteration count plus one, and so is the budget of each
     // FP64 refinement correction it took.  A replay whose FP64 residual then
@@ -1029,6 +1055,10 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
 int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
   NbHandle *h = (NbHandle *)handle;
   if (!h) return NB_ERR_INVALID;
+  if (flags & NB_FLAG_MD_STEP) {
+    h->err = "unknown evaluation flag";
+    return NB_ERR_INVALID;
+  }
   if (!h->elec_lambda) return nb_evaluate_one(h, flags, out);
   if (!h->cell_set || !h->positions_set) {
     h->err = "nb_set_cell and nb_set_positions must be called before nb_evaluate";
@@ -1260,3 +1290,9 @@ void nb_destroy(void *handle) {
 }
 
 }  // extern "C"
+
+// one velocity-Verlet step of nb_md_step (h->md_dt set): drift, evaluation
+// and kick, recorded and replayed as one graph in graph mode
+int nb_evaluate_md_step(NbHandle *h, nb_energies *out) {
+  return nb_evaluate_one(h, NB_GRAD | NB_FLAG_MD_STEP, out);
+}

@@ -11,6 +11,13 @@
 //    coefficients 22/7, -55/14, 55/21, -22/21, 5/21, -1/42.  The ring's start
 //    and fill count live in device memory and are advanced by a kernel, so a
 //    recorded graph replays the predictor unchanged.
+//  * Least-squares predictor (SCFPredictorParameters.java:144-213 and its
+//    LeastSquaresPredictor model, :315-418): the newest entry is fitted by
+//    the five before it; the 21 products of the six entries are reduced in
+//    a fixed order (k_pred_gram, k_pred_ls) and the 5 x 5 normal equations
+//    are solved by a Cholesky factorisation that drops columns depending on
+//    earlier ones, all on the device; the coefficients then act on the five
+//    newest entries through the same apply kernel as ASPC.
 //  * Velocity Verlet (VelocityVerlet.java:66-113, MolecularDynamics.java:
 //    1285-1299): v += a dt/2, x += v dt; evaluate; a = -418.4 g / m,
 //    v += a dt/2; the kinetic energy sum m v^2 / (2 418.4) is reduced in a
@@ -37,14 +44,15 @@ __device__ __forceinline__ void pred_coefs(double *c) {
   c[5] = -1.0 / 42.0;
 }
 
-// u += sum_k c_k hist[(start + k) mod 6]; the packed records follow u
+// u += sum_k c_k hist[(start + k) mod 6] once min_count entries are held;
+// the packed records follow u
 __global__ void k_pred_apply(int n, const int32_t *__restrict__ perm, const double *__restrict__ hist,
-                             const int32_t *__restrict__ ctl, double *__restrict__ u, double *__restrict__ up,
-                             void *rec, int mixed) {
+                             const int32_t *__restrict__ ctl, const double *__restrict__ coef, int min_count,
+                             double *__restrict__ u, double *__restrict__ up, void *rec, int mixed) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || ctl[1] < 6) return;
+  if (s >= n || ctl[1] < min_count) return;
   double c[6];
-  pred_coefs(c);
+  for (int k = 0; k < 6; k++) c[k] = coef[k];
   const int i = perm[s], start = ctl[0];
   const size_t stride = 6 * (size_t)n;
   for (int d = 0; d < 3; d++) {
@@ -87,6 +95,99 @@ __global__ void k_pred_save(int n, const int32_t *__restrict__ perm, const doubl
   }
 }
 
+__global__ void k_pred_aspc_coefs(double *coef) {
+  if (threadIdx.x == 0) pred_coefs(coef);
+}
+
+// index of the product of the entries of ages a <= b among the 21
+__host__ __device__ constexpr int gram_index(int a, int b) { return a * 6 - a * (a - 1) / 2 + (b - a); }
+
+// the 21 products of the six history entries ordered by age (0 the newest),
+// as per-block partial sums part[21][gridDim.x] (grid-stride over the 6n values)
+__global__ void __launch_bounds__(256) k_pred_gram(size_t n6, const double *__restrict__ hist,
+                                                   const int32_t *__restrict__ ctl, double *__restrict__ part) {
+  if (ctl[1] < 2) return;
+  const int start = ctl[0];
+  const double *e[6];
+  for (int m = 0; m < 6; m++) e[m] = hist + (size_t)((start + m) % 6) * n6;
+  double acc[21];
+  for (int t = 0; t < 21; t++) acc[t] = 0.0;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n6; q += (size_t)gridDim.x * blockDim.x) {
+    double v[6];
+    for (int m = 0; m < 6; m++) v[m] = e[m][q];
+    int t = 0;
+    for (int a = 0; a < 6; a++)
+      for (int b = a; b < 6; b++) acc[t++] += v[a] * v[b];
+  }
+  __shared__ double red[21][4];
+  for (int t = 0; t < 21; t++) {
+    const double w = wave_sum_d(acc[t]);
+    if ((threadIdx.x & 63) == 0) red[t][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 21)
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] =
+        red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+}
+
+// Least-squares coefficients: minimise |e_0 - sum_j c_j e_(1+j)|^2 over the
+// m = min(count - 1, 5) filled older entries, by the normal equations
+// A c = r, A_jl = e_(1+j).e_(1+l), r_j = e_0.e_(1+j).  Cholesky, with a column
+// whose part orthogonal to the kept earlier ones has squared norm below
+// 1e-12 of its own dropped (c = 0); unfilled columns keep c = 0.
+__global__ void k_pred_ls(const double *__restrict__ part, int nblk, const int32_t *__restrict__ ctl,
+                          double *__restrict__ coef) {
+  __shared__ double G[21];
+  const int count = ctl[1];
+  if (count < 2) return;
+  if (threadIdx.x < 21) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(size_t)threadIdx.x * nblk + b];
+    G[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int m = count - 1 < 5 ? count - 1 : 5;
+  double A[5][5], r[5], L[5][5], c[6] = {0, 0, 0, 0, 0, 0}, y[5];
+  bool keep[5];
+  for (int j = 0; j < m; j++) {
+    r[j] = G[gram_index(0, 1 + j)];
+    for (int l = 0; l < m; l++) A[j][l] = G[j <= l ? gram_index(1 + j, 1 + l) : gram_index(1 + l, 1 + j)];
+  }
+  for (int j = 0; j < m; j++) {
+    double d = A[j][j];
+    for (int k = 0; k < j; k++)
+      if (keep[k]) d -= L[j][k] * L[j][k];
+    keep[j] = A[j][j] > 0.0 && d > 1e-12 * A[j][j];
+    L[j][j] = keep[j] ? sqrt(d) : 0.0;
+    for (int i = j + 1; i < m; i++) {
+      double v = 0.0;
+      if (keep[j]) {
+        v = A[i][j];
+        for (int k = 0; k < j; k++)
+          if (keep[k]) v -= L[i][k] * L[j][k];
+        v /= L[j][j];
+      }
+      L[i][j] = v;
+    }
+  }
+  for (int j = 0; j < m; j++) {
+    if (!keep[j]) continue;
+    double v = r[j];
+    for (int k = 0; k < j; k++)
+      if (keep[k]) v -= L[j][k] * y[k];
+    y[j] = v / L[j][j];
+  }
+  for (int j = m - 1; j >= 0; j--) {
+    if (!keep[j]) continue;
+    double v = y[j];
+    for (int k = j + 1; k < m; k++)
+      if (keep[k]) v -= L[k][j] * c[k];
+    c[j] = v / L[j][j];
+  }
+  for (int k = 0; k < 6; k++) coef[k] = c[k];
+}
+
 __global__ void k_pred_advance(int32_t *ctl, const int32_t *__restrict__ flags, const double *__restrict__ scal) {
   if (threadIdx.x == 0 && blockIdx.x == 0 && pred_keep(flags, scal)) {
     ctl[0] = (ctl[0] + 5) % 6;
@@ -103,10 +204,17 @@ __global__ void k_md_drift(int n3, double dt, const double *__restrict__ mass, d
   x[q] += vq * dt;
 }
 
-// first: a from the gradient only (no half kick); partial kinetic sums per block
+// A kick recorded into a step's graph runs only when the evaluation before it
+// is kept (flags non-null: the pred_keep test of the evaluation's flags and,
+// for a mutual solve, its control scalars); the host finishes the step
+// otherwise.  first: a from the gradient only (no half kick); partial kinetic
+// sums per block
 __global__ void __launch_bounds__(256) k_md_kick(int n3, double dt, int first, const double *__restrict__ mass,
                                                  const double *__restrict__ g, double *__restrict__ v,
-                                                 double *__restrict__ a, double *__restrict__ part) {
+                                                 double *__restrict__ a, double *__restrict__ part,
+                                                 const int32_t *__restrict__ flags,
+                                                 const double *__restrict__ scal) {
+  if (flags && !pred_keep(flags, scal)) return;
   __shared__ double red[4];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   double ke = 0.0;
@@ -129,7 +237,9 @@ __global__ void __launch_bounds__(256) k_md_kick(int n3, double dt, int first, c
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ void k_md_ke(int nb, const double *__restrict__ part, double *__restrict__ ke) {
+__global__ void k_md_ke(int nb, const double *__restrict__ part, double *__restrict__ ke,
+                        const int32_t *__restrict__ flags, const double *__restrict__ scal) {
+  if (flags && !pred_keep(flags, scal)) return;
   __shared__ double red[4];
   double s = 0.0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
@@ -155,13 +265,17 @@ __device__ __forceinline__ double berendsen_scale(double ke, double dof, double 
 // drift and kick do)
 __global__ void __launch_bounds__(256) k_md_berendsen(int n3, double dof, double t0, double dt_tau,
                                                       const double *__restrict__ ke, const double *__restrict__ mass,
-                                                      double *__restrict__ v) {
+                                                      double *__restrict__ v, const int32_t *__restrict__ flags,
+                                                      const double *__restrict__ scal) {
+  if (flags && !pred_keep(flags, scal)) return;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n3 && mass[q / 3] > 0.0) v[q] *= berendsen_scale(ke[0], dof, t0, dt_tau);
 }
 
 // the kinetic energy after scaling (every block above has read the old value)
-__global__ void k_md_ke_scale(double dof, double t0, double dt_tau, double *__restrict__ ke) {
+__global__ void k_md_ke_scale(double dof, double t0, double dt_tau, double *__restrict__ ke,
+                              const int32_t *__restrict__ flags, const double *__restrict__ scal) {
+  if (flags && !pred_keep(flags, scal)) return;
   if (threadIdx.x == 0) {
     const double s = berendsen_scale(ke[0], dof, t0, dt_tau);
     ke[0] *= s * s;
@@ -194,9 +308,15 @@ static int mixed_flag(const NbHandle *h) { return h->prm.precision == NB_PREC_MI
 
 void nb_pred_apply(NbHandle *h) {
   if (h->predictor == NB_PRED_NONE || !h->d_pred) return;
+  if (h->predictor == NB_PRED_LS) {
+    NB_LAUNCH(k_pred_gram, dim3(NB_PRED_BLOCKS), dim3(256), 0, 6 * (size_t)h->n, (const double *)h->d_pred,
+              (const int32_t *)h->d_pred_ctl, h->d_pred_part);
+    NB_LAUNCH(k_pred_ls, dim3(1), dim3(64), 0, (const double *)h->d_pred_part, NB_PRED_BLOCKS,
+              (const int32_t *)h->d_pred_ctl, h->d_pred_coef);
+  }
   NB_LAUNCH(k_pred_apply, grid_for(h->n, 256), dim3(256), 0, h->n, (const int32_t *)h->d_perm,
-            (const double *)h->d_pred, (const int32_t *)h->d_pred_ctl, h->d_u, h->d_up, (void *)h->d_dip,
-            mixed_flag(h));
+            (const double *)h->d_pred, (const int32_t *)h->d_pred_ctl, (const double *)h->d_pred_coef,
+            h->predictor == NB_PRED_LS ? 2 : 6, h->d_u, h->d_up, (void *)h->d_dip, mixed_flag(h));
 }
 
 void nb_pred_save(NbHandle *h, bool device_ctl) {
@@ -209,10 +329,12 @@ void nb_pred_save(NbHandle *h, bool device_ctl) {
 }
 
 void nb_md_free(NbHandle *h) {
-  void *p[] = {h->d_pred, h->d_pred_ctl, h->d_md_v, h->d_md_a, h->d_md_mass, h->d_md_part, h->d_md_ke};
+  void *p[] = {h->d_pred, h->d_pred_ctl, h->d_pred_coef, h->d_pred_part, h->d_md_v, h->d_md_a, h->d_md_mass,
+               h->d_md_part, h->d_md_ke};
   for (void *q : p)
     if (q) (void)hipFree(q);
-  h->d_pred = h->d_md_v = h->d_md_a = h->d_md_mass = h->d_md_part = h->d_md_ke = nullptr;
+  h->d_pred = h->d_pred_coef = h->d_pred_part = h->d_md_v = h->d_md_a = h->d_md_mass = h->d_md_part = h->d_md_ke =
+      nullptr;
   h->d_pred_ctl = nullptr;
   h->md_ready = h->md_accel = false;
 }
@@ -222,7 +344,7 @@ void nb_drop_graph(NbHandle *h);
 int nb_set_scf_predictor(void *handle, int kind) {
   NbHandle *h = (NbHandle *)handle;
   if (!h) return NB_ERR_INVALID;
-  if (kind != NB_PRED_NONE && kind != NB_PRED_ASPC) {
+  if (kind != NB_PRED_NONE && kind != NB_PRED_ASPC && kind != NB_PRED_LS) {
     h->err = "unknown SCF predictor";
     return NB_ERR_INVALID;
   }
@@ -234,10 +356,37 @@ int nb_set_scf_predictor(void *handle, int kind) {
     if (kind != NB_PRED_NONE) {
       if (!h->d_pred) HIPCHECK(hipMalloc(&h->d_pred, sizeof(double) * 36 * (size_t)h->n));
       if (!h->d_pred_ctl) HIPCHECK(hipMalloc(&h->d_pred_ctl, sizeof(int32_t) * 2));
+      if (!h->d_pred_coef) HIPCHECK(hipMalloc(&h->d_pred_coef, sizeof(double) * 6));
+      if (!h->d_pred_part) HIPCHECK(hipMalloc(&h->d_pred_part, sizeof(double) * 21 * NB_PRED_BLOCKS));
       // an empty history: the next solves start from the direct dipoles
       int32_t zero[2] = {0, 0};
       HIPCHECK(hipMemcpy(h->d_pred_ctl, zero, sizeof(zero), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemset(h->d_pred, 0, sizeof(double) * 36 * (size_t)h->n));
+      HIPCHECK(hipMemset(h->d_pred_coef, 0, sizeof(double) * 6));
+      if (kind == NB_PRED_ASPC) hipLaunchKernelGGL(k_pred_aspc_coefs, dim3(1), dim3(64), 0, h->stream, h->d_pred_coef);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(h->stream));
     }
+    return NB_OK;
+  } catch (std::exception &e) {
+    h->err = e.what();
+    return NB_ERR_HIP;
+  }
+}
+
+int nb_get_scf_predictor(void *handle, double *coef, double *history, int32_t *state) {
+  NbHandle *h = (NbHandle *)handle;
+  if (!h) return NB_ERR_INVALID;
+  if (h->predictor == NB_PRED_NONE || !h->d_pred) {
+    h->err = "no SCF predictor is set";
+    return NB_ERR_INVALID;
+  }
+  try {
+    HIPCHECK(hipSetDevice(h->device));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    if (coef) HIPCHECK(hipMemcpy(coef, h->d_pred_coef, sizeof(double) * 6, hipMemcpyDeviceToHost));
+    if (history) HIPCHECK(hipMemcpy(history, h->d_pred, sizeof(double) * 36 * (size_t)h->n, hipMemcpyDeviceToHost));
+    if (state) HIPCHECK(hipMemcpy(state, h->d_pred_ctl, sizeof(int32_t) * 2, hipMemcpyDeviceToHost));
     return NB_OK;
   } catch (std::exception &e) {
     h->err = e.what();
@@ -248,6 +397,33 @@ int nb_set_scf_predictor(void *handle, int kind) {
 // ---------------------------------------------------------------------------
 // dynamics
 // ---------------------------------------------------------------------------
+// the drift of a step (x += v dt after the first half kick), on the main stream
+void nb_md_drift(NbHandle *h) {
+  const int n3 = 3 * h->n, nb = (n3 + 255) / 256;
+  NB_LAUNCH(k_md_drift, dim3(nb), dim3(256), 0, n3, h->md_dt, (const double *)h->d_md_mass, h->d_xyz, h->d_md_v,
+            (const double *)h->d_md_a);
+}
+
+// the second half kick, kinetic energy and thermostat of a step; guarded: the
+// kernels check the evaluation's flags (and the solve's control scalars)
+// themselves, as they must in a recorded step
+void nb_md_kick(NbHandle *h, bool guarded) {
+  const int n3 = 3 * h->n, nb = (n3 + 255) / 256;
+  const bool mutual = h->prm.do_pme && h->prm.polarization == NB_POL_MUTUAL && h->ph.polar;
+  const int32_t *fl = guarded ? (const int32_t *)h->d_flags : nullptr;
+  const double *sc = guarded && mutual ? (const double *)h->d_scal : nullptr;
+  const double dt = h->md_dt;
+  NB_LAUNCH(k_md_kick, dim3(nb), dim3(256), 0, n3, dt, 0, (const double *)h->d_md_mass, (const double *)h->d_grad,
+            h->d_md_v, h->d_md_a, h->d_md_part, fl, sc);
+  NB_LAUNCH(k_md_ke, dim3(1), dim3(256), 0, nb, (const double *)h->d_md_part, h->d_md_ke, fl, sc);
+  if (h->md_thermo == NB_THERMO_BERENDSEN) {
+    const double dof = 3.0 * h->md_active - 3.0, dt_tau = dt / h->md_tau;
+    NB_LAUNCH(k_md_berendsen, dim3(nb), dim3(256), 0, n3, dof, h->md_t0, dt_tau, (const double *)h->d_md_ke,
+              (const double *)h->d_md_mass, h->d_md_v, fl, sc);
+    NB_LAUNCH(k_md_ke_scale, dim3(1), dim3(64), 0, dof, h->md_t0, dt_tau, h->d_md_ke, fl, sc);
+  }
+  HIPCHECK(hipGetLastError());
+}
 int nb_md_init(void *handle, const double *mass, const double *v0) {
   NbHandle *h = (NbHandle *)handle;
   if (!h || !mass) return NB_ERR_INVALID;
@@ -300,35 +476,32 @@ int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out) {
     nb_energies e;
     std::memset(&e, 0, sizeof(e));
     long iters = 0;
-    auto kick = [&](int first) {
-      hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, h->stream, n3, dt, first,
-                         (const double *)h->d_md_mass, (const double *)h->d_grad, h->d_md_v, h->d_md_a,
-                         h->d_md_part);
-      hipLaunchKernelGGL(k_md_ke, dim3(1), dim3(256), 0, h->stream, nb, (const double *)h->d_md_part, h->d_md_ke);
-      HIPCHECK(hipGetLastError());
-    };
+    h->md_dt = dt;
     if (!h->md_accel) {
       // MolecularDynamics.init: accelerations of the starting coordinates
       int rc = nb_evaluate(h, NB_GRAD, &e);
       if (rc != NB_OK) return rc;
-      kick(1);
+      hipLaunchKernelGGL(k_md_kick, dim3(nb), dim3(256), 0, h->stream, n3, dt, 1, (const double *)h->d_md_mass,
+                         (const double *)h->d_grad, h->d_md_v, h->d_md_a, h->d_md_part, (const int32_t *)nullptr,
+                         (const double *)nullptr);
+      hipLaunchKernelGGL(k_md_ke, dim3(1), dim3(256), 0, h->stream, nb, (const double *)h->d_md_part, h->d_md_ke,
+                         (const int32_t *)nullptr, (const double *)nullptr);
+      HIPCHECK(hipGetLastError());
       h->md_accel = true;
     }
     for (int step = 0; step < nsteps; step++) {
-      hipLaunchKernelGGL(k_md_drift, dim3(nb), dim3(256), 0, h->stream, n3, dt, (const double *)h->d_md_mass,
-                         h->d_xyz, h->d_md_v, (const double *)h->d_md_a);
-      HIPCHECK(hipGetLastError());
-      int rc = nb_evaluate(h, NB_GRAD, &e);
+      int rc;
+      if (h->graph_on && !h->elec_lambda) {
+        // the whole step in one recorded graph (drift, evaluation, kick)
+        rc = nb_evaluate_md_step(h, &e);
+      } else {
+        nb_md_drift(h);
+        HIPCHECK(hipGetLastError());
+        rc = nb_evaluate(h, NB_GRAD, &e);
+        if (rc == NB_OK) nb_md_kick(h, false);
+      }
       if (rc != NB_OK) return rc;
       iters += e.pcg_iters;
-      kick(0);
-      if (h->md_thermo == NB_THERMO_BERENDSEN) {
-        const double dof = 3.0 * h->md_active - 3.0, dt_tau = dt / h->md_tau;
-        hipLaunchKernelGGL(k_md_berendsen, dim3(nb), dim3(256), 0, h->stream, n3, dof, h->md_t0, dt_tau,
-                           (const double *)h->d_md_ke, (const double *)h->d_md_mass, h->d_md_v);
-        hipLaunchKernelGGL(k_md_ke_scale, dim3(1), dim3(64), 0, h->stream, dof, h->md_t0, dt_tau, h->d_md_ke);
-        HIPCHECK(hipGetLastError());
-      }
     }
     double ke = 0.0;
     HIPCHECK(hipMemcpyAsync(&ke, h->d_md_ke, sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -213,11 +213,19 @@ struct NbHandle {
   int predictor = 0;              // NB_PRED_*
   double *d_pred = nullptr;       // [6][6n] u - u_direct history (u then uCR), caller order
   int32_t *d_pred_ctl = nullptr;  // [2] ring start, fill count (advanced on the device)
+  double *d_pred_coef = nullptr;  // [6] coefficients of the prediction (ASPC constants or the LS fit)
+  double *d_pred_part = nullptr;  // [21][NB_PRED_BLOCKS] Gram-matrix partial sums of the LS fit
   double *d_md_v = nullptr, *d_md_a = nullptr, *d_md_mass = nullptr, *d_md_part = nullptr, *d_md_ke = nullptr;
   bool md_ready = false, md_accel = false;
   int md_thermo = 0;                 // NB_THERMO_*
   int md_active = 0;                 // atoms with mass > 0 (degrees of freedom 3 md_active - 3)
   double md_t0 = 0.0, md_tau = 0.0;  // K, ps
+  // a recorded MD step (graph flags with NB_FLAG_MD_STEP): drift, evaluation,
+  // kick and thermostat with these settings
+  double md_dt = 0.0;                // ps, of the step being taken
+  double md_graph_dt = 0.0;
+  int md_graph_thermo = 0;
+  double md_graph_t0 = 0.0, md_graph_tau = 0.0;
 
   // graph mode (nb_set_graph): the evaluation sequence is recorded once and
   // replayed; the PCG then runs graph_iters iterations under device-side
@@ -283,6 +291,13 @@ void nb_tape_push(NbHandle *h, int nrz, int npq);
 void nb_pred_apply(NbHandle *h);
 void nb_pred_save(NbHandle *h, bool device_ctl);
 void nb_md_free(NbHandle *h);
+// internal evaluation flag: the evaluation is one velocity-Verlet step (drift
+// before, kick and thermostat after, all in the recorded graph)
+constexpr uint32_t NB_FLAG_MD_STEP = 1u << 30;
+constexpr int NB_PRED_BLOCKS = 256;
+void nb_md_drift(NbHandle *h);
+void nb_md_kick(NbHandle *h, bool guarded);
+int nb_evaluate_md_step(NbHandle *h, nb_energies *out);
 void nb_valence_terms(NbHandle *h);
 void nb_valence_gather(NbHandle *h);
 void nb_valence_free(NbHandle *h);

@@ -26,9 +26,9 @@ EXPORTED = ["nb_create", "nb_set_cell", "nb_set_positions", "nb_set_positions_de
             "nb_set_profiling", "nb_set_profile_filter", "nb_profile_count", "nb_profile_entry", "nb_struct_sizes",
             "nb_set_graph", "nb_graph_replays", "nb_evaluate_batch", "nb_batch_config", "nb_set_valence",
             "nb_valence_struct_size", "nb_set_scf_predictor", "nb_md_init", "nb_md_step", "nb_md_get_state",
-            "nb_md_set_thermostat", "nb_set_refinement", "nb_evaluate_batch_static"]
+            "nb_md_set_thermostat", "nb_set_refinement", "nb_evaluate_batch_static", "nb_get_scf_predictor"]
 NB_BATCH_SCRATCH_MIB = 64
-NB_PRED_NONE, NB_PRED_ASPC = 0, 1
+NB_PRED_NONE, NB_PRED_ASPC, NB_PRED_LS = 0, 1, 2
 NB_THERMO_NONE, NB_THERMO_BERENDSEN = 0, 1
 VALENCE_KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
 
@@ -155,6 +155,8 @@ def load(path=LIB_PATH):
     L.nb_profile_entry.restype = C.c_int
     L.nb_set_scf_predictor.argtypes = [vp, C.c_int]
     L.nb_set_scf_predictor.restype = C.c_int
+    L.nb_get_scf_predictor.argtypes = [vp, _dp, _dp, C.POINTER(C.c_int32)]
+    L.nb_get_scf_predictor.restype = C.c_int
     L.nb_md_init.argtypes = [vp, _dp, _dp]
     L.nb_md_init.restype = C.c_int
     L.nb_md_step.argtypes = [vp, C.c_int, C.c_double, C.POINTER(NbMdReport)]
@@ -269,6 +271,16 @@ class Handle:
 
     def set_scf_predictor(self, kind=NB_PRED_ASPC):
         self._check(self.L.nb_set_scf_predictor(self.h, int(kind)))
+
+    def scf_predictor(self):
+        """(coefficients [6], history [6, 2, n, 3] by ring slot in caller order,
+        newest slot, entries held) of the SCF predictor."""
+        c = np.zeros(6)
+        hist = np.zeros(36 * self.n)
+        st = np.zeros(2, np.int32)
+        self._check(self.L.nb_get_scf_predictor(self.h, _p(c, C.c_double), _p(hist, C.c_double),
+                                                _p(st, C.c_int32)))
+        return c, hist.reshape(6, 2, self.n, 3), int(st[0]), int(st[1])
 
     def md_init(self, mass, v0=None):
         m = np.ascontiguousarray(mass, np.float64).reshape(-1)

@@ -153,7 +153,7 @@ enum { NB_VAL_BOND = 0, NB_VAL_ANGLE, NB_VAL_STRBND, NB_VAL_UREY, NB_VAL_OPBEND,
        NB_VAL_TORTOR };
 
 /* SCF predictors (SCFPredictor.java) */
-enum { NB_PRED_NONE = 0, NB_PRED_ASPC = 1 };
+enum { NB_PRED_NONE = 0, NB_PRED_ASPC = 1, NB_PRED_LS = 2 };
 
 /* Result of nb_md_step: the energies after the last step. */
 typedef struct {
@@ -261,17 +261,43 @@ int nb_valence_struct_size(void);
  * ParticleMeshEwald.java:2272-2281, saved at :1952-1955): after six converged
  * solves, the next starts from the direct dipoles plus sum_k c_k (u -
  * u_direct) of the last six, c = 22/7, -55/14, 55/21, -22/21, 5/21, -1/42, for
- * u and uCR alike.  The history is kept per handle in caller order and reset
- * by this call; graph replays carry it.  Not applied with NB_FINITE_PCG. */
+ * u and uCR alike.  NB_PRED_LS is the least-squares predictor
+ * (SCFPredictorParameters.leastSquaresPredictor, :144-213, with the model of
+ * LeastSquaresPredictor.value, :341-375): from two saved solves on, the
+ * coefficients c_0..c_4 that best fit the newest (u - u_direct, uCR -
+ * uCR_direct) by the five before it (in least squares over all 6n values,
+ * the normal equations solved on the device; slots not yet filled and
+ * columns that depend linearly on earlier ones within 1e-12 get c = 0) are
+ * applied to the five newest, u = u_direct + sum_k c_k (u - u_direct)_(t-k).
+ * The reference minimises the same sum of squares with a Levenberg-Marquardt
+ * optimiser stopped at a relative change of 1e-4, so its coefficients are
+ * this minimiser to that tolerance.  The history is kept per handle in
+ * caller order and reset by this call; graph replays carry it.  Not applied
+ * with NB_FINITE_PCG. */
 int nb_set_scf_predictor(void *handle, int kind);
+
+/* The predictor's state: coef [6] the coefficients of the last prediction
+ * (the ASPC constants, or the last LS fit), history [6][2][n][3] the saved
+ * (u - u_direct, uCR - uCR_direct) in caller order by ring slot, state [2]
+ * the slot of the newest entry and the number of entries held (the
+ * reference's predictorStartIndex and predictorCount,
+ * SCFPredictorParameters.java:269-302).  Any pointer may be NULL. */
+int nb_get_scf_predictor(void *handle, double *coef, double *history, int32_t *state);
 
 /* Velocity-Verlet dynamics on the device (VelocityVerlet.java:66-113, driven
  * as MolecularDynamics.java:1285-1299): nb_md_init takes the masses [n]
  * (g/mol) and velocities [3n] (A/ps, NULL = zero); nb_md_step advances nsteps
- * steps of dt ps from the handle's current positions, one evaluation per step
- * (replayed from a graph in graph mode), positions and velocities staying in
- * device memory; nb_md_get_state copies them out (either may be NULL).  The
- * first step after nb_md_init evaluates the starting accelerations. */
+ * steps of dt ps from the handle's current positions, one evaluation per step,
+ * positions and velocities staying in device memory; nb_md_get_state copies
+ * them out (either may be NULL).  The first step after nb_md_init evaluates
+ * the starting accelerations.  In graph mode (nb_set_graph) a whole step --
+ * drift, evaluation, kick, kinetic energy and thermostat -- is recorded as
+ * one graph and replayed; the kick kernels run only when the evaluation is
+ * valid (no list rebuild pending, no filtered row overflowed, the solve
+ * converged within the recorded iterations), and a step whose replay is not
+ * valid is finished on the regular path from its drifted positions.  Handles
+ * with alchemical electrostatics (the OST environment and vapour handles)
+ * step with a graph per evaluation. */
 int nb_md_init(void *handle, const double *mass, const double *v0);
 int nb_md_step(void *handle, int nsteps, double dt, nb_md_report *out);
 int nb_md_get_state(void *handle, double *x, double *v);

@@ -294,3 +294,119 @@ def test_batch_of_predictor_handles_keeps_the_history():
     finally:
         a.close()
         b.close()
+
+
+def ls_coefficients(hist, start, count):
+    """The least-squares predictor's fit as the reference defines it
+    (SCFPredictorParameters.java:144-213, LeastSquaresPredictor.value and
+    updateJacobianAndTarget, :341-418): the newest entry of the ring (u and
+    uCR parts, all 6n values) against the five before it, over the entries
+    filled so far; the other coefficients stay 0."""
+    c = np.zeros(6)
+    if count < 2:
+        return c
+    m = min(count - 1, 5)
+    e = [hist[(start + k) % 6].reshape(-1) for k in range(6)]
+    A = np.stack([e[1 + j] for j in range(m)], axis=1)
+    c[:m] = np.linalg.lstsq(A, e[0], rcond=None)[0]
+    return c
+
+
+def test_ls_predictor_matches_oracle(oracle):
+    """NB_PRED_LS along a random walk (steps of sigma 0.02 A): each prediction's
+    coefficients equal numpy's least-squares fit of the history the device
+    held before the solve (1e-8), and the oracle started from the same
+    extrapolation of its own earlier solutions returns the same iteration
+    counts, energies, gradients and dipoles at 1e-8."""
+    xyz, box, top = water_box(216, 18.643, seed=1)
+    p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-6)
+    rng = np.random.default_rng(8)
+    h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+    ohist, prev, predicted = [], None, 0
+    x = xyz.reshape(-1, 3).copy()
+    try:
+        h.set_cell(box)
+        h.set_scf_predictor(ffi.NB_PRED_LS)
+        for t in range(10):
+            x = x + rng.normal(0.0, 0.02, x.shape)
+            h.set_positions(x.reshape(-1))
+            e, _ = h.evaluate(ffi.NB_GRAD)
+            u, up = h.induced()
+            coef, hist, start, count = h.scf_predictor()
+            guess = None
+            if prev is not None and prev[2] >= 2:
+                expect = ls_coefficients(*prev)
+                print(f"step {t}: {prev[2]} entries, LS coefficients {np.round(coef[:5], 6)}", flush=True)
+                assert np.abs(coef - expect).max() <= 1e-8 * max(1.0, np.abs(expect).max()), t
+                guess = sum(coef[k] * ohist[k] for k in range(min(5, len(ohist))))
+                predicted += 1
+            eo, go, uo, upo, direct = oracle.evaluate(oracle.System(x, box, top), p, scf_guess=guess,
+                                                      want_direct=True)
+            ohist = ([np.stack([uo - direct[0], upo - direct[1]])] + ohist)[:6]
+            print(f"step {t}: iterations {e['pcg_iters']} oracle {eo['pcg_iters']}", flush=True)
+            assert e["pcg_iters"] == eo["pcg_iters"], t
+            assert abs(e["total"] - eo["total"]) <= 1e-8 * abs(eo["total"]), t
+            assert rel(h.gradient(), go) <= 1e-8, t
+            assert rel(u, uo) <= 1e-8 and rel(up, upo) <= 1e-8, t
+            assert count == min(t + 1, 6)
+            prev = (hist, start, count)
+        assert predicted == 8
+    finally:
+        h.close()
+
+
+def test_ls_predictor_saves_iterations_on_a_smooth_path():
+    """Along a straight path the LS extrapolation of the last solutions
+    starts the solve closer to its answer than the direct dipoles do."""
+    xyz, box, top = water_box(216, 18.643, seed=1)
+    p = NonbondedParams(vdw_cutoff=9.0, ewald_cutoff=7.0, grid=(24, 24, 24), polar_eps=1e-6)
+    vel = np.random.default_rng(3).normal(0.0, 0.004, xyz.shape)
+    its = {}
+    for kind in (ffi.NB_PRED_NONE, ffi.NB_PRED_LS):
+        h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
+        try:
+            h.set_cell(box)
+            if kind != ffi.NB_PRED_NONE:
+                h.set_scf_predictor(kind)
+            its[kind] = []
+            for t in range(10):
+                h.set_positions((xyz + t * vel).reshape(-1))
+                e, _ = h.evaluate(ffi.NB_GRAD)
+                its[kind].append(e["pcg_iters"])
+        finally:
+            h.close()
+    print(f"PCG iterations: none {its[ffi.NB_PRED_NONE]}, LS {its[ffi.NB_PRED_LS]}", flush=True)
+    assert sum(its[ffi.NB_PRED_LS][4:]) < sum(its[ffi.NB_PRED_NONE][4:])
+
+
+def test_md_step_graph_with_list_rebuilds():
+    """Graph mode records a whole velocity-Verlet step (drift, evaluation,
+    kick) as one graph.  With a 0.3 A list buffer ubiquitin rebuilds its
+    list every few steps: those replays are not kept (their kick returns at
+    once) and the step is finished on the regular path from the drifted
+    positions.  The trajectory equals the regular path's, with the LS
+    predictor's history kept through the rejected replays."""
+    xyz, box, top, p, terms, mass = ubiquitin(polar_eps=1e-6)
+    p = dataclasses.replace(p, list_buffer=0.3)
+    v0 = velocities(mass, seed=12)
+    runs = []
+    for graph in (True, False):
+        h = md_handle(top, box, p, terms, graph=graph, predictor=ffi.NB_PRED_LS)
+        try:
+            h.set_positions(xyz.reshape(-1))
+            h.md_init(mass, v0.reshape(-1))
+            reps = [h.md_step(1, 0.001) for _ in range(30)]
+            x, v = h.md_state()
+            runs.append((reps, x, v, h.graph_replays()))
+        finally:
+            h.close()
+    (reps, x, v, replays), (sreps, sx, sv, _) = runs
+    rebuilds = sum(int(r["last"]["list_rebuilt"]) for r in sreps)
+    print(f"graph replays {replays} of 30, list rebuilds {rebuilds}, PCG iterations "
+          f"{[r['pcg_iters'] for r in reps]}", flush=True)
+    assert rebuilds >= 2
+    assert 10 <= replays <= 30 - 1 - rebuilds
+    for r, s in zip(reps, sreps):
+        assert r["pcg_iters"] == s["pcg_iters"]
+        assert abs(r["total"] - s["total"]) <= 1e-9 * abs(s["total"])
+    assert np.abs(x - sx).max() <= 1e-9 and rel(v, sv) <= 1e-9
