@@ -12,7 +12,10 @@ ParentEnergyTest.java:101).
 It also writes tests/golden/crystals: the asymmetric units, keywords and
 parameter patches of the five crystal benchmarks (EnergyTest.java:435-614,
 modules/potential/src/test/resources/<name>.xyz, .properties, .patch) with
-the tests' per-asymmetric-unit energies and counts.
+the tests' per-asymmetric-unit energies and counts; and tests/golden/snare:
+the asymmetric unit of the SNARE protein crystal in P212121 (EnergyTest.java:
+74-109, "SNARE P212121", 1n7s.P212121.xyz / .key) with the records of the
+AMOEBA_PROTEIN_2004 parameter file its atom types use.
 
 Run in a checkout that has the reference at /root/reference:
     python tests/golden/make_energy_test_fixtures.py
@@ -109,6 +112,34 @@ def write_crystals():
         json.dump(expected, f, indent=1)
 
 
+SNARE = {
+    "source": "EnergyTest.java:74-109 (SNARE P212121), per asymmetric unit, tolerance 1.0e-2 "
+              "(ParentEnergyTest.java:101); the P1 row (EnergyTest.java:110-145) holds four times these",
+    "atoms": 5357, "spacegroup": "P212121",
+    "valence": {"bond": [351.32142483, 5040], "angle": [744.19251364, 8255], "strbnd": [6.42015744, 6924],
+                "urey": [2.50163831, 322], "opbend": [135.24919366, 2487], "torsion": [417.89244418, 11449],
+                "pitors": [39.85643934, 370], "tortor": [-560.99576469, 268]},
+    "vdw": 4003.27183547, "n_vdw": 741643,
+    "perm": -12305.43400643136, "n_perm": 332114,
+    "pol": -2811.45683671, "n_pol": 332114,
+}
+
+
+def write_snare():
+    out = os.path.join(HERE, "snare")
+    os.makedirs(out, exist_ok=True)
+    ff = tinker.read_prm(REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_PROTEIN_2004")
+    xf = tinker.read_xyz(REF + "/test/resources/1n7s.P212121.xyz")
+    keys = tinker.read_keys(REF + "/test/resources/1n7s.P212121.key")
+    tinker.write_xyz(os.path.join(out, "1n7s.xyz.gz"), xf)
+    with open(os.path.join(out, "1n7s.key"), "w") as f:
+        for k, v in keys.items():
+            f.write(f"{k:<20s}{' '.join(v)}\n")
+    tinker.write_prm(os.path.join(out, "amoeba_protein_2004_1n7s.prm.gz"), ff, types=set(int(t) for t in xf.types))
+    with open(os.path.join(out, "expected.json"), "w") as f:
+        json.dump(dict(SNARE, tolerance=1.0e-2), f, indent=1)
+
+
 def main():
     ff = tinker.read_prm(REF + "/main/java/ffx/potential/parameters/ff/AMOEBA_BIO_2009")
     for name, exp in SYSTEMS.items():
@@ -129,3 +160,4 @@ def main():
 if __name__ == "__main__":
     main()
     write_crystals()
+    write_snare()

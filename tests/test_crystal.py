@@ -2,7 +2,8 @@
 tinker.load_crystal): the five crystal benchmarks of the reference's
 EnergyTest (EnergyTest.java:435-614: acetanilide in Pbca, ethylparaben,
 methylparaben, paracetamol and phenacetin in P21/c), read from
-tests/golden/crystals.  Per asymmetric unit (P1 energy / number of
+tests/golden/crystals, and the SNARE protein crystal in P212121
+(EnergyTest.java:74-109, tests/golden/snare).  Per asymmetric unit (P1 energy / number of
 operators) the oracle reproduces each test's vdW energy and every valence
 term but the angles to 1e-8 kcal/mol, the precision the test prints, which
 pins the operators; the permanent and polarization energies and the angle
@@ -17,9 +18,10 @@ import numpy as np
 import pytest
 
 from forcefieldx_amd import tinker
-from forcefieldx_amd.crystal import SPACE_GROUPS, cell_matrix, expand_to_p1
+from forcefieldx_amd.crystal import SPACE_GROUPS, cell_matrix, expand_to_p1, fold_gradient, parse_symop, symops
 
 CRYSTALS = os.path.join(os.path.dirname(__file__), "golden", "crystals")
+SNARE = os.path.join(os.path.dirname(__file__), "golden", "snare")
 NAMES = ("acetanilide", "ethylparaben", "methylparaben", "paracetamol", "phenacetin")
 KINDS = ("bond", "angle", "strbnd", "urey", "opbend", "torsion", "pitors", "tortor")
 
@@ -64,14 +66,74 @@ def test_per_asymmetric_unit_energies_match_energy_test(oracle, name):
     assert abs(v["angle"] / ns - ref["valence"]["angle"][0]) <= 1e-8
 
 
+# group orders (International Tables, Vol. A)
+ORDERS = {"P1": 1, "P-1": 2, "P21": 2, "P21/c": 4, "P21/n": 4, "P21/a": 4, "P2/c": 4, "C2/c": 8, "P212121": 4,
+          "Pbca": 8, "Pccn": 8, "P213": 12}
+
+
 @pytest.mark.parametrize("group", sorted(SPACE_GROUPS))
 def test_operators_form_a_group_modulo_lattice(group):
-    """Composing any two operators gives a third, up to a lattice vector."""
-    ops = [(np.array(r, float), np.array(t)) for r, t in SPACE_GROUPS[group]]
+    """Composing any two operators gives a third, up to a lattice vector; the
+    operators are distinct, include the identity, have determinant +-1 and
+    their number is the group's order."""
+    ops = symops(group)
+    assert len(ops) == ORDERS[group]
+    assert np.array_equal(ops[0][0], np.eye(3)) and not ops[0][1].any()
     for r1, t1 in ops:
+        assert abs(round(np.linalg.det(r1))) == 1
         for r2, t2 in ops:
-            r, t = r1 * r2, r1 * t2 + t1
-            assert any(np.array_equal(r, r3) and np.allclose((t - t3) - np.round(t - t3), 0.0) for r3, t3 in ops)
+            r, t = r1 @ r2, r1 @ t2 + t1
+            assert sum(np.array_equal(r, r3) and np.allclose((t - t3) - np.round(t - t3), 0.0)
+                       for r3, t3 in ops) == 1
+
+
+def test_parse_symop():
+    r, t = parse_symop("-x+1/2, y+1/2, -z")
+    assert np.array_equal(r, np.diag([-1, 1, -1])) and np.allclose(t, [0.5, 0.5, 0.0])
+    r, t = parse_symop("x-y,x,z+2/3")
+    assert np.array_equal(r, [[1, -1, 0], [1, 0, 0], [0, 0, 1]]) and np.allclose(t, [0, 0, 2 / 3])
+    r, t = parse_symop("1/2+z, -x, y")
+    assert np.array_equal(r, [[0, 0, 1], [-1, 0, 0], [0, 1, 0]]) and np.allclose(t, [0.5, 0, 0])
+    for bad in ("x,y", "x,,z", "x,y,w"):
+        with pytest.raises(ValueError):
+            parse_symop(bad)
+
+
+def test_general_operators_by_string_equal_the_table():
+    """A group given as operator strings expands as the tabulated one; a
+    permutation operator (P213) keeps every copy's internal geometry, and the
+    folded gradient of an energy that is a sum over copies is the gradient
+    of its asymmetric-unit form."""
+    xf = tinker.read_xyz(os.path.join(CRYSTALS, "acetanilide.xyz"))
+    box = [19.640, 9.483, 7.979, 90.0, 90.0, 90.0]
+    a = expand_to_p1(xf.xyz, xf.types, xf.bonds, xf.names, box, "P212121")
+    b = expand_to_p1(xf.xyz, xf.types, xf.bonds, xf.names, box,
+                     ["x,y,z", "-x+1/2,-y,z+1/2", "-x,y+1/2,-z+1/2", "x+1/2,-y+1/2,-z"])
+    assert np.array_equal(a[0], b[0]) and a[4] == b[4] == 4
+    cube = [15.0, 15.0, 15.0, 90.0, 90.0, 90.0]
+    x, t, bonds, names, ns = expand_to_p1(xf.xyz, xf.types, xf.bonds, xf.names, cube, "P213")
+    n = len(xf.names)
+    d0 = np.linalg.norm(xf.xyz[:, None] - xf.xyz[None], axis=-1)
+    for k in range(ns):
+        c = x[k * n:(k + 1) * n]
+        assert np.abs(np.linalg.norm(c[:, None] - c[None], axis=-1) - d0).max() <= 1e-9
+    # E = sum_k w . x_k (a linear function of every copy): per asymmetric unit
+    # E / ns, whose gradient with respect to the asymmetric unit is
+    # (1 / ns) sum_k w_k A_k^T; central differences agree
+    rng = np.random.default_rng(1)
+    w = rng.normal(size=x.shape)
+
+    def e_asu(xa):
+        xx = expand_to_p1(xa, xf.types, xf.bonds, xf.names, cube, "P213")[0]
+        return float((w * xx).sum()) / ns
+
+    g = fold_gradient(w, cube, "P213")
+    h = 1e-5
+    for i, d in ((0, 0), (3, 1), (7, 2)):
+        xp, xm = xf.xyz.copy(), xf.xyz.copy()
+        xp[i, d] += h
+        xm[i, d] -= h
+        assert abs((e_asu(xp) - e_asu(xm)) / (2 * h) - g[i, d]) <= 1e-8
 
 
 def test_expansion_keeps_molecules_whole_and_preserves_internal_geometry():
@@ -126,3 +188,36 @@ def test_folded_gradient_central_differences(oracle):
             xm = xf.xyz.copy(); xm[i, d] -= h
             fd = (energy(xp) - energy(xm)) / (2 * h)
             assert abs(gu[i, d] - fd) <= 1e-5 * max(1.0, abs(fd)), (i, d, gu[i, d], fd)
+
+
+def load_snare():
+    return tinker.load_crystal(os.path.join(SNARE, "1n7s.xyz.gz"), os.path.join(SNARE, "1n7s.key"),
+                               os.path.join(SNARE, "amoeba_protein_2004_1n7s.prm.gz"))
+
+
+def snare_expected():
+    with open(os.path.join(SNARE, "expected.json")) as f:
+        return json.load(f)
+
+
+def test_snare_p212121_matches_energy_test(oracle):
+    """The SNARE protein crystal (1n7s, 5,357 atoms per asymmetric unit,
+    AMOEBA-PROTEIN-2004) in P212121, expanded to its 21,428-atom P1 cell:
+    per asymmetric unit the oracle reproduces EnergyTest's "SNARE P212121"
+    valence, vdW and permanent energies to 1e-8 kcal/mol and every term and
+    interaction count exactly (four times the test's, which counts the
+    asymmetric unit's interactions with all mates); the polarization energy
+    lies within the test's 1e-2 kcal/mol (PCG to 1e-6 D)."""
+    xyz, box, top, p, terms, ns = load_snare()
+    ref = snare_expected()
+    assert ns == 4 and top.n == 4 * ref["atoms"] and p.grid == (48, 64, 160)
+    e = oracle.evaluate(oracle.System(xyz, box, top), p)[0]
+    assert abs(e["vdw"] / ns - ref["vdw"]) <= 1e-8
+    assert abs(e["perm"] / ns - ref["perm"]) <= 1e-8
+    assert abs(e["pol"] / ns - ref["pol"]) <= ref["tolerance"]
+    assert e["n_vdw"] == ns * ref["n_vdw"] and e["n_perm"] == ns * ref["n_perm"]
+    v = oracle.valence(terms, xyz)[0]
+    counts = terms.counts()
+    for k in KINDS:
+        assert counts[k] == ns * ref["valence"][k][1], k
+        assert abs(v[k] / ns - ref["valence"][k][0]) <= 1e-8, (k, v[k] / ns, ref["valence"][k][0])

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from forcefieldx_amd import ffi
-from test_crystal import KINDS, NAMES, expected, load_crystal
+from test_crystal import KINDS, NAMES, expected, load_crystal, load_snare, snare_expected
 
 pytestmark = pytest.mark.gpu
 
@@ -73,3 +73,44 @@ def test_crystal_mixed_against_fp64(name):
     assert eun["pcg_iters"] == e64["pcg_iters"]
     assert emx["pcg_iters"] >= e64["pcg_iters"]
     assert abs(emx["total"] - e64["total"]) <= 1e-5 * max(1.0, abs(e64["perm"]))
+
+
+def test_snare_p212121_matches_oracle_and_energy_test(oracle):
+    """The SNARE crystal (P212121 expanded to 21,428 atoms, PME 48 x 64 x
+    160): FP64 nonbonded and valence energies, gradient and dipoles equal the
+    oracle at 1e-8, the per-asymmetric-unit vdW and permanent energies equal
+    EnergyTest's at 1e-8 kcal/mol and the polarization energy lies within its
+    1e-2; the mixed form's forces lie within 1e-5 (relative RMS) of FP64."""
+    xyz, box, top, p, terms, ns = load_snare()
+    ref = snare_expected()
+    print(f"SNARE P212121: {top.n} atoms, grid {p.grid}", flush=True)
+    eo, go, uo, _ = oracle.evaluate(oracle.System(xyz, box, top), p)
+    vo, gvo = oracle.valence(terms, xyz)
+    out = {}
+    for prec in (ffi.NB_PREC_FP64, ffi.NB_PREC_MIXED):
+        h = ffi.Handle(ffi.params_from(p, p.grid, prec), top)
+        try:
+            h.set_cell(box)
+            h.set_positions(xyz.reshape(-1))
+            h.set_valence(terms)
+            e, rc = h.evaluate(ffi.NB_GRAD)
+            out[prec] = (e, h.gradient().reshape(-1, 3), h.induced()[0])
+        finally:
+            h.close()
+    e, g, u = out[ffi.NB_PREC_FP64]
+    print(f"FP64: vdW {e['vdw'] / ns:.8f} perm {e['perm'] / ns:.8f} pol {e['pol'] / ns:.8f}, PCG {e['pcg_iters']}",
+          flush=True)
+    assert e["pcg_iters"] == eo["pcg_iters"]
+    for k in ("perm_real", "perm_recip", "pol", "vdw"):
+        assert abs(e[k] - eo[k]) <= 1e-8 * abs(eo["perm"]), (k, e[k], eo[k])
+    for k in KINDS:
+        assert abs(e["valence_" + k] - vo[k]) <= 1e-10 * max(1.0, abs(vo[k])), k
+    assert abs(e["vdw"] / ns - ref["vdw"]) <= 1e-8 and abs(e["perm"] / ns - ref["perm"]) <= 1e-8
+    assert abs(e["pol"] / ns - ref["pol"]) <= ref["tolerance"]
+    gt = go + gvo
+    assert np.linalg.norm(g - gt) <= 1e-8 * np.linalg.norm(gt)
+    assert np.linalg.norm(u - uo) <= 1e-8 * np.linalg.norm(uo)
+    em, gm, _ = out[ffi.NB_PREC_MIXED]
+    err = np.linalg.norm(gm - g) / np.linalg.norm(g)
+    print(f"mixed: force relative RMS error {err:.2e}", flush=True)
+    assert err <= 1e-5
