@@ -79,7 +79,7 @@ void free_all(NbHandle *h) {
   if (h->child_env) { free_all(h->child_env); delete h->child_env; h->child_env = nullptr; }
   if (h->child_vap) { free_all(h->child_vap); delete h->child_vap; h->child_vap = nullptr; }
   void *lam[] = {h->d_lgrad_s, h->d_ltorque_s, h->d_lgrad_r, h->d_ltorque_r, h->d_vap_map, h->d_tape, h->d_tape_sc,
-                 h->d_fin, h->d_gfin};
+                 h->d_fin, h->d_gfin, h->d_vfin};
   for (void *p : lam)
     if (p) hipFree(p);
   void *ptrs[] = {h->d_frame, h->d_axis, h->d_group0, h->d_vclass0, h->d_redidx, h->d_mpole_local, h->d_polar0,
@@ -869,10 +869,6 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
     h->want_dudl = (flags & NB_DUDL) != 0;
     h->dudl_valid = false;
     h->want_finite = (flags & NB_FINITE_PCG) != 0 && h->prm.polarization == NB_POL_MUTUAL;
-    if (h->want_finite && (h->want_vir || h->ost || h->elec_lambda)) {
-      h->err = "NB_FINITE_PCG does not combine with the virial or electrostatic alchemistry";
-      return NB_ERR_INVALID;
-    }
     if (h->want_finite && !h->d_gfin) HIPCHECK(hipMalloc(&h->d_gfin, sizeof(double) * 3 * n));
     if (h->want_vir) {
       // an aperiodic system has pair and frame terms only
@@ -1070,7 +1066,7 @@ int nb_evaluate(void *handle, uint32_t flags, nb_energies *out) {
     nb_energies e;
     int status = nb_evaluate_one(h, flags, &e);
     if (status != NB_OK) return status;
-    uint32_t cf = flags & (NB_GRAD | NB_VIRIAL | NB_DUDL);
+    uint32_t cf = flags & (NB_GRAD | NB_VIRIAL | NB_DUDL | NB_FINITE_PCG);
     NbHandle *kids[2] = {h->child_env, h->child_vap};
     for (int c = 0; c < 2; c++) {
       NbHandle *k = kids[c];

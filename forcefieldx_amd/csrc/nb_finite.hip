@@ -19,6 +19,13 @@
 //    own field kernels), Fhat into d_up, and the tensor-derivative terms into
 //    d_gfin: k_bilinear over the Ewald / preconditioner rows and, per T pair,
 //    the reciprocal term from the PME potentials of the pair.
+//  * With the virial, the same terms' strain derivatives: the pair part
+//    sum (dW/dR)_a R_b of every row entry (k_bilinear<VIR>), and per T pair
+//    (a, b) the reciprocal mutual-term strain derivative with (u, uCR) ->
+//    (a, b): the atom part -(el/2)(b_b dphi_a(a) + a_b dphi_a(b))
+//    (k_recip_bilinear_vir) and the grid part from the transforms of a and b
+//    (nb_grid_virial_pair).  DirectGrad's virial comes from the energy
+//    kernels in their direct form, with uCR replaced by Fhat.
 #include <cmath>
 #include <stdexcept>
 #include <vector>
@@ -126,17 +133,22 @@ __global__ void k_seed(int n, const double *__restrict__ polar, const double *__
 // vectors scaled by alpha.  For a pair, W = a_i.T b_k + a_k.T b_i =
 // -B1 S1 + B2 S2 with S1 = a_i.b_k + a_k.b_i, S2 = (a_i.R)(b_k.R) +
 // (a_k.R)(b_i.R), R = x_k - x_i, and dW/dx_i = -grad_R W.
-template <int LPA, bool PRE>
+// VIR: also the strain derivative of the same terms, half of coef G_a R_b per
+// visit of a row entry (G the pair's sum of w_j grad_R W_j; the i-side
+// gradient of the pair is -coef G and R = x_k - x_i; the pair convention of
+// the energy kernels, (dE/dR)_a R_b), 9 partial sums per block
+template <int LPA, bool PRE, bool VIR>
 __global__ void __launch_bounds__(256) k_bilinear(int n, const double4 *__restrict__ pos, const AtomMP *__restrict__ mp,
                                                   const uint32_t *__restrict__ lst, const int32_t *__restrict__ cnt,
                                                   int maxl, CellD cell, double aewald, const double *__restrict__ A,
                                                   const double *__restrict__ B, const double *__restrict__ w, int J,
                                                   const double *__restrict__ polar, double coef,
-                                                  double *__restrict__ gout) {
+                                                  double *__restrict__ gout, double *__restrict__ vpart) {
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt / LPA, lane = gt % LPA;
   bool valid = s < n;
   V3<double> G{0, 0, 0};
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const size_t n3 = 3 * (size_t)n;
   if (valid) {
     double4 pi = pos[s];
@@ -168,6 +180,7 @@ __global__ void __launch_bounds__(256) k_bilinear(int n, const double4 *__restri
       }
       V3<double> dB1 = rr[1] * th.d3 - C1 * R, dB2 = rr[2] * th.d5 - C2 * R;
       double ksc = PRE ? polar[k] : 1.0;
+      V3<double> Gp{0, 0, 0};
       for (int j = 0; j < J; j++) {
         const double *Aj = A + j * n3, *Bj = B + j * n3;
         V3<double> a_i = asc * ld3d(Aj, s), b_i = asc * ld3d(Bj, s), a_k = ksc * ld3d(Aj, k), b_k = ksc * ld3d(Bj, k);
@@ -175,12 +188,56 @@ __global__ void __launch_bounds__(256) k_bilinear(int n, const double4 *__restri
         double air = dot(a_i, R), akr = dot(a_k, R), bir = dot(b_i, R), bkr = dot(b_k, R);
         double S2 = air * bkr + akr * bir;
         V3<double> gW = (-S1) * dB1 + S2 * dB2 + B2 * (bkr * a_i + air * b_k + bir * a_k + akr * b_i);
-        G += w[j] * gW;
+        Gp += w[j] * gW;
+      }
+      G += Gp;
+      if (VIR) {
+        const double r3[3] = {R.x, R.y, R.z}, g3[3] = {Gp.x, Gp.y, Gp.z}, hc = 0.5 * coef;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+          for (int b = 0; b < 3; b++) w9[3 * a + b] += hc * g3[a] * r3[b];
       }
     }
   }
+  if (VIR) block_store9(w9, vpart);
   group_sum3<LPA>(G);
   if (valid && lane == 0) add3(gout, s, (-coef) * G);
+}
+
+// Atom part of the reciprocal strain derivative of a T pair (a, b): the
+// mutual term of the reciprocal virial with (u, uCR) -> (a, b),
+// -scale (b_b dphi_a(a) + a_b dphi_a(b)) per atom, from the Cartesian
+// potential derivatives of the two (phia / phib [30 s + 21 + a]); 9 partial
+// sums per block
+__global__ void __launch_bounds__(128) k_recip_bilinear_vir(int n, const double *__restrict__ phia,
+                                                            const double *__restrict__ phib,
+                                                            const double *__restrict__ a, const double *__restrict__ b,
+                                                            double scale, double *__restrict__ vpart) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < n) {
+    const double *ca = phia + 30 * (size_t)s + 21, *cb = phib + 30 * (size_t)s + 21;
+    for (int x = 0; x < 3; x++)
+      for (int y = 0; y < 3; y++) w9[3 * x + y] = -scale * (b[3 * s + y] * ca[x] + a[3 * s + y] * cb[x]);
+  }
+  block_store9(w9, vpart);
+}
+
+// scale times the sum of nblk blocks of 9 partial sums, in block order, into out[9]
+__global__ void k_vir_sum(long nblk, const double *__restrict__ part, double scale, double *__restrict__ out) {
+  __shared__ double red[9][64];
+  const int q = threadIdx.x / 64, l = threadIdx.x % 64;
+  double t = 0.0;
+  if (q < 9)
+    for (long b = l; b < nblk; b += 64) t += part[9 * b + q];
+  if (q < 9) red[q][l] = t;
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    double v = 0.0;
+    for (int i = 0; i < 64; i++) v += red[threadIdx.x][i];
+    out[threadIdx.x] = scale * v;
+  }
 }
 
 // Reciprocal part of coef_j a . dT b from the 20 fractional potential
@@ -205,9 +262,13 @@ __global__ void k_recip_bilinear(int n, Frac fr, const double *__restrict__ phia
   for (int c = 0; c < 3; c++) gout[3 * s + c] += scale * (fr.a[0][c] * g[0] + fr.a[1][c] * g[1] + fr.a[2][c] * g[2]);
 }
 
-__global__ void k_add_vec(int m, const double *__restrict__ x, double *__restrict__ y) {
+// y += c x, and (y2 non-null) y2 += c2 x
+__global__ void k_add_vec(int m, const double *__restrict__ x, double c, double *__restrict__ y, double c2,
+                          double *__restrict__ y2) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < m) y[q] += x[q];
+  if (q >= m) return;
+  y[q] += c * x[q];
+  if (y2) y2[q] += c2 * x[q];
 }
 
 }  // namespace
@@ -407,27 +468,67 @@ void nb_finite_adjoint(NbHandle *h) {
   double coef = -0.5 * h->prm.electric;
   dim3 Gp = xcd_grid((long)F.n * FIN_LPA, 256);
   double aw = F.recip ? h->prm.ewald_alpha : 0.0;
-  NB_LAUNCH((k_bilinear<FIN_LPA, false>), Gp, F.B, 0, F.n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew, h->cell,
-            aw, (const double *)AT, (const double *)BT, (const double *)wT, JT, (const double *)h->d_polar, coef,
-            h->d_gfin);
-  NB_LAUNCH((k_bilinear<FIN_LPA, true>), Gp, F.B, 0, F.n, h->d_pos4, h->d_mp, h->d_pc, h->d_npc, h->maxpc, h->cell,
-            0.0, (const double *)AM, (const double *)BM, (const double *)wM, JM, (const double *)h->d_polar, coef,
-            h->d_gfin);
+  // with the virial: 9 partial sums per block of every launch below in
+  // d_vfin, summed at the end into one block of the evaluation's virial
+  const bool vir = h->want_vir;
+  const long nbil = Gp.x, natom = grid_for(F.n, 128).x, ngrid = F.recip ? nb_grid_virial_blocks(h) : 0;
+  const long vneed = vir ? 2 * nbil + (F.recip ? (long)JT * (natom + ngrid) : 0) : 0;
+  if (vir && vneed > h->vfin_cap) {
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    if (h->d_vfin) (void)hipFree(h->d_vfin);
+    h->d_vfin = nullptr;
+    HIPCHECK(hipMalloc(&h->d_vfin, sizeof(double) * 9 * vneed));
+    h->vfin_cap = vneed;
+  }
+  long vused = 0;
+  auto vregion = [&](long nblk) {
+    double *p = vir ? h->d_vfin + 9 * vused : nullptr;
+    vused += vir ? nblk : 0;
+    return p;
+  };
+  if (vir) {
+    NB_LAUNCH((k_bilinear<FIN_LPA, false, true>), Gp, F.B, 0, F.n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+              h->cell, aw, (const double *)AT, (const double *)BT, (const double *)wT, JT,
+              (const double *)h->d_polar, coef, h->d_gfin, vregion(nbil));
+    NB_LAUNCH((k_bilinear<FIN_LPA, true, true>), Gp, F.B, 0, F.n, h->d_pos4, h->d_mp, h->d_pc, h->d_npc, h->maxpc,
+              h->cell, 0.0, (const double *)AM, (const double *)BM, (const double *)wM, JM,
+              (const double *)h->d_polar, coef, h->d_gfin, vregion(nbil));
+  } else {
+    NB_LAUNCH((k_bilinear<FIN_LPA, false, false>), Gp, F.B, 0, F.n, h->d_pos4, h->d_mp, h->d_ew, h->d_new, h->maxew,
+              h->cell, aw, (const double *)AT, (const double *)BT, (const double *)wT, JT,
+              (const double *)h->d_polar, coef, h->d_gfin, (double *)nullptr);
+    NB_LAUNCH((k_bilinear<FIN_LPA, true, false>), Gp, F.B, 0, F.n, h->d_pos4, h->d_mp, h->d_pc, h->d_npc, h->maxpc,
+              h->cell, 0.0, (const double *)AM, (const double *)BM, (const double *)wM, JM,
+              (const double *)h->d_polar, coef, h->d_gfin, (double *)nullptr);
+  }
   if (F.recip) {
     Frac fr;
     for (int x = 0; x < 3; x++)
       for (int y = 0; y < 3; y++) fr.a[x][y] = h->afrac[x][y];
     for (int j = 0; j < JT; j++) {
       const double *Aj = AT + (size_t)j * n3, *Bj = BT + (size_t)j * n3;
+      const double scale = w_t[j] * 0.5 * h->prm.electric;
       nb_pme_induced(h, Aj, Bj, true);
       NB_LAUNCH(k_recip_bilinear, grid_for(F.n, 128), dim3(128), 0, F.n, fr, (const double *)h->d_phiu,
-                (const double *)h->d_phiup, Aj, Bj, w_t[j] * 0.5 * h->prm.electric, h->d_gfin);
+                (const double *)h->d_phiup, Aj, Bj, scale, h->d_gfin);
+      if (vir) {
+        NB_LAUNCH(k_recip_bilinear_vir, grid_for(F.n, 128), dim3(128), 0, F.n, (const double *)h->d_phiu,
+                  (const double *)h->d_phiup, Aj, Bj, scale, vregion(natom));
+        nb_grid_virial_pair(h, Aj, Bj, scale, vregion(ngrid));
+      }
     }
     // reciprocal potentials of (u, Fhat) for the energy and DirectGrad
     nb_pme_induced(h, h->d_u, h->d_up, true);
   }
+  // the polarization energy of an OST phase is scaled by its pol factor
+  if (vir)
+    NB_LAUNCH(k_vir_sum, dim3(1), dim3(576), 0, vused, (const double *)h->d_vfin, h->ph.pol, nb_vir_region(h, 1));
 }
 
+// the tensor terms into the gradient, scaled by the phase's pol factor (1
+// outside OST), and under OST their lambda derivative (dpol) into the lambda
+// gradient: the dipoles do not depend on lambda, only the energy's scale does
 void nb_finite_add(NbHandle *h) {
-  NB_LAUNCH(k_add_vec, grid_for(3 * h->n, 256), dim3(256), 0, 3 * h->n, (const double *)h->d_gfin, h->d_grad_s);
+  NB_LAUNCH(k_add_vec, grid_for(3 * h->n, 256), dim3(256), 0, 3 * h->n, (const double *)h->d_gfin, h->ph.pol,
+            h->d_grad_s, h->ph.dpol, h->ost ? h->d_lgrad_s : (double *)nullptr);
 }

@@ -1051,6 +1051,50 @@ __global__ void __launch_bounds__(256) k_grid_virial(int K0, int K1, int K2, con
   block_store9(w9, vpart);
 }
 
+// The mutual part alone for a pair of dipole sets packed as one grid (a real,
+// b imaginary): w = scale recip(m) Re Fa* Fb (2 (1 + pi^2 m^2 / a^2) m_a m_b /
+// m^2 - delta_ab)
+template <typename T>
+__global__ void __launch_bounds__(256) k_grid_virial_pair(int K0, int K1, int K2, const CX<T> *__restrict__ fi,
+                                                          const double *__restrict__ recip, Frac2v A, double pt,
+                                                          double scale, double *__restrict__ vpart) {
+  long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ntot = (long)K0 * K1 * K2;
+  double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (g < ntot && recip[g] != 0.0) {
+    int kx = (int)(g % K0);
+    long r1 = g / K0;
+    int ky = (int)(r1 % K1), kz = (int)(r1 / K1);
+    int hh = kx >= (K0 + 1) / 2 ? kx - K0 : kx, k = ky >= (K1 + 1) / 2 ? ky - K1 : ky;
+    int l = kz >= (K2 + 1) / 2 ? kz - K2 : kz;
+    double m[3];
+    for (int d = 0; d < 3; d++) m[d] = A.a[d][0] * hh + A.a[d][1] * k + A.a[d][2] * l;
+    double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    long ng = (long)((K0 - kx) % K0) + (long)K0 * (((K1 - ky) % K1) + (long)K1 * ((K2 - kz) % K2));
+    double ar = (double)fi[g].x, ai = (double)fi[g].y, br = (double)fi[ng].x, bi = -(double)fi[ng].y;
+    double ur = 0.5 * (ar + br), ui = 0.5 * (ai + bi);
+    double vr = 0.5 * (ai - bi), vi = -0.5 * (ar - br);
+    double w = scale * recip[g] * (ur * vr + ui * vi), f = 2.0 * (1.0 + pt * m2) / m2;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) w9[3 * a + b] = w * (f * m[a] * m[b] - (a == b ? 1.0 : 0.0));
+  }
+  block_store9(w9, vpart);
+}
+
+template <typename T>
+static void grid_virial_pair_t(NbHandle *h, const double *a, const double *b, double scale, double *vpart,
+                               const CX<T> *tw) {
+  CX<T> *g = (CX<T> *)h->d_grid;
+  spread_into<T, true>(h, a, b);
+  forward_t<T>(h, 2, g, tw);
+  Frac2v A;
+  for (int x = 0; x < 3; x++)
+    for (int y = 0; y < 3; y++) A.a[x][y] = h->cell.A[x][y];
+  double aw = h->prm.ewald_alpha, pt = (M_PI / aw) * (M_PI / aw);
+  NB_LAUNCH((k_grid_virial_pair<T>), grid_for(h->ntot, 256), dim3(256), 0, h->K[0], h->K[1], h->K[2],
+            (const CX<T> *)g, h->d_recip, A, pt, scale, vpart);
+}
+
 template <typename T>
 static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
   CX<T> *g = (CX<T> *)h->d_grid, *fpg = (CX<T> *)h->d_fgrid;
@@ -1066,7 +1110,8 @@ static void grid_virial_t(NbHandle *h, bool polar, const CX<T> *tw) {
   double aw = h->prm.ewald_alpha, pt = (M_PI / aw) * (M_PI / aw);
   dim3 G = grid_for(h->ntot, 256);
   NB_LAUNCH((k_grid_virial<T>), G, dim3(256), 0, h->K[0], h->K[1], h->K[2], (const CX<T> *)fpg,
-            polar ? (const CX<T> *)g : (const CX<T> *)nullptr, (int)(h->prm.polarization == NB_POL_MUTUAL), h->d_recip,
+            polar ? (const CX<T> *)g : (const CX<T> *)nullptr,
+            (int)(h->prm.polarization == NB_POL_MUTUAL && !h->finite_active), h->d_recip,
             A, pt, h->prm.electric, h->ph.perm, h->ph.pol, nb_vir_region(h, G.x));
 }
 
@@ -1235,6 +1280,13 @@ void nb_grid_virial(NbHandle *h, bool polar) {
   if (!h->d_fgrid) HIPCHECK(hipMalloc(&h->d_fgrid, sizeof(double) * 2 * h->ntot));
   if (mixed_grid(h)) grid_virial_t<float>(h, polar, (const CX<float> *)h->d_twf);
   else grid_virial_t<double>(h, polar, (const CX<double> *)h->d_tw);
+}
+
+long nb_grid_virial_blocks(const NbHandle *h) { return grid_for(h->ntot, 256).x; }
+
+void nb_grid_virial_pair(NbHandle *h, const double *a, const double *b, double scale, double *vpart) {
+  if (mixed_grid(h)) grid_virial_pair_t<float>(h, a, b, scale, vpart, (const CX<float> *)h->d_twf);
+  else grid_virial_pair_t<double>(h, a, b, scale, vpart, (const CX<double> *)h->d_tw);
 }
 
 Radices nb_factor(int N) { return factor(N); }

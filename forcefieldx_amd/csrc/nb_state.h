@@ -194,6 +194,8 @@ struct NbHandle {
   double *d_fin = nullptr;        // adjoint work vectors and tensor pairs
   size_t fin_cap = 0;
   double *d_gfin = nullptr;       // [3n] tensor-derivative gradient terms (sorted)
+  double *d_vfin = nullptr;       // [vfin_cap][9] strain-derivative partial sums of those terms
+  long vfin_cap = 0;
   bool induced_caller = false;    // d_u / d_up hold caller order (written by the batch kernel)
 
   // valence terms (nb_set_valence): per-term gradient contributions go to
@@ -286,6 +288,10 @@ void nb_finalize_gradient(NbHandle *h);
 void nb_zero_outputs(NbHandle *h);
 double *nb_vir_region(NbHandle *h, long nblocks);
 void nb_grid_virial(NbHandle *h, bool polar);
+// grid part of the reciprocal strain derivative of scale * (a . T_rec b)
+// terms (the mutual term's form), 9 partial sums per block into vpart
+long nb_grid_virial_blocks(const NbHandle *h);
+void nb_grid_virial_pair(NbHandle *h, const double *a, const double *b, double scale, double *vpart);
 void nb_vir_reduce(NbHandle *h);
 void nb_tape_push(NbHandle *h, int nrz, int npq);
 void nb_pred_apply(NbHandle *h);

@@ -54,9 +54,12 @@ enum { NB_PREC_FP64 = 0, NB_PREC_MIXED = 1 };
  * the gradient of the energy of the dipoles the PCG actually returned, with
  * its iteration count held fixed -- the adjoint of the recorded iterations
  * replaces uCR and the mutual terms.  Mutual polarization solved by PCG only
- * (no effect otherwise); not with NB_VIRIAL or electrostatic alchemistry
- * (NB_ERR_INVALID); evaluated on the regular path (no graph replay).  After
- * it, nb_get_induced returns the adjoint dipoles as ucr. */
+ * (no effect otherwise; a phase solved by SOR keeps the stationary form).
+ * With NB_VIRIAL the strain derivative of the same energy; under OST phases
+ * the polarization energy of each phase is its pol factor times the energy of
+ * lambda-independent dipoles, so NB_DUDL and dU/dX/dL follow.  Evaluated on
+ * the regular path (no graph replay).  After it, nb_get_induced returns the
+ * adjoint dipoles as ucr. */
 enum { NB_GRAD = 1u, NB_VIRIAL = 2u, NB_DUDL = 4u, NB_FINITE_PCG = 8u, NB_BATCH_SEPARATE = 16u };
 
 /* SCF algorithms (SCFAlgorithm.java) */

@@ -1,8 +1,9 @@
 """NB_FINITE_PCG on the GPU: the gradient of the dipoles the PCG returned
 (iteration count fixed) against the oracle's, whose adjoint terms are pinned
 by central differences (tests/test_oracle_finite_pcg.py), and against central
-differences of the GPU energy itself.  FP64 at 1e-8 of the oracle; the mixed
-form within 1e-5 of FP64."""
+differences of the GPU energy itself; its virial against central differences
+of the GPU energy under strain.  FP64 at 1e-8 of the oracle; the mixed form
+within 1e-5 of FP64."""
 import dataclasses
 
 import numpy as np
@@ -20,12 +21,14 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def gpu(xyz, box, top, p, flags, precision=ffi.NB_PREC_FP64):
+def gpu(xyz, box, top, p, flags, precision=ffi.NB_PREC_FP64, virial=False):
     h = ffi.Handle(ffi.params_from(p, p.resolved_grid(box) if box is not None else (16, 16, 16), precision), top)
     try:
         h.set_cell(box)
         h.set_positions(xyz.reshape(-1))
         e, rc = h.evaluate(flags)
+        if virial:
+            return e, h.gradient().reshape(-1, 3), h.virial()
         return e, h.gradient().reshape(-1, 3)
     finally:
         h.close()
@@ -97,13 +100,102 @@ def test_converged_finite_equals_stationary():
     assert rel(gf, gs) <= 1e-8
 
 
-def test_finite_with_virial_is_rejected():
+@pytest.mark.parametrize("case", [0, 1, 2], ids=["water", "chain", "thin triclinic"])
+def test_finite_virial_central_differences(case):
+    """The strain derivative of the energy of the dipoles the PCG returned
+    (NB_FINITE_PCG | NB_VIRIAL) against central differences of the GPU energy
+    under symmetric strains of cell and atoms (h = 1e-7, iteration count and
+    pair counts unchanged, so that no pair crosses a cutoff): the adjoint's
+    pair and reciprocal tensor terms, and DirectGrad's, with the grid and atom
+    parts of the reciprocal mutual term taken per (a, b) pair; W is symmetric.
+    The stationary virial misses by far more."""
+    from test_oracle_physics import strained
+    name, xyz, box, top, p = CASES[case]
+    xyz = xyz.reshape(-1, 3)
+    box = list(box)
+    flags = ffi.NB_GRAD | ffi.NB_VIRIAL
+    e0, _, W = gpu(xyz, box, top, p, flags | ffi.NB_FINITE_PCG, virial=True)
+    _, _, Ws = gpu(xyz, box, top, p, flags, virial=True)
+    h = 1e-7
+    fd = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(a, 3):
+            E = np.zeros((3, 3))
+            E[a, b] += 0.5
+            E[b, a] += 0.5
+            ep, _ = gpu(*strained(xyz, box, h * E), top, p, 0)
+            em, _ = gpu(*strained(xyz, box, -h * E), top, p, 0)
+            assert ep["pcg_iters"] == e0["pcg_iters"] == em["pcg_iters"]
+            assert (ep["n_perm"], ep["n_vdw"]) == (em["n_perm"], em["n_vdw"]), (a, b)
+            fd[a, b] = fd[b, a] = (ep["total"] - em["total"]) / (2 * h)
+    scale = np.abs(fd).max()
+    print(f"{name}: finite virial err {np.abs(W - fd).max() / scale:.2e}, stationary "
+          f"{np.abs(Ws - fd).max() / scale:.2e}", flush=True)
+    assert np.abs(W - W.T).max() <= 1e-8 * scale
+    assert np.abs(W - fd).max() <= 1e-6 * scale, (W, fd)
+    assert np.abs(Ws - fd).max() >= 1e-5 * scale
+
+
+def test_finite_virial_mixed_against_fp64():
     name, xyz, box, top, p = CASES[0]
-    h = ffi.Handle(ffi.params_from(p, p.resolved_grid(box), ffi.NB_PREC_FP64), top)
+    flags = ffi.NB_GRAD | ffi.NB_VIRIAL | ffi.NB_FINITE_PCG
+    _, _, W64 = gpu(xyz, box, top, p, flags, virial=True)
+    _, _, Wmx = gpu(xyz, box, top, p, flags, ffi.NB_PREC_MIXED, virial=True)
+    assert np.abs(Wmx - W64).max() <= 1e-5 * np.abs(W64).max()
+
+
+def ost_eval(xyz, box, top, p, lam, flags):
+    h = ffi.Handle(ffi.params_from(p, p.grid, ffi.NB_PREC_FP64), top)
     try:
         h.set_cell(box)
+        h.set_lambda(lam)
         h.set_positions(xyz.reshape(-1))
-        e, rc = h.evaluate(ffi.NB_GRAD | ffi.NB_FINITE_PCG | ffi.NB_VIRIAL, check=False)
-        assert rc == ffi.NB_ERR_INVALID
+        e, rc = h.evaluate(flags)
+        out = dict(e=e, g=h.gradient().reshape(-1, 3))
+        if flags & ffi.NB_DUDL:
+            out["l"] = h.dudx_dl().reshape(-1, 3)
+        return out
     finally:
         h.close()
+
+
+def test_finite_gradient_with_ost_phases():
+    """OST electrostatics (4 soft chain atoms, the condensed phase and the
+    no-ligand phase each solved by PCG to 0.02 D; no vapour phase, which the
+    reference solves by SOR): every phase's polarization energy is its pol
+    factor times the energy of dipoles that do not depend on lambda, so the
+    finite-PCG gradient of each phase is scaled by pol and its lambda
+    derivative by dpol.  Against central differences of the GPU energy: the
+    gradient in x (h = 1e-5 A), dU/dL and dU/dX/dL in lambda (h = 1e-5); the
+    stationary gradient misses."""
+    xyz, box, top = chain_in_water(24, 60, 13.0, seed=5, n_soft=4)
+    xyz = xyz.reshape(-1, 3)
+    p = NonbondedParams(vdw_cutoff=6.0, ewald_cutoff=6.0, grid=(16, 16, 16), polar_eps=0.02, elec_lambda=True,
+                        ligand_vapor_elec=False)
+    lam, h = 0.6, 1e-5
+    fl = ffi.NB_GRAD | ffi.NB_DUDL
+    r = ost_eval(xyz, box, top, p, lam, fl | ffi.NB_FINITE_PCG)
+    rs = ost_eval(xyz, box, top, p, lam, fl)
+    g, lg, k0 = r["g"], r["l"], r["e"]["pcg_iters"]
+    soft = np.nonzero(top.soft)[0]
+    worst = 0.0
+    for i in (int(soft[0]), int(soft[-1]) + 1, 30, 100):
+        for d in range(3):
+            xp = xyz.copy(); xp[i, d] += h
+            xm = xyz.copy(); xm[i, d] -= h
+            ep = ost_eval(xp, box, top, p, lam, 0)["e"]
+            em = ost_eval(xm, box, top, p, lam, 0)["e"]
+            assert ep["pcg_iters"] == k0 == em["pcg_iters"]
+            fd = (ep["total"] - em["total"]) / (2 * h)
+            worst = max(worst, abs(g[i, d] - fd) / np.abs(g).max())
+    print(f"OST finite gradient: worst {worst:.2e}, stationary vs finite {rel(rs['g'], g):.2e}", flush=True)
+    assert worst <= 1e-6
+    assert rel(rs["g"], g) >= 1e-5
+    rp = ost_eval(xyz, box, top, p, lam + h, fl | ffi.NB_FINITE_PCG)
+    rm = ost_eval(xyz, box, top, p, lam - h, fl | ffi.NB_FINITE_PCG)
+    assert rp["e"]["pcg_iters"] == k0 == rm["e"]["pcg_iters"]
+    dudl = (rp["e"]["total"] - rm["e"]["total"]) / (2 * h)
+    print(f"dU/dL {r['e']['dudl']:.8f} / {dudl:.8f}", flush=True)
+    assert abs(r["e"]["dudl"] - dudl) <= 1e-6 * max(1.0, abs(dudl))
+    lfd = (rp["g"] - rm["g"]) / (2 * h)
+    assert np.abs(lg - lfd).max() <= 1e-5 * np.abs(lfd).max(), np.abs(lg - lfd).max() / np.abs(lfd).max()
