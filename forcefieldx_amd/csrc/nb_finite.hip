@@ -39,8 +39,10 @@ namespace {
 
 constexpr int FIN_LPA = 16;
 
-// totals of the r.z and p.q partial sums of an iteration (slots S_RZN, S_PAP)
-__global__ void k_tape_scal(const double *__restrict__ part, int stride, int nrz, int npq, double *__restrict__ out) {
+// totals of the r.z and p.q partial sums of an iteration (slots S_RZN, S_PAP),
+// or the collapsed totals in scal when a count is negative
+__global__ void k_tape_scal(const double *__restrict__ part, int stride, int nrz, int npq,
+                            const double *__restrict__ scal, double *__restrict__ out) {
   __shared__ double red[2][4];
   double v[2] = {0.0, 0.0};
   for (int i = threadIdx.x; i < nrz; i += blockDim.x) v[0] += part[(size_t)S_RZN * stride + i];
@@ -49,8 +51,12 @@ __global__ void k_tape_scal(const double *__restrict__ part, int stride, int nrz
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) { red[0][w] = v[0]; red[1][w] = v[1]; }
   __syncthreads();
-  if (threadIdx.x < 2) out[threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
-                                          red[threadIdx.x][3];
+  if (threadIdx.x < 2) {
+    const bool collapsed = threadIdx.x == 0 ? nrz < 0 : npq < 0;
+    out[threadIdx.x] = collapsed ? scal[threadIdx.x == 0 ? S_RZT : S_PQT]
+                                 : red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                                       red[threadIdx.x][3];
+  }
 }
 
 // out = c0 x0 + c1 x1 + c2 x2 (null terms skipped; out may be x0)
@@ -310,7 +316,8 @@ void nb_tape_push(NbHandle *h, int nrz, int npq) {
   const double *src[4] = {h->d_r, h->d_z, h->d_p, h->d_vec};
   for (int v = 0; v < 4; v++)
     HIPCHECK(hipMemcpyAsync(slot + v * n3, src[v], sizeof(double) * n3, hipMemcpyDeviceToDevice, h->stream));
-  NB_LAUNCH(k_tape_scal, dim3(1), dim3(256), 0, h->d_part, h->part_stride, nrz, npq, h->d_tape_sc + 2 * k);
+  NB_LAUNCH(k_tape_scal, dim3(1), dim3(256), 0, h->d_part, h->part_stride, nrz, npq, (const double *)h->d_scal,
+            h->d_tape_sc + 2 * k);
   h->tape_K = k + 1;
 }
 

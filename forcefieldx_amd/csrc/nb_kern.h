@@ -14,9 +14,16 @@ enum { E_PERM_REAL = 0, E_PERM_SELF, E_PERM_RECIP, E_POL_REAL, E_POL_SELF, E_POL
 // of the next conjugation)
 // S_RDONE stops the FP64 refinement residuals recorded after a solve once
 // one of them has passed the test; S_RCYC counts the recorded corrections a
-// replay reopened the solve for
+// replay reopened the solve for.  S_RZT / S_PQT hold the r.z and p.q totals
+// when their partial sums were collapsed by k_part_totals (a negative count
+// passed to the consumers says so).
 enum { S_PAP = 0, S_PAPP, S_RZ, S_RZP, S_EPS, S_EPSP, S_RZN, S_RZPN, S_DONE, S_ITER, S_PREV, S_STAT, S_EPSV,
-       S_RZOLD, S_RZOLDP, S_RDONE, S_RCYC, S_NSLOT };
+       S_RZOLD, S_RZOLDP, S_RDONE, S_RCYC, S_RZT, S_RZTP, S_PQT, S_PQTP, S_NSLOT };
+// Partial sums of more blocks than this are collapsed into totals by one
+// block before the PCG kernels that need them: otherwise every consumer block
+// would read every producer block's partial sum (at 288,000 atoms the 9,000
+// blocks of the gather would each read the 18,000 r.z partials).
+constexpr int NB_TOTALS_MIN = 2048;
 // pinned host slots of an evaluation's read-back (doubles)
 enum { P_ENERGY = 0, P_COUNT = 8, P_FLAGS = 12, P_VIR = 16, P_DUDL = 32, P_SCAL = 40, P_VALENCE = 64, P_NSLOT = 80 };
 static_assert(P_SCAL + S_NSLOT <= P_VALENCE, "PCG scalars overlap the valence energies in the read-back");
@@ -178,7 +185,12 @@ static __device__ __forceinline__ void pcg_beta(const double *__restrict__ part,
   if (first) return;
   const int in[2] = {S_RZN, S_RZPN};
   double t[2];
-  block_totals<2>(part, in, stride, nrz, t);
+  if (nrz < 0) {
+    t[0] = scal[S_RZT];
+    t[1] = scal[S_RZTP];
+  } else {
+    block_totals<2>(part, in, stride, nrz, t);
+  }
   double d0 = scal[S_RZOLD], d1 = scal[S_RZOLDP];
   beta[0] = t[0] / (d0 == 0.0 ? 1.0 : d0);
   beta[1] = t[1] / (d1 == 0.0 ? 1.0 : d1);

@@ -157,9 +157,19 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
   if (ctl && ctl[0] != 0.0) return;
   const int in_pq[2] = {S_PAP, S_PAPP}, in_rz[2] = {S_RZN, S_RZPN};
   double pq[2], rz[2];
-  block_totals<2>(part, in_pq, stride, npq, pq);
-  __syncthreads();
-  block_totals<2>(part, in_rz, stride, nrz, rz);
+  if (npq < 0) {
+    pq[0] = scal[S_PQT];
+    pq[1] = scal[S_PQTP];
+  } else {
+    block_totals<2>(part, in_pq, stride, npq, pq);
+    __syncthreads();
+  }
+  if (nrz < 0) {
+    rz[0] = scal[S_RZT];
+    rz[1] = scal[S_RZTP];
+  } else {
+    block_totals<2>(part, in_rz, stride, nrz, rz);
+  }
   double al = rz[0], alp = rz[1];
   if (pq[0] != 0.0) al /= pq[0];
   if (pq[1] != 0.0) alp /= pq[1];
@@ -559,6 +569,31 @@ void nb_precond_field(NbHandle *h, double *f, double *fp) { nb_dip_field_real(h,
 int nb_precond_z(NbHandle *h, int neps, bool control);
 int nb_pme_pcg_direction(NbHandle *h, double a3, int nrz, int first);
 
+// the totals of two rows of partial sums into scal (one block, the fixed
+// order of block_totals, so the consumers see the same values either way)
+__global__ void __launch_bounds__(256) k_part_totals(const double *__restrict__ part, int stride, int nb, int in0,
+                                                     int in1, double *__restrict__ scal, int out0, int out1,
+                                                     const double *__restrict__ ctl) {
+  if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
+  const int in[2] = {in0, in1};
+  double t[2];
+  block_totals<2>(part, in, stride, nb, t);
+  if (threadIdx.x == 0) {
+    scal[out0] = t[0];
+    scal[out1] = t[1];
+  }
+}
+
+// nparts partial sums of rows in0 / in1 for the next consumers: passed on as
+// they are (the count), or collapsed into scal[out0 / out1] on h->cur when
+// there are more than NB_TOTALS_MIN (-1)
+int nb_collapse(NbHandle *h, int nparts, int in0, int in1, int out0, int out1) {
+  if (nparts <= NB_TOTALS_MIN) return nparts;
+  NB_LAUNCH(k_part_totals, dim3(1), dim3(256), 0, (const double *)h->d_part, h->part_stride, nparts, in0, in1,
+            h->d_scal, out0, out1, h->ctl);
+  return -1;
+}
+
 // PCG iterations (PCGSolver.scfByPCG, PCGSolver.java:487-604) from r, z (and
 // the records of z) already in place; nrz: partial sums of the current r.z.
 //   E(z): real space (side stream) || spread + transforms (main stream),
@@ -586,6 +621,8 @@ static int pcg_iterations(NbHandle *h, bool control, int iters, int nrz, int &it
     nb_fork(h);
     {
       NbSide side(h);
+      // r.z totals for the conjugation and the update, beside the spread
+      nrz = nb_collapse(h, nrz, S_RZN, S_RZPN, S_RZT, S_RZTP);
       nb_dip_field_real(h, f, fp, false);
     }
     int npq;
@@ -597,6 +634,7 @@ static int pcg_iterations(NbHandle *h, bool control, int iters, int nrz, int &it
                 h->d_part, h->part_stride, nrz, h->d_scal, (int)(it == 0), h->ctl);
       npq = nb;
     }
+    npq = nb_collapse(h, npq, S_PAP, S_PAPP, S_PQT, S_PQTP);
     if (h->tape_on) nb_tape_push(h, nrz, npq);
     NB_LAUNCH(k_pcg_update, G, B, 0, n, h->d_polar, h->d_vec, h->d_vecp, h->d_p, h->d_pp, h->d_r, h->d_rp, h->d_u,
               h->d_up, h->d_part, h->part_stride, npq, nrz, (void *)h->d_dip2, mx, h->d_scal, h->ctl);
