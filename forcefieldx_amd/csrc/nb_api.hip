@@ -874,7 +874,7 @@ static int nb_evaluate_one(void *handle, uint32_t flags, nb_energies *out) {
       // an aperiodic system has pair and frame terms only
       // 9 partial sums per block of every contributing launch
       long pb = xcd_grid((long)n * 16, 256).x, sb = xcd_grid((long)n * 4, 256).x, ab = (n + 127) / 128;
-      long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + ((long)n * 8 + 255) / 256 + 64;
+      long need = 5 * pb + 2 * sb + 2 * ab + (h->ntot + 255) / 256 + ((long)n * 8 + 255) / 256 + 64 + NB_VIR_STAGE;
       if (need > h->vpart_cap) {
         drop_graph(h);
         if (h->d_vpart) hipFree(h->d_vpart);

@@ -484,17 +484,28 @@ __global__ void k_finalize(int n, const int32_t *__restrict__ perm, const double
   if (vpart) block_store9(w9, vpart);
 }
 
-__global__ void k_vir_reduce(long nblk, const double *__restrict__ part, double *__restrict__ vir) {
-  __shared__ double red[9][4];
-  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int q = 0; q < 9; q++) {
-    double t = 0.0;
-    for (long b = threadIdx.x; b < nblk; b += blockDim.x) t += part[9 * b + q];
-    t = wave_sum_d(t);
-    if (lane == 0) red[q][wv] = t;
-  }
+// Sum of rows of 9 partial sums: block g sums rows [g chunk, (g + 1) chunk)
+// of `part` into out[9 g ..].  252 threads (28 x 9) read the rows as one
+// coalesced stream, thread t always meeting component t % 9; the 28 partial
+// sums of each component are added in thread order.  A first pass over many
+// blocks and a second over its block sums give a fixed order at any size.
+constexpr int VIR_THREADS = 252;
+constexpr long VIR_MAX_BLOCKS = NB_VIR_STAGE;
+__global__ void __launch_bounds__(VIR_THREADS) k_vir_reduce(long nrows, long chunk, const double *__restrict__ part,
+                                                            double *__restrict__ out) {
+  __shared__ double red[VIR_THREADS];
+  const long r0 = (long)blockIdx.x * chunk, r1 = r0 + chunk < nrows ? r0 + chunk : nrows;
+  const long nv = r1 > r0 ? 9 * (r1 - r0) : 0;
+  const double *p = part + 9 * r0;
+  double t = 0.0;
+  for (long v = threadIdx.x; v < nv; v += VIR_THREADS) t += p[v];
+  red[threadIdx.x] = t;
   __syncthreads();
-  if (threadIdx.x < 9) vir[threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+  if (threadIdx.x < 9) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < VIR_THREADS; k += 9) s += red[k];
+    out[9 * (size_t)blockIdx.x + threadIdx.x] = s;
+  }
 }
 
 }  // namespace
@@ -840,12 +851,24 @@ void nb_finalize_gradient(NbHandle *h) {
 // next free region of nblocks x 9 partial sums in d_vpart (host bookkeeping;
 // the regions are laid out in launch order and reset every evaluation)
 double *nb_vir_region(NbHandle *h, long nblocks) {
-  if (h->vpart_used + nblocks > h->vpart_cap) throw std::runtime_error("virial partial-sum buffer too small");
+  if (h->vpart_used + nblocks > h->vpart_cap - NB_VIR_STAGE)
+    throw std::runtime_error("virial partial-sum buffer too small");
   double *p = h->d_vpart + 9 * h->vpart_used;
   h->vpart_used += nblocks;
   return p;
 }
 
+// the evaluation's virial from every region's partial sums: many blocks,
+// then one (the stage buffer is the last VIR_MAX_BLOCKS rows of d_vpart)
 void nb_vir_reduce(NbHandle *h) {
-  NB_LAUNCH(k_vir_reduce, dim3(1), dim3(256), 0, h->vpart_used, h->d_vpart, h->d_vir);
+  const long used = h->vpart_used;
+  long chunk = (used + VIR_MAX_BLOCKS - 1) / VIR_MAX_BLOCKS;
+  if (chunk < 64) chunk = 64;
+  const long g1 = (used + chunk - 1) / chunk;
+  if (used + VIR_MAX_BLOCKS > h->vpart_cap) throw std::runtime_error("virial partial-sum buffer too small");
+  double *stage = h->d_vpart + 9 * (h->vpart_cap - VIR_MAX_BLOCKS);
+  NB_LAUNCH(k_vir_reduce, dim3(g1 > 0 ? g1 : 1), dim3(VIR_THREADS), 0, used, chunk, (const double *)h->d_vpart,
+            stage);
+  NB_LAUNCH(k_vir_reduce, dim3(1), dim3(VIR_THREADS), 0, g1 > 0 ? g1 : 1, g1 > 0 ? g1 : 1, (const double *)stage,
+            h->d_vir);
 }

@@ -287,6 +287,8 @@ void nb_vdw(NbHandle *h);
 void nb_finalize_gradient(NbHandle *h);
 void nb_zero_outputs(NbHandle *h);
 double *nb_vir_region(NbHandle *h, long nblocks);
+// rows of d_vpart kept at its end for the first pass of nb_vir_reduce
+constexpr long NB_VIR_STAGE = 512;
 void nb_grid_virial(NbHandle *h, bool polar);
 // grid part of the reciprocal strain derivative of scale * (a . T_rec b)
 // terms (the mutual term's form), 9 partial sums per block into vpart
