@@ -165,15 +165,6 @@ __global__ void k_brick_fill(int n, const int32_t *__restrict__ bid, const int32
   if (q < n) batom[q] = s;
 }
 
-// local offset of an atom's first spline point inside its brick, per axis
-__device__ __forceinline__ void brick_local(const Bricks &bk, int b, const int32_t *ig, int o[3]) {
-  int bx = b % bk.nb[0], r = b / bk.nb[0];
-  int by = r % bk.nb[1], bz = r / bk.nb[1];
-  int bb[3] = {bx, by, bz};
-#pragma unroll
-  for (int d = 0; d < 3; d++) o[d] = wrapg(ig[d] + 1, bk.K[d]) - brick_start(bb[d], bk.K[d], bk.nb[d]);
-}
-
 // Bound of the stencil values of atom s: the order-5 B-spline values, first
 // and second derivatives are at most 0.599, 0.546 and 1.25, so every product
 // of three factors in a stencil value is below 1 and the values are bounded by
@@ -228,7 +219,17 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
   const int e = bk.ext, e3 = e * e * e;
   const int b = blockIdx.x;
   const int a0 = bstart[b], na = min(bstart[b + 1], n) - a0;
-  for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) lds[t / e3][t % e3] = 0.0;
+  // the brick's first grid point per axis (runtime divisions once per block)
+  int st[3];
+  {
+    const int bx = b % bk.nb[0], r = b / bk.nb[0];
+    const int bb[3] = {bx, r % bk.nb[1], r / bk.nb[1]};
+#pragma unroll
+    for (int d = 0; d < 3; d++) st[d] = brick_start(bb[d], bk.K[d], bk.nb[d]);
+  }
+  for (int l = threadIdx.x; l < e3; l += blockDim.x)
+#pragma unroll
+    for (int p = 0; p < NP; p++) lds[p][l] = 0.0;
   double bm[NP];
 #pragma unroll
   for (int p = 0; p < NP; p++) bm[p] = 0.0;
@@ -256,8 +257,9 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
   for (int q = threadIdx.x; q < na * 25; q += blockDim.x) {
     const int a = q / 25, yz = q - a * 25, ky = yz % 5, kz = yz / 5;
     const int s = batom[a0 + a];
-    int o[3];
-    brick_local(bk, b, igrid + 3 * s, o);
+    int o[3];   // local offset of the atom's first spline point inside the brick
+#pragma unroll
+    for (int d = 0; d < 3; d++) o[d] = wrapg(igrid[3 * s + d] + 1, bk.K[d]) - st[d];
     const T *tx = theta + (size_t)s * 60, *uu = tx + 20 + 4 * ky, *vv = tx + 40 + 4 * kz;
     const int base = ((o[2] + kz) * e + (o[1] + ky)) * e + o[0];
     // every global load is issued before the first LDS atomic: the compiler
@@ -300,10 +302,9 @@ __global__ void __launch_bounds__(256) k_spread_brick(int n, Bricks bk, const in
   }
   __syncthreads();
   T *out = bbuf + (size_t)b * NP * e3;
-  for (int t = threadIdx.x; t < NP * e3; t += blockDim.x) {
-    const int p = t / e3, l = t - p * e3;
-    out[t] = (T)(lds[p][l] * qi[p]);
-  }
+  for (int l = threadIdx.x; l < e3; l += blockDim.x)
+#pragma unroll
+    for (int p = 0; p < NP; p++) out[(size_t)p * e3 + l] = (T)(lds[p][l] * qi[p]);
 }
 
 // grid point (x, y, z) = own brick at local offset o plus, along each axis
