@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -100,11 +101,25 @@ def work_model(name, w):
     return None, None, None
 
 
+def lanes_per_atom():
+    """The library's lanes-per-atom macros (NB_LPA_PCG before NB_LPA, so the
+    longer name is replaced first) and their values, from nb_kern.h."""
+    vals = {}
+    with open(os.path.join(ROOT, "forcefieldx_amd", "csrc", "nb_kern.h")) as f:
+        for line in f:
+            m = re.match(r"#define (NB_LPA\w*) (\d+)", line)
+            if m:
+                vals[m.group(1)] = m.group(2)
+    return dict(sorted(vals.items(), key=lambda kv: -len(kv[0])))
+
+
 def profile_key(name, prec="mixed"):
     """The kernel's name in the rocprofv3 summaries for a name the library's
     profiler reports (the launch expression, with template parameters by
     name): the mixed form's instantiation."""
-    key = name.strip("()").split("(")[0].replace("NB_LPA", "16")
+    key = name.strip("()").split("(")[0]
+    for macro, value in lanes_per_atom().items():
+        key = key.replace(macro, value)
     real = "float" if prec == "mixed" else "double"
     key = key.replace("<T, IND>", f"<{real}, true>").replace("<T>", f"<{real}>").replace("<T, ", f"<{real}, ")
     return key

@@ -549,7 +549,8 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
       h->refine = true;
     }
     // producers of PCG partial sums: vector kernels (3n threads), the gather
-    // (8 lanes per atom) and the field kernels (NB_LPA lanes per atom)
+    // (8 lanes per atom) and the field kernels (NB_LPA_PCG lanes per atom; 16
+    // lanes bound every row kernel)
     h->part_stride = (int)std::max({grid_for(3 * (long)n, 256).x, xcd_grid((long)n * 8, 256).x,
                                     xcd_grid((long)n * 16, 256).x});
     alloc(&h->d_part, 16 * (size_t)h->part_stride);

@@ -48,8 +48,13 @@ static inline ScaleTab scale_tab(const NbHandle *h) {
 
 static inline dim3 grid_for(long n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
-// lanes per atom of the real-space pair kernels
+// lanes per atom of the real-space pair kernels: 16 for the kernels that run
+// once per evaluation over the 9-11 A rows (lists, permanent field, energies,
+// vdW); 8 for the PCG's dipole-field kernels, which run every iteration over
+// the shorter Ewald and preconditioner rows (at 20,736 waters the whole
+// evaluation is 4 % faster mixed and 2 % FP64 than at 16; 32 lanes are slower)
 #define NB_LPA 16
+#define NB_LPA_PCG 8
 
 // Workgroups are dispatched to the 8 XCDs of the device round-robin, and each
 // XCD has its own L2.  xcd_block() gives XCD x the contiguous logical blocks

@@ -631,7 +631,7 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = (use_recip && !precond) ? h->prm.ewald_alpha : 0.0;
-  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA_PCG, 256);
   const uint32_t *lst = precond ? h->d_pc : h->d_ew;
   const int32_t *cnt = precond ? h->d_npc : h->d_new;
   int maxl = precond ? h->maxpc : h->maxew;
@@ -640,13 +640,13 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
   const DipRec<double> *rd = (const DipRec<double> *)h->d_dip;
   const double *ctl = h->ctl;
   if (mixed && precond)
-    NB_LAUNCH((k_dip_field<float, NB_LPA, true>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+    NB_LAUNCH((k_dip_field<float, NB_LPA_PCG, true>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
   else if (mixed)
-    NB_LAUNCH((k_dip_field<float, NB_LPA, false>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+    NB_LAUNCH((k_dip_field<float, NB_LPA_PCG, false>), G, B, 0, n, rf, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
   else if (precond)
-    NB_LAUNCH((k_dip_field<double, NB_LPA, true>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+    NB_LAUNCH((k_dip_field<double, NB_LPA_PCG, true>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
   else
-    NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
+    NB_LAUNCH((k_dip_field<double, NB_LPA_PCG, false>), G, B, 0, n, rd, lst, cnt, maxl, h->cell, aw, f, fp, ctl);
 }
 
 // z of the PCG from the preconditioner field of the records d_dip2 (polar *
@@ -656,16 +656,16 @@ void nb_dip_field_real(NbHandle *h, double *f, double *fp, bool precond) {
 // of r.z.
 int nb_precond_z(NbHandle *h, int neps, bool control) {
   int n = h->n;
-  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA_PCG, 256);
   int mixed = h->prm.precision == NB_PREC_MIXED ? 1 : 0;
   double psc = h->prm.precond_scale;
   int maxit = h->prm.max_scf_iter > 0 ? h->prm.max_scf_iter : 1000;
   if (mixed)
-    NB_LAUNCH((k_precond_z<float, NB_LPA>), G, B, 0, n, (const DipRec<float> *)h->d_dip2, h->d_pc, h->d_npc,
+    NB_LAUNCH((k_precond_z<float, NB_LPA_PCG>), G, B, 0, n, (const DipRec<float> *)h->d_dip2, h->d_pc, h->d_npc,
               h->maxpc, h->cell, h->d_polar, psc, h->d_r, h->d_rp, h->d_z, h->d_zp, (void *)h->d_dip, mixed,
               h->d_part, h->part_stride, neps, h->d_scal, (int)control, h->prm.polar_eps, maxit, h->ctl);
   else
-    NB_LAUNCH((k_precond_z<double, NB_LPA>), G, B, 0, n, (const DipRec<double> *)h->d_dip2, h->d_pc, h->d_npc,
+    NB_LAUNCH((k_precond_z<double, NB_LPA_PCG>), G, B, 0, n, (const DipRec<double> *)h->d_dip2, h->d_pc, h->d_npc,
               h->maxpc, h->cell, h->d_polar, psc, h->d_r, h->d_rp, h->d_z, h->d_zp, (void *)h->d_dip, mixed,
               h->d_part, h->part_stride, neps, h->d_scal, (int)control, h->prm.polar_eps, maxit, h->ctl);
   return (int)G.x;
@@ -677,8 +677,8 @@ void nb_dip_field64(NbHandle *h, double *f, double *fp) {
   int n = h->n;
   bool use_recip = !h->cell.aperiodic && h->prm.ewald_alpha > 0.0;
   double aw = use_recip ? h->prm.ewald_alpha : 0.0;
-  dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
-  NB_LAUNCH((k_dip_field<double, NB_LPA, false>), G, B, 0, n, (const DipRec<double> *)h->d_dip64, h->d_ew, h->d_new,
+  dim3 B(256), G = xcd_grid((long)n * NB_LPA_PCG, 256);
+  NB_LAUNCH((k_dip_field<double, NB_LPA_PCG, false>), G, B, 0, n, (const DipRec<double> *)h->d_dip64, h->d_ew, h->d_new,
             h->maxew, h->cell, aw, f, fp, h->ctl);
 }
 
