@@ -141,10 +141,48 @@ __global__ void __launch_bounds__(256) k_pcg_dir(int n, const double *__restrict
   block_partials<2>(v, part, sl, stride);
 }
 
+// totals of two pairs of partial rows (counts na, nb; a negative count: the
+// collapsed totals in scal) read in one pass with one barrier; the summation
+// order is block_totals'
+static __device__ __forceinline__ void totals_2x2(const double *__restrict__ part, int stride,
+                                                  const double *__restrict__ scal, int na, int a0, int a1, int ta0,
+                                                  int ta1, int nb, int b0, int b1, int tb0, int tb1, double (&ta)[2],
+                                                  double (&tb)[2]) {
+  __shared__ double red[4][4];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  const int m = max(na, nb);
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    if (i < na) {
+      v[0] += part[(size_t)a0 * stride + i];
+      v[1] += part[(size_t)a1 * stride + i];
+    }
+    if (i < nb) {
+      v[2] += part[(size_t)b0 * stride + i];
+      v[3] += part[(size_t)b1 * stride + i];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = wave_sum_d(v[k]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; k++) red[k][w] = v[k];
+  __syncthreads();
+  double t[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) t[k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
+  ta[0] = na < 0 ? scal[ta0] : t[0];
+  ta[1] = na < 0 ? scal[ta1] : t[1];
+  tb[0] = nb < 0 ? scal[tb0] : t[2];
+  tb[1] = nb < 0 ? scal[tb1] : t[3];
+}
+
 // r -= alpha q, u += alpha p with alpha = r.z / p.q per system; the records
 // of the preconditioner field receive polar * r; partial r.r into S_EPS /
 // S_EPSP (reduced by the next k_precond_z); block 0 keeps the r.z totals as
-// the denominators of the next conjugation.
+// the denominators of the next conjugation.  The thread's vector loads are
+// issued before the totals are read, so the two memory latencies overlap
+// (one of the PCG iteration's latency-bound launches).
 __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restrict__ polar,
                                                     const double *__restrict__ vec, const double *__restrict__ vecp,
                                                     const double *__restrict__ p, const double *__restrict__ pp,
@@ -155,33 +193,32 @@ __global__ void __launch_bounds__(256) k_pcg_update(int n, const double *__restr
                                                     const double *__restrict__ ctl) {
   // a recorded solve that has stopped keeps its dipoles and residuals
   if (ctl && ctl[0] != 0.0) return;
-  const int in_pq[2] = {S_PAP, S_PAPP}, in_rz[2] = {S_RZN, S_RZPN};
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = q < 3 * n;
+  double r0 = 0.0, rp0 = 0.0, v0 = 0.0, vp0 = 0.0, u0 = 0.0, up0 = 0.0, p0 = 0.0, pp0 = 0.0, a = 0.0;
+  if (ok) {
+    r0 = r[q];
+    rp0 = rp[q];
+    v0 = vec[q];
+    vp0 = vecp[q];
+    u0 = u[q];
+    up0 = up[q];
+    p0 = p[q];
+    pp0 = pp[q];
+    a = polar[q / 3];
+  }
   double pq[2], rz[2];
-  if (npq < 0) {
-    pq[0] = scal[S_PQT];
-    pq[1] = scal[S_PQTP];
-  } else {
-    block_totals<2>(part, in_pq, stride, npq, pq);
-    __syncthreads();
-  }
-  if (nrz < 0) {
-    rz[0] = scal[S_RZT];
-    rz[1] = scal[S_RZTP];
-  } else {
-    block_totals<2>(part, in_rz, stride, nrz, rz);
-  }
+  totals_2x2(part, stride, scal, npq, S_PAP, S_PAPP, S_PQT, S_PQTP, nrz, S_RZN, S_RZPN, S_RZT, S_RZTP, pq, rz);
   double al = rz[0], alp = rz[1];
   if (pq[0] != 0.0) al /= pq[0];
   if (pq[1] != 0.0) alp /= pq[1];
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
-  if (q < 3 * n) {
-    double rr = r[q] - al * vec[q], rrp = rp[q] - alp * vecp[q];
+  if (ok) {
+    const double rr = r0 - al * v0, rrp = rp0 - alp * vp0;
     r[q] = rr;
     rp[q] = rrp;
-    u[q] += al * p[q];
-    up[q] += alp * pp[q];
-    double a = polar[q / 3];
+    u[q] = u0 + al * p0;
+    up[q] = up0 + alp * pp0;
     dip_put(rec, mixed, q / 3, q % 3, a * rr, a * rrp);
     v[0] = rr * rr;
     v[1] = rrp * rrp;
