@@ -685,9 +685,10 @@ void nb_dip_field64(NbHandle *h, double *f, double *fp) {
 #define NB_LPA_SP 4
 
 // which: NB_RE_ALL both passes into the real-space arrays (grad_s, torque_s);
-// NB_RE_PERM the permanent pass alone; NB_RE_POL the polarization pass alone,
-// added into the reciprocal arrays (grad_r, torque_r, which k_recip_energy
-// stores first on the same stream), so it can run beside the permanent pass
+// NB_RE_PERM the permanent pass (and, in the mixed form, the FP64 masked
+// polarization pairs); NB_RE_POL the rest of the polarization pass, added
+// into the reciprocal arrays (grad_r, torque_r, which k_recip_energy stores
+// first on the same stream), so it can run beside the permanent pass
 template <bool VIR, bool OST>
 static void real_energy_t(NbHandle *h, bool polar, int which) {
   int n = h->n;
@@ -699,6 +700,11 @@ static void real_energy_t(NbHandle *h, bool polar, int which) {
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256), Gs = xcd_grid((long)n * NB_LPA_SP, 256);
   double el = h->prm.electric;
   bool perm = (!OST || h->ph.perm_real) && which != NB_RE_POL;
+  // the mixed form's FP64 pass over the masked polarization pairs runs with
+  // the permanent passes (side stream, into the real-space arrays), which
+  // balances the two streams: the FP32 polarization pass follows the
+  // reciprocal energy on the main stream
+  const bool masked_pol = polar && which != NB_RE_POL;
   polar = polar && which != NB_RE_PERM;
   const bool to_r = which == NB_RE_POL;
   double *g = to_r ? h->d_grad_r : h->d_grad_s, *tq = to_r ? h->d_torque_r : h->d_torque_s;
@@ -713,10 +719,8 @@ static void real_energy_t(NbHandle *h, bool polar, int which) {
       NB_RE(float, NB_LPA, false, 1, G, h->d_ew, h->d_new, h->maxew);
       NB_RE(double, NB_LPA_SP, false, 2, Gs, h->d_sp, h->d_nsp, h->maxsp);
     }
-    if (polar) {
-      NB_RE(float, NB_LPA, true, 1, G, h->d_ew, h->d_new, h->maxew);
-      NB_RE(double, NB_LPA_SP, true, 2, Gs, h->d_sp, h->d_nsp, h->maxsp);
-    }
+    if (polar) NB_RE(float, NB_LPA, true, 1, G, h->d_ew, h->d_new, h->maxew);
+    if (masked_pol) NB_RE(double, NB_LPA_SP, true, 2, Gs, h->d_sp, h->d_nsp, h->maxsp);
   } else {
     if (perm) NB_RE(double, NB_LPA, false, 0, G, h->d_ew, h->d_new, h->maxew);
     if (polar) NB_RE(double, NB_LPA, true, 0, G, h->d_ew, h->d_new, h->maxew);
