@@ -94,6 +94,7 @@ void free_all(NbHandle *h) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (h->h_pinned) hipHostFree(h->h_pinned);
+  if (h->d_pin) hipFree(h->d_pin);
   for (auto &r : h->prof_pool) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);
@@ -557,6 +558,7 @@ int nb_create(const nb_params *params, const nb_topology *top, void **handle) {
     alloc(&h->d_energy, 16);
     alloc(&h->d_count, 4);
     HIPCHECK(hipHostMalloc((void **)&h->h_pinned, sizeof(double) * P_NSLOT, hipHostMallocDefault));
+    alloc(&h->d_pin, P_NSLOT);
     memset(&h->last, 0, sizeof(h->last));
     if (params->elec_lambda && h->has_soft && params->do_pme) {
       if (!(params->perm_lambda_end > params->perm_lambda_start && params->pol_lambda_end > params->pol_lambda_start))
@@ -792,21 +794,12 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   if (third) nb_join3(h);
   if (h->finite_active) nb_finite_add(h);
   nb_finalize_gradient(h);
-  if (h->has_valence) {
-    nb_valence_gather(h);
-    HIPCHECK(hipMemcpyAsync(h->h_pinned + P_VALENCE, h->d_val_energy, sizeof(double) * 8, hipMemcpyDeviceToHost,
-                            h->stream));
-  }
-  if (h->want_vir) {
-    nb_vir_reduce(h);
-    HIPCHECK(hipMemcpyAsync(h->h_pinned + 16, h->d_vir, sizeof(double) * 9, hipMemcpyDeviceToHost, h->stream));
-  }
-  HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_energy, sizeof(double) * 8, hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(h->h_pinned + 32, h->d_energy + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(h->h_pinned + 8, h->d_count, sizeof(int64_t) * 4, hipMemcpyDeviceToHost, h->stream));
-  HIPCHECK(hipMemcpyAsync(h->h_pinned + 12, h->d_flags, sizeof(int32_t) * 8, hipMemcpyDeviceToHost, h->stream));
-  if (capture)
-    HIPCHECK(hipMemcpyAsync(h->h_pinned + 40, h->d_scal, sizeof(double) * S_NSLOT, hipMemcpyDeviceToHost, h->stream));
+  if (h->has_valence) nb_valence_gather(h);
+  if (h->want_vir) nb_vir_reduce(h);
+  // energies, counts, flags (and the virial, valence energies, PCG scalars)
+  // packed into h_pinned's layout on the device, then one transfer
+  nb_pack_outputs(h, capture);
+  HIPCHECK(hipMemcpyAsync(h->h_pinned, h->d_pin, sizeof(double) * P_NSLOT, hipMemcpyDeviceToHost, h->stream));
   return status;
 }
 

@@ -568,6 +568,34 @@ __global__ void k_zero_outputs(int m, double *__restrict__ a, double *__restrict
   if (i < 4) count[i] = 0;
 }
 
+// The evaluation's host-visible outputs in the pinned block's layout (P_*
+// offsets): energies, counts and flags always; the virial, the PCG scalars
+// and the valence energies when given (their slots are zero otherwise).
+// Words are moved as 64-bit patterns, so counts and flags keep their bits.
+__global__ void k_pack_outputs(const double *__restrict__ energy, const int64_t *__restrict__ count,
+                               const int32_t *__restrict__ flags, const double *__restrict__ vir,
+                               const double *__restrict__ scal, const double *__restrict__ val,
+                               double *__restrict__ pin) {
+  const int t = threadIdx.x;
+  if (t >= P_NSLOT) return;
+  const unsigned long long *src = nullptr;
+  int k = 0;
+  if (t < P_COUNT) { src = (const unsigned long long *)energy; k = t; }
+  else if (t < P_FLAGS) { src = (const unsigned long long *)count; k = t - P_COUNT; }
+  else if (t < P_VIR) { src = (const unsigned long long *)flags; k = t - P_FLAGS; }
+  else if (t < P_VIR + 9) { src = (const unsigned long long *)vir; k = t - P_VIR; }
+  else if (t >= P_DUDL && t < P_DUDL + 2) { src = (const unsigned long long *)energy; k = 8 + t - P_DUDL; }
+  else if (t >= P_SCAL && t < P_SCAL + S_NSLOT) { src = (const unsigned long long *)scal; k = t - P_SCAL; }
+  else if (t >= P_VALENCE && t < P_VALENCE + 8) { src = (const unsigned long long *)val; k = t - P_VALENCE; }
+  ((unsigned long long *)pin)[t] = src ? src[k] : 0ull;
+}
+
+void nb_pack_outputs(NbHandle *h, bool scal) {
+  NB_LAUNCH(k_pack_outputs, dim3(1), dim3(128), 0, h->d_energy, h->d_count, h->d_flags,
+            h->want_vir ? h->d_vir : (const double *)nullptr, scal ? h->d_scal : (const double *)nullptr,
+            h->has_valence ? h->d_val_energy : (const double *)nullptr, h->d_pin);
+}
+
 void nb_zero_outputs(NbHandle *h) {
   int n3 = 3 * h->n;
   NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_grad_s, h->d_torque_s,

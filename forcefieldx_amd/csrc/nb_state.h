@@ -181,6 +181,7 @@ struct NbHandle {
   int64_t *d_count = nullptr;
   int energy_blocks = 0;
   double *h_pinned = nullptr;
+  double *d_pin = nullptr;        // [P_NSLOT] the evaluation's outputs in h_pinned's layout (one transfer)
 
   nb_energies last;
   int fmaxa = 0, fmaxb = 0;       // row capacities of the batch kernel (preconditioner part, the rest)
@@ -286,6 +287,7 @@ void nb_real_energy(NbHandle *h, bool polar, int which);
 void nb_vdw(NbHandle *h);
 void nb_finalize_gradient(NbHandle *h);
 void nb_zero_outputs(NbHandle *h);
+void nb_pack_outputs(NbHandle *h, bool scal);
 double *nb_vir_region(NbHandle *h, long nblocks);
 // rows of d_vpart kept at its end for the first pass of nb_vir_reduce
 constexpr long NB_VIR_STAGE = 512;
