@@ -711,14 +711,16 @@ static int issue_evaluation(NbHandle *h, bool capture, nb_energies &e, bool &reb
   h->finite_active = false;
   rebuilt = false;
   if (capture) {
+    // the cleared flags include the list check's, which k_max_disp raises
+    nb_zero_outputs(h);
     nb_check_rebuild_async(h);
     nb_gather_sorted(h);
   } else {
     rebuilt = nb_need_rebuild(h);
     if (rebuilt) nb_build_lists(h);
     else nb_gather_sorted(h);
+    nb_zero_outputs(h);
   }
-  nb_zero_outputs(h);
   if (h->prm.do_vdw || h->prm.do_pme) nb_rotate(h);
   // vdW and the valence terms run on the third stream, overlapping everything
   // up to the gradient assembly

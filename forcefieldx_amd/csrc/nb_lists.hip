@@ -287,16 +287,12 @@ static int estimate_neighbors(const NbHandle *h, double r) {
   return est < 16.0 ? 16 : (int)est;
 }
 
-__global__ void k_set_flag(int32_t *__restrict__ flags, int idx, int32_t v) {
-  if (threadIdx.x == 0) flags[idx] = v;
-}
-
-// recorded form of the list check: flags[3] is raised when an atom moved more
-// than half the buffer; the caller reads it back with the energies
+// recorded form of the list check: flags[3] (cleared by k_zero_outputs just
+// before) is raised when an atom moved more than half the buffer; the caller
+// reads it back with the energies
 void nb_check_rebuild_async(NbHandle *h) {
   int n = h->n;
   double lim = 0.5 * h->prm.list_buffer;
-  NB_LAUNCH(k_set_flag, dim3(1), dim3(64), 0, h->d_flags, 3, 0);
   NB_LAUNCH(k_max_disp, dim3((n + 255) / 256), dim3(256), 0, n, h->d_xyz, h->d_xyz_list, lim * lim, h->d_flags);
 }
 

@@ -555,22 +555,15 @@ __global__ void __launch_bounds__(256) k_vdw(int n, const double4 *__restrict__ 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// flags[3] (list rebuild needed) belongs to the list check and is kept
-__global__ void k_clear_flags(int32_t *__restrict__ flags) {
-  int t = threadIdx.x;
-  if (t == 1 || t == 2 || t == 4 || t == 5) flags[t] = 0;
-}
-
 // Ewald / preconditioner / masked rows of this evaluation from the Verlet list
 // (PermanentFieldRegion.java:679-714).  No host synchronisation: overflow is
-// reported in flags[1], [2], [5] and handled by nb_grow_filtered_rows after the
-// evaluation.
+// reported in flags[1], [2], [5] (cleared by k_zero_outputs at the start of
+// the evaluation) and handled by nb_grow_filtered_rows after the evaluation.
 void nb_filter_lists(NbHandle *h) {
   int n = h->n;
   ScaleTab sc = scale_tab(h);
   dim3 B(256), G = xcd_grid((long)n * NB_LPA, 256);
   double off2 = h->prm.ewald_cutoff * h->prm.ewald_cutoff;
-  NB_LAUNCH(k_clear_flags, dim3(1), dim3(64), 0, h->d_flags);
   NB_LAUNCH(k_filter<NB_LPA>, G, B, 0, n, h->d_pos4, h->d_nbr, h->d_nnb, h->maxnb, h->cell, off2,
             h->prm.precond_cutoff, sc, h->d_ew, h->maxew, h->d_new, h->d_pc, h->maxpc, h->d_npc, h->d_sp, h->maxsp,
             h->d_nsp, h->d_flags);

@@ -96,12 +96,13 @@ __host__ __device__ __forceinline__ int brick_start(int b, int K, int nb) { retu
 
 // B-splines, grid start, fractional multipoles and brick binning.  The
 // weights are stored in the grid's real type TH: the spread and the gather
-// compute in that type.
+// compute in that type; theta64 (when given) also receives them in FP64, for
+// the FP64 refinement of the mixed solve.
 template <typename TH, bool BIN = true>
 __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const double *__restrict__ gmp, CellD cell,
                                Bricks bk, Frac fr, TH *__restrict__ theta, int32_t *__restrict__ igrid,
                                double *__restrict__ fmp, int32_t *__restrict__ bcount, int32_t *__restrict__ bid,
-                               int32_t *__restrict__ brank) {
+                               int32_t *__restrict__ brank, double *__restrict__ theta64) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   double4 p = pos[s];
@@ -119,6 +120,11 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
     TH *o = theta + ((size_t)s * 3 + d) * 20;
     for (int k = 0; k < 5; k++)
       for (int j = 0; j < 4; j++) o[4 * k + j] = (TH)th[k][j];
+    if (theta64) {
+      double *o64 = theta64 + ((size_t)s * 3 + d) * 20;
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 4; j++) o64[4 * k + j] = th[k][j];
+    }
   }
   if (BIN) {
     int b = b3[0] + bk.nb[0] * (b3[1] + bk.nb[1] * b3[2]);
@@ -126,11 +132,6 @@ __global__ void k_bspline_frac(int n, const double4 *__restrict__ pos, const dou
     brank[s] = atomicAdd(&bcount[b], 1);
     frac_multipole(fr, gmp + 10 * s, fmp + 10 * s);
   }
-}
-
-__global__ void k_zero_i(int m, int32_t *__restrict__ x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) x[i] = 0;
 }
 
 // exclusive scan of the brick counts (one workgroup of 1024)
@@ -1142,17 +1143,18 @@ void nb_pme_permanent(NbHandle *h) {
   int n = h->n;
   dim3 B(128), G((n + 127) / 128);
   Frac fr = frac_of(h);
-  NB_LAUNCH(k_zero_i, grid_for(h->nbricks, 256), dim3(256), 0, h->nbricks, h->d_bcount);
+  // the brick counts were cleared by k_zero_outputs at the start of the evaluation
+  // the FP64 weights of the mixed form's refinement come from the same launch
+  const bool w64 = mixed_grid(h) && h->refine && h->prm.polarization == NB_POL_MUTUAL;
   if (mixed_grid(h))
     NB_LAUNCH(k_bspline_frac<float>, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, bricks_of(h), fr, (float *)h->d_theta,
-              h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
+              h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank, w64 ? h->d_theta64 : (double *)nullptr);
   else
     NB_LAUNCH(k_bspline_frac<double>, G, B, 0, n, h->d_pos4, h->d_gmp, h->cell, bricks_of(h), fr, h->d_theta,
-              h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
+              h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank, (double *)nullptr);
   NB_LAUNCH(k_brick_scan, dim3(1), dim3(1024), 0, h->nbricks, h->d_bcount, h->d_bstart);
   NB_LAUNCH(k_brick_fill, grid_for(n, 256), dim3(256), 0, n, h->d_bid, h->d_brank, h->d_bstart, h->d_batom);
   if (mixed_grid(h)) {
-    if (h->refine && h->prm.polarization == NB_POL_MUTUAL) nb_pme_theta64(h);
     spread_t<float, false>(h, nullptr, nullptr);
     convolution(h, 1);
     gather_t<float, 1, true>(h, h->d_phip, nullptr);
@@ -1161,14 +1163,6 @@ void nb_pme_permanent(NbHandle *h) {
     convolution(h, 1);
     gather_t<double, 1, true>(h, h->d_phip, nullptr);
   }
-}
-
-// FP64 B-spline weights of the sorted atoms (same grid starts and bins as the
-// weights of the mixed form), for the FP64 refinement of the mixed solve
-void nb_pme_theta64(NbHandle *h) {
-  int n = h->n;
-  NB_LAUNCH((k_bspline_frac<double, false>), dim3((n + 127) / 128), dim3(128), 0, n, h->d_pos4, h->d_gmp, h->cell,
-            bricks_of(h), frac_of(h), h->d_theta64, h->d_igrid, h->d_fmp, h->d_bcount, h->d_bid, h->d_brank);
 }
 
 // FP64 reciprocal potentials (all 20 derivatives and the Cartesian field) of

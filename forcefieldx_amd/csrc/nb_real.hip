@@ -558,14 +558,20 @@ void nb_gather_sorted(NbHandle *h) {
             (void *)h->d_dip, (void *)h->d_dip2, mixed_of(h), h->cell, (const int32_t *)h->d_wrap);
 }
 
-// one launch clears the accumulated outputs of an evaluation
+// one launch clears the accumulated outputs of an evaluation, with (when
+// given) the status flags 1-5 (list check, row overflows, undefined torque)
+// and the per-brick atom counts of the PME binning
 __global__ void k_zero_outputs(int m, double *__restrict__ a, double *__restrict__ b, double *__restrict__ c,
                                double *__restrict__ d, double *__restrict__ energy, int64_t *__restrict__ count,
-                               double *__restrict__ e, double *__restrict__ f) {
+                               double *__restrict__ e, double *__restrict__ f, int32_t *__restrict__ flags,
+                               int32_t *__restrict__ bcount, int nbricks) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m) { a[i] = 0.0; b[i] = 0.0; c[i] = 0.0; d[i] = 0.0; e[i] = 0.0; f[i] = 0.0; }
   if (i < 16) energy[i] = 0.0;
   if (i < 4) count[i] = 0;
+  if (flags && i >= 1 && i <= 5) flags[i] = 0;
+  if (bcount)
+    for (int k = i; k < nbricks; k += gridDim.x * blockDim.x) bcount[k] = 0;
 }
 
 // The evaluation's host-visible outputs in the pinned block's layout (P_*
@@ -598,11 +604,14 @@ void nb_pack_outputs(NbHandle *h, bool scal) {
 
 void nb_zero_outputs(NbHandle *h) {
   int n3 = 3 * h->n;
+  const bool bins = h->prm.do_pme && !h->cell.aperiodic && h->prm.ewald_alpha > 0.0 && h->d_bcount;
   NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_grad_s, h->d_torque_s,
-            h->d_gred_s, h->d_grad, h->d_energy, h->d_count, h->d_gred_l, h->d_dudx);
+            h->d_gred_s, h->d_grad, h->d_energy, h->d_count, h->d_gred_l, h->d_dudx, h->d_flags,
+            bins ? h->d_bcount : (int32_t *)nullptr, bins ? h->nbricks : 0);
   if (h->ost)
     NB_LAUNCH(k_zero_outputs, grid_for(n3 > 16 ? n3 : 16, 256), dim3(256), 0, n3, h->d_lgrad_s, h->d_ltorque_s,
-              h->d_lgrad_r, h->d_ltorque_r, h->d_energy, h->d_count, h->d_lgrad_s, h->d_ltorque_s);
+              h->d_lgrad_r, h->d_ltorque_r, h->d_energy, h->d_count, h->d_lgrad_s, h->d_ltorque_s,
+              (int32_t *)nullptr, (int32_t *)nullptr, 0);
 }
 
 void nb_direct(NbHandle *h) {

@@ -266,7 +266,6 @@ void nb_rotate(NbHandle *h);
 void nb_pme_setup(NbHandle *h);
 void nb_pme_permanent(NbHandle *h);
 void nb_pme_induced(NbHandle *h, const double *u, const double *up, bool full);
-void nb_pme_theta64(NbHandle *h);
 void nb_pme_induced64(NbHandle *h, const double *u, const double *up);
 void nb_dip_field64(NbHandle *h, double *f, double *fp);
 void nb_filter_lists(NbHandle *h);
