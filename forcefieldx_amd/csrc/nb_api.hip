@@ -331,7 +331,8 @@ void create_ost_children(NbHandle *h, const nb_params *params, const nb_topology
 
 }  // namespace
 
-int nb_prof_begin(NbHandle *h, const char *name) {
+// the next pair of timing events of the profiled evaluation
+int nb_prof_slot(NbHandle *h, const char *name) {
   if (h->prof_n == (int)h->prof_pool.size()) {
     NbHandle::ProfRec r;
     HIPCHECK(hipEventCreate(&r.a));
@@ -340,11 +341,8 @@ int nb_prof_begin(NbHandle *h, const char *name) {
   }
   int slot = h->prof_n++;
   h->prof_pool[slot].name = name;
-  HIPCHECK(hipEventRecord(h->prof_pool[slot].a, h->cur));
   return slot;
 }
-
-void nb_prof_end(NbHandle *h, int slot) { HIPCHECK(hipEventRecord(h->prof_pool[slot].b, h->cur)); }
 
 // accumulate elapsed times by kernel name; requires a completed stream
 void nb_prof_harvest(NbHandle *h) {

@@ -12,6 +12,7 @@
 // Gradients are returned in caller order.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <string>
 #include <map>
@@ -323,22 +324,27 @@ int nb_batch_evaluate_static(NbHandle **hv, int count, uint32_t flags, nb_energi
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Optional per-launch HIP-event timing (nb_set_profiling); the events are
-// recorded on the handle's stream around each launch and harvested after the
+// Optional per-launch timing (nb_set_profiling): the launch carries a start
+// and a stop event that the runtime records with the kernel's own dispatch
+// (hipExtLaunchKernelGGL), so the measured time is the kernel's, without the
+// launch gaps that separately recorded events would add; harvested after the
 // evaluation's final synchronisation.
-int nb_prof_begin(NbHandle *h, const char *name);
+int nb_prof_slot(NbHandle *h, const char *name);
 static inline bool nb_prof_wanted(const NbHandle *h, const char *name) {
   return h->profiling && (h->prof_filter.empty() || h->prof_filter.find(name) != std::string::npos ||
                           std::string(name).find(h->prof_filter) != std::string::npos);
 }
-void nb_prof_end(NbHandle *h, int slot);
 void nb_prof_harvest(NbHandle *h);
 
-#define NB_LAUNCH(KERNEL, G, B, SH, ...)                              \
-  do {                                                                \
-    int slot_ = nb_prof_wanted(h, #KERNEL) ? nb_prof_begin(h, #KERNEL) : -1; \
-    hipLaunchKernelGGL(KERNEL, G, B, SH, h->cur, __VA_ARGS__);        \
-    if (slot_ >= 0) nb_prof_end(h, slot_);                            \
+#define NB_LAUNCH(KERNEL, G, B, SH, ...)                                                              \
+  do {                                                                                                \
+    if (nb_prof_wanted(h, #KERNEL)) {                                                                 \
+      const int slot_ = nb_prof_slot(h, #KERNEL);                                                     \
+      hipExtLaunchKernelGGL(KERNEL, G, B, SH, h->cur, h->prof_pool[slot_].a, h->prof_pool[slot_].b, 0, \
+                            __VA_ARGS__);                                                             \
+    } else {                                                                                          \
+      hipLaunchKernelGGL(KERNEL, G, B, SH, h->cur, __VA_ARGS__);                                      \
+    }                                                                                                 \
   } while (0)
 
 // Fork / join between the main and the side stream.  nb_fork makes the side
