@@ -411,8 +411,9 @@ int nb_device_count(void);
 /* Kernel timing with HIP events on the handle's stream (no counterpart in the
  * reference, whose timers are PMETimings.java:47-120 and
  * ReciprocalSpace.printTimings :645-718).  When on, every launch of an
- * evaluation is bracketed by events; totals accumulate per kernel name until
- * profiling is switched again. */
+ * evaluation carries a start and a stop event that the runtime records with
+ * the kernel's dispatch, so each time is the kernel's own; totals accumulate
+ * per kernel name until profiling is switched again. */
 int nb_set_profiling(void *handle, int on);
 /* Restrict the timing to kernels whose name contains `substring` ("" = all). */
 int nb_set_profile_filter(void *handle, const char *substring);
