@@ -226,7 +226,7 @@ template <typename T, int NP, bool FULL, bool PREFETCH = false, typename TH = do
 __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const TH *__restrict__ theta,
                                             const int32_t *__restrict__ igrid, const CX<T> *__restrict__ grid,
                                             const Frac &fr, int K0, int K1, int K2, double *__restrict__ out0,
-                                            double *__restrict__ out1) {
+                                            double *__restrict__ out1, double *fld = nullptr) {
   constexpr int NC = FULL ? 20 : 4;
   T phi[NP][NC];
 #pragma unroll
@@ -312,6 +312,15 @@ __device__ __forceinline__ void gather_atom(int s, int lane, bool valid, const T
   for (int p = 0; p < NP; p++)
 #pragma unroll
     for (int c = 0; c < NC; c++) phi[p][c] = group8(phi[p][c]);
+  // fld (when given): the Cartesian field of each part in every lane of the 8
+  if (!FULL && fld) {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        fld[3 * p + c] =
+            fr.a[0][c] * (double)phi[p][1] + fr.a[1][c] * (double)phi[p][2] + fr.a[2][c] * (double)phi[p][3];
+  }
   if (valid && lane == 0) {
     double *outs[2] = {out0, out1};
 #pragma unroll

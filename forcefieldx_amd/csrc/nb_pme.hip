@@ -394,26 +394,39 @@ __global__ void __launch_bounds__(256) k_gather_ap(int n, const T *__restrict__ 
                                                    int nrz, const double *__restrict__ scal, int first,
                                                    const double *__restrict__ ctl) {
   if (ctl && ctl[0] != 0.0) return;  // recorded solve already stopped
-  double beta[2];
-  pcg_beta(part, stride, nrz, scal, first, beta);
   int gt = xcd_block() * blockDim.x + threadIdx.x;
   int s = gt >> 3, lane = gt & 7;
-  gather_atom<T, 2, false, false, T>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, phiu, phiup);
+  // lane d < 3 of the atom's 8 updates component d; its vectors are loaded
+  // first, so their latency overlaps the totals and the grid gather
+  const bool mine = s < n && lane < 3;
+  const size_t q = 3 * (size_t)s + (lane < 3 ? lane : 0);
+  double a = 0.0, zq = 0.0, zpq = 0.0, fq = 0.0, fpq = 0.0, pn = 0.0, qn = 0.0, pnp = 0.0, qnp = 0.0;
+  if (mine) {
+    a = polar[s];
+    zq = z[q];
+    zpq = zp[q];
+    fq = f[q];
+    fpq = fp[q];
+    pn = p[q];
+    qn = vec[q];
+    pnp = pp[q];
+    qnp = vecp[q];
+  }
+  double beta[2];
+  pcg_beta(part, stride, nrz, scal, first, beta);
+  double fld[6];
+  gather_atom<T, 2, false, false, T>(s, lane, s < n, theta, igrid, grid, fr, K0, K1, K2, phiu, phiup, fld);
   double v[2] = {0.0, 0.0};
-  if (s < n && lane == 0) {
-    const double a = polar[s];
-    for (int d = 0; d < 3; d++) {
-      const size_t q = 3 * (size_t)s + d;
-      double pn = p[q], qn = vec[q], pnp = pp[q], qnp = vecp[q];
-      pcg_direction(a, z[q], f[q] + a3 * z[q] - phiu[30 * (size_t)s + 21 + d], beta[0], first, pn, qn);
-      pcg_direction(a, zp[q], fp[q] + a3 * zp[q] - phiup[30 * (size_t)s + 21 + d], beta[1], first, pnp, qnp);
-      p[q] = pn;
-      vec[q] = qn;
-      pp[q] = pnp;
-      vecp[q] = qnp;
-      v[0] += pn * qn;
-      v[1] += pnp * qnp;
-    }
+  if (mine) {
+    const int d = lane;
+    pcg_direction(a, zq, fq + a3 * zq - fld[d], beta[0], first, pn, qn);
+    pcg_direction(a, zpq, fpq + a3 * zpq - fld[3 + d], beta[1], first, pnp, qnp);
+    p[q] = pn;
+    vec[q] = qn;
+    pp[q] = pnp;
+    vecp[q] = qnp;
+    v[0] = pn * qn;
+    v[1] = pnp * qnp;
   }
   __syncthreads();
   const int sl[2] = {S_PAP, S_PAPP};
